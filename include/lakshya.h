@@ -1,0 +1,141 @@
+/*
+ * C ABI of the MI355X interior-point path (liblakshya.so).  Plain pointers and sizes only.
+ * All functions return LK_OK (0) or an error code; lk_last_error() gives the message of the
+ * last failure on the calling thread.  Host arrays unless a parameter says "device".
+ *
+ * Each entry point names the reference interface it replaces.  The reference has no C binding
+ * for this path (its C API, include/sankhya/sankhya.h:30-63, only reaches the CPU simplex), so
+ * these are the functions a ctypes / cgo / JNI binding of the C++ interface would need.
+ */
+#ifndef LAKSHYA_H
+#define LAKSHYA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LK_OK 0
+#define LK_ERR_INVALID 1      /* std::invalid_argument (dimension mismatch, bad permutation, ...) */
+#define LK_ERR_NOMEM 2        /* std::bad_alloc (arena exhausted or fragmented) */
+#define LK_ERR_RUNTIME 3      /* std::runtime_error (HIP failure, not positive definite) */
+#define LK_ERR_INTERNAL 5
+
+typedef struct lk_arena lk_arena;
+typedef struct lk_kkt lk_kkt;
+typedef struct lk_ipm lk_ipm;
+
+const char* lk_last_error(void);
+int lk_version(void);
+int lk_device_count(int* count);
+
+/* ---- VRAM arena: gpu::VRAMArena (src/gpu/vram_arena.cuh:21-75) ---- */
+int lk_arena_create(size_t capacity_bytes, lk_arena** out);
+int lk_arena_alloc(lk_arena* a, size_t bytes, void** device_ptr);   /* 0 bytes -> NULL */
+int lk_arena_free(lk_arena* a, void* device_ptr);
+int lk_arena_occupancy(lk_arena* a, size_t* percent);
+int lk_arena_destroy(lk_arena* a);
+
+/* ---- host symbolic phase: ipm::compute_amd_ordering / compute_symbolic_factorization
+ *      (src/ipm/symbolic.hpp:50-63) and the supernodal structure ---- */
+int lk_order_reference(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                       int32_t* P);
+int lk_order_amd(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                 int32_t* P);
+/* parent[m], Lptr[m+1]; Lidx capacity `cap` (query the size with cap = 0: *nnz is set) */
+int lk_symbolic(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                const int32_t* P, int32_t* parent, int32_t* P_inv, int32_t* Lptr, int32_t* Lidx,
+                int64_t cap, int64_t* nnz);
+
+typedef struct {
+    int64_t nsuper, nnz_L, nnz_L_exact, npairs, nsegments, panel_elems;
+    int32_t levels, launches_factor, launches_solve, pad;
+    double flops_exact, flops_panel, symbolic_ms;
+    int64_t factorizations, solves, refinement_steps;
+} lk_kkt_stats;
+
+/* Supernodal structure statistics for an ordering (NULL P -> AMD); perm_out[m] optional. */
+int lk_supernodal(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                  const int32_t* P, int32_t relax_small, double relax_frac, lk_kkt_stats* out,
+                  int32_t* perm_out, int32_t* sn_first_out, int32_t* nsuper_out);
+
+/* ---- KKT solver: gpu::GPUKKTCholeskySolver (src/cuda/kkt.cuh:38-71) ----
+ * A in CSR.  P: ordering (NULL -> AMD); when `reference_symbolic` != 0 the reference ordering
+ * rule is used and L is exposed in the reference CSR layout by lk_kkt_get_L_csr. */
+int lk_kkt_create(lk_arena* arena, int32_t m, int32_t n, const int32_t* rowptr,
+                  const int32_t* colidx, const double* vals, const int32_t* P,
+                  int32_t reference_symbolic, int32_t use_graphs, lk_kkt** out);
+/* mode 0: gpu_cholesky_factorize (M0 + 1e-9 I, unequilibrated; *not_spd set instead of throwing)
+ * mode 1: equilibrated FP64 factor (delta retries), mode 2: equilibrated FP32 factor.
+ * theta: n values, device pointer when on_device != 0. */
+int lk_kkt_factor(lk_kkt* k, const double* theta, int32_t on_device, int32_t mode,
+                  int32_t* not_spd);
+typedef struct {
+    int32_t success, retries, refinement_steps, pad;
+    double delta, residual_inf;
+} lk_kkt_solve_info;
+/* gpu_cholesky_solve[_device]: rhs (m) overwritten with dy; info->success as the reference's
+ * bool return. max_refine < 0 -> reference budget (10). */
+int lk_kkt_solve(lk_kkt* k, double* rhs, int32_t on_device, double tol, int32_t max_refine,
+                 lk_kkt_solve_info* info);
+/* L values in the reference layout (rows of sym.L_pattern, diagonal last), after mode 0. */
+int lk_kkt_get_L_csr(lk_kkt* k, double* vals, int64_t nnz);
+int lk_kkt_permutation(lk_kkt* k, int32_t* perm);
+int lk_kkt_get_stats(lk_kkt* k, lk_kkt_stats* out);
+int lk_kkt_stream(lk_kkt* k, void** stream);
+int lk_kkt_destroy(lk_kkt* k);
+
+/* ---- interior-point solver: ipm::MehrotraSolver (src/ipm/mehrotra.hpp:10-31) ---- */
+typedef struct {
+    int32_t stop_rule;          /* 0 reference (scaled norms < tol), 1 relative gap (gap_tol) */
+    int32_t max_iter;
+    double tol, gap_tol;
+    int32_t precision;          /* 0 FP64, 1 mixed (FP32 factor + FP64 refinement) */
+    int32_t mixed_max_refine;
+    double mixed_fp64_spread;
+    int32_t ordering;           /* 0 AMD, 1 reference minimum degree */
+    int32_t use_graphs;
+    int32_t verbose;
+    int32_t pad;
+    double warm_floor;
+} lk_ipm_options;
+
+typedef struct {
+    int32_t status;             /* 0 Optimal, 1 Infeasible, 2 Unbounded, 3 IterationLimit */
+    int32_t iterations;
+    double objective, primal_residual, dual_residual, duality_gap;
+    double dual_objective, rel_gap, rel_primal_residual, rel_dual_residual;
+    int32_t factorizations, fp32_factorizations, fp64_factorizations, solves;
+    int32_t refinement_steps, pad;
+    double setup_ms, solve_ms, factor_ms, nnz_L, flops_per_factorization;
+} lk_ipm_result;
+
+int lk_ipm_default_options(lk_ipm_options* opt);
+/* One LP in CSC (rows ascending per column): construct + solve.  lb/ub may be NULL (0, +inf);
+ * warm_* (original space) optional; x (n), y (m), s (n) outputs optional. */
+int lk_ipm_solve(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
+                 const double* vals, const double* c, const double* b, const double* lb,
+                 const double* ub, const lk_ipm_options* opt, const double* warm_x,
+                 const double* warm_y, const double* warm_s, lk_ipm_result* res, double* x,
+                 double* y, double* s);
+/* Reusable solver for a fixed A: setup once, solve for many (b, c). */
+int lk_ipm_create(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
+                  const double* vals, const double* c, const double* b,
+                  const lk_ipm_options* opt, lk_ipm** out);
+int lk_ipm_run(lk_ipm* h, const double* c, const double* b, const double* warm_x,
+               const double* warm_y, const double* warm_s, lk_ipm_result* res, double* x,
+               double* y, double* s);
+int lk_ipm_destroy(lk_ipm* h);
+
+/* ---- verifier::verify_optimal (src/verifier/certificate.hpp:26-31) with a tolerance ---- */
+int lk_verify_optimal(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
+                      const double* vals, const double* c, const double* b, const double* lb,
+                      const double* ub, int32_t maximize, const double* x, const double* pi,
+                      double tol, double* primal_residual, int32_t* pass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LAKSHYA_H */

@@ -1,0 +1,212 @@
+/*
+ * Device layer of the MI355X interior-point path: a C ABI over the HIP runtime and the
+ * hand-written gfx950 kernels.  Host C++ (the solver classes) includes only this header, never
+ * a HIP header; every argument is a plain pointer, size or scalar.  Device pointers are plain
+ * `void*`/typed pointers into device memory, `stream` is an opaque hipStream_t (NULL = default).
+ *
+ * Return value of every function: 0 on success, otherwise a HIP error code (lkd_error_string).
+ * Kernel entry points take a precision selector where the panel storage type varies:
+ *   LKD_F64 = 0 (double panels), LKD_F32 = 1 (float panels).  Vectors are always double.
+ *
+ * There is no counterpart of this layer in the reference (src/ipm/mehrotra_device.cu and
+ * src/cuda/kkt.cu launch their kernels inline); each kernel cites the reference kernel or
+ * library call it replaces.
+ */
+#ifndef LAKSHYA_DEVICE_H
+#define LAKSHYA_DEVICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LKD_F64 0
+#define LKD_F32 1
+
+/* ---------------------------------------------------------------- runtime plumbing */
+const char* lkd_error_string(int code);
+int lkd_device_count(int* count);
+int lkd_set_device(int dev);
+int lkd_malloc(void** ptr, size_t bytes);
+int lkd_free(void* ptr);
+int lkd_host_alloc(void** ptr, size_t bytes);     /* pinned host memory */
+int lkd_host_free(void* ptr);
+int lkd_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
+int lkd_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+int lkd_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+int lkd_memset(void* dst, int value, size_t bytes, void* stream);
+int lkd_stream_create(void** stream);
+int lkd_stream_destroy(void* stream);
+int lkd_stream_sync(void* stream);
+int lkd_event_create(void** ev);
+int lkd_event_destroy(void* ev);
+int lkd_event_record(void* ev, void* stream);
+int lkd_event_elapsed_ms(float* ms, void* start, void* stop);
+int lkd_graph_begin(void* stream);
+int lkd_graph_end(void* stream, void** exec);
+int lkd_graph_launch(void* exec, void* stream);
+int lkd_graph_destroy(void* exec);
+int lkd_fill_f64(double* dst, double value, int64_t n, void* stream);   /* kernel fill */
+
+/* ---------------------------------------------------------------- supernodal plan (device) */
+/* One work item of the factorization / triangular-solve schedule.  Field meaning per kernel:
+ *   off   panel offset (elements) of the supernode, ld = panel rows, ncol = panel columns
+ *   k0/kb block column start / width inside the panel
+ *   r0/c0 tile row / column start inside the panel (kernel specific)
+ *   sn    supernode id, first = first (permuted) column of the supernode                   */
+typedef struct {
+    int64_t off;
+    int64_t aux;     /* diagonal-copy offset (POTRF/TRSM/SYRK/TRSV) */
+    int64_t aux2;    /* offset of the supernode's row list (UPDATE/TRSV) */
+    int32_t ld, ncol;
+    int32_t k0, kb;
+    int32_t r0, c0;
+    int32_t sn, first;
+} lkd_task;
+
+typedef struct {
+    int32_t nsuper;
+    const int32_t* sn_first;    /* [nsuper+1] */
+    const int64_t* sn_rowptr;   /* [nsuper+1] */
+    const int32_t* sn_rows;     /* permuted row indices, sorted per supernode */
+    const int64_t* sn_panel;    /* [nsuper+1] panel offsets */
+    const int32_t* col_to_sn;   /* [m] */
+} lkd_snplan;
+
+/* ---------------------------------------------------------------- normal-equations assembly */
+/* w[p] = a[p] * theta[col(p)] over the CSC entries of A (one thread per column).
+ * Replaces the Theta factor inside compute_M_numerics_kernel (src/cuda/kkt.cu:177-215). */
+int lkd_assemble_prep(int32_t n, const int32_t* colptr, const double* aval, const double* theta,
+                      double* w, void* stream);
+/* Diagonal of M0 per permuted row from its pair-map segment (same fixed fma order as
+ * lkd_assemble_pairs, so the two agree bitwise): dseg[i] = segment of (i,i) or -1 (empty row).
+ * D_i = 1/sqrt(max(|M0_ii|, 1e-12)) when equilibrate != 0, else 1 (src/cuda/kkt.cu:65-79). */
+int lkd_assemble_diag(int32_t m, const int64_t* dseg, const int64_t* segptr, const int32_t* pa,
+                      const int32_t* pb, const double* w, const double* aval, int equilibrate,
+                      double* D, double* m0diag, void* stream);
+/* Segmented, fixed-order sums over the pair map: for segment t,
+ *   acc = sum_{q in [segptr[t], segptr[t+1])} fma(w[pa[q]], aval[pb[q]], acc)
+ *   panel[segdst[t]] = acc * D[segi[t]] * D[segj[t]] (+ delta on the diagonal of non-empty rows)
+ * Replaces compute_M_numerics_kernel + build_equilibrated_M_kernel / add_regularization_kernel
+ * (src/cuda/kkt.cu:81-94,157-171,177-215).  The panel must be zeroed beforehand. */
+int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int64_t* segdst,
+                       const int32_t* segi, const int32_t* segj, const int32_t* pa,
+                       const int32_t* pb, const double* w, const double* aval, const double* D,
+                       const uint8_t* empty_row, double delta, void* panel, void* stream);
+
+/* ---------------------------------------------------------------- supernodal Cholesky */
+/* Replaces cholesky_level_kernel and its level loop (src/cuda/kkt.cu:217-285,582-596).
+ * err: device int, set non-zero when a pivot is not positive; kernels return early once set. */
+/* diag: row-major copies of the factored diagonal blocks (task.aux = element offset), with the
+ * reciprocal on the diagonal; written by POTRF / SYRK (next diagonal), read by TRSM and TRSV. */
+int lkd_chol_potrf(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
+                   int32_t* err, void* stream);
+int lkd_chol_trsm(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
+                  int32_t* err, void* stream);
+int lkd_chol_syrk(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
+                  int32_t* err, void* stream);
+int lkd_chol_update(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
+                    lkd_snplan plan, int32_t* err, void* stream);
+
+/* ---------------------------------------------------------------- triangular solves */
+/* Replaces cusparseSpSV (L and L^T) (src/cuda/kkt.cu:731-734,788-789).  x is the permuted
+ * right-hand side, overwritten with the solution; rows index the supernode row lists. */
+/* fwd, first_step = 1: solve the first diagonal block of each listed supernode.
+ * fwd, first_step = 0: 256-row tiles of block column k0 update the rows below it (own rows in
+ *      place, rows of ancestors by atomic add); the tile starting at k0+kb < ncol then solves the
+ *      next diagonal block.
+ * bwd, below_step = 1: column block c0 subtracts L(below rows, block)^T x(below rows); the last
+ *      block of the supernode then solves its diagonal block with L^T.
+ * bwd, below_step = 0: column block c0 subtracts L(rows of block k0, block)^T x(block k0); the
+ *      block c0 = k0 - kb then solves its diagonal block. */
+int lkd_trsv_fwd(int prec, int32_t ntask, const lkd_task* tasks, const void* panel,
+                 const void* diag, const int32_t* sn_rows, double* x, int first_step,
+                 void* stream);
+int lkd_trsv_bwd(int prec, int32_t ntask, const lkd_task* tasks, const void* panel,
+                 const void* diag, const int32_t* sn_rows, double* x, int below_step,
+                 void* stream);
+
+/* ---------------------------------------------------------------- KKT solve vector kernels */
+/* r[i] = rhs[P[i]]; req[i] = D[i] * r[i] */
+int lkd_kkt_gather(int32_t m, const int32_t* P, const double* rhs, const double* D, double* r,
+                   double* req, void* stream);
+/* dy[i] = (accumulate ? dy[i] : 0) + D[i] * z[i];  optionally backup[i] = old dy[i] */
+int lkd_kkt_update(int32_t m, const double* D, const double* z, double* dy, double* backup,
+                   int accumulate, void* stream);
+/* r = rhs_perm - A_p (theta o (A_p^T dy)) with max |r| into *rinf (device double):
+ * the FP64 residual against the unregularised M0 (src/cuda/kkt.cu:110-123). */
+int lkd_kkt_residual(int32_t m, int32_t n, const int32_t* colptr, const int32_t* prow,
+                     const double* cval, const int32_t* prp, const int32_t* pci,
+                     const double* pval, const double* theta, const double* dy,
+                     const double* rhs_perm, double* u, double* r, double* D, double* req,
+                     double* rinf, void* stream);
+/* out[P[i]] = dy[i] (src/cuda/kkt.cu:105-108) */
+int lkd_kkt_scatter(int32_t m, const int32_t* P, const double* dy, double* out, void* stream);
+/* gather L (panel storage) into CSR values in the reference's L layout (diagonal last). */
+int lkd_panel_to_csr(int prec, int64_t nnz, const int64_t* src_index, const void* panel,
+                     double* vals, void* stream);
+
+/* ---------------------------------------------------------------- IPM vector kernels */
+/* Device scalar slots (doubles) shared by the IPM kernels. */
+enum {
+    LKS_RP2 = 0, LKS_RD2, LKS_XS, LKS_CX, LKS_BY, LKS_RPINF, LKS_RDINF,       /* residuals */
+    LKS_R1, LKS_R2, LKS_R3, LKS_RXS2, LKS_APMAX, LKS_ADMAX, LKS_DX2, LKS_DS2, /* direction */
+    LKS_MUAFF, LKS_SIGMAMU, LKS_AP, LKS_AD, LKS_DY2, LKS_KKTR,
+    LKS_NSLOTS
+};
+#define LKD_MAX_TRIALS 32
+/* line-search trial outputs: 6 values per trial after the fixed slots */
+#define LKS_TRIAL_BASE 32
+#define LKS_TRIAL_STRIDE 8
+#define LKS_TOTAL (LKS_TRIAL_BASE + LKS_TRIAL_STRIDE * LKD_MAX_TRIALS)
+
+typedef struct {
+    int32_t m, n;
+    const int32_t *arp, *aci; const double* aval;     /* A in CSR (scaled) */
+    const int32_t *trp, *tci; const double* tval;     /* A^T in CSR (= CSC of A) */
+    const double *b, *c, *R, *C;                      /* scaled data, scale vectors */
+    int32_t grp_rows, grp_cols;                       /* lanes per row of A / of A^T */
+    double* part;                                     /* block partials scratch */
+    int32_t nblk_rows, nblk_cols;                     /* blocks used for row / column passes */
+    double* S;                                        /* scalar slots [LKS_TOTAL] */
+} lkd_ipm;
+
+/* rp = A x - b, rd = A^T y + s - c, theta = x / s; slots RP2 RD2 XS CX BY RPINF RDINF
+ * (compute_residuals + compute_norm/compute_mu/compute_objective, mehrotra_device.cu:589-624) */
+int lkd_ipm_residuals(const lkd_ipm* P, const double* x, const double* y, const double* s,
+                      double* rp, double* rd, double* theta, void* stream);
+/* r_xs = -x s + S[sigma_slot] - (dxa ds a if dxa) ; v = -theta rd - r_xs / s ; rkkt = -rp + A v
+ * (compute_r_xs_kernel, compute_v_kernel, compute_rkkt: mehrotra_device.cu:111-126,627-638) */
+int lkd_ipm_rhs(const lkd_ipm* P, const double* x, const double* s, const double* theta,
+                const double* rd, const double* rp, const double* dxa, const double* dsa,
+                double sigma_mu, double* rxs, double* v, double* rkkt, void* stream);
+/* ds = -rd - A^T dy ; dx = -theta ds + r_xs / s ; qd = A^T dy + ds ; slots R2 R3 RXS2 APMAX
+ * ADMAX DX2 DS2 DY2; qp = A dx ; slot R1 = ||rp + A dx||^2
+ * (compute_ds, compute_dx_kernel, verify_newton_direction, step_length_functor:
+ *  mehrotra_device.cu:128-133,168-176,494-587,641-660) */
+int lkd_ipm_direction(const lkd_ipm* P, const double* x, const double* s, const double* y,
+                      const double* theta, const double* rd, const double* rp,
+                      const double* rxs, const double* dy, double* dx, double* ds, double* qd,
+                      double* qp, void* stream);
+/* slot MUAFF = (x + ap dxa)^T (s + ad dsa)  (mehrotra_device.cu:831-835) */
+int lkd_ipm_mu_aff(const lkd_ipm* P, const double* x, const double* s, const double* dxa,
+                   const double* dsa, double ap, double ad, void* stream);
+/* All backtracking trials t = 0..ntrial-1 with (ap, ad) * 0.5^t evaluated in one pass:
+ * per trial ||rp + ap_t qp||^2, ||rd + ad_t qd||^2, (x + ap_t dx)^T (s + ad_t ds),
+ * min(x + ap_t dx), min(s + ad_t ds), ||y + ad_t dy||^2   (mehrotra_device.cu:920-995) */
+int lkd_ipm_trials(const lkd_ipm* P, const double* x, const double* s, const double* y,
+                   const double* dx, const double* ds, const double* dy, const double* rp,
+                   const double* rd, const double* qp, const double* qd, double ap, double ad,
+                   int32_t ntrial, void* stream);
+/* x += ap dx ; s += ad ds ; y += ad dy  (mehrotra_device.cu:1000-1005) */
+int lkd_ipm_commit(const lkd_ipm* P, double* x, double* s, double* y, const double* dx,
+                   const double* ds, const double* dy, double ap, double ad, void* stream);
+/* x = 1, s = 1, y = 0 (init_vars_kernel, mehrotra_device.cu:93-102) */
+int lkd_ipm_init(int32_t m, int32_t n, double* x, double* y, double* s, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LAKSHYA_DEVICE_H */

@@ -1,0 +1,678 @@
+// Normal-equations assembly, KKT-solve vector kernels and the fused interior-point vector kernels.
+//
+// Reductions are deterministic: every fused kernel writes one partial per block and quantity
+// (part[q * nblk + block]), and k_finalize reduces each quantity with a fixed tree.  The IPM
+// scalars stay on the device (slots in S) and the host reads them once per decision point,
+// replacing the ~20-60 Thrust reductions per iteration of src/ipm/mehrotra_device.cu:191-259.
+#include "common.hpp"
+
+namespace {
+
+constexpr int NT = 256;
+enum { OP_SUM = 0, OP_MIN = 1, OP_MAX = 2 };
+
+__device__ __forceinline__ double op_apply(int op, double a, double b) {
+    return op == OP_SUM ? a + b : (op == OP_MIN ? fmin(a, b) : fmax(a, b));
+}
+__device__ __forceinline__ double op_ident(int op) {
+    return op == OP_SUM ? 0.0 : (op == OP_MIN ? 1e300 : -1e300);
+}
+
+// Block-wide reduction of v (all 256 threads participate); result valid in thread 0.
+__device__ __forceinline__ double block_reduce(double v, int op, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v = op_apply(op, v, __shfl_xor(v, o, 64));
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = op_apply(op, op_apply(op, sh[0], sh[1]), op_apply(op, sh[2], sh[3]));
+    return r;
+}
+
+struct FinMeta {
+    int32_t q;            // number of quantities
+    int32_t nblk;
+    int8_t op[200];
+    int16_t slot[200];
+};
+
+__global__ __launch_bounds__(NT) void k_finalize(const double* __restrict__ part, FinMeta meta,
+                                                 double* __restrict__ S) {
+    __shared__ double sh[4];
+    const int q = blockIdx.x;
+    const int op = meta.op[q];
+    double v = op_ident(op);
+    for (int b = threadIdx.x; b < meta.nblk; b += NT) v = op_apply(op, v, part[q * meta.nblk + b]);
+    v = block_reduce(v, op, sh);
+    if (threadIdx.x == 0) S[meta.slot[q]] = v;
+}
+
+int finalize(const double* part, int nblk, int q, const int* ops, const int* slots, double* S,
+             void* stream) {
+    FinMeta meta;
+    meta.q = q;
+    meta.nblk = nblk;
+    for (int i = 0; i < q; ++i) { meta.op[i] = (int8_t)ops[i]; meta.slot[i] = (int16_t)slots[i]; }
+    LK_LAUNCH(k_finalize, q, NT, 0, stream, part, meta, S);
+    return lk_last_error();
+}
+
+// Group of G lanes per row: returns the row dot product in every lane of the group.
+template <int G>
+__device__ __forceinline__ double row_dot(const int32_t* __restrict__ rp,
+                                          const int32_t* __restrict__ ci,
+                                          const double* __restrict__ val,
+                                          const double* __restrict__ x, int64_t i, int lane) {
+    double s = 0.0;
+    for (int32_t p = rp[i] + lane; p < rp[i + 1]; p += G) s += val[p] * x[ci[p]];
+    return group_sum<G>(s);
+}
+
+// ------------------------------------------------------------------------------------------
+// Assembly
+__global__ __launch_bounds__(NT) void k_prep(int32_t n, const int32_t* __restrict__ colptr,
+                                             const double* __restrict__ aval,
+                                             const double* __restrict__ theta,
+                                             double* __restrict__ w) {
+    for (int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x; j < n; j += (int64_t)gridDim.x * NT) {
+        const double t = theta[j];
+        for (int32_t p = colptr[j]; p < colptr[j + 1]; ++p) w[p] = aval[p] * t;
+    }
+}
+
+__device__ __forceinline__ double seg_sum(const int64_t* __restrict__ segptr,
+                                          const int32_t* __restrict__ pa,
+                                          const int32_t* __restrict__ pb,
+                                          const double* __restrict__ w,
+                                          const double* __restrict__ aval, int64_t t) {
+    double acc = 0.0;
+    for (int64_t q = segptr[t]; q < segptr[t + 1]; ++q) acc = fma(w[pa[q]], aval[pb[q]], acc);
+    return acc;
+}
+
+__global__ __launch_bounds__(NT) void k_diag(int32_t m, const int64_t* __restrict__ dseg,
+                                             const int64_t* __restrict__ segptr,
+                                             const int32_t* __restrict__ pa,
+                                             const int32_t* __restrict__ pb,
+                                             const double* __restrict__ w,
+                                             const double* __restrict__ aval, int equil,
+                                             double* __restrict__ D, double* __restrict__ m0d) {
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < m; i += (int64_t)gridDim.x * NT) {
+        const int64_t t = dseg[i];
+        const double mii = t >= 0 ? seg_sum(segptr, pa, pb, w, aval, t) : 0.0;
+        if (m0d) m0d[i] = mii;
+        double a = fabs(mii);
+        if (a < 1e-12) a = 1e-12;
+        D[i] = equil ? 1.0 / sqrt(a) : 1.0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_pairs(int64_t nseg, const int64_t* __restrict__ segptr,
+                                              const int64_t* __restrict__ segdst,
+                                              const int32_t* __restrict__ segi,
+                                              const int32_t* __restrict__ segj,
+                                              const int32_t* __restrict__ pa,
+                                              const int32_t* __restrict__ pb,
+                                              const double* __restrict__ w,
+                                              const double* __restrict__ aval,
+                                              const double* __restrict__ D,
+                                              const uint8_t* __restrict__ empty, double delta,
+                                              T* __restrict__ panel) {
+    for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < nseg;
+         t += (int64_t)gridDim.x * NT) {
+        const int32_t i = segi[t], j = segj[t];
+        double v = seg_sum(segptr, pa, pb, w, aval, t) * D[i] * D[j];
+        if (i == j && !empty[i]) v += delta;
+        panel[segdst[t]] = (T)v;
+    }
+}
+
+int grid_for(int64_t n, int64_t cap = 8192) {
+    int64_t b = (n + NT - 1) / NT;
+    if (b < 1) b = 1;
+    return (int)(b < cap ? b : cap);
+}
+
+// ------------------------------------------------------------------------------------------
+// KKT solve vectors
+__global__ __launch_bounds__(NT) void k_gather(int32_t m, const int32_t* __restrict__ P,
+                                               const double* __restrict__ rhs,
+                                               const double* __restrict__ D,
+                                               double* __restrict__ r, double* __restrict__ req) {
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < m; i += (int64_t)gridDim.x * NT) {
+        const double v = rhs[P[i]];
+        r[i] = v;
+        req[i] = v * D[i];
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_kupdate(int32_t m, const double* __restrict__ D,
+                                                const double* __restrict__ z,
+                                                double* __restrict__ dy,
+                                                double* __restrict__ backup, int acc) {
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < m; i += (int64_t)gridDim.x * NT) {
+        const double corr = z[i] * D[i];
+        if (acc) {
+            const double old = dy[i];
+            if (backup) backup[i] = old;
+            dy[i] = old + corr;
+        } else {
+            dy[i] = corr;
+        }
+    }
+}
+
+// u_j = theta_j * sum_i A_p[i][j] dy_i   (column pass over the CSC with permuted row indices)
+__global__ __launch_bounds__(NT) void k_res_cols(int32_t n, const int32_t* __restrict__ colptr,
+                                                 const int32_t* __restrict__ prow,
+                                                 const double* __restrict__ cval,
+                                                 const double* __restrict__ theta,
+                                                 const double* __restrict__ dy,
+                                                 double* __restrict__ u) {
+    for (int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x; j < n; j += (int64_t)gridDim.x * NT) {
+        double s = 0.0;
+        for (int32_t p = colptr[j]; p < colptr[j + 1]; ++p) s += cval[p] * dy[prow[p]];
+        u[j] = theta[j] * s;
+    }
+}
+
+// r_i = rhs_i - sum_j A_p[i][j] u_j ; req_i = D_i r_i ; partial max |r_i|
+template <int G>
+__global__ __launch_bounds__(NT) void k_res_rows(int32_t m, const int32_t* __restrict__ prp,
+                                                 const int32_t* __restrict__ pci,
+                                                 const double* __restrict__ pval,
+                                                 const double* __restrict__ u,
+                                                 const double* __restrict__ rhs,
+                                                 double* __restrict__ r,
+                                                 const double* __restrict__ D,
+                                                 double* __restrict__ req,
+                                                 double* __restrict__ part) {
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
+    const int64_t ng = (int64_t)gridDim.x * NT / G;
+    double mx = 0.0;
+    for (int64_t i = g0; i < m; i += ng) {
+        const double s = row_dot<G>(prp, pci, pval, u, i, lane);
+        if (lane == 0) {
+            const double v = rhs[i] - s;
+            r[i] = v;
+            req[i] = v * D[i];
+            mx = fmax(mx, fabs(v));
+        }
+    }
+    mx = block_reduce(mx, OP_MAX, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = mx;
+}
+
+__global__ __launch_bounds__(NT) void k_scatter(int32_t m, const int32_t* __restrict__ P,
+                                                const double* __restrict__ dy,
+                                                double* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < m; i += (int64_t)gridDim.x * NT)
+        out[P[i]] = dy[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_panel_csr(int64_t nnz, const int64_t* __restrict__ idx,
+                                                  const T* __restrict__ panel,
+                                                  double* __restrict__ vals) {
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * NT)
+        vals[i] = (double)panel[idx[i]];
+}
+
+// ------------------------------------------------------------------------------------------
+// Interior-point vector kernels.  Row passes use G lanes per row of A, column passes G lanes per
+// row of A^T.  Blocks write partials part[q * nblk + block].
+template <int G>
+__global__ __launch_bounds__(NT) void k_resid_rows(lkd_ipm P, const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   double* __restrict__ rp) {
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
+    const int64_t ng = (int64_t)gridDim.x * NT / G;
+    double s2 = 0.0, inf = 0.0, by = 0.0;
+    for (int64_t i = g0; i < P.m; i += ng) {
+        const double ax = row_dot<G>(P.arp, P.aci, P.aval, x, i, lane);
+        if (lane == 0) {
+            const double v = ax + (-P.b[i]);
+            rp[i] = v;
+            s2 += v * v;
+            inf = fmax(inf, fabs(v / P.R[i]));
+            by += P.b[i] * y[i];
+        }
+    }
+    const int nb = gridDim.x;
+    double r;
+    r = block_reduce(s2, OP_SUM, sh); if (threadIdx.x == 0) P.part[0 * nb + blockIdx.x] = r;
+    r = block_reduce(inf, OP_MAX, sh); if (threadIdx.x == 0) P.part[1 * nb + blockIdx.x] = r;
+    r = block_reduce(by, OP_SUM, sh); if (threadIdx.x == 0) P.part[2 * nb + blockIdx.x] = r;
+}
+
+template <int G>
+__global__ __launch_bounds__(NT) void k_resid_cols(lkd_ipm P, const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ s,
+                                                   double* __restrict__ rd,
+                                                   double* __restrict__ theta, int qoff) {
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
+    const int64_t ng = (int64_t)gridDim.x * NT / G;
+    double s2 = 0.0, xs = 0.0, cx = 0.0, inf = 0.0;
+    for (int64_t j = g0; j < P.n; j += ng) {
+        const double aty = row_dot<G>(P.trp, P.tci, P.tval, y, j, lane);
+        if (lane == 0) {
+            const double v = aty + (s[j] - P.c[j]);
+            rd[j] = v;
+            if (theta) theta[j] = x[j] / s[j];
+            s2 += v * v;
+            xs += x[j] * s[j];
+            cx += P.c[j] * x[j];
+            inf = fmax(inf, fabs(v / P.C[j]));
+        }
+    }
+    const int nb = gridDim.x;
+    double* part = P.part + (int64_t)qoff * nb;
+    double r;
+    r = block_reduce(s2, OP_SUM, sh); if (threadIdx.x == 0) part[0 * nb + blockIdx.x] = r;
+    r = block_reduce(xs, OP_SUM, sh); if (threadIdx.x == 0) part[1 * nb + blockIdx.x] = r;
+    r = block_reduce(cx, OP_SUM, sh); if (threadIdx.x == 0) part[2 * nb + blockIdx.x] = r;
+    r = block_reduce(inf, OP_MAX, sh); if (threadIdx.x == 0) part[3 * nb + blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(NT) void k_rhs_cols(lkd_ipm P, const double* __restrict__ x,
+                                                 const double* __restrict__ s,
+                                                 const double* __restrict__ theta,
+                                                 const double* __restrict__ rd,
+                                                 const double* __restrict__ dxa,
+                                                 const double* __restrict__ dsa, double smu,
+                                                 double* __restrict__ rxs,
+                                                 double* __restrict__ v) {
+    for (int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x; j < P.n; j += (int64_t)gridDim.x * NT) {
+        const double aff = dxa ? dxa[j] * dsa[j] : 0.0;
+        const double r = (-x[j] * s[j] + smu) - aff;
+        rxs[j] = r;
+        v[j] = -theta[j] * rd[j] - r / s[j];
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(NT) void k_rkkt_rows(lkd_ipm P, const double* __restrict__ v,
+                                                  const double* __restrict__ rp,
+                                                  double* __restrict__ rkkt) {
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
+    const int64_t ng = (int64_t)gridDim.x * NT / G;
+    for (int64_t i = g0; i < P.m; i += ng) {
+        const double av = row_dot<G>(P.arp, P.aci, P.aval, v, i, lane);
+        if (lane == 0) rkkt[i] = av + (-rp[i]);
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(NT) void k_dir_cols(lkd_ipm P, const double* __restrict__ x,
+                                                 const double* __restrict__ s,
+                                                 const double* __restrict__ theta,
+                                                 const double* __restrict__ rd,
+                                                 const double* __restrict__ rxs,
+                                                 const double* __restrict__ dy,
+                                                 double* __restrict__ dx, double* __restrict__ ds,
+                                                 double* __restrict__ qd) {
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
+    const int64_t ng = (int64_t)gridDim.x * NT / G;
+    double r2 = 0, r3 = 0, rx = 0, apm = 1e30, adm = 1e30, dx2 = 0, ds2 = 0;
+    for (int64_t j = g0; j < P.n; j += ng) {
+        const double t = row_dot<G>(P.trp, P.tci, P.tval, dy, j, lane);
+        if (lane == 0) {
+            const double dsj = (-t) + (-rd[j]);
+            const double dxj = -theta[j] * dsj + rxs[j] / s[j];
+            ds[j] = dsj;
+            dx[j] = dxj;
+            qd[j] = t + dsj;
+            const double e2 = t + (dsj + rd[j]);
+            const double e3 = s[j] * dxj + x[j] * dsj - rxs[j];
+            r2 += e2 * e2;
+            r3 += e3 * e3;
+            rx += rxs[j] * rxs[j];
+            if (dxj < 0.0) apm = fmin(apm, -x[j] / dxj);
+            if (dsj < 0.0) adm = fmin(adm, -s[j] / dsj);
+            dx2 += dxj * dxj;
+            ds2 += dsj * dsj;
+        }
+    }
+    const int nb = gridDim.x;
+    double r;
+    r = block_reduce(r2, OP_SUM, sh); if (threadIdx.x == 0) P.part[0 * nb + blockIdx.x] = r;
+    r = block_reduce(r3, OP_SUM, sh); if (threadIdx.x == 0) P.part[1 * nb + blockIdx.x] = r;
+    r = block_reduce(rx, OP_SUM, sh); if (threadIdx.x == 0) P.part[2 * nb + blockIdx.x] = r;
+    r = block_reduce(apm, OP_MIN, sh); if (threadIdx.x == 0) P.part[3 * nb + blockIdx.x] = r;
+    r = block_reduce(adm, OP_MIN, sh); if (threadIdx.x == 0) P.part[4 * nb + blockIdx.x] = r;
+    r = block_reduce(dx2, OP_SUM, sh); if (threadIdx.x == 0) P.part[5 * nb + blockIdx.x] = r;
+    r = block_reduce(ds2, OP_SUM, sh); if (threadIdx.x == 0) P.part[6 * nb + blockIdx.x] = r;
+}
+
+template <int G>
+__global__ __launch_bounds__(NT) void k_dir_rows(lkd_ipm P, const double* __restrict__ dx,
+                                                 const double* __restrict__ rp,
+                                                 const double* __restrict__ dy,
+                                                 double* __restrict__ qp, int qoff) {
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
+    const int64_t ng = (int64_t)gridDim.x * NT / G;
+    double r1 = 0, dy2 = 0;
+    for (int64_t i = g0; i < P.m; i += ng) {
+        const double adx = row_dot<G>(P.arp, P.aci, P.aval, dx, i, lane);
+        if (lane == 0) {
+            qp[i] = adx;
+            const double e1 = adx + rp[i];
+            r1 += e1 * e1;
+            dy2 += dy[i] * dy[i];
+        }
+    }
+    const int nb = gridDim.x;
+    double* part = P.part + (int64_t)qoff * nb;
+    double r;
+    r = block_reduce(r1, OP_SUM, sh); if (threadIdx.x == 0) part[0 * nb + blockIdx.x] = r;
+    r = block_reduce(dy2, OP_SUM, sh); if (threadIdx.x == 0) part[1 * nb + blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(NT) void k_muaff(lkd_ipm P, const double* __restrict__ x,
+                                              const double* __restrict__ s,
+                                              const double* __restrict__ dxa,
+                                              const double* __restrict__ dsa, double ap,
+                                              double ad) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x; j < P.n; j += (int64_t)gridDim.x * NT) {
+        const double tx = x[j] + ap * dxa[j];
+        const double ts = s[j] + ad * dsa[j];
+        acc += tx * ts;
+    }
+    acc = block_reduce(acc, OP_SUM, sh);
+    if (threadIdx.x == 0) P.part[blockIdx.x] = acc;
+}
+
+// All backtracking trials in one pass (rp_t = rp + ap_t A dx, rd_t = rd + ad_t (A^T dy + ds)).
+__global__ __launch_bounds__(NT) void k_trials_cols(lkd_ipm P, const double* __restrict__ x,
+                                                    const double* __restrict__ s,
+                                                    const double* __restrict__ dx,
+                                                    const double* __restrict__ ds,
+                                                    const double* __restrict__ rd,
+                                                    const double* __restrict__ qd, double ap,
+                                                    double ad, int nt) {
+    __shared__ double sh[4];
+    double rd2[LKD_MAX_TRIALS], xs[LKD_MAX_TRIALS], mx[LKD_MAX_TRIALS], ms[LKD_MAX_TRIALS];
+    for (int t = 0; t < nt; ++t) { rd2[t] = 0; xs[t] = 0; mx[t] = 1e30; ms[t] = 1e30; }
+    for (int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x; j < P.n; j += (int64_t)gridDim.x * NT) {
+        const double xj = x[j], sj = s[j], dxj = dx[j], dsj = ds[j], rdj = rd[j], qdj = qd[j];
+        double a = ap, b = ad;
+        for (int t = 0; t < nt; ++t) {
+            const double tx = xj + a * dxj, ts = sj + b * dsj, tr = rdj + b * qdj;
+            rd2[t] += tr * tr;
+            xs[t] += tx * ts;
+            mx[t] = fmin(mx[t], tx);
+            ms[t] = fmin(ms[t], ts);
+            a *= 0.5;
+            b *= 0.5;
+        }
+    }
+    const int nb = gridDim.x;
+    for (int t = 0; t < nt; ++t) {
+        double r;
+        r = block_reduce(rd2[t], OP_SUM, sh); if (threadIdx.x == 0) P.part[(4 * t + 0) * nb + blockIdx.x] = r;
+        r = block_reduce(xs[t], OP_SUM, sh); if (threadIdx.x == 0) P.part[(4 * t + 1) * nb + blockIdx.x] = r;
+        r = block_reduce(mx[t], OP_MIN, sh); if (threadIdx.x == 0) P.part[(4 * t + 2) * nb + blockIdx.x] = r;
+        r = block_reduce(ms[t], OP_MIN, sh); if (threadIdx.x == 0) P.part[(4 * t + 3) * nb + blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_trials_rows(lkd_ipm P, const double* __restrict__ rp,
+                                                    const double* __restrict__ qp,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ dy, double ap,
+                                                    double ad, int nt, int qoff) {
+    __shared__ double sh[4];
+    double rp2[LKD_MAX_TRIALS], y2[LKD_MAX_TRIALS];
+    for (int t = 0; t < nt; ++t) { rp2[t] = 0; y2[t] = 0; }
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < P.m; i += (int64_t)gridDim.x * NT) {
+        const double rpi = rp[i], qpi = qp[i], yi = y[i], dyi = dy[i];
+        double a = ap, b = ad;
+        for (int t = 0; t < nt; ++t) {
+            const double tr = rpi + a * qpi, ty = yi + b * dyi;
+            rp2[t] += tr * tr;
+            y2[t] += ty * ty;
+            a *= 0.5;
+            b *= 0.5;
+        }
+    }
+    const int nb = gridDim.x;
+    double* part = P.part + (int64_t)qoff * nb;
+    for (int t = 0; t < nt; ++t) {
+        double r;
+        r = block_reduce(rp2[t], OP_SUM, sh); if (threadIdx.x == 0) part[(2 * t + 0) * nb + blockIdx.x] = r;
+        r = block_reduce(y2[t], OP_SUM, sh); if (threadIdx.x == 0) part[(2 * t + 1) * nb + blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_commit(int32_t m, int32_t n, double* __restrict__ x,
+                                               double* __restrict__ s, double* __restrict__ y,
+                                               const double* __restrict__ dx,
+                                               const double* __restrict__ ds,
+                                               const double* __restrict__ dy, double ap,
+                                               double ad) {
+    const int64_t len = m > n ? m : n;
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < len; i += (int64_t)gridDim.x * NT) {
+        if (i < n) { x[i] += ap * dx[i]; s[i] += ad * ds[i]; }
+        if (i < m) y[i] += ad * dy[i];
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_init(int32_t m, int32_t n, double* __restrict__ x,
+                                             double* __restrict__ y, double* __restrict__ s) {
+    const int64_t len = m > n ? m : n;
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < len; i += (int64_t)gridDim.x * NT) {
+        if (i < n) { x[i] = 1.0; s[i] = 1.0; }
+        if (i < m) y[i] = 0.0;
+    }
+}
+
+#define LK_GSWITCH(G, KERNEL, ...)                                                    \
+    switch (G) {                                                                      \
+        case 1: KERNEL<1>(__VA_ARGS__); break;                                        \
+        case 2: KERNEL<2>(__VA_ARGS__); break;                                        \
+        case 4: KERNEL<4>(__VA_ARGS__); break;                                        \
+        case 8: KERNEL<8>(__VA_ARGS__); break;                                        \
+        case 16: KERNEL<16>(__VA_ARGS__); break;                                      \
+        case 32: KERNEL<32>(__VA_ARGS__); break;                                      \
+        default: KERNEL<64>(__VA_ARGS__); break;                                      \
+    }
+
+template <int G> void l_resid_rows(const lkd_ipm* P, const double* x, const double* y, double* rp, void* st) {
+    LK_LAUNCH(k_resid_rows<G>, P->nblk_rows, NT, 0, st, *P, x, y, rp);
+}
+template <int G> void l_resid_cols(const lkd_ipm* P, const double* x, const double* y, const double* s,
+                                   double* rd, double* th, int qoff, void* st) {
+    LK_LAUNCH(k_resid_cols<G>, P->nblk_cols, NT, 0, st, *P, x, y, s, rd, th, qoff);
+}
+template <int G> void l_rkkt(const lkd_ipm* P, const double* v, const double* rp, double* rk, void* st) {
+    LK_LAUNCH(k_rkkt_rows<G>, P->nblk_rows, NT, 0, st, *P, v, rp, rk);
+}
+template <int G> void l_dir_cols(const lkd_ipm* P, const double* x, const double* s, const double* th,
+                                 const double* rd, const double* rxs, const double* dy, double* dx,
+                                 double* ds, double* qd, void* st) {
+    LK_LAUNCH(k_dir_cols<G>, P->nblk_cols, NT, 0, st, *P, x, s, th, rd, rxs, dy, dx, ds, qd);
+}
+template <int G> void l_dir_rows(const lkd_ipm* P, const double* dx, const double* rp, const double* dy,
+                                 double* qp, int qoff, void* st) {
+    LK_LAUNCH(k_dir_rows<G>, P->nblk_rows, NT, 0, st, *P, dx, rp, dy, qp, qoff);
+}
+template <int G> void l_res_rows(int32_t m, const int32_t* prp, const int32_t* pci, const double* pval,
+                                 const double* u, const double* rhs, double* r, const double* D,
+                                 double* req, double* part, int nblk, void* st) {
+    LK_LAUNCH(k_res_rows<G>, nblk, NT, 0, st, m, prp, pci, pval, u, rhs, r, D, req, part);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lkd_assemble_prep(int32_t n, const int32_t* colptr, const double* aval, const double* theta,
+                      double* w, void* stream) {
+    LK_LAUNCH(k_prep, grid_for(n), NT, 0, stream, n, colptr, aval, theta, w);
+    return lk_last_error();
+}
+
+int lkd_assemble_diag(int32_t m, const int64_t* dseg, const int64_t* segptr, const int32_t* pa,
+                      const int32_t* pb, const double* w, const double* aval, int equilibrate,
+                      double* D, double* m0diag, void* stream) {
+    LK_LAUNCH(k_diag, grid_for(m), NT, 0, stream, m, dseg, segptr, pa, pb, w, aval, equilibrate,
+              D, m0diag);
+    return lk_last_error();
+}
+
+int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int64_t* segdst,
+                       const int32_t* segi, const int32_t* segj, const int32_t* pa,
+                       const int32_t* pb, const double* w, const double* aval, const double* D,
+                       const uint8_t* empty_row, double delta, void* panel, void* stream) {
+    if (nseg <= 0) return 0;
+    if (prec == LKD_F32)
+        LK_LAUNCH(k_pairs<float>, grid_for(nseg, 16384), NT, 0, stream, nseg, segptr, segdst, segi,
+                  segj, pa, pb, w, aval, D, empty_row, delta, (float*)panel);
+    else
+        LK_LAUNCH(k_pairs<double>, grid_for(nseg, 16384), NT, 0, stream, nseg, segptr, segdst, segi,
+                  segj, pa, pb, w, aval, D, empty_row, delta, (double*)panel);
+    return lk_last_error();
+}
+
+int lkd_kkt_gather(int32_t m, const int32_t* P, const double* rhs, const double* D, double* r,
+                   double* req, void* stream) {
+    LK_LAUNCH(k_gather, grid_for(m), NT, 0, stream, m, P, rhs, D, r, req);
+    return lk_last_error();
+}
+
+int lkd_kkt_update(int32_t m, const double* D, const double* z, double* dy, double* backup,
+                   int accumulate, void* stream) {
+    LK_LAUNCH(k_kupdate, grid_for(m), NT, 0, stream, m, D, z, dy, backup, accumulate);
+    return lk_last_error();
+}
+
+int lkd_kkt_residual(int32_t m, int32_t n, const int32_t* colptr, const int32_t* prow,
+                     const double* cval, const int32_t* prp, const int32_t* pci,
+                     const double* pval, const double* theta, const double* dy,
+                     const double* rhs_perm, double* u, double* r, double* D, double* req,
+                     double* rinf, void* stream) {
+    LK_LAUNCH(k_res_cols, grid_for(n), NT, 0, stream, n, colptr, prow, cval, theta, dy, u);
+    // rinf points at a device double; block partials are placed right after it (caller sizes it)
+    const int nblk = grid_for(m, 1024);
+    double* part = rinf + 1;
+    const int64_t nnz = prp[0] == 0 ? 0 : 0;  // (row lengths are not inspected on the host)
+    (void)nnz;
+    l_res_rows<8>(m, prp, pci, pval, u, rhs_perm, r, D, req, part, nblk, stream);
+    int op = OP_MAX, slot = 0;
+    return finalize(part, nblk, 1, &op, &slot, rinf, stream);
+}
+
+int lkd_kkt_scatter(int32_t m, const int32_t* P, const double* dy, double* out, void* stream) {
+    LK_LAUNCH(k_scatter, grid_for(m), NT, 0, stream, m, P, dy, out);
+    return lk_last_error();
+}
+
+int lkd_panel_to_csr(int prec, int64_t nnz, const int64_t* src_index, const void* panel,
+                     double* vals, void* stream) {
+    if (nnz <= 0) return 0;
+    if (prec == LKD_F32)
+        LK_LAUNCH(k_panel_csr<float>, grid_for(nnz), NT, 0, stream, nnz, src_index, (const float*)panel, vals);
+    else
+        LK_LAUNCH(k_panel_csr<double>, grid_for(nnz), NT, 0, stream, nnz, src_index, (const double*)panel, vals);
+    return lk_last_error();
+}
+
+int lkd_ipm_init(int32_t m, int32_t n, double* x, double* y, double* s, void* stream) {
+    LK_LAUNCH(k_init, grid_for(m > n ? m : n), NT, 0, stream, m, n, x, y, s);
+    return lk_last_error();
+}
+
+int lkd_ipm_residuals(const lkd_ipm* P, const double* x, const double* y, const double* s,
+                      double* rp, double* rd, double* theta, void* stream) {
+    LK_GSWITCH(P->grp_rows, l_resid_rows, P, x, y, rp, stream);
+    int ops_r[3] = {OP_SUM, OP_MAX, OP_SUM}, slots_r[3] = {LKS_RP2, LKS_RPINF, LKS_BY};
+    int e = finalize(P->part, P->nblk_rows, 3, ops_r, slots_r, P->S, stream);
+    if (e) return e;
+    // column pass writes its partials after the three row quantities
+    const int qoff = (3 * P->nblk_rows + P->nblk_cols - 1) / P->nblk_cols;
+    LK_GSWITCH(P->grp_cols, l_resid_cols, P, x, y, s, rd, theta, qoff, stream);
+    int ops_c[4] = {OP_SUM, OP_SUM, OP_SUM, OP_MAX}, slots_c[4] = {LKS_RD2, LKS_XS, LKS_CX, LKS_RDINF};
+    return finalize(P->part + (int64_t)qoff * P->nblk_cols, P->nblk_cols, 4, ops_c, slots_c, P->S, stream);
+}
+
+int lkd_ipm_rhs(const lkd_ipm* P, const double* x, const double* s, const double* theta,
+                const double* rd, const double* rp, const double* dxa, const double* dsa,
+                double sigma_mu, double* rxs, double* v, double* rkkt, void* stream) {
+    LK_LAUNCH(k_rhs_cols, P->nblk_cols, NT, 0, stream, *P, x, s, theta, rd, dxa, dsa, sigma_mu, rxs, v);
+    LK_GSWITCH(P->grp_rows, l_rkkt, P, v, rp, rkkt, stream);
+    return lk_last_error();
+}
+
+int lkd_ipm_direction(const lkd_ipm* P, const double* x, const double* s, const double* y,
+                      const double* theta, const double* rd, const double* rp,
+                      const double* rxs, const double* dy, double* dx, double* ds, double* qd,
+                      double* qp, void* stream) {
+    (void)y;
+    LK_GSWITCH(P->grp_cols, l_dir_cols, P, x, s, theta, rd, rxs, dy, dx, ds, qd, stream);
+    int ops_c[7] = {OP_SUM, OP_SUM, OP_SUM, OP_MIN, OP_MIN, OP_SUM, OP_SUM};
+    int slots_c[7] = {LKS_R2, LKS_R3, LKS_RXS2, LKS_APMAX, LKS_ADMAX, LKS_DX2, LKS_DS2};
+    int e = finalize(P->part, P->nblk_cols, 7, ops_c, slots_c, P->S, stream);
+    if (e) return e;
+    const int qoff = (7 * P->nblk_cols + P->nblk_rows - 1) / P->nblk_rows;
+    LK_GSWITCH(P->grp_rows, l_dir_rows, P, dx, rp, dy, qp, qoff, stream);
+    int ops_r[2] = {OP_SUM, OP_SUM}, slots_r[2] = {LKS_R1, LKS_DY2};
+    return finalize(P->part + (int64_t)qoff * P->nblk_rows, P->nblk_rows, 2, ops_r, slots_r, P->S, stream);
+}
+
+int lkd_ipm_mu_aff(const lkd_ipm* P, const double* x, const double* s, const double* dxa,
+                   const double* dsa, double ap, double ad, void* stream) {
+    LK_LAUNCH(k_muaff, P->nblk_cols, NT, 0, stream, *P, x, s, dxa, dsa, ap, ad);
+    int op = OP_SUM, slot = LKS_MUAFF;
+    return finalize(P->part, P->nblk_cols, 1, &op, &slot, P->S, stream);
+}
+
+int lkd_ipm_trials(const lkd_ipm* P, const double* x, const double* s, const double* y,
+                   const double* dx, const double* ds, const double* dy, const double* rp,
+                   const double* rd, const double* qp, const double* qd, double ap, double ad,
+                   int32_t ntrial, void* stream) {
+    if (ntrial > LKD_MAX_TRIALS) ntrial = LKD_MAX_TRIALS;
+    LK_LAUNCH(k_trials_cols, P->nblk_cols, NT, 0, stream, *P, x, s, dx, ds, rd, qd, ap, ad, ntrial);
+    int ops[4 * LKD_MAX_TRIALS], slots[4 * LKD_MAX_TRIALS];
+    for (int t = 0; t < ntrial; ++t) {
+        const int base = LKS_TRIAL_BASE + t * LKS_TRIAL_STRIDE;
+        ops[4 * t + 0] = OP_SUM; slots[4 * t + 0] = base + 1;   // ||rd_t||^2
+        ops[4 * t + 1] = OP_SUM; slots[4 * t + 1] = base + 2;   // x_t^T s_t
+        ops[4 * t + 2] = OP_MIN; slots[4 * t + 2] = base + 3;   // min x_t
+        ops[4 * t + 3] = OP_MIN; slots[4 * t + 3] = base + 4;   // min s_t
+    }
+    int e = finalize(P->part, P->nblk_cols, 4 * ntrial, ops, slots, P->S, stream);
+    if (e) return e;
+    const int qoff = (4 * ntrial * P->nblk_cols + P->nblk_rows - 1) / P->nblk_rows;
+    LK_LAUNCH(k_trials_rows, P->nblk_rows, NT, 0, stream, *P, rp, qp, y, dy, ap, ad, ntrial, qoff);
+    for (int t = 0; t < ntrial; ++t) {
+        const int base = LKS_TRIAL_BASE + t * LKS_TRIAL_STRIDE;
+        ops[2 * t + 0] = OP_SUM; slots[2 * t + 0] = base + 0;   // ||rp_t||^2
+        ops[2 * t + 1] = OP_SUM; slots[2 * t + 1] = base + 5;   // ||y_t||^2
+    }
+    return finalize(P->part + (int64_t)qoff * P->nblk_rows, P->nblk_rows, 2 * ntrial, ops, slots, P->S, stream);
+}
+
+int lkd_ipm_commit(const lkd_ipm* P, double* x, double* s, double* y, const double* dx,
+                   const double* ds, const double* dy, double ap, double ad, void* stream) {
+    LK_LAUNCH(k_commit, grid_for(P->m > P->n ? P->m : P->n), NT, 0, stream, P->m, P->n, x, s, y,
+              dx, ds, dy, ap, ad);
+    return lk_last_error();
+}
+
+}  // extern "C"

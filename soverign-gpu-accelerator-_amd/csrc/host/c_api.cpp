@@ -1,0 +1,444 @@
+// C ABI over the C++ solver classes (include/lakshya.h).  Exceptions never cross the boundary:
+// they become error codes with the message kept per thread for lk_last_error().
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "core/problem.hpp"
+#include "gpu/kkt.hpp"
+#include "gpu/vram_arena.hpp"
+#include "ipm/mehrotra.hpp"
+#include "ipm/symbolic.hpp"
+#include "lakshya.h"
+#include "lakshya_device.h"
+#include "verifier/certificate.hpp"
+
+using namespace sankhya;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return LK_OK;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return LK_ERR_INVALID;
+    } catch (const std::bad_alloc& e) {
+        g_err = e.what();
+        return LK_ERR_NOMEM;
+    } catch (const std::runtime_error& e) {
+        g_err = e.what();
+        return LK_ERR_RUNTIME;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return LK_ERR_INTERNAL;
+    } catch (...) {
+        g_err = "unknown exception";
+        return LK_ERR_INTERNAL;
+    }
+}
+
+core::CSRMatrix make_csr(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                         const double* vals) {
+    if (m < 0 || n < 0 || !rowptr) throw std::invalid_argument("bad CSR arguments");
+    core::CSRMatrix A;
+    A.rows = m;
+    A.cols = n;
+    A.row_ptrs.assign(rowptr, rowptr + m + 1);
+    const int32_t nz = rowptr[m];
+    A.col_indices.assign(colidx, colidx + nz);
+    if (vals) A.values.assign(vals, vals + nz);
+    else A.values.assign(static_cast<size_t>(nz), 1.0);
+    for (int32_t i = 0; i < m; ++i) {
+        if (rowptr[i + 1] < rowptr[i]) throw std::invalid_argument("row pointers not monotone");
+        for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+            if (colidx[p] < 0 || colidx[p] >= n) throw std::invalid_argument("column index out of range");
+            if (p > rowptr[i] && colidx[p] <= colidx[p - 1])
+                throw std::invalid_argument("column indices must increase within a row");
+        }
+    }
+    return A;
+}
+
+void make_model(core::Model& model, int32_t m, int32_t n, const int32_t* colptr,
+                const int32_t* rowidx, const double* vals, const double* c, const double* b,
+                const double* lb, const double* ub) {
+    for (int32_t j = 0; j < n; ++j)
+        for (int32_t p = colptr[j]; p < colptr[j + 1]; ++p)
+            if (p > colptr[j] && rowidx[p] <= rowidx[p - 1])
+                throw std::invalid_argument("row indices must increase within a column");
+    model.set_from_csc(m, n, colptr, rowidx, vals, c, b, lb, ub);
+}
+
+ipm::IpmOptions to_options(const lk_ipm_options* o) {
+    ipm::IpmOptions r;
+    if (!o) return r;
+    r.stop_rule = o->stop_rule == 1 ? ipm::IpmOptions::StopRule::RelativeGap
+                                    : ipm::IpmOptions::StopRule::Reference;
+    r.max_iter = o->max_iter;
+    r.tol = o->tol;
+    r.gap_tol = o->gap_tol;
+    r.precision = o->precision == 1 ? ipm::IpmOptions::Precision::Mixed
+                                    : ipm::IpmOptions::Precision::FP64;
+    r.mixed_max_refine = o->mixed_max_refine;
+    r.mixed_fp64_spread = o->mixed_fp64_spread;
+    r.ordering = o->ordering == 1 ? ipm::IpmOptions::Ordering::ReferenceMinimumDegree
+                                  : ipm::IpmOptions::Ordering::Amd;
+    r.use_graphs = o->use_graphs != 0;
+    r.verbose = o->verbose != 0;
+    r.warm_floor = o->warm_floor;
+    return r;
+}
+
+void fill_result(const ipm::MehrotraResult& r, lk_ipm_result* out, double* x, double* y,
+                 double* s) {
+    if (out) {
+        std::memset(out, 0, sizeof(*out));
+        out->status = static_cast<int32_t>(r.status);
+        out->iterations = r.iterations;
+        out->objective = r.objective_value;
+        out->primal_residual = r.primal_residual;
+        out->dual_residual = r.dual_residual;
+        out->duality_gap = r.duality_gap;
+        out->dual_objective = r.dual_objective;
+        out->rel_gap = r.rel_gap;
+        out->rel_primal_residual = r.rel_primal_residual;
+        out->rel_dual_residual = r.rel_dual_residual;
+        out->factorizations = r.factorizations;
+        out->fp32_factorizations = r.fp32_factorizations;
+        out->fp64_factorizations = r.fp64_factorizations;
+        out->solves = r.solves;
+        out->refinement_steps = r.refinement_steps;
+        out->setup_ms = r.setup_ms;
+        out->solve_ms = r.solve_ms;
+        out->factor_ms = r.factor_ms;
+        out->nnz_L = r.nnz_L;
+        out->flops_per_factorization = r.flops_per_factorization;
+    }
+    if (x) std::copy(r.x.begin(), r.x.end(), x);
+    if (y) std::copy(r.pi.begin(), r.pi.end(), y);
+    if (s) std::copy(r.s.begin(), r.s.end(), s);
+}
+
+void copy_stats(const gpu::KKTStats& s, lk_kkt_stats* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->nsuper = s.nsuper;
+    o->nnz_L = s.nnz_L;
+    o->nnz_L_exact = s.nnz_L_exact;
+    o->npairs = s.npairs;
+    o->nsegments = s.nsegments;
+    o->panel_elems = s.panel_elems;
+    o->levels = s.levels;
+    o->launches_factor = s.launches_factor;
+    o->launches_solve = s.launches_solve;
+    o->flops_exact = s.flops_exact;
+    o->flops_panel = s.flops_panel;
+    o->symbolic_ms = s.symbolic_ms;
+    o->factorizations = s.factorizations;
+    o->solves = s.solves;
+    o->refinement_steps = s.refinement_steps;
+}
+
+}  // namespace
+
+struct lk_arena { gpu::VRAMArena arena; explicit lk_arena(size_t b) : arena(b) {} };
+struct lk_kkt {
+    std::unique_ptr<gpu::GPUKKTCholeskySolver> solver;
+    core::CSRMatrix A;
+    ipm::SymbolicFactorization sym;
+    bool reference_layout = false;
+    double* d_rhs = nullptr;
+    lk_arena* arena = nullptr;
+};
+struct lk_ipm {
+    core::Model model;
+    std::unique_ptr<ipm::MehrotraSolver> solver;
+};
+
+extern "C" {
+
+const char* lk_last_error(void) { return g_err.c_str(); }
+int lk_version(void) { return 100; }
+int lk_device_count(int* count) {
+    const int e = lkd_device_count(count);
+    if (e) { g_err = lkd_error_string(e); *count = 0; return LK_ERR_RUNTIME; }
+    return LK_OK;
+}
+
+int lk_arena_create(size_t capacity_bytes, lk_arena** out) {
+    return guarded([&] { *out = new lk_arena(capacity_bytes); });
+}
+int lk_arena_alloc(lk_arena* a, size_t bytes, void** ptr) {
+    return guarded([&] { *ptr = a->arena.allocate(bytes); });
+}
+int lk_arena_free(lk_arena* a, void* ptr) { return guarded([&] { a->arena.free(ptr); }); }
+int lk_arena_occupancy(lk_arena* a, size_t* pct) {
+    return guarded([&] { *pct = a->arena.occupancy_percentage(); });
+}
+int lk_arena_destroy(lk_arena* a) { return guarded([&] { delete a; }); }
+
+int lk_order_reference(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                       int32_t* P) {
+    return guarded([&] {
+        auto A = make_csr(m, n, rowptr, colidx, nullptr);
+        auto p = ipm::compute_amd_ordering(A);
+        std::copy(p.begin(), p.end(), P);
+    });
+}
+int lk_order_amd(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                 int32_t* P) {
+    return guarded([&] {
+        auto A = make_csr(m, n, rowptr, colidx, nullptr);
+        auto p = ipm::compute_fill_reducing_ordering(A);
+        std::copy(p.begin(), p.end(), P);
+    });
+}
+int lk_symbolic(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                const int32_t* P, int32_t* parent, int32_t* P_inv, int32_t* Lptr, int32_t* Lidx,
+                int64_t cap, int64_t* nnz) {
+    return guarded([&] {
+        auto A = make_csr(m, n, rowptr, colidx, nullptr);
+        ipm::SymbolicFactorization sym;
+        ipm::compute_symbolic_factorization(A, std::vector<Index>(P, P + m), sym);
+        *nnz = static_cast<int64_t>(sym.L_pattern.col_indices.size());
+        if (parent) std::copy(sym.parent.begin(), sym.parent.end(), parent);
+        if (P_inv) std::copy(sym.P_inv.begin(), sym.P_inv.end(), P_inv);
+        if (Lptr) std::copy(sym.L_pattern.row_ptrs.begin(), sym.L_pattern.row_ptrs.end(), Lptr);
+        if (Lidx && cap >= *nnz)
+            std::copy(sym.L_pattern.col_indices.begin(), sym.L_pattern.col_indices.end(), Lidx);
+    });
+}
+int lk_supernodal(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                  const int32_t* P, int32_t relax_small, double relax_frac, lk_kkt_stats* out,
+                  int32_t* perm_out, int32_t* sn_first_out, int32_t* nsuper_out) {
+    return guarded([&] {
+        auto A = make_csr(m, n, rowptr, colidx, nullptr);
+        std::vector<Index> p = P ? std::vector<Index>(P, P + m) : ipm::compute_fill_reducing_ordering(A);
+        ipm::SupernodeOptions so;
+        if (relax_small >= 0) so.relax_small = relax_small;
+        if (relax_frac >= 0) so.relax_frac = relax_frac;
+        ipm::SupernodalStructure S;
+        ipm::compute_supernodal_structure(A, p, so, S);
+        if (out) {
+            std::memset(out, 0, sizeof(*out));
+            out->nsuper = S.nsuper;
+            out->nnz_L = S.nnz_L;
+            out->nnz_L_exact = S.nnz_L_exact;
+            out->panel_elems = S.sn_panel[S.nsuper];
+            out->levels = S.max_level + 1;
+            out->flops_exact = S.flops_exact;
+            out->flops_panel = S.flops_panel;
+        }
+        if (perm_out) std::copy(S.perm.begin(), S.perm.end(), perm_out);
+        if (sn_first_out) std::copy(S.sn_first.begin(), S.sn_first.end(), sn_first_out);
+        if (nsuper_out) *nsuper_out = S.nsuper;
+    });
+}
+
+int lk_kkt_create(lk_arena* arena, int32_t m, int32_t n, const int32_t* rowptr,
+                  const int32_t* colidx, const double* vals, const int32_t* P,
+                  int32_t reference_symbolic, int32_t use_graphs, lk_kkt** out) {
+    return guarded([&] {
+        std::unique_ptr<lk_kkt> k(new lk_kkt);
+        k->A = make_csr(m, n, rowptr, colidx, vals);
+        std::vector<Index> p;
+        if (P) p.assign(P, P + m);
+        else if (reference_symbolic) p = ipm::compute_amd_ordering(k->A);
+        else p = ipm::compute_fill_reducing_ordering(k->A);
+        ipm::compute_symbolic_factorization(k->A, p, k->sym);
+        k->reference_layout = true;
+        gpu::KKTOptions ko;
+        ko.use_graphs = use_graphs != 0;
+        k->solver.reset(new gpu::GPUKKTCholeskySolver(arena->arena, k->A, k->sym, ko));
+        k->arena = arena;
+        k->d_rhs = static_cast<double*>(arena->arena.allocate(sizeof(double) * (m > 0 ? m : 1)));
+        *out = k.release();
+    });
+}
+
+int lk_kkt_factor(lk_kkt* k, const double* theta, int32_t on_device, int32_t mode,
+                  int32_t* not_spd) {
+    return guarded([&] {
+        const Index n = k->solver->cols();
+        *not_spd = 0;
+        if (mode == 0) {
+            try {
+                if (on_device) k->solver->gpu_cholesky_factorize_device(theta);
+                else k->solver->gpu_cholesky_factorize(k->A, std::vector<Float>(theta, theta + n));
+            } catch (const std::runtime_error& e) {
+                if (std::string(e.what()).find("not positive definite") == std::string::npos) throw;
+                *not_spd = 1;
+            }
+            return;
+        }
+        const double* d = theta;
+        std::vector<double> tmp;
+        double* buf = nullptr;
+        if (!on_device) {
+            buf = static_cast<double*>(k->arena->arena.allocate(sizeof(double) * (n > 0 ? n : 1)));
+            if (lkd_memcpy_htod(buf, theta, sizeof(double) * n, k->solver->stream()))
+                throw std::runtime_error("theta upload failed");
+            d = buf;
+        }
+        const int r = k->solver->factor_equilibrated(
+            d, mode == 2 ? gpu::FactorPrecision::FP32 : gpu::FactorPrecision::FP64);
+        lkd_stream_sync(k->solver->stream());
+        if (buf) k->arena->arena.free(buf);
+        *not_spd = r < 0 ? 1 : 0;
+    });
+}
+
+int lk_kkt_solve(lk_kkt* k, double* rhs, int32_t on_device, double tol, int32_t max_refine,
+                 lk_kkt_solve_info* info) {
+    return guarded([&] {
+        const Index m = k->solver->rows();
+        double* d = rhs;
+        if (!on_device) {
+            d = k->d_rhs;
+            if (lkd_memcpy_htod(d, rhs, sizeof(double) * m, k->solver->stream()))
+                throw std::runtime_error("rhs upload failed");
+        }
+        bool ok;
+        gpu::KKTSolveInfo si;
+        if (max_refine < 0) {
+            ok = k->solver->gpu_cholesky_solve_device(d, tol);
+        } else {
+            ok = k->solver->solve_cached(d, tol, &si, max_refine);
+        }
+        if (!on_device) {
+            if (lkd_memcpy_dtoh(rhs, d, sizeof(double) * m, k->solver->stream()))
+                throw std::runtime_error("rhs download failed");
+        } else {
+            lkd_stream_sync(k->solver->stream());
+        }
+        if (info) {
+            std::memset(info, 0, sizeof(*info));
+            info->success = ok ? 1 : 0;
+            info->retries = si.retries;
+            info->refinement_steps = si.refinement_steps;
+            info->delta = si.delta;
+            info->residual_inf = si.residual_inf;
+        }
+    });
+}
+
+int lk_kkt_get_L_csr(lk_kkt* k, double* vals, int64_t nnz) {
+    return guarded([&] {
+        const int64_t expect = static_cast<int64_t>(k->sym.L_pattern.col_indices.size()) + k->solver->rows();
+        if (nnz != expect) throw std::invalid_argument("lk_kkt_get_L_csr: nnz mismatch");
+        const double* d = k->solver->get_device_L_values();
+        if (!d) throw std::runtime_error("lk_kkt_get_L_csr: no factor available");
+        if (lkd_memcpy_dtoh(vals, d, sizeof(double) * nnz, k->solver->stream()))
+            throw std::runtime_error("L download failed");
+    });
+}
+int lk_kkt_permutation(lk_kkt* k, int32_t* perm) {
+    return guarded([&] {
+        const auto& p = k->solver->permutation();
+        std::copy(p.begin(), p.end(), perm);
+    });
+}
+int lk_kkt_get_stats(lk_kkt* k, lk_kkt_stats* out) {
+    return guarded([&] { copy_stats(k->solver->stats(), out); });
+}
+int lk_kkt_stream(lk_kkt* k, void** stream) { return guarded([&] { *stream = k->solver->stream(); }); }
+int lk_kkt_destroy(lk_kkt* k) {
+    return guarded([&] {
+        if (!k) return;
+        lk_arena* a = k->arena;
+        double* d = k->d_rhs;
+        k->solver.reset();
+        if (a && d) a->arena.free(d);
+        delete k;
+    });
+}
+
+int lk_ipm_default_options(lk_ipm_options* o) {
+    return guarded([&] {
+        ipm::IpmOptions d;
+        std::memset(o, 0, sizeof(*o));
+        o->stop_rule = 0;
+        o->max_iter = d.max_iter;
+        o->tol = d.tol;
+        o->gap_tol = d.gap_tol;
+        o->precision = 0;
+        o->mixed_max_refine = d.mixed_max_refine;
+        o->mixed_fp64_spread = d.mixed_fp64_spread;
+        o->ordering = 0;
+        o->use_graphs = 0;
+        o->verbose = 0;
+        o->warm_floor = d.warm_floor;
+    });
+}
+
+int lk_ipm_solve(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
+                 const double* vals, const double* c, const double* b, const double* lb,
+                 const double* ub, const lk_ipm_options* opt, const double* warm_x,
+                 const double* warm_y, const double* warm_s, lk_ipm_result* res, double* x,
+                 double* y, double* s) {
+    return guarded([&] {
+        core::Model model;
+        make_model(model, m, n, colptr, rowidx, vals, c, b, lb, ub);
+        ipm::IpmOptions o = to_options(opt);
+        o.warm_x = warm_x;
+        o.warm_y = warm_y;
+        o.warm_s = warm_s;
+        ipm::MehrotraSolver solver(model, o);
+        fill_result(solver.solve(), res, x, y, s);
+    });
+}
+
+int lk_ipm_create(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
+                  const double* vals, const double* c, const double* b,
+                  const lk_ipm_options* opt, lk_ipm** out) {
+    return guarded([&] {
+        std::unique_ptr<lk_ipm> h(new lk_ipm);
+        make_model(h->model, m, n, colptr, rowidx, vals, c, b, nullptr, nullptr);
+        h->solver.reset(new ipm::MehrotraSolver(h->model, to_options(opt)));
+        *out = h.release();
+    });
+}
+
+int lk_ipm_run(lk_ipm* h, const double* c, const double* b, const double* warm_x,
+               const double* warm_y, const double* warm_s, lk_ipm_result* res, double* x,
+               double* y, double* s) {
+    return guarded([&] {
+        const Index m = h->model.A.rows, n = h->model.A.cols;
+        if (c && b)
+            h->solver->update_objective_and_rhs(std::vector<Float>(c, c + n),
+                                                std::vector<Float>(b, b + m));
+        if (warm_x && warm_y && warm_s)
+            h->solver->set_warm_start(std::vector<Float>(warm_x, warm_x + n),
+                                      std::vector<Float>(warm_y, warm_y + m),
+                                      std::vector<Float>(warm_s, warm_s + n));
+        else
+            h->solver->clear_warm_start();
+        fill_result(h->solver->solve(), res, x, y, s);
+    });
+}
+
+int lk_ipm_destroy(lk_ipm* h) { return guarded([&] { delete h; }); }
+
+int lk_verify_optimal(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
+                      const double* vals, const double* c, const double* b, const double* lb,
+                      const double* ub, int32_t maximize, const double* x, const double* pi,
+                      double tol, double* primal_residual, int32_t* pass) {
+    return guarded([&] {
+        core::Model model;
+        model.set_from_csc(m, n, colptr, rowidx, vals, c, b, lb, ub);
+        model.sense = maximize ? OptimizationSense::Maximize : OptimizationSense::Minimize;
+        double res = 0;
+        *pass = verifier::verify_optimal(model, std::vector<Float>(x, x + n),
+                                         std::vector<Float>(pi, pi + m), &res, tol)
+                    ? 1 : 0;
+        if (primal_residual) *primal_residual = res;
+    });
+}
+
+}  // extern "C"

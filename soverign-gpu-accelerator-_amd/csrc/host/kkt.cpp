@@ -1,0 +1,852 @@
+// GPUKKTCholeskySolver: plan construction on the host, numeric work through the device C ABI.
+//
+// Follows src/cuda/kkt.cu (reference) for semantics:
+//   factorize_device  kkt.cu:556-603  (M0 + 1e-9 I, unequilibrated, throws if not SPD)
+//   solve_device      kkt.cu:620-870  (D = 1/sqrt(max|M0_ii|,1e-12), delta 1e-9 x10 up to 5
+//                                      tries, refinement against M0 with ||.||_inf, rollback on
+//                                      stagnation, no larger delta after a refinement failure)
+// and replaces its mechanisms:
+//   per-row merge assembly       -> pair map sorted by (destination, column), fixed fma order
+//   level-by-level row Cholesky  -> supernodal tile schedule (POTRF/TRSM/SYRK/UPDATE)
+//   cuSPARSE SpSV per solve      -> supernodal triangular-solve schedule
+//   refactorization per solve    -> one cached factor per Theta
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include "gpu/kkt.hpp"
+#include "lakshya_device.h"
+
+namespace sankhya {
+namespace gpu {
+
+namespace {
+
+void check(int e, const char* what) {
+    if (e != 0)
+        throw std::runtime_error(std::string("HIP call failed (") + what + "): " +
+                                 lkd_error_string(e));
+}
+
+constexpr int TILE = 64;
+constexpr int TRSM_ROWS = 256;
+constexpr int FWD_ROWS = 256;
+
+struct Launch {
+    int kind;        // 0 potrf 1 trsm 2 syrk 3 update | 10 fwd-first 11 fwd-step 12 bwd-below 13 bwd-step
+    int64_t begin;
+    int32_t count;
+};
+
+template <typename T>
+T* dev_upload(VRAMArena& arena, const std::vector<T>& v, std::vector<void*>& owned) {
+    if (v.empty()) return nullptr;
+    T* p = static_cast<T*>(arena.allocate_uninitialized(v.size() * sizeof(T)));
+    owned.push_back(p);
+    check(lkd_memcpy_htod(p, v.data(), v.size() * sizeof(T), nullptr), "upload");
+    return p;
+}
+
+}  // namespace
+
+struct GPUKKTCholeskySolver::Impl {
+    VRAMArena& arena;
+    Index m = 0, n = 0;
+    KKTOptions opt;
+    KKTStats st;
+    void* stream = nullptr;
+    bool own_stream = false;
+    std::vector<void*> owned;
+
+    ipm::SupernodalStructure S;
+    std::vector<Index> perm, pinv;   // perm[i] = original row at permuted position i
+
+    // device structure
+    int32_t *d_perm = nullptr, *d_colptr = nullptr, *d_prow = nullptr, *d_prp = nullptr,
+            *d_pci = nullptr;
+    double *d_cval = nullptr, *d_pval = nullptr;
+    int64_t *d_segptr = nullptr, *d_segdst = nullptr, *d_dseg = nullptr;
+    int32_t *d_segi = nullptr, *d_segj = nullptr, *d_pa = nullptr, *d_pb = nullptr;
+    uint8_t* d_empty = nullptr;
+    int32_t *d_sn_first = nullptr, *d_sn_rows = nullptr, *d_col_to_sn = nullptr;
+    int64_t *d_sn_rowptr = nullptr, *d_sn_panel = nullptr;
+    lkd_task *d_ftasks = nullptr, *d_stasks = nullptr;
+    int64_t nseg = 0, npairs = 0;
+    std::vector<Launch> flaunch, slaunch;
+
+    // device numeric buffers
+    void* d_panel = nullptr;       // sized for FP64
+    void* d_diag = nullptr;
+    int64_t diag_elems = 0;
+    int32_t* d_err = nullptr;
+    double *d_w = nullptr, *d_D = nullptr, *d_m0d = nullptr, *d_theta = nullptr;
+    double *d_r = nullptr, *d_req = nullptr, *d_z = nullptr, *d_dy = nullptr, *d_bak = nullptr,
+           *d_rhsp = nullptr,
+           *d_u = nullptr, *d_rinf = nullptr, *d_csr = nullptr;
+    double* d_pcg = nullptr;       // PCG work vectors (4m)
+    int64_t* d_csr_index = nullptr;
+    int64_t nnz_csr = 0;
+
+    // cache state
+    bool have_eq = false;          // equilibrated factor valid for the current M0
+    bool have_theta = false;
+    FactorPrecision cached_prec = FactorPrecision::FP64;
+    double cached_delta = 0;
+    int factor_kind = 0;           // 0 none, 1 unequilibrated (reference factorize), 2 equilibrated
+    void* fgraph[2] = {nullptr, nullptr};   // factor graph per precision
+    void* sgraph[2] = {nullptr, nullptr};   // solve graph per precision (on d_z)
+
+    Impl(VRAMArena& a, const core::CSRMatrix& A, const ipm::SymbolicFactorization* sym,
+         ipm::SupernodalStructure* pre, const KKTOptions& o)
+        : arena(a), m(A.rows), n(A.cols), opt(o) {
+        auto t0 = std::chrono::steady_clock::now();
+        stream = opt.stream;
+        if (!stream) {
+            check(lkd_stream_create(&stream), "stream create");
+            own_stream = true;
+        }
+        if (pre) {
+            S = std::move(*pre);
+        } else {
+            if (static_cast<Index>(sym->P.size()) != m)
+                throw std::invalid_argument("GPUKKTCholeskySolver: permutation size mismatch");
+            ipm::compute_supernodal_structure(A, sym->P, opt.supernodes, S);
+        }
+        if (S.m != m) throw std::invalid_argument("GPUKKTCholeskySolver: structure size mismatch");
+        perm = S.perm;
+        pinv.assign(static_cast<size_t>(m), 0);
+        for (Index i = 0; i < m; ++i) pinv[perm[i]] = i;
+        build_matrices(A);
+        build_pairs();
+        build_factor_plan();
+        build_solve_plan();
+        if (sym) build_csr_index(*sym);
+        allocate_numeric();
+        st.m = m;
+        st.n = n;
+        st.nnz_A = A.row_ptrs.empty() ? 0 : A.row_ptrs[m];
+        st.nsuper = S.nsuper;
+        st.nnz_L = S.nnz_L;
+        st.nnz_L_exact = S.nnz_L_exact;
+        st.flops_exact = S.flops_exact;
+        st.flops_panel = S.flops_panel;
+        st.levels = S.max_level + 1;
+        st.panel_elems = S.sn_panel[S.nsuper];
+        st.npairs = npairs;
+        st.nsegments = nseg;
+        st.launches_factor = static_cast<int32_t>(flaunch.size());
+        st.launches_solve = static_cast<int32_t>(slaunch.size());
+        st.symbolic_ms = std::chrono::duration<double, std::milli>(
+                             std::chrono::steady_clock::now() - t0).count();
+    }
+
+    ~Impl() {
+        for (int p = 0; p < 2; ++p) {
+            if (fgraph[p]) lkd_graph_destroy(fgraph[p]);
+            if (sgraph[p]) lkd_graph_destroy(sgraph[p]);
+        }
+        lkd_stream_sync(stream);
+        for (auto it = owned.rbegin(); it != owned.rend(); ++it) {
+            try { arena.free(*it); } catch (...) {}
+        }
+        if (own_stream) lkd_stream_destroy(stream);
+    }
+
+    // -------------------------------------------------------------------- plan construction
+    void build_matrices(const core::CSRMatrix& A) {
+        const int64_t nz = A.row_ptrs[m];
+        // CSC of A with permuted row indices (columns keep the original order)
+        std::vector<Index> colptr(static_cast<size_t>(n) + 1, 0), prow(static_cast<size_t>(nz));
+        std::vector<double> cval(static_cast<size_t>(nz));
+        for (int64_t p = 0; p < nz; ++p) colptr[A.col_indices[p] + 1]++;
+        for (Index j = 0; j < n; ++j) colptr[j + 1] += colptr[j];
+        {
+            std::vector<Index> nxt(colptr.begin(), colptr.end() - 1);
+            for (Index i = 0; i < m; ++i)       // rows visited in original order
+                for (Index p = A.row_ptrs[i]; p < A.row_ptrs[i + 1]; ++p) {
+                    const Index q = nxt[A.col_indices[p]]++;
+                    prow[q] = pinv[i];
+                    cval[q] = A.values[p];
+                }
+        }
+        // permuted CSR
+        std::vector<Index> prp(static_cast<size_t>(m) + 1, 0), pci(static_cast<size_t>(nz));
+        std::vector<double> pval(static_cast<size_t>(nz));
+        for (Index k = 0; k < m; ++k) {
+            const Index i = perm[k];
+            prp[k + 1] = prp[k] + (A.row_ptrs[i + 1] - A.row_ptrs[i]);
+            std::copy(A.col_indices.begin() + A.row_ptrs[i], A.col_indices.begin() + A.row_ptrs[i + 1],
+                      pci.begin() + prp[k]);
+            std::copy(A.values.begin() + A.row_ptrs[i], A.values.begin() + A.row_ptrs[i + 1],
+                      pval.begin() + prp[k]);
+        }
+        h_colptr = colptr; h_prow = prow; h_cval_n = nz;
+        d_colptr = dev_upload(arena, colptr, owned);
+        d_prow = dev_upload(arena, prow, owned);
+        d_cval = dev_upload(arena, cval, owned);
+        d_prp = dev_upload(arena, prp, owned);
+        d_pci = dev_upload(arena, pci, owned);
+        d_pval = dev_upload(arena, pval, owned);
+        d_perm = dev_upload(arena, perm, owned);
+        d_empty = dev_upload(arena, S.empty_row, owned);
+    }
+    std::vector<Index> h_colptr, h_prow;
+    int64_t h_cval_n = 0;
+
+    int64_t panel_index(Index i, Index j) const {   // i >= j, permuted
+        const Index s = S.col_to_sn[j];
+        const int64_t r0 = S.sn_rowptr[s], r1 = S.sn_rowptr[s + 1];
+        const Index f = S.sn_first[s];
+        auto b = S.sn_rows.begin() + r0 + (j - f), e = S.sn_rows.begin() + r1;
+        auto it = std::lower_bound(b, e, i);
+        if (it == e || *it != i) throw std::logic_error("KKT plan: entry outside the L structure");
+        return S.sn_panel[s] + static_cast<int64_t>(j - f) * (r1 - r0) + (it - (S.sn_rows.begin() + r0));
+    }
+
+    void build_pairs() {
+        struct Pair { int64_t dst; int32_t a, b; };
+        std::vector<Pair> pairs;
+        int64_t total = 0;
+        for (Index k = 0; k < n; ++k) {
+            const int64_t c = h_colptr[k + 1] - h_colptr[k];
+            total += c * (c + 1) / 2;
+        }
+        pairs.reserve(static_cast<size_t>(total));
+        for (Index k = 0; k < n; ++k)
+            for (Index a = h_colptr[k]; a < h_colptr[k + 1]; ++a)
+                for (Index b = h_colptr[k]; b < h_colptr[k + 1]; ++b) {
+                    const Index i = h_prow[a], j = h_prow[b];
+                    if (i < j) continue;
+                    pairs.push_back({panel_index(i, j), a, b});
+                }
+        std::stable_sort(pairs.begin(), pairs.end(),
+                         [](const Pair& x, const Pair& y) { return x.dst < y.dst; });
+        npairs = static_cast<int64_t>(pairs.size());
+        std::vector<int64_t> segptr, segdst;
+        std::vector<int32_t> segi, segj, pa(pairs.size()), pb(pairs.size());
+        std::vector<int64_t> dseg(static_cast<size_t>(m), -1);
+        for (size_t q = 0; q < pairs.size(); ++q) {
+            if (q == 0 || pairs[q].dst != pairs[q - 1].dst) {
+                segptr.push_back(static_cast<int64_t>(q));
+                segdst.push_back(pairs[q].dst);
+                const Index i = h_prow[pairs[q].a], j = h_prow[pairs[q].b];
+                segi.push_back(i);
+                segj.push_back(j);
+                if (i == j) dseg[i] = static_cast<int64_t>(segdst.size()) - 1;
+            }
+            pa[q] = pairs[q].a;
+            pb[q] = pairs[q].b;
+        }
+        segptr.push_back(static_cast<int64_t>(pairs.size()));
+        nseg = static_cast<int64_t>(segdst.size());
+        pairs.clear();
+        pairs.shrink_to_fit();
+        d_segptr = dev_upload(arena, segptr, owned);
+        d_segdst = dev_upload(arena, segdst, owned);
+        d_segi = dev_upload(arena, segi, owned);
+        d_segj = dev_upload(arena, segj, owned);
+        d_pa = dev_upload(arena, pa, owned);
+        d_pb = dev_upload(arena, pb, owned);
+        d_dseg = dev_upload(arena, dseg, owned);
+    }
+
+    // diag-copy offsets per supernode block column
+    std::vector<int64_t> diag_off_start;   // [nsuper+1] start offset for the supernode's blocks
+
+    int64_t diag_off(Index s, int blk) const {
+        const Index nc = S.sn_first[s + 1] - S.sn_first[s];
+        int64_t off = diag_off_start[s];
+        for (int b = 0; b < blk; ++b) {
+            const int kb = std::min(TILE, nc - b * TILE);
+            off += static_cast<int64_t>(kb) * kb;
+        }
+        return off;
+    }
+
+    void build_factor_plan() {
+        const Index ns = S.nsuper;
+        diag_off_start.assign(static_cast<size_t>(ns) + 1, 0);
+        for (Index s = 0; s < ns; ++s) {
+            const Index nc = S.sn_first[s + 1] - S.sn_first[s];
+            int64_t sz = 0;
+            for (Index k0 = 0; k0 < nc; k0 += TILE) {
+                const int kb = std::min<Index>(TILE, nc - k0);
+                sz += static_cast<int64_t>(kb) * kb;
+            }
+            diag_off_start[s + 1] = diag_off_start[s] + sz;
+        }
+        diag_elems = diag_off_start[ns];
+        std::vector<std::vector<Index>> by_level(static_cast<size_t>(S.max_level) + 1);
+        for (Index s = 0; s < ns; ++s) by_level[S.sn_level[s]].push_back(s);
+        levels = by_level;
+
+        std::vector<lkd_task> tasks;
+        auto base = [&](Index s) {
+            lkd_task t{};
+            t.off = S.sn_panel[s];
+            t.ld = static_cast<int32_t>(S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
+            t.ncol = S.sn_first[s + 1] - S.sn_first[s];
+            t.sn = s;
+            t.first = S.sn_first[s];
+            t.aux2 = S.sn_rowptr[s];
+            return t;
+        };
+        auto emit = [&](int kind, int64_t begin) {
+            const int64_t cnt = static_cast<int64_t>(tasks.size()) - begin;
+            if (cnt > 0) flaunch.push_back({kind, begin, static_cast<int32_t>(cnt)});
+        };
+        for (const auto& lv : by_level) {
+            int maxblk = 0;
+            int64_t b0 = static_cast<int64_t>(tasks.size());
+            for (Index s : lv) {
+                lkd_task t = base(s);
+                t.k0 = 0;
+                t.kb = std::min(TILE, t.ncol);
+                t.aux = diag_off(s, 0);
+                tasks.push_back(t);
+                maxblk = std::max(maxblk, (t.ncol + TILE - 1) / TILE);
+            }
+            emit(0, b0);
+            for (int k = 0; k < maxblk; ++k) {
+                b0 = static_cast<int64_t>(tasks.size());
+                for (Index s : lv) {
+                    lkd_task t = base(s);
+                    const int nb = (t.ncol + TILE - 1) / TILE;
+                    if (k >= nb) continue;
+                    t.k0 = k * TILE;
+                    t.kb = std::min(TILE, t.ncol - t.k0);
+                    t.aux = diag_off(s, k);
+                    for (int r0 = t.k0 + t.kb; r0 < t.ld; r0 += TRSM_ROWS) {
+                        lkd_task u = t;
+                        u.r0 = r0;
+                        tasks.push_back(u);
+                    }
+                }
+                emit(1, b0);
+                b0 = static_cast<int64_t>(tasks.size());
+                for (Index s : lv) {
+                    lkd_task t = base(s);
+                    const int nb = (t.ncol + TILE - 1) / TILE;
+                    if (k + 1 >= nb) continue;
+                    t.k0 = k * TILE;
+                    t.kb = TILE;
+                    t.aux = diag_off(s, k + 1);
+                    for (int c0 = t.k0 + t.kb; c0 < t.ncol; c0 += TILE)
+                        for (int r0 = c0; r0 < t.ld; r0 += TILE) {
+                            lkd_task u = t;
+                            u.r0 = r0;
+                            u.c0 = c0;
+                            tasks.push_back(u);
+                        }
+                }
+                // the next-diagonal tiles first: they carry the longest critical path
+                std::stable_partition(tasks.begin() + b0, tasks.end(), [](const lkd_task& u) {
+                    return u.r0 == u.c0 && u.c0 == u.k0 + u.kb;
+                });
+                emit(2, b0);
+            }
+            b0 = static_cast<int64_t>(tasks.size());
+            for (Index s : lv) {
+                lkd_task t = base(s);
+                if (t.ld <= t.ncol) continue;
+                for (int c0 = t.ncol; c0 < t.ld; c0 += TILE)
+                    for (int r0 = c0; r0 < t.ld; r0 += TILE) {
+                        lkd_task u = t;
+                        u.r0 = r0;
+                        u.c0 = c0;
+                        tasks.push_back(u);
+                    }
+            }
+            emit(3, b0);
+        }
+        for (const auto& L : flaunch)
+            for (int64_t q = L.begin; q < L.begin + L.count; ++q) validate(tasks[q], L.kind);
+        d_ftasks = dev_upload(arena, tasks, owned);
+        n_ftasks = static_cast<int64_t>(tasks.size());
+    }
+    std::vector<std::vector<Index>> levels;
+    int64_t n_ftasks = 0, n_stasks = 0;
+
+    // Every task's accesses must stay inside its supernode's panel, row list and diagonal copy.
+    void validate(const lkd_task& t, int kind) const {
+        auto bad = [&](const char* why) {
+            throw std::logic_error(std::string("KKT plan: invalid task (") + why + ")");
+        };
+        const Index s = t.sn;
+        if (s < 0 || s >= S.nsuper) bad("supernode");
+        const int64_t ld = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
+        const Index nc = S.sn_first[s + 1] - S.sn_first[s];
+        if (t.ld != ld || t.ncol != nc || t.off != S.sn_panel[s] || t.first != S.sn_first[s] ||
+            t.aux2 != S.sn_rowptr[s])
+            bad("geometry");
+        if (S.sn_panel[s] + ld * nc > S.sn_panel[S.nsuper]) bad("panel extent");
+        if (t.first + nc > m) bad("columns");
+        if (t.k0 < 0 || t.kb < 0 || t.k0 + t.kb > nc || t.kb > TILE) bad("block column");
+        auto diag_ok = [&](int64_t off, int kb) { return off >= 0 && off + int64_t(kb) * kb <= diag_elems; };
+        switch (kind) {
+            case 0: if (t.kb < 1 || !diag_ok(t.aux, t.kb)) bad("potrf"); break;
+            case 1: if (t.r0 < t.k0 + t.kb || t.r0 >= ld || !diag_ok(t.aux, t.kb)) bad("trsm"); break;
+            case 2:
+                if (t.c0 < t.k0 + t.kb || t.c0 >= nc || t.r0 < t.c0 || t.r0 >= ld) bad("syrk");
+                if (t.r0 == t.c0 && t.c0 == t.k0 + t.kb && !diag_ok(t.aux, std::min<int>(TILE, nc - t.c0))) bad("syrk diag");
+                break;
+            case 3: if (t.c0 < nc || t.c0 >= ld || t.r0 < t.c0 || t.r0 >= ld) bad("update"); break;
+            case 10: if (t.k0 != 0 || !diag_ok(t.aux, t.kb)) bad("fwd first"); break;
+            case 11:
+                if (t.r0 < t.k0 + t.kb || t.r0 >= ld) bad("fwd step");
+                if (t.r0 == t.k0 + t.kb && t.r0 < nc && !diag_ok(t.aux, std::min<int>(TILE, nc - t.r0))) bad("fwd diag");
+                break;
+            case 12: if (t.c0 < 0 || t.c0 >= nc || !diag_ok(t.aux, std::min<int>(TILE, nc - t.c0))) bad("bwd below"); break;
+            default:
+                if (t.c0 < 0 || t.c0 + TILE > t.k0 || t.k0 >= nc || !diag_ok(t.aux, std::min<int>(TILE, nc - t.c0))) bad("bwd step");
+                break;
+        }
+    }
+
+    void build_solve_plan() {
+        std::vector<lkd_task> tasks;
+        auto base = [&](Index s) {
+            lkd_task t{};
+            t.off = S.sn_panel[s];
+            t.ld = static_cast<int32_t>(S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
+            t.ncol = S.sn_first[s + 1] - S.sn_first[s];
+            t.sn = s;
+            t.first = S.sn_first[s];
+            t.aux2 = S.sn_rowptr[s];
+            return t;
+        };
+        auto emit = [&](int kind, int64_t begin) {
+            const int64_t cnt = static_cast<int64_t>(tasks.size()) - begin;
+            if (cnt > 0) slaunch.push_back({kind, begin, static_cast<int32_t>(cnt)});
+        };
+        // forward, leaves first
+        for (const auto& lv : levels) {
+            int maxblk = 0;
+            int64_t b0 = static_cast<int64_t>(tasks.size());
+            for (Index s : lv) {
+                lkd_task t = base(s);
+                t.k0 = 0;
+                t.kb = std::min(TILE, t.ncol);
+                t.aux = diag_off(s, 0);
+                tasks.push_back(t);
+                maxblk = std::max(maxblk, (t.ncol + TILE - 1) / TILE);
+            }
+            emit(10, b0);
+            for (int k = 0; k < maxblk; ++k) {
+                b0 = static_cast<int64_t>(tasks.size());
+                for (Index s : lv) {
+                    lkd_task t = base(s);
+                    const int nb = (t.ncol + TILE - 1) / TILE;
+                    if (k >= nb) continue;
+                    t.k0 = k * TILE;
+                    t.kb = std::min(TILE, t.ncol - t.k0);
+                    for (int r0 = t.k0 + t.kb; r0 < t.ld; r0 += FWD_ROWS) {
+                        lkd_task u = t;
+                        u.r0 = r0;
+                        u.aux = (r0 < t.ncol) ? diag_off(s, k + 1) : 0;
+                        tasks.push_back(u);
+                    }
+                }
+                emit(11, b0);
+            }
+        }
+        // backward, root first
+        for (auto it = levels.rbegin(); it != levels.rend(); ++it) {
+            const auto& lv = *it;
+            int maxblk = 0;
+            int64_t b0 = static_cast<int64_t>(tasks.size());
+            for (Index s : lv) {
+                lkd_task t = base(s);
+                const int nb = (t.ncol + TILE - 1) / TILE;
+                maxblk = std::max(maxblk, nb);
+                for (int cb = (t.ld > t.ncol ? 0 : nb - 1); cb < nb; ++cb) {
+                    lkd_task u = t;
+                    u.c0 = cb * TILE;
+                    u.aux = diag_off(s, cb);
+                    tasks.push_back(u);
+                }
+            }
+            emit(12, b0);
+            for (int k = maxblk - 1; k >= 1; --k) {
+                b0 = static_cast<int64_t>(tasks.size());
+                for (Index s : lv) {
+                    lkd_task t = base(s);
+                    const int nb = (t.ncol + TILE - 1) / TILE;
+                    if (k >= nb) continue;
+                    t.k0 = k * TILE;
+                    t.kb = std::min(TILE, t.ncol - t.k0);
+                    for (int cb = 0; cb < k; ++cb) {
+                        lkd_task u = t;
+                        u.c0 = cb * TILE;
+                        u.aux = diag_off(s, cb);
+                        tasks.push_back(u);
+                    }
+                }
+                emit(13, b0);
+            }
+        }
+        for (const auto& L : slaunch)
+            for (int64_t q = L.begin; q < L.begin + L.count; ++q) validate(tasks[q], L.kind);
+        d_stasks = dev_upload(arena, tasks, owned);
+        n_stasks = static_cast<int64_t>(tasks.size());
+    }
+
+    // reference CSR layout of L (sym.L_pattern rows + diagonal last) -> panel indices
+    void build_csr_index(const ipm::SymbolicFactorization& sym) {
+        const auto& Lp = sym.L_pattern;
+        if (static_cast<Index>(Lp.row_ptrs.size()) != m + 1) return;
+        // sym ordering index -> build (postordered) index
+        std::vector<Index> q(static_cast<size_t>(m));
+        for (Index i = 0; i < m; ++i) q[i] = pinv[sym.P[i]];
+        std::vector<int64_t> idx;
+        idx.reserve(Lp.col_indices.size() + static_cast<size_t>(m));
+        for (Index i = 0; i < m; ++i) {
+            for (Index p = Lp.row_ptrs[i]; p < Lp.row_ptrs[i + 1]; ++p) {
+                const Index j = Lp.col_indices[p];
+                Index a = q[i], b = q[j];
+                if (a < b) std::swap(a, b);
+                int64_t pi = -1;
+                try { pi = panel_index(a, b); } catch (...) { pi = -1; }
+                idx.push_back(pi < 0 ? 0 : pi);
+            }
+            idx.push_back(panel_index(q[i], q[i]));
+        }
+        nnz_csr = static_cast<int64_t>(idx.size());
+        d_csr_index = dev_upload(arena, idx, owned);
+    }
+
+    void allocate_numeric() {
+        const int64_t pe = std::max<int64_t>(S.sn_panel[S.nsuper], 1);
+        d_panel = arena.allocate_uninitialized(static_cast<size_t>(pe) * sizeof(double));
+        owned.push_back(d_panel);
+        d_diag = arena.allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(diag_elems, 1)) * sizeof(double));
+        owned.push_back(d_diag);
+        auto dv = [&](int64_t len) {
+            double* p = static_cast<double*>(arena.allocate(static_cast<size_t>(std::max<int64_t>(len, 1)) * sizeof(double)));
+            owned.push_back(p);
+            return p;
+        };
+        d_err = static_cast<int32_t*>(arena.allocate(sizeof(int32_t) * 4));
+        owned.push_back(d_err);
+        d_w = dv(std::max<int64_t>(h_cval_n, 1));
+        d_D = dv(m);
+        d_m0d = dv(m);
+        d_theta = dv(n);
+        d_r = dv(m);
+        d_req = dv(m);
+        d_z = dv(m);
+        d_dy = dv(m);
+        d_bak = dv(m);
+        d_rhsp = dv(m);
+        d_u = dv(n);
+        d_rinf = dv(2 + 1024);
+        d_pcg = dv(4 * static_cast<int64_t>(m));
+        if (nnz_csr > 0) d_csr = dv(nnz_csr);
+        std::vector<int32_t> sn_first(S.sn_first.begin(), S.sn_first.end());
+        d_sn_first = dev_upload(arena, sn_first, owned);
+        d_sn_rowptr = dev_upload(arena, S.sn_rowptr, owned);
+        d_sn_rows = dev_upload(arena, S.sn_rows, owned);
+        d_sn_panel = dev_upload(arena, S.sn_panel, owned);
+        d_col_to_sn = dev_upload(arena, S.col_to_sn, owned);
+        check(lkd_stream_sync(nullptr), "plan upload");
+    }
+
+    lkd_snplan plan() const {
+        lkd_snplan p;
+        p.nsuper = S.nsuper;
+        p.sn_first = d_sn_first;
+        p.sn_rowptr = d_sn_rowptr;
+        p.sn_rows = d_sn_rows;
+        p.sn_panel = d_sn_panel;
+        p.col_to_sn = d_col_to_sn;
+        return p;
+    }
+
+    // -------------------------------------------------------------------- numeric phases
+    void run_factor_schedule(int prec) {
+        const lkd_snplan pl = plan();
+        for (const auto& L : flaunch) {
+            const lkd_task* t = d_ftasks + L.begin;
+            int e = 0;
+            switch (L.kind) {
+                case 0: e = lkd_chol_potrf(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
+                case 1: e = lkd_chol_trsm(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
+                case 2: e = lkd_chol_syrk(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
+                default: e = lkd_chol_update(prec, L.count, t, d_panel, pl, d_err, stream); break;
+            }
+            check(e, "factor kernel");
+        }
+    }
+
+    void run_solve_schedule(int prec, double* x) {
+        for (const auto& L : slaunch) {
+            const lkd_task* t = d_stasks + L.begin;
+            int e = 0;
+            switch (L.kind) {
+                case 10: e = lkd_trsv_fwd(prec, L.count, t, d_panel, d_diag, d_sn_rows, x, 1, stream); break;
+                case 11: e = lkd_trsv_fwd(prec, L.count, t, d_panel, d_diag, d_sn_rows, x, 0, stream); break;
+                case 12: e = lkd_trsv_bwd(prec, L.count, t, d_panel, d_diag, d_sn_rows, x, 1, stream); break;
+                default: e = lkd_trsv_bwd(prec, L.count, t, d_panel, d_diag, d_sn_rows, x, 0, stream); break;
+            }
+            check(e, "solve kernel");
+        }
+    }
+
+    void trsv(int prec, double* x) {
+        if (opt.use_graphs && x == d_z) {
+            void*& g = sgraph[prec];
+            if (!g) {
+                check(lkd_graph_begin(stream), "graph begin");
+                run_solve_schedule(prec, x);
+                check(lkd_graph_end(stream, &g), "graph end");
+            }
+            check(lkd_graph_launch(g, stream), "graph launch");
+            return;
+        }
+        run_solve_schedule(prec, x);
+    }
+
+    void factor_numeric(int prec) {
+        if (opt.use_graphs) {
+            void*& g = fgraph[prec];
+            if (!g) {
+                check(lkd_graph_begin(stream), "graph begin");
+                run_factor_schedule(prec);
+                check(lkd_graph_end(stream, &g), "graph end");
+            }
+            check(lkd_graph_launch(g, stream), "graph launch");
+            return;
+        }
+        run_factor_schedule(prec);
+    }
+
+    // assemble (D, delta) and factor; returns true when every pivot was positive
+    bool assemble_and_factor(const double* theta, bool equil, double delta, int prec,
+                             bool need_diag) {
+        const size_t esz = prec == LKD_F32 ? sizeof(float) : sizeof(double);
+        if (need_diag) {
+            check(lkd_assemble_prep(n, d_colptr, d_cval, theta, d_w, stream), "prep");
+            check(lkd_assemble_diag(m, d_dseg, d_segptr, d_pa, d_pb, d_w, d_cval, equil ? 1 : 0,
+                                    d_D, d_m0d, stream), "diag");
+        }
+        check(lkd_memset(d_panel, 0, static_cast<size_t>(S.sn_panel[S.nsuper]) * esz, stream), "panel zero");
+        check(lkd_memset(d_err, 0, sizeof(int32_t), stream), "err zero");
+        check(lkd_assemble_pairs(prec, nseg, d_segptr, d_segdst, d_segi, d_segj, d_pa, d_pb, d_w,
+                                 d_cval, d_D, d_empty, delta, d_panel, stream), "assemble");
+        factor_numeric(prec);
+        int32_t err = 0;
+        check(lkd_memcpy_dtoh(&err, d_err, sizeof(int32_t), stream), "err read");
+        st.factorizations++;
+        if (err & 2) throw std::logic_error("KKT factorization: update outside the L structure");
+        return err == 0;
+    }
+
+    void keep_theta(const double* theta) {
+        if (theta != d_theta)
+            check(lkd_memcpy_dtod(d_theta, theta, static_cast<size_t>(n) * sizeof(double), stream), "theta copy");
+        have_theta = true;
+    }
+
+    int factor_equilibrated(const double* theta, FactorPrecision p) {
+        keep_theta(theta);
+        const int prec = p == FactorPrecision::FP32 ? LKD_F32 : LKD_F64;
+        double delta = 1e-9;
+        have_eq = false;
+        factor_kind = 0;
+        for (int r = 0; r < 5; ++r) {
+            if (r > 0) delta *= 10.0;
+            if (assemble_and_factor(d_theta, true, delta, prec, r == 0)) {
+                have_eq = true;
+                factor_kind = 2;
+                cached_prec = p;
+                cached_delta = delta;
+                retries_used = r;
+                return r;
+            }
+        }
+        return -1;
+    }
+    int retries_used = 0;
+
+    double read_rinf() {
+        double v = 0;
+        check(lkd_memcpy_dtoh(&v, d_rinf, sizeof(double), stream), "residual read");
+        return v;
+    }
+
+    // r = rhs_perm - M0 dy (M0 dy = A_p (theta o (A_p^T dy))), req = D r, rinf = max |r|
+    void residual() {
+        check(lkd_kkt_residual(m, n, d_colptr, d_prow, d_cval, d_prp, d_pci, d_pval, d_theta,
+                               d_dy, d_rhsp, d_u, d_r, d_D, d_req, d_rinf, stream), "residual");
+    }
+
+    bool solve_cached(double* d_rhs, double tol, KKTSolveInfo* info, int max_refine) {
+        if (!have_eq) throw std::logic_error("KKT solve: no factorization");
+        const int prec = cached_prec == FactorPrecision::FP32 ? LKD_F32 : LKD_F64;
+        st.solves++;
+        check(lkd_kkt_gather(m, d_perm, d_rhs, d_D, d_rhsp, d_req, stream), "gather");
+        check(lkd_memcpy_dtod(d_z, d_req, static_cast<size_t>(m) * sizeof(double), stream), "copy");
+        trsv(prec, d_z);
+        check(lkd_kkt_update(m, d_D, d_z, d_dy, nullptr, 0, stream), "update");
+        residual();
+        double before = read_rinf();
+        bool ok = before <= tol;
+        int steps = 0;
+        double after = before;
+        if (!ok) {
+            for (int it = 0; it < max_refine; ++it) {
+                check(lkd_memcpy_dtod(d_z, d_req, static_cast<size_t>(m) * sizeof(double), stream), "copy");
+                trsv(prec, d_z);
+                check(lkd_kkt_update(m, d_D, d_z, d_dy, d_bak, 1, stream), "update");
+                residual();
+                after = read_rinf();
+                ++steps;
+                if (after <= tol) { ok = true; break; }
+                if (after >= before) {
+                    check(lkd_memcpy_dtod(d_dy, d_bak, static_cast<size_t>(m) * sizeof(double), stream), "rollback");
+                    after = before;
+                    break;
+                }
+                before = after;
+            }
+        }
+        st.refinement_steps += steps;
+        check(lkd_kkt_scatter(m, d_perm, d_dy, d_rhs, stream), "scatter");
+        if (info) {
+            info->retries = retries_used;
+            info->refinement_steps = steps;
+            info->delta = cached_delta;
+            info->residual_inf = after;
+        }
+        return ok;
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+GPUKKTCholeskySolver::GPUKKTCholeskySolver(VRAMArena& arena, const core::CSRMatrix& A,
+                                           const ipm::SymbolicFactorization& sym,
+                                           const core::CSRMatrix& M_pattern)
+    : impl_(nullptr) {
+    if (M_pattern.rows != A.rows || M_pattern.cols != A.rows)
+        throw std::invalid_argument("GPUKKTCholeskySolver: M_pattern dimension mismatch");
+    impl_.reset(new Impl(arena, A, &sym, nullptr, KKTOptions{}));
+}
+
+GPUKKTCholeskySolver::GPUKKTCholeskySolver(VRAMArena& arena, const core::CSRMatrix& A,
+                                           const ipm::SymbolicFactorization& sym,
+                                           const KKTOptions& opt)
+    : impl_(new Impl(arena, A, &sym, nullptr, opt)) {}
+
+GPUKKTCholeskySolver::GPUKKTCholeskySolver(VRAMArena& arena, const core::CSRMatrix& A,
+                                           ipm::SupernodalStructure&& structure,
+                                           const KKTOptions& opt)
+    : impl_(new Impl(arena, A, nullptr, &structure, opt)) {}
+
+std::size_t GPUKKTCholeskySolver::estimate_bytes(const core::CSRMatrix& A,
+                                                 const ipm::SupernodalStructure& S) {
+    const int64_t m = A.rows, n = A.cols, nz = A.row_ptrs.empty() ? 0 : A.row_ptrs[A.rows];
+    std::vector<int64_t> cnt(static_cast<size_t>(n), 0);
+    for (int64_t p = 0; p < nz; ++p) cnt[A.col_indices[p]]++;
+    int64_t pairs = 0;
+    for (int64_t c : cnt) pairs += c * (c + 1) / 2;
+    int64_t diag = 0, tasks = 0;
+    for (Index s = 0; s < S.nsuper; ++s) {
+        const int64_t nc = S.sn_first[s + 1] - S.sn_first[s];
+        const int64_t ld = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
+        diag += nc * 64;
+        const int64_t bc = (nc + 63) / 64, br = (ld + 63) / 64;
+        tasks += 4 + bc * (br + 4) + bc * br * (br + 1) / 2 + br * br;
+    }
+    const int64_t bytes = 8 * S.sn_panel[S.nsuper] + 8 * diag + 8 * pairs + 36 * pairs +
+                          2 * (16 * nz + 8 * (m + n)) + 8 * nz + 8 * 24 * (m + n) +
+                          48 * 2 * tasks + 8 * 4096 + static_cast<int64_t>(S.sn_rows.size()) * 4 +
+                          24 * static_cast<int64_t>(S.nsuper) + 8 * m + 256 * 96;
+    return static_cast<std::size_t>(bytes);
+}
+
+GPUKKTCholeskySolver::~GPUKKTCholeskySolver() = default;
+
+void GPUKKTCholeskySolver::gpu_cholesky_factorize(const core::CSRMatrix& A,
+                                                  const std::vector<Float>& Theta) {
+    (void)A;
+    if (static_cast<Index>(Theta.size()) != impl_->n)
+        throw std::invalid_argument("Theta dimension mismatch");
+    check(lkd_memcpy_htod(impl_->d_theta, Theta.data(), Theta.size() * sizeof(double),
+                          impl_->stream), "theta upload");
+    gpu_cholesky_factorize_device(impl_->d_theta);
+}
+
+void GPUKKTCholeskySolver::gpu_cholesky_factorize_device(const Float* d_Theta) {
+    auto& I = *impl_;
+    I.keep_theta(d_Theta);
+    I.have_eq = false;
+    const bool ok = I.assemble_and_factor(I.d_theta, false, 1e-9, LKD_F64, true);
+    I.factor_kind = 1;
+    if (I.d_csr) check(lkd_panel_to_csr(LKD_F64, I.nnz_csr, I.d_csr_index, I.d_panel, I.d_csr, I.stream), "L gather");
+    check(lkd_stream_sync(I.stream), "sync");
+    if (!ok) throw std::runtime_error("Numerical factorization failed: matrix is not positive definite.");
+}
+
+bool GPUKKTCholeskySolver::gpu_cholesky_solve(std::vector<Float>& rhs, Float tol) {
+    if (static_cast<Index>(rhs.size()) != impl_->m)
+        throw std::invalid_argument("rhs dimension mismatch");
+    auto& I = *impl_;
+    double* d = I.d_pcg;   // 4m scratch; first m used as the rhs buffer
+    check(lkd_memcpy_htod(d, rhs.data(), rhs.size() * sizeof(double), I.stream), "rhs upload");
+    const bool ok = gpu_cholesky_solve_device(d, tol);
+    check(lkd_memcpy_dtoh(rhs.data(), d, rhs.size() * sizeof(double), I.stream), "rhs download");
+    return ok;
+}
+
+bool GPUKKTCholeskySolver::gpu_cholesky_solve_device(Float* d_rhs, Float tol) {
+    auto& I = *impl_;
+    if (!I.have_theta) throw std::logic_error("gpu_cholesky_solve_device: factorize first");
+    if (!I.have_eq || I.cached_prec != FactorPrecision::FP64) {
+        if (I.factor_equilibrated(I.d_theta, FactorPrecision::FP64) < 0) {
+            // every delta failed: dy = 0 (reference returns the zeroed dy)
+            check(lkd_memset(I.d_dy, 0, static_cast<size_t>(I.m) * sizeof(double), I.stream), "zero");
+            check(lkd_kkt_scatter(I.m, I.d_perm, I.d_dy, d_rhs, I.stream), "scatter");
+            check(lkd_stream_sync(I.stream), "sync");
+            return false;
+        }
+    }
+    const bool ok = I.solve_cached(d_rhs, tol, nullptr, 10);
+    check(lkd_stream_sync(I.stream), "sync");
+    return ok;
+}
+
+Float* GPUKKTCholeskySolver::get_device_L_values() const { return impl_->d_csr; }
+
+int GPUKKTCholeskySolver::factor_equilibrated(const Float* d_Theta, FactorPrecision prec) {
+    return impl_->factor_equilibrated(d_Theta, prec);
+}
+
+bool GPUKKTCholeskySolver::solve_cached(Float* d_rhs, Float tol, KKTSolveInfo* info,
+                                        int max_refine) {
+    return impl_->solve_cached(d_rhs, tol, info, max_refine);
+}
+
+bool GPUKKTCholeskySolver::solve_pcg(Float* d_rhs, Float tol, int max_iter, KKTSolveInfo* info) {
+    // refinement with the (possibly FP32) factor is the preconditioned Richardson iteration;
+    // the mixed-precision loop uses it with a larger step budget.
+    return impl_->solve_cached(d_rhs, tol, info, max_iter);
+}
+
+void GPUKKTCholeskySolver::triangular_solve_permuted(Float* d_x) {
+    auto& I = *impl_;
+    const int prec = I.cached_prec == FactorPrecision::FP32 && I.factor_kind == 2 ? LKD_F32 : LKD_F64;
+    I.run_solve_schedule(prec, d_x);
+    check(lkd_stream_sync(I.stream), "sync");
+}
+
+const KKTStats& GPUKKTCholeskySolver::stats() const { return impl_->st; }
+const std::vector<Index>& GPUKKTCholeskySolver::permutation() const { return impl_->perm; }
+void* GPUKKTCholeskySolver::stream() const { return impl_->stream; }
+Index GPUKKTCholeskySolver::rows() const { return impl_->m; }
+Index GPUKKTCholeskySolver::cols() const { return impl_->n; }
+
+}  // namespace gpu
+}  // namespace sankhya

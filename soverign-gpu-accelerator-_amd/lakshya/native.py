@@ -1,0 +1,354 @@
+"""ctypes binding of liblakshya.so (include/lakshya.h).  The product path: every call goes to the
+HIP library; nothing here falls back to a CPU implementation -- a missing library raises."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "liblakshya.so")
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+ERRORS = {1: ValueError, 2: MemoryError, 3: RuntimeError, 5: RuntimeError}
+
+
+class KKTStats(C.Structure):
+    _fields_ = [("nsuper", C.c_int64), ("nnz_L", C.c_int64), ("nnz_L_exact", C.c_int64),
+                ("npairs", C.c_int64), ("nsegments", C.c_int64), ("panel_elems", C.c_int64),
+                ("levels", C.c_int32), ("launches_factor", C.c_int32),
+                ("launches_solve", C.c_int32), ("pad", C.c_int32),
+                ("flops_exact", C.c_double), ("flops_panel", C.c_double),
+                ("symbolic_ms", C.c_double), ("factorizations", C.c_int64),
+                ("solves", C.c_int64), ("refinement_steps", C.c_int64)]
+
+
+class KKTSolveInfo(C.Structure):
+    _fields_ = [("success", C.c_int32), ("retries", C.c_int32),
+                ("refinement_steps", C.c_int32), ("pad", C.c_int32),
+                ("delta", C.c_double), ("residual_inf", C.c_double)]
+
+
+class IpmOptions(C.Structure):
+    _fields_ = [("stop_rule", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
+                ("gap_tol", C.c_double), ("precision", C.c_int32),
+                ("mixed_max_refine", C.c_int32), ("mixed_fp64_spread", C.c_double),
+                ("ordering", C.c_int32), ("use_graphs", C.c_int32), ("verbose", C.c_int32),
+                ("pad", C.c_int32), ("warm_floor", C.c_double)]
+
+
+class IpmResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("objective", C.c_double),
+                ("primal_residual", C.c_double), ("dual_residual", C.c_double),
+                ("duality_gap", C.c_double), ("dual_objective", C.c_double),
+                ("rel_gap", C.c_double), ("rel_primal_residual", C.c_double),
+                ("rel_dual_residual", C.c_double), ("factorizations", C.c_int32),
+                ("fp32_factorizations", C.c_int32), ("fp64_factorizations", C.c_int32),
+                ("solves", C.c_int32), ("refinement_steps", C.c_int32), ("pad", C.c_int32),
+                ("setup_ms", C.c_double), ("solve_ms", C.c_double), ("factor_ms", C.c_double),
+                ("nnz_L", C.c_double), ("flops_per_factorization", C.c_double)]
+
+
+STATUS = {0: "Optimal", 1: "Infeasible", 2: "Unbounded", 3: "IterationLimit"}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"liblakshya.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    L.lk_last_error.restype = C.c_char_p
+    sig = {
+        "lk_device_count": [C.POINTER(C.c_int)],
+        "lk_arena_create": [C.c_size_t, C.POINTER(_vp)],
+        "lk_arena_alloc": [_vp, C.c_size_t, C.POINTER(_vp)],
+        "lk_arena_free": [_vp, _vp],
+        "lk_arena_occupancy": [_vp, C.POINTER(C.c_size_t)],
+        "lk_arena_destroy": [_vp],
+        "lk_order_reference": [C.c_int32, C.c_int32, _i32p, _i32p, _i32p],
+        "lk_order_amd": [C.c_int32, C.c_int32, _i32p, _i32p, _i32p],
+        "lk_symbolic": [C.c_int32, C.c_int32, _i32p, _i32p, _i32p, _vp, _vp, _vp, _vp,
+                        C.c_int64, C.POINTER(C.c_int64)],
+        "lk_supernodal": [C.c_int32, C.c_int32, _i32p, _i32p, _vp, C.c_int32, C.c_double,
+                          C.POINTER(KKTStats), _vp, _vp, C.POINTER(C.c_int32)],
+        "lk_kkt_create": [_vp, C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _vp, C.c_int32,
+                          C.c_int32, C.POINTER(_vp)],
+        "lk_kkt_factor": [_vp, _vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32)],
+        "lk_kkt_solve": [_vp, _vp, C.c_int32, C.c_double, C.c_int32,
+                         C.POINTER(KKTSolveInfo)],
+        "lk_kkt_get_L_csr": [_vp, _f64p, C.c_int64],
+        "lk_kkt_permutation": [_vp, _i32p],
+        "lk_kkt_get_stats": [_vp, C.POINTER(KKTStats)],
+        "lk_kkt_stream": [_vp, C.POINTER(_vp)],
+        "lk_kkt_destroy": [_vp],
+        "lk_ipm_default_options": [C.POINTER(IpmOptions)],
+        "lk_ipm_solve": [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _vp, _vp,
+                         C.POINTER(IpmOptions), _vp, _vp, _vp, C.POINTER(IpmResult), _vp, _vp,
+                         _vp],
+        "lk_ipm_create": [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p,
+                          C.POINTER(IpmOptions), C.POINTER(_vp)],
+        "lk_ipm_run": [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(IpmResult), _vp, _vp, _vp],
+        "lk_ipm_destroy": [_vp],
+        "lk_verify_optimal": [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _vp, _vp,
+                              C.c_int32, _f64p, _f64p, C.c_double, C.POINTER(C.c_double),
+                              C.POINTER(C.c_int32)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _ck(rc: int):
+    if rc != 0:
+        msg = lib().lk_last_error().decode(errors="replace")
+        raise ERRORS.get(rc, RuntimeError)(msg)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    lib().lk_device_count(C.byref(c))
+    return c.value
+
+
+# ------------------------------------------------------------------ arena
+class Arena:
+    def __init__(self, capacity: int):
+        self.h = _vp()
+        _ck(lib().lk_arena_create(capacity, C.byref(self.h)))
+
+    def allocate(self, nbytes: int) -> int:
+        p = _vp()
+        _ck(lib().lk_arena_alloc(self.h, nbytes, C.byref(p)))
+        return p.value or 0
+
+    def free(self, ptr):
+        _ck(lib().lk_arena_free(self.h, ptr))
+
+    def occupancy_percentage(self) -> int:
+        v = C.c_size_t(0)
+        _ck(lib().lk_arena_occupancy(self.h, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h:
+            _ck(lib().lk_arena_destroy(self.h))
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ symbolic
+def order_reference(rp, ci, n):
+    m = len(rp) - 1
+    P = np.zeros(m, np.int32)
+    _ck(lib().lk_order_reference(m, n, _i32(rp), _i32(ci), P))
+    return P
+
+
+def order_amd(rp, ci, n):
+    m = len(rp) - 1
+    P = np.zeros(m, np.int32)
+    _ck(lib().lk_order_amd(m, n, _i32(rp), _i32(ci), P))
+    return P
+
+
+def symbolic(rp, ci, n, P):
+    m = len(rp) - 1
+    nnz = C.c_int64(0)
+    parent = np.zeros(m, np.int32)
+    pinv = np.zeros(m, np.int32)
+    Lptr = np.zeros(m + 1, np.int32)
+    _ck(lib().lk_symbolic(m, n, _i32(rp), _i32(ci), _i32(P), None, None, None, None, 0,
+                          C.byref(nnz)))
+    Lidx = np.zeros(max(nnz.value, 1), np.int32)
+    _ck(lib().lk_symbolic(m, n, _i32(rp), _i32(ci), _i32(P), parent.ctypes.data,
+                          pinv.ctypes.data, Lptr.ctypes.data, Lidx.ctypes.data, nnz.value,
+                          C.byref(nnz)))
+    return {"P_inv": pinv, "parent": parent, "Lptr": Lptr, "Lidx": Lidx[:nnz.value]}
+
+
+def supernodal(rp, ci, n, P=None, relax_small=-1, relax_frac=-1.0):
+    m = len(rp) - 1
+    st = KKTStats()
+    perm = np.zeros(m, np.int32)
+    first = np.zeros(m + 1, np.int32)
+    ns = C.c_int32(0)
+    Pp = _i32(P) if P is not None else None
+    _ck(lib().lk_supernodal(m, n, _i32(rp), _i32(ci), _ptr(Pp), relax_small, relax_frac,
+                            C.byref(st), perm.ctypes.data, first.ctypes.data, C.byref(ns)))
+    out = {f: getattr(st, f) for f, _ in KKTStats._fields_ if f != "pad"}
+    out["perm"] = perm
+    out["sn_first"] = first[:ns.value + 1]
+    return out
+
+
+# ------------------------------------------------------------------ KKT
+class KKT:
+    """gpu::GPUKKTCholeskySolver through the C ABI (A in CSR)."""
+
+    def __init__(self, arena: Arena, rp, ci, val, n, P=None, reference_symbolic=True,
+                 use_graphs=False):
+        self.m = len(rp) - 1
+        self.n = n
+        self.arena = arena
+        self.h = _vp()
+        Pp = _i32(P) if P is not None else None
+        _ck(lib().lk_kkt_create(arena.h, self.m, n, _i32(rp), _i32(ci), _f64(val), _ptr(Pp),
+                                int(reference_symbolic), int(use_graphs), C.byref(self.h)))
+
+    def factorize(self, theta, mode=0, device_ptr=None):
+        """mode 0: reference factorize (M0 + 1e-9 I); 1: equilibrated FP64; 2: FP32.
+        Returns True when positive definite."""
+        ns = C.c_int32(0)
+        if device_ptr is not None:
+            _ck(lib().lk_kkt_factor(self.h, device_ptr, 1, mode, C.byref(ns)))
+        else:
+            t = _f64(theta)
+            _ck(lib().lk_kkt_factor(self.h, t.ctypes.data, 0, mode, C.byref(ns)))
+        return ns.value == 0
+
+    def solve(self, rhs, tol=1e-10, max_refine=-1, device_ptr=None):
+        info = KKTSolveInfo()
+        if device_ptr is not None:
+            _ck(lib().lk_kkt_solve(self.h, device_ptr, 1, tol, max_refine, C.byref(info)))
+            return None, info
+        r = _f64(rhs).copy()
+        _ck(lib().lk_kkt_solve(self.h, r.ctypes.data, 0, tol, max_refine, C.byref(info)))
+        return r, info
+
+    def L_csr(self, nnz):
+        v = np.zeros(nnz)
+        _ck(lib().lk_kkt_get_L_csr(self.h, v, nnz))
+        return v
+
+    def permutation(self):
+        p = np.zeros(self.m, np.int32)
+        _ck(lib().lk_kkt_permutation(self.h, p))
+        return p
+
+    def stats(self):
+        st = KKTStats()
+        _ck(lib().lk_kkt_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in KKTStats._fields_ if f != "pad"}
+
+    def stream(self):
+        s = _vp()
+        _ck(lib().lk_kkt_stream(self.h, C.byref(s)))
+        return s.value
+
+    def close(self):
+        if self.h:
+            _ck(lib().lk_kkt_destroy(self.h))
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ IPM
+def default_options(**kw) -> IpmOptions:
+    o = IpmOptions()
+    _ck(lib().lk_ipm_default_options(C.byref(o)))
+    aliases = {"relative": ("stop_rule", 1)}
+    for k, v in kw.items():
+        if k in aliases:
+            if v:
+                setattr(o, aliases[k][0], aliases[k][1])
+            continue
+        setattr(o, k, v)
+    return o
+
+
+def _result(res: IpmResult, x, y, s):
+    out = {f: getattr(res, f) for f, _ in IpmResult._fields_ if f != "pad"}
+    out["status_name"] = STATUS.get(res.status, str(res.status))
+    out.update(x=x, y=y, s=s)
+    return out
+
+
+def ipm_solve(lp, warm=None, **opts):
+    """Solve one gen.LP (CSC) with MehrotraSolver; opts map onto IpmOptions fields."""
+    o = default_options(**opts)
+    res = IpmResult()
+    x, y, s = np.zeros(lp.n), np.zeros(lp.m), np.zeros(lp.n)
+    w = [None, None, None] if warm is None else [_f64(a) for a in warm]
+    _ck(lib().lk_ipm_solve(lp.m, lp.n, _i32(lp.indptr), _i32(lp.indices), _f64(lp.data),
+                           _f64(lp.c), _f64(lp.b), None, None, C.byref(o), _ptr(w[0]),
+                           _ptr(w[1]), _ptr(w[2]), C.byref(res), x.ctypes.data, y.ctypes.data,
+                           s.ctypes.data))
+    return _result(res, x, y, s)
+
+
+class IPM:
+    """Reusable MehrotraSolver for a fixed A (setup once, many (b, c))."""
+
+    def __init__(self, lp, **opts):
+        self.m, self.n = lp.m, lp.n
+        o = default_options(**opts)
+        self.h = _vp()
+        _ck(lib().lk_ipm_create(lp.m, lp.n, _i32(lp.indptr), _i32(lp.indices), _f64(lp.data),
+                                _f64(lp.c), _f64(lp.b), C.byref(o), C.byref(self.h)))
+
+    def run(self, c=None, b=None, warm=None):
+        res = IpmResult()
+        x, y, s = np.zeros(self.n), np.zeros(self.m), np.zeros(self.n)
+        cc = _f64(c) if c is not None else None
+        bb = _f64(b) if b is not None else None
+        w = [None, None, None] if warm is None else [_f64(a) for a in warm]
+        _ck(lib().lk_ipm_run(self.h, _ptr(cc), _ptr(bb), _ptr(w[0]), _ptr(w[1]), _ptr(w[2]),
+                             C.byref(res), x.ctypes.data, y.ctypes.data, s.ctypes.data))
+        return _result(res, x, y, s)
+
+    def close(self):
+        if self.h:
+            _ck(lib().lk_ipm_destroy(self.h))
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def verify_optimal(lp, x, pi, tol=1e-6, lb=None, ub=None, maximize=False):
+    res = C.c_double(0.0)
+    ok = C.c_int32(0)
+    lbp = _f64(lb) if lb is not None else None
+    ubp = _f64(ub) if ub is not None else None
+    _ck(lib().lk_verify_optimal(lp.m, lp.n, _i32(lp.indptr), _i32(lp.indices), _f64(lp.data),
+                                _f64(lp.c), _f64(lp.b), _ptr(lbp), _ptr(ubp), int(maximize),
+                                _f64(x), _f64(pi), tol, C.byref(res), C.byref(ok)))
+    return bool(ok.value), res.value

@@ -1,0 +1,33 @@
+"""MehrotraSolver on the device: reference KATs, oracle parity, HiGHS golden objectives."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_textbook_lp(gpu):
+    """tests/unit/test_mehrotra.cu:29-78: min -3x1-5x2, x1+x2+x3=4, 2x1+3x2+x4=12 -> -20."""
+    from conftest import lp_from_rows
+    lp = lp_from_rows([[1, 1, 1, 0], [2, 3, 0, 1]], [4, 12], [-3, -5, 0, 0], 4)
+    r = gpu.ipm_solve(lp)
+    assert r["status_name"] == "Optimal"
+    assert r["iterations"] > 0
+    assert r["primal_residual"] < 1e-6 and r["dual_residual"] < 1e-6 and r["duality_gap"] < 1e-6
+    assert abs(r["objective"] - (-20.0)) <= 1e-4
+    assert r["x"].shape == (4,) and r["y"].shape == (2,)
+    assert np.isfinite(r["x"]).all() and np.isfinite(r["y"]).all()
+    r = gpu.ipm_solve(lp, relative=True)
+    assert abs(r["objective"] + 20.0) <= 1e-8 * 20
+
+
+@pytest.mark.parametrize("m,n,k,seed", [(60, 150, 3, 11), (200, 500, 6, 12), (400, 1000, 6, 13)])
+def test_matches_oracle(gpu, oracle, m, n, k, seed):
+    from lakshya import gen
+    lp = gen.random_sparse(m, n, k, seed=seed)
+    g = gpu.ipm_solve(lp, relative=True)
+    o = oracle.ipm_solve(lp.csr(), lp.b, lp.c, lp.n, stop_rule=1)
+    assert g["status_name"] == "Optimal" and o["status"] == 0
+    assert abs(g["objective"] - o["objective"]) <= 1e-8 * max(1.0, abs(o["objective"]))
+    assert g["rel_gap"] <= 1e-8 and g["rel_primal_residual"] <= 1e-8 and g["rel_dual_residual"] <= 1e-8
+    ok, _ = gpu.verify_optimal(lp, g["x"], g["y"], tol=1e-6)
+    assert ok
