@@ -85,8 +85,11 @@ public:
     // order and is overwritten with dy.  Same contract as gpu_cholesky_solve_device.
     bool solve_cached(Float* d_rhs, Float required_abs_tol, KKTSolveInfo* info = nullptr,
                       int max_refine = 10);
-    // Mixed precision: preconditioned CG on M0 dy = rhs with the FP32 factor as preconditioner.
-    bool solve_pcg(Float* d_rhs, Float required_abs_tol, int max_iter, KKTSolveInfo* info);
+    // Direct solve + preconditioned CG on M0 dy = rhs with the cached (FP64 or FP32) factor as
+    // preconditioner.  Success when ||rhs - M0 dy||_inf <= tol or, if backward_error_acceptance,
+    // when ||rhs - M0 dy||_inf <= 1e-12 (||rhs||_inf + max_i M0_ii ||dy||_inf).
+    bool solve_pcg(Float* d_rhs, Float required_abs_tol, int max_iter, KKTSolveInfo* info,
+                   bool backward_error_acceptance);
     // Raw triangular solves with the cached factor on a permuted device vector (tests).
     void triangular_solve_permuted(Float* d_x);
 

@@ -53,7 +53,7 @@ struct IpmOptions {
     Index max_iter = 200;
     Precision precision = Precision::FP64;
     Float mixed_fp64_spread = 1e12;   // max(Theta)/min(Theta) beyond which FP64 factors are used
-    int mixed_max_refine = 40;        // refinement budget with an FP32 factor before FP64 retry
+    int mixed_max_refine = 60;        // PCG budget with an FP32 factor before the FP64 retry
     Ordering ordering = Ordering::Amd;
     bool use_graphs = false;
     bool verbose = false;

@@ -135,13 +135,33 @@ int lkd_kkt_gather(int32_t m, const int32_t* P, const double* rhs, const double*
 /* dy[i] = (accumulate ? dy[i] : 0) + D[i] * z[i];  optionally backup[i] = old dy[i] */
 int lkd_kkt_update(int32_t m, const double* D, const double* z, double* dy, double* backup,
                    int accumulate, void* stream);
-/* r = rhs_perm - A_p (theta o (A_p^T dy)) with max |r| into *rinf (device double):
- * the FP64 residual against the unregularised M0 (src/cuda/kkt.cu:110-123). */
+/* r = rhs_perm - A_p (theta o (A_p^T dy)), req = D r.  kS (device doubles) receives
+ * kS[0] = max|r|, kS[1] = max|rhs_perm|, kS[2] = max|dy|, kS[3] = max M0_ii (m0d); block
+ * partials use kS + LKD_KS_PART.  The FP64 residual against the unregularised M0
+ * (src/cuda/kkt.cu:110-123, computed without storing M0). */
+#define LKD_KS_SLOTS 16
+#define LKD_KS_PART 16
 int lkd_kkt_residual(int32_t m, int32_t n, const int32_t* colptr, const int32_t* prow,
                      const double* cval, const int32_t* prp, const int32_t* pci,
                      const double* pval, const double* theta, const double* dy,
-                     const double* rhs_perm, double* u, double* r, double* D, double* req,
-                     double* rinf, void* stream);
+                     const double* rhs_perm, const double* m0d, double* u, double* r,
+                     const double* D, double* req, double* kS, void* stream);
+/* Preconditioned CG pieces on the permuted normal equations (kS slots 4..15):
+ *   matvec:  q = M0 p (two passes), kS[pq] = p^T q
+ *   update:  a = kS[rz]/kS[pq]; dy += a p; r -= a q; req = D r; kS[4] = max|r|, kS[2] = max|dy|
+ *   precdot: z <- D z (after the triangular solves), kS[rz_out] = r^T z
+ *   dir:     p = z + (kS[rz_new]/kS[rz_old]) p */
+int lkd_pcg_matvec(int32_t m, int32_t n, const int32_t* colptr, const int32_t* prow,
+                   const double* cval, const int32_t* prp, const int32_t* pci,
+                   const double* pval, const double* theta, const double* p, double* u,
+                   double* q, double* kS, int32_t slot_pq, void* stream);
+int lkd_pcg_update(int32_t m, double* kS, int32_t slot_rz, int32_t slot_pq, const double* p,
+                   const double* q, double* dy, double* r, const double* D, double* req,
+                   void* stream);
+int lkd_pcg_precdot(int32_t m, const double* D, double* z, const double* r, double* kS,
+                    int32_t slot_rz, void* stream);
+int lkd_pcg_dir(int32_t m, const double* kS, int32_t slot_new, int32_t slot_old,
+                const double* z, double* p, void* stream);
 /* out[P[i]] = dy[i] (src/cuda/kkt.cu:105-108) */
 int lkd_kkt_scatter(int32_t m, const int32_t* P, const double* dy, double* out, void* stream);
 /* gather L (panel storage) into CSR values in the reference's L layout (diagonal last). */

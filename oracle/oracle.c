@@ -17,7 +17,10 @@
  * Dense Cholesky (O(m^3)) -- sized for m up to a few hundred.
  * Deliberate differences from the reference: the factorization of D*M0*D + delta*I is cached per
  * IPM iteration (the reference recomputes the identical factor in every solve call), and nothing
- * is printed.  Sums use the reference's term order; multiply-adds use fma() so the assembled
+ * is printed.  Relative-gap mode (stop_rule 1, an extension with no reference counterpart) uses
+ * PCG instead of plain refinement, accepts a solve at the backward-error level, and skips the
+ * e1 re-check of the KKT residual: the reference's absolute 1e-9 acceptance cannot be met near
+ * a 1e-8 gap (its own loop stops at 1e-6).  The device path implements the same rule.  Sums use the reference's term order; multiply-adds use fma() so the assembled
  * M0 is bit-identical to an fma-based device kernel with the same order.
  *
  * Parity pins (tests/test_oracle.py): the symbolic KATs of tests/unit/test_ipm_symbolic.cpp,
@@ -49,6 +52,8 @@ typedef struct {
     double primal_residual, dual_residual, duality_gap;  /* scaled ||rp||2, ||rd||2, mu */
     double rel_gap, rel_primal_inf, rel_dual_inf, dual_objective;
     int factorizations, solves;
+    int fail_code;      /* 0 none, 1 factorization, 2 predictor solve, 3 predictor check,
+                           4 corrector solve, 5 no step accepted */
 } or_ipm_result;
 
 /* ------------------------------------------------------------------------------------------ */
@@ -281,7 +286,66 @@ static void m0_residual(const or_kkt* k, const double* dy, const double* rhs, do
 }
 
 /* src/cuda/kkt.cu:620-870.  rhs (original order) is overwritten by dy even on failure. */
-static int kkt_solve(or_kkt* k, double* rhs_orig, double tol) {
+/* Relative-gap mode only: a solve is also accepted when its residual is at the level a backward
+ * stable solve can reach, ||r||_inf <= 1e-12 (||rhs||_inf + max_i M0_ii ||dy||_inf). */
+static int backward_ok(const or_kkt* k, const double* rhs, const double* dy, double rinf) {
+    int m = k->m;
+    double mx = 0.0;
+    for (int i = 0; i < m; ++i) if (k->M0[(size_t)i * m + i] > mx) mx = k->M0[(size_t)i * m + i];
+    return rinf <= 1e-12 * (inf_norm(m, rhs) + mx * inf_norm(m, dy));
+}
+
+/* Relative-gap mode: preconditioned conjugate gradients on M0 dy = rhs (permuted), started from
+ * the direct solve, preconditioner D (L L^T)^{-1} D with the cached equilibrated factor.  The
+ * true residual is recomputed whenever the recurrence claims convergence and at the end; the
+ * best iterate (smallest true ||r||_inf) is returned.  Returns 1 on convergence. */
+static int kkt_pcg(or_kkt* k, const double* rhs, double* dy, double tol, int max_it) {
+    int m = k->m;
+    double *r = malloc(sizeof(double) * (m + 1)), *z = malloc(sizeof(double) * (m + 1)),
+           *p = malloc(sizeof(double) * (m + 1)), *q = malloc(sizeof(double) * (m + 1)),
+           *best = malloc(sizeof(double) * (m + 1));
+    m0_residual(k, dy, rhs, r);
+    double rbest = inf_norm(m, r);
+    memcpy(best, dy, sizeof(double) * m);
+    int ok = rbest <= tol || backward_ok(k, rhs, dy, rbest);
+    double rz = 0.0;
+    if (!ok) {
+        for (int i = 0; i < m; ++i) z[i] = r[i] * k->D[i];
+        chol_solve(m, k->Leq, z);
+        for (int i = 0; i < m; ++i) { z[i] *= k->D[i]; p[i] = z[i]; rz += r[i] * z[i]; }
+    }
+    for (int it = 0; it < max_it && !ok; ++it) {
+        for (int i = 0; i < m; ++i) {
+            double t = 0.0;
+            for (int j = 0; j < m; ++j) t += k->M0[(size_t)i * m + j] * p[j];
+            q[i] = t;
+        }
+        double pq = 0.0;
+        for (int i = 0; i < m; ++i) pq += p[i] * q[i];
+        if (!(pq > 0.0)) break;
+        double alpha = rz / pq;
+        for (int i = 0; i < m; ++i) { dy[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
+        double rr = inf_norm(m, r);
+        if (rr <= tol || backward_ok(k, rhs, dy, rr) || it == max_it - 1 || (it + 1) % 10 == 0) {
+            m0_residual(k, dy, rhs, r);          /* true residual */
+            rr = inf_norm(m, r);
+            if (rr < rbest) { rbest = rr; memcpy(best, dy, sizeof(double) * m); }
+            if (rr <= tol || backward_ok(k, rhs, dy, rr)) { ok = 1; break; }
+        }
+        for (int i = 0; i < m; ++i) z[i] = r[i] * k->D[i];
+        chol_solve(m, k->Leq, z);
+        double rz2 = 0.0;
+        for (int i = 0; i < m; ++i) { z[i] *= k->D[i]; rz2 += r[i] * z[i]; }
+        double beta = rz2 / rz;
+        rz = rz2;
+        for (int i = 0; i < m; ++i) p[i] = z[i] + beta * p[i];
+    }
+    if (!ok) memcpy(dy, best, sizeof(double) * m);
+    free(r); free(z); free(p); free(q); free(best);
+    return ok;
+}
+
+static int kkt_solve(or_kkt* k, double* rhs_orig, double tol, int relmode) {
     int m = k->m;
     double* rhs = malloc(sizeof(double) * (m + 1));
     double* dy = calloc((size_t)m + 1, sizeof(double));
@@ -304,6 +368,7 @@ static int kkt_solve(or_kkt* k, double* rhs_orig, double tol) {
         m0_residual(k, dy, rhs, r);
         double before = inf_norm(m, r);
         if (before <= tol) { success = 1; break; }
+        if (relmode) { success = kkt_pcg(k, rhs, dy, tol, 60); break; }
         int converged = 0;
         double after = before;
         for (int it = 0; it < 10; ++it) {
@@ -351,7 +416,7 @@ int or_kkt_factor_solve(int m, const int* rp, const int* ci, const double* val,
         for (int i = 0; i < m; ++i) if (hd[i]) Lout_eq_free[(size_t)i * m + i] += 1e-9;
         dense_chol(m, Lout_eq_free);
     }
-    if (st == OR_OK && rhs) *success = kkt_solve(&k, rhs, tol);
+    if (st == OR_OK && rhs) *success = kkt_solve(&k, rhs, tol, 0);
     free(hd); free(k.M0); free(k.Leq); free(k.D);
     return st;
 }
@@ -432,11 +497,13 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
     for (int j = 0; j < n; ++j) { x[j] = 1.0; s[j] = 1.0; }
     res->factorizations = 0;
     res->solves = 0;
+    res->fail_code = 0;
     res->status = 0;
     res->iterations = 0;
     double rp0 = 1, rd0 = 1, mu0 = 1;
     int rc = OR_OK;
     int max_iter = opt ? opt->max_iter : 200;
+    int relmode = opt && opt->stop_rule == 1;
     int done = 0;
 
 #define RESID(X, Y, S, RP, RD)                                                   \
@@ -458,6 +525,7 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
         double e1 = nrm2(m, r1) / fmax(1.0, nrp);                                 \
         double e2 = nrm2(n, r2) / fmax(1.0, nrd);                                 \
         double e3 = nrm2(n, r3) / fmax(1.0, nrm2(n, rxs));                        \
+        if (relmode) e1 = 0.0;                                                    \
         OK = !(e1 > 1e-8 || e2 > 1e-8 || e3 > 1e-8);                              \
     } while (0)
 
@@ -468,7 +536,7 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
         for (int i_ = 0; i_ < m; ++i_) dy[i_] = -rpv[i_];                         \
         spmv(m, rp, ci, A, v, dy);                                                \
         res->solves++;                                                            \
-        OK = kkt_solve(&k, dy, tol_kkt);                                          \
+        OK = kkt_solve(&k, dy, tol_kkt, relmode);                                          \
         for (int j_ = 0; j_ < n; ++j_) DSO[j_] = -rd[j_];                         \
         for (int i_ = 0; i_ < m; ++i_)                                            \
             for (int p_ = rp[i_]; p_ < rp[i_ + 1]; ++p_) DSO[ci[p_]] += -1.0 * A[p_] * dy[i_]; \
@@ -496,16 +564,16 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
         for (int j = 0; j < n; ++j) th[j] = x[j] / s[j];
         or_assemble_dense(m, rp, ci, A, th, P, k.M0);
         res->factorizations++;
-        if (kkt_factor_check(&k) != OR_OK) { rc = OR_NOT_SPD; res->status = -1; res->iterations = iter; break; }
+        if (kkt_factor_check(&k) != OR_OK) { rc = OR_NOT_SPD; res->status = -1; res->iterations = iter; res->fail_code = 1; break; }
         double tol_kkt = 1e-9 * fmax(1.0, nrp);
 
         /* predictor */
         for (int j = 0; j < n; ++j) rxs[j] = -x[j] * s[j] + 0.0;
         int ok;
         DIRECTION(dxa, dsa, ok);
-        if (!ok) { res->status = 3; res->iterations = iter; break; }
+        if (!ok) { res->status = 3; res->iterations = iter; res->fail_code = 2; break; }
         VERIFY(dxa, dy, dsa, ok);
-        if (!ok) { res->status = 3; res->iterations = iter; break; }
+        if (!ok) { res->status = 3; res->iterations = iter; res->fail_code = 3; break; }
         double ap_aff = fmin(1.0, step_len(n, x, dxa)), ad_aff = fmin(1.0, step_len(n, s, dsa));
         for (int j = 0; j < n; ++j) { tx[j] = x[j] + ap_aff * dxa[j]; ts[j] = s[j] + ad_aff * dsa[j]; }
         double mu_aff = dot(n, tx, ts) / n;
@@ -514,7 +582,7 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
         /* corrector */
         for (int j = 0; j < n; ++j) rxs[j] = -x[j] * s[j] + sigma * mu - dxa[j] * dsa[j];
         DIRECTION(dx, ds, ok);
-        if (!ok) { res->status = 3; res->iterations = iter; break; }
+        if (!ok) { res->status = 3; res->iterations = iter; res->fail_code = 4; break; }
         int corr_valid;
         VERIFY(dx, dy, ds, corr_valid);
         double ap = fmin(1.0, 0.995 * step_len(n, x, dx)), ad = fmin(1.0, 0.995 * step_len(n, s, ds));
@@ -550,7 +618,7 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                 ad *= 0.5;
             }
         }
-        if (!accepted) { res->status = 3; res->iterations = iter; break; }
+        if (!accepted) { res->status = 3; res->iterations = iter; res->fail_code = 5; break; }
         memcpy(x, bx, sizeof(double) * n);
         memcpy(y, by, sizeof(double) * m);
         memcpy(s, bs, sizeof(double) * n);

@@ -26,7 +26,7 @@ class IpmResult(C.Structure):
                 ("duality_gap", C.c_double), ("rel_gap", C.c_double),
                 ("rel_primal_inf", C.c_double), ("rel_dual_inf", C.c_double),
                 ("dual_objective", C.c_double), ("factorizations", C.c_int),
-                ("solves", C.c_int)]
+                ("solves", C.c_int), ("fail_code", C.c_int)]
 
 
 def build(force: bool = False) -> str:

@@ -178,13 +178,15 @@ __global__ __launch_bounds__(NT) void k_res_cols(int32_t n, const int32_t* __res
     }
 }
 
-// r_i = rhs_i - sum_j A_p[i][j] u_j ; req_i = D_i r_i ; partial max |r_i|
+// r_i = rhs_i - sum_j A_p[i][j] u_j ; req_i = D_i r_i ; partial max |r|, |rhs|, |dy|, M0_ii
 template <int G>
 __global__ __launch_bounds__(NT) void k_res_rows(int32_t m, const int32_t* __restrict__ prp,
                                                  const int32_t* __restrict__ pci,
                                                  const double* __restrict__ pval,
                                                  const double* __restrict__ u,
                                                  const double* __restrict__ rhs,
+                                                 const double* __restrict__ dy,
+                                                 const double* __restrict__ m0d,
                                                  double* __restrict__ r,
                                                  const double* __restrict__ D,
                                                  double* __restrict__ req,
@@ -193,18 +195,98 @@ __global__ __launch_bounds__(NT) void k_res_rows(int32_t m, const int32_t* __res
     const int lane = threadIdx.x & (G - 1);
     const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
     const int64_t ng = (int64_t)gridDim.x * NT / G;
-    double mx = 0.0;
+    double mr = 0.0, mb = 0.0, md = 0.0, mm = 0.0;
     for (int64_t i = g0; i < m; i += ng) {
         const double s = row_dot<G>(prp, pci, pval, u, i, lane);
         if (lane == 0) {
             const double v = rhs[i] - s;
             r[i] = v;
             req[i] = v * D[i];
-            mx = fmax(mx, fabs(v));
+            mr = fmax(mr, fabs(v));
+            mb = fmax(mb, fabs(rhs[i]));
+            md = fmax(md, fabs(dy[i]));
+            mm = fmax(mm, m0d[i]);
         }
     }
-    mx = block_reduce(mx, OP_MAX, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = mx;
+    const int nb = gridDim.x;
+    double t;
+    t = block_reduce(mr, OP_MAX, sh); if (threadIdx.x == 0) part[0 * nb + blockIdx.x] = t;
+    t = block_reduce(mb, OP_MAX, sh); if (threadIdx.x == 0) part[1 * nb + blockIdx.x] = t;
+    t = block_reduce(md, OP_MAX, sh); if (threadIdx.x == 0) part[2 * nb + blockIdx.x] = t;
+    t = block_reduce(mm, OP_MAX, sh); if (threadIdx.x == 0) part[3 * nb + blockIdx.x] = t;
+}
+
+// q_i = sum_j A_p[i][j] u_j ; partial p^T q
+template <int G>
+__global__ __launch_bounds__(NT) void k_pcg_rows(int32_t m, const int32_t* __restrict__ prp,
+                                                 const int32_t* __restrict__ pci,
+                                                 const double* __restrict__ pval,
+                                                 const double* __restrict__ u,
+                                                 const double* __restrict__ p,
+                                                 double* __restrict__ q,
+                                                 double* __restrict__ part) {
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
+    const int64_t ng = (int64_t)gridDim.x * NT / G;
+    double acc = 0.0;
+    for (int64_t i = g0; i < m; i += ng) {
+        const double s = row_dot<G>(prp, pci, pval, u, i, lane);
+        if (lane == 0) { q[i] = s; acc += p[i] * s; }
+    }
+    acc = block_reduce(acc, OP_SUM, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(NT) void k_pcg_update(int32_t m, const double* __restrict__ kS,
+                                                   int32_t srz, int32_t spq,
+                                                   const double* __restrict__ p,
+                                                   const double* __restrict__ q,
+                                                   double* __restrict__ dy,
+                                                   double* __restrict__ r,
+                                                   const double* __restrict__ D,
+                                                   double* __restrict__ req,
+                                                   double* __restrict__ part) {
+    __shared__ double sh[4];
+    const double a = kS[srz] / kS[spq];
+    double mr = 0.0, md = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < m; i += (int64_t)gridDim.x * NT) {
+        const double d = dy[i] + a * p[i];
+        const double v = r[i] - a * q[i];
+        dy[i] = d;
+        r[i] = v;
+        req[i] = v * D[i];
+        mr = fmax(mr, fabs(v));
+        md = fmax(md, fabs(d));
+    }
+    const int nb = gridDim.x;
+    double t;
+    t = block_reduce(mr, OP_MAX, sh); if (threadIdx.x == 0) part[0 * nb + blockIdx.x] = t;
+    t = block_reduce(md, OP_MAX, sh); if (threadIdx.x == 0) part[1 * nb + blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(NT) void k_pcg_precdot(int32_t m, const double* __restrict__ D,
+                                                    double* __restrict__ z,
+                                                    const double* __restrict__ r,
+                                                    double* __restrict__ part) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < m; i += (int64_t)gridDim.x * NT) {
+        const double v = z[i] * D[i];
+        z[i] = v;
+        acc += r[i] * v;
+    }
+    acc = block_reduce(acc, OP_SUM, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(NT) void k_pcg_dir(int32_t m, const double* __restrict__ kS,
+                                                int32_t snew, int32_t sold,
+                                                const double* __restrict__ z,
+                                                double* __restrict__ p) {
+    const double beta = kS[snew] / kS[sold];
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < m; i += (int64_t)gridDim.x * NT)
+        p[i] = z[i] + beta * p[i];
 }
 
 __global__ __launch_bounds__(NT) void k_scatter(int32_t m, const int32_t* __restrict__ P,
@@ -513,9 +595,14 @@ template <int G> void l_dir_rows(const lkd_ipm* P, const double* dx, const doubl
     LK_LAUNCH(k_dir_rows<G>, P->nblk_rows, NT, 0, st, *P, dx, rp, dy, qp, qoff);
 }
 template <int G> void l_res_rows(int32_t m, const int32_t* prp, const int32_t* pci, const double* pval,
-                                 const double* u, const double* rhs, double* r, const double* D,
-                                 double* req, double* part, int nblk, void* st) {
-    LK_LAUNCH(k_res_rows<G>, nblk, NT, 0, st, m, prp, pci, pval, u, rhs, r, D, req, part);
+                                 const double* u, const double* rhs, const double* dy, const double* m0d,
+                                 double* r, const double* D, double* req, double* part, int nblk, void* st) {
+    LK_LAUNCH(k_res_rows<G>, nblk, NT, 0, st, m, prp, pci, pval, u, rhs, dy, m0d, r, D, req, part);
+}
+template <int G> void l_pcg_rows(int32_t m, const int32_t* prp, const int32_t* pci, const double* pval,
+                                 const double* u, const double* p, double* q, double* part, int nblk,
+                                 void* st) {
+    LK_LAUNCH(k_pcg_rows<G>, nblk, NT, 0, st, m, prp, pci, pval, u, p, q, part);
 }
 
 }  // namespace
@@ -565,17 +652,51 @@ int lkd_kkt_update(int32_t m, const double* D, const double* z, double* dy, doub
 int lkd_kkt_residual(int32_t m, int32_t n, const int32_t* colptr, const int32_t* prow,
                      const double* cval, const int32_t* prp, const int32_t* pci,
                      const double* pval, const double* theta, const double* dy,
-                     const double* rhs_perm, double* u, double* r, double* D, double* req,
-                     double* rinf, void* stream) {
+                     const double* rhs_perm, const double* m0d, double* u, double* r,
+                     const double* D, double* req, double* kS, void* stream) {
     LK_LAUNCH(k_res_cols, grid_for(n), NT, 0, stream, n, colptr, prow, cval, theta, dy, u);
-    // rinf points at a device double; block partials are placed right after it (caller sizes it)
     const int nblk = grid_for(m, 1024);
-    double* part = rinf + 1;
-    const int64_t nnz = prp[0] == 0 ? 0 : 0;  // (row lengths are not inspected on the host)
-    (void)nnz;
-    l_res_rows<8>(m, prp, pci, pval, u, rhs_perm, r, D, req, part, nblk, stream);
-    int op = OP_MAX, slot = 0;
-    return finalize(part, nblk, 1, &op, &slot, rinf, stream);
+    double* part = kS + LKD_KS_PART;
+    l_res_rows<8>(m, prp, pci, pval, u, rhs_perm, dy, m0d, r, D, req, part, nblk, stream);
+    int ops[4] = {OP_MAX, OP_MAX, OP_MAX, OP_MAX}, slots[4] = {0, 1, 2, 3};
+    return finalize(part, nblk, 4, ops, slots, kS, stream);
+}
+
+int lkd_pcg_matvec(int32_t m, int32_t n, const int32_t* colptr, const int32_t* prow,
+                   const double* cval, const int32_t* prp, const int32_t* pci,
+                   const double* pval, const double* theta, const double* p, double* u,
+                   double* q, double* kS, int32_t slot_pq, void* stream) {
+    LK_LAUNCH(k_res_cols, grid_for(n), NT, 0, stream, n, colptr, prow, cval, theta, p, u);
+    const int nblk = grid_for(m, 1024);
+    double* part = kS + LKD_KS_PART;
+    l_pcg_rows<8>(m, prp, pci, pval, u, p, q, part, nblk, stream);
+    int op = OP_SUM;
+    return finalize(part, nblk, 1, &op, &slot_pq, kS, stream);
+}
+
+int lkd_pcg_update(int32_t m, double* kS, int32_t slot_rz, int32_t slot_pq, const double* p,
+                   const double* q, double* dy, double* r, const double* D, double* req,
+                   void* stream) {
+    const int nblk = grid_for(m, 1024);
+    double* part = kS + LKD_KS_PART;
+    LK_LAUNCH(k_pcg_update, nblk, NT, 0, stream, m, kS, slot_rz, slot_pq, p, q, dy, r, D, req, part);
+    int ops[2] = {OP_MAX, OP_MAX}, slots[2] = {4, 2};
+    return finalize(part, nblk, 2, ops, slots, kS, stream);
+}
+
+int lkd_pcg_precdot(int32_t m, const double* D, double* z, const double* r, double* kS,
+                    int32_t slot_rz, void* stream) {
+    const int nblk = grid_for(m, 1024);
+    double* part = kS + LKD_KS_PART;
+    LK_LAUNCH(k_pcg_precdot, nblk, NT, 0, stream, m, D, z, r, part);
+    int op = OP_SUM;
+    return finalize(part, nblk, 1, &op, &slot_rz, kS, stream);
+}
+
+int lkd_pcg_dir(int32_t m, const double* kS, int32_t slot_new, int32_t slot_old,
+                const double* z, double* p, void* stream) {
+    LK_LAUNCH(k_pcg_dir, grid_for(m), NT, 0, stream, m, kS, slot_new, slot_old, z, p);
+    return lk_last_error();
 }
 
 int lkd_kkt_scatter(int32_t m, const int32_t* P, const double* dy, double* out, void* stream) {

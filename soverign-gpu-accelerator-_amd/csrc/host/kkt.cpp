@@ -542,7 +542,7 @@ struct GPUKKTCholeskySolver::Impl {
         d_bak = dv(m);
         d_rhsp = dv(m);
         d_u = dv(n);
-        d_rinf = dv(2 + 1024);
+        d_rinf = dv(LKD_KS_PART + 8 * 1024);
         d_pcg = dv(4 * static_cast<int64_t>(m));
         if (nnz_csr > 0) d_csr = dv(nnz_csr);
         std::vector<int32_t> sn_first(S.sn_first.begin(), S.sn_first.end());
@@ -671,16 +671,83 @@ struct GPUKKTCholeskySolver::Impl {
     }
     int retries_used = 0;
 
-    double read_rinf() {
-        double v = 0;
-        check(lkd_memcpy_dtoh(&v, d_rinf, sizeof(double), stream), "residual read");
-        return v;
-    }
-
-    // r = rhs_perm - M0 dy (M0 dy = A_p (theta o (A_p^T dy))), req = D r, rinf = max |r|
+    // r = rhs_perm - M0 dy (M0 dy = A_p (theta o (A_p^T dy))), req = D r;
+    // kS[0..3] = max|r|, max|rhs|, max|dy|, max M0_ii
     void residual() {
         check(lkd_kkt_residual(m, n, d_colptr, d_prow, d_cval, d_prp, d_pci, d_pval, d_theta,
-                               d_dy, d_rhsp, d_u, d_r, d_D, d_req, d_rinf, stream), "residual");
+                               d_dy, d_rhsp, d_m0d, d_u, d_r, d_D, d_req, d_rinf, stream), "residual");
+    }
+    double hk[LKD_KS_SLOTS];
+    void read_ks() {
+        check(lkd_memcpy_dtoh(hk, d_rinf, sizeof(hk), stream), "kkt scalars");
+    }
+    // backward-error acceptance (relative-gap mode): ||r|| <= 1e-12 (||rhs|| + max M0_ii ||dy||)
+    bool backward_ok(double rinf) const { return rinf <= 1e-12 * (hk[1] + hk[3] * hk[2]); }
+
+    // Direct solve with the cached factor, then preconditioned CG on M0 dy = rhs (preconditioner
+    // D (L L^T)^{-1} D with the cached FP64 or FP32 factor).  Residual replacement every 10 steps
+    // and before accepting; the best true-residual iterate is kept.
+    bool solve_pcg(double* d_rhs, double tol, int max_it, bool backward, KKTSolveInfo* info) {
+        if (!have_eq) throw std::logic_error("KKT solve: no factorization");
+        const int prec = cached_prec == FactorPrecision::FP32 ? LKD_F32 : LKD_F64;
+        const size_t vb = static_cast<size_t>(m) * sizeof(double);
+        double* d_p = d_pcg;
+        double* d_q = d_pcg + m;
+        double* d_best = d_pcg + 2 * static_cast<int64_t>(m);
+        st.solves++;
+        check(lkd_kkt_gather(m, d_perm, d_rhs, d_D, d_rhsp, d_req, stream), "gather");
+        check(lkd_memcpy_dtod(d_z, d_req, vb, stream), "copy");
+        trsv(prec, d_z);
+        check(lkd_kkt_update(m, d_D, d_z, d_dy, nullptr, 0, stream), "update");
+        residual();
+        read_ks();
+        double best = hk[0];
+        bool ok = best <= tol || (backward && backward_ok(best));
+        int its = 0;
+        if (!ok) {
+            check(lkd_memcpy_dtod(d_best, d_dy, vb, stream), "copy");
+            check(lkd_memcpy_dtod(d_z, d_req, vb, stream), "copy");
+            trsv(prec, d_z);
+            int sa = 5, sb = 7;   // rz slots (ping-pong), pq slot 6
+            check(lkd_pcg_precdot(m, d_D, d_z, d_r, d_rinf, sa, stream), "precdot");
+            check(lkd_memcpy_dtod(d_p, d_z, vb, stream), "copy");
+            for (int it = 0; it < max_it; ++it) {
+                its = it + 1;
+                check(lkd_pcg_matvec(m, n, d_colptr, d_prow, d_cval, d_prp, d_pci, d_pval, d_theta,
+                                     d_p, d_u, d_q, d_rinf, 6, stream), "matvec");
+                check(lkd_pcg_update(m, d_rinf, sa, 6, d_p, d_q, d_dy, d_r, d_D, d_req, stream), "pcg update");
+                read_ks();
+                if (!(hk[6] > 0.0)) break;
+                const double rr = hk[4];
+                const double bw = 1e-12 * (hk[1] + hk[3] * hk[2]);
+                if (rr <= tol || (backward && rr <= bw) || it == max_it - 1 || (it + 1) % 10 == 0) {
+                    residual();
+                    read_ks();
+                    const double rt = hk[0];
+                    if (rt < best) {
+                        best = rt;
+                        check(lkd_memcpy_dtod(d_best, d_dy, vb, stream), "copy");
+                    }
+                    if (rt <= tol || (backward && backward_ok(rt))) { ok = true; break; }
+                }
+                check(lkd_memcpy_dtod(d_z, d_req, vb, stream), "copy");
+                trsv(prec, d_z);
+                check(lkd_pcg_precdot(m, d_D, d_z, d_r, d_rinf, sb, stream), "precdot");
+                check(lkd_pcg_dir(m, d_rinf, sb, sa, d_z, d_p, stream), "pcg dir");
+                std::swap(sa, sb);
+            }
+            if (!ok) check(lkd_memcpy_dtod(d_dy, d_best, vb, stream), "copy");
+        }
+        st.refinement_steps += its;
+        check(lkd_kkt_scatter(m, d_perm, d_dy, d_rhs, stream), "scatter");
+        if (info) {
+            info->retries = retries_used;
+            info->refinement_steps = its;
+            info->pcg_iterations = its;
+            info->delta = cached_delta;
+            info->residual_inf = best;
+        }
+        return ok;
     }
 
     bool solve_cached(double* d_rhs, double tol, KKTSolveInfo* info, int max_refine) {
@@ -692,7 +759,8 @@ struct GPUKKTCholeskySolver::Impl {
         trsv(prec, d_z);
         check(lkd_kkt_update(m, d_D, d_z, d_dy, nullptr, 0, stream), "update");
         residual();
-        double before = read_rinf();
+        read_ks();
+        double before = hk[0];
         bool ok = before <= tol;
         int steps = 0;
         double after = before;
@@ -702,7 +770,8 @@ struct GPUKKTCholeskySolver::Impl {
                 trsv(prec, d_z);
                 check(lkd_kkt_update(m, d_D, d_z, d_dy, d_bak, 1, stream), "update");
                 residual();
-                after = read_rinf();
+                read_ks();
+                after = hk[0];
                 ++steps;
                 if (after <= tol) { ok = true; break; }
                 if (after >= before) {
@@ -829,10 +898,9 @@ bool GPUKKTCholeskySolver::solve_cached(Float* d_rhs, Float tol, KKTSolveInfo* i
     return impl_->solve_cached(d_rhs, tol, info, max_refine);
 }
 
-bool GPUKKTCholeskySolver::solve_pcg(Float* d_rhs, Float tol, int max_iter, KKTSolveInfo* info) {
-    // refinement with the (possibly FP32) factor is the preconditioned Richardson iteration;
-    // the mixed-precision loop uses it with a larger step budget.
-    return impl_->solve_cached(d_rhs, tol, info, max_iter);
+bool GPUKKTCholeskySolver::solve_pcg(Float* d_rhs, Float tol, int max_iter, KKTSolveInfo* info,
+                                     bool backward_error_acceptance) {
+    return impl_->solve_pcg(d_rhs, tol, max_iter, backward_error_acceptance, info);
 }
 
 void GPUKKTCholeskySolver::triangular_solve_permuted(Float* d_x) {

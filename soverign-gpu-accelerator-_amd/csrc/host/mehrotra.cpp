@@ -220,31 +220,39 @@ public:
                    double tol, const Norms& nr, Dir& d, MehrotraResult& res, bool& used_fp32) {
         check(lkd_ipm_rhs(&P_, x_, s_, th_, rd_, rp_, dxa, dsa, sigma_mu, rxs_, v_, dy_, stream_), "rhs");
         gpu::KKTSolveInfo info;
-        bool ok;
-        if (used_fp32) {
-            // keep a copy of the right-hand side in case the FP64 factor has to take over
-            check(lkd_memcpy_dtod(qp_, dy_, sizeof(double) * m_, stream_), "copy");
-            ok = kkt_->solve_cached(dy_, tol, &info, opt_.mixed_max_refine);
+        const bool rel = opt_.stop_rule == IpmOptions::StopRule::RelativeGap;
+        // reference mode: refinement against M0 as src/cuda/kkt.cu:771-842; relative-gap mode and
+        // FP32 factors: PCG preconditioned by the factor (backward-error acceptance in rel mode)
+        auto run = [&](bool fp32) {
+            bool r;
+            if (rel || fp32) r = kkt_->solve_pcg(dy_, tol, fp32 ? opt_.mixed_max_refine : 60, &info, rel);
+            else r = kkt_->solve_cached(dy_, tol, &info, 10);
             res.refinement_steps += info.refinement_steps;
             res.solves++;
+            return r;
+        };
+        bool ok;
+        if (used_fp32) {
+            // keep the right-hand side in case the FP64 factor has to take over
+            check(lkd_memcpy_dtod(qp_, dy_, sizeof(double) * m_, stream_), "copy");
+            ok = run(true);
             if (!ok) {
                 factor(false, res);
                 used_fp32 = false;
-                check(lkd_memcpy_dtod(dy_, qp_, sizeof(double) * m_, stream_), "copy");
                 if (fp64_fail_) return false;
-                ok = kkt_->solve_cached(dy_, tol, &info, 10);
-                res.refinement_steps += info.refinement_steps;
-                res.solves++;
+                check(lkd_memcpy_dtod(dy_, qp_, sizeof(double) * m_, stream_), "copy");
+                ok = run(false);
             }
         } else {
-            ok = kkt_->solve_cached(dy_, tol, &info, 10);
-            res.refinement_steps += info.refinement_steps;
-            res.solves++;
+            ok = run(false);
         }
         if (!ok) return false;
         check(lkd_ipm_direction(&P_, x_, s_, y_, th_, rd_, rp_, rxs_, dy_, dx, ds, qd_, qp_, stream_), "direction");
         read_scalars(LKS_NSLOTS);
-        d.e1 = std::sqrt(hS_[LKS_R1]) / std::max(1.0, nr.rp);
+        // e1 re-checks the KKT residual; in relative-gap mode the solve's own (backward-error)
+        // acceptance replaces it
+        d.e1 = opt_.stop_rule == IpmOptions::StopRule::RelativeGap
+                   ? 0.0 : std::sqrt(hS_[LKS_R1]) / std::max(1.0, nr.rp);
         d.e2 = std::sqrt(hS_[LKS_R2]) / std::max(1.0, nr.rd);
         d.e3 = std::sqrt(hS_[LKS_R3]) / std::max(1.0, std::sqrt(hS_[LKS_RXS2]));
         d.apmax = hS_[LKS_APMAX];
