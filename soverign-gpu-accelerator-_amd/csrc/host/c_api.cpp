@@ -147,14 +147,20 @@ void copy_stats(const gpu::KKTStats& s, lk_kkt_stats* o) {
 
 }  // namespace
 
-struct lk_arena { gpu::VRAMArena arena; explicit lk_arena(size_t b) : arena(b) {} };
+// The arena is shared by the handles created from it, so destroying the lk_arena handle before
+// a KKT solver that allocated from it (any order a garbage collector may pick) stays safe.
+struct lk_arena {
+    std::shared_ptr<gpu::VRAMArena> ptr;
+    gpu::VRAMArena& arena;
+    explicit lk_arena(size_t b) : ptr(std::make_shared<gpu::VRAMArena>(b)), arena(*ptr) {}
+};
 struct lk_kkt {
     std::unique_ptr<gpu::GPUKKTCholeskySolver> solver;
     core::CSRMatrix A;
     ipm::SymbolicFactorization sym;
     bool reference_layout = false;
     double* d_rhs = nullptr;
-    lk_arena* arena = nullptr;
+    std::shared_ptr<gpu::VRAMArena> arena;
 };
 struct lk_ipm {
     core::Model model;
@@ -255,9 +261,9 @@ int lk_kkt_create(lk_arena* arena, int32_t m, int32_t n, const int32_t* rowptr,
         k->reference_layout = true;
         gpu::KKTOptions ko;
         ko.use_graphs = use_graphs != 0;
-        k->solver.reset(new gpu::GPUKKTCholeskySolver(arena->arena, k->A, k->sym, ko));
-        k->arena = arena;
-        k->d_rhs = static_cast<double*>(arena->arena.allocate(sizeof(double) * (m > 0 ? m : 1)));
+        k->arena = arena->ptr;
+        k->solver.reset(new gpu::GPUKKTCholeskySolver(*k->arena, k->A, k->sym, ko));
+        k->d_rhs = static_cast<double*>(k->arena->allocate(sizeof(double) * (m > 0 ? m : 1)));
         *out = k.release();
     });
 }
@@ -281,7 +287,7 @@ int lk_kkt_factor(lk_kkt* k, const double* theta, int32_t on_device, int32_t mod
         std::vector<double> tmp;
         double* buf = nullptr;
         if (!on_device) {
-            buf = static_cast<double*>(k->arena->arena.allocate(sizeof(double) * (n > 0 ? n : 1)));
+            buf = static_cast<double*>(k->arena->allocate(sizeof(double) * (n > 0 ? n : 1)));
             if (lkd_memcpy_htod(buf, theta, sizeof(double) * n, k->solver->stream()))
                 throw std::runtime_error("theta upload failed");
             d = buf;
@@ -289,7 +295,7 @@ int lk_kkt_factor(lk_kkt* k, const double* theta, int32_t on_device, int32_t mod
         const int r = k->solver->factor_equilibrated(
             d, mode == 2 ? gpu::FactorPrecision::FP32 : gpu::FactorPrecision::FP64);
         lkd_stream_sync(k->solver->stream());
-        if (buf) k->arena->arena.free(buf);
+        if (buf) k->arena->free(buf);
         *not_spd = r < 0 ? 1 : 0;
     });
 }
@@ -351,10 +357,10 @@ int lk_kkt_stream(lk_kkt* k, void** stream) { return guarded([&] { *stream = k->
 int lk_kkt_destroy(lk_kkt* k) {
     return guarded([&] {
         if (!k) return;
-        lk_arena* a = k->arena;
+        std::shared_ptr<gpu::VRAMArena> a = k->arena;
         double* d = k->d_rhs;
         k->solver.reset();
-        if (a && d) a->arena.free(d);
+        if (a && d) a->free(d);
         delete k;
     });
 }
