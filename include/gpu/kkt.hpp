@@ -19,7 +19,9 @@
 // M_pattern is accepted for interface compatibility; the structure is derived from A and P.
 #pragma once
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <string>
 #include <vector>
 #include "core/sparse_matrix.hpp"
 #include "gpu/vram_arena.hpp"
@@ -67,6 +69,10 @@ public:
     // Device bytes the solver will take from its arena for this structure (upper bound).
     static std::size_t estimate_bytes(const core::CSRMatrix& A,
                                       const ipm::SupernodalStructure& S);
+    // Host-only construction of the complete device plan (structure, pair map, task lists) for
+    // inspection and CPU emulation; keys name the arrays the device would receive.
+    static std::map<std::string, std::vector<char>> export_plan(
+        const core::CSRMatrix& A, const std::vector<Index>& P, const ipm::SupernodeOptions& so);
     ~GPUKKTCholeskySolver();
     GPUKKTCholeskySolver(const GPUKKTCholeskySolver&) = delete;
     GPUKKTCholeskySolver& operator=(const GPUKKTCholeskySolver&) = delete;

@@ -61,6 +61,15 @@ int lk_supernodal(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* co
                   const int32_t* P, int32_t relax_small, double relax_frac, lk_kkt_stats* out,
                   int32_t* perm_out, int32_t* sn_first_out, int32_t* nsuper_out);
 
+/* ---- plan inspection (host only): the device plan of the KKT solver for an ordering P
+ * (NULL -> AMD).  lk_plan_array copies the named array (call with dst NULL for its size). */
+typedef struct lk_plan lk_plan;
+int lk_plan_create(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                   const double* vals, const int32_t* P, int32_t relax_small, double relax_frac,
+                   lk_plan** out);
+int lk_plan_array(lk_plan* p, const char* name, void* dst, size_t capacity, size_t* bytes);
+int lk_plan_destroy(lk_plan* p);
+
 /* ---- KKT solver: gpu::GPUKKTCholeskySolver (src/cuda/kkt.cuh:38-71) ----
  * A in CSR.  P: ordering (NULL -> AMD); when `reference_symbolic` != 0 the reference ordering
  * rule is used and L is exposed in the reference CSR layout by lk_kkt_get_L_csr. */

@@ -226,6 +226,13 @@ int lkd_ipm_commit(const lkd_ipm* P, double* x, double* s, double* y, const doub
 /* x = 1, s = 1, y = 0 (init_vars_kernel, mehrotra_device.cu:93-102) */
 int lkd_ipm_init(int32_t m, int32_t n, double* x, double* y, double* s, void* stream);
 
+/* ---------------------------------------------------------------- probes (tests) */
+/* One 16x16x4 MFMA (FP64 or FP32) on 64 lanes: lane l supplies a[l], b[l]; out[l*4+e] = D[e]. */
+int lkd_probe_mfma(int prec, const void* a, const void* b, void* out, void* stream);
+/* out (64x64, column-major) = A (64 x K, ld lda, ra valid rows) * B (64 x K, ldb, rb rows)^T */
+int lkd_probe_tile_gemm(int prec, const void* a, int64_t lda, int ra, const void* b, int64_t ldb,
+                        int rb, int K, void* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

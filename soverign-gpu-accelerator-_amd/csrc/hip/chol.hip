@@ -344,9 +344,34 @@ int dispatch(int prec, int kind, int32_t ntask, const lkd_task* tasks, void* pan
                : launch_chol<double>(kind, ntask, tasks, panel, diag, plan, err, stream);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_probe_gemm(const T* pa, int64_t lda, int ra, const T* pb,
+                                                    int64_t ldb, int rb, int K, T* out) {
+    __shared__ T lds[gemm_lds_elems<T>()];
+    typename Mfma<T>::acc acc[2][2];
+    zero_acc<T>(acc);
+    tile_gemm_nt<T>(pa, lda, ra, pb, ldb, rb, K, lds, acc);
+    stage_acc<T>(acc, lds);
+    for (int idx = threadIdx.x; idx < TS * TS; idx += 256) {
+        const int r = idx & 63, c = idx >> 6;
+        out[c * TS + r] = lds[c * DP + r];
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+// out (64x64 column-major) = A(64 x K) * B(64 x K)^T with the tile GEMM of the SYRK/UPDATE kernels
+int lkd_probe_tile_gemm(int prec, const void* a, int64_t lda, int ra, const void* b, int64_t ldb,
+                        int rb, int K, void* out, void* stream) {
+    if (prec == LKD_F32)
+        LK_LAUNCH(k_probe_gemm<float>, 1, 256, 0, stream, (const float*)a, lda, ra, (const float*)b, ldb, rb, K, (float*)out);
+    else
+        LK_LAUNCH(k_probe_gemm<double>, 1, 256, 0, stream, (const double*)a, lda, ra, (const double*)b, ldb, rb, K, (double*)out);
+    return lk_last_error();
+}
+
 
 int lkd_chol_potrf(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
                    int32_t* err, void* stream) {

@@ -1,6 +1,7 @@
 // C ABI over the C++ solver classes (include/lakshya.h).  Exceptions never cross the boundary:
 // they become error codes with the message kept per thread for lk_last_error().
 #include <cstring>
+#include <map>
 #include <memory>
 #include <new>
 #include <stdexcept>
@@ -162,6 +163,9 @@ struct lk_kkt {
     double* d_rhs = nullptr;
     std::shared_ptr<gpu::VRAMArena> arena;
 };
+struct lk_plan {
+    std::map<std::string, std::vector<char>> arrays;
+};
 struct lk_ipm {
     core::Model model;
     std::unique_ptr<ipm::MehrotraSolver> solver;
@@ -246,6 +250,33 @@ int lk_supernodal(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* co
         if (nsuper_out) *nsuper_out = S.nsuper;
     });
 }
+
+int lk_plan_create(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                   const double* vals, const int32_t* P, int32_t relax_small, double relax_frac,
+                   lk_plan** out) {
+    return guarded([&] {
+        auto A = make_csr(m, n, rowptr, colidx, vals);
+        std::vector<Index> p = P ? std::vector<Index>(P, P + m) : ipm::compute_fill_reducing_ordering(A);
+        ipm::SupernodeOptions so;
+        if (relax_small >= 0) so.relax_small = relax_small;
+        if (relax_frac >= 0) so.relax_frac = relax_frac;
+        std::unique_ptr<lk_plan> h(new lk_plan);
+        h->arrays = gpu::GPUKKTCholeskySolver::export_plan(A, p, so);
+        *out = h.release();
+    });
+}
+int lk_plan_array(lk_plan* p, const char* name, void* dst, size_t capacity, size_t* bytes) {
+    return guarded([&] {
+        auto it = p->arrays.find(name);
+        if (it == p->arrays.end()) throw std::invalid_argument(std::string("no plan array ") + name);
+        *bytes = it->second.size();
+        if (dst) {
+            if (capacity < it->second.size()) throw std::invalid_argument("plan array: buffer too small");
+            std::memcpy(dst, it->second.data(), it->second.size());
+        }
+    });
+}
+int lk_plan_destroy(lk_plan* p) { return guarded([&] { delete p; }); }
 
 int lk_kkt_create(lk_arena* arena, int32_t m, int32_t n, const int32_t* rowptr,
                   const int32_t* colidx, const double* vals, const int32_t* P,

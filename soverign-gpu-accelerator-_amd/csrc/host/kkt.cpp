@@ -16,6 +16,7 @@
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
+#include <map>
 #include <string>
 #include "gpu/kkt.hpp"
 #include "lakshya_device.h"
@@ -53,7 +54,19 @@ T* dev_upload(VRAMArena& arena, const std::vector<T>& v, std::vector<void*>& own
 }  // namespace
 
 struct GPUKKTCholeskySolver::Impl {
-    VRAMArena& arena;
+    VRAMArena* arena_p;            // nullptr: host-only plan (inspection / CPU emulation)
+    std::map<std::string, std::vector<char>>* keep = nullptr;   // host copies of uploads
+
+    template <typename T>
+    T* up(const std::vector<T>& v, const char* name) {
+        if (keep) {
+            auto& dst = (*keep)[name];
+            dst.resize(v.size() * sizeof(T));
+            if (!v.empty()) std::memcpy(dst.data(), v.data(), dst.size());
+        }
+        if (!arena_p) return nullptr;
+        return dev_upload(*arena_p, v, owned);
+    }
     Index m = 0, n = 0;
     KKTOptions opt;
     KKTStats st;
@@ -99,12 +112,13 @@ struct GPUKKTCholeskySolver::Impl {
     void* fgraph[2] = {nullptr, nullptr};   // factor graph per precision
     void* sgraph[2] = {nullptr, nullptr};   // solve graph per precision (on d_z)
 
-    Impl(VRAMArena& a, const core::CSRMatrix& A, const ipm::SymbolicFactorization* sym,
-         ipm::SupernodalStructure* pre, const KKTOptions& o)
-        : arena(a), m(A.rows), n(A.cols), opt(o) {
+    Impl(VRAMArena* a, const core::CSRMatrix& A, const ipm::SymbolicFactorization* sym,
+         ipm::SupernodalStructure* pre, const KKTOptions& o,
+         std::map<std::string, std::vector<char>>* keep_host = nullptr)
+        : arena_p(a), keep(keep_host), m(A.rows), n(A.cols), opt(o) {
         auto t0 = std::chrono::steady_clock::now();
         stream = opt.stream;
-        if (!stream) {
+        if (!stream && arena_p) {
             check(lkd_stream_create(&stream), "stream create");
             own_stream = true;
         }
@@ -124,7 +138,23 @@ struct GPUKKTCholeskySolver::Impl {
         build_factor_plan();
         build_solve_plan();
         if (sym) build_csr_index(*sym);
-        allocate_numeric();
+        if (keep) {
+            up(perm, "perm");
+            up(S.sn_rowptr, "sn_rowptr");
+            up(S.sn_rows, "sn_rows");
+            up(S.sn_panel, "sn_panel");
+            up(S.col_to_sn, "col_to_sn");
+            std::vector<int32_t> f(S.sn_first.begin(), S.sn_first.end());
+            up(f, "sn_first");
+            std::vector<int64_t> fl, sl;
+            for (const auto& L : flaunch) { fl.push_back(L.kind); fl.push_back(L.begin); fl.push_back(L.count); }
+            for (const auto& L : slaunch) { sl.push_back(L.kind); sl.push_back(L.begin); sl.push_back(L.count); }
+            up(fl, "flaunch");
+            up(sl, "slaunch");
+            std::vector<int64_t> de{diag_elems};
+            up(de, "diag_elems");
+        }
+        if (arena_p) allocate_numeric();
         st.m = m;
         st.n = n;
         st.nnz_A = A.row_ptrs.empty() ? 0 : A.row_ptrs[m];
@@ -148,9 +178,9 @@ struct GPUKKTCholeskySolver::Impl {
             if (fgraph[p]) lkd_graph_destroy(fgraph[p]);
             if (sgraph[p]) lkd_graph_destroy(sgraph[p]);
         }
-        lkd_stream_sync(stream);
+        if (stream) lkd_stream_sync(stream);
         for (auto it = owned.rbegin(); it != owned.rend(); ++it) {
-            try { arena.free(*it); } catch (...) {}
+            try { arena_p->free(*it); } catch (...) {}
         }
         if (own_stream) lkd_stream_destroy(stream);
     }
@@ -184,14 +214,14 @@ struct GPUKKTCholeskySolver::Impl {
                       pval.begin() + prp[k]);
         }
         h_colptr = colptr; h_prow = prow; h_cval_n = nz;
-        d_colptr = dev_upload(arena, colptr, owned);
-        d_prow = dev_upload(arena, prow, owned);
-        d_cval = dev_upload(arena, cval, owned);
-        d_prp = dev_upload(arena, prp, owned);
-        d_pci = dev_upload(arena, pci, owned);
-        d_pval = dev_upload(arena, pval, owned);
-        d_perm = dev_upload(arena, perm, owned);
-        d_empty = dev_upload(arena, S.empty_row, owned);
+        d_colptr = up(colptr, "colptr");
+        d_prow = up(prow, "prow");
+        d_cval = up(cval, "cval");
+        d_prp = up(prp, "prp");
+        d_pci = up(pci, "pci");
+        d_pval = up(pval, "pval");
+        d_perm = up(perm, "perm");
+        d_empty = up(S.empty_row, "empty_row");
     }
     std::vector<Index> h_colptr, h_prow;
     int64_t h_cval_n = 0;
@@ -244,13 +274,13 @@ struct GPUKKTCholeskySolver::Impl {
         nseg = static_cast<int64_t>(segdst.size());
         pairs.clear();
         pairs.shrink_to_fit();
-        d_segptr = dev_upload(arena, segptr, owned);
-        d_segdst = dev_upload(arena, segdst, owned);
-        d_segi = dev_upload(arena, segi, owned);
-        d_segj = dev_upload(arena, segj, owned);
-        d_pa = dev_upload(arena, pa, owned);
-        d_pb = dev_upload(arena, pb, owned);
-        d_dseg = dev_upload(arena, dseg, owned);
+        d_segptr = up(segptr, "segptr");
+        d_segdst = up(segdst, "segdst");
+        d_segi = up(segi, "segi");
+        d_segj = up(segj, "segj");
+        d_pa = up(pa, "pa");
+        d_pb = up(pb, "pb");
+        d_dseg = up(dseg, "dseg");
     }
 
     // diag-copy offsets per supernode block column
@@ -364,7 +394,7 @@ struct GPUKKTCholeskySolver::Impl {
         }
         for (const auto& L : flaunch)
             for (int64_t q = L.begin; q < L.begin + L.count; ++q) validate(tasks[q], L.kind);
-        d_ftasks = dev_upload(arena, tasks, owned);
+        d_ftasks = up(tasks, "ftasks");
         n_ftasks = static_cast<int64_t>(tasks.size());
     }
     std::vector<std::vector<Index>> levels;
@@ -490,7 +520,7 @@ struct GPUKKTCholeskySolver::Impl {
         }
         for (const auto& L : slaunch)
             for (int64_t q = L.begin; q < L.begin + L.count; ++q) validate(tasks[q], L.kind);
-        d_stasks = dev_upload(arena, tasks, owned);
+        d_stasks = up(tasks, "stasks");
         n_stasks = static_cast<int64_t>(tasks.size());
     }
 
@@ -515,21 +545,21 @@ struct GPUKKTCholeskySolver::Impl {
             idx.push_back(panel_index(q[i], q[i]));
         }
         nnz_csr = static_cast<int64_t>(idx.size());
-        d_csr_index = dev_upload(arena, idx, owned);
+        d_csr_index = up(idx, "csr_index");
     }
 
     void allocate_numeric() {
         const int64_t pe = std::max<int64_t>(S.sn_panel[S.nsuper], 1);
-        d_panel = arena.allocate_uninitialized(static_cast<size_t>(pe) * sizeof(double));
+        d_panel = arena_p->allocate_uninitialized(static_cast<size_t>(pe) * sizeof(double));
         owned.push_back(d_panel);
-        d_diag = arena.allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(diag_elems, 1)) * sizeof(double));
+        d_diag = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(diag_elems, 1)) * sizeof(double));
         owned.push_back(d_diag);
         auto dv = [&](int64_t len) {
-            double* p = static_cast<double*>(arena.allocate(static_cast<size_t>(std::max<int64_t>(len, 1)) * sizeof(double)));
+            double* p = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(len, 1)) * sizeof(double)));
             owned.push_back(p);
             return p;
         };
-        d_err = static_cast<int32_t*>(arena.allocate(sizeof(int32_t) * 4));
+        d_err = static_cast<int32_t*>(arena_p->allocate(sizeof(int32_t) * 4));
         owned.push_back(d_err);
         d_w = dv(std::max<int64_t>(h_cval_n, 1));
         d_D = dv(m);
@@ -546,11 +576,11 @@ struct GPUKKTCholeskySolver::Impl {
         d_pcg = dv(4 * static_cast<int64_t>(m));
         if (nnz_csr > 0) d_csr = dv(nnz_csr);
         std::vector<int32_t> sn_first(S.sn_first.begin(), S.sn_first.end());
-        d_sn_first = dev_upload(arena, sn_first, owned);
-        d_sn_rowptr = dev_upload(arena, S.sn_rowptr, owned);
-        d_sn_rows = dev_upload(arena, S.sn_rows, owned);
-        d_sn_panel = dev_upload(arena, S.sn_panel, owned);
-        d_col_to_sn = dev_upload(arena, S.col_to_sn, owned);
+        d_sn_first = dev_upload(*arena_p, sn_first, owned);
+        d_sn_rowptr = dev_upload(*arena_p, S.sn_rowptr, owned);
+        d_sn_rows = dev_upload(*arena_p, S.sn_rows, owned);
+        d_sn_panel = dev_upload(*arena_p, S.sn_panel, owned);
+        d_col_to_sn = dev_upload(*arena_p, S.col_to_sn, owned);
         check(lkd_stream_sync(nullptr), "plan upload");
     }
 
@@ -801,18 +831,29 @@ GPUKKTCholeskySolver::GPUKKTCholeskySolver(VRAMArena& arena, const core::CSRMatr
     : impl_(nullptr) {
     if (M_pattern.rows != A.rows || M_pattern.cols != A.rows)
         throw std::invalid_argument("GPUKKTCholeskySolver: M_pattern dimension mismatch");
-    impl_.reset(new Impl(arena, A, &sym, nullptr, KKTOptions{}));
+    impl_.reset(new Impl(&arena, A, &sym, nullptr, KKTOptions{}));
 }
 
 GPUKKTCholeskySolver::GPUKKTCholeskySolver(VRAMArena& arena, const core::CSRMatrix& A,
                                            const ipm::SymbolicFactorization& sym,
                                            const KKTOptions& opt)
-    : impl_(new Impl(arena, A, &sym, nullptr, opt)) {}
+    : impl_(new Impl(&arena, A, &sym, nullptr, opt)) {}
 
 GPUKKTCholeskySolver::GPUKKTCholeskySolver(VRAMArena& arena, const core::CSRMatrix& A,
                                            ipm::SupernodalStructure&& structure,
                                            const KKTOptions& opt)
-    : impl_(new Impl(arena, A, nullptr, &structure, opt)) {}
+    : impl_(new Impl(&arena, A, nullptr, &structure, opt)) {}
+
+std::map<std::string, std::vector<char>> GPUKKTCholeskySolver::export_plan(
+    const core::CSRMatrix& A, const std::vector<Index>& P, const ipm::SupernodeOptions& so) {
+    std::map<std::string, std::vector<char>> out;
+    ipm::SymbolicFactorization sym;
+    ipm::compute_symbolic_factorization(A, P, sym);
+    KKTOptions ko;
+    ko.supernodes = so;
+    Impl impl(nullptr, A, &sym, nullptr, ko, &out);
+    return out;
+}
 
 std::size_t GPUKKTCholeskySolver::estimate_bytes(const core::CSRMatrix& A,
                                                  const ipm::SupernodalStructure& S) {

@@ -352,3 +352,43 @@ def verify_optimal(lp, x, pi, tol=1e-6, lb=None, ub=None, maximize=False):
                                 _f64(lp.c), _f64(lp.b), _ptr(lbp), _ptr(ubp), int(maximize),
                                 _f64(x), _f64(pi), tol, C.byref(res), C.byref(ok)))
     return bool(ok.value), res.value
+
+
+# ------------------------------------------------------------------ plan inspection (host only)
+TASK_DTYPE = np.dtype([("off", "<i8"), ("aux", "<i8"), ("aux2", "<i8"), ("ld", "<i4"),
+                       ("ncol", "<i4"), ("k0", "<i4"), ("kb", "<i4"), ("r0", "<i4"),
+                       ("c0", "<i4"), ("sn", "<i4"), ("first", "<i4")])
+PLAN_DTYPES = {"perm": np.int32, "sn_rowptr": np.int64, "sn_rows": np.int32,
+               "sn_panel": np.int64, "col_to_sn": np.int32, "sn_first": np.int32,
+               "flaunch": np.int64, "slaunch": np.int64, "diag_elems": np.int64,
+               "colptr": np.int32, "prow": np.int32, "cval": np.float64, "prp": np.int32,
+               "pci": np.int32, "pval": np.float64, "empty_row": np.uint8, "segptr": np.int64,
+               "segdst": np.int64, "segi": np.int32, "segj": np.int32, "pa": np.int32,
+               "pb": np.int32, "dseg": np.int64, "ftasks": TASK_DTYPE, "stasks": TASK_DTYPE,
+               "csr_index": np.int64}
+
+
+def plan_export(rp, ci, val, n, P=None, relax_small=-1, relax_frac=-1.0):
+    """Host-only device plan of the KKT solver (dict of numpy arrays)."""
+    L = lib()
+    L.lk_plan_create.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _vp, C.c_int32,
+                                 C.c_double, C.POINTER(_vp)]
+    L.lk_plan_array.argtypes = [_vp, C.c_char_p, _vp, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.lk_plan_destroy.argtypes = [_vp]
+    m = len(rp) - 1
+    h = _vp()
+    Pp = _i32(P) if P is not None else None
+    _ck(L.lk_plan_create(m, n, _i32(rp), _i32(ci), _f64(val), _ptr(Pp), relax_small,
+                         relax_frac, C.byref(h)))
+    out = {}
+    try:
+        for name, dt in PLAN_DTYPES.items():
+            nb = C.c_size_t(0)
+            if L.lk_plan_array(h, name.encode(), None, 0, C.byref(nb)) != 0:
+                continue
+            arr = np.zeros(nb.value // np.dtype(dt).itemsize, dtype=dt)
+            _ck(L.lk_plan_array(h, name.encode(), arr.ctypes.data, nb.value, C.byref(nb)))
+            out[name] = arr
+    finally:
+        L.lk_plan_destroy(h)
+    return out

@@ -1,0 +1,126 @@
+"""CPU emulation of the device factorization / triangular-solve schedule (numpy).
+
+Executes the exported task lists with the semantics of each kernel in
+soverign-gpu-accelerator-_amd/csrc/hip/{chol,trsv,vec}.hip, so planning errors (wrong tiles,
+levels, offsets, ancestor targets) show up on the CPU against a dense Cholesky."""
+import numpy as np
+
+TS = 64
+
+
+class Emulator:
+    def __init__(self, plan):
+        self.p = plan
+        self.fl = plan["flaunch"].reshape(-1, 3)
+        self.sl = plan["slaunch"].reshape(-1, 3)
+        self.panel = np.zeros(int(plan["sn_panel"][-1]))
+        self.diag = np.zeros(max(int(plan["diag_elems"][0]), 1))
+        self.err = 0
+
+    def blk(self, t, r0, r1, c0, c1):
+        """view of panel rows r0:r1, cols c0:c1 of the task's supernode (column-major)."""
+        ld = int(t["ld"])
+        base = int(t["off"])
+        M = self.panel[base:base + ld * int(t["ncol"])].reshape(int(t["ncol"]), ld).T
+        return M[r0:r1, c0:c1]
+
+    def assemble(self, theta, equil=False, delta=1e-9):
+        p = self.p
+        w = p["cval"] * np.repeat(theta, np.diff(p["colptr"]))
+        prod = w[p["pa"]] * p["cval"][p["pb"]]
+        seg = np.add.reduceat(prod, p["segptr"][:-1]) if len(prod) else np.zeros(0)
+        m = len(p["perm"])
+        D = np.ones(m)
+        if equil:
+            mii = np.zeros(m)
+            ok = p["dseg"] >= 0
+            mii[ok] = seg[p["dseg"][ok]]
+            D = 1.0 / np.sqrt(np.maximum(np.abs(mii), 1e-12))
+        v = seg * D[p["segi"]] * D[p["segj"]]
+        diag = (p["segi"] == p["segj"]) & (p["empty_row"][p["segi"]] == 0)
+        v[diag] += delta
+        self.panel[:] = 0
+        self.panel[p["segdst"]] = v
+        return D
+
+    def _factor_block(self, A):
+        try:
+            return np.linalg.cholesky(A)
+        except np.linalg.LinAlgError:
+            self.err |= 1
+            return None
+
+    def _diag_copy(self, L, off):
+        kb = L.shape[0]
+        Lr = np.tril(L, -1).copy()
+        Lr[np.arange(kb), np.arange(kb)] = 1.0 / np.diag(L)
+        self.diag[off:off + kb * kb] = Lr.reshape(-1)
+
+    def factor(self):
+        p = self.p
+        T = p["ftasks"]
+        for kind, b, c in self.fl:
+            for t in T[b:b + c]:
+                if self.err:
+                    return False
+                k0, kb, r0, c0, ld, nc = (int(t[x]) for x in ("k0", "kb", "r0", "c0", "ld", "ncol"))
+                if kind == 0:
+                    D = self.blk(t, k0, k0 + kb, k0, k0 + kb)
+                    L = self._factor_block(np.tril(D) + np.tril(D, -1).T)
+                    if L is None:
+                        return False
+                    D[:] = np.where(np.tri(kb, dtype=bool), L, D)
+                    self._diag_copy(L, int(t["aux"]))
+                elif kind == 1:
+                    Lr = self.diag[int(t["aux"]):int(t["aux"]) + kb * kb].reshape(kb, kb)
+                    L = np.tril(Lr, -1) + np.diag(1.0 / np.diag(Lr))
+                    X = self.blk(t, r0, min(r0 + 256, ld), k0, k0 + kb)
+                    X[:] = np.linalg.solve(L, X.T).T
+                elif kind == 2:
+                    rb = min(TS, nc - c0)
+                    Bt = self.blk(t, r0, min(r0 + TS, ld), k0, k0 + kb)
+                    Bc = self.blk(t, c0, c0 + rb, k0, k0 + kb)
+                    C = self.blk(t, r0, min(r0 + TS, ld), c0, c0 + rb)
+                    C -= Bt @ Bc.T
+                    if r0 == c0 and c0 == k0 + kb:
+                        D = self.blk(t, c0, c0 + rb, c0, c0 + rb)
+                        L = self._factor_block(np.tril(D) + np.tril(D, -1).T)
+                        if L is None:
+                            return False
+                        D[:] = np.where(np.tri(rb, dtype=bool), L, D)
+                        self._diag_copy(L, int(t["aux"]))
+                else:
+                    rows = p["sn_rows"][int(t["aux2"]):int(t["aux2"]) + ld]
+                    ra, rb = min(TS, ld - r0), min(TS, ld - c0)
+                    U = self.blk(t, r0, r0 + ra, 0, nc) @ self.blk(t, c0, c0 + rb, 0, nc).T
+                    for i in range(ra):
+                        for j in range(rb):
+                            if r0 + i < c0 + j:
+                                continue
+                            gi, gj = rows[r0 + i], rows[c0 + j]
+                            s = p["col_to_sn"][gj]
+                            f = p["sn_first"][s]
+                            rp0, rp1 = p["sn_rowptr"][s], p["sn_rowptr"][s + 1]
+                            trows = p["sn_rows"][rp0:rp1]
+                            pos = np.searchsorted(trows, gi)
+                            if pos >= len(trows) or trows[pos] != gi:
+                                self.err |= 2
+                                return False
+                            self.panel[p["sn_panel"][s] + (gj - f) * (rp1 - rp0) + pos] -= U[i, j]
+        return True
+
+    def dense_L(self):
+        """L in the plan's (postordered) permutation as a dense lower triangle."""
+        p = self.p
+        m = len(p["perm"])
+        L = np.zeros((m, m))
+        for s in range(len(p["sn_first"]) - 1):
+            f, l = p["sn_first"][s], p["sn_first"][s + 1]
+            rows = p["sn_rows"][p["sn_rowptr"][s]:p["sn_rowptr"][s + 1]]
+            ld = len(rows)
+            blk = self.panel[p["sn_panel"][s]:p["sn_panel"][s] + ld * (l - f)].reshape(l - f, ld).T
+            for c in range(l - f):
+                for r in range(ld):
+                    if rows[r] >= f + c:
+                        L[rows[r], f + c] = blk[r, c]
+        return L
