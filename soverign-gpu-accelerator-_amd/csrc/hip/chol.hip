@@ -71,7 +71,7 @@ __device__ __forceinline__ void stage_acc(typename Mfma<T>::acc (&acc)[2][2], T*
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int r = wr + 16 * x + 4 * (lane >> 4) + e;
+                const int r = wr + 16 * x + Mfma<T>::row(lane, e);
                 const int c = wc + 16 * y + (lane & 15);
                 Cs[c * DP + r] = acc[x][y][e];
             }

@@ -15,21 +15,25 @@ static inline int lk_last_error() { return (int)hipGetLastError(); }
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// One 16x16 output block, K = 4 per instruction.  Operand layout (CDNA3/CDNA4):
-//   A (16x4): lane l holds A[l % 16][l / 16];  B (4x16): lane l holds B[l / 16][l % 16];
-//   D (16x16): lane l, element e holds D[4 * (l / 16) + e][l % 16].
+// One 16x16 output block, K = 4 per instruction.  Operand layout on gfx950 (both types):
+//   A (16x4): lane l holds A[l % 16][l / 16];  B (4x16): lane l holds B[l / 16][l % 16].
+// Result element e of lane l (measured on the device by tests/test_gpu_probe.py):
+//   FP64 v_mfma_f64_16x16x4_f64: D[l / 16 + 4 e][l % 16]
+//   FP32 v_mfma_f32_16x16x4_f32: D[4 (l / 16) + e][l % 16]
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
     using acc = v4d;
     static __device__ __forceinline__ acc mma(double a, double b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int e) { return (lane >> 4) + 4 * e; }
 };
 template <> struct Mfma<float> {
     using acc = v4f;
     static __device__ __forceinline__ acc mma(float a, float b, acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int e) { return 4 * (lane >> 4) + e; }
 };
 
 __device__ __forceinline__ double lk_shfl(double v, int lane) {

@@ -392,3 +392,58 @@ def plan_export(rp, ci, val, n, P=None, relax_small=-1, relax_frac=-1.0):
     finally:
         L.lk_plan_destroy(h)
     return out
+
+
+# ------------------------------------------------------------------ raw device buffers (tests)
+class DeviceBuffer:
+    """Device memory from the library's own allocator (lkd_malloc), with host copies."""
+
+    def __init__(self, arr_or_nbytes, dtype=None):
+        L = lib()
+        for name, args in (("lkd_malloc", [C.POINTER(_vp), C.c_size_t]), ("lkd_free", [_vp]),
+                           ("lkd_memcpy_htod", [_vp, _vp, C.c_size_t, _vp]),
+                           ("lkd_memcpy_dtoh", [_vp, _vp, C.c_size_t, _vp]),
+                           ("lkd_stream_sync", [_vp])):
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = C.c_int
+        if isinstance(arr_or_nbytes, np.ndarray):
+            a = np.ascontiguousarray(arr_or_nbytes)
+            self.nbytes, self.dtype, self.shape = a.nbytes, a.dtype, a.shape
+        else:
+            a = None
+            self.nbytes, self.dtype, self.shape = int(arr_or_nbytes), np.dtype(dtype or np.uint8), None
+        self.ptr = _vp()
+        rc = L.lkd_malloc(C.byref(self.ptr), max(self.nbytes, 1))
+        if rc:
+            raise RuntimeError(f"lkd_malloc failed ({rc})")
+        if a is not None:
+            self.upload(a)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if lib().lkd_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, None) or lib().lkd_stream_sync(None):
+            raise RuntimeError("upload failed")
+
+    def download(self, dtype=None, count=None):
+        dt = np.dtype(dtype or self.dtype)
+        n = self.nbytes // dt.itemsize if count is None else count
+        out = np.zeros(n, dt)
+        if lib().lkd_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes, None):
+            raise RuntimeError("download failed")
+        return out
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr:
+            lib().lkd_free(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
