@@ -19,8 +19,10 @@
  * IPM iteration (the reference recomputes the identical factor in every solve call), and nothing
  * is printed.  Relative-gap mode (stop_rule 1, an extension with no reference counterpart) uses
  * PCG instead of plain refinement, accepts a solve at the backward-error level, and skips the
- * e1 re-check of the KKT residual: the reference's absolute 1e-9 acceptance cannot be met near
- * a 1e-8 gap (its own loop stops at 1e-6).  The device path implements the same rule.  Sums use the reference's term order; multiply-adds use fma() so the assembled
+ * e1 re-check of the KKT residual, and takes the longest finite, positive centred trial when both
+ * merit searches fail (non-monotone fallback): the reference's absolute 1e-9 acceptance and
+ * monotone merit search stall before a 1e-8 gap (its own loop stops at 1e-6).  The device path
+ * implements the same rules.  Sums use the reference's term order; multiply-adds use fma() so the assembled
  * M0 is bit-identical to an fma-based device kernel with the same order.
  *
  * Parity pins (tests/test_oracle.py): the symbolic KATs of tests/unit/test_ipm_symbolic.cpp,
@@ -600,6 +602,7 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                 ap = ad = fmin(1.0, 0.995 * bnd);
             }
             int maxbt = dir == 0 ? 15 : 30;
+            int fallback_bt = -1;   /* relative mode: first finite, positive centred trial */
             for (int bt = 0; bt < maxbt; ++bt) {
                 for (int j = 0; j < n; ++j) { bx[j] = x[j] + ap * dx[j]; bs[j] = s[j] + ad * ds[j]; }
                 for (int i = 0; i < m; ++i) by[i] = y[i] + ad * dy[i];
@@ -614,8 +617,17 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                 int pos = mx > 0.0 && ms > 0.0 && mu2 > 0.0 && st > 0.0;
                 int prog = merit < merit0 && merit <= (1.0 - 1e-4 * st) * merit0;
                 if (finite && pos && prog) { accepted = 1; break; }
+                if (relmode && dir == 1 && finite && pos && fallback_bt < 0) fallback_bt = bt;
                 ap *= 0.5;
                 ad *= 0.5;
+            }
+            if (!accepted && relmode && dir == 1 && fallback_bt >= 0) {
+                /* non-monotone fallback: take the longest admissible centred step */
+                double a2 = fmin(1.0, 0.995 * fmin(step_len(n, x, dx), step_len(n, s, ds)));
+                for (int t = 0; t < fallback_bt; ++t) a2 *= 0.5;
+                for (int j = 0; j < n; ++j) { bx[j] = x[j] + a2 * dx[j]; bs[j] = s[j] + a2 * ds[j]; }
+                for (int i = 0; i < m; ++i) by[i] = y[i] + a2 * dy[i];
+                accepted = 1;
             }
         }
         if (!accepted) { res->status = 3; res->iterations = iter; res->fail_code = 5; break; }

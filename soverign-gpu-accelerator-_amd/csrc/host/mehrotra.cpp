@@ -13,6 +13,9 @@
 //    A(x + a dx) - b and A^T(y + a dy) + s + a ds - c per trial;
 //  * Ruiz scaling, the fill-reducing ordering and the symbolic phase run once in the constructor
 //    as in the reference, with AMD instead of the O(m^2) explicit minimum degree by default.
+// Relative-gap mode (extension; the reference's loop stops at an absolute 1e-6): KKT solves use
+// PCG with a backward-error acceptance, the e1 re-check of the KKT residual is dropped, and when
+// both merit searches fail the longest finite, positive centred trial is taken (non-monotone).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -380,6 +383,7 @@ public:
                 break;
             }
             const bool corrector_valid = !(d.e1 > 1e-8 || d.e2 > 1e-8 || d.e3 > 1e-8);
+            const bool rel = opt_.stop_rule == IpmOptions::StopRule::RelativeGap;
             double ap = std::min(1.0, 0.995 * d.apmax), ad = std::min(1.0, 0.995 * d.admax);
             const double merit0 = nr.rp / rp0 + nr.rd / rd0 + nr.mu / mu0;
             bool accepted = false;
@@ -394,6 +398,7 @@ public:
                 check(lkd_ipm_trials(&P_, x_, s_, y_, dx_, ds_, dy_, rp_, rd_, qp_, qd_, ap, ad, ntr, stream_), "trials");
                 read_scalars(LKS_TRIAL_BASE + LKS_TRIAL_STRIDE * ntr);
                 double a = ap, b = ad;
+                double fa = -1.0, fb = -1.0;   // relative mode: first finite, positive centred trial
                 for (int t = 0; t < ntr; ++t) {
                     const double* T = &hS_[LKS_TRIAL_BASE + t * LKS_TRIAL_STRIDE];
                     const double nrp2 = std::sqrt(T[0]), nrd2 = std::sqrt(T[1]), mu2 = T[2] / n_;
@@ -411,8 +416,14 @@ public:
                         accepted = true;
                         break;
                     }
+                    if (rel && dir == 1 && finite && positive && fa < 0) { fa = a; fb = b; }
                     a *= 0.5;
                     b *= 0.5;
+                }
+                if (!accepted && rel && dir == 1 && fa > 0) {
+                    // relative-gap mode: non-monotone fallback to the longest admissible centred step
+                    check(lkd_ipm_commit(&P_, x_, s_, y_, dx_, ds_, dy_, fa, fb, stream_), "commit");
+                    accepted = true;
                 }
             }
             if (!accepted) {
