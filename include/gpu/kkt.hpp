@@ -100,6 +100,10 @@ public:
     void triangular_solve_permuted(Float* d_x);
 
     const KKTStats& stats() const;
+    // One factorization of the last Theta with events around every launch: per kind (0 POTRF,
+    // 1 TRSM, 2 SYRK, 3 UPDATE) device ms, algorithmic flops and launch count; index 4 = the
+    // whole factorization including assembly (flops = sum_j c_j^2 of the exact factor).
+    void profile_factor(FactorPrecision prec, double ms[5], double flops[5], double launches[5]);
     const std::vector<Index>& permutation() const;     // ordering used (postordered)
     void* stream() const;
     Index rows() const;

@@ -78,6 +78,10 @@ public:
     void set_warm_start(const std::vector<Float>& x, const std::vector<Float>& y,
                         const std::vector<Float>& s);
     void clear_warm_start();
+    // Extension: factorization profile at the last iterate (see GPUKKTCholeskySolver) and the
+    // symbolic statistics of the normal equations.
+    void profile_factor(bool fp32, double ms[5], double flops[5], double launches[5]);
+    void kkt_statistics(double out[12]);
 
     class Impl;
 

@@ -137,6 +137,12 @@ int lk_ipm_run(lk_ipm* h, const double* c, const double* b, const double* warm_x
                const double* warm_y, const double* warm_s, lk_ipm_result* res, double* x,
                double* y, double* s);
 int lk_ipm_destroy(lk_ipm* h);
+/* Factorization profile at the last iterate (per launch kind 0 POTRF, 1 TRSM, 2 SYRK,
+ * 3 UPDATE; 4 = whole factorization): device ms, algorithmic flops, launches (5 each). */
+int lk_ipm_profile_factor(lk_ipm* h, int32_t fp32, double* ms, double* flops, double* launches);
+/* m, n, nnz(A), supernodes, nnz(L) stored, nnz(L) exact, flops exact, flops stored, levels,
+ * factor launches, solve launches, symbolic ms */
+int lk_ipm_kkt_stats(lk_ipm* h, double* out12);
 
 /* ---- verifier::verify_optimal (src/verifier/certificate.hpp:26-31) with a tolerance ---- */
 int lk_verify_optimal(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,

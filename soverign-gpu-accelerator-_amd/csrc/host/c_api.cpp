@@ -461,6 +461,12 @@ int lk_ipm_run(lk_ipm* h, const double* c, const double* b, const double* warm_x
 }
 
 int lk_ipm_destroy(lk_ipm* h) { return guarded([&] { delete h; }); }
+int lk_ipm_profile_factor(lk_ipm* h, int32_t fp32, double* ms, double* flops, double* launches) {
+    return guarded([&] { h->solver->profile_factor(fp32 != 0, ms, flops, launches); });
+}
+int lk_ipm_kkt_stats(lk_ipm* h, double* out12) {
+    return guarded([&] { h->solver->kkt_statistics(out12); });
+}
 
 int lk_verify_optimal(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
                       const double* vals, const double* c, const double* b, const double* lb,

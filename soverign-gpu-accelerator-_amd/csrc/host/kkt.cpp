@@ -395,6 +395,7 @@ struct GPUKKTCholeskySolver::Impl {
         for (const auto& L : flaunch)
             for (int64_t q = L.begin; q < L.begin + L.count; ++q) validate(tasks[q], L.kind);
         d_ftasks = up(tasks, "ftasks");
+        h_ftasks = tasks;
         n_ftasks = static_cast<int64_t>(tasks.size());
     }
     std::vector<std::vector<Index>> levels;
@@ -608,6 +609,72 @@ struct GPUKKTCholeskySolver::Impl {
                 default: e = lkd_chol_update(prec, L.count, t, d_panel, pl, d_err, stream); break;
             }
             check(e, "factor kernel");
+        }
+    }
+
+    // One factorization of the current M with HIP events around every launch: per launch kind
+    // (0 POTRF, 1 TRSM, 2 SYRK, 3 UPDATE) the summed device time, the launch count and the
+    // algorithmic flops of the tiles; slot 4 = whole factorization incl. assembly.
+    void profile_factor(int prec, double ms[5], double flops[5], double launches[5]) {
+        for (int k = 0; k < 5; ++k) ms[k] = flops[k] = launches[k] = 0;
+        if (!have_theta) throw std::logic_error("profile_factor: no Theta");
+        const lkd_snplan pl = plan();
+        void *e0, *e1, *a0, *a1;
+        check(lkd_event_create(&e0), "event");
+        check(lkd_event_create(&e1), "event");
+        check(lkd_event_create(&a0), "event");
+        check(lkd_event_create(&a1), "event");
+        const size_t esz = prec == LKD_F32 ? sizeof(float) : sizeof(double);
+        check(lkd_event_record(a0, stream), "event");
+        check(lkd_assemble_prep(n, d_colptr, d_cval, d_theta, d_w, stream), "prep");
+        check(lkd_assemble_diag(m, d_dseg, d_segptr, d_pa, d_pb, d_w, d_cval, 1, d_D, d_m0d, stream), "diag");
+        check(lkd_memset(d_panel, 0, static_cast<size_t>(S.sn_panel[S.nsuper]) * esz, stream), "zero");
+        check(lkd_memset(d_err, 0, sizeof(int32_t), stream), "zero");
+        check(lkd_assemble_pairs(prec, nseg, d_segptr, d_segdst, d_segi, d_segj, d_pa, d_pb, d_w,
+                                 d_cval, d_D, d_empty, 1e-9, d_panel, stream), "assemble");
+        for (const auto& L : flaunch) {
+            const lkd_task* t = d_ftasks + L.begin;
+            check(lkd_event_record(e0, stream), "event");
+            int e = 0;
+            switch (L.kind) {
+                case 0: e = lkd_chol_potrf(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
+                case 1: e = lkd_chol_trsm(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
+                case 2: e = lkd_chol_syrk(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
+                default: e = lkd_chol_update(prec, L.count, t, d_panel, pl, d_err, stream); break;
+            }
+            check(e, "factor kernel");
+            check(lkd_event_record(e1, stream), "event");
+            float t_ms = 0;
+            check(lkd_event_elapsed_ms(&t_ms, e0, e1), "elapsed");
+            ms[L.kind] += t_ms;
+            launches[L.kind] += 1;
+            for (int64_t q = L.begin; q < L.begin + L.count; ++q) flops[L.kind] += task_flops(h_ftasks[q], L.kind);
+        }
+        check(lkd_event_record(a1, stream), "event");
+        float tot = 0;
+        check(lkd_event_elapsed_ms(&tot, a0, a1), "elapsed");
+        ms[4] = tot;
+        flops[4] = S.flops_exact;
+        launches[4] = static_cast<double>(flaunch.size());
+        lkd_event_destroy(e0); lkd_event_destroy(e1); lkd_event_destroy(a0); lkd_event_destroy(a1);
+        have_eq = false;   // the profiled factor uses delta 1e-9 only; force a normal refactor next
+    }
+    std::vector<lkd_task> h_ftasks;
+    static double task_flops(const lkd_task& t, int kind) {
+        const double kb = t.kb;
+        switch (kind) {
+            case 0: return kb * kb * kb / 3.0;
+            case 1: return static_cast<double>(std::min<int64_t>(TRSM_ROWS, t.ld - t.r0)) * kb * kb;
+            case 2: {
+                const double ra = std::min(TILE, t.ld - t.r0), rb = std::min(TILE, t.ncol - t.c0);
+                const double useful = (t.r0 == t.c0) ? rb * (rb + 1) / 2 + (ra - rb) * rb : ra * rb;
+                return 2.0 * useful * kb;
+            }
+            default: {
+                const double ra = std::min(TILE, t.ld - t.r0), rb = std::min(TILE, t.ld - t.c0);
+                const double useful = (t.r0 == t.c0) ? rb * (rb + 1) / 2 + (ra - rb) * rb : ra * rb;
+                return 2.0 * useful * t.ncol;
+            }
         }
     }
 
@@ -952,6 +1019,10 @@ void GPUKKTCholeskySolver::triangular_solve_permuted(Float* d_x) {
 }
 
 const KKTStats& GPUKKTCholeskySolver::stats() const { return impl_->st; }
+void GPUKKTCholeskySolver::profile_factor(FactorPrecision prec, double ms[5], double flops[5],
+                                          double launches[5]) {
+    impl_->profile_factor(prec == FactorPrecision::FP32 ? LKD_F32 : LKD_F64, ms, flops, launches);
+}
 const std::vector<Index>& GPUKKTCholeskySolver::permutation() const { return impl_->perm; }
 void* GPUKKTCholeskySolver::stream() const { return impl_->stream; }
 Index GPUKKTCholeskySolver::rows() const { return impl_->m; }

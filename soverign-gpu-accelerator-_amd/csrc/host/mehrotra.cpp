@@ -492,6 +492,19 @@ void MehrotraSolver::set_warm_start(const std::vector<Float>& x, const std::vect
     impl_->set_warm(x, y, s);
 }
 void MehrotraSolver::clear_warm_start() { impl_->clear_warm(); }
+void MehrotraSolver::profile_factor(bool fp32, double ms[5], double flops[5], double launches[5]) {
+    impl_->kkt_->profile_factor(fp32 ? gpu::FactorPrecision::FP32 : gpu::FactorPrecision::FP64,
+                                ms, flops, launches);
+    check(lkd_stream_sync(impl_->stream_), "sync");
+}
+void MehrotraSolver::kkt_statistics(double out[12]) {
+    const auto& st = impl_->kkt_->stats();
+    const double v[12] = {double(st.m), double(st.n), double(st.nnz_A), double(st.nsuper),
+                          double(st.nnz_L), double(st.nnz_L_exact), st.flops_exact, st.flops_panel,
+                          double(st.levels), double(st.launches_factor), double(st.launches_solve),
+                          st.symbolic_ms};
+    for (int i = 0; i < 12; ++i) out[i] = v[i];
+}
 
 }  // namespace ipm
 }  // namespace sankhya

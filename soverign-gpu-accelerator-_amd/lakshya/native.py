@@ -331,6 +331,24 @@ class IPM:
                              C.byref(res), x.ctypes.data, y.ctypes.data, s.ctypes.data))
         return _result(res, x, y, s)
 
+    def profile_factor(self, fp32=False):
+        L = lib()
+        L.lk_ipm_profile_factor.argtypes = [_vp, C.c_int32, _f64p, _f64p, _f64p]
+        ms, fl, la = np.zeros(5), np.zeros(5), np.zeros(5)
+        _ck(L.lk_ipm_profile_factor(self.h, int(fp32), ms, fl, la))
+        names = ["potrf", "trsm", "syrk", "update", "factorization"]
+        return {k: {"ms": float(ms[i]), "flops": float(fl[i]), "launches": int(la[i])}
+                for i, k in enumerate(names)}
+
+    def kkt_stats(self):
+        L = lib()
+        L.lk_ipm_kkt_stats.argtypes = [_vp, _f64p]
+        v = np.zeros(12)
+        _ck(L.lk_ipm_kkt_stats(self.h, v))
+        keys = ["m", "n", "nnz_A", "nsuper", "nnz_L", "nnz_L_exact", "flops_exact",
+                "flops_panel", "levels", "launches_factor", "launches_solve", "symbolic_ms"]
+        return dict(zip(keys, v.tolist()))
+
     def close(self):
         if self.h:
             _ck(lib().lk_ipm_destroy(self.h))
