@@ -309,7 +309,8 @@ static int kkt_pcg(or_kkt* k, const double* rhs, double* dy, double tol, int max
     m0_residual(k, dy, rhs, r);
     double rbest = inf_norm(m, r);
     memcpy(best, dy, sizeof(double) * m);
-    int ok = rbest <= tol || backward_ok(k, rhs, dy, rbest);
+    int ok = rbest <= tol;
+    int stall = 0;      /* true-residual checks without a 2x improvement of the best */
     double rz = 0.0;
     if (!ok) {
         for (int i = 0; i < m; ++i) z[i] = r[i] * k->D[i];
@@ -328,11 +329,13 @@ static int kkt_pcg(or_kkt* k, const double* rhs, double* dy, double tol, int max
         double alpha = rz / pq;
         for (int i = 0; i < m; ++i) { dy[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
         double rr = inf_norm(m, r);
-        if (rr <= tol || backward_ok(k, rhs, dy, rr) || it == max_it - 1 || (it + 1) % 10 == 0) {
+        if (rr <= tol || it == max_it - 1 || (it + 1) % 5 == 0) {
             m0_residual(k, dy, rhs, r);          /* true residual */
             rr = inf_norm(m, r);
+            if (rr < 0.5 * rbest) stall = 0; else stall++;
             if (rr < rbest) { rbest = rr; memcpy(best, dy, sizeof(double) * m); }
-            if (rr <= tol || backward_ok(k, rhs, dy, rr)) { ok = 1; break; }
+            if (rr <= tol) { ok = 1; break; }
+            if (stall >= 2) break;              /* stagnated: accept at the backward level */
         }
         for (int i = 0; i < m; ++i) z[i] = r[i] * k->D[i];
         chol_solve(m, k->Leq, z);
@@ -342,7 +345,10 @@ static int kkt_pcg(or_kkt* k, const double* rhs, double* dy, double tol, int max
         rz = rz2;
         for (int i = 0; i < m; ++i) p[i] = z[i] + beta * p[i];
     }
-    if (!ok) memcpy(dy, best, sizeof(double) * m);
+    if (!ok) {
+        memcpy(dy, best, sizeof(double) * m);
+        ok = backward_ok(k, rhs, dy, rbest);
+    }
     free(r); free(z); free(p); free(q); free(best);
     return ok;
 }

@@ -799,8 +799,10 @@ struct GPUKKTCholeskySolver::Impl {
         residual();
         read_ks();
         double best = hk[0];
-        bool ok = best <= tol || (backward && backward_ok(best));
-        int its = 0;
+        double bw_hk[LKD_KS_SLOTS];
+        std::copy(hk, hk + LKD_KS_SLOTS, bw_hk);   // norms at the best iterate
+        bool ok = best <= tol;
+        int its = 0, stall = 0;
         if (!ok) {
             check(lkd_memcpy_dtod(d_best, d_dy, vb, stream), "copy");
             check(lkd_memcpy_dtod(d_z, d_req, vb, stream), "copy");
@@ -816,16 +818,18 @@ struct GPUKKTCholeskySolver::Impl {
                 read_ks();
                 if (!(hk[6] > 0.0)) break;
                 const double rr = hk[4];
-                const double bw = 1e-12 * (hk[1] + hk[3] * hk[2]);
-                if (rr <= tol || (backward && rr <= bw) || it == max_it - 1 || (it + 1) % 10 == 0) {
+                if (rr <= tol || it == max_it - 1 || (it + 1) % 5 == 0) {
                     residual();
                     read_ks();
                     const double rt = hk[0];
+                    if (rt < 0.5 * best) stall = 0; else ++stall;
                     if (rt < best) {
                         best = rt;
+                        std::copy(hk, hk + LKD_KS_SLOTS, bw_hk);
                         check(lkd_memcpy_dtod(d_best, d_dy, vb, stream), "copy");
                     }
-                    if (rt <= tol || (backward && backward_ok(rt))) { ok = true; break; }
+                    if (rt <= tol) { ok = true; break; }
+                    if (stall >= 2) break;     // stagnated: accept at the backward level
                 }
                 check(lkd_memcpy_dtod(d_z, d_req, vb, stream), "copy");
                 trsv(prec, d_z);
@@ -834,6 +838,10 @@ struct GPUKKTCholeskySolver::Impl {
                 std::swap(sa, sb);
             }
             if (!ok) check(lkd_memcpy_dtod(d_dy, d_best, vb, stream), "copy");
+        }
+        if (!ok && backward) {
+            std::copy(bw_hk, bw_hk + LKD_KS_SLOTS, hk);
+            ok = backward_ok(best);
         }
         st.refinement_steps += its;
         check(lkd_kkt_scatter(m, d_perm, d_dy, d_rhs, stream), "scatter");

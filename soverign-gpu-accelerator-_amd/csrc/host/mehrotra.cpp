@@ -414,6 +414,8 @@ public:
                     if (finite && positive && progress) {
                         check(lkd_ipm_commit(&P_, x_, s_, y_, dx_, ds_, dy_, a, b, stream_), "commit");
                         accepted = true;
+                        if (opt_.verbose)
+                            std::printf("    step: dir %d trial %d ap %.3e ad %.3e sigma %.3e pcg %d\n", dir, t, a, b, sigma, res.refinement_steps);
                         break;
                     }
                     if (rel && dir == 1 && finite && positive && fa < 0) { fa = a; fb = b; }
@@ -424,6 +426,7 @@ public:
                     // relative-gap mode: non-monotone fallback to the longest admissible centred step
                     check(lkd_ipm_commit(&P_, x_, s_, y_, dx_, ds_, dy_, fa, fb, stream_), "commit");
                     accepted = true;
+                    if (opt_.verbose) std::printf("    step: centred fallback a %.3e\n", fa);
                 }
             }
             if (!accepted) {

@@ -124,3 +124,50 @@ class Emulator:
                     if rows[r] >= f + c:
                         L[rows[r], f + c] = blk[r, c]
         return L
+
+    # ---------------------------------------------------------------- triangular solves
+    def _diag_L(self, off, kb):
+        Lr = self.diag[off:off + kb * kb].reshape(kb, kb)
+        return np.tril(Lr, -1) + np.diag(1.0 / np.diag(Lr))
+
+    def solve(self, x):
+        """In-place L L^T solve of the permuted vector x with the solve schedule."""
+        p = self.p
+        T = p["stasks"]
+        for kind, b, c in self.sl:
+            for t in T[b:b + c]:
+                k0, kb, r0, c0, ld, nc = (int(t[q]) for q in ("k0", "kb", "r0", "c0", "ld", "ncol"))
+                first = int(t["first"])
+                rows = p["sn_rows"][int(t["aux2"]):int(t["aux2"]) + ld]
+                if kind == 10:
+                    L = self._diag_L(int(t["aux"]), kb)
+                    x[first + k0:first + k0 + kb] = np.linalg.solve(L, x[first + k0:first + k0 + kb])
+                elif kind == 11:
+                    z = x[first + k0:first + k0 + kb].copy()
+                    r1 = min(r0 + 256, ld)
+                    acc = self.blk(t, r0, r1, k0, k0 + kb) @ z
+                    for i, row in enumerate(range(r0, r1)):
+                        if row < nc:
+                            x[first + row] -= acc[i]
+                        else:
+                            x[rows[row]] -= acc[i]
+                    if r0 == k0 + kb and r0 < nc:
+                        kb2 = min(TS, nc - r0)
+                        L = self._diag_L(int(t["aux"]), kb2)
+                        x[first + r0:first + r0 + kb2] = np.linalg.solve(L, x[first + r0:first + r0 + kb2])
+                else:
+                    kbc = min(TS, nc - c0)
+                    if kind == 12:
+                        rr = np.arange(nc, ld)
+                        xv = x[rows[nc:ld]]
+                        last = c0 + kbc == nc
+                    else:
+                        rr = np.arange(k0, k0 + kb)
+                        xv = x[first + k0:first + k0 + kb]
+                        last = c0 + kbc == k0
+                    if len(rr):
+                        x[first + c0:first + c0 + kbc] -= self.blk(t, rr[0], rr[-1] + 1, c0, c0 + kbc).T @ xv
+                    if last:
+                        L = self._diag_L(int(t["aux"]), kbc)
+                        x[first + c0:first + c0 + kbc] = np.linalg.solve(L.T, x[first + c0:first + c0 + kbc])
+        return x
