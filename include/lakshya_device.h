@@ -174,6 +174,7 @@ enum {
     LKS_RP2 = 0, LKS_RD2, LKS_XS, LKS_CX, LKS_BY, LKS_RPINF, LKS_RDINF,       /* residuals */
     LKS_R1, LKS_R2, LKS_R3, LKS_RXS2, LKS_APMAX, LKS_ADMAX, LKS_DX2, LKS_DS2, /* direction */
     LKS_MUAFF, LKS_SIGMAMU, LKS_AP, LKS_AD, LKS_DY2, LKS_KKTR,
+    LKS_XSMAX,                                          /* max_j |x_j (s_j - rd_j)| */
     LKS_NSLOTS
 };
 #define LKD_MAX_TRIALS 32

@@ -564,8 +564,17 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
             double gap = fabs(pobj - dobj) / fmax(1.0, fmax(fabs(pobj), fabs(dobj)));
             double rpi = 0, rdi = 0, bi = 0, cinf = 0;
             for (int i = 0; i < m; ++i) { rpi = fmax(rpi, fabs(rpv[i] / R[i])); bi = fmax(bi, fabs(b_in[i])); }
-            for (int j = 0; j < n; ++j) { rdi = fmax(rdi, fabs(rd[j] / C[j])); cinf = fmax(cinf, fabs(c_in[j])); }
-            stop = gap <= opt->gap_tol && rpi / (1 + bi) <= opt->gap_tol && rdi / (1 + cinf) <= opt->gap_tol;
+            double xsmax = 0;
+            for (int j = 0; j < n; ++j) {
+                rdi = fmax(rdi, fabs(rd[j] / C[j])); cinf = fmax(cinf, fabs(c_in[j]));
+                xsmax = fmax(xsmax, fabs(x[j] * (s[j] - rd[j])));   /* x_j (c - A^T y)_j */
+            }
+            /* absolute targets as well (the FP64 certificate checks |Ax-b| and the reduced costs
+             * absolutely), floored where FP64 cannot resolve them */
+            stop = gap <= opt->gap_tol && rpi / (1 + bi) <= opt->gap_tol && rdi / (1 + cinf) <= opt->gap_tol &&
+                   rpi <= fmax(0.5 * opt->gap_tol, 1e-12 * (1 + bi)) &&
+                   rdi <= fmax(0.5 * opt->gap_tol, 1e-12 * (1 + cinf)) &&
+                   xsmax <= fmax(0.5 * opt->gap_tol, 1e-14 * fmax(1.0, fmax(fabs(pobj), fabs(dobj))));
         }
         if (stop) { res->iterations = iter; res->status = 0; done = 1; break; }
 

@@ -343,7 +343,7 @@ __global__ __launch_bounds__(NT) void k_resid_cols(lkd_ipm P, const double* __re
     const int lane = threadIdx.x & (G - 1);
     const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
     const int64_t ng = (int64_t)gridDim.x * NT / G;
-    double s2 = 0.0, xs = 0.0, cx = 0.0, inf = 0.0;
+    double s2 = 0.0, xs = 0.0, cx = 0.0, inf = 0.0, xsmax = 0.0;
     for (int64_t j = g0; j < P.n; j += ng) {
         const double aty = row_dot<G>(P.trp, P.tci, P.tval, y, j, lane);
         if (lane == 0) {
@@ -354,6 +354,7 @@ __global__ __launch_bounds__(NT) void k_resid_cols(lkd_ipm P, const double* __re
             xs += x[j] * s[j];
             cx += P.c[j] * x[j];
             inf = fmax(inf, fabs(v / P.C[j]));
+            xsmax = fmax(xsmax, fabs(x[j] * (s[j] - v)));   // x_j (c - A^T y)_j, scale-free
         }
     }
     const int nb = gridDim.x;
@@ -363,6 +364,7 @@ __global__ __launch_bounds__(NT) void k_resid_cols(lkd_ipm P, const double* __re
     r = block_reduce(xs, OP_SUM, sh); if (threadIdx.x == 0) part[1 * nb + blockIdx.x] = r;
     r = block_reduce(cx, OP_SUM, sh); if (threadIdx.x == 0) part[2 * nb + blockIdx.x] = r;
     r = block_reduce(inf, OP_MAX, sh); if (threadIdx.x == 0) part[3 * nb + blockIdx.x] = r;
+    r = block_reduce(xsmax, OP_MAX, sh); if (threadIdx.x == 0) part[4 * nb + blockIdx.x] = r;
 }
 
 __global__ __launch_bounds__(NT) void k_rhs_cols(lkd_ipm P, const double* __restrict__ x,
@@ -728,8 +730,9 @@ int lkd_ipm_residuals(const lkd_ipm* P, const double* x, const double* y, const 
     // column pass writes its partials after the three row quantities
     const int qoff = (3 * P->nblk_rows + P->nblk_cols - 1) / P->nblk_cols;
     LK_GSWITCH(P->grp_cols, l_resid_cols, P, x, y, s, rd, theta, qoff, stream);
-    int ops_c[4] = {OP_SUM, OP_SUM, OP_SUM, OP_MAX}, slots_c[4] = {LKS_RD2, LKS_XS, LKS_CX, LKS_RDINF};
-    return finalize(P->part + (int64_t)qoff * P->nblk_cols, P->nblk_cols, 4, ops_c, slots_c, P->S, stream);
+    int ops_c[5] = {OP_SUM, OP_SUM, OP_SUM, OP_MAX, OP_MAX};
+    int slots_c[5] = {LKS_RD2, LKS_XS, LKS_CX, LKS_RDINF, LKS_XSMAX};
+    return finalize(P->part + (int64_t)qoff * P->nblk_cols, P->nblk_cols, 5, ops_c, slots_c, P->S, stream);
 }
 
 int lkd_ipm_rhs(const lkd_ipm* P, const double* x, const double* s, const double* theta,

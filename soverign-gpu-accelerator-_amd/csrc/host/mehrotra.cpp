@@ -200,7 +200,7 @@ public:
         check(lkd_memcpy_dtoh(hS_.data(), S_, sizeof(double) * count, stream_), "scalars");
     }
 
-    struct Norms { double rp, rd, mu, cx, by, rpinf, rdinf; };
+    struct Norms { double rp, rd, mu, cx, by, rpinf, rdinf, xsmax; };
 
     Norms residuals() {
         check(lkd_ipm_residuals(&P_, x_, y_, s_, rp_, rd_, th_, stream_), "residuals");
@@ -213,6 +213,7 @@ public:
         r.by = hS_[LKS_BY];
         r.rpinf = hS_[LKS_RPINF];
         r.rdinf = hS_[LKS_RDINF];
+        r.xsmax = hS_[LKS_XSMAX];
         return r;
     }
 
@@ -305,8 +306,13 @@ public:
         if (opt_.stop_rule == IpmOptions::StopRule::Reference)
             return nr.rp < opt_.tol && nr.rd < opt_.tol && nr.mu < opt_.tol;
         const double gap = std::abs(nr.cx - nr.by) / std::max({1.0, std::abs(nr.cx), std::abs(nr.by)});
+        // Absolute targets as well (the FP64 certificate checks |Ax-b| and the reduced costs
+        // absolutely), floored where FP64 cannot resolve them.
         return gap <= opt_.gap_tol && nr.rpinf / (1.0 + binf_) <= opt_.gap_tol &&
-               nr.rdinf / (1.0 + cinf_) <= opt_.gap_tol;
+               nr.rdinf / (1.0 + cinf_) <= opt_.gap_tol &&
+               nr.rpinf <= std::max(0.5 * opt_.gap_tol, 1e-12 * (1.0 + binf_)) &&
+               nr.rdinf <= std::max(0.5 * opt_.gap_tol, 1e-12 * (1.0 + cinf_)) &&
+               nr.xsmax <= std::max(0.5 * opt_.gap_tol, 1e-14 * std::max({1.0, std::abs(nr.cx), std::abs(nr.by)}));
     }
 
     void init_point() {
