@@ -143,7 +143,7 @@ def main():
             f"gap={r['rel_gap']:.2e} rp={r['rel_primal_residual']:.2e} "
             f"rd={r['rel_dual_residual']:.2e} cert1e-8={c8} cert1e-6={c6} obj_err={oe} "
             f"factor={r['factor_ms']:.0f}/{r['solve_ms']:.0f} ms fact={r['factorizations']} "
-            f"pcg={r['refinement_steps']} ok={ok}")
+            f"pcg={r['refinement_steps']} fail={r.get('fail_code')} ok={ok}")
     iters = [r["iterations"] for r in results]
     value = args.steps / elapsed if all_ok else 0.0
 

@@ -37,10 +37,15 @@ struct KKTOptions {
     ipm::SupernodeOptions supernodes;
     bool use_graphs = false;        // capture factorization / solve launch sequences in hipGraphs
     void* stream = nullptr;         // hipStream_t to run on (nullptr: the solver creates one)
+    // factor_equilibrated regularization: delta_r = delta0 * 10^r, r < delta_retries
+    // (src/cuda/kkt.cu:652-690 uses 1e-9 and 5 for both precisions)
+    double delta0_fp64 = 1e-9;
+    double delta0_fp32 = 1e-9;
+    int delta_retries = 5;
 };
 
 struct KKTSolveInfo {
-    int retries = 0;               // delta increases used (0 = delta 1e-9)
+    int retries = 0;               // delta increases used (0 = delta0)
     int refinement_steps = 0;
     double delta = 0.0;
     double residual_inf = 0.0;     // final ||rhs - M0 dy||_inf
@@ -86,6 +91,8 @@ public:
     // ---- extensions used by the interior-point loop ----
     // Assemble D*M0*D + delta*I for d_Theta and factor it (delta = 1e-9 * 10^r, r < 5).
     // Returns the retry index used, or -1 when every delta failed.
+    // Regularization of the equilibrated factor (defaults: the reference's 1e-9 x10, 5 tries).
+    void set_regularization(double delta0_fp64, double delta0_fp32, int retries);
     int factor_equilibrated(const Float* d_Theta, FactorPrecision prec = FactorPrecision::FP64);
     // Solve with the cached factor (refining against M0 in FP64).  d_rhs is in original row
     // order and is overwritten with dy.  Same contract as gpu_cholesky_solve_device.

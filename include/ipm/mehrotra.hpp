@@ -39,6 +39,8 @@ struct MehrotraResult {
     Float rel_dual_residual = 0;          // ||A^T y + s - c||_inf / (1 + ||c||_inf)
     int factorizations = 0, fp32_factorizations = 0, fp64_factorizations = 0;
     int solves = 0, refinement_steps = 0;
+    int fail_code = 0;                    // 0; 1 factorization, 2 predictor solve, 3 predictor
+                                          // check, 4 corrector solve, 5 no step accepted
     double setup_ms = 0, solve_ms = 0, factor_ms = 0;
     double nnz_L = 0, flops_per_factorization = 0;
 };

@@ -117,7 +117,9 @@ typedef struct {
     double objective, primal_residual, dual_residual, duality_gap;
     double dual_objective, rel_gap, rel_primal_residual, rel_dual_residual;
     int32_t factorizations, fp32_factorizations, fp64_factorizations, solves;
-    int32_t refinement_steps, pad;
+    int32_t refinement_steps;
+    int32_t fail_code;          /* 0 none; 1 factorization, 2 predictor solve, 3 predictor check,
+                                   4 corrector solve, 5 no step accepted */
     double setup_ms, solve_ms, factor_ms, nnz_L, flops_per_factorization;
 } lk_ipm_result;
 

@@ -18,8 +18,9 @@
  * Deliberate differences from the reference: the factorization of D*M0*D + delta*I is cached per
  * IPM iteration (the reference recomputes the identical factor in every solve call), and nothing
  * is printed.  Relative-gap mode (stop_rule 1, an extension with no reference counterpart) uses
- * PCG instead of plain refinement, accepts a solve at the backward-error level, and skips the
- * e1 re-check of the KKT residual, and takes the longest finite, positive centred trial when both
+ * PCG instead of plain refinement, regularizes from delta = 1e-14 (not 1e-9), accepts a solve at
+ * the backward-error level only when PCG stagnates, skips the e1 re-check of the KKT residual,
+ * stops on absolute residual and per-column complementarity targets as well, and takes the longest finite, positive centred trial when both
  * merit searches fail (non-monotone fallback): the reference's absolute 1e-9 acceptance and
  * monotone merit search stall before a 1e-8 gap (its own loop stops at 1e-6).  The device path
  * implements the same rules.  Sums use the reference's term order; multiply-adds use fma() so the assembled
@@ -362,8 +363,11 @@ static int kkt_solve(or_kkt* k, double* rhs_orig, double tol, int relmode) {
     double* corr = malloc(sizeof(double) * (m + 1));
     for (int i = 0; i < m; ++i) rhs[i] = rhs_orig[k->P[i]];
     int success = 0;
-    double delta = 1e-9;
-    for (int retry = 0; retry < 5; ++retry) {
+    /* reference: 1e-9 x 10^r, r < 5; relative-gap mode starts at 1e-14 (PCG then needs a step or
+     * two instead of tens) and keeps the same ceiling 1e-5 */
+    double delta = relmode ? 1e-14 : 1e-9;
+    const int nretry = relmode ? 10 : 5;
+    for (int retry = 0; retry < nretry; ++retry) {
         if (retry > 0) delta *= 10.0;
         if (!(k->have_eq && k->delta_eq == delta)) {
             if (kkt_factor_eq(k, delta)) { k->have_eq = 0; continue; }

@@ -116,6 +116,7 @@ void fill_result(const ipm::MehrotraResult& r, lk_ipm_result* out, double* x, do
         out->fp64_factorizations = r.fp64_factorizations;
         out->solves = r.solves;
         out->refinement_steps = r.refinement_steps;
+        out->fail_code = r.fail_code;
         out->setup_ms = r.setup_ms;
         out->solve_ms = r.solve_ms;
         out->factor_ms = r.factor_ms;

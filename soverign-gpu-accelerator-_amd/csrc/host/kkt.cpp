@@ -750,10 +750,10 @@ struct GPUKKTCholeskySolver::Impl {
     int factor_equilibrated(const double* theta, FactorPrecision p) {
         keep_theta(theta);
         const int prec = p == FactorPrecision::FP32 ? LKD_F32 : LKD_F64;
-        double delta = 1e-9;
+        double delta = p == FactorPrecision::FP32 ? opt.delta0_fp32 : opt.delta0_fp64;
         have_eq = false;
         factor_kind = 0;
-        for (int r = 0; r < 5; ++r) {
+        for (int r = 0; r < opt.delta_retries; ++r) {
             if (r > 0) delta *= 10.0;
             if (assemble_and_factor(d_theta, true, delta, prec, r == 0)) {
                 have_eq = true;
@@ -1004,6 +1004,14 @@ bool GPUKKTCholeskySolver::gpu_cholesky_solve_device(Float* d_rhs, Float tol) {
 }
 
 Float* GPUKKTCholeskySolver::get_device_L_values() const { return impl_->d_csr; }
+
+void GPUKKTCholeskySolver::set_regularization(double delta0_fp64, double delta0_fp32, int retries) {
+    if (!(delta0_fp64 > 0.0) || !(delta0_fp32 > 0.0) || retries < 1)
+        throw std::invalid_argument("set_regularization: delta0 > 0 and retries >= 1 required");
+    impl_->opt.delta0_fp64 = delta0_fp64;
+    impl_->opt.delta0_fp32 = delta0_fp32;
+    impl_->opt.delta_retries = retries;
+}
 
 int GPUKKTCholeskySolver::factor_equilibrated(const Float* d_Theta, FactorPrecision prec) {
     return impl_->factor_equilibrated(d_Theta, prec);

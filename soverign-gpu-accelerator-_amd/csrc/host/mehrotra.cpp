@@ -13,8 +13,10 @@
 //    A(x + a dx) - b and A^T(y + a dy) + s + a ds - c per trial;
 //  * Ruiz scaling, the fill-reducing ordering and the symbolic phase run once in the constructor
 //    as in the reference, with AMD instead of the O(m^2) explicit minimum degree by default.
-// Relative-gap mode (extension; the reference's loop stops at an absolute 1e-6): KKT solves use
-// PCG with a backward-error acceptance, the e1 re-check of the KKT residual is dropped, and when
+// Relative-gap mode (extension; the reference's loop stops at an absolute 1e-6): the factor is
+// regularized from delta = 1e-14 instead of 1e-9, KKT solves use PCG preconditioned by it (a
+// backward-error acceptance only once PCG stagnates), the e1 re-check of the KKT residual is
+// dropped, the stop rule adds absolute residual and per-column complementarity targets, and when
 // both merit searches fail the longest finite, positive centred trial is taken (non-monotone).
 #include <algorithm>
 #include <chrono>
@@ -97,6 +99,13 @@ public:
         gpu::KKTOptions ko;
         ko.stream = stream_;
         ko.use_graphs = opt_.use_graphs;
+        if (opt_.stop_rule == IpmOptions::StopRule::RelativeGap) {
+            // 1e-14 keeps the preconditioner exact down to eigenvalues ~1e-13 of D M0 D, so PCG
+            // converges in a step or two late in the solve; FP32 panels need 1e-6 not to break down
+            ko.delta0_fp64 = 1e-14;
+            ko.delta0_fp32 = 1e-6;
+            ko.delta_retries = 10;
+        }
         kkt_.reset(new gpu::GPUKKTCholeskySolver(*arena_, A_, std::move(S), ko));
 
         // IPM device data
@@ -250,9 +259,16 @@ public:
         } else {
             ok = run(false);
         }
+        if (opt_.verbose)
+            std::printf("    solve: sigma_mu %.3e ok %d its %d res %.3e delta %.1e tol %.1e\n", sigma_mu, ok,
+                        info.refinement_steps, info.residual_inf, info.delta, tol);
         if (!ok) return false;
         check(lkd_ipm_direction(&P_, x_, s_, y_, th_, rd_, rp_, rxs_, dy_, dx, ds, qd_, qp_, stream_), "direction");
         read_scalars(LKS_NSLOTS);
+        if (opt_.verbose)
+            std::printf("    dir: e1 %.3e e2 %.3e e3 %.3e apmax %.3e admax %.3e\n",
+                        std::sqrt(hS_[LKS_R1]) / std::max(1.0, nr.rp), std::sqrt(hS_[LKS_R2]) / std::max(1.0, nr.rd),
+                        std::sqrt(hS_[LKS_R3]) / std::max(1.0, std::sqrt(hS_[LKS_RXS2])), hS_[LKS_APMAX], hS_[LKS_ADMAX]);
         // e1 re-checks the KKT residual; in relative-gap mode the solve's own (backward-error)
         // acceptance replaces it
         d.e1 = opt_.stop_rule == IpmOptions::StopRule::RelativeGap
@@ -363,15 +379,17 @@ public:
             if (use_fp32 && theta_spread() > opt_.mixed_fp64_spread) use_fp32 = false;
             bool fok = factor(use_fp32, res);
             if (!fok && use_fp32) { use_fp32 = false; fok = factor(false, res); }
-            if (!fok) { res.status = simplex::SimplexStatus::IterationLimit; res.iterations = iter; broke = true; break; }
+            if (!fok) { res.status = simplex::SimplexStatus::IterationLimit; res.iterations = iter; res.fail_code = 1; broke = true; break; }
             const double tol = 1e-9 * std::max(1.0, nr.rp);
 
             // ---- predictor ----
             Dir d;
-            if (!direction(0.0, nullptr, nullptr, dxa_, dsa_, tol, nr, d, res, use_fp32) ||
-                d.e1 > 1e-8 || d.e2 > 1e-8 || d.e3 > 1e-8) {
+            const bool pok = direction(0.0, nullptr, nullptr, dxa_, dsa_, tol, nr, d, res, use_fp32);
+            if (!pok || d.e1 > 1e-8 || d.e2 > 1e-8 || d.e3 > 1e-8) {
                 res.status = simplex::SimplexStatus::IterationLimit;
                 res.iterations = iter;
+                res.fail_code = pok ? 3 : 2;
+                if (opt_.verbose) std::printf("    predictor failed: solve %d e %.2e %.2e %.2e\n", pok, d.e1, d.e2, d.e3);
                 broke = true;
                 break;
             }
@@ -385,6 +403,7 @@ public:
             if (!direction(sigma * nr.mu, dxa_, dsa_, dx_, ds_, tol, nr, d, res, use_fp32)) {
                 res.status = simplex::SimplexStatus::IterationLimit;
                 res.iterations = iter;
+                res.fail_code = 4;
                 broke = true;
                 break;
             }
@@ -438,6 +457,7 @@ public:
             if (!accepted) {
                 res.status = simplex::SimplexStatus::IterationLimit;
                 res.iterations = iter;
+                res.fail_code = 5;
                 broke = true;
                 break;
             }

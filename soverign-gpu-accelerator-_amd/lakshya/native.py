@@ -50,7 +50,7 @@ class IpmResult(C.Structure):
                 ("rel_gap", C.c_double), ("rel_primal_residual", C.c_double),
                 ("rel_dual_residual", C.c_double), ("factorizations", C.c_int32),
                 ("fp32_factorizations", C.c_int32), ("fp64_factorizations", C.c_int32),
-                ("solves", C.c_int32), ("refinement_steps", C.c_int32), ("pad", C.c_int32),
+                ("solves", C.c_int32), ("refinement_steps", C.c_int32), ("fail_code", C.c_int32),
                 ("setup_ms", C.c_double), ("solve_ms", C.c_double), ("factor_ms", C.c_double),
                 ("nnz_L", C.c_double), ("flops_per_factorization", C.c_double)]
 
