@@ -89,6 +89,36 @@ def cpu_baseline(gpu_iters, blocks, seed):
                        f"blocks x {gpu_iters} iterations = {solve_s:.0f} s per solve")}
 
 
+def copy_calibration(mib, reps):
+    """Streaming copy kernel over `mib` MiB, `reps` launches: the known-size reference for the HBM
+    traffic counters (FETCH_SIZE / WRITE_SIZE of one launch read / write exactly mib MiB)."""
+    import ctypes as C
+    from lakshya import native
+    L = native.lib()
+    vp = C.c_void_p
+    for name, args in (("lkd_copy_f64", [vp, vp, C.c_int64, vp]), ("lkd_event_create", [C.POINTER(vp)]),
+                       ("lkd_event_record", [vp, vp]), ("lkd_event_elapsed_ms", [C.POINTER(C.c_float), vp, vp]),
+                       ("lkd_stream_sync", [vp])):
+        getattr(L, name).argtypes = args
+        getattr(L, name).restype = C.c_int
+    nbytes = mib << 20
+    src = native.DeviceBuffer(nbytes)
+    dst = native.DeviceBuffer(nbytes)
+    n = nbytes // 8
+    e0, e1 = vp(), vp()
+    assert L.lkd_event_create(C.byref(e0)) == 0 and L.lkd_event_create(C.byref(e1)) == 0
+    assert L.lkd_copy_f64(dst.ptr, src.ptr, n, None) == 0 and L.lkd_stream_sync(None) == 0
+    assert L.lkd_event_record(e0, None) == 0
+    for _ in range(reps):
+        assert L.lkd_copy_f64(dst.ptr, src.ptr, n, None) == 0
+    assert L.lkd_event_record(e1, None) == 0 and L.lkd_stream_sync(None) == 0
+    ms = C.c_float()
+    assert L.lkd_event_elapsed_ms(C.byref(ms), e0, e1) == 0
+    per = ms.value / reps
+    print(json.dumps({"copy_kernel": "k_copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "launches": reps + 1, "avg_ms": per, "GB_per_s": 2 * nbytes / (per * 1e-3) / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +131,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--graphs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the streaming copy kernel over MIB MiB (traffic-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (--gpus {args.gpus} requested): this build runs on a "
@@ -111,6 +143,9 @@ def main():
     if native.device_count() < 1:
         log("no GPU visible")
         sys.exit(1)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, max(args.steps, 1))
+        return
     lp, name = build_workload(args)
     g = golden(name)
     gold = g["objective"] if g else None

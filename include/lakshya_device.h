@@ -49,6 +49,8 @@ int lkd_graph_end(void* stream, void** exec);
 int lkd_graph_launch(void* exec, void* stream);
 int lkd_graph_destroy(void* exec);
 int lkd_fill_f64(double* dst, double value, int64_t n, void* stream);   /* kernel fill */
+/* streaming copy kernel (n even, 16-byte aligned): calibrates HBM traffic counters */
+int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream);
 
 /* ---------------------------------------------------------------- supernodal plan (device) */
 /* One work item of the factorization / triangular-solve schedule.  Field meaning per kernel:
@@ -75,26 +77,39 @@ typedef struct {
     const int32_t* col_to_sn;   /* [m] */
 } lkd_snplan;
 
+/* Load balance of sparse row passes: rows (or pair-map segments) longer than the threshold are
+ * listed once at set-up and each handled by a whole 256-thread block, in a fixed order. */
+#define LKD_LONG_ROW 512
+#define LKD_LONG_SEG 64
+typedef struct {
+    int32_t n;              /* number of listed rows / segments */
+    const int32_t* idx;     /* their indices (device) */
+} lkd_long;
+
 /* ---------------------------------------------------------------- normal-equations assembly */
 /* w[p] = a[p] * theta[col(p)] over the CSC entries of A (one thread per column).
  * Replaces the Theta factor inside compute_M_numerics_kernel (src/cuda/kkt.cu:177-215). */
 int lkd_assemble_prep(int32_t n, const int32_t* colptr, const double* aval, const double* theta,
                       double* w, void* stream);
-/* Diagonal of M0 per permuted row from its pair-map segment (same fixed fma order as
- * lkd_assemble_pairs, so the two agree bitwise): dseg[i] = segment of (i,i) or -1 (empty row).
+/* Diagonal of M0 per permuted row from its pair-map segment (same fixed order as
+ * lkd_assemble_pairs, so the two agree bitwise): dseg[i] = segment of (i,i) or -1 (empty row);
+ * long_diag lists the rows whose diagonal segment has more than LKD_LONG_SEG pairs.
  * D_i = 1/sqrt(max(|M0_ii|, 1e-12)) when equilibrate != 0, else 1 (src/cuda/kkt.cu:65-79). */
 int lkd_assemble_diag(int32_t m, const int64_t* dseg, const int64_t* segptr, const int32_t* pa,
                       const int32_t* pb, const double* w, const double* aval, int equilibrate,
-                      double* D, double* m0diag, void* stream);
+                      double* D, double* m0diag, lkd_long long_diag, void* stream);
 /* Segmented, fixed-order sums over the pair map: for segment t,
  *   acc = sum_{q in [segptr[t], segptr[t+1])} fma(w[pa[q]], aval[pb[q]], acc)
  *   panel[segdst[t]] = acc * D[segi[t]] * D[segj[t]] (+ delta on the diagonal of non-empty rows)
+ * sequentially for segments of up to LKD_LONG_SEG pairs; the segments in long_segs (all longer
+ * ones) are summed by a block: per-thread strided fma partials, then a fixed tree.
  * Replaces compute_M_numerics_kernel + build_equilibrated_M_kernel / add_regularization_kernel
  * (src/cuda/kkt.cu:81-94,157-171,177-215).  The panel must be zeroed beforehand. */
 int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int64_t* segdst,
                        const int32_t* segi, const int32_t* segj, const int32_t* pa,
                        const int32_t* pb, const double* w, const double* aval, const double* D,
-                       const uint8_t* empty_row, double delta, void* panel, void* stream);
+                       const uint8_t* empty_row, double delta, void* panel, lkd_long long_segs,
+                       void* stream);
 
 /* ---------------------------------------------------------------- supernodal Cholesky */
 /* Replaces cholesky_level_kernel and its level loop (src/cuda/kkt.cu:217-285,582-596).
@@ -135,7 +150,8 @@ int lkd_kkt_gather(int32_t m, const int32_t* P, const double* rhs, const double*
 /* dy[i] = (accumulate ? dy[i] : 0) + D[i] * z[i];  optionally backup[i] = old dy[i] */
 int lkd_kkt_update(int32_t m, const double* D, const double* z, double* dy, double* backup,
                    int accumulate, void* stream);
-/* r = rhs_perm - A_p (theta o (A_p^T dy)), req = D r.  kS (device doubles) receives
+/* r = rhs_perm - A_p (theta o (A_p^T dy)), req = D r; long_rows / long_cols list the rows of A_p
+ * and the columns with more than LKD_LONG_ROW entries.  kS (device doubles) receives
  * kS[0] = max|r|, kS[1] = max|rhs_perm|, kS[2] = max|dy|, kS[3] = max M0_ii (m0d); block
  * partials use kS + LKD_KS_PART.  The FP64 residual against the unregularised M0
  * (src/cuda/kkt.cu:110-123, computed without storing M0). */
@@ -145,7 +161,8 @@ int lkd_kkt_residual(int32_t m, int32_t n, const int32_t* colptr, const int32_t*
                      const double* cval, const int32_t* prp, const int32_t* pci,
                      const double* pval, const double* theta, const double* dy,
                      const double* rhs_perm, const double* m0d, double* u, double* r,
-                     const double* D, double* req, double* kS, void* stream);
+                     const double* D, double* req, double* kS, lkd_long long_rows,
+                     lkd_long long_cols, void* stream);
 /* Preconditioned CG pieces on the permuted normal equations (kS slots 4..15):
  *   matvec:  q = M0 p (two passes), kS[pq] = p^T q
  *   update:  a = kS[rz]/kS[pq]; dy += a p; r -= a q; req = D r; kS[4] = max|r|, kS[2] = max|dy|
@@ -154,7 +171,8 @@ int lkd_kkt_residual(int32_t m, int32_t n, const int32_t* colptr, const int32_t*
 int lkd_pcg_matvec(int32_t m, int32_t n, const int32_t* colptr, const int32_t* prow,
                    const double* cval, const int32_t* prp, const int32_t* pci,
                    const double* pval, const double* theta, const double* p, double* u,
-                   double* q, double* kS, int32_t slot_pq, void* stream);
+                   double* q, double* kS, int32_t slot_pq, lkd_long long_rows, lkd_long long_cols,
+                   void* stream);
 int lkd_pcg_update(int32_t m, double* kS, int32_t slot_rz, int32_t slot_pq, const double* p,
                    const double* q, double* dy, double* r, const double* D, double* req,
                    void* stream);
@@ -189,6 +207,7 @@ typedef struct {
     const int32_t *trp, *tci; const double* tval;     /* A^T in CSR (= CSC of A) */
     const double *b, *c, *R, *C;                      /* scaled data, scale vectors */
     int32_t grp_rows, grp_cols;                       /* lanes per row of A / of A^T */
+    lkd_long long_rows, long_cols;                    /* rows / columns > LKD_LONG_ROW entries */
     double* part;                                     /* block partials scratch */
     int32_t nblk_rows, nblk_cols;                     /* blocks used for row / column passes */
     double* S;                                        /* scalar slots [LKS_TOTAL] */

@@ -7,6 +7,12 @@ __global__ void k_fill(double* __restrict__ p, double v, int64_t n) {
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
+// Plain streaming copy (16-byte vectors): the reference kernel for HBM traffic counters.
+__global__ void k_copy(double2* __restrict__ d, const double2* __restrict__ s, int64_t n2) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2;
+         i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
 }  // namespace
 
 extern "C" {
@@ -82,6 +88,14 @@ int lkd_fill_f64(double* dst, double value, int64_t n, void* stream) {
     if (n <= 0) return 0;
     const int64_t blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
     LK_LAUNCH(k_fill, (int)blocks, 256, 0, stream, dst, value, n);
+    return lk_last_error();
+}
+int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream) {
+    if (n <= 0) return 0;
+    if ((n & 1) || (((uintptr_t)dst | (uintptr_t)src) & 15)) return (int)hipErrorInvalidValue;
+    const int64_t n2 = n / 2;
+    const int64_t blocks = (n2 + 255) / 256 < 8192 ? (n2 + 255) / 256 : 8192;
+    LK_LAUNCH(k_copy, (int)blocks, 256, 0, stream, (double2*)dst, (const double2*)src, n2);
     return lk_last_error();
 }
 

@@ -58,15 +58,34 @@ int finalize(const double* part, int nblk, int q, const int* ops, const int* slo
     return lk_last_error();
 }
 
-// Group of G lanes per row: returns the row dot product in every lane of the group.
-template <int G>
-__device__ __forceinline__ double row_dot(const int32_t* __restrict__ rp,
-                                          const int32_t* __restrict__ ci,
-                                          const double* __restrict__ val,
-                                          const double* __restrict__ x, int64_t i, int lane) {
-    double s = 0.0;
-    for (int32_t p = rp[i] + lane; p < rp[i + 1]; p += G) s += val[p] * x[ci[p]];
-    return group_sum<G>(s);
+// Sparse row pass y_i = sum_p val[p] x[ci[p]] with the load balanced: rows longer than
+// LKD_LONG_ROW entries (listed in `longs`, e.g. the linking rows of a block-angular matrix) take
+// a whole block each, in a fixed thread-stride + tree order; the others take G lanes.  f(i, y_i)
+// runs once per row, in thread 0 of the block (long rows) or lane 0 of the group.
+template <int G, typename F>
+__device__ __forceinline__ void for_rows(int64_t m, const int32_t* __restrict__ rp,
+                                         const int32_t* __restrict__ ci,
+                                         const double* __restrict__ val,
+                                         const double* __restrict__ x, int32_t nlong,
+                                         const int32_t* __restrict__ longs, double* sh, F&& f) {
+    for (int32_t k = blockIdx.x; k < nlong; k += gridDim.x) {
+        const int32_t i = longs[k];
+        double s = 0.0;
+        for (int32_t p = rp[i] + threadIdx.x; p < rp[i + 1]; p += NT) s += val[p] * x[ci[p]];
+        s = block_reduce(s, OP_SUM, sh);
+        if (threadIdx.x == 0) f((int64_t)i, s);
+    }
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
+    const int64_t ng = (int64_t)gridDim.x * NT / G;
+    for (int64_t i = g0; i < m; i += ng) {
+        const int32_t b = rp[i], e = rp[i + 1];
+        if (e - b > LKD_LONG_ROW) continue;
+        double s = 0.0;
+        for (int32_t p = b + lane; p < e; p += G) s += val[p] * x[ci[p]];
+        s = group_sum<G>(s);
+        if (lane == 0) f(i, s);
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -91,20 +110,51 @@ __device__ __forceinline__ double seg_sum(const int64_t* __restrict__ segptr,
     return acc;
 }
 
+// Long segments (more than LKD_LONG_SEG pairs): thread-stride fma partials, then the fixed tree
+// of block_reduce; the result is valid in thread 0.  All threads of the block must call it.
+__device__ __forceinline__ double seg_sum_block(const int64_t* __restrict__ segptr,
+                                                const int32_t* __restrict__ pa,
+                                                const int32_t* __restrict__ pb,
+                                                const double* __restrict__ w,
+                                                const double* __restrict__ aval, int64_t t,
+                                                double* sh) {
+    double acc = 0.0;
+    for (int64_t q = segptr[t] + threadIdx.x; q < segptr[t + 1]; q += NT)
+        acc = fma(w[pa[q]], aval[pb[q]], acc);
+    return block_reduce(acc, OP_SUM, sh);
+}
+
+__device__ __forceinline__ bool seg_long(const int64_t* __restrict__ segptr, int64_t t) {
+    return segptr[t + 1] - segptr[t] > LKD_LONG_SEG;
+}
+
+__device__ __forceinline__ void store_diag(int64_t i, double mii, int equil, double* D,
+                                           double* m0d) {
+    if (m0d) m0d[i] = mii;
+    double a = fabs(mii);
+    if (a < 1e-12) a = 1e-12;
+    D[i] = equil ? 1.0 / sqrt(a) : 1.0;
+}
+
+// ld lists the rows whose diagonal segment is long; blocks take those first.
 __global__ __launch_bounds__(NT) void k_diag(int32_t m, const int64_t* __restrict__ dseg,
                                              const int64_t* __restrict__ segptr,
                                              const int32_t* __restrict__ pa,
                                              const int32_t* __restrict__ pb,
                                              const double* __restrict__ w,
                                              const double* __restrict__ aval, int equil,
-                                             double* __restrict__ D, double* __restrict__ m0d) {
+                                             double* __restrict__ D, double* __restrict__ m0d,
+                                             lkd_long ld) {
+    __shared__ double sh[4];
+    for (int32_t k = blockIdx.x; k < ld.n; k += gridDim.x) {
+        const int32_t i = ld.idx[k];
+        const double mii = seg_sum_block(segptr, pa, pb, w, aval, dseg[i], sh);
+        if (threadIdx.x == 0) store_diag(i, mii, equil, D, m0d);
+    }
     for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < m; i += (int64_t)gridDim.x * NT) {
         const int64_t t = dseg[i];
-        const double mii = t >= 0 ? seg_sum(segptr, pa, pb, w, aval, t) : 0.0;
-        if (m0d) m0d[i] = mii;
-        double a = fabs(mii);
-        if (a < 1e-12) a = 1e-12;
-        D[i] = equil ? 1.0 / sqrt(a) : 1.0;
+        if (t >= 0 && seg_long(segptr, t)) continue;
+        store_diag(i, t >= 0 ? seg_sum(segptr, pa, pb, w, aval, t) : 0.0, equil, D, m0d);
     }
 }
 
@@ -119,13 +169,23 @@ __global__ __launch_bounds__(NT) void k_pairs(int64_t nseg, const int64_t* __res
                                               const double* __restrict__ aval,
                                               const double* __restrict__ D,
                                               const uint8_t* __restrict__ empty, double delta,
-                                              T* __restrict__ panel) {
-    for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < nseg;
-         t += (int64_t)gridDim.x * NT) {
+                                              T* __restrict__ panel, lkd_long ls) {
+    __shared__ double sh[4];
+    auto store = [&](int64_t t, double acc) {
         const int32_t i = segi[t], j = segj[t];
-        double v = seg_sum(segptr, pa, pb, w, aval, t) * D[i] * D[j];
+        double v = acc * D[i] * D[j];
         if (i == j && !empty[i]) v += delta;
         panel[segdst[t]] = (T)v;
+    };
+    for (int32_t k = blockIdx.x; k < ls.n; k += gridDim.x) {
+        const int32_t t = ls.idx[k];
+        const double acc = seg_sum_block(segptr, pa, pb, w, aval, t, sh);
+        if (threadIdx.x == 0) store(t, acc);
+    }
+    for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < nseg;
+         t += (int64_t)gridDim.x * NT) {
+        if (seg_long(segptr, t)) continue;
+        store(t, seg_sum(segptr, pa, pb, w, aval, t));
     }
 }
 
@@ -165,17 +225,16 @@ __global__ __launch_bounds__(NT) void k_kupdate(int32_t m, const double* __restr
 }
 
 // u_j = theta_j * sum_i A_p[i][j] dy_i   (column pass over the CSC with permuted row indices)
+template <int G>
 __global__ __launch_bounds__(NT) void k_res_cols(int32_t n, const int32_t* __restrict__ colptr,
                                                  const int32_t* __restrict__ prow,
                                                  const double* __restrict__ cval,
                                                  const double* __restrict__ theta,
                                                  const double* __restrict__ dy,
-                                                 double* __restrict__ u) {
-    for (int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x; j < n; j += (int64_t)gridDim.x * NT) {
-        double s = 0.0;
-        for (int32_t p = colptr[j]; p < colptr[j + 1]; ++p) s += cval[p] * dy[prow[p]];
-        u[j] = theta[j] * s;
-    }
+                                                 double* __restrict__ u, lkd_long lc) {
+    __shared__ double sh[4];
+    for_rows<G>(n, colptr, prow, cval, dy, lc.n, lc.idx, sh,
+                [&](int64_t j, double s) { u[j] = theta[j] * s; });
 }
 
 // r_i = rhs_i - sum_j A_p[i][j] u_j ; req_i = D_i r_i ; partial max |r|, |rhs|, |dy|, M0_ii
@@ -190,24 +249,18 @@ __global__ __launch_bounds__(NT) void k_res_rows(int32_t m, const int32_t* __res
                                                  double* __restrict__ r,
                                                  const double* __restrict__ D,
                                                  double* __restrict__ req,
-                                                 double* __restrict__ part) {
+                                                 double* __restrict__ part, lkd_long lr) {
     __shared__ double sh[4];
-    const int lane = threadIdx.x & (G - 1);
-    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
-    const int64_t ng = (int64_t)gridDim.x * NT / G;
     double mr = 0.0, mb = 0.0, md = 0.0, mm = 0.0;
-    for (int64_t i = g0; i < m; i += ng) {
-        const double s = row_dot<G>(prp, pci, pval, u, i, lane);
-        if (lane == 0) {
-            const double v = rhs[i] - s;
-            r[i] = v;
-            req[i] = v * D[i];
-            mr = fmax(mr, fabs(v));
-            mb = fmax(mb, fabs(rhs[i]));
-            md = fmax(md, fabs(dy[i]));
-            mm = fmax(mm, m0d[i]);
-        }
-    }
+    for_rows<G>(m, prp, pci, pval, u, lr.n, lr.idx, sh, [&](int64_t i, double s) {
+        const double v = rhs[i] - s;
+        r[i] = v;
+        req[i] = v * D[i];
+        mr = fmax(mr, fabs(v));
+        mb = fmax(mb, fabs(rhs[i]));
+        md = fmax(md, fabs(dy[i]));
+        mm = fmax(mm, m0d[i]);
+    });
     const int nb = gridDim.x;
     double t;
     t = block_reduce(mr, OP_MAX, sh); if (threadIdx.x == 0) part[0 * nb + blockIdx.x] = t;
@@ -224,16 +277,13 @@ __global__ __launch_bounds__(NT) void k_pcg_rows(int32_t m, const int32_t* __res
                                                  const double* __restrict__ u,
                                                  const double* __restrict__ p,
                                                  double* __restrict__ q,
-                                                 double* __restrict__ part) {
+                                                 double* __restrict__ part, lkd_long lr) {
     __shared__ double sh[4];
-    const int lane = threadIdx.x & (G - 1);
-    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
-    const int64_t ng = (int64_t)gridDim.x * NT / G;
     double acc = 0.0;
-    for (int64_t i = g0; i < m; i += ng) {
-        const double s = row_dot<G>(prp, pci, pval, u, i, lane);
-        if (lane == 0) { q[i] = s; acc += p[i] * s; }
-    }
+    for_rows<G>(m, prp, pci, pval, u, lr.n, lr.idx, sh, [&](int64_t i, double s) {
+        q[i] = s;
+        acc += p[i] * s;
+    });
     acc = block_reduce(acc, OP_SUM, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
@@ -312,20 +362,14 @@ __global__ __launch_bounds__(NT) void k_resid_rows(lkd_ipm P, const double* __re
                                                    const double* __restrict__ y,
                                                    double* __restrict__ rp) {
     __shared__ double sh[4];
-    const int lane = threadIdx.x & (G - 1);
-    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
-    const int64_t ng = (int64_t)gridDim.x * NT / G;
     double s2 = 0.0, inf = 0.0, by = 0.0;
-    for (int64_t i = g0; i < P.m; i += ng) {
-        const double ax = row_dot<G>(P.arp, P.aci, P.aval, x, i, lane);
-        if (lane == 0) {
-            const double v = ax + (-P.b[i]);
-            rp[i] = v;
-            s2 += v * v;
-            inf = fmax(inf, fabs(v / P.R[i]));
-            by += P.b[i] * y[i];
-        }
-    }
+    for_rows<G>(P.m, P.arp, P.aci, P.aval, x, P.long_rows.n, P.long_rows.idx, sh, [&](int64_t i, double ax) {
+        const double v = ax + (-P.b[i]);
+        rp[i] = v;
+        s2 += v * v;
+        inf = fmax(inf, fabs(v / P.R[i]));
+        by += P.b[i] * y[i];
+    });
     const int nb = gridDim.x;
     double r;
     r = block_reduce(s2, OP_SUM, sh); if (threadIdx.x == 0) P.part[0 * nb + blockIdx.x] = r;
@@ -340,23 +384,17 @@ __global__ __launch_bounds__(NT) void k_resid_cols(lkd_ipm P, const double* __re
                                                    double* __restrict__ rd,
                                                    double* __restrict__ theta, int qoff) {
     __shared__ double sh[4];
-    const int lane = threadIdx.x & (G - 1);
-    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
-    const int64_t ng = (int64_t)gridDim.x * NT / G;
     double s2 = 0.0, xs = 0.0, cx = 0.0, inf = 0.0, xsmax = 0.0;
-    for (int64_t j = g0; j < P.n; j += ng) {
-        const double aty = row_dot<G>(P.trp, P.tci, P.tval, y, j, lane);
-        if (lane == 0) {
-            const double v = aty + (s[j] - P.c[j]);
-            rd[j] = v;
-            if (theta) theta[j] = x[j] / s[j];
-            s2 += v * v;
-            xs += x[j] * s[j];
-            cx += P.c[j] * x[j];
-            inf = fmax(inf, fabs(v / P.C[j]));
-            xsmax = fmax(xsmax, fabs(x[j] * (s[j] - v)));   // x_j (c - A^T y)_j, scale-free
-        }
-    }
+    for_rows<G>(P.n, P.trp, P.tci, P.tval, y, P.long_cols.n, P.long_cols.idx, sh, [&](int64_t j, double aty) {
+        const double v = aty + (s[j] - P.c[j]);
+        rd[j] = v;
+        if (theta) theta[j] = x[j] / s[j];
+        s2 += v * v;
+        xs += x[j] * s[j];
+        cx += P.c[j] * x[j];
+        inf = fmax(inf, fabs(v / P.C[j]));
+        xsmax = fmax(xsmax, fabs(x[j] * (s[j] - v)));   // x_j (c - A^T y)_j, scale-free
+    });
     const int nb = gridDim.x;
     double* part = P.part + (int64_t)qoff * nb;
     double r;
@@ -387,13 +425,9 @@ template <int G>
 __global__ __launch_bounds__(NT) void k_rkkt_rows(lkd_ipm P, const double* __restrict__ v,
                                                   const double* __restrict__ rp,
                                                   double* __restrict__ rkkt) {
-    const int lane = threadIdx.x & (G - 1);
-    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
-    const int64_t ng = (int64_t)gridDim.x * NT / G;
-    for (int64_t i = g0; i < P.m; i += ng) {
-        const double av = row_dot<G>(P.arp, P.aci, P.aval, v, i, lane);
-        if (lane == 0) rkkt[i] = av + (-rp[i]);
-    }
+    __shared__ double sh[4];
+    for_rows<G>(P.m, P.arp, P.aci, P.aval, v, P.long_rows.n, P.long_rows.idx, sh,
+                [&](int64_t i, double av) { rkkt[i] = av + (-rp[i]); });
 }
 
 template <int G>
@@ -406,29 +440,23 @@ __global__ __launch_bounds__(NT) void k_dir_cols(lkd_ipm P, const double* __rest
                                                  double* __restrict__ dx, double* __restrict__ ds,
                                                  double* __restrict__ qd) {
     __shared__ double sh[4];
-    const int lane = threadIdx.x & (G - 1);
-    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
-    const int64_t ng = (int64_t)gridDim.x * NT / G;
     double r2 = 0, r3 = 0, rx = 0, apm = 1e30, adm = 1e30, dx2 = 0, ds2 = 0;
-    for (int64_t j = g0; j < P.n; j += ng) {
-        const double t = row_dot<G>(P.trp, P.tci, P.tval, dy, j, lane);
-        if (lane == 0) {
-            const double dsj = (-t) + (-rd[j]);
-            const double dxj = -theta[j] * dsj + rxs[j] / s[j];
-            ds[j] = dsj;
-            dx[j] = dxj;
-            qd[j] = t + dsj;
-            const double e2 = t + (dsj + rd[j]);
-            const double e3 = s[j] * dxj + x[j] * dsj - rxs[j];
-            r2 += e2 * e2;
-            r3 += e3 * e3;
-            rx += rxs[j] * rxs[j];
-            if (dxj < 0.0) apm = fmin(apm, -x[j] / dxj);
-            if (dsj < 0.0) adm = fmin(adm, -s[j] / dsj);
-            dx2 += dxj * dxj;
-            ds2 += dsj * dsj;
-        }
-    }
+    for_rows<G>(P.n, P.trp, P.tci, P.tval, dy, P.long_cols.n, P.long_cols.idx, sh, [&](int64_t j, double t) {
+        const double dsj = (-t) + (-rd[j]);
+        const double dxj = -theta[j] * dsj + rxs[j] / s[j];
+        ds[j] = dsj;
+        dx[j] = dxj;
+        qd[j] = t + dsj;
+        const double e2 = t + (dsj + rd[j]);
+        const double e3 = s[j] * dxj + x[j] * dsj - rxs[j];
+        r2 += e2 * e2;
+        r3 += e3 * e3;
+        rx += rxs[j] * rxs[j];
+        if (dxj < 0.0) apm = fmin(apm, -x[j] / dxj);
+        if (dsj < 0.0) adm = fmin(adm, -s[j] / dsj);
+        dx2 += dxj * dxj;
+        ds2 += dsj * dsj;
+    });
     const int nb = gridDim.x;
     double r;
     r = block_reduce(r2, OP_SUM, sh); if (threadIdx.x == 0) P.part[0 * nb + blockIdx.x] = r;
@@ -446,19 +474,13 @@ __global__ __launch_bounds__(NT) void k_dir_rows(lkd_ipm P, const double* __rest
                                                  const double* __restrict__ dy,
                                                  double* __restrict__ qp, int qoff) {
     __shared__ double sh[4];
-    const int lane = threadIdx.x & (G - 1);
-    const int64_t g0 = (blockIdx.x * (int64_t)NT + threadIdx.x) / G;
-    const int64_t ng = (int64_t)gridDim.x * NT / G;
     double r1 = 0, dy2 = 0;
-    for (int64_t i = g0; i < P.m; i += ng) {
-        const double adx = row_dot<G>(P.arp, P.aci, P.aval, dx, i, lane);
-        if (lane == 0) {
-            qp[i] = adx;
-            const double e1 = adx + rp[i];
-            r1 += e1 * e1;
-            dy2 += dy[i] * dy[i];
-        }
-    }
+    for_rows<G>(P.m, P.arp, P.aci, P.aval, dx, P.long_rows.n, P.long_rows.idx, sh, [&](int64_t i, double adx) {
+        qp[i] = adx;
+        const double e1 = adx + rp[i];
+        r1 += e1 * e1;
+        dy2 += dy[i] * dy[i];
+    });
     const int nb = gridDim.x;
     double* part = P.part + (int64_t)qoff * nb;
     double r;
@@ -598,13 +620,14 @@ template <int G> void l_dir_rows(const lkd_ipm* P, const double* dx, const doubl
 }
 template <int G> void l_res_rows(int32_t m, const int32_t* prp, const int32_t* pci, const double* pval,
                                  const double* u, const double* rhs, const double* dy, const double* m0d,
-                                 double* r, const double* D, double* req, double* part, int nblk, void* st) {
-    LK_LAUNCH(k_res_rows<G>, nblk, NT, 0, st, m, prp, pci, pval, u, rhs, dy, m0d, r, D, req, part);
+                                 double* r, const double* D, double* req, double* part, int nblk,
+                                 lkd_long lr, void* st) {
+    LK_LAUNCH(k_res_rows<G>, nblk, NT, 0, st, m, prp, pci, pval, u, rhs, dy, m0d, r, D, req, part, lr);
 }
 template <int G> void l_pcg_rows(int32_t m, const int32_t* prp, const int32_t* pci, const double* pval,
                                  const double* u, const double* p, double* q, double* part, int nblk,
-                                 void* st) {
-    LK_LAUNCH(k_pcg_rows<G>, nblk, NT, 0, st, m, prp, pci, pval, u, p, q, part);
+                                 lkd_long lr, void* st) {
+    LK_LAUNCH(k_pcg_rows<G>, nblk, NT, 0, st, m, prp, pci, pval, u, p, q, part, lr);
 }
 
 }  // namespace
@@ -619,23 +642,24 @@ int lkd_assemble_prep(int32_t n, const int32_t* colptr, const double* aval, cons
 
 int lkd_assemble_diag(int32_t m, const int64_t* dseg, const int64_t* segptr, const int32_t* pa,
                       const int32_t* pb, const double* w, const double* aval, int equilibrate,
-                      double* D, double* m0diag, void* stream) {
+                      double* D, double* m0diag, lkd_long long_diag, void* stream) {
     LK_LAUNCH(k_diag, grid_for(m), NT, 0, stream, m, dseg, segptr, pa, pb, w, aval, equilibrate,
-              D, m0diag);
+              D, m0diag, long_diag);
     return lk_last_error();
 }
 
 int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int64_t* segdst,
                        const int32_t* segi, const int32_t* segj, const int32_t* pa,
                        const int32_t* pb, const double* w, const double* aval, const double* D,
-                       const uint8_t* empty_row, double delta, void* panel, void* stream) {
+                       const uint8_t* empty_row, double delta, void* panel, lkd_long long_segs,
+                       void* stream) {
     if (nseg <= 0) return 0;
     if (prec == LKD_F32)
         LK_LAUNCH(k_pairs<float>, grid_for(nseg, 16384), NT, 0, stream, nseg, segptr, segdst, segi,
-                  segj, pa, pb, w, aval, D, empty_row, delta, (float*)panel);
+                  segj, pa, pb, w, aval, D, empty_row, delta, (float*)panel, long_segs);
     else
         LK_LAUNCH(k_pairs<double>, grid_for(nseg, 16384), NT, 0, stream, nseg, segptr, segdst, segi,
-                  segj, pa, pb, w, aval, D, empty_row, delta, (double*)panel);
+                  segj, pa, pb, w, aval, D, empty_row, delta, (double*)panel, long_segs);
     return lk_last_error();
 }
 
@@ -655,11 +679,12 @@ int lkd_kkt_residual(int32_t m, int32_t n, const int32_t* colptr, const int32_t*
                      const double* cval, const int32_t* prp, const int32_t* pci,
                      const double* pval, const double* theta, const double* dy,
                      const double* rhs_perm, const double* m0d, double* u, double* r,
-                     const double* D, double* req, double* kS, void* stream) {
-    LK_LAUNCH(k_res_cols, grid_for(n), NT, 0, stream, n, colptr, prow, cval, theta, dy, u);
+                     const double* D, double* req, double* kS, lkd_long long_rows,
+                     lkd_long long_cols, void* stream) {
+    LK_LAUNCH(k_res_cols<4>, grid_for(4 * (int64_t)n), NT, 0, stream, n, colptr, prow, cval, theta, dy, u, long_cols);
     const int nblk = grid_for(m, 1024);
     double* part = kS + LKD_KS_PART;
-    l_res_rows<8>(m, prp, pci, pval, u, rhs_perm, dy, m0d, r, D, req, part, nblk, stream);
+    l_res_rows<8>(m, prp, pci, pval, u, rhs_perm, dy, m0d, r, D, req, part, nblk, long_rows, stream);
     int ops[4] = {OP_MAX, OP_MAX, OP_MAX, OP_MAX}, slots[4] = {0, 1, 2, 3};
     return finalize(part, nblk, 4, ops, slots, kS, stream);
 }
@@ -667,11 +692,12 @@ int lkd_kkt_residual(int32_t m, int32_t n, const int32_t* colptr, const int32_t*
 int lkd_pcg_matvec(int32_t m, int32_t n, const int32_t* colptr, const int32_t* prow,
                    const double* cval, const int32_t* prp, const int32_t* pci,
                    const double* pval, const double* theta, const double* p, double* u,
-                   double* q, double* kS, int32_t slot_pq, void* stream) {
-    LK_LAUNCH(k_res_cols, grid_for(n), NT, 0, stream, n, colptr, prow, cval, theta, p, u);
+                   double* q, double* kS, int32_t slot_pq, lkd_long long_rows, lkd_long long_cols,
+                   void* stream) {
+    LK_LAUNCH(k_res_cols<4>, grid_for(4 * (int64_t)n), NT, 0, stream, n, colptr, prow, cval, theta, p, u, long_cols);
     const int nblk = grid_for(m, 1024);
     double* part = kS + LKD_KS_PART;
-    l_pcg_rows<8>(m, prp, pci, pval, u, p, q, part, nblk, stream);
+    l_pcg_rows<8>(m, prp, pci, pval, u, p, q, part, nblk, long_rows, stream);
     int op = OP_SUM;
     return finalize(part, nblk, 1, &op, &slot_pq, kS, stream);
 }

@@ -84,6 +84,8 @@ struct GPUKKTCholeskySolver::Impl {
     int64_t *d_segptr = nullptr, *d_segdst = nullptr, *d_dseg = nullptr;
     int32_t *d_segi = nullptr, *d_segj = nullptr, *d_pa = nullptr, *d_pb = nullptr;
     uint8_t* d_empty = nullptr;
+    // rows / columns / pair-map segments handled by whole blocks (LKD_LONG_ROW / LKD_LONG_SEG)
+    lkd_long long_prows{0, nullptr}, long_cols{0, nullptr}, long_diag{0, nullptr}, long_segs{0, nullptr};
     int32_t *d_sn_first = nullptr, *d_sn_rows = nullptr, *d_col_to_sn = nullptr;
     int64_t *d_sn_rowptr = nullptr, *d_sn_panel = nullptr;
     lkd_task *d_ftasks = nullptr, *d_stasks = nullptr;
@@ -214,6 +216,8 @@ struct GPUKKTCholeskySolver::Impl {
                       pval.begin() + prp[k]);
         }
         h_colptr = colptr; h_prow = prow; h_cval_n = nz;
+        long_prows = long_list(prp, m, LKD_LONG_ROW, "long_prows");
+        long_cols = long_list(colptr, n, LKD_LONG_ROW, "long_cols");
         d_colptr = up(colptr, "colptr");
         d_prow = up(prow, "prow");
         d_cval = up(cval, "cval");
@@ -225,6 +229,16 @@ struct GPUKKTCholeskySolver::Impl {
     }
     std::vector<Index> h_colptr, h_prow;
     int64_t h_cval_n = 0;
+
+    // indices k < count with ptr[k+1] - ptr[k] > thr, uploaded
+    template <typename P>
+    lkd_long long_list(const std::vector<P>& ptr, int64_t count, int64_t thr, const char* name) {
+        std::vector<int32_t> idx;
+        for (int64_t k = 0; k < count; ++k)
+            if (static_cast<int64_t>(ptr[k + 1]) - static_cast<int64_t>(ptr[k]) > thr)
+                idx.push_back(static_cast<int32_t>(k));
+        return {static_cast<int32_t>(idx.size()), up(idx, name)};
+    }
 
     int64_t panel_index(Index i, Index j) const {   // i >= j, permuted
         const Index s = S.col_to_sn[j];
@@ -274,6 +288,14 @@ struct GPUKKTCholeskySolver::Impl {
         nseg = static_cast<int64_t>(segdst.size());
         pairs.clear();
         pairs.shrink_to_fit();
+        if (nseg >= (int64_t(1) << 31)) throw std::length_error("KKT plan: more than 2^31 M0 entries");
+        long_segs = long_list(segptr, nseg, LKD_LONG_SEG, "long_segs");
+        {
+            std::vector<int32_t> ld;
+            for (Index i = 0; i < m; ++i)
+                if (dseg[i] >= 0 && segptr[dseg[i] + 1] - segptr[dseg[i]] > LKD_LONG_SEG) ld.push_back(i);
+            long_diag = {static_cast<int32_t>(ld.size()), up(ld, "long_diag")};
+        }
         d_segptr = up(segptr, "segptr");
         d_segdst = up(segdst, "segdst");
         d_segi = up(segi, "segi");
@@ -627,11 +649,11 @@ struct GPUKKTCholeskySolver::Impl {
         const size_t esz = prec == LKD_F32 ? sizeof(float) : sizeof(double);
         check(lkd_event_record(a0, stream), "event");
         check(lkd_assemble_prep(n, d_colptr, d_cval, d_theta, d_w, stream), "prep");
-        check(lkd_assemble_diag(m, d_dseg, d_segptr, d_pa, d_pb, d_w, d_cval, 1, d_D, d_m0d, stream), "diag");
+        check(lkd_assemble_diag(m, d_dseg, d_segptr, d_pa, d_pb, d_w, d_cval, 1, d_D, d_m0d, long_diag, stream), "diag");
         check(lkd_memset(d_panel, 0, static_cast<size_t>(S.sn_panel[S.nsuper]) * esz, stream), "zero");
         check(lkd_memset(d_err, 0, sizeof(int32_t), stream), "zero");
         check(lkd_assemble_pairs(prec, nseg, d_segptr, d_segdst, d_segi, d_segj, d_pa, d_pb, d_w,
-                                 d_cval, d_D, d_empty, 1e-9, d_panel, stream), "assemble");
+                                 d_cval, d_D, d_empty, 1e-9, d_panel, long_segs, stream), "assemble");
         for (const auto& L : flaunch) {
             const lkd_task* t = d_ftasks + L.begin;
             check(lkd_event_record(e0, stream), "event");
@@ -727,12 +749,12 @@ struct GPUKKTCholeskySolver::Impl {
         if (need_diag) {
             check(lkd_assemble_prep(n, d_colptr, d_cval, theta, d_w, stream), "prep");
             check(lkd_assemble_diag(m, d_dseg, d_segptr, d_pa, d_pb, d_w, d_cval, equil ? 1 : 0,
-                                    d_D, d_m0d, stream), "diag");
+                                    d_D, d_m0d, long_diag, stream), "diag");
         }
         check(lkd_memset(d_panel, 0, static_cast<size_t>(S.sn_panel[S.nsuper]) * esz, stream), "panel zero");
         check(lkd_memset(d_err, 0, sizeof(int32_t), stream), "err zero");
         check(lkd_assemble_pairs(prec, nseg, d_segptr, d_segdst, d_segi, d_segj, d_pa, d_pb, d_w,
-                                 d_cval, d_D, d_empty, delta, d_panel, stream), "assemble");
+                                 d_cval, d_D, d_empty, delta, d_panel, long_segs, stream), "assemble");
         factor_numeric(prec);
         int32_t err = 0;
         check(lkd_memcpy_dtoh(&err, d_err, sizeof(int32_t), stream), "err read");
@@ -772,7 +794,8 @@ struct GPUKKTCholeskySolver::Impl {
     // kS[0..3] = max|r|, max|rhs|, max|dy|, max M0_ii
     void residual() {
         check(lkd_kkt_residual(m, n, d_colptr, d_prow, d_cval, d_prp, d_pci, d_pval, d_theta,
-                               d_dy, d_rhsp, d_m0d, d_u, d_r, d_D, d_req, d_rinf, stream), "residual");
+                               d_dy, d_rhsp, d_m0d, d_u, d_r, d_D, d_req, d_rinf, long_prows, long_cols,
+                               stream), "residual");
     }
     double hk[LKD_KS_SLOTS];
     void read_ks() {
@@ -813,7 +836,7 @@ struct GPUKKTCholeskySolver::Impl {
             for (int it = 0; it < max_it; ++it) {
                 its = it + 1;
                 check(lkd_pcg_matvec(m, n, d_colptr, d_prow, d_cval, d_prp, d_pci, d_pval, d_theta,
-                                     d_p, d_u, d_q, d_rinf, 6, stream), "matvec");
+                                     d_p, d_u, d_q, d_rinf, 6, long_prows, long_cols, stream), "matvec");
                 check(lkd_pcg_update(m, d_rinf, sa, 6, d_p, d_q, d_dy, d_r, d_D, d_req, stream), "pcg update");
                 read_ks();
                 if (!(hk[6] > 0.0)) break;

@@ -119,6 +119,22 @@ public:
             check(lkd_memcpy_htod(p, v.data(), v.size() * sizeof(double), stream_), "upload");
             return p;
         };
+        // rows / columns longer than LKD_LONG_ROW go to whole blocks; the lane-group width of the
+        // other rows follows their mean length
+        auto long_list = [&](const std::vector<Index>& ptr, Index count, int32_t& nl, double& mean) {
+            std::vector<Index> idx;
+            int64_t short_nz = 0;
+            for (Index k = 0; k < count; ++k) {
+                const int64_t len = ptr[k + 1] - ptr[k];
+                if (len > LKD_LONG_ROW) idx.push_back(k);
+                else short_nz += len;
+            }
+            nl = static_cast<int32_t>(idx.size());
+            mean = static_cast<double>(short_nz) / std::max<int64_t>(1, count - nl);
+            return idx.empty() ? nullptr : up_i(idx);
+        };
+        double mean_row = 0, mean_col = 0;
+        P_.long_rows.idx = long_list(A_.row_ptrs, m_, P_.long_rows.n, mean_row);
         core::CSCMatrix Acsc(m_, n_, {}, {}, {});
         {   // A^T in CSR = CSC of the scaled A
             std::vector<Index> cp(static_cast<size_t>(n_) + 1, 0), ri(static_cast<size_t>(nz));
@@ -132,6 +148,7 @@ public:
                     ri[q] = i;
                     va[q] = A_.values[p];
                 }
+            P_.long_cols.idx = long_list(cp, n_, P_.long_cols.n, mean_col);
             P_.trp = up_i(cp);
             P_.tci = up_i(ri);
             P_.tval = up_d(va);
@@ -145,8 +162,8 @@ public:
         P_.c = up_d(c_);
         P_.R = up_d(R_);
         P_.C = up_d(C_);
-        P_.grp_rows = pow2_group(static_cast<double>(nz) / m_);
-        P_.grp_cols = pow2_group(static_cast<double>(nz) / n_);
+        P_.grp_rows = pow2_group(mean_row);
+        P_.grp_cols = pow2_group(mean_col);
         P_.nblk_rows = std::max(1, std::min<int>(nb_rows, static_cast<int>((static_cast<int64_t>(m_) * P_.grp_rows + 255) / 256)));
         P_.nblk_cols = std::max(1, std::min<int>(nb_cols, static_cast<int>((static_cast<int64_t>(n_) * P_.grp_cols + 255) / 256)));
         auto dv = [&](int64_t len) {

@@ -23,7 +23,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 TAGS = {"random_sparse": 1, "block_angular": 2, "transport": 3, "batch": 4,
-        "parametric": 5, "dense_small": 6}
+        "parametric": 5, "dense_small": 6, "dense_columns": 7}
 
 
 @dataclass
@@ -110,6 +110,27 @@ def random_sparse(m=2000, n=5000, k=6, seed=0) -> LP:
         return LP(m, n, indptr, indices, data, b, c, f"random_sparse_m{m}_n{n}_k{k}_s{seed}",
                   {"attempt": attempt})
     raise RuntimeError("random_sparse: no instance without empty rows")
+
+
+def dense_columns(m=600, n=1500, k=4, ndense=3, seed=0) -> LP:
+    """random_sparse-style columns plus `ndense` fully dense columns (entries N(0, 1/m))."""
+    import scipy.sparse as sp
+    for attempt in range(1000):
+        rng = _rng("dense_columns", seed, attempt)
+        rows = _distinct_rows(rng, m, n - ndense, k)
+        vals = rng.standard_normal((n - ndense, k))
+        indptr, indices, data = _csc_from_cols(m, n - ndense, rows, vals)
+        if _has_empty_row(m, indices):
+            continue
+        S = sp.csc_matrix((data, indices, indptr), shape=(m, n - ndense))
+        Dn = sp.csc_matrix(rng.standard_normal((m, ndense)) / np.sqrt(m))
+        A = sp.hstack([S, Dn]).tocsc()
+        A.sort_indices()
+        b, c = _standard_rhs(rng, A, m, n)
+        return LP(m, n, A.indptr.astype(np.int32), A.indices.astype(np.int32),
+                  A.data.astype(np.float64), b, c, f"dense_columns_m{m}_n{n}_k{k}_d{ndense}_s{seed}",
+                  {"attempt": attempt})
+    raise RuntimeError("dense_columns: no instance without empty rows")
 
 
 def block_angular(nblocks=64, mb=1500, nb=4000, k=8, nlink=50, dens=0.05, seed=0) -> LP:

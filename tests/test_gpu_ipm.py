@@ -31,3 +31,22 @@ def test_matches_oracle(gpu, oracle, m, n, k, seed):
     assert g["rel_gap"] <= 1e-8 and g["rel_primal_residual"] <= 1e-8 and g["rel_dual_residual"] <= 1e-8
     ok, _ = gpu.verify_optimal(lp, g["x"], g["y"], tol=1e-6)
     assert ok
+
+
+@pytest.mark.parametrize("case", ["linking_rows", "dense_columns"])
+def test_long_rows_and_columns_match_oracle(gpu, oracle, case):
+    """Rows / columns above the whole-block threshold (512 entries) and M0 entries summed from more
+    than 64 column pairs go through the block-per-row / block-per-segment paths."""
+    from lakshya import gen
+    if case == "linking_rows":
+        lp = gen.block_angular(4, 60, 200, 4, 5, 0.9, seed=7)      # 5 linking rows of ~720 entries
+        assert np.diff(lp.csr()[0]).max() > 512
+    else:
+        lp = gen.dense_columns(600, 1500, 4, 3, seed=8)            # 3 columns of 600 entries
+        assert np.diff(lp.indptr).max() > 512
+    g = gpu.ipm_solve(lp, relative=True)
+    o = oracle.ipm_solve(lp.csr(), lp.b, lp.c, lp.n, stop_rule=1)
+    assert g["status_name"] == "Optimal" and o["status"] == 0
+    assert abs(g["objective"] - o["objective"]) <= 1e-8 * max(1.0, abs(o["objective"]))
+    ok, _ = gpu.verify_optimal(lp, g["x"], g["y"], tol=1e-8)
+    assert ok
