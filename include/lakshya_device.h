@@ -54,18 +54,25 @@ int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream);
 
 /* ---------------------------------------------------------------- supernodal plan (device) */
 /* One work item of the factorization / triangular-solve schedule.  Field meaning per kernel:
- *   off   panel offset (elements) of the supernode, ld = panel rows, ncol = panel columns
- *   k0/kb block column start / width inside the panel
- *   r0/c0 tile row / column start inside the panel (kernel specific)
- *   sn    supernode id, first = first (permuted) column of the supernode                   */
+ *   off    panel offset (elements) of the supernode, ld = panel rows, ncol = panel columns
+ *   k0/kb  block column start / width inside the panel
+ *   r0/c0  tile row / column start inside the panel (kernel specific)
+ *   kbeg/kend  K range of a GEMM task (COLUPD, UPDATE)
+ *   ws     COLUPD: workspace offset of its split-K partial tile (-1: update the panel in place);
+ *          DIAG / ROWS / DIAGWB: staging slot offset
+ *   nsplit DIAG / ROWS: partial tiles per 128-row tile (0: none), read from ws + aux2
+ *   sn     supernode id, first = first (permuted) column of the supernode                   */
 typedef struct {
     int64_t off;
-    int64_t aux;     /* diagonal-copy offset (POTRF/TRSM/SYRK/TRSV) */
-    int64_t aux2;    /* offset of the supernode's row list (UPDATE/TRSV) */
+    int64_t aux;     /* diagonal-copy offset (DIAG / DIAGWB / TRSV) */
+    int64_t aux2;    /* row-list offset (UPDATE / TRSV); DIAG / ROWS: partials of tile 0 */
+    int64_t ws;
     int32_t ld, ncol;
     int32_t k0, kb;
     int32_t r0, c0;
-    int32_t sn, first;
+    int32_t kbeg, kend;
+    int32_t nsplit, sn;
+    int32_t first, pad;
 } lkd_task;
 
 typedef struct {
@@ -113,15 +120,26 @@ int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int6
 
 /* ---------------------------------------------------------------- supernodal Cholesky */
 /* Replaces cholesky_level_kernel and its level loop (src/cuda/kkt.cu:217-285,582-596).
- * err: device int, set non-zero when a pivot is not positive; kernels return early once set. */
-/* diag: row-major copies of the factored diagonal blocks (task.aux = element offset), with the
- * reciprocal on the diagonal; written by POTRF / SYRK (next diagonal), read by TRSM and TRSV. */
-int lkd_chol_potrf(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
-                   int32_t* err, void* stream);
-int lkd_chol_trsm(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
-                  int32_t* err, void* stream);
-int lkd_chol_syrk(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
-                  int32_t* err, void* stream);
+ * Left-looking per 64-column block column (see the task fields above), per step:
+ *   COLUPD  GEMM update of the block column (split K: partial tiles into ws)
+ *   DIAG    factor the diagonal block (ws = staging slot of LKD_STAGE elements, aux2 = partials)
+ *   ROWS    solve 128-row tiles below the diagonal against the staged factor
+ * then UPDATE (ancestors) per level and DIAGWB (diagonal blocks into the panel) at the end.
+ * err: device int, bit 0 set when a pivot is not positive, bit 1 when an update falls outside
+ * the L structure; kernels return early once set.
+ * diag: row-major copies of the factored diagonal blocks (task.aux = element offset), with the
+ * reciprocal on the diagonal; written by DIAG, read by TRSV.  piv: the pivots L_jj per permuted
+ * column (panel precision).  ws: split-K workspace; stage: staged diagonal factors (both panel
+ * precision). */
+#define LKD_STAGE (64 * 64 + 4 * 256)
+int lkd_chol_colupd(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* ws,
+                    int32_t* err, void* stream);
+int lkd_chol_diag(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* diag,
+                  void* piv, const void* ws, void* stage, int32_t* err, void* stream);
+int lkd_chol_rows(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* ws,
+                  const void* stage, int32_t* err, void* stream);
+int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* diag,
+                    const void* piv, void* stream);
 int lkd_chol_update(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
                     lkd_snplan plan, int32_t* err, void* stream);
 
@@ -249,9 +267,12 @@ int lkd_ipm_init(int32_t m, int32_t n, double* x, double* y, double* s, void* st
 /* ---------------------------------------------------------------- probes (tests) */
 /* One 16x16x4 MFMA (FP64 or FP32) on 64 lanes: lane l supplies a[l], b[l]; out[l*4+e] = D[e]. */
 int lkd_probe_mfma(int prec, const void* a, const void* b, void* out, void* stream);
-/* out (64x64, column-major) = A (64 x K, ld lda, ra valid rows) * B (64 x K, ldb, rb rows)^T */
+/* out (128x64, column-major) = A (ra x K, ld lda, ra <= 128) * B (rb x K, ldb, rb <= 64)^T */
 int lkd_probe_tile_gemm(int prec, const void* a, int64_t lda, int ra, const void* b, int64_t ldb,
                         int rb, int K, void* out, void* stream);
+/* in-place Cholesky of one 64x64 column-major SPD block (lower part returned, upper zeroed);
+ * *ok = 0 when a pivot is not positive */
+int lkd_probe_potrf(int prec, void* a, int32_t* ok, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,121 +1,246 @@
-// Supernodal Cholesky tile kernels for gfx950 (replace cholesky_level_kernel and its per-level
+// Supernodal Cholesky kernels for gfx950 (replace cholesky_level_kernel and its per-level
 // launch loop, src/cuda/kkt.cu:217-285,582-596, and the repeated refactorization in
 // gpu_cholesky_solve_device, src/cuda/kkt.cu:671-689).
 //
 // Storage: every supernode owns a column-major panel (ld = rows of its structure, ncol columns)
-// at panel + off.  The host plan drives a right-looking blocked factorization per etree level:
-//   POTRF  first 64-column diagonal block of each supernode (one workgroup, LDS)
-//   TRSM   64..256-row tiles of the current block column:  X <- X * Lkk^{-T}
-//   SYRK   64x64 trailing tiles  C -= B_r * B_c^T  on MFMA (16x16x4 FP64 / FP32); the workgroup
-//          owning the next diagonal tile also factors it (look-ahead, no extra launch)
-//   UPDATE below-diagonal rows:  U = L21 * L21^T  scattered (atomic add) into ancestor panels
-// All workgroups are 256 threads (4 wavefronts of 64).
+// at panel + off.  The host plan drives a left-looking blocked factorization per etree level,
+// one step per 64-column block column k (columns k0 .. k0+kb):
+//   COLUPD   C(r0:r0+128, k0:k0+kb) -= L(r0:r0+128, kbeg:kend) * L(k0:k0+kb, kbeg:kend)^T on MFMA
+//            (16x16x4 FP64 / FP32), double-buffered through LDS.  With split K the partial
+//            products go to a workspace instead of the panel.
+//   DIAG     one workgroup per supernode factors the updated 64x64 diagonal block (panel minus
+//            partials) in LDS (blocked by 16, MFMA) and publishes Lkk, the inverses of its 16x16
+//            diagonal blocks, the row-major copy with reciprocal diagonal and the pivots.  The
+//            panel's diagonal block is not written (DIAGWB does that at the end).
+//   ROWS     128-row tiles: X <- (C - partials) * Lkk^{-T} on MFMA (transposed blocked solve).
+//   DIAGWB   after the last step, Lkk of every block column is copied into the panel.
+//   UPDATE   below-diagonal rows:  U = L21 * L21^T (split K) scattered (atomic add) into the
+//            ancestor panels.
+// All workgroups are 256 threads (4 wavefronts of 64).  Task i of a launch runs on workgroup
+// (i % per) * 8 + i / per (per = grid / 8): consecutive tasks -- the tiles of one supernode --
+// share an XCD and its L2.
 #include "common.hpp"
 
 namespace {
 
-constexpr int TS = 64;   // tile edge
-constexpr int DP = 65;   // LDS pitch of a 64x64 tile (one element of padding)
+constexpr int TS = 64;        // block-column width, diagonal block edge
+constexpr int DP = 65;        // LDS pitch of a 64x64 block
+constexpr int MT = 128;       // GEMM tile rows
+constexpr int NC = 64;        // GEMM tile columns
+constexpr int PA = MT + 8;    // LDS pitch of the A chunk (k-major)
+constexpr int PB = NC + 8;    // LDS pitch of the B chunk
 
-template <typename T> struct KChunk { static constexpr int v = 32; };
-template <> struct KChunk<float> { static constexpr int v = 64; };
+template <typename T> struct KChunk;
+template <> struct KChunk<double> { static constexpr int v = 16; };
+template <> struct KChunk<float> { static constexpr int v = 32; };
 
-template <typename T> constexpr int gemm_lds_elems() {
-    return (2 * KChunk<T>::v * TS > TS * DP) ? 2 * KChunk<T>::v * TS : TS * DP;
+template <typename T> struct GemmLds {
+    T a[2][KChunk<T>::v * PA];
+    T b[2][KChunk<T>::v * PB];
+};
+static_assert(sizeof(GemmLds<double>) >= 64 * DP * sizeof(double), "UPDATE staging tile");
+static_assert(sizeof(GemmLds<float>) >= 64 * DP * sizeof(float), "UPDATE staging tile");
+
+__device__ __forceinline__ int task_index() {
+    const int per = gridDim.x >> 3;   // grid is a multiple of 8
+    return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
 }
 
-// acc (this wave's 2x2 blocks of 16x16 in the 64x64 tile) += sum_k A(r,k) * B(c,k)
-// A(r,k) = pa[k*lda + r] (r < ra), B(c,k) = pb[k*ldb + c] (c < rb); other rows read as zero.
+// Error flag read once per workgroup: every thread takes the same early exit.
+__device__ __forceinline__ bool wg_error(const int32_t* err, int* s) {
+    if (threadIdx.x == 0) *s = *(volatile const int32_t*)err;
+    __syncthreads();
+    return *s != 0;
+}
+
+template <typename T> using Acc = typename Mfma<T>::acc;
+
 template <typename T>
-__device__ __forceinline__ void tile_gemm_nt(const T* __restrict__ pa, int64_t lda, int ra,
-                                             const T* __restrict__ pb, int64_t ldb, int rb,
-                                             int K, T* lds, typename Mfma<T>::acc (&acc)[2][2]) {
-    constexpr int KC = KChunk<T>::v;
-    T* As = lds;
-    T* Bs = lds + KC * TS;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
-    const int li = lane & 15, lk = lane >> 4;
-    const int lr = t & 63, lq = t >> 6;
-    for (int kc = 0; kc < K; kc += KC) {
-        __syncthreads();
+__device__ __forceinline__ void zero_acc(Acc<T> (&acc)[2][4]) {
 #pragma unroll
-        for (int q = 0; q < KC / 4; ++q) {
-            const int k = lq + 4 * q;
-            const int kg = kc + k;
-            As[k * TS + lr] = (lr < ra && kg < K) ? pa[(int64_t)kg * lda + lr] : T(0);
-            Bs[k * TS + lr] = (lr < rb && kg < K) ? pb[(int64_t)kg * ldb + lr] : T(0);
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = Acc<T>{0, 0, 0, 0};
+}
+
+// acc (wave w: tile rows 32w..32w+31, all 64 columns) += sum_{k in [kbeg,kend)} A(r,k) B(c,k)
+// A(r,k) = pa[k*lda + r] (r < ra), B(c,k) = pb[k*ldb + c] (c < rb); other rows read as zero.
+// Chunks of KC columns: the next chunk is loaded into registers while the MFMAs consume the
+// current one from LDS; one barrier per chunk.
+template <typename T>
+__device__ __forceinline__ void gemm_nt(const T* __restrict__ pa, int64_t lda, int ra,
+                                        const T* __restrict__ pb, int64_t ldb, int rb, int kbeg,
+                                        int kend, GemmLds<T>& sm, Acc<T> (&acc)[2][4]) {
+    constexpr int KC = KChunk<T>::v;
+    constexpr int NA = KC * MT / 256, NB = KC * NC / 256;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int ar = t & (MT - 1), ak = t >> 7;
+    const int br = t & (NC - 1), bk = t >> 6;
+    const bool arow = ar < ra, brow = br < rb;
+    const int li = lane & 15, lk = lane >> 4;
+    T va[NA], vb[NB];
+    auto load = [&](int kc) {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int k = kc + ak + 2 * q;
+            va[q] = (arow && k < kend) ? pa[(int64_t)k * lda + ar] : T(0);
         }
-        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int k = kc + bk + 4 * q;
+            vb[q] = (brow && k < kend) ? pb[(int64_t)k * ldb + br] : T(0);
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) sm.a[buf][(ak + 2 * q) * PA + ar] = va[q];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) sm.b[buf][(bk + 4 * q) * PB + br] = vb[q];
+    };
+    const int nch = (kend - kbeg + KC - 1) / KC;
+    if (nch <= 0) return;
+    load(kbeg);
+    store(0);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        const int buf = c & 1;
+        if (c + 1 < nch) load(kbeg + (c + 1) * KC);
+        const T* As = sm.a[buf];
+        const T* Bs = sm.b[buf];
 #pragma unroll
         for (int k = 0; k < KC; k += 4) {
-            const T a0 = As[(k + lk) * TS + wr + li], a1 = As[(k + lk) * TS + wr + 16 + li];
-            const T b0 = Bs[(k + lk) * TS + wc + li], b1 = Bs[(k + lk) * TS + wc + 16 + li];
-            acc[0][0] = Mfma<T>::mma(a0, b0, acc[0][0]);
-            acc[0][1] = Mfma<T>::mma(a0, b1, acc[0][1]);
-            acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
-            acc[1][1] = Mfma<T>::mma(a1, b1, acc[1][1]);
-        }
-    }
-    __syncthreads();
-}
-
-// Accumulators -> LDS tile Cs[c * DP + r].
-template <typename T>
-__device__ __forceinline__ void stage_acc(typename Mfma<T>::acc (&acc)[2][2], T* Cs) {
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+            const T a0 = As[(k + lk) * PA + 32 * w + li];
+            const T a1 = As[(k + lk) * PA + 32 * w + 16 + li];
+            T b[4];
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+            for (int y = 0; y < 4; ++y) b[y] = Bs[(k + lk) * PB + 16 * y + li];
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int r = wr + 16 * x + Mfma<T>::row(lane, e);
-                const int c = wc + 16 * y + (lane & 15);
-                Cs[c * DP + r] = acc[x][y][e];
+            for (int y = 0; y < 4; ++y) {
+                acc[0][y] = Mfma<T>::mma(a0, b[y], acc[0][y]);
+                acc[1][y] = Mfma<T>::mma(a1, b[y], acc[1][y]);
             }
-    __syncthreads();
-}
-
-template <typename T>
-__device__ __forceinline__ void zero_acc(typename Mfma<T>::acc (&acc)[2][2]) {
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = typename Mfma<T>::acc{0, 0, 0, 0};
-}
-
-// In-place right-looking Cholesky of the kb x kb lower block Ds[c*DP + r] (kb <= 64).
-// Thread (r = t % 64, cg = t / 64) owns rows r of columns cg, cg+4, ...  One barrier per column:
-// column j is scaled during iteration j+1, when nobody reads it any more.
-// Returns false (uniformly across the workgroup) when a pivot is not positive.
-template <typename T>
-__device__ bool lds_potrf(T* Ds, int kb) {
-    const int t = threadIdx.x;
-    const int r = t & 63, cg = t >> 6;
-    T sd_prev = T(1);
-    for (int j = 0; j < kb; ++j) {
+        }
+        if (c + 1 < nch) store(buf ^ 1);
         __syncthreads();
-        const T d = Ds[j * DP + j];
-        if (!(d > T(0))) return false;
-        if (j > 0 && ((j - 1) & 3) == cg && r >= j - 1 && r < kb) {
-            Ds[(j - 1) * DP + r] = (r == j - 1) ? sd_prev : Ds[(j - 1) * DP + r] / sd_prev;
-        }
-        const T invd = T(1) / d;
-        if (r < kb && r > j) {
-            const T arj = Ds[j * DP + r] * invd;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int k = cg + 4 * q;
-                if (k > j && k <= r) Ds[k * DP + r] -= arj * Ds[j * DP + k];
-            }
-        }
-        sd_prev = sqrt(d);
     }
-    __syncthreads();
-    if (kb > 0 && ((kb - 1) & 3) == cg && r == kb - 1) Ds[(kb - 1) * DP + r] = sd_prev;
-    __syncthreads();
+}
+
+// f(row, col, value) for every accumulator element of this thread (tile coordinates).
+template <typename T, typename F>
+__device__ __forceinline__ void for_acc(Acc<T> (&acc)[2][4], F&& f) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                f(32 * w + 16 * x + Mfma<T>::row(lane, e), 16 * y + (lane & 15), acc[x][y][e]);
+}
+
+// Wave-uniform broadcast of lane `src`'s value (src is a compile-time constant at every use).
+__device__ __forceinline__ double bcast_lane(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float bcast_lane(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// One wavefront factors the 16x16 block at Ds(o, o) and inverts its factor: lane r (mod 16)
+// holds row r in registers, columns travel by readlane (no LDS round trips, no barriers).
+// Writes L (lower) back to Ds and L^{-1} to Lv (column-major, pitch 16, upper part zero).
+// Returns false (wave-uniform) when a pivot is not positive.
+template <typename T>
+__device__ __forceinline__ bool potrf16_wave(T* Ds, int o, T* Lv) {
+    const int l = threadIdx.x & 63, r = l & 15;
+    T a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = Ds[(o + k) * DP + o + r];
+    bool ok = true;
+    T invd = T(0);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const T d = bcast_lane(a[j], j);
+        ok = ok && (d > T(0));
+        const T sd = sqrt(d);
+        const T is = T(1) / sd;
+        a[j] = r == j ? sd : a[j] * is;          // r > j: L[r][j]; r < j: unused upper part
+        if (r == j) invd = is;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (k > j) a[k] = fma(-a[j], bcast_lane(a[j], k), a[k]);
+    }
+    // column c = r of L^{-1}: x[i] = (delta_ic - sum_{k<i} L[i][k] x[k]) / L[i][i]
+    T x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        T acc = i == r ? T(1) : T(0);
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (k < i) acc = fma(-bcast_lane(a[k], i), x[k], acc);
+        x[i] = acc * bcast_lane(invd, i);
+    }
+    if (l < 16) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (k <= r) Ds[(o + k) * DP + o + r] = a[k];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) Lv[r * 16 + i] = i >= r ? x[i] : T(0);
+    }
+    return ok;
+}
+
+// In-place Cholesky of the 64x64 block Ds[c*DP + r] (lower part; rows/columns past the valid
+// block must hold the identity), blocked by 16: wave 0 factors and inverts the diagonal block,
+// the 16-row panel below is solved on MFMA (L_ip^T = Linv_pp A_ip^T) and the trailing blocks
+// are updated on MFMA (A_ij -= L_ip L_jp^T).  Lv receives the four diagonal inverses
+// (Lv + 256 p, see potrf16_wave).  Three barriers per 16 columns.  Returns false, uniformly
+// across the workgroup, when a pivot is not positive.
+template <typename T>
+__device__ __forceinline__ bool potrf64(T* Ds, T* Lv, int* s_ok) {
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15;
+    using R = Mfma<T>;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int o = 16 * p;
+        if (w == 0) {
+            const bool ok = potrf16_wave<T>(Ds, o, Lv + 256 * p);
+            if (l == 0) *s_ok = ok ? 1 : 0;
+        }
+        __syncthreads();
+        if (!*s_ok) return false;
+        if (w < 3 - p) {                          // panel block i: L_ip^T = Linv_pp * A_ip^T
+            const int i = p + 1 + w;
+            Acc<T> d = Acc<T>{0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int k = R::row(l, s);
+                d = R::mma(Lv[256 * p + k * 16 + li], Ds[(o + k) * DP + 16 * i + li], d);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Ds[(o + R::row(l, e)) * DP + 16 * i + li] = d[e];
+        }
+        __syncthreads();
+        // trailing blocks (i, jj), p < jj <= i <= 3, round-robin over the waves
+        int b = 0;
+#pragma unroll
+        for (int i = p + 1; i < 4; ++i)
+#pragma unroll
+            for (int jj = p + 1; jj <= i; ++jj, ++b) {
+                if ((b & 3) != w) continue;
+                Acc<T> d = Acc<T>{0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int k = R::row(l, s);
+                    d = R::mma(Ds[(o + k) * DP + 16 * i + li], Ds[(o + k) * DP + 16 * jj + li], d);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) Ds[(16 * jj + li) * DP + 16 * i + R::row(l, e)] -= d[e];
+            }
+        __syncthreads();
+    }
     return true;
 }
 
@@ -133,134 +258,178 @@ __device__ __forceinline__ void write_diag_copy(const T* Ds, int kb, T* out) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf(const lkd_task* __restrict__ tasks,
-                                               T* __restrict__ L, T* __restrict__ diag,
-                                               int32_t* err) {
-    if (*(volatile int32_t*)err) return;
-    const lkd_task tk = tasks[blockIdx.x];
-    __shared__ T Ds[TS * DP];
+__global__ __launch_bounds__(256) void k_colupd(const lkd_task* __restrict__ tasks, int ntask,
+                                                T* __restrict__ L, T* __restrict__ ws,
+                                                const int32_t* err) {
+    const int ti = task_index();
+    if (ti >= ntask) return;
+    __shared__ int s_err;
+    if (wg_error(err, &s_err)) return;
+    const lkd_task tk = tasks[ti];
+    __shared__ GemmLds<T> sm;
     T* P = L + tk.off;
     const int64_t ld = tk.ld;
-    const int k0 = tk.k0, kb = tk.kb;
-    for (int idx = threadIdx.x; idx < TS * TS; idx += 256) {
-        const int r = idx & 63, c = idx >> 6;
-        if (r < kb && c < kb && r >= c) Ds[c * DP + r] = P[(int64_t)(k0 + c) * ld + k0 + r];
+    const int ra = min(MT, tk.ld - tk.r0), kb = tk.kb;
+    Acc<T> acc[2][4];
+    zero_acc<T>(acc);
+    gemm_nt<T>(P + tk.r0, ld, ra, P + tk.k0, ld, kb, tk.kbeg, tk.kend, sm, acc);
+    if (tk.ws < 0) {
+        T* C = P + (int64_t)tk.k0 * ld + tk.r0;
+        for_acc<T>(acc, [&](int r, int c, T v) {
+            if (r < ra && c < kb) C[(int64_t)c * ld + r] -= v;
+        });
+    } else {
+        T* W = ws + tk.ws;
+        for_acc<T>(acc, [&](int r, int c, T v) { W[c * MT + r] = v; });
     }
-    if (!lds_potrf<T>(Ds, kb)) {
-        if (threadIdx.x == 0) atomicOr(err, 1);
+}
+
+// Staging slot of a factored diagonal block: L (64x64 column-major, pitch 64, identity past kb)
+// followed by the inverses of its four 16x16 diagonal blocks (column-major, pitch 16).
+constexpr int STAGE = TS * TS + 4 * 256;
+
+// Rows r of the split-K partial tiles: tile g (rows k0 + 128 g ..), part j at
+// ws[aux2 + (g * nsplit + j) * MT * NC], column-major 128 x 64.
+template <typename T>
+__device__ __forceinline__ T minus_partials(T v, const T* __restrict__ W, int ns, int c, int r) {
+    for (int j = 0; j < ns; ++j) v -= W[(int64_t)j * MT * NC + c * MT + r];
+    return v;
+}
+
+// DIAG: factor the updated 64x64 diagonal block of block column k (panel minus partials) in LDS
+// and publish it: staging slot (ws offset), row-major copy with reciprocal diagonal (aux), pivots.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag(const lkd_task* __restrict__ tasks, int ntask,
+                                              const T* __restrict__ L, T* __restrict__ diag,
+                                              T* __restrict__ piv, const T* __restrict__ ws,
+                                              T* __restrict__ stage, int32_t* err) {
+    const int ti = task_index();
+    if (ti >= ntask) return;
+    __shared__ int s_err;
+    if (wg_error(err, &s_err)) return;
+    const lkd_task tk = tasks[ti];
+    __shared__ T Ds[TS * DP];
+    __shared__ T Lv[4 * 256];
+    __shared__ int s_ok;
+    const T* P = L + tk.off;
+    const int64_t ld = tk.ld;
+    const int k0 = tk.k0, kb = tk.kb, ns = tk.nsplit;
+    const int t = threadIdx.x;
+    const T* W = ws + tk.aux2;
+    for (int idx = t; idx < TS * TS; idx += 256) {
+        const int r = idx & 63, c = idx >> 6;
+        T v = r == c ? T(1) : T(0);
+        if (r < kb && c < kb)
+            v = r >= c ? minus_partials(P[(int64_t)(k0 + c) * ld + k0 + r], W, ns, c, r) : T(0);
+        Ds[c * DP + r] = v;
+    }
+    __syncthreads();
+    if (!potrf64<T>(Ds, Lv, &s_ok)) {
+        if (t == 0) atomicOr(err, 1);
         return;
     }
-    for (int idx = threadIdx.x; idx < TS * TS; idx += 256) {
+    T* S = stage + tk.ws;
+    for (int idx = t; idx < TS * TS; idx += 256) {
         const int r = idx & 63, c = idx >> 6;
-        if (r < kb && c < kb && r >= c) P[(int64_t)(k0 + c) * ld + k0 + r] = Ds[c * DP + r];
+        S[idx] = r >= c ? Ds[c * DP + r] : T(0);
     }
+    for (int idx = t; idx < 4 * 256; idx += 256) S[TS * TS + idx] = Lv[idx];
+    if (t < kb) piv[tk.first + k0 + t] = Ds[t * DP + t];
     write_diag_copy<T>(Ds, kb, diag + tk.aux);
 }
 
-// X (256 rows starting at r0, columns k0..k0+kb) <- X * Lkk^{-T}: one thread per row, forward
-// substitution with the unknowns in registers.  Lkk comes from the row-major diagonal copy
-// (uniform addresses -> scalar loads, so the coefficients never occupy vector registers).
-template <typename T, int KB>
-__device__ __forceinline__ void trsm_rows(const T* __restrict__ Lr, T* __restrict__ P, int64_t ld,
-                                          int k0, int64_t row, bool valid) {
-    T x[KB];
-#pragma unroll
-    for (int k = 0; k < KB; ++k) x[k] = valid ? P[(int64_t)(k0 + k) * ld + row] : T(0);
-#pragma unroll
-    for (int j = 0; j < KB; ++j) {
-        T s0 = x[j], s1 = T(0);
-#pragma unroll
-        for (int k = 0; k < j; k += 2) {
-            s0 -= x[k] * Lr[j * KB + k];
-            if (k + 1 < j) s1 -= x[k + 1] * Lr[j * KB + k + 1];
-        }
-        x[j] = (s0 + s1) * Lr[j * KB + j];
-    }
-    if (valid) {
-#pragma unroll
-        for (int k = 0; k < KB; ++k) P[(int64_t)(k0 + k) * ld + row] = x[k];
-    }
-}
-
+// ROWS: X <- (C - partials) Lkk^{-T} for rows max(r0, k0+kb) .. r0+127 of block column k, done
+// transposed on MFMA: Y = X^T satisfies Lkk Y = X^T, so by 16-row blocks of Y,
+// Y_p = Linv_pp (X_p^T - sum_{q<p} L_pq Y_q).  Each wavefront owns 32 rows of X (two 16-column
+// blocks of Y); an accumulator tile D (element e of lane l = D[row(l,e)][l%16]) is exactly the
+// B operand the next products need, so Y stays in registers.  The A operands (blocks of Lkk and
+// the inverses) are read straight from the staging slot.
 template <typename T>
-__device__ void trsm_rows_dyn(const T* __restrict__ Lr, int kb, T* __restrict__ P, int64_t ld,
-                              int k0, int64_t row, bool valid) {
-    // generic width (kb < 64, last block column): unknowns kept in the panel itself
-    if (!valid) return;
-    for (int j = 0; j < kb; ++j) {
-        T s = P[(int64_t)(k0 + j) * ld + row];
-        for (int k = 0; k < j; ++k) s -= P[(int64_t)(k0 + k) * ld + row] * Lr[j * kb + k];
-        P[(int64_t)(k0 + j) * ld + row] = s * Lr[j * kb + j];
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsm(const lkd_task* __restrict__ tasks,
-                                              T* __restrict__ L, const T* __restrict__ diag,
+__global__ __launch_bounds__(256) void k_rows(const lkd_task* __restrict__ tasks, int ntask,
+                                              T* __restrict__ L, const T* __restrict__ ws,
+                                              const T* __restrict__ stage,
                                               const int32_t* err) {
-    if (*(volatile const int32_t*)err) return;
-    const lkd_task tk = tasks[blockIdx.x];
+    const int ti = task_index();
+    if (ti >= ntask) return;
+    __shared__ int s_err;
+    if (wg_error(err, &s_err)) return;
+    const lkd_task tk = tasks[ti];
+    using R = Mfma<T>;
     T* P = L + tk.off;
     const int64_t ld = tk.ld;
-    const int64_t row = (int64_t)tk.r0 + threadIdx.x;
-    const bool valid = row < ld;
-    const T* Lr = diag + tk.aux;
-    if (tk.kb == TS) trsm_rows<T, TS>(Lr, P, ld, tk.k0, row, valid);
-    else trsm_rows_dyn<T>(Lr, tk.kb, P, ld, tk.k0, row, valid);
+    const int k0 = tk.k0, kb = tk.kb, ns = tk.nsplit;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15;
+    const T* S = stage + tk.ws;
+    const T* Wg = ws + tk.aux2 + (int64_t)((tk.r0 - k0) / MT) * ns * MT * NC;
+    const int kbm = kb - 1;
+    int64_t row[2];
+    bool valid[2];
+    Acc<T> y[2][4];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        row[b] = (int64_t)tk.r0 + 32 * w + 16 * b + li;
+        valid[b] = row[b] >= k0 + kb && row[b] < ld;
+        const int64_t rc = valid[b] ? row[b] : tk.r0;     // a row inside the panel
+        const int rr = (int)(rc - tk.r0);
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int c = min(16 * p + R::row(l, e), kbm);   // columns past kb: padding
+                y[b][p][e] = minus_partials(P[(int64_t)(k0 + c) * ld + rc], Wg, ns, c, rr);
+            }
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        Acc<T> tp[2] = {y[0][p], y[1][p]};
+#pragma unroll
+        for (int q = 0; q < p; ++q)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const T a = -S[(16 * q + R::row(l, s)) * TS + 16 * p + li];   // -L_pq[li][k]
+#pragma unroll
+                for (int b = 0; b < 2; ++b) tp[b] = R::mma(a, y[b][q][s], tp[b]);
+            }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) y[b][p] = Acc<T>{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const T a = S[TS * TS + 256 * p + R::row(l, s) * 16 + li];       // Linv_pp[li][k]
+#pragma unroll
+            for (int b = 0; b < 2; ++b) y[b][p] = R::mma(a, tp[b][s], y[b][p]);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        if (!valid[b]) continue;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int c = 16 * p + R::row(l, e);
+                if (c < kb) P[(int64_t)(k0 + c) * ld + row[b]] = y[b][p][e];
+            }
+    }
 }
 
-// Trailing update of one 64x64 tile (rows r0.., columns c0..) with block column k0..k0+kb.
-// The tile at (k0+kb, k0+kb) is the next diagonal block: its workgroup factors it afterwards.
+// Lkk of each task's block column into the panel: strictly lower part from the row-major copy,
+// the diagonal from the pivots (exact, not the stored reciprocal).
 template <typename T>
-__global__ __launch_bounds__(256) void k_syrk(const lkd_task* __restrict__ tasks,
-                                              T* __restrict__ L, T* __restrict__ diag,
-                                              int32_t* err) {
-    if (*(volatile int32_t*)err) return;
-    const lkd_task tk = tasks[blockIdx.x];
-    __shared__ T lds[gemm_lds_elems<T>()];
+__global__ __launch_bounds__(256) void k_diagwb(const lkd_task* __restrict__ tasks, int ntask,
+                                                T* __restrict__ L, const T* __restrict__ diag,
+                                                const T* __restrict__ piv) {
+    const int ti = task_index();
+    if (ti >= ntask) return;
+    const lkd_task tk = tasks[ti];
     T* P = L + tk.off;
     const int64_t ld = tk.ld;
-    const int ra = min(TS, tk.ld - tk.r0), rb = min(TS, tk.ncol - tk.c0);
-    typename Mfma<T>::acc acc[2][2];
-    zero_acc<T>(acc);
-    tile_gemm_nt<T>(P + (int64_t)tk.k0 * ld + tk.r0, ld, ra, P + (int64_t)tk.k0 * ld + tk.c0, ld,
-                    rb, tk.kb, lds, acc);
-    stage_acc<T>(acc, lds);
-    T* C = P + (int64_t)tk.c0 * ld + tk.r0;
-    const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
-    const bool next_diag = (tk.r0 == tk.c0) && (tk.c0 == tk.k0 + tk.kb);
-    if (!next_diag) {
-        if (r < ra) {
-#pragma unroll 4
-            for (int q = 0; q < 16; ++q) {
-                const int c = cg + 4 * q;
-                if (c < rb) C[(int64_t)c * ld + r] -= lds[c * DP + r];
-            }
-        }
-        return;
+    const int k0 = tk.k0, kb = tk.kb;
+    const T* Lr = diag + tk.aux;
+    for (int idx = threadIdx.x; idx < kb * kb; idx += 256) {
+        const int c = idx / kb, r = idx - c * kb;
+        if (r > c) P[(int64_t)(k0 + c) * ld + k0 + r] = Lr[r * kb + c];
+        else if (r == c) P[(int64_t)(k0 + c) * ld + k0 + r] = piv[tk.first + k0 + r];
     }
-    const int kb2 = rb;
-    if (r < ra) {
-        for (int q = 0; q < 16; ++q) {
-            const int c = cg + 4 * q;
-            if (c < rb) {
-                const T v = C[(int64_t)c * ld + r] - lds[c * DP + r];
-                if (r < kb2) lds[c * DP + r] = v;
-                else C[(int64_t)c * ld + r] = v;
-            }
-        }
-    }
-    if (!lds_potrf<T>(lds, kb2)) {
-        if (threadIdx.x == 0) atomicOr(err, 1);
-        return;
-    }
-    if (r < kb2) {
-        for (int q = 0; q < 16; ++q) {
-            const int c = cg + 4 * q;
-            if (c <= r) C[(int64_t)c * ld + r] = lds[c * DP + r];
-        }
-    }
-    write_diag_copy<T>(lds, kb2, diag + tk.aux);
 }
 
 __device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t lo, int64_t hi,
@@ -273,88 +442,119 @@ __device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t lo,
     return lo;
 }
 
-// Contribution of the below-diagonal rows of supernode sn to its ancestors:
-// U = L21[r0 rows] * L21[c0 rows]^T (K = ncol) subtracted from the ancestor panels.
+// Contribution of the below-diagonal rows of a supernode to its ancestors, K range
+// [kbeg, kend): U = L21[r0 rows] * L21[c0 rows]^T subtracted (atomic add) from the ancestor
+// panels.  The tile is staged through LDS 64 rows at a time for the scatter.
 template <typename T>
-__global__ __launch_bounds__(256) void k_update(const lkd_task* __restrict__ tasks,
+__global__ __launch_bounds__(256) void k_update(const lkd_task* __restrict__ tasks, int ntask,
                                                 T* __restrict__ L, lkd_snplan plan,
                                                 int32_t* err) {
-    if (*(volatile int32_t*)err) return;
-    const lkd_task tk = tasks[blockIdx.x];
-    __shared__ T lds[gemm_lds_elems<T>()];
-    __shared__ int64_t tbase[TS];
-    __shared__ int64_t trow0[TS];
-    __shared__ int64_t tlo[TS];
-    __shared__ int32_t tld[TS];
+    const int ti = task_index();
+    if (ti >= ntask) return;
+    __shared__ int s_err;
+    if (wg_error(err, &s_err)) return;
+    const lkd_task tk = tasks[ti];
+    __shared__ GemmLds<T> sm;
+    __shared__ int64_t tbase[NC];
+    __shared__ int64_t trow0[NC];
+    __shared__ int64_t tlo[NC];
+    __shared__ int32_t tld[NC];
     T* P = L + tk.off;
     const int64_t ld = tk.ld;
-    const int ra = min(TS, tk.ld - tk.r0), rb = min(TS, tk.ld - tk.c0);
-    typename Mfma<T>::acc acc[2][2];
+    const int ra = min(MT, tk.ld - tk.r0), rb = min(NC, tk.ld - tk.c0);
+    Acc<T> acc[2][4];
     zero_acc<T>(acc);
-    tile_gemm_nt<T>(P + tk.r0, ld, ra, P + tk.c0, ld, rb, tk.ncol, lds, acc);
-    stage_acc<T>(acc, lds);
+    gemm_nt<T>(P + tk.r0, ld, ra, P + tk.c0, ld, rb, tk.kbeg, tk.kend, sm, acc);
+    const int t = threadIdx.x, w = t >> 6;
     const int32_t* rows = plan.sn_rows + tk.aux2;
-    if (threadIdx.x < rb) {
-        const int32_t gj = rows[tk.c0 + threadIdx.x];
-        const int32_t t = plan.col_to_sn[gj];
-        const int32_t f = plan.sn_first[t];
-        const int64_t rp = plan.sn_rowptr[t];
-        const int32_t ldt = (int32_t)(plan.sn_rowptr[t + 1] - rp);
-        tbase[threadIdx.x] = plan.sn_panel[t] + (int64_t)(gj - f) * ldt;
-        trow0[threadIdx.x] = rp;
-        tlo[threadIdx.x] = gj - f;
-        tld[threadIdx.x] = ldt;
+    if (t < rb) {
+        const int32_t gj = rows[tk.c0 + t];
+        const int32_t s = plan.col_to_sn[gj];
+        const int32_t f = plan.sn_first[s];
+        const int64_t rp = plan.sn_rowptr[s];
+        const int32_t ldt = (int32_t)(plan.sn_rowptr[s + 1] - rp);
+        tbase[t] = plan.sn_panel[s] + (int64_t)(gj - f) * ldt;
+        trow0[t] = rp;
+        tlo[t] = gj - f;
+        tld[t] = ldt;
     }
-    __syncthreads();
-    const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
-    if (r >= ra) return;
-    const int32_t gi = rows[tk.r0 + r];
-    for (int q = 0; q < 16; ++q) {
-        const int c = cg + 4 * q;
-        if (c >= rb || tk.r0 + r < tk.c0 + c) continue;
-        const int32_t* trows = plan.sn_rows + trow0[c];
-        const int64_t pos = lower_bound_i32(trows, tlo[c], tld[c], gi);
-        if (pos >= tld[c] || trows[pos] != gi) {   // structure violated: report, never write
-            atomicOr(err, 2);
-            continue;
+    T* Cs = reinterpret_cast<T*>(&sm);   // 64 x DP staging tile (the GEMM buffers are free now)
+    for (int half = 0; half < 2; ++half) {
+        __syncthreads();
+        if ((w >> 1) == half)
+            for_acc<T>(acc, [&](int r, int c, T v) { Cs[c * DP + (r - 64 * half)] = v; });
+        __syncthreads();
+        const int rl = t & 63, cg = t >> 6, rr = 64 * half + rl;
+        if (rr >= ra) continue;
+        const int32_t gi = rows[tk.r0 + rr];
+        for (int q = 0; q < 16; ++q) {
+            const int c = cg + 4 * q;
+            if (c >= rb || tk.r0 + rr < tk.c0 + c) continue;
+            const int32_t* trows = plan.sn_rows + trow0[c];
+            const int64_t pos = lower_bound_i32(trows, tlo[c], tld[c], gi);
+            if (pos >= tld[c] || trows[pos] != gi) {   // structure violated: report, never write
+                atomicOr(err, 2);
+                continue;
+            }
+            atomic_add(L + tbase[c] + pos, -Cs[c * DP + rl]);
         }
-        atomic_add(L + tbase[c] + pos, -lds[c * DP + r]);
     }
 }
 
+int grid8(int32_t ntask) { return (ntask + 7) & ~7; }
+
+struct CholArgs {
+    void *panel, *diag, *piv, *ws, *stage;
+    const lkd_snplan* plan;
+    int32_t* err;
+};
+
 template <typename T>
-int launch_chol(int kind, int32_t ntask, const lkd_task* tasks, void* panel, void* diagbuf,
-                const lkd_snplan* plan, int32_t* err, void* stream) {
-    T* L = static_cast<T*>(panel);
-    T* D = static_cast<T*>(diagbuf);
+int launch_chol(int kind, int32_t ntask, const lkd_task* tasks, const CholArgs& a, void* stream) {
+    T* L = static_cast<T*>(a.panel);
     switch (kind) {
-        case 0: LK_LAUNCH(k_potrf<T>, ntask, 256, 0, stream, tasks, L, D, err); break;
-        case 1: LK_LAUNCH(k_trsm<T>, ntask, 256, 0, stream, tasks, L, D, err); break;
-        case 2: LK_LAUNCH(k_syrk<T>, ntask, 256, 0, stream, tasks, L, D, err); break;
-        default: LK_LAUNCH(k_update<T>, ntask, 256, 0, stream, tasks, L, *plan, err); break;
+        case 0: LK_LAUNCH(k_colupd<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.ws, a.err); break;
+        case 1: LK_LAUNCH(k_diag<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (const T*)a.ws, (T*)a.stage, a.err); break;
+        case 2: LK_LAUNCH(k_update<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, *a.plan, a.err); break;
+        case 3: LK_LAUNCH(k_diagwb<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (const T*)a.diag, (const T*)a.piv); break;
+        default: LK_LAUNCH(k_rows<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (const T*)a.ws, (const T*)a.stage, a.err); break;
     }
     return lk_last_error();
 }
 
-int dispatch(int prec, int kind, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
-             const lkd_snplan* plan, int32_t* err, void* stream) {
+int dispatch(int prec, int kind, int32_t ntask, const lkd_task* tasks, const CholArgs& a,
+             void* stream) {
     if (ntask <= 0) return 0;
-    return prec == LKD_F32
-               ? launch_chol<float>(kind, ntask, tasks, panel, diag, plan, err, stream)
-               : launch_chol<double>(kind, ntask, tasks, panel, diag, plan, err, stream);
+    return prec == LKD_F32 ? launch_chol<float>(kind, ntask, tasks, a, stream)
+                           : launch_chol<double>(kind, ntask, tasks, a, stream);
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_probe_gemm(const T* pa, int64_t lda, int ra, const T* pb,
                                                     int64_t ldb, int rb, int K, T* out) {
-    __shared__ T lds[gemm_lds_elems<T>()];
-    typename Mfma<T>::acc acc[2][2];
+    __shared__ GemmLds<T> sm;
+    Acc<T> acc[2][4];
     zero_acc<T>(acc);
-    tile_gemm_nt<T>(pa, lda, ra, pb, ldb, rb, K, lds, acc);
-    stage_acc<T>(acc, lds);
+    gemm_nt<T>(pa, lda, ra, pb, ldb, rb, 0, K, sm, acc);
+    for_acc<T>(acc, [&](int r, int c, T v) { out[c * MT + r] = v; });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_probe_potrf(T* a, int32_t* ok) {
+    __shared__ T Ds[TS * DP];
+    __shared__ T Lv[4 * 256];
+    __shared__ int s_ok;
     for (int idx = threadIdx.x; idx < TS * TS; idx += 256) {
         const int r = idx & 63, c = idx >> 6;
-        out[c * TS + r] = lds[c * DP + r];
+        Ds[c * DP + r] = a[c * TS + r];
+    }
+    __syncthreads();
+    const bool good = potrf64<T>(Ds, Lv, &s_ok);
+    if (threadIdx.x == 0) *ok = good ? 1 : 0;
+    if (!good) return;
+    for (int idx = threadIdx.x; idx < TS * TS; idx += 256) {
+        const int r = idx & 63, c = idx >> 6;
+        a[c * TS + r] = r >= c ? Ds[c * DP + r] : T(0);
     }
 }
 
@@ -362,9 +562,10 @@ __global__ __launch_bounds__(256) void k_probe_gemm(const T* pa, int64_t lda, in
 
 extern "C" {
 
-// out (64x64 column-major) = A(64 x K) * B(64 x K)^T with the tile GEMM of the SYRK/UPDATE kernels
+// out (128 x 64, column-major) = A(ra x K) * B(rb x K)^T with the GEMM core of COLUPD / UPDATE
 int lkd_probe_tile_gemm(int prec, const void* a, int64_t lda, int ra, const void* b, int64_t ldb,
                         int rb, int K, void* out, void* stream) {
+    if (ra < 0 || ra > MT || rb < 0 || rb > NC || K < 0) return (int)hipErrorInvalidValue;
     if (prec == LKD_F32)
         LK_LAUNCH(k_probe_gemm<float>, 1, 256, 0, stream, (const float*)a, lda, ra, (const float*)b, ldb, rb, K, (float*)out);
     else
@@ -372,22 +573,35 @@ int lkd_probe_tile_gemm(int prec, const void* a, int64_t lda, int ra, const void
     return lk_last_error();
 }
 
+// In-place Cholesky of one 64 x 64 column-major block with the DIAG factorization
+int lkd_probe_potrf(int prec, void* a, int32_t* ok, void* stream) {
+    if (prec == LKD_F32) LK_LAUNCH(k_probe_potrf<float>, 1, 256, 0, stream, (float*)a, ok);
+    else LK_LAUNCH(k_probe_potrf<double>, 1, 256, 0, stream, (double*)a, ok);
+    return lk_last_error();
+}
 
-int lkd_chol_potrf(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
-                   int32_t* err, void* stream) {
-    return dispatch(prec, 0, ntask, tasks, panel, diag, nullptr, err, stream);
+int lkd_chol_colupd(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* ws,
+                    int32_t* err, void* stream) {
+    return dispatch(prec, 0, ntask, tasks, {panel, nullptr, nullptr, ws, nullptr, nullptr, err}, stream);
 }
-int lkd_chol_trsm(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
-                  int32_t* err, void* stream) {
-    return dispatch(prec, 1, ntask, tasks, panel, diag, nullptr, err, stream);
+int lkd_chol_diag(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* diag,
+                  void* piv, const void* ws, void* stage, int32_t* err, void* stream) {
+    return dispatch(prec, 1, ntask, tasks,
+                    {const_cast<void*>(panel), diag, piv, const_cast<void*>(ws), stage, nullptr, err}, stream);
 }
-int lkd_chol_syrk(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
-                  int32_t* err, void* stream) {
-    return dispatch(prec, 2, ntask, tasks, panel, diag, nullptr, err, stream);
+int lkd_chol_rows(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* ws,
+                  const void* stage, int32_t* err, void* stream) {
+    return dispatch(prec, 4, ntask, tasks,
+                    {panel, nullptr, nullptr, const_cast<void*>(ws), const_cast<void*>(stage), nullptr, err}, stream);
 }
 int lkd_chol_update(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
                     lkd_snplan plan, int32_t* err, void* stream) {
-    return dispatch(prec, 3, ntask, tasks, panel, nullptr, &plan, err, stream);
+    return dispatch(prec, 2, ntask, tasks, {panel, nullptr, nullptr, nullptr, nullptr, &plan, err}, stream);
+}
+int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* diag,
+                    const void* piv, void* stream) {
+    return dispatch(prec, 3, ntask, tasks,
+                    {panel, const_cast<void*>(diag), const_cast<void*>(piv), nullptr, nullptr, nullptr, nullptr}, stream);
 }
 
 }  // extern "C"

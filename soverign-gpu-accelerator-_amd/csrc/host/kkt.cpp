@@ -32,12 +32,16 @@ void check(int e, const char* what) {
                                  lkd_error_string(e));
 }
 
-constexpr int TILE = 64;
-constexpr int TRSM_ROWS = 256;
+constexpr int TILE = 64;           // block-column width (diagonal blocks, substitution blocks)
+constexpr int TRSM_ROWS = 128;      // rows per ROWS workgroup (one GEMM tile)
+constexpr int GEMM_MT = 128;        // COLUPD / UPDATE tile rows
+constexpr int GEMM_NC = 64;         // COLUPD / UPDATE tile columns
+constexpr int GEMM_TARGET_WG = 1024;   // K splitting aims at this many GEMM workgroups per launch
+constexpr int KSPLIT_MIN = 128;     // minimum K columns per split part
 constexpr int FWD_ROWS = 256;
 
 struct Launch {
-    int kind;        // 0 potrf 1 trsm 2 syrk 3 update | 10 fwd-first 11 fwd-step 12 bwd-below 13 bwd-step
+    int kind;        // 0 colupd 1 diag 2 update 3 diagwb 4 rows | 10 fwd-first 11 fwd-step 12 bwd-below 13 bwd-step
     int64_t begin;
     int32_t count;
 };
@@ -96,6 +100,11 @@ struct GPUKKTCholeskySolver::Impl {
     void* d_panel = nullptr;       // sized for FP64
     void* d_diag = nullptr;
     int64_t diag_elems = 0;
+    void* d_ws = nullptr;          // split-K partial tiles (sized for FP64)
+    int64_t ws_elems = 0;
+    void* d_piv = nullptr;         // pivots L_jj per permuted column (sized for FP64)
+    void* d_stage = nullptr;       // staged diagonal factors, LKD_STAGE per supernode of a level
+    int64_t stage_elems = 0;
     int32_t* d_err = nullptr;
     double *d_w = nullptr, *d_D = nullptr, *d_m0d = nullptr, *d_theta = nullptr;
     double *d_r = nullptr, *d_req = nullptr, *d_z = nullptr, *d_dy = nullptr, *d_bak = nullptr,
@@ -155,6 +164,10 @@ struct GPUKKTCholeskySolver::Impl {
             up(sl, "slaunch");
             std::vector<int64_t> de{diag_elems};
             up(de, "diag_elems");
+            std::vector<int64_t> we{ws_elems};
+            up(we, "ws_elems");
+            std::vector<int64_t> se{stage_elems};
+            up(se, "stage_elems");
         }
         if (arena_p) allocate_numeric();
         st.m = m;
@@ -344,73 +357,127 @@ struct GPUKKTCholeskySolver::Impl {
             t.sn = s;
             t.first = S.sn_first[s];
             t.aux2 = S.sn_rowptr[s];
+            t.ws = -1;
             return t;
         };
         auto emit = [&](int kind, int64_t begin) {
             const int64_t cnt = static_cast<int64_t>(tasks.size()) - begin;
             if (cnt > 0) flaunch.push_back({kind, begin, static_cast<int32_t>(cnt)});
         };
+        // K split: enough workgroups to cover the CUs, at least KSPLIT_MIN columns per part
+        auto nsplit_for = [](int64_t tiles, int64_t K) {
+            if (tiles <= 0 || K <= 0) return 1;
+            const int64_t want = (GEMM_TARGET_WG + tiles - 1) / tiles;
+            return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({want, K / KSPLIT_MIN, 64})));
+        };
+        auto kpart = [](int K, int ns, int j) {   // split boundaries on multiples of 16
+            return static_cast<int32_t>((static_cast<int64_t>(K) * j / ns) / 16 * 16);
+        };
+        ws_elems = 0;
         for (const auto& lv : by_level) {
             int maxblk = 0;
-            int64_t b0 = static_cast<int64_t>(tasks.size());
             for (Index s : lv) {
-                lkd_task t = base(s);
-                t.k0 = 0;
-                t.kb = std::min(TILE, t.ncol);
-                t.aux = diag_off(s, 0);
-                tasks.push_back(t);
-                maxblk = std::max(maxblk, (t.ncol + TILE - 1) / TILE);
+                const Index nc = S.sn_first[s + 1] - S.sn_first[s];
+                maxblk = std::max<int>(maxblk, (nc + TILE - 1) / TILE);
             }
-            emit(0, b0);
             for (int k = 0; k < maxblk; ++k) {
-                b0 = static_cast<int64_t>(tasks.size());
+                // COLUPD: block column k of every supernode that has one, K = k * 64 columns
+                int64_t tiles = 0;
                 for (Index s : lv) {
-                    lkd_task t = base(s);
-                    const int nb = (t.ncol + TILE - 1) / TILE;
-                    if (k >= nb) continue;
+                    const lkd_task t = base(s);
+                    if (k == 0 || k * TILE >= t.ncol) continue;
+                    tiles += (t.ld - k * TILE + GEMM_MT - 1) / GEMM_MT;
+                }
+                const int ns = k == 0 ? 1 : nsplit_for(tiles, int64_t(k) * TILE);
+                std::vector<int64_t> wsbase(lv.size(), -1);
+                int64_t wsoff = 0;
+                int64_t b0 = static_cast<int64_t>(tasks.size());
+                for (size_t si = 0; si < lv.size() && k > 0; ++si) {
+                    lkd_task t = base(lv[si]);
+                    if (k * TILE >= t.ncol) continue;
                     t.k0 = k * TILE;
                     t.kb = std::min(TILE, t.ncol - t.k0);
-                    t.aux = diag_off(s, k);
-                    for (int r0 = t.k0 + t.kb; r0 < t.ld; r0 += TRSM_ROWS) {
+                    const int K = t.k0;
+                    if (ns > 1) wsbase[si] = wsoff;
+                    int g = 0;
+                    for (int r0 = t.k0; r0 < t.ld; r0 += GEMM_MT, ++g)
+                        for (int j = 0; j < ns; ++j) {
+                            lkd_task u = t;
+                            u.r0 = r0;
+                            u.kbeg = kpart(K, ns, j);
+                            u.kend = j + 1 == ns ? K : kpart(K, ns, j + 1);
+                            u.ws = ns > 1 ? wsoff + (int64_t(g) * ns + j) * GEMM_MT * GEMM_NC : -1;
+                            tasks.push_back(u);
+                        }
+                    if (ns > 1) wsoff += int64_t(g) * ns * GEMM_MT * GEMM_NC;
+                }
+                ws_elems = std::max(ws_elems, wsoff);
+                emit(0, b0);
+                // DIAG: factor the diagonal block of block column k into staging slot si
+                auto diag_task = [&](size_t si) {
+                    lkd_task t = base(lv[si]);
+                    t.k0 = k * TILE;
+                    t.kb = std::min(TILE, t.ncol - t.k0);
+                    t.aux = diag_off(lv[si], k);
+                    t.nsplit = wsbase[si] >= 0 ? ns : 0;
+                    t.aux2 = wsbase[si] >= 0 ? wsbase[si] : 0;
+                    t.ws = static_cast<int64_t>(si) * LKD_STAGE;
+                    t.r0 = t.k0;
+                    return t;
+                };
+                b0 = static_cast<int64_t>(tasks.size());
+                for (size_t si = 0; si < lv.size(); ++si)
+                    if (k * TILE < S.sn_first[lv[si] + 1] - S.sn_first[lv[si]]) tasks.push_back(diag_task(si));
+                emit(1, b0);
+                stage_elems = std::max<int64_t>(stage_elems, static_cast<int64_t>(lv.size()) * LKD_STAGE);
+                // ROWS: 128-row tiles below the diagonal block
+                b0 = static_cast<int64_t>(tasks.size());
+                for (size_t si = 0; si < lv.size(); ++si) {
+                    if (k * TILE >= S.sn_first[lv[si] + 1] - S.sn_first[lv[si]]) continue;
+                    const lkd_task t = diag_task(si);
+                    for (int r0 = t.k0; r0 < t.ld; r0 += TRSM_ROWS) {
+                        if (std::max(r0, t.k0 + t.kb) >= std::min(r0 + TRSM_ROWS, t.ld)) continue;   // no rows
                         lkd_task u = t;
                         u.r0 = r0;
                         tasks.push_back(u);
                     }
                 }
-                emit(1, b0);
-                b0 = static_cast<int64_t>(tasks.size());
-                for (Index s : lv) {
-                    lkd_task t = base(s);
-                    const int nb = (t.ncol + TILE - 1) / TILE;
-                    if (k + 1 >= nb) continue;
-                    t.k0 = k * TILE;
-                    t.kb = TILE;
-                    t.aux = diag_off(s, k + 1);
-                    for (int c0 = t.k0 + t.kb; c0 < t.ncol; c0 += TILE)
-                        for (int r0 = c0; r0 < t.ld; r0 += TILE) {
-                            lkd_task u = t;
-                            u.r0 = r0;
-                            u.c0 = c0;
-                            tasks.push_back(u);
-                        }
-                }
-                // the next-diagonal tiles first: they carry the longest critical path
-                std::stable_partition(tasks.begin() + b0, tasks.end(), [](const lkd_task& u) {
-                    return u.r0 == u.c0 && u.c0 == u.k0 + u.kb;
-                });
-                emit(2, b0);
+                emit(4, b0);
             }
-            b0 = static_cast<int64_t>(tasks.size());
+            // UPDATE: L21 L21^T of every supernode into its ancestors, split K
+            int64_t tiles = 0, maxK = 0;
+            for (Index s : lv) {
+                const lkd_task t = base(s);
+                if (t.ld <= t.ncol) continue;
+                for (int r0 = t.ncol; r0 < t.ld; r0 += GEMM_MT)
+                    for (int c0 = t.ncol; c0 < std::min(t.ld, r0 + GEMM_MT); c0 += GEMM_NC) ++tiles;
+                maxK = std::max<int64_t>(maxK, t.ncol);
+            }
+            const int64_t b0 = static_cast<int64_t>(tasks.size());
             for (Index s : lv) {
                 lkd_task t = base(s);
                 if (t.ld <= t.ncol) continue;
-                for (int c0 = t.ncol; c0 < t.ld; c0 += TILE)
-                    for (int r0 = c0; r0 < t.ld; r0 += TILE) {
-                        lkd_task u = t;
-                        u.r0 = r0;
-                        u.c0 = c0;
-                        tasks.push_back(u);
-                    }
+                const int ns = nsplit_for(tiles, t.ncol);
+                for (int r0 = t.ncol; r0 < t.ld; r0 += GEMM_MT)
+                    for (int c0 = t.ncol; c0 < std::min(t.ld, r0 + GEMM_MT); c0 += GEMM_NC)
+                        for (int j = 0; j < ns; ++j) {
+                            lkd_task u = t;
+                            u.r0 = r0;
+                            u.c0 = c0;
+                            u.kbeg = kpart(t.ncol, ns, j);
+                            u.kend = j + 1 == ns ? t.ncol : kpart(t.ncol, ns, j + 1);
+                            tasks.push_back(u);
+                        }
+            }
+            (void)maxK;
+            emit(2, b0);
+        }
+        // DIAGWB: every factored diagonal block into the panel (no kernel reads them before)
+        {
+            const int64_t b0 = static_cast<int64_t>(tasks.size());
+            for (const auto& L : flaunch) {
+                if (L.kind != 1) continue;
+                for (int64_t q = L.begin; q < L.begin + L.count; ++q) tasks.push_back(tasks[q]);
             }
             emit(3, b0);
         }
@@ -433,20 +500,36 @@ struct GPUKKTCholeskySolver::Impl {
         const int64_t ld = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
         const Index nc = S.sn_first[s + 1] - S.sn_first[s];
         if (t.ld != ld || t.ncol != nc || t.off != S.sn_panel[s] || t.first != S.sn_first[s] ||
-            t.aux2 != S.sn_rowptr[s])
+            (kind != 1 && kind != 3 && kind != 4 && t.aux2 != S.sn_rowptr[s]))   // aux2 = partials
             bad("geometry");
         if (S.sn_panel[s] + ld * nc > S.sn_panel[S.nsuper]) bad("panel extent");
         if (t.first + nc > m) bad("columns");
         if (t.k0 < 0 || t.kb < 0 || t.k0 + t.kb > nc || t.kb > TILE) bad("block column");
         auto diag_ok = [&](int64_t off, int kb) { return off >= 0 && off + int64_t(kb) * kb <= diag_elems; };
+        const int64_t tile = int64_t(GEMM_MT) * GEMM_NC;
         switch (kind) {
-            case 0: if (t.kb < 1 || !diag_ok(t.aux, t.kb)) bad("potrf"); break;
-            case 1: if (t.r0 < t.k0 + t.kb || t.r0 >= ld || !diag_ok(t.aux, t.kb)) bad("trsm"); break;
-            case 2:
-                if (t.c0 < t.k0 + t.kb || t.c0 >= nc || t.r0 < t.c0 || t.r0 >= ld) bad("syrk");
-                if (t.r0 == t.c0 && t.c0 == t.k0 + t.kb && !diag_ok(t.aux, std::min<int>(TILE, nc - t.c0))) bad("syrk diag");
+            case 0:   // COLUPD: rows r0.. of block column k0, K range inside the previous columns
+                if (t.kb < 1 || t.k0 < 1 || t.r0 < t.k0 || t.r0 >= ld) bad("colupd rows");
+                if (t.kbeg < 0 || t.kend > t.k0 || t.kbeg >= t.kend) bad("colupd K");
+                if (t.ws >= 0 && t.ws + tile > ws_elems) bad("colupd workspace");
                 break;
-            case 3: if (t.c0 < nc || t.c0 >= ld || t.r0 < t.c0 || t.r0 >= ld) bad("update"); break;
+            case 1:   // DIAG / DIAGWB: the diagonal block, its staging slot and partials
+            case 3:
+            case 4: { // ROWS: a 128-row tile with rows below the diagonal block
+                if (t.kb < 1 || t.r0 < t.k0 || t.r0 >= ld || (t.r0 - t.k0) % TRSM_ROWS) bad("rows");
+                if (kind != 4 && t.r0 != t.k0) bad("diag task");
+                if (kind == 4 && std::max(t.r0, t.k0 + t.kb) >= std::min<int64_t>(t.r0 + TRSM_ROWS, ld)) bad("rows tile");
+                if (kind != 4 && !diag_ok(t.aux, t.kb)) bad("diag copy");
+                if (t.ws < 0 || t.ws + LKD_STAGE > stage_elems) bad("staging slot");
+                if (t.nsplit < 0 || (t.nsplit > 0 && (t.aux2 < 0 ||
+                    t.aux2 + (int64_t(ld - t.k0 + GEMM_MT - 1) / GEMM_MT) * t.nsplit * tile > ws_elems)))
+                    bad("partials");
+                break;
+            }
+            case 2:   // UPDATE: tile of the L21 rows, lower triangle, K inside the columns
+                if (t.c0 < nc || t.c0 >= ld || t.r0 < nc || t.r0 >= ld || t.c0 >= t.r0 + GEMM_MT) bad("update");
+                if (t.kbeg < 0 || t.kend > nc || t.kbeg >= t.kend) bad("update K");
+                break;
             case 10: if (t.k0 != 0 || !diag_ok(t.aux, t.kb)) bad("fwd first"); break;
             case 11:
                 if (t.r0 < t.k0 + t.kb || t.r0 >= ld) bad("fwd step");
@@ -577,6 +660,12 @@ struct GPUKKTCholeskySolver::Impl {
         owned.push_back(d_panel);
         d_diag = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(diag_elems, 1)) * sizeof(double));
         owned.push_back(d_diag);
+        d_ws = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(ws_elems, 1)) * sizeof(double));
+        owned.push_back(d_ws);
+        d_piv = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<Index>(m, 1)) * sizeof(double));
+        owned.push_back(d_piv);
+        d_stage = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(stage_elems, 1)) * sizeof(double));
+        owned.push_back(d_stage);
         auto dv = [&](int64_t len) {
             double* p = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(len, 1)) * sizeof(double)));
             owned.push_back(p);
@@ -621,22 +710,24 @@ struct GPUKKTCholeskySolver::Impl {
     // -------------------------------------------------------------------- numeric phases
     void run_factor_schedule(int prec) {
         const lkd_snplan pl = plan();
-        for (const auto& L : flaunch) {
-            const lkd_task* t = d_ftasks + L.begin;
-            int e = 0;
-            switch (L.kind) {
-                case 0: e = lkd_chol_potrf(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
-                case 1: e = lkd_chol_trsm(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
-                case 2: e = lkd_chol_syrk(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
-                default: e = lkd_chol_update(prec, L.count, t, d_panel, pl, d_err, stream); break;
-            }
-            check(e, "factor kernel");
+        for (const auto& L : flaunch) check(launch_factor(prec, L, pl), "factor kernel");
+    }
+
+    int launch_factor(int prec, const Launch& L, const lkd_snplan& pl) {
+        const lkd_task* t = d_ftasks + L.begin;
+        switch (L.kind) {
+            case 0: return lkd_chol_colupd(prec, L.count, t, d_panel, d_ws, d_err, stream);
+            case 1: return lkd_chol_diag(prec, L.count, t, d_panel, d_diag, d_piv, d_ws, d_stage, d_err, stream);
+            case 2: return lkd_chol_update(prec, L.count, t, d_panel, pl, d_err, stream);
+            case 3: return lkd_chol_diagwb(prec, L.count, t, d_panel, d_diag, d_piv, stream);
+            default: return lkd_chol_rows(prec, L.count, t, d_panel, d_ws, d_stage, d_err, stream);
         }
     }
 
     // One factorization of the current M with HIP events around every launch: per launch kind
-    // (0 POTRF, 1 TRSM, 2 SYRK, 3 UPDATE) the summed device time, the launch count and the
-    // algorithmic flops of the tiles; slot 4 = whole factorization incl. assembly.
+    // (0 COLUPD, 1 DIAG + ROWS + DIAGWB, 2 UPDATE) the summed device time, the launch count and the
+    // algorithmic flops of the tasks; slot 3 = assembly (prep, diagonal, zero fill, pair sums);
+    // slot 4 = whole factorization incl. assembly (flops: sum over columns of L of c_j^2).
     void profile_factor(int prec, double ms[5], double flops[5], double launches[5]) {
         for (int k = 0; k < 5; ++k) ms[k] = flops[k] = launches[k] = 0;
         if (!have_theta) throw std::logic_error("profile_factor: no Theta");
@@ -648,29 +739,31 @@ struct GPUKKTCholeskySolver::Impl {
         check(lkd_event_create(&a1), "event");
         const size_t esz = prec == LKD_F32 ? sizeof(float) : sizeof(double);
         check(lkd_event_record(a0, stream), "event");
+        check(lkd_event_record(e0, stream), "event");
         check(lkd_assemble_prep(n, d_colptr, d_cval, d_theta, d_w, stream), "prep");
         check(lkd_assemble_diag(m, d_dseg, d_segptr, d_pa, d_pb, d_w, d_cval, 1, d_D, d_m0d, long_diag, stream), "diag");
         check(lkd_memset(d_panel, 0, static_cast<size_t>(S.sn_panel[S.nsuper]) * esz, stream), "zero");
         check(lkd_memset(d_err, 0, sizeof(int32_t), stream), "zero");
         check(lkd_assemble_pairs(prec, nseg, d_segptr, d_segdst, d_segi, d_segj, d_pa, d_pb, d_w,
                                  d_cval, d_D, d_empty, 1e-9, d_panel, long_segs, stream), "assemble");
+        check(lkd_event_record(e1, stream), "event");
+        {
+            float t_ms = 0;
+            check(lkd_event_elapsed_ms(&t_ms, e0, e1), "elapsed");
+            ms[3] = t_ms;
+            launches[3] = 4;
+            flops[3] = 2.0 * static_cast<double>(npairs);
+        }
         for (const auto& L : flaunch) {
-            const lkd_task* t = d_ftasks + L.begin;
             check(lkd_event_record(e0, stream), "event");
-            int e = 0;
-            switch (L.kind) {
-                case 0: e = lkd_chol_potrf(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
-                case 1: e = lkd_chol_trsm(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
-                case 2: e = lkd_chol_syrk(prec, L.count, t, d_panel, d_diag, d_err, stream); break;
-                default: e = lkd_chol_update(prec, L.count, t, d_panel, pl, d_err, stream); break;
-            }
-            check(e, "factor kernel");
+            check(launch_factor(prec, L, pl), "factor kernel");
             check(lkd_event_record(e1, stream), "event");
             float t_ms = 0;
             check(lkd_event_elapsed_ms(&t_ms, e0, e1), "elapsed");
-            ms[L.kind] += t_ms;
-            launches[L.kind] += 1;
-            for (int64_t q = L.begin; q < L.begin + L.count; ++q) flops[L.kind] += task_flops(h_ftasks[q], L.kind);
+            const int slot = L.kind >= 3 ? 1 : L.kind;   // ROWS and the write-back count with DIAG
+            ms[slot] += t_ms;
+            launches[slot] += 1;
+            for (int64_t q = L.begin; q < L.begin + L.count; ++q) flops[slot] += task_flops(h_ftasks[q], L.kind);
         }
         check(lkd_event_record(a1, stream), "event");
         float tot = 0;
@@ -682,20 +775,27 @@ struct GPUKKTCholeskySolver::Impl {
         have_eq = false;   // the profiled factor uses delta 1e-9 only; force a normal refactor next
     }
     std::vector<lkd_task> h_ftasks;
+    // algorithmic flops of one task (lower-triangle work only)
     static double task_flops(const lkd_task& t, int kind) {
         const double kb = t.kb;
         switch (kind) {
-            case 0: return kb * kb * kb / 3.0;
-            case 1: return static_cast<double>(std::min<int64_t>(TRSM_ROWS, t.ld - t.r0)) * kb * kb;
-            case 2: {
-                const double ra = std::min(TILE, t.ld - t.r0), rb = std::min(TILE, t.ncol - t.c0);
-                const double useful = (t.r0 == t.c0) ? rb * (rb + 1) / 2 + (ra - rb) * rb : ra * rb;
-                return 2.0 * useful * kb;
+            case 0: {
+                const double ra = std::min(GEMM_MT, t.ld - t.r0);
+                const double useful = t.r0 == t.k0 ? kb * (kb + 1) / 2 + (ra - kb) * kb : ra * kb;
+                return 2.0 * useful * (t.kend - t.kbeg);
             }
+            case 1: return kb * kb * kb / 3.0;
+            case 4: {
+                const double rows = static_cast<double>(std::min(TRSM_ROWS, t.ld - t.r0)) -
+                                    std::max(0.0, t.k0 + kb - static_cast<double>(t.r0));
+                return std::max(rows, 0.0) * kb * kb;
+            }
+            case 3: return 0.0;
             default: {
-                const double ra = std::min(TILE, t.ld - t.r0), rb = std::min(TILE, t.ld - t.c0);
-                const double useful = (t.r0 == t.c0) ? rb * (rb + 1) / 2 + (ra - rb) * rb : ra * rb;
-                return 2.0 * useful * t.ncol;
+                const double ra = std::min(GEMM_MT, t.ld - t.r0), rb = std::min(GEMM_NC, t.ld - t.c0);
+                double useful = 0;   // entries with row >= column
+                for (int c = 0; c < rb; ++c) useful += std::max(0.0, ra - std::max(0, t.c0 + c - t.r0));
+                return 2.0 * useful * (t.kend - t.kbeg);
             }
         }
     }

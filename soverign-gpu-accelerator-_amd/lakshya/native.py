@@ -336,7 +336,7 @@ class IPM:
         L.lk_ipm_profile_factor.argtypes = [_vp, C.c_int32, _f64p, _f64p, _f64p]
         ms, fl, la = np.zeros(5), np.zeros(5), np.zeros(5)
         _ck(L.lk_ipm_profile_factor(self.h, int(fp32), ms, fl, la))
-        names = ["potrf", "trsm", "syrk", "update", "factorization"]
+        names = ["gemm", "potrf_trsm", "update", "assembly", "factorization"]
         return {k: {"ms": float(ms[i]), "flops": float(fl[i]), "launches": int(la[i])}
                 for i, k in enumerate(names)}
 
@@ -373,9 +373,10 @@ def verify_optimal(lp, x, pi, tol=1e-6, lb=None, ub=None, maximize=False):
 
 
 # ------------------------------------------------------------------ plan inspection (host only)
-TASK_DTYPE = np.dtype([("off", "<i8"), ("aux", "<i8"), ("aux2", "<i8"), ("ld", "<i4"),
-                       ("ncol", "<i4"), ("k0", "<i4"), ("kb", "<i4"), ("r0", "<i4"),
-                       ("c0", "<i4"), ("sn", "<i4"), ("first", "<i4")])
+TASK_DTYPE = np.dtype([("off", "<i8"), ("aux", "<i8"), ("aux2", "<i8"), ("ws", "<i8"),
+                       ("ld", "<i4"), ("ncol", "<i4"), ("k0", "<i4"), ("kb", "<i4"),
+                       ("r0", "<i4"), ("c0", "<i4"), ("kbeg", "<i4"), ("kend", "<i4"),
+                       ("nsplit", "<i4"), ("sn", "<i4"), ("first", "<i4"), ("pad", "<i4")])
 PLAN_DTYPES = {"perm": np.int32, "sn_rowptr": np.int64, "sn_rows": np.int32,
                "sn_panel": np.int64, "col_to_sn": np.int32, "sn_first": np.int32,
                "flaunch": np.int64, "slaunch": np.int64, "diag_elems": np.int64,
@@ -383,7 +384,8 @@ PLAN_DTYPES = {"perm": np.int32, "sn_rowptr": np.int64, "sn_rows": np.int32,
                "pci": np.int32, "pval": np.float64, "empty_row": np.uint8, "segptr": np.int64,
                "segdst": np.int64, "segi": np.int32, "segj": np.int32, "pa": np.int32,
                "pb": np.int32, "dseg": np.int64, "ftasks": TASK_DTYPE, "stasks": TASK_DTYPE,
-               "csr_index": np.int64}
+               "csr_index": np.int64, "ws_elems": np.int64, "stage_elems": np.int64, "long_prows": np.int32,
+               "long_cols": np.int32, "long_segs": np.int32, "long_diag": np.int32}
 
 
 def plan_export(rp, ci, val, n, P=None, relax_small=-1, relax_frac=-1.0):

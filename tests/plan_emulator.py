@@ -15,6 +15,7 @@ class Emulator:
         self.sl = plan["slaunch"].reshape(-1, 3)
         self.panel = np.zeros(int(plan["sn_panel"][-1]))
         self.diag = np.zeros(max(int(plan["diag_elems"][0]), 1))
+        self.piv = np.zeros(len(plan["perm"]))
         self.err = 0
 
     def blk(self, t, r0, r1, c0, c1):
@@ -57,42 +58,67 @@ class Emulator:
         self.diag[off:off + kb * kb] = Lr.reshape(-1)
 
     def factor(self):
+        """Left-looking schedule: COLUPD (kind 0), DIAG (1), ROWS (4), UPDATE (2), DIAGWB (3).
+        DIAG stages Lkk and never writes the panel's diagonal block; the pivots and the
+        row-major copy carry it until DIAGWB."""
         p = self.p
         T = p["ftasks"]
+        MT, NC, ROWS = 128, 64, 128
+        ws = np.zeros(max(int(p["ws_elems"][0]), 1))
+        self.stage = {}
         for kind, b, c in self.fl:
             for t in T[b:b + c]:
                 if self.err:
                     return False
                 k0, kb, r0, c0, ld, nc = (int(t[x]) for x in ("k0", "kb", "r0", "c0", "ld", "ncol"))
+                kbeg, kend = int(t["kbeg"]), int(t["kend"])
+                assert kind in (0, 1, 2, 3, 4)
                 if kind == 0:
-                    D = self.blk(t, k0, k0 + kb, k0, k0 + kb)
-                    L = self._factor_block(np.tril(D) + np.tril(D, -1).T)
-                    if L is None:
-                        return False
-                    D[:] = np.where(np.tri(kb, dtype=bool), L, D)
-                    self._diag_copy(L, int(t["aux"]))
-                elif kind == 1:
-                    Lr = self.diag[int(t["aux"]):int(t["aux"]) + kb * kb].reshape(kb, kb)
-                    L = np.tril(Lr, -1) + np.diag(1.0 / np.diag(Lr))
-                    X = self.blk(t, r0, min(r0 + 256, ld), k0, k0 + kb)
-                    X[:] = np.linalg.solve(L, X.T).T
-                elif kind == 2:
-                    rb = min(TS, nc - c0)
-                    Bt = self.blk(t, r0, min(r0 + TS, ld), k0, k0 + kb)
-                    Bc = self.blk(t, c0, c0 + rb, k0, k0 + kb)
-                    C = self.blk(t, r0, min(r0 + TS, ld), c0, c0 + rb)
-                    C -= Bt @ Bc.T
-                    if r0 == c0 and c0 == k0 + kb:
-                        D = self.blk(t, c0, c0 + rb, c0, c0 + rb)
+                    ra = min(MT, ld - r0)
+                    prod = self.blk(t, r0, r0 + ra, kbeg, kend) @ self.blk(t, k0, k0 + kb, kbeg, kend).T
+                    if int(t["ws"]) < 0:
+                        self.blk(t, r0, r0 + ra, k0, k0 + kb)[:] -= prod
+                    else:
+                        W = np.zeros((NC, MT))
+                        W[:kb, :ra] = prod.T
+                        ws[int(t["ws"]):int(t["ws"]) + MT * NC] = W.reshape(-1)
+                elif kind in (1, 4):
+                    ns, base = int(t["nsplit"]), int(t["aux2"])
+
+                    def partial(rows):   # sum of the split-K partials for panel rows `rows`
+                        acc = np.zeros((len(rows), kb))
+                        for i, row in enumerate(rows):
+                            g, rr = divmod(row - k0, MT)
+                            for j in range(ns):
+                                o = base + (g * ns + j) * MT * NC
+                                acc[i] += ws[o:o + MT * NC].reshape(NC, MT)[:kb, rr]
+                        return acc
+                    slot = int(t["ws"])
+                    if kind == 1:        # DIAG: factor, stage, copy, pivots (panel untouched)
+                        D = self.blk(t, k0, k0 + kb, k0, k0 + kb).copy() - partial(range(k0, k0 + kb))
                         L = self._factor_block(np.tril(D) + np.tril(D, -1).T)
                         if L is None:
                             return False
-                        D[:] = np.where(np.tri(rb, dtype=bool), L, D)
+                        self.stage[slot] = L
+                        first = int(t["first"])
+                        self.piv[first + k0:first + k0 + kb] = np.diag(L)
                         self._diag_copy(L, int(t["aux"]))
+                    else:                # ROWS: solve the tile's rows below the diagonal block
+                        L = self.stage[slot]
+                        lo, hi = max(r0, k0 + kb), min(r0 + ROWS, ld)
+                        if hi > lo:
+                            X = self.blk(t, lo, hi, k0, k0 + kb)
+                            X[:] = np.linalg.solve(L, (X - partial(range(lo, hi))).T).T
+                elif kind == 3:
+                    first = int(t["first"])
+                    Lr = self.diag[int(t["aux"]):int(t["aux"]) + kb * kb].reshape(kb, kb)
+                    Lk = np.tril(Lr, -1) + np.diag(self.piv[first + k0:first + k0 + kb])
+                    Dp = self.blk(t, k0, k0 + kb, k0, k0 + kb)
+                    Dp[:] = np.where(np.tri(kb, dtype=bool), Lk, Dp)
                 else:
                     rows = p["sn_rows"][int(t["aux2"]):int(t["aux2"]) + ld]
-                    ra, rb = min(TS, ld - r0), min(TS, ld - c0)
-                    U = self.blk(t, r0, r0 + ra, 0, nc) @ self.blk(t, c0, c0 + rb, 0, nc).T
+                    ra, rb = min(MT, ld - r0), min(NC, ld - c0)
+                    U = self.blk(t, r0, r0 + ra, kbeg, kend) @ self.blk(t, c0, c0 + rb, kbeg, kend).T
                     for i in range(ra):
                         for j in range(rb):
                             if r0 + i < c0 + j:

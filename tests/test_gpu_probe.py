@@ -1,4 +1,5 @@
-"""Lane layouts of the MFMA builtins and the 64x64 tile GEMM used by the Cholesky kernels."""
+"""Lane layouts of the MFMA builtins, the 128x64 tile GEMM and the 64x64 diagonal factorization
+used by the Cholesky kernels."""
 import ctypes as C
 
 import numpy as np
@@ -31,22 +32,48 @@ def test_mfma_16x16x4_layout(gpu, prec, dt):
     np.testing.assert_array_equal(got, exp)
 
 
-@pytest.mark.parametrize("prec,dt,K,ra,rb", [(0, np.float64, 64, 64, 64), (0, np.float64, 37, 50, 21),
-                                             (1, np.float32, 128, 64, 64), (0, np.float64, 200, 64, 9)])
+@pytest.mark.parametrize("prec,dt,K,ra,rb", [(0, np.float64, 64, 128, 64), (0, np.float64, 37, 50, 21),
+                                             (1, np.float32, 128, 128, 64), (0, np.float64, 200, 100, 9),
+                                             (1, np.float32, 70, 77, 33), (0, np.float64, 3, 128, 64)])
 def test_tile_gemm(gpu, prec, dt, K, ra, rb):
     L = gpu.lib()
     L.lkd_probe_tile_gemm.argtypes = [C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_void_p,
                                       C.c_int64, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
     rng = np.random.default_rng(K)
-    lda, ldb = 70, 90
+    lda, ldb = 130, 90
     A = rng.standard_normal((K, lda)).astype(dt)     # column k of the panel = A[k, :]
     B = rng.standard_normal((K, ldb)).astype(dt)
     da, db = _dev(gpu, A), _dev(gpu, B)
-    out = gpu.DeviceBuffer(64 * 64 * np.dtype(dt).itemsize, dt)
+    out = gpu.DeviceBuffer(128 * 64 * np.dtype(dt).itemsize, dt)
     assert L.lkd_probe_tile_gemm(prec, da.addr, lda, ra, db.addr, ldb, rb, K, out.addr, None) == 0
-    got = out.download().reshape(64, 64).T            # [r][c]
-    Am = np.zeros((64, K)); Am[:ra] = A[:, :ra].T
+    got = out.download().reshape(64, 128).T           # [r][c]
+    Am = np.zeros((128, K)); Am[:ra] = A[:, :ra].T
     Bm = np.zeros((64, K)); Bm[:rb] = B[:, :rb].T
     exp = Am @ Bm.T
     tol = 1e-12 if dt == np.float64 else 1e-4
     assert np.abs(got - exp).max() <= tol * max(1.0, np.abs(exp).max())
+
+
+@pytest.mark.parametrize("prec,dt,kb", [(0, np.float64, 64), (0, np.float64, 17), (1, np.float32, 64)])
+def test_diag_potrf(gpu, prec, dt, kb):
+    """The in-LDS factorization of DIAG (identity padding past kb), against numpy."""
+    L = gpu.lib()
+    L.lkd_probe_potrf.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(kb)
+    G = rng.standard_normal((kb, kb + 8))
+    A = np.eye(64)
+    A[:kb, :kb] = G @ G.T / kb + np.eye(kb)
+    da = _dev(gpu, A.T.astype(dt))                   # column-major
+    ok = gpu.DeviceBuffer(np.zeros(1, np.int32))
+    assert L.lkd_probe_potrf(prec, da.addr, ok.addr, None) == 0
+    assert ok.download()[0] == 1
+    got = da.download().reshape(64, 64).T
+    exp = np.linalg.cholesky(A)
+    tol = 1e-13 if dt == np.float64 else 2e-5
+    assert np.abs(got - exp).max() <= tol * np.abs(exp).max()
+    # a matrix that is not positive definite is reported, not factored
+    bad = A.copy()
+    bad[3, 3] = -1.0
+    db = _dev(gpu, bad.T.astype(dt))
+    assert L.lkd_probe_potrf(prec, db.addr, ok.addr, None) == 0
+    assert ok.download()[0] == 0
