@@ -184,13 +184,13 @@ def main():
 
     prof = solver.profile_factor(fp32=args.precision == "mixed")
     fac = prof["factorization"]
-    gemm = prof["gemm"]
+    gemm = prof["step"]
     peak = PEAK_FP32 if args.precision == "mixed" else PEAK_FP64
     achieved = fac["flops"] / (fac["ms"] * 1e-3)
     log("profile " + json.dumps(prof))
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                "kernel": "supernodal factorization (k_colupd/k_coltrsm/k_update/k_diagwb)",
+                "kernel": "supernodal factorization (k_step/k_diag/k_update/k_diagwb)",
                 "flops_per_factorization": fac["flops"], "factorization_ms": fac["ms"],
                 "gemm": {"achieved_tflops": gemm["flops"] / max(gemm["ms"], 1e-9) / 1e9,
                          "launches": gemm["launches"], "avg_launch_ms": gemm["ms"] / max(gemm["launches"], 1)}}

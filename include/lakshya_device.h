@@ -57,15 +57,14 @@ int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream);
  *   off    panel offset (elements) of the supernode, ld = panel rows, ncol = panel columns
  *   k0/kb  block column start / width inside the panel
  *   r0/c0  tile row / column start inside the panel (kernel specific)
- *   kbeg/kend  K range of a GEMM task (COLUPD, UPDATE)
- *   ws     COLUPD: workspace offset of its split-K partial tile (-1: update the panel in place);
- *          DIAG / ROWS / DIAGWB: staging slot offset
- *   nsplit DIAG / ROWS: partial tiles per 128-row tile (0: none), read from ws + aux2
+ *   kbeg/kend  K range of an UPDATE task; STEP: kbeg = width of the next diagonal block
+ *   ws     DIAG / STEP: staging slot offset (STEP reads it)
+ *   nsplit STEP: 1 on the tile that also factors the next diagonal block
  *   sn     supernode id, first = first (permuted) column of the supernode                   */
 typedef struct {
     int64_t off;
-    int64_t aux;     /* diagonal-copy offset (DIAG / DIAGWB / TRSV) */
-    int64_t aux2;    /* row-list offset (UPDATE / TRSV); DIAG / ROWS: partials of tile 0 */
+    int64_t aux;     /* diagonal-copy offset (DIAG / STEP next block / DIAGWB / TRSV) */
+    int64_t aux2;    /* row-list offset (UPDATE / TRSV); STEP: staging slot of the next block */
     int64_t ws;
     int32_t ld, ncol;
     int32_t k0, kb;
@@ -120,28 +119,27 @@ int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int6
 
 /* ---------------------------------------------------------------- supernodal Cholesky */
 /* Replaces cholesky_level_kernel and its level loop (src/cuda/kkt.cu:217-285,582-596).
- * Left-looking per 64-column block column (see the task fields above), per step:
- *   COLUPD  GEMM update of the block column (split K: partial tiles into ws)
- *   DIAG    factor the diagonal block (ws = staging slot of LKD_STAGE elements, aux2 = partials)
- *   ROWS    solve 128-row tiles below the diagonal against the staged factor
- * then UPDATE (ancestors) per level and DIAGWB (diagonal blocks into the panel) at the end.
+ * Left-looking per 64-column block column, per etree level:
+ *   DIAG    factor block column 0's diagonal block (ws = staging slot, aux = diag copy)
+ *   STEP k  128-row tiles below the diagonal: GEMM update + solve against the staged factor;
+ *           the tile holding the next diagonal block (nsplit = 1, width kbeg) also factors it
+ *           (staging slot aux2, diag copy aux)
+ *   UPDATE  ancestors (split K over kbeg..kend)
+ * and DIAGWB (diagonal blocks into the panel) after the last level.
  * err: device int, bit 0 set when a pivot is not positive, bit 1 when an update falls outside
  * the L structure; kernels return early once set.
- * diag: row-major copies of the factored diagonal blocks (task.aux = element offset), with the
- * reciprocal on the diagonal; written by DIAG, read by TRSV.  piv: the pivots L_jj per permuted
- * column (panel precision).  ws: split-K workspace; stage: staged diagonal factors (both panel
- * precision). */
+ * diag: row-major copies of the factored diagonal blocks (element offsets), with the
+ * reciprocal on the diagonal, read by TRSV.  piv: the pivots L_jj per permuted column.
+ * stage: staged diagonal factors, LKD_STAGE elements per slot (all panel precision). */
 #define LKD_STAGE (64 * 64 + 4 * 256)
-int lkd_chol_colupd(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* ws,
-                    int32_t* err, void* stream);
 int lkd_chol_diag(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* diag,
-                  void* piv, const void* ws, void* stage, int32_t* err, void* stream);
-int lkd_chol_rows(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* ws,
-                  const void* stage, int32_t* err, void* stream);
-int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* diag,
-                    const void* piv, void* stream);
+                  void* piv, void* stage, int32_t* err, void* stream);
+int lkd_chol_step(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
+                  void* piv, void* stage, int32_t* err, void* stream);
 int lkd_chol_update(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
                     lkd_snplan plan, int32_t* err, void* stream);
+int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* diag,
+                    const void* piv, void* stream);
 
 /* ---------------------------------------------------------------- triangular solves */
 /* Replaces cusparseSpSV (L and L^T) (src/cuda/kkt.cu:731-734,788-789).  x is the permuted
@@ -267,7 +265,8 @@ int lkd_ipm_init(int32_t m, int32_t n, double* x, double* y, double* s, void* st
 /* ---------------------------------------------------------------- probes (tests) */
 /* One 16x16x4 MFMA (FP64 or FP32) on 64 lanes: lane l supplies a[l], b[l]; out[l*4+e] = D[e]. */
 int lkd_probe_mfma(int prec, const void* a, const void* b, void* out, void* stream);
-/* out (128x64, column-major) = A (ra x K, ld lda, ra <= 128) * B (rb x K, ldb, rb <= 64)^T */
+/* out[0, 8192) (128x64, column-major) = A (ra x K, ld lda, ra <= 128) * B (rb x K, ldb, rb <= 64)^T
+ * from the NT GEMM core; out[8192, 16384) = the same from the transposed (TN) core */
 int lkd_probe_tile_gemm(int prec, const void* a, int64_t lda, int ra, const void* b, int64_t ldb,
                         int rb, int K, void* out, void* stream);
 /* in-place Cholesky of one 64x64 column-major SPD block (lower part returned, upper zeroed);

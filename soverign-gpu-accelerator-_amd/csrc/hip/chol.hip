@@ -3,19 +3,21 @@
 // gpu_cholesky_solve_device, src/cuda/kkt.cu:671-689).
 //
 // Storage: every supernode owns a column-major panel (ld = rows of its structure, ncol columns)
-// at panel + off.  The host plan drives a left-looking blocked factorization per etree level,
-// one step per 64-column block column k (columns k0 .. k0+kb):
-//   COLUPD   C(r0:r0+128, k0:k0+kb) -= L(r0:r0+128, kbeg:kend) * L(k0:k0+kb, kbeg:kend)^T on MFMA
-//            (16x16x4 FP64 / FP32), double-buffered through LDS.  With split K the partial
-//            products go to a workspace instead of the panel.
-//   DIAG     one workgroup per supernode factors the updated 64x64 diagonal block (panel minus
-//            partials) in LDS (blocked by 16, MFMA) and publishes Lkk, the inverses of its 16x16
-//            diagonal blocks, the row-major copy with reciprocal diagonal and the pivots.  The
-//            panel's diagonal block is not written (DIAGWB does that at the end).
-//   ROWS     128-row tiles: X <- (C - partials) * Lkk^{-T} on MFMA (transposed blocked solve).
-//   DIAGWB   after the last step, Lkk of every block column is copied into the panel.
+// at panel + off.  The host plan drives a left-looking blocked factorization per etree level
+// with 64-column block columns k (columns k0 .. k0+kb), one launch per step:
+//   DIAG     (block 0 only) one workgroup per supernode factors its first 64x64 diagonal block
+//            and stages it (Lkk, the inverses of its 16x16 diagonal blocks), writes the row-major
+//            copy with reciprocal diagonal (for the substitution kernels) and the pivots.
+//   STEP k   every 128-row tile R below the diagonal block:
+//              X = C(R, k) - L(R, 0:k0) L(k, 0:k0)^T   on MFMA (16x16x4, FP64 / FP32), computed
+//                  transposed so that the accumulators are already in the layout of the solve
+//              X <- X Lkk^{-T}                         on MFMA (transposed blocked solve)
+//            and the tile holding the next diagonal block also forms, factors and stages it.
+//            Nothing waits for another workgroup: the launch boundary is the only barrier.
 //   UPDATE   below-diagonal rows:  U = L21 * L21^T (split K) scattered (atomic add) into the
 //            ancestor panels.
+//   DIAGWB   after the last step, Lkk of every block column is copied into the panel (the
+//            steps never write the panel's diagonal blocks: they live in the staging slots).
 // All workgroups are 256 threads (4 wavefronts of 64).  Task i of a launch runs on workgroup
 // (i % per) * 8 + i / per (per = grid / 8): consecutive tasks -- the tiles of one supernode --
 // share an XCD and its L2.
@@ -30,6 +32,11 @@ constexpr int NC = 64;        // GEMM tile columns
 constexpr int PA = MT + 8;    // LDS pitch of the A chunk (k-major)
 constexpr int PB = NC + 8;    // LDS pitch of the B chunk
 
+// Staging slot of a factored diagonal block: L (64x64 column-major, pitch 64, identity past kb)
+// followed by the inverses of its four 16x16 diagonal blocks (column-major, pitch 16).
+constexpr int STAGE = TS * TS + 4 * 256;
+static_assert(STAGE == LKD_STAGE, "staging slot size");
+
 template <typename T> struct KChunk;
 template <> struct KChunk<double> { static constexpr int v = 16; };
 template <> struct KChunk<float> { static constexpr int v = 32; };
@@ -38,8 +45,11 @@ template <typename T> struct GemmLds {
     T a[2][KChunk<T>::v * PA];
     T b[2][KChunk<T>::v * PB];
 };
-static_assert(sizeof(GemmLds<double>) >= 64 * DP * sizeof(double), "UPDATE staging tile");
-static_assert(sizeof(GemmLds<float>) >= 64 * DP * sizeof(float), "UPDATE staging tile");
+// the GEMM buffers are reused afterwards: 64 x DP staging tile (UPDATE), staged factor (STEP),
+// diagonal block + inverses (STEP, next diagonal)
+static_assert(sizeof(GemmLds<double>) >= STAGE * sizeof(double), "staged factor");
+static_assert(sizeof(GemmLds<double>) >= (64 * DP + 1024) * sizeof(double), "diagonal block");
+static_assert(sizeof(GemmLds<float>) >= (64 * DP + 1024) * sizeof(float), "diagonal block");
 
 __device__ __forceinline__ int task_index() {
     const int per = gridDim.x >> 3;   // grid is a multiple of 8
@@ -55,29 +65,29 @@ __device__ __forceinline__ bool wg_error(const int32_t* err, int* s) {
 
 template <typename T> using Acc = typename Mfma<T>::acc;
 
-template <typename T>
-__device__ __forceinline__ void zero_acc(Acc<T> (&acc)[2][4]) {
+template <typename T, int X, int Y>
+__device__ __forceinline__ void zero_acc(Acc<T> (&acc)[X][Y]) {
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int x = 0; x < X; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] = Acc<T>{0, 0, 0, 0};
+        for (int y = 0; y < Y; ++y) acc[x][y] = Acc<T>{0, 0, 0, 0};
 }
 
-// acc (wave w: tile rows 32w..32w+31, all 64 columns) += sum_{k in [kbeg,kend)} A(r,k) B(c,k)
-// A(r,k) = pa[k*lda + r] (r < ra), B(c,k) = pb[k*ldb + c] (c < rb); other rows read as zero.
-// Chunks of KC columns: the next chunk is loaded into registers while the MFMAs consume the
-// current one from LDS; one barrier per chunk.
-template <typename T>
-__device__ __forceinline__ void gemm_nt(const T* __restrict__ pa, int64_t lda, int ra,
-                                        const T* __restrict__ pb, int64_t ldb, int rb, int kbeg,
-                                        int kend, GemmLds<T>& sm, Acc<T> (&acc)[2][4]) {
+// Double-buffered K loop shared by the GEMM variants: chunks of KC columns of
+// A (rows 0..ra of pa, column-major, lda) and B (rows 0..rb of pb, ldb) are staged k-major in
+// LDS (rows past ra / rb and columns past kend read as zero); the next chunk is loaded into
+// registers while compute(As, Bs, k) consumes the current one, for k = 0, 4, .., KC-4.
+// One barrier per chunk; the LDS is free again when it returns.
+template <typename T, typename F>
+__device__ __forceinline__ void gemm_loop(const T* __restrict__ pa, int64_t lda, int ra,
+                                          const T* __restrict__ pb, int64_t ldb, int rb,
+                                          int kbeg, int kend, GemmLds<T>& sm, F&& compute) {
     constexpr int KC = KChunk<T>::v;
     constexpr int NA = KC * MT / 256, NB = KC * NC / 256;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int t = threadIdx.x;
     const int ar = t & (MT - 1), ak = t >> 7;
     const int br = t & (NC - 1), bk = t >> 6;
     const bool arow = ar < ra, brow = br < rb;
-    const int li = lane & 15, lk = lane >> 4;
     T va[NA], vb[NB];
     auto load = [&](int kc) {
 #pragma unroll
@@ -105,27 +115,57 @@ __device__ __forceinline__ void gemm_nt(const T* __restrict__ pa, int64_t lda, i
     for (int c = 0; c < nch; ++c) {
         const int buf = c & 1;
         if (c + 1 < nch) load(kbeg + (c + 1) * KC);
-        const T* As = sm.a[buf];
-        const T* Bs = sm.b[buf];
 #pragma unroll
-        for (int k = 0; k < KC; k += 4) {
-            const T a0 = As[(k + lk) * PA + 32 * w + li];
-            const T a1 = As[(k + lk) * PA + 32 * w + 16 + li];
-            T b[4];
-#pragma unroll
-            for (int y = 0; y < 4; ++y) b[y] = Bs[(k + lk) * PB + 16 * y + li];
-#pragma unroll
-            for (int y = 0; y < 4; ++y) {
-                acc[0][y] = Mfma<T>::mma(a0, b[y], acc[0][y]);
-                acc[1][y] = Mfma<T>::mma(a1, b[y], acc[1][y]);
-            }
-        }
+        for (int k = 0; k < KC; k += 4) compute(sm.a[buf], sm.b[buf], k);
         if (c + 1 < nch) store(buf ^ 1);
         __syncthreads();
     }
 }
 
-// f(row, col, value) for every accumulator element of this thread (tile coordinates).
+// NT: acc[x][y] (wave w) = tile rows 32w + 16x .., columns 16y ..:
+//   acc += sum_k A(r, k) B(c, k);  element e of lane l: row 32w + 16x + row(l,e), col 16y + l%16
+template <typename T>
+__device__ __forceinline__ void gemm_nt(const T* __restrict__ pa, int64_t lda, int ra,
+                                        const T* __restrict__ pb, int64_t ldb, int rb, int kbeg,
+                                        int kend, GemmLds<T>& sm, Acc<T> (&acc)[2][4]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+    gemm_loop<T>(pa, lda, ra, pb, ldb, rb, kbeg, kend, sm, [&](const T* As, const T* Bs, int k) {
+        const T a0 = As[(k + lk) * PA + 32 * w + li];
+        const T a1 = As[(k + lk) * PA + 32 * w + 16 + li];
+        T b[4];
+#pragma unroll
+        for (int y = 0; y < 4; ++y) b[y] = Bs[(k + lk) * PB + 16 * y + li];
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            acc[0][y] = Mfma<T>::mma(a0, b[y], acc[0][y]);
+            acc[1][y] = Mfma<T>::mma(a1, b[y], acc[1][y]);
+        }
+    });
+}
+
+// TN (the transposed product): acc[p][b] (wave w) holds D = B A^T for B columns 16p .. and tile
+// rows 32w + 16b ..:  element e of lane l = sum_k A(32w + 16b + l%16, k) B(16p + row(l,e), k),
+// i.e. tile row from the lane, column from the element -- the B-operand layout of the solve.
+template <typename T>
+__device__ __forceinline__ void gemm_tn(const T* __restrict__ pa, int64_t lda, int ra,
+                                        const T* __restrict__ pb, int64_t ldb, int rb, int kbeg,
+                                        int kend, GemmLds<T>& sm, Acc<T> (&acc)[4][2]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+    gemm_loop<T>(pa, lda, ra, pb, ldb, rb, kbeg, kend, sm, [&](const T* As, const T* Bs, int k) {
+        const T a0 = As[(k + lk) * PA + 32 * w + li];
+        const T a1 = As[(k + lk) * PA + 32 * w + 16 + li];
+        T b[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) b[p] = Bs[(k + lk) * PB + 16 * p + li];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            acc[p][0] = Mfma<T>::mma(b[p], a0, acc[p][0]);
+            acc[p][1] = Mfma<T>::mma(b[p], a1, acc[p][1]);
+        }
+    });
+}
+
+// f(row, col, value) for every NT accumulator element of this thread (tile coordinates).
 template <typename T, typename F>
 __device__ __forceinline__ void for_acc(Acc<T> (&acc)[2][4], F&& f) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -148,32 +188,56 @@ __device__ __forceinline__ float bcast_lane(float v, int src) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
 
-// One wavefront factors the 16x16 block at Ds(o, o) and inverts its factor: lane r (mod 16)
-// holds row r in registers, columns travel by readlane (no LDS round trips, no barriers).
-// Writes L (lower) back to Ds and L^{-1} to Lv (column-major, pitch 16, upper part zero).
-// Returns false (wave-uniform) when a pivot is not positive.
+// 1/sqrt(d) for d > 0: hardware estimate refined by two Newton steps (each squares the relative
+// error), avoiding the long correctly-rounded sqrt + divide sequences on the critical path.
+__device__ __forceinline__ double rsqrt_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    y = y * fma(-0.5 * d * y, y, 1.5);
+    y = y * fma(-0.5 * d * y, y, 1.5);
+    return y;
+}
+__device__ __forceinline__ float rsqrt_nr(float d) {
+    float y = __builtin_amdgcn_rsqf(d);
+    y = y * fmaf(-0.5f * d * y, y, 1.5f);
+    return y;
+}
+
+// One wavefront factors the 16x16 block at Ds(o, o) and inverts its factor.  Lane r (mod 16)
+// holds row r in registers; each finished column goes to LDS (cb, 16 values) and comes back as
+// broadcast reads, in order within the wavefront, so no barriers are needed.  Writes L (lower)
+// back to Ds and L^{-1} to Lv (column-major, pitch 16, upper part zero).  Returns false
+// (wave-uniform) when a pivot is not positive.
 template <typename T>
-__device__ __forceinline__ bool potrf16_wave(T* Ds, int o, T* Lv) {
+__device__ __forceinline__ bool potrf16_wave(T* Ds, int o, T* Lv, T* cb) {
     const int l = threadIdx.x & 63, r = l & 15;
     T a[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) a[k] = Ds[(o + k) * DP + o + r];
     bool ok = true;
-    T invd = T(0);
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const T d = bcast_lane(a[j], j);
         ok = ok && (d > T(0));
-        const T sd = sqrt(d);
-        const T is = T(1) / sd;
-        a[j] = r == j ? sd : a[j] * is;          // r > j: L[r][j]; r < j: unused upper part
-        if (r == j) invd = is;
+        const T is = rsqrt_nr(d);                 // 1 / L[j][j]
+        a[j] = r == j ? d * is : a[j] * is;       // r > j: L[r][j]; r < j: unused upper part
+        if (l < 16) {
+            cb[16 * j + r] = a[j];                // column j (diagonal entry included)
+            if (r == j) cb[256 + j] = is;
+        }
+        __builtin_amdgcn_wave_barrier();
+        T lkj[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k)
-            if (k > j) a[k] = fma(-a[j], bcast_lane(a[j], k), a[k]);
+            if (k > j) lkj[k] = cb[16 * j + k];
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (k > j) a[k] = fma(-a[j], lkj[k], a[k]);
+        __builtin_amdgcn_sched_barrier(0);   // keep each column's loads next to their use
     }
-    // column c = r of L^{-1}: x[i] = (delta_ic - sum_{k<i} L[i][k] x[k]) / L[i][i]
+    // column c = r of L^{-1}: x[i] = (delta_ic - sum_{k<i} L[i][k] x[k]) / L[i][i], with
+    // L[i][k] = lane i's a[k] broadcast by readlane (scalar registers: no load hoisting, few VGPRs)
     T x[16];
+    const T invd = cb[256 + r];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         T acc = i == r ? T(1) : T(0);
@@ -202,11 +266,12 @@ template <typename T>
 __device__ __forceinline__ bool potrf64(T* Ds, T* Lv, int* s_ok) {
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15;
     using R = Mfma<T>;
+    __shared__ T cb[256 + 16];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         const int o = 16 * p;
         if (w == 0) {
-            const bool ok = potrf16_wave<T>(Ds, o, Lv + 256 * p);
+            const bool ok = potrf16_wave<T>(Ds, o, Lv + 256 * p, cb);
             if (l == 0) *s_ok = ok ? 1 : 0;
         }
         __syncthreads();
@@ -257,52 +322,28 @@ __device__ __forceinline__ void write_diag_copy(const T* Ds, int kb, T* out) {
     }
 }
 
+// Publish a factored diagonal block (Ds, Lv as left by potrf64): staging slot S, pivots of the
+// permuted columns first .. first+kb, row-major copy with reciprocal diagonal at dcopy.
 template <typename T>
-__global__ __launch_bounds__(256) void k_colupd(const lkd_task* __restrict__ tasks, int ntask,
-                                                T* __restrict__ L, T* __restrict__ ws,
-                                                const int32_t* err) {
-    const int ti = task_index();
-    if (ti >= ntask) return;
-    __shared__ int s_err;
-    if (wg_error(err, &s_err)) return;
-    const lkd_task tk = tasks[ti];
-    __shared__ GemmLds<T> sm;
-    T* P = L + tk.off;
-    const int64_t ld = tk.ld;
-    const int ra = min(MT, tk.ld - tk.r0), kb = tk.kb;
-    Acc<T> acc[2][4];
-    zero_acc<T>(acc);
-    gemm_nt<T>(P + tk.r0, ld, ra, P + tk.k0, ld, kb, tk.kbeg, tk.kend, sm, acc);
-    if (tk.ws < 0) {
-        T* C = P + (int64_t)tk.k0 * ld + tk.r0;
-        for_acc<T>(acc, [&](int r, int c, T v) {
-            if (r < ra && c < kb) C[(int64_t)c * ld + r] -= v;
-        });
-    } else {
-        T* W = ws + tk.ws;
-        for_acc<T>(acc, [&](int r, int c, T v) { W[c * MT + r] = v; });
+__device__ __forceinline__ void publish_diag(const T* Ds, const T* Lv, int kb, T* __restrict__ S,
+                                             T* __restrict__ piv, T* __restrict__ dcopy) {
+    const int t = threadIdx.x;
+    for (int idx = t; idx < TS * TS; idx += 256) {
+        const int r = idx & 63, c = idx >> 6;
+        S[idx] = r >= c ? Ds[c * DP + r] : T(0);
     }
+    for (int idx = t; idx < 4 * 256; idx += 256) S[TS * TS + idx] = Lv[idx];
+    if (t < kb) piv[t] = Ds[t * DP + t];
+    write_diag_copy<T>(Ds, kb, dcopy);
 }
 
-// Staging slot of a factored diagonal block: L (64x64 column-major, pitch 64, identity past kb)
-// followed by the inverses of its four 16x16 diagonal blocks (column-major, pitch 16).
-constexpr int STAGE = TS * TS + 4 * 256;
-
-// Rows r of the split-K partial tiles: tile g (rows k0 + 128 g ..), part j at
-// ws[aux2 + (g * nsplit + j) * MT * NC], column-major 128 x 64.
-template <typename T>
-__device__ __forceinline__ T minus_partials(T v, const T* __restrict__ W, int ns, int c, int r) {
-    for (int j = 0; j < ns; ++j) v -= W[(int64_t)j * MT * NC + c * MT + r];
-    return v;
-}
-
-// DIAG: factor the updated 64x64 diagonal block of block column k (panel minus partials) in LDS
-// and publish it: staging slot (ws offset), row-major copy with reciprocal diagonal (aux), pivots.
+// DIAG: factor the first diagonal block (columns k0 .. k0+kb, nothing to subtract yet) of each
+// task's supernode; staging slot at stage + ws, row-major copy at diag + aux.
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag(const lkd_task* __restrict__ tasks, int ntask,
                                               const T* __restrict__ L, T* __restrict__ diag,
-                                              T* __restrict__ piv, const T* __restrict__ ws,
-                                              T* __restrict__ stage, int32_t* err) {
+                                              T* __restrict__ piv, T* __restrict__ stage,
+                                              int32_t* err) {
     const int ti = task_index();
     if (ti >= ntask) return;
     __shared__ int s_err;
@@ -313,72 +354,76 @@ __global__ __launch_bounds__(256) void k_diag(const lkd_task* __restrict__ tasks
     __shared__ int s_ok;
     const T* P = L + tk.off;
     const int64_t ld = tk.ld;
-    const int k0 = tk.k0, kb = tk.kb, ns = tk.nsplit;
-    const int t = threadIdx.x;
-    const T* W = ws + tk.aux2;
-    for (int idx = t; idx < TS * TS; idx += 256) {
+    const int k0 = tk.k0, kb = tk.kb;
+    for (int idx = threadIdx.x; idx < TS * TS; idx += 256) {
         const int r = idx & 63, c = idx >> 6;
         T v = r == c ? T(1) : T(0);
-        if (r < kb && c < kb)
-            v = r >= c ? minus_partials(P[(int64_t)(k0 + c) * ld + k0 + r], W, ns, c, r) : T(0);
+        if (r < kb && c < kb) v = r >= c ? P[(int64_t)(k0 + c) * ld + k0 + r] : T(0);
         Ds[c * DP + r] = v;
     }
     __syncthreads();
     if (!potrf64<T>(Ds, Lv, &s_ok)) {
-        if (t == 0) atomicOr(err, 1);
+        if (threadIdx.x == 0) atomicOr(err, 1);
         return;
     }
-    T* S = stage + tk.ws;
-    for (int idx = t; idx < TS * TS; idx += 256) {
-        const int r = idx & 63, c = idx >> 6;
-        S[idx] = r >= c ? Ds[c * DP + r] : T(0);
-    }
-    for (int idx = t; idx < 4 * 256; idx += 256) S[TS * TS + idx] = Lv[idx];
-    if (t < kb) piv[tk.first + k0 + t] = Ds[t * DP + t];
-    write_diag_copy<T>(Ds, kb, diag + tk.aux);
+    publish_diag<T>(Ds, Lv, kb, stage + tk.ws, piv + tk.first + k0, diag + tk.aux);
 }
 
-// ROWS: X <- (C - partials) Lkk^{-T} for rows max(r0, k0+kb) .. r0+127 of block column k, done
-// transposed on MFMA: Y = X^T satisfies Lkk Y = X^T, so by 16-row blocks of Y,
-// Y_p = Linv_pp (X_p^T - sum_{q<p} L_pq Y_q).  Each wavefront owns 32 rows of X (two 16-column
-// blocks of Y); an accumulator tile D (element e of lane l = D[row(l,e)][l%16]) is exactly the
-// B operand the next products need, so Y stays in registers.  The A operands (blocks of Lkk and
-// the inverses) are read straight from the staging slot.
+// STEP k (one task = tile R = rows r0 .. r0+c0-1, all below the diagonal block of block column k):
+//   X = C(R, k) - L(R, 0:k0) L(k, 0:k0)^T          (gemm_tn: accumulators in solve layout)
+//   X <- X Lkk^{-T}: Y = X^T solves Lkk Y = X^T, by 16-row blocks of Y
+//       Y_p = Linv_pp (X_p^T - sum_{q<p} L_pq Y_q)  (A operands from the staged factor in LDS;
+//       an accumulator is exactly the B operand of the next product, so Y stays in registers)
+//   every diagonal block j whose rows lie in R: C(j, j) -= L(j, k) L(j, k)^T (the diagonal
+//       blocks are kept up to date step by step, so none needs a long update when its turn
+//       comes)
+// Task fields: ws = staging slot of block k (written by the previous launch).  nsplit = 1 marks
+// the tile that is exactly the next diagonal block k1 = k0 + kb (c0 = kbeg rows): after its own
+// rank-kb update it factors C(k1, k1) and publishes it (slot aux2, copy aux).
 template <typename T>
-__global__ __launch_bounds__(256) void k_rows(const lkd_task* __restrict__ tasks, int ntask,
-                                              T* __restrict__ L, const T* __restrict__ ws,
-                                              const T* __restrict__ stage,
-                                              const int32_t* err) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T* __restrict__ diag,
+            T* __restrict__ piv, T* __restrict__ stage, int32_t* err) {
     const int ti = task_index();
     if (ti >= ntask) return;
     __shared__ int s_err;
     if (wg_error(err, &s_err)) return;
     const lkd_task tk = tasks[ti];
+    __shared__ GemmLds<T> sm;
+    __shared__ int s_ok;
     using R = Mfma<T>;
     T* P = L + tk.off;
     const int64_t ld = tk.ld;
-    const int k0 = tk.k0, kb = tk.kb, ns = tk.nsplit;
+    const int k0 = tk.k0, kb = tk.kb, kbm = kb - 1;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15;
-    const T* S = stage + tk.ws;
-    const T* Wg = ws + tk.aux2 + (int64_t)((tk.r0 - k0) / MT) * ns * MT * NC;
-    const int kbm = kb - 1;
+    const int ra = tk.c0;
+    Acc<T> acc[4][2];
+    zero_acc<T, 4, 2>(acc);
+    gemm_tn<T>(P + tk.r0, ld, ra, P + k0, ld, kb, 0, k0, sm, acc);
+    T* S = reinterpret_cast<T*>(&sm);
+    {
+        const T* Sg = stage + tk.ws;
+#pragma unroll 4
+        for (int idx = t; idx < STAGE; idx += 256) S[idx] = Sg[idx];
+    }
     int64_t row[2];
     bool valid[2];
     Acc<T> y[2][4];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-        row[b] = (int64_t)tk.r0 + 32 * w + 16 * b + li;
-        valid[b] = row[b] >= k0 + kb && row[b] < ld;
+        const int rl = 32 * w + 16 * b + li;
+        row[b] = (int64_t)tk.r0 + rl;
+        valid[b] = rl < ra;
         const int64_t rc = valid[b] ? row[b] : tk.r0;     // a row inside the panel
-        const int rr = (int)(rc - tk.r0);
 #pragma unroll
         for (int p = 0; p < 4; ++p)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int c = min(16 * p + R::row(l, e), kbm);   // columns past kb: padding
-                y[b][p][e] = minus_partials(P[(int64_t)(k0 + c) * ld + rc], Wg, ns, c, rr);
+                y[b][p][e] = P[(int64_t)(k0 + c) * ld + rc] - acc[p][b][e];
             }
     }
+    __syncthreads();
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         Acc<T> tp[2] = {y[0][p], y[1][p]};
@@ -410,6 +455,38 @@ __global__ __launch_bounds__(256) void k_rows(const lkd_task* __restrict__ tasks
                 if (c < kb) P[(int64_t)(k0 + c) * ld + row[b]] = y[b][p][e];
             }
     }
+    // ---- rank-kb updates of the diagonal blocks inside the tile (rows below ncol only)
+    for (int h = 0; h < 2; ++h) {
+        const int d0 = tk.r0 + 64 * h;                    // first row = first column of block
+        if (64 * h >= ra || d0 >= tk.ncol) break;         // uniform
+        const int db = min(64, min(ra - 64 * h, tk.ncol - d0));
+        __syncthreads();                                  // the solved rows are visible
+        Acc<T> d[2][4];
+        zero_acc<T, 2, 4>(d);
+        gemm_nt<T>(P + d0, ld, db, P + d0, ld, db, k0, k0 + kb, sm, d);
+        if (w < 2)
+            for_acc<T>(d, [&](int r, int c, T v) {
+                if (r < db && c <= r) P[(int64_t)(d0 + c) * ld + d0 + r] -= v;
+            });
+    }
+    if (!tk.nsplit) return;
+    // ---- the next diagonal block (this tile) is complete: factor and publish it
+    const int k1 = tk.r0, kb1 = ra;
+    __syncthreads();
+    T* Ds = S;
+    T* Lv = S + TS * DP;
+    for (int idx = t; idx < TS * TS; idx += 256) {
+        const int r = idx & 63, c = idx >> 6;
+        T v = r == c ? T(1) : T(0);
+        if (r < kb1 && c < kb1) v = r >= c ? P[(int64_t)(k1 + c) * ld + k1 + r] : T(0);
+        Ds[c * DP + r] = v;
+    }
+    __syncthreads();
+    if (!potrf64<T>(Ds, Lv, &s_ok)) {
+        if (t == 0) atomicOr(err, 1);
+        return;
+    }
+    publish_diag<T>(Ds, Lv, kb1, stage + tk.aux2, piv + tk.first + k1, diag + tk.aux);
 }
 
 // Lkk of each task's block column into the panel: strictly lower part from the row-major copy,
@@ -463,7 +540,7 @@ __global__ __launch_bounds__(256) void k_update(const lkd_task* __restrict__ tas
     const int64_t ld = tk.ld;
     const int ra = min(MT, tk.ld - tk.r0), rb = min(NC, tk.ld - tk.c0);
     Acc<T> acc[2][4];
-    zero_acc<T>(acc);
+    zero_acc<T, 2, 4>(acc);
     gemm_nt<T>(P + tk.r0, ld, ra, P + tk.c0, ld, rb, tk.kbeg, tk.kend, sm, acc);
     const int t = threadIdx.x, w = t >> 6;
     const int32_t* rows = plan.sn_rows + tk.aux2;
@@ -504,7 +581,7 @@ __global__ __launch_bounds__(256) void k_update(const lkd_task* __restrict__ tas
 int grid8(int32_t ntask) { return (ntask + 7) & ~7; }
 
 struct CholArgs {
-    void *panel, *diag, *piv, *ws, *stage;
+    void *panel, *diag, *piv, *stage;
     const lkd_snplan* plan;
     int32_t* err;
 };
@@ -513,11 +590,10 @@ template <typename T>
 int launch_chol(int kind, int32_t ntask, const lkd_task* tasks, const CholArgs& a, void* stream) {
     T* L = static_cast<T*>(a.panel);
     switch (kind) {
-        case 0: LK_LAUNCH(k_colupd<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.ws, a.err); break;
-        case 1: LK_LAUNCH(k_diag<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (const T*)a.ws, (T*)a.stage, a.err); break;
+        case 1: LK_LAUNCH(k_diag<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (T*)a.stage, a.err); break;
         case 2: LK_LAUNCH(k_update<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, *a.plan, a.err); break;
         case 3: LK_LAUNCH(k_diagwb<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (const T*)a.diag, (const T*)a.piv); break;
-        default: LK_LAUNCH(k_rows<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (const T*)a.ws, (const T*)a.stage, a.err); break;
+        default: LK_LAUNCH(k_step<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (T*)a.stage, a.err); break;
     }
     return lk_last_error();
 }
@@ -534,9 +610,21 @@ __global__ __launch_bounds__(256) void k_probe_gemm(const T* pa, int64_t lda, in
                                                     int64_t ldb, int rb, int K, T* out) {
     __shared__ GemmLds<T> sm;
     Acc<T> acc[2][4];
-    zero_acc<T>(acc);
+    zero_acc<T, 2, 4>(acc);
     gemm_nt<T>(pa, lda, ra, pb, ldb, rb, 0, K, sm, acc);
     for_acc<T>(acc, [&](int r, int c, T v) { out[c * MT + r] = v; });
+    // the transposed variant must produce the same product
+    Acc<T> tr[4][2];
+    zero_acc<T, 4, 2>(tr);
+    gemm_tn<T>(pa, lda, ra, pb, ldb, rb, 0, K, sm, tr);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                out[MT * NC + (16 * p + Mfma<T>::row(lane, e)) * MT + 32 * w + 16 * b + (lane & 15)] = tr[p][b][e];
 }
 
 template <typename T>
@@ -562,7 +650,8 @@ __global__ __launch_bounds__(256) void k_probe_potrf(T* a, int32_t* ok) {
 
 extern "C" {
 
-// out (128 x 64, column-major) = A(ra x K) * B(rb x K)^T with the GEMM core of COLUPD / UPDATE
+// out[0 .. 128*64) (column-major 128 x 64) = A(ra x K) * B(rb x K)^T with the NT GEMM core;
+// out[128*64 .. 2*128*64) = the same product from the transposed (TN) core
 int lkd_probe_tile_gemm(int prec, const void* a, int64_t lda, int ra, const void* b, int64_t ldb,
                         int rb, int K, void* out, void* stream) {
     if (ra < 0 || ra > MT || rb < 0 || rb > NC || K < 0) return (int)hipErrorInvalidValue;
@@ -573,35 +662,29 @@ int lkd_probe_tile_gemm(int prec, const void* a, int64_t lda, int ra, const void
     return lk_last_error();
 }
 
-// In-place Cholesky of one 64 x 64 column-major block with the DIAG factorization
+// In-place Cholesky of one 64 x 64 column-major block with the diagonal factorization
 int lkd_probe_potrf(int prec, void* a, int32_t* ok, void* stream) {
     if (prec == LKD_F32) LK_LAUNCH(k_probe_potrf<float>, 1, 256, 0, stream, (float*)a, ok);
     else LK_LAUNCH(k_probe_potrf<double>, 1, 256, 0, stream, (double*)a, ok);
     return lk_last_error();
 }
 
-int lkd_chol_colupd(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* ws,
-                    int32_t* err, void* stream) {
-    return dispatch(prec, 0, ntask, tasks, {panel, nullptr, nullptr, ws, nullptr, nullptr, err}, stream);
-}
 int lkd_chol_diag(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* diag,
-                  void* piv, const void* ws, void* stage, int32_t* err, void* stream) {
-    return dispatch(prec, 1, ntask, tasks,
-                    {const_cast<void*>(panel), diag, piv, const_cast<void*>(ws), stage, nullptr, err}, stream);
+                  void* piv, void* stage, int32_t* err, void* stream) {
+    return dispatch(prec, 1, ntask, tasks, {const_cast<void*>(panel), diag, piv, stage, nullptr, err}, stream);
 }
-int lkd_chol_rows(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* ws,
-                  const void* stage, int32_t* err, void* stream) {
-    return dispatch(prec, 4, ntask, tasks,
-                    {panel, nullptr, nullptr, const_cast<void*>(ws), const_cast<void*>(stage), nullptr, err}, stream);
+int lkd_chol_step(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
+                  void* piv, void* stage, int32_t* err, void* stream) {
+    return dispatch(prec, 5, ntask, tasks, {panel, diag, piv, stage, nullptr, err}, stream);
 }
 int lkd_chol_update(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
                     lkd_snplan plan, int32_t* err, void* stream) {
-    return dispatch(prec, 2, ntask, tasks, {panel, nullptr, nullptr, nullptr, nullptr, &plan, err}, stream);
+    return dispatch(prec, 2, ntask, tasks, {panel, nullptr, nullptr, nullptr, &plan, err}, stream);
 }
 int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* diag,
                     const void* piv, void* stream) {
     return dispatch(prec, 3, ntask, tasks,
-                    {panel, const_cast<void*>(diag), const_cast<void*>(piv), nullptr, nullptr, nullptr, nullptr}, stream);
+                    {panel, const_cast<void*>(diag), const_cast<void*>(piv), nullptr, nullptr, nullptr}, stream);
 }
 
 }  // extern "C"

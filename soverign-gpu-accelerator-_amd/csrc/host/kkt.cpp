@@ -33,15 +33,14 @@ void check(int e, const char* what) {
 }
 
 constexpr int TILE = 64;           // block-column width (diagonal blocks, substitution blocks)
-constexpr int TRSM_ROWS = 128;      // rows per ROWS workgroup (one GEMM tile)
 constexpr int GEMM_MT = 128;        // COLUPD / UPDATE tile rows
 constexpr int GEMM_NC = 64;         // COLUPD / UPDATE tile columns
-constexpr int GEMM_TARGET_WG = 1024;   // K splitting aims at this many GEMM workgroups per launch
+constexpr int GEMM_TARGET_WG = 1024;   // UPDATE K splitting aims at this many workgroups
 constexpr int KSPLIT_MIN = 128;     // minimum K columns per split part
 constexpr int FWD_ROWS = 256;
 
 struct Launch {
-    int kind;        // 0 colupd 1 diag 2 update 3 diagwb 4 rows | 10 fwd-first 11 fwd-step 12 bwd-below 13 bwd-step
+    int kind;        // 1 diag 2 update 3 diagwb 5 step | 10 fwd-first 11 fwd-step 12 bwd-below 13 bwd-step
     int64_t begin;
     int32_t count;
 };
@@ -100,8 +99,6 @@ struct GPUKKTCholeskySolver::Impl {
     void* d_panel = nullptr;       // sized for FP64
     void* d_diag = nullptr;
     int64_t diag_elems = 0;
-    void* d_ws = nullptr;          // split-K partial tiles (sized for FP64)
-    int64_t ws_elems = 0;
     void* d_piv = nullptr;         // pivots L_jj per permuted column (sized for FP64)
     void* d_stage = nullptr;       // staged diagonal factors, LKD_STAGE per supernode of a level
     int64_t stage_elems = 0;
@@ -164,8 +161,6 @@ struct GPUKKTCholeskySolver::Impl {
             up(sl, "slaunch");
             std::vector<int64_t> de{diag_elems};
             up(de, "diag_elems");
-            std::vector<int64_t> we{ws_elems};
-            up(we, "ws_elems");
             std::vector<int64_t> se{stage_elems};
             up(se, "stage_elems");
         }
@@ -373,76 +368,67 @@ struct GPUKKTCholeskySolver::Impl {
         auto kpart = [](int K, int ns, int j) {   // split boundaries on multiples of 16
             return static_cast<int32_t>((static_cast<int64_t>(K) * j / ns) / 16 * 16);
         };
-        ws_elems = 0;
+        std::vector<lkd_task> diag_tasks;   // one per (supernode, block column): DIAGWB
         for (const auto& lv : by_level) {
             int maxblk = 0;
             for (Index s : lv) {
                 const Index nc = S.sn_first[s + 1] - S.sn_first[s];
                 maxblk = std::max<int>(maxblk, (nc + TILE - 1) / TILE);
             }
-            for (int k = 0; k < maxblk; ++k) {
-                // COLUPD: block column k of every supernode that has one, K = k * 64 columns
-                int64_t tiles = 0;
-                for (Index s : lv) {
-                    const lkd_task t = base(s);
-                    if (k == 0 || k * TILE >= t.ncol) continue;
-                    tiles += (t.ld - k * TILE + GEMM_MT - 1) / GEMM_MT;
+            // two staging slots per supernode of the level: block k is read from slot k % 2
+            // while block k + 1 is written to the other one
+            auto slot = [&](size_t si, int k) { return (2 * static_cast<int64_t>(si) + (k & 1)) * LKD_STAGE; };
+            stage_elems = std::max<int64_t>(stage_elems, 2 * static_cast<int64_t>(lv.size()) * LKD_STAGE);
+            for (Index s : lv) {
+                lkd_task t = base(s);
+                for (int k = 0; k * TILE < t.ncol; ++k) {
+                    lkd_task u = t;
+                    u.k0 = k * TILE;
+                    u.kb = std::min(TILE, t.ncol - u.k0);
+                    u.r0 = u.k0;
+                    u.aux = diag_off(s, k);
+                    diag_tasks.push_back(u);
                 }
-                const int ns = k == 0 ? 1 : nsplit_for(tiles, int64_t(k) * TILE);
-                std::vector<int64_t> wsbase(lv.size(), -1);
-                int64_t wsoff = 0;
-                int64_t b0 = static_cast<int64_t>(tasks.size());
-                for (size_t si = 0; si < lv.size() && k > 0; ++si) {
+            }
+            // DIAG: block column 0
+            int64_t b0 = static_cast<int64_t>(tasks.size());
+            for (size_t si = 0; si < lv.size(); ++si) {
+                lkd_task t = base(lv[si]);
+                t.kb = std::min(TILE, t.ncol);
+                t.aux = diag_off(lv[si], 0);
+                t.ws = slot(si, 0);
+                tasks.push_back(t);
+            }
+            emit(1, b0);
+            // STEP k: 128-row tiles below the diagonal block of block column k
+            for (int k = 0; k < maxblk; ++k) {
+                b0 = static_cast<int64_t>(tasks.size());
+                for (size_t si = 0; si < lv.size(); ++si) {
                     lkd_task t = base(lv[si]);
                     if (k * TILE >= t.ncol) continue;
                     t.k0 = k * TILE;
                     t.kb = std::min(TILE, t.ncol - t.k0);
-                    const int K = t.k0;
-                    if (ns > 1) wsbase[si] = wsoff;
-                    int g = 0;
-                    for (int r0 = t.k0; r0 < t.ld; r0 += GEMM_MT, ++g)
-                        for (int j = 0; j < ns; ++j) {
-                            lkd_task u = t;
-                            u.r0 = r0;
-                            u.kbeg = kpart(K, ns, j);
-                            u.kend = j + 1 == ns ? K : kpart(K, ns, j + 1);
-                            u.ws = ns > 1 ? wsoff + (int64_t(g) * ns + j) * GEMM_MT * GEMM_NC : -1;
-                            tasks.push_back(u);
-                        }
-                    if (ns > 1) wsoff += int64_t(g) * ns * GEMM_MT * GEMM_NC;
-                }
-                ws_elems = std::max(ws_elems, wsoff);
-                emit(0, b0);
-                // DIAG: factor the diagonal block of block column k into staging slot si
-                auto diag_task = [&](size_t si) {
-                    lkd_task t = base(lv[si]);
-                    t.k0 = k * TILE;
-                    t.kb = std::min(TILE, t.ncol - t.k0);
-                    t.aux = diag_off(lv[si], k);
-                    t.nsplit = wsbase[si] >= 0 ? ns : 0;
-                    t.aux2 = wsbase[si] >= 0 ? wsbase[si] : 0;
-                    t.ws = static_cast<int64_t>(si) * LKD_STAGE;
-                    t.r0 = t.k0;
-                    return t;
-                };
-                b0 = static_cast<int64_t>(tasks.size());
-                for (size_t si = 0; si < lv.size(); ++si)
-                    if (k * TILE < S.sn_first[lv[si] + 1] - S.sn_first[lv[si]]) tasks.push_back(diag_task(si));
-                emit(1, b0);
-                stage_elems = std::max<int64_t>(stage_elems, static_cast<int64_t>(lv.size()) * LKD_STAGE);
-                // ROWS: 128-row tiles below the diagonal block
-                b0 = static_cast<int64_t>(tasks.size());
-                for (size_t si = 0; si < lv.size(); ++si) {
-                    if (k * TILE >= S.sn_first[lv[si] + 1] - S.sn_first[lv[si]]) continue;
-                    const lkd_task t = diag_task(si);
-                    for (int r0 = t.k0; r0 < t.ld; r0 += TRSM_ROWS) {
-                        if (std::max(r0, t.k0 + t.kb) >= std::min(r0 + TRSM_ROWS, t.ld)) continue;   // no rows
+                    t.ws = slot(si, k);
+                    const int k1 = t.k0 + t.kb;
+                    int r0 = k1;
+                    if (k1 < t.ncol) {   // the next diagonal block: its own tile, first in the list
+                        lkd_task u = t;
+                        u.r0 = k1;
+                        u.c0 = std::min(TILE, t.ncol - k1);
+                        u.nsplit = 1;
+                        u.aux = diag_off(lv[si], k + 1);
+                        u.aux2 = slot(si, k + 1);
+                        tasks.push_back(u);
+                        r0 += u.c0;
+                    }
+                    for (; r0 < t.ld; r0 += GEMM_MT) {
                         lkd_task u = t;
                         u.r0 = r0;
+                        u.c0 = std::min(GEMM_MT, t.ld - r0);
                         tasks.push_back(u);
                     }
                 }
-                emit(4, b0);
+                emit(5, b0);
             }
             // UPDATE: L21 L21^T of every supernode into its ancestors, split K
             int64_t tiles = 0, maxK = 0;
@@ -453,7 +439,7 @@ struct GPUKKTCholeskySolver::Impl {
                     for (int c0 = t.ncol; c0 < std::min(t.ld, r0 + GEMM_MT); c0 += GEMM_NC) ++tiles;
                 maxK = std::max<int64_t>(maxK, t.ncol);
             }
-            const int64_t b0 = static_cast<int64_t>(tasks.size());
+            b0 = static_cast<int64_t>(tasks.size());
             for (Index s : lv) {
                 lkd_task t = base(s);
                 if (t.ld <= t.ncol) continue;
@@ -475,10 +461,7 @@ struct GPUKKTCholeskySolver::Impl {
         // DIAGWB: every factored diagonal block into the panel (no kernel reads them before)
         {
             const int64_t b0 = static_cast<int64_t>(tasks.size());
-            for (const auto& L : flaunch) {
-                if (L.kind != 1) continue;
-                for (int64_t q = L.begin; q < L.begin + L.count; ++q) tasks.push_back(tasks[q]);
-            }
+            tasks.insert(tasks.end(), diag_tasks.begin(), diag_tasks.end());
             emit(3, b0);
         }
         for (const auto& L : flaunch)
@@ -500,30 +483,30 @@ struct GPUKKTCholeskySolver::Impl {
         const int64_t ld = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
         const Index nc = S.sn_first[s + 1] - S.sn_first[s];
         if (t.ld != ld || t.ncol != nc || t.off != S.sn_panel[s] || t.first != S.sn_first[s] ||
-            (kind != 1 && kind != 3 && kind != 4 && t.aux2 != S.sn_rowptr[s]))   // aux2 = partials
+            (kind != 5 && t.aux2 != S.sn_rowptr[s]))   // STEP: aux2 = next staging slot
             bad("geometry");
         if (S.sn_panel[s] + ld * nc > S.sn_panel[S.nsuper]) bad("panel extent");
         if (t.first + nc > m) bad("columns");
         if (t.k0 < 0 || t.kb < 0 || t.k0 + t.kb > nc || t.kb > TILE) bad("block column");
         auto diag_ok = [&](int64_t off, int kb) { return off >= 0 && off + int64_t(kb) * kb <= diag_elems; };
-        const int64_t tile = int64_t(GEMM_MT) * GEMM_NC;
+        auto slot_ok = [&](int64_t off) { return off >= 0 && off + LKD_STAGE <= stage_elems; };
         switch (kind) {
-            case 0:   // COLUPD: rows r0.. of block column k0, K range inside the previous columns
-                if (t.kb < 1 || t.k0 < 1 || t.r0 < t.k0 || t.r0 >= ld) bad("colupd rows");
-                if (t.kbeg < 0 || t.kend > t.k0 || t.kbeg >= t.kend) bad("colupd K");
-                if (t.ws >= 0 && t.ws + tile > ws_elems) bad("colupd workspace");
+            case 1:   // DIAG: first diagonal block, its staging slot and copy
+                if (t.k0 != 0 || t.kb < 1 || !diag_ok(t.aux, t.kb) || !slot_ok(t.ws)) bad("diag");
                 break;
-            case 1:   // DIAG / DIAGWB: the diagonal block, its staging slot and partials
-            case 3:
-            case 4: { // ROWS: a 128-row tile with rows below the diagonal block
-                if (t.kb < 1 || t.r0 < t.k0 || t.r0 >= ld || (t.r0 - t.k0) % TRSM_ROWS) bad("rows");
-                if (kind != 4 && t.r0 != t.k0) bad("diag task");
-                if (kind == 4 && std::max(t.r0, t.k0 + t.kb) >= std::min<int64_t>(t.r0 + TRSM_ROWS, ld)) bad("rows tile");
-                if (kind != 4 && !diag_ok(t.aux, t.kb)) bad("diag copy");
-                if (t.ws < 0 || t.ws + LKD_STAGE > stage_elems) bad("staging slot");
-                if (t.nsplit < 0 || (t.nsplit > 0 && (t.aux2 < 0 ||
-                    t.aux2 + (int64_t(ld - t.k0 + GEMM_MT - 1) / GEMM_MT) * t.nsplit * tile > ws_elems)))
-                    bad("partials");
+            case 3:   // DIAGWB: a diagonal block and its copy
+                if (t.kb < 1 || t.r0 != t.k0 || !diag_ok(t.aux, t.kb)) bad("diagwb");
+                break;
+            case 5: { // STEP: a tile of c0 <= 128 rows below the diagonal block of block column k
+                if (t.kb < 1 || t.r0 < t.k0 + t.kb || t.c0 < 1 || t.c0 > GEMM_MT || t.r0 + t.c0 > ld) bad("step rows");
+                if (t.r0 < nc && (t.r0 % TILE)) bad("step tile alignment");   // diagonal blocks whole
+                if (!slot_ok(t.ws)) bad("step slot");
+                if (t.nsplit) {
+                    const int k1 = t.k0 + t.kb;
+                    if (t.r0 != k1 || t.c0 > TILE || k1 + t.c0 > nc || !diag_ok(t.aux, t.c0) ||
+                        !slot_ok(t.aux2) || t.aux2 == t.ws)
+                        bad("step next block");
+                }
                 break;
             }
             case 2:   // UPDATE: tile of the L21 rows, lower triangle, K inside the columns
@@ -660,8 +643,6 @@ struct GPUKKTCholeskySolver::Impl {
         owned.push_back(d_panel);
         d_diag = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(diag_elems, 1)) * sizeof(double));
         owned.push_back(d_diag);
-        d_ws = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(ws_elems, 1)) * sizeof(double));
-        owned.push_back(d_ws);
         d_piv = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<Index>(m, 1)) * sizeof(double));
         owned.push_back(d_piv);
         d_stage = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(stage_elems, 1)) * sizeof(double));
@@ -716,16 +697,15 @@ struct GPUKKTCholeskySolver::Impl {
     int launch_factor(int prec, const Launch& L, const lkd_snplan& pl) {
         const lkd_task* t = d_ftasks + L.begin;
         switch (L.kind) {
-            case 0: return lkd_chol_colupd(prec, L.count, t, d_panel, d_ws, d_err, stream);
-            case 1: return lkd_chol_diag(prec, L.count, t, d_panel, d_diag, d_piv, d_ws, d_stage, d_err, stream);
+            case 1: return lkd_chol_diag(prec, L.count, t, d_panel, d_diag, d_piv, d_stage, d_err, stream);
             case 2: return lkd_chol_update(prec, L.count, t, d_panel, pl, d_err, stream);
             case 3: return lkd_chol_diagwb(prec, L.count, t, d_panel, d_diag, d_piv, stream);
-            default: return lkd_chol_rows(prec, L.count, t, d_panel, d_ws, d_stage, d_err, stream);
+            default: return lkd_chol_step(prec, L.count, t, d_panel, d_diag, d_piv, d_stage, d_err, stream);
         }
     }
 
     // One factorization of the current M with HIP events around every launch: per launch kind
-    // (0 COLUPD, 1 DIAG + ROWS + DIAGWB, 2 UPDATE) the summed device time, the launch count and the
+    // (0 STEP, 1 DIAG + DIAGWB, 2 UPDATE) the summed device time, the launch count and the
     // algorithmic flops of the tasks; slot 3 = assembly (prep, diagonal, zero fill, pair sums);
     // slot 4 = whole factorization incl. assembly (flops: sum over columns of L of c_j^2).
     void profile_factor(int prec, double ms[5], double flops[5], double launches[5]) {
@@ -760,7 +740,7 @@ struct GPUKKTCholeskySolver::Impl {
             check(lkd_event_record(e1, stream), "event");
             float t_ms = 0;
             check(lkd_event_elapsed_ms(&t_ms, e0, e1), "elapsed");
-            const int slot = L.kind >= 3 ? 1 : L.kind;   // ROWS and the write-back count with DIAG
+            const int slot = L.kind == 5 ? 0 : 1 + (L.kind == 2);   // 0 STEP, 1 DIAG + DIAGWB, 2 UPDATE
             ms[slot] += t_ms;
             launches[slot] += 1;
             for (int64_t q = L.begin; q < L.begin + L.count; ++q) flops[slot] += task_flops(h_ftasks[q], L.kind);
@@ -779,17 +759,19 @@ struct GPUKKTCholeskySolver::Impl {
     static double task_flops(const lkd_task& t, int kind) {
         const double kb = t.kb;
         switch (kind) {
-            case 0: {
-                const double ra = std::min(GEMM_MT, t.ld - t.r0);
-                const double useful = t.r0 == t.k0 ? kb * (kb + 1) / 2 + (ra - kb) * kb : ra * kb;
-                return 2.0 * useful * (t.kend - t.kbeg);
+            case 5: {   // GEMM + solve of the tile; diagonal-block updates; next block's factor
+                const double ra = t.c0;
+                double f = 2.0 * ra * kb * t.k0 + ra * kb * kb;
+                for (int h = 0; h < 2; ++h) {
+                    const int d0 = t.r0 + 64 * h;
+                    if (64 * h >= t.c0 || d0 >= t.ncol) break;
+                    const double db = std::min({64, t.c0 - 64 * h, t.ncol - d0});
+                    f += db * (db + 1) * kb;
+                }
+                if (t.nsplit) f += ra * ra * ra / 3.0;
+                return f;
             }
             case 1: return kb * kb * kb / 3.0;
-            case 4: {
-                const double rows = static_cast<double>(std::min(TRSM_ROWS, t.ld - t.r0)) -
-                                    std::max(0.0, t.k0 + kb - static_cast<double>(t.r0));
-                return std::max(rows, 0.0) * kb * kb;
-            }
             case 3: return 0.0;
             default: {
                 const double ra = std::min(GEMM_MT, t.ld - t.r0), rb = std::min(GEMM_NC, t.ld - t.c0);

@@ -336,7 +336,7 @@ class IPM:
         L.lk_ipm_profile_factor.argtypes = [_vp, C.c_int32, _f64p, _f64p, _f64p]
         ms, fl, la = np.zeros(5), np.zeros(5), np.zeros(5)
         _ck(L.lk_ipm_profile_factor(self.h, int(fp32), ms, fl, la))
-        names = ["gemm", "potrf_trsm", "update", "assembly", "factorization"]
+        names = ["step", "diag", "update", "assembly", "factorization"]
         return {k: {"ms": float(ms[i]), "flops": float(fl[i]), "launches": int(la[i])}
                 for i, k in enumerate(names)}
 
@@ -384,7 +384,7 @@ PLAN_DTYPES = {"perm": np.int32, "sn_rowptr": np.int64, "sn_rows": np.int32,
                "pci": np.int32, "pval": np.float64, "empty_row": np.uint8, "segptr": np.int64,
                "segdst": np.int64, "segi": np.int32, "segj": np.int32, "pa": np.int32,
                "pb": np.int32, "dseg": np.int64, "ftasks": TASK_DTYPE, "stasks": TASK_DTYPE,
-               "csr_index": np.int64, "ws_elems": np.int64, "stage_elems": np.int64, "long_prows": np.int32,
+               "csr_index": np.int64, "stage_elems": np.int64, "long_prows": np.int32,
                "long_cols": np.int32, "long_segs": np.int32, "long_diag": np.int32}
 
 

@@ -58,66 +58,55 @@ class Emulator:
         self.diag[off:off + kb * kb] = Lr.reshape(-1)
 
     def factor(self):
-        """Left-looking schedule: COLUPD (kind 0), DIAG (1), ROWS (4), UPDATE (2), DIAGWB (3).
-        DIAG stages Lkk and never writes the panel's diagonal block; the pivots and the
-        row-major copy carry it until DIAGWB."""
+        """Left-looking schedule: DIAG (kind 1, block column 0), STEP (5), UPDATE (2), DIAGWB (3).
+        Diagonal factors live in staging slots between launches; the panel's diagonal blocks
+        are only written by DIAGWB."""
         p = self.p
         T = p["ftasks"]
-        MT, NC, ROWS = 128, 64, 128
-        ws = np.zeros(max(int(p["ws_elems"][0]), 1))
+        MT = 128
         self.stage = {}
         for kind, b, c in self.fl:
+            assert kind in (1, 2, 3, 5)
             for t in T[b:b + c]:
                 if self.err:
                     return False
                 k0, kb, r0, c0, ld, nc = (int(t[x]) for x in ("k0", "kb", "r0", "c0", "ld", "ncol"))
-                kbeg, kend = int(t["kbeg"]), int(t["kend"])
-                assert kind in (0, 1, 2, 3, 4)
-                if kind == 0:
-                    ra = min(MT, ld - r0)
-                    prod = self.blk(t, r0, r0 + ra, kbeg, kend) @ self.blk(t, k0, k0 + kb, kbeg, kend).T
-                    if int(t["ws"]) < 0:
-                        self.blk(t, r0, r0 + ra, k0, k0 + kb)[:] -= prod
-                    else:
-                        W = np.zeros((NC, MT))
-                        W[:kb, :ra] = prod.T
-                        ws[int(t["ws"]):int(t["ws"]) + MT * NC] = W.reshape(-1)
-                elif kind in (1, 4):
-                    ns, base = int(t["nsplit"]), int(t["aux2"])
-
-                    def partial(rows):   # sum of the split-K partials for panel rows `rows`
-                        acc = np.zeros((len(rows), kb))
-                        for i, row in enumerate(rows):
-                            g, rr = divmod(row - k0, MT)
-                            for j in range(ns):
-                                o = base + (g * ns + j) * MT * NC
-                                acc[i] += ws[o:o + MT * NC].reshape(NC, MT)[:kb, rr]
-                        return acc
-                    slot = int(t["ws"])
-                    if kind == 1:        # DIAG: factor, stage, copy, pivots (panel untouched)
-                        D = self.blk(t, k0, k0 + kb, k0, k0 + kb).copy() - partial(range(k0, k0 + kb))
-                        L = self._factor_block(np.tril(D) + np.tril(D, -1).T)
-                        if L is None:
+                first = int(t["first"])
+                if kind == 1:
+                    D = self.blk(t, k0, k0 + kb, k0, k0 + kb).copy()
+                    L = self._factor_block(np.tril(D) + np.tril(D, -1).T)
+                    if L is None:
+                        return False
+                    self._publish(L, int(t["ws"]), first + k0, int(t["aux"]))
+                elif kind == 5:
+                    ra = int(t["c0"])
+                    X = self.blk(t, r0, r0 + ra, k0, k0 + kb)
+                    X -= self.blk(t, r0, r0 + ra, 0, k0) @ self.blk(t, k0, k0 + kb, 0, k0).T
+                    L = self.stage[int(t["ws"])]
+                    X[:] = np.linalg.solve(L, X.T).T
+                    for h in range(2):            # diagonal blocks inside the tile
+                        d0 = r0 + 64 * h
+                        if 64 * h >= ra or d0 >= nc:
+                            break
+                        db = min(64, ra - 64 * h, nc - d0)
+                        Xd = self.blk(t, d0, d0 + db, k0, k0 + kb)
+                        C = self.blk(t, d0, d0 + db, d0, d0 + db)
+                        C -= np.tril(Xd @ Xd.T)
+                    if int(t["nsplit"]):
+                        D = self.blk(t, r0, r0 + ra, r0, r0 + ra).copy()
+                        L1 = self._factor_block(np.tril(D) + np.tril(D, -1).T)
+                        if L1 is None:
                             return False
-                        self.stage[slot] = L
-                        first = int(t["first"])
-                        self.piv[first + k0:first + k0 + kb] = np.diag(L)
-                        self._diag_copy(L, int(t["aux"]))
-                    else:                # ROWS: solve the tile's rows below the diagonal block
-                        L = self.stage[slot]
-                        lo, hi = max(r0, k0 + kb), min(r0 + ROWS, ld)
-                        if hi > lo:
-                            X = self.blk(t, lo, hi, k0, k0 + kb)
-                            X[:] = np.linalg.solve(L, (X - partial(range(lo, hi))).T).T
+                        self._publish(L1, int(t["aux2"]), first + r0, int(t["aux"]))
                 elif kind == 3:
-                    first = int(t["first"])
                     Lr = self.diag[int(t["aux"]):int(t["aux"]) + kb * kb].reshape(kb, kb)
                     Lk = np.tril(Lr, -1) + np.diag(self.piv[first + k0:first + k0 + kb])
                     Dp = self.blk(t, k0, k0 + kb, k0, k0 + kb)
                     Dp[:] = np.where(np.tri(kb, dtype=bool), Lk, Dp)
                 else:
+                    kbeg, kend = int(t["kbeg"]), int(t["kend"])
                     rows = p["sn_rows"][int(t["aux2"]):int(t["aux2"]) + ld]
-                    ra, rb = min(MT, ld - r0), min(NC, ld - c0)
+                    ra, rb = min(MT, ld - r0), min(64, ld - c0)
                     U = self.blk(t, r0, r0 + ra, kbeg, kend) @ self.blk(t, c0, c0 + rb, kbeg, kend).T
                     for i in range(ra):
                         for j in range(rb):
@@ -134,6 +123,11 @@ class Emulator:
                                 return False
                             self.panel[p["sn_panel"][s] + (gj - f) * (rp1 - rp0) + pos] -= U[i, j]
         return True
+
+    def _publish(self, L, slot, col, dcopy):
+        self.stage[slot] = L
+        self.piv[col:col + L.shape[0]] = np.diag(L)
+        self._diag_copy(L, dcopy)
 
     def dense_L(self):
         """L in the plan's (postordered) permutation as a dense lower triangle."""

@@ -1,5 +1,5 @@
-"""Lane layouts of the MFMA builtins, the 128x64 tile GEMM and the 64x64 diagonal factorization
-used by the Cholesky kernels."""
+"""Lane layouts of the MFMA builtins, the 128x64 tile GEMM (NT and transposed TN accumulator
+layouts) and the 64x64 diagonal factorization used by the Cholesky kernels."""
 import ctypes as C
 
 import numpy as np
@@ -44,14 +44,16 @@ def test_tile_gemm(gpu, prec, dt, K, ra, rb):
     A = rng.standard_normal((K, lda)).astype(dt)     # column k of the panel = A[k, :]
     B = rng.standard_normal((K, ldb)).astype(dt)
     da, db = _dev(gpu, A), _dev(gpu, B)
-    out = gpu.DeviceBuffer(128 * 64 * np.dtype(dt).itemsize, dt)
+    out = gpu.DeviceBuffer(2 * 128 * 64 * np.dtype(dt).itemsize, dt)
     assert L.lkd_probe_tile_gemm(prec, da.addr, lda, ra, db.addr, ldb, rb, K, out.addr, None) == 0
-    got = out.download().reshape(64, 128).T           # [r][c]
+    both = out.download().reshape(2, 64, 128)
+    got, got_tn = both[0].T, both[1].T                 # [r][c] from the NT and the TN core
     Am = np.zeros((128, K)); Am[:ra] = A[:, :ra].T
     Bm = np.zeros((64, K)); Bm[:rb] = B[:, :rb].T
     exp = Am @ Bm.T
     tol = 1e-12 if dt == np.float64 else 1e-4
     assert np.abs(got - exp).max() <= tol * max(1.0, np.abs(exp).max())
+    assert np.abs(got_tn - exp).max() <= tol * max(1.0, np.abs(exp).max())
 
 
 @pytest.mark.parametrize("prec,dt,kb", [(0, np.float64, 64), (0, np.float64, 17), (1, np.float32, 64)])
