@@ -143,21 +143,16 @@ int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
 
 /* ---------------------------------------------------------------- triangular solves */
 /* Replaces cusparseSpSV (L and L^T) (src/cuda/kkt.cu:731-734,788-789).  x is the permuted
- * right-hand side, overwritten with the solution; rows index the supernode row lists. */
-/* fwd, first_step = 1: solve the first diagonal block of each listed supernode.
- * fwd, first_step = 0: 256-row tiles of block column k0 update the rows below it (own rows in
- *      place, rows of ancestors by atomic add); the tile starting at k0+kb < ncol then solves the
- *      next diagonal block.
- * bwd, below_step = 1: column block c0 subtracts L(below rows, block)^T x(below rows); the last
- *      block of the supernode then solves its diagonal block with L^T.
- * bwd, below_step = 0: column block c0 subtracts L(rows of block k0, block)^T x(block k0); the
- *      block c0 = k0 - kb then solves its diagonal block. */
-int lkd_trsv_fwd(int prec, int32_t ntask, const lkd_task* tasks, const void* panel,
-                 const void* diag, const int32_t* sn_rows, double* x, int first_step,
-                 void* stream);
-int lkd_trsv_bwd(int prec, int32_t ntask, const lkd_task* tasks, const void* panel,
-                 const void* diag, const int32_t* sn_rows, double* x, int below_step,
-                 void* stream);
+ * vector, solved in place: L z = x (dir 0, levels leaves first) or L^T x = z (dir 1, root
+ * first), one launch per etree level, one workgroup of nw (4 or 8) wavefronts per supernode.
+ * task.aux = offset of the supernode's first diagonal copy, task.aux2 = row-list offset,
+ * task.kbeg = offset of its rows below in `below` (scratch, one double per row below a
+ * supernode), task.ws = offset of its partial sums in `pbuf` (backward: nb * nw * 64 doubles,
+ * nb = 64-column blocks of the supernode).  Waits between wavefronts are bounded: a timeout
+ * sets err bit 0x10 and the result is invalid. */
+int lkd_trsv_level(int prec, int dir, int nw, int32_t ntask, const lkd_task* tasks,
+                   const void* panel, const void* diag, const int32_t* sn_rows, double* x,
+                   double* below, double* pbuf, int32_t* err, void* stream);
 
 /* ---------------------------------------------------------------- KKT solve vector kernels */
 /* r[i] = rhs[P[i]]; req[i] = D[i] * r[i] */

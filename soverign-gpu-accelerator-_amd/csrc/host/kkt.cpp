@@ -37,12 +37,14 @@ constexpr int GEMM_MT = 128;        // COLUPD / UPDATE tile rows
 constexpr int GEMM_NC = 64;         // COLUPD / UPDATE tile columns
 constexpr int GEMM_TARGET_WG = 1024;   // UPDATE K splitting aims at this many workgroups
 constexpr int KSPLIT_MIN = 128;     // minimum K columns per split part
+constexpr int SOLVE_WIDE_TILES = 8;   // level solves: 8 wavefronts per supernode above this
 constexpr int FWD_ROWS = 256;
 
 struct Launch {
-    int kind;        // 1 diag 2 update 3 diagwb 5 step | 10 fwd-first 11 fwd-step 12 bwd-below 13 bwd-step
+    int kind;        // 1 diag 2 update 3 diagwb 5 step | 20 forward level 21 backward level
     int64_t begin;
     int32_t count;
+    int32_t nw = 0;  // level solves: wavefronts per workgroup
 };
 
 template <typename T>
@@ -102,6 +104,8 @@ struct GPUKKTCholeskySolver::Impl {
     void* d_piv = nullptr;         // pivots L_jj per permuted column (sized for FP64)
     void* d_stage = nullptr;       // staged diagonal factors, LKD_STAGE per supernode of a level
     int64_t stage_elems = 0;
+    double* d_below = nullptr;     // level solves: rows below each supernode
+    double* d_pbuf = nullptr;      // level solves: backward partial sums
     int32_t* d_err = nullptr;
     double *d_w = nullptr, *d_D = nullptr, *d_m0d = nullptr, *d_theta = nullptr;
     double *d_r = nullptr, *d_req = nullptr, *d_z = nullptr, *d_dy = nullptr, *d_bak = nullptr,
@@ -513,105 +517,69 @@ struct GPUKKTCholeskySolver::Impl {
                 if (t.c0 < nc || t.c0 >= ld || t.r0 < nc || t.r0 >= ld || t.c0 >= t.r0 + GEMM_MT) bad("update");
                 if (t.kbeg < 0 || t.kend > nc || t.kbeg >= t.kend) bad("update K");
                 break;
-            case 10: if (t.k0 != 0 || !diag_ok(t.aux, t.kb)) bad("fwd first"); break;
-            case 11:
-                if (t.r0 < t.k0 + t.kb || t.r0 >= ld) bad("fwd step");
-                if (t.r0 == t.k0 + t.kb && t.r0 < nc && !diag_ok(t.aux, std::min<int>(TILE, nc - t.r0))) bad("fwd diag");
+            case 20:
+            case 21: {  // level solves: wavefronts per workgroup, scratch / partial-sum offsets
+                const int nb = (nc + TILE - 1) / TILE;
+                if (t.c0 != 4 && t.c0 != 8) bad("solve wavefronts");
+                if (t.ws < 0 || t.ws + int64_t(nb) * t.c0 * TILE > solve_pbuf || t.kbeg < 0 ||
+                    t.kbeg + (ld - nc) > solve_below)
+                    bad("solve offsets");
+                if (!diag_ok(t.aux, 0) || t.aux + int64_t(nb - 1) * TILE * TILE + int64_t(nc - (nb - 1) * TILE) * (nc - (nb - 1) * TILE) > diag_elems)
+                    bad("solve diagonal copies");
                 break;
-            case 12: if (t.c0 < 0 || t.c0 >= nc || !diag_ok(t.aux, std::min<int>(TILE, nc - t.c0))) bad("bwd below"); break;
-            default:
-                if (t.c0 < 0 || t.c0 + TILE > t.k0 || t.k0 >= nc || !diag_ok(t.aux, std::min<int>(TILE, nc - t.c0))) bad("bwd step");
-                break;
+            }
+            default: bad("kind");
         }
     }
 
+    // Solve plan: per etree level one launch per direction, one workgroup per supernode with
+    // 8 wavefronts when some supernode of the level has more than SOLVE_WIDE_TILES row tiles,
+    // else 4.  task.c0 = wavefronts, task.ws = partial-sum offset, task.kbeg = scratch offset.
     void build_solve_plan() {
         std::vector<lkd_task> tasks;
-        auto base = [&](Index s) {
-            lkd_task t{};
-            t.off = S.sn_panel[s];
-            t.ld = static_cast<int32_t>(S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
-            t.ncol = S.sn_first[s + 1] - S.sn_first[s];
-            t.sn = s;
-            t.first = S.sn_first[s];
-            t.aux2 = S.sn_rowptr[s];
-            return t;
-        };
-        auto emit = [&](int kind, int64_t begin) {
-            const int64_t cnt = static_cast<int64_t>(tasks.size()) - begin;
-            if (cnt > 0) slaunch.push_back({kind, begin, static_cast<int32_t>(cnt)});
-        };
-        // forward, leaves first
+        std::vector<std::vector<lkd_task>> per_level;
+        int64_t nbelow = 0, npbuf = 0;
         for (const auto& lv : levels) {
-            int maxblk = 0;
-            int64_t b0 = static_cast<int64_t>(tasks.size());
-            for (Index s : lv) {
-                lkd_task t = base(s);
-                t.k0 = 0;
-                t.kb = std::min(TILE, t.ncol);
+            int nw = 4;
+            for (const Index s : lv) {
+                const Index nc = S.sn_first[s + 1] - S.sn_first[s];
+                const int64_t ld = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
+                if ((nc + TILE - 1) / TILE + (ld - nc + TILE - 1) / TILE > SOLVE_WIDE_TILES) nw = 8;
+            }
+            std::vector<lkd_task> lt;
+            for (const Index s : lv) {
+                lkd_task t{};
+                t.off = S.sn_panel[s];
+                t.ld = static_cast<int32_t>(S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
+                t.ncol = S.sn_first[s + 1] - S.sn_first[s];
+                t.sn = s;
+                t.first = S.sn_first[s];
                 t.aux = diag_off(s, 0);
-                tasks.push_back(t);
-                maxblk = std::max(maxblk, (t.ncol + TILE - 1) / TILE);
+                t.aux2 = S.sn_rowptr[s];
+                t.c0 = nw;
+                t.ws = npbuf;
+                t.kbeg = static_cast<int32_t>(nbelow);
+                npbuf += int64_t((t.ncol + TILE - 1) / TILE) * nw * TILE;
+                nbelow += t.ld - t.ncol;
+                lt.push_back(t);
             }
-            emit(10, b0);
-            for (int k = 0; k < maxblk; ++k) {
-                b0 = static_cast<int64_t>(tasks.size());
-                for (Index s : lv) {
-                    lkd_task t = base(s);
-                    const int nb = (t.ncol + TILE - 1) / TILE;
-                    if (k >= nb) continue;
-                    t.k0 = k * TILE;
-                    t.kb = std::min(TILE, t.ncol - t.k0);
-                    for (int r0 = t.k0 + t.kb; r0 < t.ld; r0 += FWD_ROWS) {
-                        lkd_task u = t;
-                        u.r0 = r0;
-                        u.aux = (r0 < t.ncol) ? diag_off(s, k + 1) : 0;
-                        tasks.push_back(u);
-                    }
-                }
-                emit(11, b0);
-            }
+            per_level.push_back(std::move(lt));
         }
-        // backward, root first
-        for (auto it = levels.rbegin(); it != levels.rend(); ++it) {
-            const auto& lv = *it;
-            int maxblk = 0;
-            int64_t b0 = static_cast<int64_t>(tasks.size());
-            for (Index s : lv) {
-                lkd_task t = base(s);
-                const int nb = (t.ncol + TILE - 1) / TILE;
-                maxblk = std::max(maxblk, nb);
-                for (int cb = (t.ld > t.ncol ? 0 : nb - 1); cb < nb; ++cb) {
-                    lkd_task u = t;
-                    u.c0 = cb * TILE;
-                    u.aux = diag_off(s, cb);
-                    tasks.push_back(u);
-                }
-            }
-            emit(12, b0);
-            for (int k = maxblk - 1; k >= 1; --k) {
-                b0 = static_cast<int64_t>(tasks.size());
-                for (Index s : lv) {
-                    lkd_task t = base(s);
-                    const int nb = (t.ncol + TILE - 1) / TILE;
-                    if (k >= nb) continue;
-                    t.k0 = k * TILE;
-                    t.kb = std::min(TILE, t.ncol - t.k0);
-                    for (int cb = 0; cb < k; ++cb) {
-                        lkd_task u = t;
-                        u.c0 = cb * TILE;
-                        u.aux = diag_off(s, cb);
-                        tasks.push_back(u);
-                    }
-                }
-                emit(13, b0);
-            }
-        }
+        auto emit = [&](int kind, const std::vector<lkd_task>& lt) {
+            slaunch.push_back({kind, static_cast<int64_t>(tasks.size()), static_cast<int32_t>(lt.size()),
+                               lt.empty() ? 4 : lt[0].c0});
+            tasks.insert(tasks.end(), lt.begin(), lt.end());
+        };
+        for (const auto& lt : per_level) emit(20, lt);                                       // forward
+        for (auto it = per_level.rbegin(); it != per_level.rend(); ++it) emit(21, *it);      // backward
+        solve_pbuf = npbuf;
+        solve_below = nbelow;
         for (const auto& L : slaunch)
             for (int64_t q = L.begin; q < L.begin + L.count; ++q) validate(tasks[q], L.kind);
         d_stasks = up(tasks, "stasks");
         n_stasks = static_cast<int64_t>(tasks.size());
     }
+    int64_t solve_pbuf = 0, solve_below = 0;
 
     // reference CSR layout of L (sym.L_pattern rows + diagonal last) -> panel indices
     void build_csr_index(const ipm::SymbolicFactorization& sym) {
@@ -647,6 +615,10 @@ struct GPUKKTCholeskySolver::Impl {
         owned.push_back(d_piv);
         d_stage = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(stage_elems, 1)) * sizeof(double));
         owned.push_back(d_stage);
+        d_below = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(solve_below, 1)) * sizeof(double)));
+        owned.push_back(d_below);
+        d_pbuf = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(solve_pbuf, 1)) * sizeof(double)));
+        owned.push_back(d_pbuf);
         auto dv = [&](int64_t len) {
             double* p = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(len, 1)) * sizeof(double)));
             owned.push_back(p);
@@ -784,16 +756,20 @@ struct GPUKKTCholeskySolver::Impl {
 
     void run_solve_schedule(int prec, double* x) {
         for (const auto& L : slaunch) {
-            const lkd_task* t = d_stasks + L.begin;
-            int e = 0;
-            switch (L.kind) {
-                case 10: e = lkd_trsv_fwd(prec, L.count, t, d_panel, d_diag, d_sn_rows, x, 1, stream); break;
-                case 11: e = lkd_trsv_fwd(prec, L.count, t, d_panel, d_diag, d_sn_rows, x, 0, stream); break;
-                case 12: e = lkd_trsv_bwd(prec, L.count, t, d_panel, d_diag, d_sn_rows, x, 1, stream); break;
-                default: e = lkd_trsv_bwd(prec, L.count, t, d_panel, d_diag, d_sn_rows, x, 0, stream); break;
-            }
-            check(e, "solve kernel");
+            const int dir = L.kind == 20 ? 0 : 1;
+            check(lkd_trsv_level(prec, dir, L.nw, L.count, d_stasks + L.begin, d_panel, d_diag, d_sn_rows,
+                                 x, d_below, d_pbuf, d_err + 1, stream), "solve kernel");
         }
+    }
+
+    // A level solve whose waits timed out produced an invalid vector: report it.
+    void check_solve_sync() {
+        int32_t e = 0;
+        check(lkd_memcpy_dtoh(&e, d_err + 1, sizeof(int32_t), stream), "solve status");
+        if (e == 0) return;
+        check(lkd_memset(d_err + 1, 0, sizeof(int32_t), stream), "reset");
+        check(lkd_stream_sync(stream), "reset");
+        throw std::runtime_error("KKT solve: level synchronisation timed out");
     }
 
     void trsv(int prec, double* x) {
@@ -949,6 +925,7 @@ struct GPUKKTCholeskySolver::Impl {
             ok = backward_ok(best);
         }
         st.refinement_steps += its;
+        check_solve_sync();
         check(lkd_kkt_scatter(m, d_perm, d_dy, d_rhs, stream), "scatter");
         if (info) {
             info->retries = retries_used;
@@ -993,6 +970,7 @@ struct GPUKKTCholeskySolver::Impl {
             }
         }
         st.refinement_steps += steps;
+        check_solve_sync();
         check(lkd_kkt_scatter(m, d_perm, d_dy, d_rhs, stream), "scatter");
         if (info) {
             info->retries = retries_used;
@@ -1137,6 +1115,7 @@ void GPUKKTCholeskySolver::triangular_solve_permuted(Float* d_x) {
     const int prec = I.cached_prec == FactorPrecision::FP32 && I.factor_kind == 2 ? LKD_F32 : LKD_F64;
     I.run_solve_schedule(prec, d_x);
     check(lkd_stream_sync(I.stream), "sync");
+    I.check_solve_sync();
 }
 
 const KKTStats& GPUKKTCholeskySolver::stats() const { return impl_->st; }

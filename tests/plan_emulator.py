@@ -151,43 +151,44 @@ class Emulator:
         return np.tril(Lr, -1) + np.diag(1.0 / np.diag(Lr))
 
     def solve(self, x):
-        """In-place L L^T solve of the permuted vector x with the solve schedule."""
+        """In-place L L^T solve of the permuted vector x with the level schedule (kinds 20 / 21):
+        one workgroup per supernode of a level, all with the same wavefront count; per
+        supernode the blocks are processed in order, rows below go to the ancestors' entries
+        (forward) or are read from them (backward).  Checks the plan's scratch and partial-sum
+        ranges (disjoint per supernode) on the way."""
         p = self.p
         T = p["stasks"]
         for kind, b, c in self.sl:
-            for t in T[b:b + c]:
-                k0, kb, r0, c0, ld, nc = (int(t[q]) for q in ("k0", "kb", "r0", "c0", "ld", "ncol"))
-                first = int(t["first"])
+            assert kind in (20, 21)
+            tl = T[b:b + c]
+            assert len({int(u["c0"]) for u in tl}) == 1 and int(tl[0]["c0"]) in (4, 8)
+            assert len({int(u["sn"]) for u in tl}) == len(tl)
+            for t in tl:
+                nbk = (int(t["ncol"]) + TS - 1) // TS
+                for u in tl:
+                    if u["sn"] != t["sn"]:
+                        a0, a1 = int(t["ws"]), int(t["ws"]) + nbk * int(t["c0"]) * TS
+                        b0 = int(u["ws"])
+                        b1 = b0 + (int(u["ncol"]) + TS - 1) // TS * int(u["c0"]) * TS
+                        assert a1 <= b0 or b1 <= a0
+                        a0, a1 = int(t["kbeg"]), int(t["kbeg"]) + int(t["ld"]) - int(t["ncol"])
+                        b0, b1 = int(u["kbeg"]), int(u["kbeg"]) + int(u["ld"]) - int(u["ncol"])
+                        assert a1 <= b0 or b1 <= a0 or a0 == a1 or b0 == b1
+                ld, nc, first = int(t["ld"]), int(t["ncol"]), int(t["first"])
                 rows = p["sn_rows"][int(t["aux2"]):int(t["aux2"]) + ld]
-                if kind == 10:
-                    L = self._diag_L(int(t["aux"]), kb)
-                    x[first + k0:first + k0 + kb] = np.linalg.solve(L, x[first + k0:first + k0 + kb])
-                elif kind == 11:
-                    z = x[first + k0:first + k0 + kb].copy()
-                    r1 = min(r0 + 256, ld)
-                    acc = self.blk(t, r0, r1, k0, k0 + kb) @ z
-                    for i, row in enumerate(range(r0, r1)):
-                        if row < nc:
-                            x[first + row] -= acc[i]
-                        else:
-                            x[rows[row]] -= acc[i]
-                    if r0 == k0 + kb and r0 < nc:
-                        kb2 = min(TS, nc - r0)
-                        L = self._diag_L(int(t["aux"]), kb2)
-                        x[first + r0:first + r0 + kb2] = np.linalg.solve(L, x[first + r0:first + r0 + kb2])
+                nb = (nc + TS - 1) // TS
+                Lfull = self.blk(t, 0, ld, 0, nc)
+                Ld = np.zeros((nc, nc))
+                for j in range(nb):
+                    kb = min(TS, nc - j * TS)
+                    Ld[j * TS:j * TS + kb, j * TS:j * TS + kb] = self._diag_L(int(t["aux"]) + j * TS * TS, kb)
+                    Ld[j * TS + kb:, j * TS:j * TS + kb] = Lfull[j * TS + kb:nc, j * TS:j * TS + kb]
+                if kind == 20:
+                    z = np.linalg.solve(Ld, x[first:first + nc])
+                    x[first:first + nc] = z
+                    if ld > nc:
+                        np.subtract.at(x, rows[nc:ld], Lfull[nc:ld] @ z)
                 else:
-                    kbc = min(TS, nc - c0)
-                    if kind == 12:
-                        rr = np.arange(nc, ld)
-                        xv = x[rows[nc:ld]]
-                        last = c0 + kbc == nc
-                    else:
-                        rr = np.arange(k0, k0 + kb)
-                        xv = x[first + k0:first + k0 + kb]
-                        last = c0 + kbc == k0
-                    if len(rr):
-                        x[first + c0:first + c0 + kbc] -= self.blk(t, rr[0], rr[-1] + 1, c0, c0 + kbc).T @ xv
-                    if last:
-                        L = self._diag_L(int(t["aux"]), kbc)
-                        x[first + c0:first + c0 + kbc] = np.linalg.solve(L.T, x[first + c0:first + c0 + kbc])
+                    y = x[first:first + nc] - (Lfull[nc:ld].T @ x[rows[nc:ld]] if ld > nc else 0.0)
+                    x[first:first + nc] = np.linalg.solve(Ld.T, y)
         return x
