@@ -8,12 +8,18 @@ set up once (Ruiz scaling, AMD ordering, supernodal plan and data resident in HB
 timed region).  Every timed solve is checked: status Optimal, the 1e-8 criteria, the FP64
 certificate, and the objective against the HiGHS golden value when one is recorded.
 
-roofline: the supernodal FP64 factorization (MFMA-bound).  achieved = sum_j c_j^2 (exact column
-counts of L) / device time of one factorization, measured with HIP events on the solver's
-stream; the SYRK tile kernel (the dominant launch kind) is reported beside it.
+roofline: the dominant kernel, k_step (one launch = one 64-column block-column step of the
+supernodal FP64 factorization, MFMA-bound).  Its algorithmic flops per factorization are
+F_chol = sum_j c_j^2 (exact column counts of L) minus the panel flops of the other launch kinds
+(first diagonal blocks, ancestor updates); achieved = those flops x factorizations timed / the
+summed device time of its launches, measured live over the timed solves with HIP events on the
+solver's stream at every change of launch kind (runs of consecutive k_step launches are timed
+as a whole, so the short gaps between them count against the kernel).  traffic: HBM bytes per k_step launch from
+the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), scaled by the same counters
+on a copy kernel of known size.
 cpu_baseline: the C oracle (same algorithm, dense FP64 Cholesky, 1 thread) on one block of the
-workload (1,550 x 4,000 incl. the 50 linking rows) for two IPM iterations, scaled to 64 blocks x
-the GPU's iteration count.
+workload (1,550 x 4,000 incl. the 50 linking rows) for CPU_ITERS IPM iterations, scaled to 64
+blocks x the GPU's iteration count.
 """
 import argparse
 import json
@@ -30,6 +36,8 @@ sys.path[:0] = [os.path.join(ROOT, "soverign-gpu-accelerator-_amd"), os.path.joi
 METRIC = "LP solves/sec & IPM iters to 1e-8 gap: chosen form vs plain FP64, 1 MI355X"
 PEAK_FP64 = 78.6e12
 PEAK_FP32 = 157.3e12
+CPU_ITERS = 8
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic_k_step_block_angular_64x1500x4000_fp64.json")
 
 
 def log(*a):
@@ -79,14 +87,40 @@ def cpu_baseline(gpu_iters, blocks, seed):
     lp = gen.block_angular(1, 1500, 4000, 8, 50, 0.05, seed=seed)
     csr = lp.csr()
     t = time.perf_counter()
-    pyoracle.ipm_solve(csr, lp.b, lp.c, lp.n, stop_rule=1, max_iter=2)
+    pyoracle.ipm_solve(csr, lp.b, lp.c, lp.n, stop_rule=1, max_iter=CPU_ITERS)
     dt = time.perf_counter() - t
-    per_iter = dt / 2.0
+    per_iter = dt / CPU_ITERS
     solve_s = per_iter * blocks * max(gpu_iters, 1)
     return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": (f"C oracle (same IPM, dense FP64 Cholesky, 1 thread) on 1 of {blocks} "
-                       f"blocks (m=1550, n=4000) for 2 iterations = {dt:.1f} s; scaled x{blocks} "
+                       f"blocks (m=1550, n=4000) for {CPU_ITERS} iterations = {dt:.1f} s; scaled x{blocks} "
                        f"blocks x {gpu_iters} iterations = {solve_s:.0f} s per solve")}
+
+
+def step_roofline(live, st, fp32):
+    peak = PEAK_FP32 if fp32 else PEAK_FP64
+    nfact = max(live["factorization"]["launches"], 1)
+    step = live["step"]
+    # algorithmic flops of the step launches: the exact Cholesky flops minus the (stored) work
+    # of the other kinds, so the step's share is not overstated
+    f_step = st["flops_exact"] - (live["diag"]["flops"] + live["update"]["flops"]) / nfact
+    achieved = f_step * nfact / (step["ms"] * 1e-3)
+    per_launch = step["ms"] / max(step["launches"], 1)
+    traffic = None
+    if os.path.exists(TRAFFIC):
+        with open(TRAFFIC) as f:
+            tr = json.load(f)
+        traffic = tr["bytes_per_launch"]
+    fac = live["factorization"]
+    return {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": traffic,
+            "kernel": "k_step (supernodal block-column step: MFMA GEMM + solve + next diagonal)",
+            "launches": step["launches"], "avg_launch_ms": per_launch,
+            "algorithmic_flops_per_launch": f_step * nfact / max(step["launches"], 1),
+            "stored_flops_per_launch": step["flops"] / max(step["launches"], 1),
+            "factorizations": nfact,
+            "factorization": {"ms_avg": fac["ms"] / nfact, "flops_exact": st["flops_exact"],
+                              "achieved_tflops": st["flops_exact"] / (fac["ms"] / nfact * 1e-3) / 1e12}}
 
 
 def copy_calibration(mib, reps):
@@ -164,6 +198,7 @@ def main():
         log(f"warmup {w}: {r['status_name']} iters={r['iterations']} obj={r['objective']:.12g} "
             f"gap={r['rel_gap']:.2e} {r['solve_ms']:.0f} ms")
 
+    solver.set_launch_timing(True)
     results = []
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -181,19 +216,10 @@ def main():
             f"pcg={r['refinement_steps']} fail={r.get('fail_code')} ok={ok}")
     iters = [r["iterations"] for r in results]
     value = args.steps / elapsed if all_ok else 0.0
-
-    prof = solver.profile_factor(fp32=args.precision == "mixed")
-    fac = prof["factorization"]
-    gemm = prof["step"]
-    peak = PEAK_FP32 if args.precision == "mixed" else PEAK_FP64
-    achieved = fac["flops"] / (fac["ms"] * 1e-3)
-    log("profile " + json.dumps(prof))
-    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                "kernel": "supernodal factorization (k_step/k_diag/k_update/k_diagwb)",
-                "flops_per_factorization": fac["flops"], "factorization_ms": fac["ms"],
-                "gemm": {"achieved_tflops": gemm["flops"] / max(gemm["ms"], 1e-9) / 1e9,
-                         "launches": gemm["launches"], "avg_launch_ms": gemm["ms"] / max(gemm["launches"], 1)}}
+    live = solver.launch_timing()
+    solver.set_launch_timing(False)
+    log("launch timing " + json.dumps(live))
+    roofline = step_roofline(live, st, args.precision == "mixed")
     cpu = None if args.no_cpu else cpu_baseline(int(np.median(iters)), args.blocks, args.seed)
 
     out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,

@@ -83,6 +83,8 @@ public:
     // Extension: factorization profile at the last iterate (see GPUKKTCholeskySolver) and the
     // symbolic statistics of the normal equations.
     void profile_factor(bool fp32, double ms[5], double flops[5], double launches[5]);
+    void set_launch_timing(bool on);
+    void launch_timing(double ms[5], double flops[5], double launches[5]);
     void kkt_statistics(double out[12]);
 
     class Impl;

@@ -139,9 +139,17 @@ int lk_ipm_run(lk_ipm* h, const double* c, const double* b, const double* warm_x
                const double* warm_y, const double* warm_s, lk_ipm_result* res, double* x,
                double* y, double* s);
 int lk_ipm_destroy(lk_ipm* h);
-/* Factorization profile at the last iterate (per launch kind 0 POTRF, 1 TRSM, 2 SYRK,
- * 3 UPDATE; 4 = whole factorization): device ms, algorithmic flops, launches (5 each). */
+/* Factorization profile at the last iterate, one extra factorization with events around every
+ * launch (slot 0 STEP, 1 DIAG + DIAGWB, 2 UPDATE, 3 assembly, 4 whole factorization): device
+ * ms, flops (stored panel work; slot 4: sum_j c_j^2 of the exact factor), launches (5 each).
+ * Extension: the reference's profiler (src/ipm/mehrotra_device.cu:45-91) times host phases. */
 int lk_ipm_profile_factor(lk_ipm* h, int32_t fp32, double* ms, double* flops, double* launches);
+/* Live timing of the factorizations run by lk_ipm_run: on != 0 starts (and resets) HIP-event
+ * timing of every factorization launch; read returns the totals so far in the slots above
+ * (slot 3 unused; slot 4: factorizations timed end to end).  Extension, no reference
+ * counterpart. */
+int lk_ipm_launch_timing(lk_ipm* h, int32_t on);
+int lk_ipm_read_launch_timing(lk_ipm* h, double* ms, double* flops, double* launches);
 /* m, n, nnz(A), supernodes, nnz(L) stored, nnz(L) exact, flops exact, flops stored, levels,
  * factor launches, solve launches, symbolic ms */
 int lk_ipm_kkt_stats(lk_ipm* h, double* out12);

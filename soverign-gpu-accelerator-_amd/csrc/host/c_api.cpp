@@ -465,6 +465,12 @@ int lk_ipm_destroy(lk_ipm* h) { return guarded([&] { delete h; }); }
 int lk_ipm_profile_factor(lk_ipm* h, int32_t fp32, double* ms, double* flops, double* launches) {
     return guarded([&] { h->solver->profile_factor(fp32 != 0, ms, flops, launches); });
 }
+int lk_ipm_launch_timing(lk_ipm* h, int32_t on) {
+    return guarded([&] { h->solver->set_launch_timing(on != 0); });
+}
+int lk_ipm_read_launch_timing(lk_ipm* h, double* ms, double* flops, double* launches) {
+    return guarded([&] { h->solver->launch_timing(ms, flops, launches); });
+}
 int lk_ipm_kkt_stats(lk_ipm* h, double* out12) {
     return guarded([&] { h->solver->kkt_statistics(out12); });
 }

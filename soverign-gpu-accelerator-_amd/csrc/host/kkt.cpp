@@ -193,6 +193,7 @@ struct GPUKKTCholeskySolver::Impl {
             if (sgraph[p]) lkd_graph_destroy(sgraph[p]);
         }
         if (stream) lkd_stream_sync(stream);
+        for (void* e : timer.events) lkd_event_destroy(e);
         for (auto it = owned.rbegin(); it != owned.rend(); ++it) {
             try { arena_p->free(*it); } catch (...) {}
         }
@@ -663,7 +664,93 @@ struct GPUKKTCholeskySolver::Impl {
     // -------------------------------------------------------------------- numeric phases
     void run_factor_schedule(int prec) {
         const lkd_snplan pl = plan();
-        for (const auto& L : flaunch) check(launch_factor(prec, L, pl), "factor kernel");
+        if (!timer.on) {
+            for (const auto& L : flaunch) check(launch_factor(prec, L, pl), "factor kernel");
+            return;
+        }
+        // one event at every change of launch kind: a segment of consecutive launches of one
+        // kind is timed as a whole (its launches follow each other on the stream)
+        const size_t need = flaunch.size() + 1;
+        if (timer.used + need > timer.events.size()) {
+            check(lkd_stream_sync(stream), "timing sync");
+            timer_collect();
+            while (timer.events.size() < need) {
+                void* e = nullptr;
+                check(lkd_event_create(&e), "event");
+                timer.events.push_back(e);
+            }
+        }
+        const size_t first = timer.used;
+        check(lkd_event_record(timer.events[timer.used++], stream), "event");
+        for (size_t i = 0; i < flaunch.size(); ++i) {
+            const int slot = slot_of(flaunch[i]);
+            if (i == 0 || slot != slot_of(flaunch[i - 1]))
+                timer.pending.push_back({slot, timer.used - 1, 0, 0, 0.0});
+            check(launch_factor(prec, flaunch[i], pl), "factor kernel");
+            auto& seg = timer.pending.back();
+            seg.count += 1;
+            seg.flops += launch_flops[i];
+            if (i + 1 == flaunch.size() || slot_of(flaunch[i + 1]) != slot) {
+                check(lkd_event_record(timer.events[timer.used++], stream), "event");
+                seg.end = timer.used - 1;
+            }
+        }
+        timer.pending.push_back({4, first, timer.used - 1, 1, S.flops_exact});
+    }
+
+    static int slot_of(const Launch& L) { return L.kind == 5 ? 0 : 1 + (L.kind == 2); }
+
+    // Live launch timing: HIP events on the solver's stream at every change of launch kind in
+    // the factorization schedule, resolved after the stream has drained.  Slots as
+    // profile_factor (0 STEP, 1 DIAG + DIAGWB, 2 UPDATE; 4 = whole factorizations).
+    struct Segment {
+        int slot;
+        size_t begin, end;
+        int count;
+        double flops;
+    };
+    struct LaunchTimer {
+        bool on = false;
+        std::vector<void*> events;
+        size_t used = 0;
+        std::vector<Segment> pending;
+        double ms[5] = {}, flops[5] = {}, launches[5] = {};
+    } timer;
+    std::vector<double> launch_flops;   // per factorization launch, stored (panel) flops
+
+    void timer_collect() {
+        for (const auto& sg : timer.pending) {
+            float t = 0;
+            check(lkd_event_elapsed_ms(&t, timer.events[sg.begin], timer.events[sg.end]), "elapsed");
+            timer.ms[sg.slot] += t;
+            timer.launches[sg.slot] += sg.count;
+            timer.flops[sg.slot] += sg.flops;
+        }
+        timer.pending.clear();
+        timer.used = 0;
+    }
+
+    void set_timing(bool on) {
+        if (launch_flops.size() != flaunch.size()) {
+            launch_flops.assign(flaunch.size(), 0.0);
+            for (size_t i = 0; i < flaunch.size(); ++i)
+                for (int64_t q = flaunch[i].begin; q < flaunch[i].begin + flaunch[i].count; ++q)
+                    launch_flops[i] += task_flops(h_ftasks[static_cast<size_t>(q)], flaunch[i].kind);
+        }
+        check(lkd_stream_sync(stream), "timing sync");
+        timer_collect();
+        timer.on = on;
+        for (int k = 0; k < 5; ++k) timer.ms[k] = timer.flops[k] = timer.launches[k] = 0;
+    }
+
+    void read_timing(double ms[5], double flops[5], double launches[5]) {
+        check(lkd_stream_sync(stream), "timing sync");
+        timer_collect();
+        for (int k = 0; k < 5; ++k) {
+            ms[k] = timer.ms[k];
+            flops[k] = timer.flops[k];
+            launches[k] = timer.launches[k];
+        }
     }
 
     int launch_factor(int prec, const Launch& L, const lkd_snplan& pl) {
@@ -1119,6 +1206,10 @@ void GPUKKTCholeskySolver::triangular_solve_permuted(Float* d_x) {
 }
 
 const KKTStats& GPUKKTCholeskySolver::stats() const { return impl_->st; }
+void GPUKKTCholeskySolver::set_launch_timing(bool on) { impl_->set_timing(on); }
+void GPUKKTCholeskySolver::launch_timing(double ms[5], double flops[5], double launches[5]) {
+    impl_->read_timing(ms, flops, launches);
+}
 void GPUKKTCholeskySolver::profile_factor(FactorPrecision prec, double ms[5], double flops[5],
                                           double launches[5]) {
     impl_->profile_factor(prec == FactorPrecision::FP32 ? LKD_F32 : LKD_F64, ms, flops, launches);

@@ -543,6 +543,10 @@ void MehrotraSolver::profile_factor(bool fp32, double ms[5], double flops[5], do
                                 ms, flops, launches);
     check(lkd_stream_sync(impl_->stream_), "sync");
 }
+void MehrotraSolver::set_launch_timing(bool on) { impl_->kkt_->set_launch_timing(on); }
+void MehrotraSolver::launch_timing(double ms[5], double flops[5], double launches[5]) {
+    impl_->kkt_->launch_timing(ms, flops, launches);
+}
 void MehrotraSolver::kkt_statistics(double out[12]) {
     const auto& st = impl_->kkt_->stats();
     const double v[12] = {double(st.m), double(st.n), double(st.nnz_A), double(st.nsuper),

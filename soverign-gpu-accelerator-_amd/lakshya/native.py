@@ -340,6 +340,21 @@ class IPM:
         return {k: {"ms": float(ms[i]), "flops": float(fl[i]), "launches": int(la[i])}
                 for i, k in enumerate(names)}
 
+    def set_launch_timing(self, on=True):
+        """HIP-event timing of every factorization launch of the following runs (resets)."""
+        L = lib()
+        L.lk_ipm_launch_timing.argtypes = [_vp, C.c_int32]
+        _ck(L.lk_ipm_launch_timing(self.h, int(on)))
+
+    def launch_timing(self):
+        L = lib()
+        L.lk_ipm_read_launch_timing.argtypes = [_vp, _f64p, _f64p, _f64p]
+        ms, fl, la = np.zeros(5), np.zeros(5), np.zeros(5)
+        _ck(L.lk_ipm_read_launch_timing(self.h, ms, fl, la))
+        names = ["step", "diag", "update", "assembly", "factorization"]
+        return {k: {"ms": float(ms[i]), "flops": float(fl[i]), "launches": int(la[i])}
+                for i, k in enumerate(names)}
+
     def kkt_stats(self):
         L = lib()
         L.lk_ipm_kkt_stats.argtypes = [_vp, _f64p]
