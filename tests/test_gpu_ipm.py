@@ -50,3 +50,44 @@ def test_long_rows_and_columns_match_oracle(gpu, oracle, case):
     assert abs(g["objective"] - o["objective"]) <= 1e-8 * max(1.0, abs(o["objective"]))
     ok, _ = gpu.verify_optimal(lp, g["x"], g["y"], tol=1e-8)
     assert ok
+
+
+GOLDEN_GPU = ["rs_60_150_3_s1", "rs_200_500_6_s2", "ba_4x60x150_L5_s3", "tr_12x12_side30_s4",
+              "dense_80_s5", "tr_100x100_side2000_s0", "rs_2000_5000_6_s0",
+              "ba_8x1500x4000_L50_s0", "ba_64x1500x4000_L50_s0"]
+
+
+def _golden(name):
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "objectives.json")) as f:
+        return json.load(f)[name]["objective"]
+
+
+def _check_certified(gpu, lp, r, gold):
+    assert r["status_name"] == "Optimal", r.get("fail_code")
+    assert abs(r["objective"] - gold) <= 1e-8 * max(1.0, abs(gold)), (r["objective"], gold)
+    assert r["rel_gap"] <= 1e-8 and r["rel_primal_residual"] <= 1e-8 and r["rel_dual_residual"] <= 1e-8
+    ok, _ = gpu.verify_optimal(lp, r["x"], r["y"], tol=1e-8)
+    assert ok
+
+
+@pytest.mark.parametrize("name", GOLDEN_GPU)
+def test_golden_objective_fp64(gpu, name):
+    """The plain FP64 device IPM on the seeded instances whose HiGHS objectives are committed
+    (rs_2000_5000: the 2,000 x 5,000 random LP; ba_8 / ba_64: the block-angular LP with 8 / 64
+    blocks of 1,500 x 4,000)."""
+    from golden_cases import build_case
+    lp = build_case(name)
+    _check_certified(gpu, lp, gpu.ipm_solve(lp, relative=True), _golden(name))
+
+
+@pytest.mark.parametrize("name", ["rs_200_500_6_s2", "tr_100x100_side2000_s0", "rs_2000_5000_6_s0",
+                                  "ba_8x1500x4000_L50_s0"])
+def test_golden_objective_mixed(gpu, name):
+    """FP32 factorization with FP64 PCG / refinement reaches the same certified optimum."""
+    from golden_cases import build_case
+    lp = build_case(name)
+    r = gpu.ipm_solve(lp, relative=True, precision=1)
+    _check_certified(gpu, lp, r, _golden(name))
+    assert r["fp32_factorizations"] > 0
