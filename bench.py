@@ -11,7 +11,7 @@ certificate, and the objective against the HiGHS golden value when one is record
 roofline: the dominant kernel, k_step (one launch = one 64-column block-column step of the
 supernodal FP64 factorization, MFMA-bound).  Its algorithmic flops per factorization are
 F_chol = sum_j c_j^2 (exact column counts of L) minus the panel flops of the other launch kinds
-(first diagonal blocks, ancestor updates); achieved = those flops x factorizations timed / the
+(first diagonal blocks, ancestor updates, split-K partial products); achieved = those flops x factorizations timed / the
 summed device time of its launches, measured live over the timed solves with HIP events on the
 solver's stream at every change of launch kind (runs of consecutive k_step launches are timed
 as a whole, so the short gaps between them count against the kernel).  traffic: HBM bytes per k_step launch from
@@ -103,7 +103,8 @@ def step_roofline(live, st, fp32):
     step = live["step"]
     # algorithmic flops of the step launches: the exact Cholesky flops minus the (stored) work
     # of the other kinds, so the step's share is not overstated
-    f_step = st["flops_exact"] - (live["diag"]["flops"] + live["update"]["flops"]) / nfact
+    f_step = st["flops_exact"] - (live["diag"]["flops"] + live["update"]["flops"] +
+                                  live["part"]["flops"]) / nfact
     achieved = f_step * nfact / (step["ms"] * 1e-3)
     per_launch = step["ms"] / max(step["launches"], 1)
     traffic = None
@@ -119,6 +120,7 @@ def step_roofline(live, st, fp32):
             "algorithmic_flops_per_launch": f_step * nfact / max(step["launches"], 1),
             "stored_flops_per_launch": step["flops"] / max(step["launches"], 1),
             "factorizations": nfact,
+            "part_launches": live["part"]["launches"], "part_ms": live["part"]["ms"],
             "factorization": {"ms_avg": fac["ms"] / nfact, "flops_exact": st["flops_exact"],
                               "achieved_tflops": st["flops_exact"] / (fac["ms"] / nfact * 1e-3) / 1e12}}
 

@@ -108,15 +108,15 @@ public:
 
     const KKTStats& stats() const;
     // One factorization of the last Theta with events around every launch: per kind (0 STEP,
-    // 1 DIAG + DIAGWB, 2 UPDATE, 3 assembly) device ms, stored-panel flops and launch count;
-    // index 4 = the whole factorization including assembly (flops = sum_j c_j^2 of the exact
-    // factor).
-    void profile_factor(FactorPrecision prec, double ms[5], double flops[5], double launches[5]);
+    // 1 DIAG + DIAGWB, 2 UPDATE, 3 assembly, 5 PART) device ms, stored-panel flops and launch
+    // count; index 4 = the whole factorization including assembly (flops = sum_j c_j^2 of the
+    // exact factor).
+    void profile_factor(FactorPrecision prec, double ms[6], double flops[6], double launches[6]);
     // Extension: live timing of the factorizations this object runs (HIP events around every
     // launch on its stream, slots as profile_factor; slot 4 counts factorizations and their
     // exact flops).  set_launch_timing resets the totals; launch_timing drains the stream.
     void set_launch_timing(bool on);
-    void launch_timing(double ms[5], double flops[5], double launches[5]);
+    void launch_timing(double ms[6], double flops[6], double launches[6]);
     const std::vector<Index>& permutation() const;     // ordering used (postordered)
     void* stream() const;
     Index rows() const;

@@ -82,9 +82,9 @@ public:
     void clear_warm_start();
     // Extension: factorization profile at the last iterate (see GPUKKTCholeskySolver) and the
     // symbolic statistics of the normal equations.
-    void profile_factor(bool fp32, double ms[5], double flops[5], double launches[5]);
+    void profile_factor(bool fp32, double ms[6], double flops[6], double launches[6]);
     void set_launch_timing(bool on);
-    void launch_timing(double ms[5], double flops[5], double launches[5]);
+    void launch_timing(double ms[6], double flops[6], double launches[6]);
     void kkt_statistics(double out[12]);
 
     class Impl;

@@ -140,8 +140,9 @@ int lk_ipm_run(lk_ipm* h, const double* c, const double* b, const double* warm_x
                double* y, double* s);
 int lk_ipm_destroy(lk_ipm* h);
 /* Factorization profile at the last iterate, one extra factorization with events around every
- * launch (slot 0 STEP, 1 DIAG + DIAGWB, 2 UPDATE, 3 assembly, 4 whole factorization): device
- * ms, flops (stored panel work; slot 4: sum_j c_j^2 of the exact factor), launches (5 each).
+ * launch (slot 0 STEP, 1 DIAG + DIAGWB, 2 UPDATE, 3 assembly, 4 whole factorization, 5 PART):
+ * device ms, flops (stored panel work; slot 4: sum_j c_j^2 of the exact factor), launches
+ * (6 each).
  * Extension: the reference's profiler (src/ipm/mehrotra_device.cu:45-91) times host phases. */
 int lk_ipm_profile_factor(lk_ipm* h, int32_t fp32, double* ms, double* flops, double* launches);
 /* Live timing of the factorizations run by lk_ipm_run: on != 0 starts (and resets) HIP-event

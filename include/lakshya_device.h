@@ -57,10 +57,13 @@ int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream);
  *   off    panel offset (elements) of the supernode, ld = panel rows, ncol = panel columns
  *   k0/kb  block column start / width inside the panel
  *   r0/c0  tile row / column start inside the panel (kernel specific)
- *   kbeg/kend  K range of an UPDATE task; STEP: kbeg = width of the next diagonal block
+ *   kbeg/kend  K range of an UPDATE task, of a PART task, and of a STEP tile's own GEMM
  *   ws     DIAG / STEP: staging slot offset (STEP reads it)
  *   nsplit STEP: 1 on the tile that also factors the next diagonal block
- *   sn     supernode id, first = first (permuted) column of the supernode                   */
+ *   sn     supernode id, first = first (permuted) column of the supernode
+ *   np/px  STEP: number of partial products (from the PART launch before it) added to the
+ *          tile's own GEMM, stored at work + px + s * LKD_PART (s < np); PART: px = where its
+ *          product goes                                                                    */
 typedef struct {
     int64_t off;
     int64_t aux;     /* diagonal-copy offset (DIAG / STEP next block / DIAGWB / TRSV) */
@@ -71,7 +74,8 @@ typedef struct {
     int32_t r0, c0;
     int32_t kbeg, kend;
     int32_t nsplit, sn;
-    int32_t first, pad;
+    int32_t first, np;
+    int64_t px;
 } lkd_task;
 
 typedef struct {
@@ -121,9 +125,12 @@ int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int6
 /* Replaces cholesky_level_kernel and its level loop (src/cuda/kkt.cu:217-285,582-596).
  * Left-looking per 64-column block column, per etree level:
  *   DIAG    factor block column 0's diagonal block (ws = staging slot, aux = diag copy)
- *   STEP k  128-row tiles below the diagonal: GEMM update + solve against the staged factor;
- *           the tile holding the next diagonal block (nsplit = 1, width kbeg) also factors it
- *           (staging slot aux2, diag copy aux)
+ *   PART    split K of a STEP launch with few tiles: one K range [kbeg, kend) of a tile's
+ *           GEMM, written to work (LKD_PART elements per product, the tile's thread layout)
+ *   STEP k  128-row tiles below the diagonal: GEMM update (own range kbeg..kend plus np
+ *           partial products) + solve against the staged factor; the tile holding the next
+ *           diagonal block (nsplit = 1, c0 rows) also factors it (staging slot aux2, diag
+ *           copy aux)
  *   UPDATE  ancestors (split K over kbeg..kend)
  * and DIAGWB (diagonal blocks into the panel) after the last level.
  * err: device int, bit 0 set when a pivot is not positive, bit 1 when an update falls outside
@@ -132,10 +139,13 @@ int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int6
  * reciprocal on the diagonal, read by TRSV.  piv: the pivots L_jj per permuted column.
  * stage: staged diagonal factors, LKD_STAGE elements per slot (all panel precision). */
 #define LKD_STAGE (64 * 64 + 4 * 256)
+#define LKD_PART (128 * 64)
 int lkd_chol_diag(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* diag,
                   void* piv, void* stage, int32_t* err, void* stream);
+int lkd_chol_part(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* work,
+                  int32_t* err, void* stream);
 int lkd_chol_step(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
-                  void* piv, void* stage, int32_t* err, void* stream);
+                  void* piv, void* stage, const void* work, int32_t* err, void* stream);
 int lkd_chol_update(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
                     lkd_snplan plan, int32_t* err, void* stream);
 int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* diag,

@@ -14,6 +14,8 @@
 //              X <- X Lkk^{-T}                         on MFMA (transposed blocked solve)
 //            and the tile holding the next diagonal block also forms, factors and stages it.
 //            Nothing waits for another workgroup: the launch boundary is the only barrier.
+//   PART     (steps with few tiles) the K range of each tile is split: a launch before the STEP
+//            computes the partial products, the STEP adds them in a fixed order.
 //   UPDATE   below-diagonal rows:  U = L21 * L21^T (split K) scattered (atomic add) into the
 //            ancestor panels.
 //   DIAGWB   after the last step, Lkk of every block column is copied into the panel (the
@@ -36,6 +38,7 @@ constexpr int PB = NC + 8;    // LDS pitch of the B chunk
 // followed by the inverses of its four 16x16 diagonal blocks (column-major, pitch 16).
 constexpr int STAGE = TS * TS + 4 * 256;
 static_assert(STAGE == LKD_STAGE, "staging slot size");
+static_assert(LKD_PART == 32 * 256, "partial product: 32 accumulator elements per thread");
 
 template <typename T> struct KChunk;
 template <> struct KChunk<double> { static constexpr int v = 16; };
@@ -380,10 +383,36 @@ __global__ __launch_bounds__(256) void k_diag(const lkd_task* __restrict__ tasks
 // Task fields: ws = staging slot of block k (written by the previous launch).  nsplit = 1 marks
 // the tile that is exactly the next diagonal block k1 = k0 + kb (c0 = kbeg rows): after its own
 // rank-kb update it factors C(k1, k1) and publishes it (slot aux2, copy aux).
+// PART: one K range [kbeg, kend) of a STEP tile's GEMM, L(R, kbeg:kend) L(k, kbeg:kend)^T,
+// stored as this thread's accumulator elements: work[px + i * 256 + thread], i < 32.  The STEP
+// launch that follows reads them back in the same layout (coalesced) and adds them in order.
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_part(const lkd_task* __restrict__ tasks, int ntask, const T* __restrict__ L,
+            T* __restrict__ work, int32_t* err) {
+    const int ti = task_index();
+    if (ti >= ntask) return;
+    __shared__ int s_err;
+    if (wg_error(err, &s_err)) return;
+    const lkd_task tk = tasks[ti];
+    __shared__ GemmLds<T> sm;
+    const T* P = L + tk.off;
+    Acc<T> acc[4][2];
+    zero_acc<T, 4, 2>(acc);
+    gemm_tn<T>(P + tk.r0, tk.ld, tk.c0, P + tk.k0, tk.ld, tk.kb, tk.kbeg, tk.kend, sm, acc);
+    T* out = work + tk.px + threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) out[((p * 2 + b) * 4 + e) * 256] = acc[p][b][e];
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T* __restrict__ diag,
-            T* __restrict__ piv, T* __restrict__ stage, int32_t* err) {
+            T* __restrict__ piv, T* __restrict__ stage, const T* __restrict__ work, int32_t* err) {
     const int ti = task_index();
     if (ti >= ntask) return;
     __shared__ int s_err;
@@ -399,7 +428,16 @@ void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T*
     const int ra = tk.c0;
     Acc<T> acc[4][2];
     zero_acc<T, 4, 2>(acc);
-    gemm_tn<T>(P + tk.r0, ld, ra, P + k0, ld, kb, 0, k0, sm, acc);
+    gemm_tn<T>(P + tk.r0, ld, ra, P + k0, ld, kb, tk.kbeg, tk.kend, sm, acc);
+    for (int s = 0; s < tk.np; ++s) {                 // K ranges computed by the PART launch
+        const T* in = work + tk.px + (int64_t)s * LKD_PART + t;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[p][b][e] += in[((p * 2 + b) * 4 + e) * 256];
+    }
     T* S = reinterpret_cast<T*>(&sm);
     {
         const T* Sg = stage + tk.ws;
@@ -584,6 +622,7 @@ struct CholArgs {
     void *panel, *diag, *piv, *stage;
     const lkd_snplan* plan;
     int32_t* err;
+    void* work;
 };
 
 template <typename T>
@@ -593,7 +632,8 @@ int launch_chol(int kind, int32_t ntask, const lkd_task* tasks, const CholArgs& 
         case 1: LK_LAUNCH(k_diag<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (T*)a.stage, a.err); break;
         case 2: LK_LAUNCH(k_update<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, *a.plan, a.err); break;
         case 3: LK_LAUNCH(k_diagwb<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (const T*)a.diag, (const T*)a.piv); break;
-        default: LK_LAUNCH(k_step<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (T*)a.stage, a.err); break;
+        case 6: LK_LAUNCH(k_part<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.work, a.err); break;
+        default: LK_LAUNCH(k_step<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (T*)a.stage, (const T*)a.work, a.err); break;
     }
     return lk_last_error();
 }
@@ -671,20 +711,25 @@ int lkd_probe_potrf(int prec, void* a, int32_t* ok, void* stream) {
 
 int lkd_chol_diag(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* diag,
                   void* piv, void* stage, int32_t* err, void* stream) {
-    return dispatch(prec, 1, ntask, tasks, {const_cast<void*>(panel), diag, piv, stage, nullptr, err}, stream);
+    return dispatch(prec, 1, ntask, tasks, {const_cast<void*>(panel), diag, piv, stage, nullptr, err, nullptr}, stream);
+}
+int lkd_chol_part(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* work,
+                  int32_t* err, void* stream) {
+    return dispatch(prec, 6, ntask, tasks,
+                    {const_cast<void*>(panel), nullptr, nullptr, nullptr, nullptr, err, work}, stream);
 }
 int lkd_chol_step(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
-                  void* piv, void* stage, int32_t* err, void* stream) {
-    return dispatch(prec, 5, ntask, tasks, {panel, diag, piv, stage, nullptr, err}, stream);
+                  void* piv, void* stage, const void* work, int32_t* err, void* stream) {
+    return dispatch(prec, 5, ntask, tasks, {panel, diag, piv, stage, nullptr, err, const_cast<void*>(work)}, stream);
 }
 int lkd_chol_update(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
                     lkd_snplan plan, int32_t* err, void* stream) {
-    return dispatch(prec, 2, ntask, tasks, {panel, nullptr, nullptr, nullptr, &plan, err}, stream);
+    return dispatch(prec, 2, ntask, tasks, {panel, nullptr, nullptr, nullptr, &plan, err, nullptr}, stream);
 }
 int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* diag,
                     const void* piv, void* stream) {
     return dispatch(prec, 3, ntask, tasks,
-                    {panel, const_cast<void*>(diag), const_cast<void*>(piv), nullptr, nullptr, nullptr}, stream);
+                    {panel, const_cast<void*>(diag), const_cast<void*>(piv), nullptr, nullptr, nullptr, nullptr}, stream);
 }
 
 }  // extern "C"

@@ -37,11 +37,30 @@ constexpr int GEMM_MT = 128;        // COLUPD / UPDATE tile rows
 constexpr int GEMM_NC = 64;         // COLUPD / UPDATE tile columns
 constexpr int GEMM_TARGET_WG = 1024;   // UPDATE K splitting aims at this many workgroups
 constexpr int KSPLIT_MIN = 128;     // minimum K columns per split part
+constexpr int STEP_TARGET_WG = 512; // a STEP launch with fewer tiles splits K (PART launch) ...
+constexpr int STEP_KMIN = 64;       // ... into parts of at least this many columns
 constexpr int SOLVE_WIDE_TILES = 8;   // level solves: 8 wavefronts per supernode above this
 constexpr int FWD_ROWS = 256;
 
+// Order a launch's tasks for dispatch.  Workgroup b runs task (b % 8) * per + b / 8 with
+// per = ceil(n / 8): segment x of `per` consecutive tasks runs on XCD x, in list order.  Inside
+// every segment the critical tasks move to the front, so they start with the first workgroups.
+void order_for_dispatch(std::vector<lkd_task>& v, const std::vector<char>& crit) {
+    const size_t n = v.size(), per = (n + 7) / 8;
+    std::vector<lkd_task> out;
+    out.reserve(n);
+    for (size_t a = 0; a < n; a += per) {
+        const size_t b = std::min(n, a + per);
+        for (size_t i = a; i < b; ++i)
+            if (crit[i]) out.push_back(v[i]);
+        for (size_t i = a; i < b; ++i)
+            if (!crit[i]) out.push_back(v[i]);
+    }
+    v.swap(out);
+}
+
 struct Launch {
-    int kind;        // 1 diag 2 update 3 diagwb 5 step | 20 forward level 21 backward level
+    int kind;        // 1 diag 2 update 3 diagwb 5 step 6 part | 20 forward level 21 backward level
     int64_t begin;
     int32_t count;
     int32_t nw = 0;  // level solves: wavefronts per workgroup
@@ -104,6 +123,8 @@ struct GPUKKTCholeskySolver::Impl {
     void* d_piv = nullptr;         // pivots L_jj per permuted column (sized for FP64)
     void* d_stage = nullptr;       // staged diagonal factors, LKD_STAGE per supernode of a level
     int64_t stage_elems = 0;
+    void* d_work = nullptr;        // PART products (sized for FP64)
+    int64_t work_elems = 0;
     double* d_below = nullptr;     // level solves: rows below each supernode
     double* d_pbuf = nullptr;      // level solves: backward partial sums
     int32_t* d_err = nullptr;
@@ -407,7 +428,7 @@ struct GPUKKTCholeskySolver::Impl {
             emit(1, b0);
             // STEP k: 128-row tiles below the diagonal block of block column k
             for (int k = 0; k < maxblk; ++k) {
-                b0 = static_cast<int64_t>(tasks.size());
+                std::vector<lkd_task> tl;
                 for (size_t si = 0; si < lv.size(); ++si) {
                     lkd_task t = base(lv[si]);
                     if (k * TILE >= t.ncol) continue;
@@ -423,16 +444,62 @@ struct GPUKKTCholeskySolver::Impl {
                         u.nsplit = 1;
                         u.aux = diag_off(lv[si], k + 1);
                         u.aux2 = slot(si, k + 1);
-                        tasks.push_back(u);
+                        tl.push_back(u);
                         r0 += u.c0;
                     }
                     for (; r0 < t.ld; r0 += GEMM_MT) {
                         lkd_task u = t;
                         u.r0 = r0;
                         u.c0 = std::min(GEMM_MT, t.ld - r0);
-                        tasks.push_back(u);
+                        tl.push_back(u);
                     }
                 }
+                if (tl.empty()) continue;
+                // few tiles: split every tile's K = k0 columns into S parts computed by a PART
+                // launch (the STEP then only adds them); the launch count per step is one more
+                int kmax = 0;
+                for (const auto& u : tl) kmax = std::max(kmax, u.k0);
+                const int64_t nt = static_cast<int64_t>(tl.size());
+                int S = 1;
+                if (nt < STEP_TARGET_WG && kmax >= 2 * STEP_KMIN)
+                    S = static_cast<int>(std::min<int64_t>(STEP_TARGET_WG / nt, kmax / STEP_KMIN));
+                std::vector<lkd_task> parts;
+                std::vector<char> pcrit, scrit;
+                int64_t px = 0;
+                for (auto& u : tl) {
+                    const int Su = std::min(S, std::max(1, u.k0 / STEP_KMIN));
+                    if (Su > 1) {
+                        u.kbeg = u.kend = 0;
+                        u.np = Su;
+                        u.px = px;
+                        for (int j = 0; j < Su; ++j) {
+                            lkd_task q = u;
+                            q.kbeg = kpart(u.k0, Su, j);
+                            q.kend = j + 1 == Su ? u.k0 : kpart(u.k0, Su, j + 1);
+                            q.np = 0;
+                            q.px = px + static_cast<int64_t>(j) * LKD_PART;
+                            parts.push_back(q);
+                            pcrit.push_back(u.nsplit ? 1 : 0);
+                        }
+                        px += static_cast<int64_t>(Su) * LKD_PART;
+                    } else {
+                        u.kbeg = 0;
+                        u.kend = u.k0;
+                        u.np = 0;
+                        u.px = 0;
+                    }
+                    scrit.push_back(u.nsplit ? 1 : 0);
+                }
+                work_elems = std::max(work_elems, px);
+                if (!parts.empty()) {
+                    order_for_dispatch(parts, pcrit);
+                    b0 = static_cast<int64_t>(tasks.size());
+                    tasks.insert(tasks.end(), parts.begin(), parts.end());
+                    emit(6, b0);
+                }
+                order_for_dispatch(tl, scrit);
+                b0 = static_cast<int64_t>(tasks.size());
+                tasks.insert(tasks.end(), tl.begin(), tl.end());
                 emit(5, b0);
             }
             // UPDATE: L21 L21^T of every supernode into its ancestors, split K
@@ -488,7 +555,7 @@ struct GPUKKTCholeskySolver::Impl {
         const int64_t ld = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
         const Index nc = S.sn_first[s + 1] - S.sn_first[s];
         if (t.ld != ld || t.ncol != nc || t.off != S.sn_panel[s] || t.first != S.sn_first[s] ||
-            (kind != 5 && t.aux2 != S.sn_rowptr[s]))   // STEP: aux2 = next staging slot
+            (kind != 5 && kind != 6 && t.aux2 != S.sn_rowptr[s]))   // STEP: aux2 = next staging slot
             bad("geometry");
         if (S.sn_panel[s] + ld * nc > S.sn_panel[S.nsuper]) bad("panel extent");
         if (t.first + nc > m) bad("columns");
@@ -502,8 +569,16 @@ struct GPUKKTCholeskySolver::Impl {
             case 3:   // DIAGWB: a diagonal block and its copy
                 if (t.kb < 1 || t.r0 != t.k0 || !diag_ok(t.aux, t.kb)) bad("diagwb");
                 break;
+            case 6:   // PART: one K range of a STEP tile's GEMM into the work buffer
+                if (t.kb < 1 || t.r0 < t.k0 + t.kb || t.c0 < 1 || t.c0 > GEMM_MT || t.r0 + t.c0 > ld) bad("part rows");
+                if (t.kbeg < 0 || t.kbeg >= t.kend || t.kend > t.k0) bad("part K");
+                if (t.px < 0 || t.px + LKD_PART > work_elems) bad("part work");
+                break;
             case 5: { // STEP: a tile of c0 <= 128 rows below the diagonal block of block column k
                 if (t.kb < 1 || t.r0 < t.k0 + t.kb || t.c0 < 1 || t.c0 > GEMM_MT || t.r0 + t.c0 > ld) bad("step rows");
+                if (t.kbeg < 0 || t.kbeg > t.kend || t.kend > t.k0) bad("step K");
+                if (t.np < 0 || (t.np > 0 && (t.px < 0 || t.px + int64_t(t.np) * LKD_PART > work_elems))) bad("step work");
+                if (t.np == 0 && (t.kbeg != 0 || t.kend != t.k0)) bad("step K cover");
                 if (t.r0 < nc && (t.r0 % TILE)) bad("step tile alignment");   // diagonal blocks whole
                 if (!slot_ok(t.ws)) bad("step slot");
                 if (t.nsplit) {
@@ -616,6 +691,8 @@ struct GPUKKTCholeskySolver::Impl {
         owned.push_back(d_piv);
         d_stage = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(stage_elems, 1)) * sizeof(double));
         owned.push_back(d_stage);
+        d_work = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(work_elems, 1)) * sizeof(double));
+        owned.push_back(d_work);
         d_below = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(solve_below, 1)) * sizeof(double)));
         owned.push_back(d_below);
         d_pbuf = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(solve_pbuf, 1)) * sizeof(double)));
@@ -698,11 +775,14 @@ struct GPUKKTCholeskySolver::Impl {
         timer.pending.push_back({4, first, timer.used - 1, 1, S.flops_exact});
     }
 
-    static int slot_of(const Launch& L) { return L.kind == 5 ? 0 : 1 + (L.kind == 2); }
+    // 0 STEP, 1 DIAG + DIAGWB, 2 UPDATE, 5 PART (3 assembly and 4 totals are not launch kinds)
+    static int slot_of(const Launch& L) {
+        return L.kind == 5 ? 0 : L.kind == 6 ? 5 : L.kind == 2 ? 2 : 1;
+    }
 
     // Live launch timing: HIP events on the solver's stream at every change of launch kind in
     // the factorization schedule, resolved after the stream has drained.  Slots as
-    // profile_factor (0 STEP, 1 DIAG + DIAGWB, 2 UPDATE; 4 = whole factorizations).
+    // profile_factor (0 STEP, 1 DIAG + DIAGWB, 2 UPDATE, 5 PART; 4 = whole factorizations).
     struct Segment {
         int slot;
         size_t begin, end;
@@ -714,7 +794,7 @@ struct GPUKKTCholeskySolver::Impl {
         std::vector<void*> events;
         size_t used = 0;
         std::vector<Segment> pending;
-        double ms[5] = {}, flops[5] = {}, launches[5] = {};
+        double ms[6] = {}, flops[6] = {}, launches[6] = {};
     } timer;
     std::vector<double> launch_flops;   // per factorization launch, stored (panel) flops
 
@@ -740,13 +820,13 @@ struct GPUKKTCholeskySolver::Impl {
         check(lkd_stream_sync(stream), "timing sync");
         timer_collect();
         timer.on = on;
-        for (int k = 0; k < 5; ++k) timer.ms[k] = timer.flops[k] = timer.launches[k] = 0;
+        for (int k = 0; k < 6; ++k) timer.ms[k] = timer.flops[k] = timer.launches[k] = 0;
     }
 
-    void read_timing(double ms[5], double flops[5], double launches[5]) {
+    void read_timing(double ms[6], double flops[6], double launches[6]) {
         check(lkd_stream_sync(stream), "timing sync");
         timer_collect();
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < 6; ++k) {
             ms[k] = timer.ms[k];
             flops[k] = timer.flops[k];
             launches[k] = timer.launches[k];
@@ -759,7 +839,8 @@ struct GPUKKTCholeskySolver::Impl {
             case 1: return lkd_chol_diag(prec, L.count, t, d_panel, d_diag, d_piv, d_stage, d_err, stream);
             case 2: return lkd_chol_update(prec, L.count, t, d_panel, pl, d_err, stream);
             case 3: return lkd_chol_diagwb(prec, L.count, t, d_panel, d_diag, d_piv, stream);
-            default: return lkd_chol_step(prec, L.count, t, d_panel, d_diag, d_piv, d_stage, d_err, stream);
+            case 6: return lkd_chol_part(prec, L.count, t, d_panel, d_work, d_err, stream);
+            default: return lkd_chol_step(prec, L.count, t, d_panel, d_diag, d_piv, d_stage, d_work, d_err, stream);
         }
     }
 
@@ -767,8 +848,8 @@ struct GPUKKTCholeskySolver::Impl {
     // (0 STEP, 1 DIAG + DIAGWB, 2 UPDATE) the summed device time, the launch count and the
     // algorithmic flops of the tasks; slot 3 = assembly (prep, diagonal, zero fill, pair sums);
     // slot 4 = whole factorization incl. assembly (flops: sum over columns of L of c_j^2).
-    void profile_factor(int prec, double ms[5], double flops[5], double launches[5]) {
-        for (int k = 0; k < 5; ++k) ms[k] = flops[k] = launches[k] = 0;
+    void profile_factor(int prec, double ms[6], double flops[6], double launches[6]) {
+        for (int k = 0; k < 6; ++k) ms[k] = flops[k] = launches[k] = 0;
         if (!have_theta) throw std::logic_error("profile_factor: no Theta");
         const lkd_snplan pl = plan();
         void *e0, *e1, *a0, *a1;
@@ -799,7 +880,7 @@ struct GPUKKTCholeskySolver::Impl {
             check(lkd_event_record(e1, stream), "event");
             float t_ms = 0;
             check(lkd_event_elapsed_ms(&t_ms, e0, e1), "elapsed");
-            const int slot = L.kind == 5 ? 0 : 1 + (L.kind == 2);   // 0 STEP, 1 DIAG + DIAGWB, 2 UPDATE
+            const int slot = slot_of(L);
             ms[slot] += t_ms;
             launches[slot] += 1;
             for (int64_t q = L.begin; q < L.begin + L.count; ++q) flops[slot] += task_flops(h_ftasks[q], L.kind);
@@ -818,9 +899,10 @@ struct GPUKKTCholeskySolver::Impl {
     static double task_flops(const lkd_task& t, int kind) {
         const double kb = t.kb;
         switch (kind) {
+            case 6: return 2.0 * t.c0 * kb * (t.kend - t.kbeg);
             case 5: {   // GEMM + solve of the tile; diagonal-block updates; next block's factor
                 const double ra = t.c0;
-                double f = 2.0 * ra * kb * t.k0 + ra * kb * kb;
+                double f = 2.0 * ra * kb * (t.kend - t.kbeg) + ra * kb * kb;
                 for (int h = 0; h < 2; ++h) {
                     const int d0 = t.r0 + 64 * h;
                     if (64 * h >= t.c0 || d0 >= t.ncol) break;
@@ -1207,11 +1289,11 @@ void GPUKKTCholeskySolver::triangular_solve_permuted(Float* d_x) {
 
 const KKTStats& GPUKKTCholeskySolver::stats() const { return impl_->st; }
 void GPUKKTCholeskySolver::set_launch_timing(bool on) { impl_->set_timing(on); }
-void GPUKKTCholeskySolver::launch_timing(double ms[5], double flops[5], double launches[5]) {
+void GPUKKTCholeskySolver::launch_timing(double ms[6], double flops[6], double launches[6]) {
     impl_->read_timing(ms, flops, launches);
 }
-void GPUKKTCholeskySolver::profile_factor(FactorPrecision prec, double ms[5], double flops[5],
-                                          double launches[5]) {
+void GPUKKTCholeskySolver::profile_factor(FactorPrecision prec, double ms[6], double flops[6],
+                                          double launches[6]) {
     impl_->profile_factor(prec == FactorPrecision::FP32 ? LKD_F32 : LKD_F64, ms, flops, launches);
 }
 const std::vector<Index>& GPUKKTCholeskySolver::permutation() const { return impl_->perm; }

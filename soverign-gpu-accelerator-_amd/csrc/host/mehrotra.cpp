@@ -538,13 +538,13 @@ void MehrotraSolver::set_warm_start(const std::vector<Float>& x, const std::vect
     impl_->set_warm(x, y, s);
 }
 void MehrotraSolver::clear_warm_start() { impl_->clear_warm(); }
-void MehrotraSolver::profile_factor(bool fp32, double ms[5], double flops[5], double launches[5]) {
+void MehrotraSolver::profile_factor(bool fp32, double ms[6], double flops[6], double launches[6]) {
     impl_->kkt_->profile_factor(fp32 ? gpu::FactorPrecision::FP32 : gpu::FactorPrecision::FP64,
                                 ms, flops, launches);
     check(lkd_stream_sync(impl_->stream_), "sync");
 }
 void MehrotraSolver::set_launch_timing(bool on) { impl_->kkt_->set_launch_timing(on); }
-void MehrotraSolver::launch_timing(double ms[5], double flops[5], double launches[5]) {
+void MehrotraSolver::launch_timing(double ms[6], double flops[6], double launches[6]) {
     impl_->kkt_->launch_timing(ms, flops, launches);
 }
 void MehrotraSolver::kkt_statistics(double out[12]) {

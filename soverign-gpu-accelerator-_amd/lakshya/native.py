@@ -334,9 +334,9 @@ class IPM:
     def profile_factor(self, fp32=False):
         L = lib()
         L.lk_ipm_profile_factor.argtypes = [_vp, C.c_int32, _f64p, _f64p, _f64p]
-        ms, fl, la = np.zeros(5), np.zeros(5), np.zeros(5)
+        ms, fl, la = np.zeros(6), np.zeros(6), np.zeros(6)
         _ck(L.lk_ipm_profile_factor(self.h, int(fp32), ms, fl, la))
-        names = ["step", "diag", "update", "assembly", "factorization"]
+        names = ["step", "diag", "update", "assembly", "factorization", "part"]
         return {k: {"ms": float(ms[i]), "flops": float(fl[i]), "launches": int(la[i])}
                 for i, k in enumerate(names)}
 
@@ -349,9 +349,9 @@ class IPM:
     def launch_timing(self):
         L = lib()
         L.lk_ipm_read_launch_timing.argtypes = [_vp, _f64p, _f64p, _f64p]
-        ms, fl, la = np.zeros(5), np.zeros(5), np.zeros(5)
+        ms, fl, la = np.zeros(6), np.zeros(6), np.zeros(6)
         _ck(L.lk_ipm_read_launch_timing(self.h, ms, fl, la))
-        names = ["step", "diag", "update", "assembly", "factorization"]
+        names = ["step", "diag", "update", "assembly", "factorization", "part"]
         return {k: {"ms": float(ms[i]), "flops": float(fl[i]), "launches": int(la[i])}
                 for i, k in enumerate(names)}
 
@@ -391,7 +391,8 @@ def verify_optimal(lp, x, pi, tol=1e-6, lb=None, ub=None, maximize=False):
 TASK_DTYPE = np.dtype([("off", "<i8"), ("aux", "<i8"), ("aux2", "<i8"), ("ws", "<i8"),
                        ("ld", "<i4"), ("ncol", "<i4"), ("k0", "<i4"), ("kb", "<i4"),
                        ("r0", "<i4"), ("c0", "<i4"), ("kbeg", "<i4"), ("kend", "<i4"),
-                       ("nsplit", "<i4"), ("sn", "<i4"), ("first", "<i4"), ("pad", "<i4")])
+                       ("nsplit", "<i4"), ("sn", "<i4"), ("first", "<i4"), ("np", "<i4"),
+                       ("px", "<i8")])
 PLAN_DTYPES = {"perm": np.int32, "sn_rowptr": np.int64, "sn_rows": np.int32,
                "sn_panel": np.int64, "col_to_sn": np.int32, "sn_first": np.int32,
                "flaunch": np.int64, "slaunch": np.int64, "diag_elems": np.int64,

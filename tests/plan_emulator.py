@@ -58,15 +58,17 @@ class Emulator:
         self.diag[off:off + kb * kb] = Lr.reshape(-1)
 
     def factor(self):
-        """Left-looking schedule: DIAG (kind 1, block column 0), STEP (5), UPDATE (2), DIAGWB (3).
+        """Left-looking schedule: DIAG (kind 1, block column 0), PART (6), STEP (5), UPDATE (2),
+        DIAGWB (3).
         Diagonal factors live in staging slots between launches; the panel's diagonal blocks
         are only written by DIAGWB."""
         p = self.p
         T = p["ftasks"]
         MT = 128
         self.stage = {}
+        self.work = {}
         for kind, b, c in self.fl:
-            assert kind in (1, 2, 3, 5)
+            assert kind in (1, 2, 3, 5, 6)
             for t in T[b:b + c]:
                 if self.err:
                     return False
@@ -78,10 +80,28 @@ class Emulator:
                     if L is None:
                         return False
                     self._publish(L, int(t["ws"]), first + k0, int(t["aux"]))
+                elif kind == 6:                   # PART: one K range of a STEP tile's GEMM
+                    ra, kbeg, kend = int(t["c0"]), int(t["kbeg"]), int(t["kend"])
+                    assert 0 <= kbeg < kend <= k0
+                    px = int(t["px"])
+                    assert px not in self.work
+                    self.work[px] = (kbeg, kend, self.blk(t, r0, r0 + ra, kbeg, kend)
+                                     @ self.blk(t, k0, k0 + kb, kbeg, kend).T)
                 elif kind == 5:
                     ra = int(t["c0"])
                     X = self.blk(t, r0, r0 + ra, k0, k0 + kb)
-                    X -= self.blk(t, r0, r0 + ra, 0, k0) @ self.blk(t, k0, k0 + kb, 0, k0).T
+                    kbeg, kend = int(t["kbeg"]), int(t["kend"])
+                    cover = [(kbeg, kend)] if kend > kbeg else []
+                    G = self.blk(t, r0, r0 + ra, kbeg, kend) @ self.blk(t, k0, k0 + kb, kbeg, kend).T
+                    for s in range(int(t["np"])):  # partial products of the PART launch before
+                        pb, pe, Ps = self.work.pop(int(t["px"]) + s * 128 * 64)
+                        cover.append((pb, pe))
+                        G = G + Ps
+                    cover.sort()                  # the K ranges tile [0, k0) exactly
+                    assert [x for rg in cover for x in rg][:1] in ([], [0]) and \
+                        all(cover[i][1] == cover[i + 1][0] for i in range(len(cover) - 1)) and \
+                        (cover[-1][1] if cover else 0) == k0, cover
+                    X -= G
                     L = self.stage[int(t["ws"])]
                     X[:] = np.linalg.solve(L, X.T).T
                     for h in range(2):            # diagonal blocks inside the tile
@@ -122,6 +142,7 @@ class Emulator:
                                 self.err |= 2
                                 return False
                             self.panel[p["sn_panel"][s] + (gj - f) * (rp1 - rp0) + pos] -= U[i, j]
+        assert not self.work, "partial products never added"
         return True
 
     def _publish(self, L, slot, col, dcopy):
