@@ -238,16 +238,17 @@ __device__ __forceinline__ bool potrf16_wave(T* Ds, int o, T* Lv, T* cb) {
         __builtin_amdgcn_sched_barrier(0);   // keep each column's loads next to their use
     }
     // column c = r of L^{-1}: x[i] = (delta_ic - sum_{k<i} L[i][k] x[k]) / L[i][i], with
-    // L[i][k] = lane i's a[k] broadcast by readlane (scalar registers: no load hoisting, few VGPRs)
+    // L[i][k] = cb[16 k + i] (column k as published above) and 1 / L[i][i] = cb[256 + i]: LDS
+    // broadcast reads that do not depend on x, so only one fma and one multiply per i are on
+    // the dependency chain (x[i] = 0 for i < c follows from the zero start)
+    __builtin_amdgcn_wave_barrier();
     T x[16];
-    const T invd = cb[256 + r];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         T acc = i == r ? T(1) : T(0);
 #pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (k < i) acc = fma(-bcast_lane(a[k], i), x[k], acc);
-        x[i] = acc * bcast_lane(invd, i);
+        for (int k = 0; k < i; ++k) acc = fma(-cb[16 * k + i], x[k], acc);
+        x[i] = acc * cb[256 + i];
     }
     if (l < 16) {
 #pragma unroll

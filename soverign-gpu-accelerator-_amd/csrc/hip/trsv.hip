@@ -15,36 +15,64 @@ namespace {
 constexpr int TS = 64;
 constexpr int DP = 65;
 
-// Forward substitution on a diagonal block by wave 0.  b: this lane's right-hand side entry
-// (lane < kb), Lr: row-major copy (stride kb, reciprocal diagonal).  Returns z for this lane.
+// Wave-uniform broadcast of lane `src`'s value (src is a compile-time constant at every use):
+// two readlanes into scalar registers, no LDS round trip.
+__device__ __forceinline__ double bcast_lane(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+
+// Stage the row-major copy Lr (kb x kb, reciprocal diagonal) of a diagonal block into Ds (64
+// rows of pitch 65 doubles; zero past kb) in four batches of 16 rows: coalesced loads, issued
+// ahead of the solve that reads them.
 template <typename T>
-__device__ __forceinline__ double wave_fsub(double b, const T* __restrict__ Lr, int kb) {
+__device__ __forceinline__ void stage_diag(const T* __restrict__ Lr, int kb, double* Ds) {
     const int lane = threadIdx.x & 63;
-    T row[TS];
 #pragma unroll
-    for (int k = 0; k < TS; ++k) row[k] = (k < lane && lane < kb) ? Lr[lane * kb + k] : T(0);
+    for (int q = 0; q < 4; ++q) {
+        T v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = 16 * q + i;
+            v[i] = (r < kb && lane < kb) ? Lr[r * kb + lane] : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) Ds[(16 * q + i) * 65 + lane] = (double)v[i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// L z = b on a staged diagonal block, one wavefront: lane i holds b_i and reads row i of L
+// from LDS; z_j is formed in lane j (b_j times the reciprocal diagonal) and broadcast by
+// readlane, so each step's dependency chain is one multiply, two readlanes and one fma.
+__device__ __forceinline__ double fsub_staged(double b, const double* Ds, int kb) {
+    const int lane = threadIdx.x & 63;
+    const double* row = Ds + lane * 65;
+    const double rd = lane < kb ? row[lane] : 0.0;
 #pragma unroll
     for (int j = 0; j < TS; ++j) {
         if (j < kb) {
-            const double zj = lk_shfl(b, j) * (double)Lr[j * kb + j];
+            const double zj = bcast_lane(b * rd, j);
             if (lane == j) b = zj;
-            else if (lane > j) b -= (double)row[j] * zj;
+            else if (lane > j) b -= row[j] * zj;
         }
     }
     return b;
 }
 
-// Backward substitution L^T x = r on a diagonal block by wave 0 (lane j keeps column j).
+// L^T x = r on a diagonal block held in registers: lane j holds r_j and col[i] = L[i][j] (row i
+// of the row-major copy, element j: coalesced loads), rd = 1 / L[j][j]; x_i is formed in lane i
+// and broadcast by readlane.
 template <typename T>
-__device__ __forceinline__ double wave_bsub(double r, const T* __restrict__ Lr, int kb) {
+__device__ __forceinline__ double bsub_regs(double r, const T (&col)[TS], double rd, int kb) {
     const int lane = threadIdx.x & 63;
-    T col[TS];
-#pragma unroll
-    for (int i = 0; i < TS; ++i) col[i] = (i > lane && i < kb) ? Lr[i * kb + lane] : T(0);
 #pragma unroll
     for (int i = TS - 1; i >= 0; --i) {
         if (i < kb) {
-            const double xi = lk_shfl(r, i) * (double)Lr[i * kb + i];
+            const double xi = bcast_lane(r * rd, i);
             if (lane == i) r = xi;
             else if (lane < i) r -= (double)col[i] * xi;
         }
@@ -122,33 +150,6 @@ __device__ __forceinline__ void load_tile(T (&lv)[TS], const T* __restrict__ col
     for (int c = 0; c < TS; ++c) lv[c] = (row_ok && c < kb) ? col[(int64_t)c * ld] : T(0);
 }
 
-// Forward substitution on a diagonal block, staged through LDS: Ds (64 x 65 doubles) receives
-// the row-major copy Lr (stride kb, reciprocal diagonal) with coalesced loads, then lane i
-// reads its row from LDS while the solve broadcasts z_j by lane shuffles.
-template <typename T>
-__device__ __forceinline__ double wave_fsub_lds(double b, const T* __restrict__ Lr, int kb, double* Ds) {
-    const int lane = threadIdx.x & 63;
-    T v[TS];
-#pragma unroll
-    for (int i = 0; i < TS; ++i) v[i] = (i < kb && lane < kb) ? Lr[i * kb + lane] : T(0);
-#pragma unroll
-    for (int i = 0; i < TS; ++i) Ds[i * 65 + lane] = (double)v[i];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const double* row = Ds + lane * 65;
-#pragma unroll
-    for (int j = 0; j < TS; ++j) {
-        if (j < kb) {
-            const double zj = lk_shfl(b, j) * (double)Lr[j * kb + j];
-            if (lane == j) b = zj;
-            else if (lane > j) b -= row[j] * zj;
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    return b;
-}
-
 // x_t -= L(t, j) z_j for the 64 columns of block j held in lv (tile t: diagonal tiles update
 // the supernode's own entries xs, tiles below update the scratch vector bl).
 template <typename T>
@@ -169,13 +170,13 @@ __device__ __forceinline__ void fwd_apply(const Tiles tl, int t, const T (&lv)[T
     else bl[row - tl.nc] -= a;
 }
 
-// Solve diagonal block jb of the supernode (tile jb fully updated by the calling wavefront),
-// put z in x and in the LDS ring, publish `done`.  Ring slot reuse waits until every wavefront
-// finished iteration jb - ZR.
-template <typename T, int NW>
-__device__ __forceinline__ bool fwd_solve_block(const lkd_task& tk, int jb, const T* __restrict__ diag,
-                                                double* x, double* zs, double* Ds, int* done,
-                                                const int* prog, int32_t* err) {
+// Solve diagonal block jb of the supernode (tile jb fully updated by the calling wavefront,
+// its factor staged in Ds by the same wavefront), put z in x and in the LDS ring, publish
+// `done`.  Ring slot reuse waits until every wavefront finished iteration jb - ZR.
+template <int NW>
+__device__ __forceinline__ bool fwd_solve_block(const lkd_task& tk, int jb, double* x, double* zs,
+                                                const double* Ds, int* done, const int* prog,
+                                                int32_t* err) {
     const int v = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (jb >= ZR)
         for (int u = 0; u < NW; ++u)
@@ -183,7 +184,7 @@ __device__ __forceinline__ bool fwd_solve_block(const lkd_task& tk, int jb, cons
     const int kb = min(TS, tk.ncol - jb * TS);
     double* xj = x + tk.first + jb * TS;
     const double b = lane < kb ? xj[lane] : 0.0;
-    const double zj = wave_fsub_lds<T>(b, diag + tk.aux + (int64_t)jb * TS * TS, kb, Ds);
+    const double zj = fsub_staged(b, Ds, kb);
     if (lane < kb) xj[lane] = zj;
     zs[(jb % ZR) * TS + lane] = lane < kb ? zj : 0.0;
     lds_publish(done, jb + 1);
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(64 * NW) void k_fwd_sn(const lkd_task* __restrict__
                                                     double* __restrict__ below, int32_t* err) {
     const lkd_task tk = tasks[blockIdx.x];
     __shared__ double zs[ZR * TS];                    // ring of solved blocks (zero past kb)
-    __shared__ double Ds[TS * 65];                    // diagonal block being solved
+    __shared__ double Ds[2][TS * 65];                 // staged diagonal blocks (block jb: jb & 1)
     __shared__ int done;                              // blocks solved
     __shared__ int prog[NW];                          // iterations finished per wavefront
     const T* P = L + tk.off;
@@ -213,11 +214,20 @@ __global__ __launch_bounds__(64 * NW) void k_fwd_sn(const lkd_task* __restrict__
     for (int t = first_tile<NW>(nb, v); t < nt; t += NW)
         if (tl.beg(t) + lane < tl.end(t)) bl[tl.beg(t) + lane - nc] = 0.0;
     __syncthreads();
-    if (v == 0 && !fwd_solve_block<T, NW>(tk, 0, diag, x, zs, Ds, &done, prog, err)) return;
+    auto stage = [&](int jb) {
+        stage_diag<T>(diag + tk.aux + (int64_t)jb * TS * TS, min(TS, nc - jb * TS), Ds[jb & 1]);
+    };
+    if (v == 0) {
+        stage(0);
+        if (!fwd_solve_block<NW>(tk, 0, x, zs, Ds[0], &done, prog, err)) return;
+    }
     for (int j = 0; j < nb; ++j) {
         const int kbj = min(TS, nc - j * TS);
         const T* Pj = P + (int64_t)j * TS * ld;
         const int t1 = first_tile<NW>(j + 1, v);
+        // the owner of block j + 1 stages its factor now: slot (j + 1) & 1 was last read by the
+        // solve of block j - 1, which finished before this wavefront passed iteration j - 1
+        if (t1 == j + 1 && t1 < nb) stage(t1);
         T lv[TS];
         // the first tile's panel data does not depend on z_j: load it before waiting
         load_tile<T>(lv, Pj + tl.beg(min(t1, nt - 1)) + lane, ld, kbj, t1 < nt && tl.beg(t1) + lane < tl.end(t1));
@@ -225,7 +235,7 @@ __global__ __launch_bounds__(64 * NW) void k_fwd_sn(const lkd_task* __restrict__
         const double* z = zs + (j % ZR) * TS;
         if (t1 < nt) {
             fwd_apply<T>(tl, t1, lv, z, x + tk.first, bl);
-            if (t1 == j + 1 && t1 < nb && !fwd_solve_block<T, NW>(tk, t1, diag, x, zs, Ds, &done, prog, err))
+            if (t1 == j + 1 && t1 < nb && !fwd_solve_block<NW>(tk, t1, x, zs, Ds[t1 & 1], &done, prog, err))
                 return;   // look-ahead: the next block first
         }
         for (int t = t1 + NW; t < nt; t += NW) {
@@ -319,9 +329,15 @@ __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__
             for (int t = first_tile<NW>(j + 1, v); t < nt; t += NW) contrib(pre, t, j - 1);
             pre_blk = j - 1;
         }
+        // the diagonal block's columns load while the other parts arrive
+        const int kbj = min(TS, nc - j * TS);
+        const T* Lr = diag + tk.aux + (int64_t)j * TS * TS;
+        T col[TS];
+#pragma unroll
+        for (int i = 0; i < TS; ++i) col[i] = (i < kbj && lane < kbj) ? Lr[i * kbj + lane] : T(0);
+        const double rd = lane < kbj ? (double)Lr[lane * kbj + lane] : 0.0;
         for (int u = 0; u < NW; ++u)
             if (u != v && !lds_wait(&pub[u], [&](int p) { return p <= j; }, err)) return;
-        const int kbj = min(TS, nc - j * TS);
         double val = 0.0;
         if (lane < kbj) {
             double s = 0.0;
@@ -329,7 +345,7 @@ __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__
             for (int u = 0; u < NW; ++u) s += p0[(int64_t)u * TS];
             val = x[tk.first + j * TS + lane] - s;
         }
-        const double sol = wave_bsub<T>(val, diag + tk.aux + (int64_t)j * TS * TS, kbj);
+        const double sol = bsub_regs<T>(val, col, rd, kbj);
         if (lane < kbj) x[tk.first + j * TS + lane] = sol;
     }
 }
