@@ -28,6 +28,7 @@ extern "C" {
 /* ---------------------------------------------------------------- runtime plumbing */
 const char* lkd_error_string(int code);
 int lkd_device_count(int* count);
+int lkd_device_cus(int* cus);          /* compute units of the current device */
 int lkd_set_device(int dev);
 int lkd_malloc(void** ptr, size_t bytes);
 int lkd_free(void* ptr);
@@ -154,15 +155,21 @@ int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
 /* ---------------------------------------------------------------- triangular solves */
 /* Replaces cusparseSpSV (L and L^T) (src/cuda/kkt.cu:731-734,788-789).  x is the permuted
  * vector, solved in place: L z = x (dir 0, levels leaves first) or L^T x = z (dir 1, root
- * first), one launch per etree level, one workgroup of nw (4 or 8) wavefronts per supernode.
+ * first), one launch per etree level, g = task.c0 workgroups of nw (4 or 8) wavefronts per
+ * supernode (task.r0 = workgroup index; g = 1, or 4 for forward launches with nw = 8, when every
+ * workgroup of the launch must be resident at once).
  * task.aux = offset of the supernode's first diagonal copy, task.aux2 = row-list offset,
  * task.kbeg = offset of its rows below in `below` (scratch, one double per row below a
- * supernode), task.ws = offset of its partial sums in `pbuf` (backward: nb * nw * 64 doubles,
- * nb = 64-column blocks of the supernode).  Waits between wavefronts are bounded: a timeout
- * sets err bit 0x10 and the result is invalid. */
-int lkd_trsv_level(int prec, int dir, int nw, int32_t ntask, const lkd_task* tasks,
+ * supernode), task.ws = offset of its area in `pbuf` (nb * G * nw * 64 doubles, nb = 64-column
+ * blocks of the supernode: forward z exchange, backward partial sums), task.px = offset of its
+ * nb + nb * G flags in `flags`.  *epoch is the solve number, advanced by lkd_trsv_epoch before
+ * every solve (flags hold it, so they are never reset).  Waits are bounded: a timeout sets err
+ * bit 0x10 and the result is invalid. */
+int lkd_trsv_epoch(uint32_t* epoch, void* stream);
+int lkd_trsv_level(int prec, int dir, int nw, int g, int32_t ntask, const lkd_task* tasks,
                    const void* panel, const void* diag, const int32_t* sn_rows, double* x,
-                   double* below, double* pbuf, int32_t* err, void* stream);
+                   double* below, double* pbuf, uint32_t* flags, const uint32_t* epoch,
+                   int32_t* err, void* stream);
 
 /* ---------------------------------------------------------------- KKT solve vector kernels */
 /* r[i] = rhs[P[i]]; req[i] = D[i] * r[i] */

@@ -19,6 +19,12 @@ extern "C" {
 
 const char* lkd_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 int lkd_device_count(int* count) { return (int)hipGetDeviceCount(count); }
+int lkd_device_cus(int* cus) {
+    int dev = 0;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    return (int)hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev);
+}
 int lkd_set_device(int dev) { return (int)hipSetDevice(dev); }
 int lkd_malloc(void** ptr, size_t bytes) { return (int)hipMalloc(ptr, bytes ? bytes : 1); }
 int lkd_free(void* ptr) { return ptr ? (int)hipFree(ptr) : 0; }

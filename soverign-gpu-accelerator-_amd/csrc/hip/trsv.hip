@@ -29,77 +29,82 @@ __device__ __forceinline__ double bcast_lane(double v, int src) {
 template <typename T>
 __device__ __forceinline__ void stage_diag(const T* __restrict__ Lr, int kb, double* Ds) {
     const int lane = threadIdx.x & 63;
+    if (kb == TS) {                               // full block: unconditional loads
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        T v[16];
+        for (int q = 0; q < 4; ++q) {
+            T v[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int r = 16 * q + i;
-            v[i] = (r < kb && lane < kb) ? Lr[r * kb + lane] : T(0);
+            for (int i = 0; i < 16; ++i) v[i] = Lr[(16 * q + i) * TS + lane];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) Ds[(16 * q + i) * 65 + lane] = (double)v[i];
         }
-#pragma unroll
-        for (int i = 0; i < 16; ++i) Ds[(16 * q + i) * 65 + lane] = (double)v[i];
+    } else {                                      // clamped loads times a 0/1 mask: no branches
+        const int lc = min(lane, kb - 1);
+        for (int r = 0; r < TS; ++r)
+            Ds[r * 65 + lane] = (double)Lr[min(r, kb - 1) * kb + lc] * ((r < kb && lane < kb) ? 1.0 : 0.0);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// L z = b on a staged diagonal block, one wavefront: lane i holds b_i and reads row i of L
-// from LDS; z_j is formed in lane j (b_j times the reciprocal diagonal) and broadcast by
-// readlane, so each step's dependency chain is one multiply, two readlanes and one fma.
+// L z = b on a staged diagonal block, one wavefront.  Lane i keeps b_i and its row of L in
+// registers with the diagonal entry zeroed (the row-major copy is zero above the diagonal), so
+// step j is b -= L[i][j] z_j for every lane at once: lanes <= j are left unchanged by the zero
+// entries, lane j keeps its updated right-hand side and z = b / L_ii is formed at the end.
+// z_j itself comes from lane j by readlane: the dependency chain per step is one multiply, two
+// readlanes and one fma, with no branch and no memory access.  Past kb the staged rows and the
+// reciprocal diagonal are zero, so all 64 steps run for any kb.
 __device__ __forceinline__ double fsub_staged(double b, const double* Ds, int kb) {
     const int lane = threadIdx.x & 63;
     const double* row = Ds + lane * 65;
+    double rv[TS];
+#pragma unroll
+    for (int j = 0; j < TS; ++j) rv[j] = row[j];
     const double rd = lane < kb ? row[lane] : 0.0;
 #pragma unroll
-    for (int j = 0; j < TS; ++j) {
-        if (j < kb) {
-            const double zj = bcast_lane(b * rd, j);
-            if (lane == j) b = zj;
-            else if (lane > j) b -= row[j] * zj;
-        }
-    }
-    return b;
+    for (int j = 0; j < TS; ++j) rv[j] = lane == j ? 0.0 : rv[j];
+#pragma unroll
+    for (int j = 0; j < TS; ++j) b = fma(-rv[j], bcast_lane(b * rd, j), b);
+    return b * rd;
 }
 
 // L^T x = r on a diagonal block held in registers: lane j holds r_j and col[i] = L[i][j] (row i
 // of the row-major copy, element j: coalesced loads), rd = 1 / L[j][j]; x_i is formed in lane i
 // and broadcast by readlane.
+// Same scheme as fsub_staged: col[lane] (the diagonal) must be zero and col[i] = 0 for i < lane
+// (upper part of the copy), so step i is r -= L[i][j] x_i for every lane; x = r / L_jj at the end.
 template <typename T>
-__device__ __forceinline__ double bsub_regs(double r, const T (&col)[TS], double rd, int kb) {
-    const int lane = threadIdx.x & 63;
+__device__ __forceinline__ double bsub_regs(double r, const T (&col)[TS], double rd) {
 #pragma unroll
-    for (int i = TS - 1; i >= 0; --i) {
-        if (i < kb) {
-            const double xi = bcast_lane(r * rd, i);
-            if (lane == i) r = xi;
-            else if (lane < i) r -= (double)col[i] * xi;
-        }
-    }
-    return r;
+    for (int i = TS - 1; i >= 0; --i) r = fma(-(double)col[i], bcast_lane(r * rd, i), r);
+    return r * rd;
 }
 
 // ------------------------------------------------------------------------------------------
-// Per-level solves: one workgroup of NW wavefronts per supernode of the level, one launch per
-// level.  The row tiles of the panel (t < nb: the rows of diagonal block t; t >= nb: 64-row
-// groups of the rows below the supernode, owned by ancestors) belong to wavefront t % NW.
-// Wavefronts hand results to each other through workgroup-scope release / acquire on LDS
-// progress words, so nobody waits at a full barrier and no device-scope fence is needed
-// (device-scope acquire invalidates a whole L2 on this chip).
+// Per-level solves: one launch per level, G workgroups of NW wavefronts per supernode (G = 1
+// unless the level has very few supernodes).  The row tiles of the panel (t < nb: the rows of
+// diagonal block t; t >= nb: 64-row groups of the rows below the supernode, owned by ancestors)
+// belong to wavefront gw = t % (G NW) (workgroup gw / NW).  Inside a workgroup, wavefronts hand
+// results to each other through workgroup-scope release / acquire on LDS progress words, so
+// nobody waits at a full barrier; between workgroups a flag in device memory is released and
+// acquired at agent scope, once per hand-off (each such fence writes back or invalidates an
+// L2, which is why G stays 1 when many supernodes run side by side).
 //  forward   for j = 0 .. nb-1: the owner of tile j solves its diagonal block (it applied all
-//            updates of tile j itself), publishes z_j in an LDS ring of ZR blocks and advances
-//            `done`; every wavefront applies L(t, j) z_j to its tiles t > j (diagonal tiles in
-//            x, rows below in a scratch vector that is added to the ancestors' entries with
-//            atomics at the end).  A ring slot is reused only after every wavefront finished
-//            the iteration that read it (`prog`).
+//            updates of tile j itself), publishes z_j in its workgroup's LDS ring of ZR blocks
+//            (and, G > 1, in device memory, where wavefront 0 of every other workgroup picks it
+//            up) and advances `done`; every wavefront applies L(t, j) z_j to its tiles t > j
+//            (diagonal tiles in x, rows below in a scratch vector that is added to the
+//            ancestors' entries with atomics at the end).  A ring slot is reused only after
+//            every wavefront of the workgroup finished the iteration that read it (`prog`).
 //  backward  for j = nb-1 .. 0: every wavefront forms part_v = sum over its tiles t > j of
 //            L(t, j)^T x_t (LDS-transposed CW-column chunks, fixed reduction order), stores it
-//            in `pbuf` and publishes; the owner of j waits for all NW parts, sums them in
+//            in `pbuf` and publishes; the owner of j waits for all parts, sums them in
 //            wavefront order (deterministic), solves its diagonal block and keeps x_j.
 // Task fields: aux = diagonal-copy offset of block 0 (block j at aux + 4096 j), aux2 = row-list
-// offset, kbeg = offset of the rows below in `below`, ws = offset of the supernode's partial
-// sums in `pbuf` (nb * NW * 64 doubles).  Waits are bounded; a timeout sets err bit 0x10.
+// offset, kbeg = offset of the rows below in `below`, r0 = g, c0 = G, ws = offset of the
+// supernode's area in `pbuf`, px = offset of its flags.  Waits are bounded; a timeout sets
+// err bit 0x10.
 constexpr long long kSpinLimit = 1ll << 26;
 constexpr int ZR = 32;
 
@@ -142,12 +147,41 @@ __device__ __forceinline__ bool lds_wait(const int* w, Pred pred, int32_t* err) 
 }
 
 // One 64-row tile x 64 columns of a panel, a column per register (lane = row): all loads are
-// issued together (coalesced 512-byte rows of 64 lanes).  Columns >= kb and rows outside the
-// tile read as zero.
+// issued together (coalesced 512-byte rows of 64 lanes).  `col` must point at a row inside the
+// panel for every lane (callers clamp it); columns >= kb and lanes with !row_ok read as zero.
+// Full blocks (kb = 64, all but a supernode's last) take a path without any column condition.
 template <typename T>
 __device__ __forceinline__ void load_tile(T (&lv)[TS], const T* __restrict__ col, int64_t ld, int kb, bool row_ok) {
+    if (kb == TS) {                               // full block: unconditional loads
 #pragma unroll
-    for (int c = 0; c < TS; ++c) lv[c] = (row_ok && c < kb) ? col[(int64_t)c * ld] : T(0);
+        for (int c = 0; c < TS; ++c) {
+            const T e = col[(int64_t)c * ld];
+            lv[c] = row_ok ? e : T(0);
+        }
+    } else {                                      // clamped loads times a 0/1 mask: no branches
+#pragma unroll
+        for (int c = 0; c < TS; ++c)
+            lv[c] = col[(int64_t)min(c, kb - 1) * ld] * ((row_ok && c < kb) ? T(1) : T(0));
+    }
+}
+
+// Half a tile (32 columns c0 .. c0+31) into registers, same conventions as load_tile.
+template <typename T>
+__device__ __forceinline__ void load_half(T (&h)[TS / 2], const T* __restrict__ col, int64_t ld, int c0,
+                                          int kb, bool row_ok) {
+    if (kb == TS) {
+#pragma unroll
+        for (int c = 0; c < TS / 2; ++c) {
+            const T e = col[(int64_t)(c0 + c) * ld];
+            h[c] = row_ok ? e : T(0);
+        }
+    } else {                                      // clamped loads times a 0/1 mask: no branches
+#pragma unroll
+        for (int c = 0; c < TS / 2; ++c) {
+            const int cc = c0 + c;
+            h[c] = col[(int64_t)min(cc, kb - 1) * ld] * ((row_ok && cc < kb) ? T(1) : T(0));
+        }
+    }
 }
 
 // x_t -= L(t, j) z_j for the 64 columns of block j held in lv (tile t: diagonal tiles update
@@ -170,138 +204,234 @@ __device__ __forceinline__ void fwd_apply(const Tiles tl, int t, const T (&lv)[T
     else bl[row - tl.nc] -= a;
 }
 
-// Solve diagonal block jb of the supernode (tile jb fully updated by the calling wavefront,
-// its factor staged in Ds by the same wavefront), put z in x and in the LDS ring, publish
-// `done`.  Ring slot reuse waits until every wavefront finished iteration jb - ZR.
+// Tile ownership among the G workgroups of a supernode (G * NW wavefronts, global wavefront
+// index gw = g * NW + v): tile t belongs to gw = t % (G * NW), so runs of NW consecutive
+// blocks live in one workgroup and most hand-offs on the forward critical path stay in LDS.
+template <int G, int NW>
+struct Own {
+    static constexpr int nwt = G * NW;
+    int gw, g;
+    __device__ __forceinline__ int first(int t0) const { return t0 + ((gw - t0 % nwt + nwt) % nwt); }
+    __device__ __forceinline__ int wg_of(int t) const { return (t % nwt) / NW; }
+};
+
+// Device-scope hand-off between the workgroups of one supernode: data first, then a release
+// at agent scope, then the flag (relaxed); the reader spins on the flag with relaxed loads and
+// acquires once.  Flags hold the solve's epoch, so they never need resetting.
+__device__ __forceinline__ void dev_publish(uint32_t* flag, uint32_t ep) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(flag, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool dev_wait(const uint32_t* flag, uint32_t ep, int32_t* err) {
+    long long n = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != ep) {
+        if (++n >= kSpinLimit) {
+            if ((threadIdx.x & 63) == 0) atomicOr(err, 0x10);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
+// Ring slot of block jb may be (re)written once every wavefront of the workgroup finished
+// iteration jb - ZR.
 template <int NW>
-__device__ __forceinline__ bool fwd_solve_block(const lkd_task& tk, int jb, double* x, double* zs,
-                                                const double* Ds, int* done, const int* prog,
-                                                int32_t* err) {
-    const int v = threadIdx.x >> 6, lane = threadIdx.x & 63;
+__device__ __forceinline__ bool ring_free(int jb, const int* prog, int32_t* err) {
+    const int v = threadIdx.x >> 6;
     if (jb >= ZR)
         for (int u = 0; u < NW; ++u)
             if (u != v && !lds_wait(&prog[u], [&](int p) { return p > jb - ZR; }, err)) return false;
+    return true;
+}
+
+// Solve diagonal block jb of the supernode (tile jb fully updated by the calling wavefront,
+// its factor staged in Ds by the same wavefront), put z in x and in the LDS ring, publish
+// `done`; with several workgroups also in the supernode's exchange area zg and its flag.
+template <int NW, int G>
+__device__ __forceinline__ bool fwd_solve_block(const lkd_task& tk, int jb, double* x, double* zs,
+                                                const double* Ds, int* done, const int* prog,
+                                                double* zg, uint32_t* fl, uint32_t ep,
+                                                int32_t* err) {
+    const int lane = threadIdx.x & 63;
+    if (!ring_free<NW>(jb, prog, err)) return false;
     const int kb = min(TS, tk.ncol - jb * TS);
     double* xj = x + tk.first + jb * TS;
     const double b = lane < kb ? xj[lane] : 0.0;
     const double zj = fsub_staged(b, Ds, kb);
+    const double zv = lane < kb ? zj : 0.0;
     if (lane < kb) xj[lane] = zj;
-    zs[(jb % ZR) * TS + lane] = lane < kb ? zj : 0.0;
+    zs[(jb % ZR) * TS + lane] = zv;
+    if (G > 1) {
+        zg[jb * TS + lane] = zv;
+        dev_publish(fl + jb, ep);
+    }
     lds_publish(done, jb + 1);
     return true;
 }
 
-template <typename T, int NW>
+// Forward level solve.  Task: r0 = g, c0 = G (workgroups of the supernode), ws = exchange area
+// in zbuf (nb * 64 doubles), px = flag area (nb flags); see above for the other fields.
+template <typename T, int NW, int G>
 __global__ __launch_bounds__(64 * NW) void k_fwd_sn(const lkd_task* __restrict__ tasks,
                                                     const T* __restrict__ L,
                                                     const T* __restrict__ diag,
                                                     const int32_t* __restrict__ sn_rows,
                                                     double* __restrict__ x,
-                                                    double* __restrict__ below, int32_t* err) {
+                                                    double* __restrict__ below, double* zbuf,
+                                                    uint32_t* flags, const uint32_t* epoch,
+                                                    int32_t* err) {
     const lkd_task tk = tasks[blockIdx.x];
     __shared__ double zs[ZR * TS];                    // ring of solved blocks (zero past kb)
     __shared__ double Ds[2][TS * 65];                 // staged diagonal blocks (block jb: jb & 1)
-    __shared__ int done;                              // blocks solved
+    __shared__ int done;                              // blocks available in the ring
     __shared__ int prog[NW];                          // iterations finished per wavefront
     const T* P = L + tk.off;
     const int64_t ld = tk.ld;
     const Tiles tl(ld, tk.ncol);
     const int nb = tl.nb, nt = tl.nt, nc = tk.ncol;
     const int v = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const Own<G, NW> own{tk.r0 * NW + v, tk.r0};
+    const uint32_t ep = G > 1 ? *epoch : 0u;
+    double* zg = zbuf + tk.ws;
+    uint32_t* fl = flags + tk.px;
     double* bl = below + tk.kbeg;
     if (threadIdx.x == 0) done = 0;
     if (lane == 0) prog[v] = 0;
-    for (int t = first_tile<NW>(nb, v); t < nt; t += NW)
+    for (int t = own.first(nb); t < nt; t += own.nwt)
         if (tl.beg(t) + lane < tl.end(t)) bl[tl.beg(t) + lane - nc] = 0.0;
     __syncthreads();
     auto stage = [&](int jb) {
         stage_diag<T>(diag + tk.aux + (int64_t)jb * TS * TS, min(TS, nc - jb * TS), Ds[jb & 1]);
     };
-    if (v == 0) {
+    if (own.gw == 0) {
         stage(0);
-        if (!fwd_solve_block<NW>(tk, 0, x, zs, Ds[0], &done, prog, err)) return;
+        if (!fwd_solve_block<NW, G>(tk, 0, x, zs, Ds[0], &done, prog, zg, fl, ep, err)) return;
     }
+#pragma unroll 1
     for (int j = 0; j < nb; ++j) {
         const int kbj = min(TS, nc - j * TS);
         const T* Pj = P + (int64_t)j * TS * ld;
-        const int t1 = first_tile<NW>(j + 1, v);
+        const int t1 = own.first(j + 1);
         // the owner of block j + 1 stages its factor now: slot (j + 1) & 1 was last read by the
-        // solve of block j - 1, which finished before this wavefront passed iteration j - 1
+        // solve of an earlier block of this workgroup, finished before iteration j - 1 ended here
         if (t1 == j + 1 && t1 < nb) stage(t1);
         T lv[TS];
         // the first tile's panel data does not depend on z_j: load it before waiting
-        load_tile<T>(lv, Pj + tl.beg(min(t1, nt - 1)) + lane, ld, kbj, t1 < nt && tl.beg(t1) + lane < tl.end(t1));
+        {
+            const int tt = min(t1, nt - 1);
+            load_tile<T>(lv, Pj + min(tl.beg(tt) + lane, tl.end(tt) - 1), ld, kbj,
+                         t1 < nt && tl.beg(t1) + lane < tl.end(t1));
+        }
+        // z_j solved in another workgroup: wavefront 0 brings it into this workgroup's ring
+        if (G > 1 && v == 0 && own.wg_of(j) != own.g) {
+            if (!ring_free<NW>(j, prog, err) || !dev_wait(fl + j, ep, err)) return;
+            zs[(j % ZR) * TS + lane] = zg[j * TS + lane];
+            lds_publish(&done, j + 1);
+        }
         if (!lds_wait(&done, [&](int d) { return d > j; }, err)) return;
         const double* z = zs + (j % ZR) * TS;
         if (t1 < nt) {
             fwd_apply<T>(tl, t1, lv, z, x + tk.first, bl);
-            if (t1 == j + 1 && t1 < nb && !fwd_solve_block<NW>(tk, t1, x, zs, Ds[t1 & 1], &done, prog, err))
+            if (t1 == j + 1 && t1 < nb &&
+                !fwd_solve_block<NW, G>(tk, t1, x, zs, Ds[t1 & 1], &done, prog, zg, fl, ep, err))
                 return;   // look-ahead: the next block first
         }
-        for (int t = t1 + NW; t < nt; t += NW) {
-            load_tile<T>(lv, Pj + tl.beg(t) + lane, ld, kbj, tl.beg(t) + lane < tl.end(t));
+#pragma unroll 1
+        for (int t = t1 + own.nwt; t < nt; t += own.nwt) {
+            load_tile<T>(lv, Pj + min(tl.beg(t) + lane, tl.end(t) - 1), ld, kbj, tl.beg(t) + lane < tl.end(t));
             fwd_apply<T>(tl, t, lv, z, x + tk.first, bl);
         }
         if (nb > ZR) lds_publish(&prog[v], j + 1);
     }
-    for (int t = first_tile<NW>(nb, v); t < nt; t += NW) {
+    for (int t = own.first(nb); t < nt; t += own.nwt) {
         const int64_t row = tl.beg(t) + lane;
         if (row < tl.end(t)) atomic_add(x + sn_rows[tk.aux2 + row], bl[row - nc]);
     }
 }
 
-template <typename T, int NW>
+// Backward level solve.  Task: r0 = g, c0 = G, ws = partial sums in pbuf (nb * G * NW * 64
+// doubles: [block][global wavefront][64]), px = flag area (forward: nb flags, then backward:
+// nb * G flags, one per block and workgroup).  In a workgroup that does not own block j, the
+// last wavefront to store its part (an LDS counter per block) publishes the workgroup's parts.
+template <typename T, int NW, int G>
 __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__ tasks,
                                                     const T* __restrict__ L,
                                                     const T* __restrict__ diag,
                                                     const int32_t* __restrict__ sn_rows,
                                                     double* __restrict__ x,
                                                     double* __restrict__ below,
-                                                    double* __restrict__ pbuf, int32_t* err) {
+                                                    double* pbuf, uint32_t* flags,
+                                                    const uint32_t* epoch, int32_t* err) {
     constexpr int CW = 16;                            // columns per transposed chunk
     constexpr int NQ = TS / CW;
+    constexpr int NC = 64;                            // part counters (blocks in flight < NC)
     const lkd_task tk = tasks[blockIdx.x];
     __shared__ double Ts[NW][CW * 65];                // per wavefront: CW columns x 64 rows
     __shared__ int pub[NW];                           // lowest block each wavefront published
+    __shared__ int cnt[NC];                           // parts stored per block (mod NC)
     const T* P = L + tk.off;
     const int64_t ld = tk.ld;
     const Tiles tl(ld, tk.ncol);
     const int nb = tl.nb, nt = tl.nt, nc = tk.ncol;
     const int v = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const Own<G, NW> own{tk.r0 * NW + v, tk.r0};
+    const uint32_t ep = G > 1 ? *epoch : 0u;
+    uint32_t* fl = flags + tk.px + nb;                // [block][workgroup]
     const int cl = lane % CW, rg = lane / CW;         // column of a chunk, row group
     double* bl = below + tk.kbeg;
-    double* pb = pbuf + tk.ws;                        // [nb][NW][64]
+    double* pb = pbuf + tk.ws;                        // [nb][G * NW][64]
     double* Tw = Ts[v];
-    for (int t = first_tile<NW>(nb, v); t < nt; t += NW) {
+    for (int t = own.first(nb); t < nt; t += own.nwt) {
         const int64_t row = tl.beg(t) + lane;
         if (row < tl.end(t)) bl[row - nc] = x[sn_rows[tk.aux2 + row]];
     }
     if (lane == 0) pub[v] = nb;
+    for (int i = threadIdx.x; i < NC; i += 64 * NW) cnt[i] = 0;
     __syncthreads();
-    // part[q] (lanes cl): += sum over the rows of tile t of L(t, jb)^T x_t, columns CW q + cl.
-    auto contrib = [&, tl](double (&part)[NQ], int t, int jb) {
+    // part[q] (lanes cl): += sum over the rows of tile t of L(t, jb)^T x_t, columns CW q + cl,
+    // for the tiles t = t0, t0 + G NW, ... < tlim; software-pipelined by halves (32 columns): the
+    // next half's loads are in flight while the current one goes through the LDS transpose.
+    auto chunk = [&](double& acc, const T* h, double xr) {
+#pragma unroll
+        for (int c = 0; c < CW; ++c) Tw[c * 65 + lane] = (double)h[c] * xr;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < CW; ++i) s += Tw[cl * 65 + CW * rg + i];
+#pragma unroll
+        for (int o = CW; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
+        acc += s;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    auto contrib_list = [&, tl](double (&part)[NQ], int t0, int jb, int tlim) {
+        if (t0 >= tlim) return;
         const int kb = min(TS, nc - jb * TS);
-        const int64_t r0 = tl.beg(t);
-        const bool ok = r0 + lane < tl.end(t);
-        const double xr = ok ? (t < nb ? x[tk.first + r0 + lane] : bl[r0 + lane - nc]) : 0.0;
-        T lv[TS];
-        load_tile<T>(lv, P + (int64_t)jb * TS * ld + r0 + lane, ld, kb, ok);
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-#pragma unroll
-            for (int c = 0; c < CW; ++c) Tw[c * 65 + lane] = (double)lv[CW * q + c] * xr;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            double s = 0.0;
-#pragma unroll
-            for (int i = 0; i < CW; ++i) s += Tw[cl * 65 + CW * rg + i];
-#pragma unroll
-            for (int o = CW; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
-            part[q] += s;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const T* Pb = P + (int64_t)jb * TS * ld;
+        auto rowp = [&](int u) { return Pb + min(tl.beg(u) + lane, tl.end(u) - 1); };
+        auto rok = [&](int u) { return tl.beg(u) + lane < tl.end(u); };
+        auto xval = [&](int u) {
+            const int64_t r = tl.beg(u) + lane;
+            return r < tl.end(u) ? (u < nb ? x[tk.first + r] : bl[r - nc]) : 0.0;
+        };
+        T h0[TS / 2], h1[TS / 2];
+        load_half<T>(h0, rowp(t0), ld, 0, kb, rok(t0));
+        load_half<T>(h1, rowp(t0), ld, TS / 2, kb, rok(t0));
+#pragma unroll 1
+        for (int t = t0; t < tlim; t += own.nwt) {
+            const int tn = t + own.nwt < tlim ? t + own.nwt : t;    // past the end: reload, unused
+            const double xr = xval(t);
+            chunk(part[0], h0, xr);
+            chunk(part[1], h0 + CW, xr);
+            load_half<T>(h0, rowp(tn), ld, 0, kb, rok(tn));
+            chunk(part[2], h1, xr);
+            chunk(part[3], h1 + CW, xr);
+            load_half<T>(h1, rowp(tn), ld, TS / 2, kb, rok(tn));
         }
     };
     double pre[NQ];          // look-ahead contributions to block pre_blk (tiles t > pre_blk + 1)
@@ -311,78 +441,113 @@ __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__
         const bool have = pre_blk == j;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) part[q] = have ? pre[q] : 0.0;
-        if (have) contrib(part, j + 1, j);            // the tile solved last is the only one left
-        else
-            for (int t = first_tile<NW>(j + 1, v); t < nt; t += NW) contrib(part, t, j);
-        double* pj = pb + ((int64_t)j * NW + v) * TS;
+        if (have) contrib_list(part, j + 1, j, j + 2);   // the tile solved last is the only one left
+        else contrib_list(part, own.first(j + 1), j, nt);
+        double* pj = pb + ((int64_t)j * own.nwt + own.gw) * TS;
         if (lane < CW) {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) pj[CW * q + lane] = part[q];
         }
         lds_publish(&pub[v], j);
-        if (j % NW != v) continue;
+        const int ow = own.wg_of(j);
+        if (G > 1 && ow != own.g) {
+            // the last of this workgroup's wavefronts to store its part publishes them all
+            int last = 0;
+            if (lane == 0) last = (atomicAdd(&cnt[j % NC], 1) + 1) % NW == 0;
+            if (__builtin_amdgcn_readfirstlane(last)) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                dev_publish(fl + (int64_t)j * G + own.g, ep);
+            }
+        }
+        if ((j % own.nwt) != own.gw) continue;
         // owner of block j: while the other parts arrive, start block j - 1 with the tiles
         // that do not need x_j
         if (j > 0) {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) pre[q] = 0.0;
-            for (int t = first_tile<NW>(j + 1, v); t < nt; t += NW) contrib(pre, t, j - 1);
+            contrib_list(pre, own.first(j + 1), j - 1, nt);
             pre_blk = j - 1;
         }
+        __builtin_amdgcn_sched_barrier(0);
         // the diagonal block's columns load while the other parts arrive
         const int kbj = min(TS, nc - j * TS);
         const T* Lr = diag + tk.aux + (int64_t)j * TS * TS;
         T col[TS];
+        if (kbj == TS) {
 #pragma unroll
-        for (int i = 0; i < TS; ++i) col[i] = (i < kbj && lane < kbj) ? Lr[i * kbj + lane] : T(0);
-        const double rd = lane < kbj ? (double)Lr[lane * kbj + lane] : 0.0;
+            for (int i = 0; i < TS; ++i) col[i] = Lr[i * TS + lane];
+        } else {
+#pragma unroll
+            for (int i = 0; i < TS; ++i)       // clamped loads times a 0/1 mask: no branches
+                col[i] = Lr[min(i, kbj - 1) * kbj + min(lane, kbj - 1)] * ((i < kbj && lane < kbj) ? T(1) : T(0));
+        }
+        const int ld_ = min(lane, kbj - 1);
+        const double rd = lane < kbj ? (double)Lr[ld_ * kbj + ld_] : 0.0;
+#pragma unroll
+        for (int i = 0; i < TS; ++i) col[i] = i == lane ? T(0) : col[i];
         for (int u = 0; u < NW; ++u)
             if (u != v && !lds_wait(&pub[u], [&](int p) { return p <= j; }, err)) return;
+        for (int h = 0; h < G; ++h)
+            if (h != own.g && !dev_wait(fl + (int64_t)j * G + h, ep, err)) return;
         double val = 0.0;
         if (lane < kbj) {
             double s = 0.0;
-            const double* p0 = pb + (int64_t)j * NW * TS + lane;
-            for (int u = 0; u < NW; ++u) s += p0[(int64_t)u * TS];
+            const double* p0 = pb + (int64_t)j * own.nwt * TS + lane;
+            for (int u = 0; u < own.nwt; ++u) s += p0[(int64_t)u * TS];
             val = x[tk.first + j * TS + lane] - s;
         }
-        const double sol = bsub_regs<T>(val, col, rd, kbj);
+        const double sol = bsub_regs<T>(val, col, rd);
         if (lane < kbj) x[tk.first + j * TS + lane] = sol;
     }
 }
 
-template <typename T, int NW>
+template <typename T, int NW, int G>
 int level_solve(int dir, int32_t ntask, const lkd_task* tasks, const void* panel, const void* diag,
-                const int32_t* rows, double* x, double* below, double* pbuf, int32_t* err,
-                void* stream) {
+                const int32_t* rows, double* x, double* below, double* pbuf, uint32_t* flags,
+                const uint32_t* epoch, int32_t* err, void* stream) {
     if (dir == 0)
-        LK_LAUNCH((k_fwd_sn<T, NW>), ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
-                  (const T*)diag, rows, x, below, err);
+        LK_LAUNCH((k_fwd_sn<T, NW, G>), ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
+                  (const T*)diag, rows, x, below, pbuf, flags, epoch, err);
     else
-        LK_LAUNCH((k_bwd_sn<T, NW>), ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
-                  (const T*)diag, rows, x, below, pbuf, err);
+        LK_LAUNCH((k_bwd_sn<T, NW, G>), ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
+                  (const T*)diag, rows, x, below, pbuf, flags, epoch, err);
     return lk_last_error();
 }
 
+__global__ void k_epoch(uint32_t* epoch) {
+    if (threadIdx.x == 0) atomicAdd(epoch, 1u);
+}
+
+// wavefronts per workgroup 4 or 8, workgroups per supernode 1 or 4 (the backward uses 1)
 template <typename T>
-int level_solve_nw(int nw, int dir, int32_t ntask, const lkd_task* tasks, const void* panel,
-                   const void* diag, const int32_t* rows, double* x, double* below, double* pbuf,
-                   int32_t* err, void* stream) {
-    return nw == 8 ? level_solve<T, 8>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, err, stream)
-                    : level_solve<T, 4>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, err, stream);
+int level_solve_cfg(int nw, int g, int dir, int32_t ntask, const lkd_task* tasks, const void* panel,
+                    const void* diag, const int32_t* rows, double* x, double* below, double* pbuf,
+                    uint32_t* flags, const uint32_t* epoch, int32_t* err, void* stream) {
+    if (nw == 8 && g == 4 && dir == 0)
+        return level_solve<T, 8, 4>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, stream);
+    if (nw == 8)
+        return level_solve<T, 8, 1>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, stream);
+    return level_solve<T, 4, 1>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, stream);
 }
 
 }  // namespace
 
 extern "C" {
 
-int lkd_trsv_level(int prec, int dir, int nw, int32_t ntask, const lkd_task* tasks,
+int lkd_trsv_level(int prec, int dir, int nw, int g, int32_t ntask, const lkd_task* tasks,
                    const void* panel, const void* diag, const int32_t* sn_rows, double* x,
-                   double* below, double* pbuf, int32_t* err, void* stream) {
+                   double* below, double* pbuf, uint32_t* flags, const uint32_t* epoch,
+                   int32_t* err, void* stream) {
     if (ntask <= 0) return 0;
-    if (nw != 4 && nw != 8) return (int)hipErrorInvalidValue;
+    if ((nw != 4 && nw != 8) || (g != 1 && !(g == 4 && nw == 8 && dir == 0))) return (int)hipErrorInvalidValue;
     return prec == LKD_F32
-               ? level_solve_nw<float>(nw, dir, ntask, tasks, panel, diag, sn_rows, x, below, pbuf, err, stream)
-               : level_solve_nw<double>(nw, dir, ntask, tasks, panel, diag, sn_rows, x, below, pbuf, err, stream);
+               ? level_solve_cfg<float>(nw, g, dir, ntask, tasks, panel, diag, sn_rows, x, below, pbuf, flags, epoch, err, stream)
+               : level_solve_cfg<double>(nw, g, dir, ntask, tasks, panel, diag, sn_rows, x, below, pbuf, flags, epoch, err, stream);
+}
+
+int lkd_trsv_epoch(uint32_t* epoch, void* stream) {
+    LK_LAUNCH(k_epoch, 1, 64, 0, stream, epoch);
+    return lk_last_error();
 }
 
 }  // extern "C"

@@ -40,6 +40,8 @@ constexpr int KSPLIT_MIN = 128;     // minimum K columns per split part
 constexpr int STEP_TARGET_WG = 512; // a STEP launch with fewer tiles splits K (PART launch) ...
 constexpr int STEP_KMIN = 64;       // ... into parts of at least this many columns
 constexpr int SOLVE_WIDE_TILES = 8;   // level solves: 8 wavefronts per supernode above this
+constexpr int SOLVE_MAX_G = 4;        // forward level solves: workgroups per supernode (all resident)
+constexpr int SOLVE_G_LEVEL = 4;      // ... only for levels of at most this many supernodes
 constexpr int FWD_ROWS = 256;
 
 // Order a launch's tasks for dispatch.  Workgroup b runs task (b % 8) * per + b / 8 with
@@ -64,6 +66,7 @@ struct Launch {
     int64_t begin;
     int32_t count;
     int32_t nw = 0;  // level solves: wavefronts per workgroup
+    int32_t g = 1;   // level solves: workgroups per supernode
 };
 
 template <typename T>
@@ -126,7 +129,8 @@ struct GPUKKTCholeskySolver::Impl {
     void* d_work = nullptr;        // PART products (sized for FP64)
     int64_t work_elems = 0;
     double* d_below = nullptr;     // level solves: rows below each supernode
-    double* d_pbuf = nullptr;      // level solves: backward partial sums
+    double* d_pbuf = nullptr;      // level solves: z exchange / backward partial sums
+    uint32_t* d_flags = nullptr;   // level solves: hand-off flags, then the epoch
     int32_t* d_err = nullptr;
     double *d_w = nullptr, *d_D = nullptr, *d_m0d = nullptr, *d_theta = nullptr;
     double *d_r = nullptr, *d_req = nullptr, *d_z = nullptr, *d_dy = nullptr, *d_bak = nullptr,
@@ -594,11 +598,13 @@ struct GPUKKTCholeskySolver::Impl {
                 if (t.kbeg < 0 || t.kend > nc || t.kbeg >= t.kend) bad("update K");
                 break;
             case 20:
-            case 21: {  // level solves: wavefronts per workgroup, scratch / partial-sum offsets
+            case 21: {  // level solves: workgroups, wavefronts, scratch / exchange / flag offsets
                 const int nb = (nc + TILE - 1) / TILE;
-                if (t.c0 != 4 && t.c0 != 8) bad("solve wavefronts");
-                if (t.ws < 0 || t.ws + int64_t(nb) * t.c0 * TILE > solve_pbuf || t.kbeg < 0 ||
-                    t.kbeg + (ld - nc) > solve_below)
+                if (t.nsplit != 4 && t.nsplit != 8) bad("solve wavefronts");
+                if ((t.c0 != 1 && !(t.c0 == SOLVE_MAX_G && t.nsplit == 8 && kind == 20)) || t.r0 < 0 || t.r0 >= t.c0)
+                    bad("solve workgroups");
+                if (t.ws < 0 || t.ws + int64_t(nb) * t.c0 * t.nsplit * TILE > solve_pbuf || t.kbeg < 0 ||
+                    t.kbeg + (ld - nc) > solve_below || t.px < 0 || t.px + nb + int64_t(nb) * t.c0 > solve_flags)
                     bad("solve offsets");
                 if (!diag_ok(t.aux, 0) || t.aux + int64_t(nb - 1) * TILE * TILE + int64_t(nc - (nb - 1) * TILE) * (nc - (nb - 1) * TILE) > diag_elems)
                     bad("solve diagonal copies");
@@ -608,21 +614,37 @@ struct GPUKKTCholeskySolver::Impl {
         }
     }
 
-    // Solve plan: per etree level one launch per direction, one workgroup per supernode with
-    // 8 wavefronts when some supernode of the level has more than SOLVE_WIDE_TILES row tiles,
-    // else 4.  task.c0 = wavefronts, task.ws = partial-sum offset, task.kbeg = scratch offset.
+    // Solve plan: per etree level one launch per direction.  Wavefronts per workgroup: 8 when
+    // some supernode of the level has more than SOLVE_WIDE_TILES row tiles, else 4.  Forward
+    // launches of wide levels with at most SOLVE_G_LEVEL supernodes spread each supernode over
+    // G <= SOLVE_MAX_G workgroups (all workgroups of the launch fit on the compute units at once:
+    // they wait on each other).  Device-scope hand-offs cost an L2 write-back / invalidate each,
+    // which outweighs the extra bandwidth when many supernodes run side by side, and the
+    // backward direction needs one cross-workgroup reduction per block: those keep G = 1.
+    // task.nsplit = wavefronts, c0 = G, r0 = g, ws = exchange offset, px = flag offset,
+    // kbeg = scratch offset.
     void build_solve_plan() {
         std::vector<lkd_task> tasks;
-        std::vector<std::vector<lkd_task>> per_level;
-        int64_t nbelow = 0, npbuf = 0;
+        std::vector<std::vector<lkd_task>> per_level, per_level_bwd;
+        int cus = 0;
+        if (lkd_device_cus(&cus) != 0 || cus < 1) cus = 0;   // no device: one workgroup each
+        int64_t nbelow = 0, npbuf = 0, nflags = 0;
         for (const auto& lv : levels) {
-            int nw = 4;
+            int nw = 4, maxnt = 1;
             for (const Index s : lv) {
                 const Index nc = S.sn_first[s + 1] - S.sn_first[s];
                 const int64_t ld = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
-                if ((nc + TILE - 1) / TILE + (ld - nc + TILE - 1) / TILE > SOLVE_WIDE_TILES) nw = 8;
+                const int nt = static_cast<int>((nc + TILE - 1) / TILE + (ld - nc + TILE - 1) / TILE);
+                maxnt = std::max(maxnt, nt);
+                if (nt > SOLVE_WIDE_TILES) nw = 8;
             }
-            std::vector<lkd_task> lt;
+            int G = 1;
+            if (nw == 8 && cus > 0 && static_cast<int64_t>(lv.size()) <= SOLVE_G_LEVEL) {
+                const int64_t fit = cus / static_cast<int64_t>(lv.size());
+                G = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({SOLVE_MAX_G, fit, (maxnt + nw - 1) / nw})));
+                if (G != SOLVE_MAX_G) G = 1;                      // kernels exist for G = 1 and 4
+            }
+            std::vector<lkd_task> lt, lb;
             for (const Index s : lv) {
                 lkd_task t{};
                 t.off = S.sn_panel[s];
@@ -632,30 +654,43 @@ struct GPUKKTCholeskySolver::Impl {
                 t.first = S.sn_first[s];
                 t.aux = diag_off(s, 0);
                 t.aux2 = S.sn_rowptr[s];
-                t.c0 = nw;
+                t.nsplit = nw;
+                t.c0 = G;
                 t.ws = npbuf;
+                t.px = nflags;
                 t.kbeg = static_cast<int32_t>(nbelow);
-                npbuf += int64_t((t.ncol + TILE - 1) / TILE) * nw * TILE;
+                const int64_t nb = (t.ncol + TILE - 1) / TILE;
+                npbuf += nb * G * nw * TILE;
+                nflags += nb + nb * G;
                 nbelow += t.ld - t.ncol;
-                lt.push_back(t);
+                for (int g = 0; g < G; ++g) {
+                    lkd_task u = t;
+                    u.r0 = g;
+                    lt.push_back(u);
+                }
+                lkd_task u = t;
+                u.c0 = 1;
+                lb.push_back(u);
             }
             per_level.push_back(std::move(lt));
+            per_level_bwd.push_back(std::move(lb));
         }
         auto emit = [&](int kind, const std::vector<lkd_task>& lt) {
             slaunch.push_back({kind, static_cast<int64_t>(tasks.size()), static_cast<int32_t>(lt.size()),
-                               lt.empty() ? 4 : lt[0].c0});
+                               lt.empty() ? 4 : lt[0].nsplit, lt.empty() ? 1 : lt[0].c0});
             tasks.insert(tasks.end(), lt.begin(), lt.end());
         };
         for (const auto& lt : per_level) emit(20, lt);                                       // forward
-        for (auto it = per_level.rbegin(); it != per_level.rend(); ++it) emit(21, *it);      // backward
+        for (auto it = per_level_bwd.rbegin(); it != per_level_bwd.rend(); ++it) emit(21, *it);   // backward
         solve_pbuf = npbuf;
+        solve_flags = nflags;
         solve_below = nbelow;
         for (const auto& L : slaunch)
             for (int64_t q = L.begin; q < L.begin + L.count; ++q) validate(tasks[q], L.kind);
         d_stasks = up(tasks, "stasks");
         n_stasks = static_cast<int64_t>(tasks.size());
     }
-    int64_t solve_pbuf = 0, solve_below = 0;
+    int64_t solve_pbuf = 0, solve_below = 0, solve_flags = 0;
 
     // reference CSR layout of L (sym.L_pattern rows + diagonal last) -> panel indices
     void build_csr_index(const ipm::SymbolicFactorization& sym) {
@@ -697,6 +732,8 @@ struct GPUKKTCholeskySolver::Impl {
         owned.push_back(d_below);
         d_pbuf = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(solve_pbuf, 1)) * sizeof(double)));
         owned.push_back(d_pbuf);
+        d_flags = static_cast<uint32_t*>(arena_p->allocate(static_cast<size_t>(solve_flags + 1) * sizeof(uint32_t)));
+        owned.push_back(d_flags);   // flags [0, solve_flags), epoch [solve_flags]; zero-filled
         auto dv = [&](int64_t len) {
             double* p = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(len, 1)) * sizeof(double)));
             owned.push_back(p);
@@ -924,10 +961,12 @@ struct GPUKKTCholeskySolver::Impl {
     }
 
     void run_solve_schedule(int prec, double* x) {
+        uint32_t* epoch = d_flags + solve_flags;
+        check(lkd_trsv_epoch(epoch, stream), "solve epoch");
         for (const auto& L : slaunch) {
             const int dir = L.kind == 20 ? 0 : 1;
-            check(lkd_trsv_level(prec, dir, L.nw, L.count, d_stasks + L.begin, d_panel, d_diag, d_sn_rows,
-                                 x, d_below, d_pbuf, d_err + 1, stream), "solve kernel");
+            check(lkd_trsv_level(prec, dir, L.nw, L.g, L.count, d_stasks + L.begin, d_panel, d_diag, d_sn_rows,
+                                 x, d_below, d_pbuf, d_flags, epoch, d_err + 1, stream), "solve kernel");
         }
     }
 

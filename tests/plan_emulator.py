@@ -173,28 +173,37 @@ class Emulator:
 
     def solve(self, x):
         """In-place L L^T solve of the permuted vector x with the level schedule (kinds 20 / 21):
-        one workgroup per supernode of a level, all with the same wavefront count; per
-        supernode the blocks are processed in order, rows below go to the ancestors' entries
-        (forward) or are read from them (backward).  Checks the plan's scratch and partial-sum
-        ranges (disjoint per supernode) on the way."""
+        G workgroups per supernode of a level (task g = r0 of c0), all with the same wavefront
+        count; per supernode the blocks are processed in order, rows below go to the ancestors'
+        entries (forward) or are read from them (backward).  Checks the plan's workgroup sets
+        and its scratch, exchange and flag ranges (disjoint per supernode) on the way."""
         p = self.p
         T = p["stasks"]
+        seen_flags = []
         for kind, b, c in self.sl:
             assert kind in (20, 21)
             tl = T[b:b + c]
-            assert len({int(u["c0"]) for u in tl}) == 1 and int(tl[0]["c0"]) in (4, 8)
-            assert len({int(u["sn"]) for u in tl}) == len(tl)
-            for t in tl:
+            assert len({int(u["nsplit"]) for u in tl}) == 1 and int(tl[0]["nsplit"]) in (4, 8)
+            for t in tl:                            # G workgroups per supernode: g = 0 .. G-1 once each
+                G = int(t["c0"])
+                assert 1 <= G and 0 <= int(t["r0"]) < G
+                gs = sorted(int(u["r0"]) for u in tl if u["sn"] == t["sn"])
+                assert gs == list(range(G))
+            heads = [t for t in tl if int(t["r0"]) == 0]
+            for t in heads:
                 nbk = (int(t["ncol"]) + TS - 1) // TS
-                for u in tl:
+                G, nw = int(t["c0"]), int(t["nsplit"])
+                for u in heads:
                     if u["sn"] != t["sn"]:
-                        a0, a1 = int(t["ws"]), int(t["ws"]) + nbk * int(t["c0"]) * TS
+                        a0, a1 = int(t["ws"]), int(t["ws"]) + nbk * G * nw * TS
                         b0 = int(u["ws"])
-                        b1 = b0 + (int(u["ncol"]) + TS - 1) // TS * int(u["c0"]) * TS
+                        b1 = b0 + (int(u["ncol"]) + TS - 1) // TS * int(u["c0"]) * int(u["nsplit"]) * TS
                         assert a1 <= b0 or b1 <= a0
                         a0, a1 = int(t["kbeg"]), int(t["kbeg"]) + int(t["ld"]) - int(t["ncol"])
                         b0, b1 = int(u["kbeg"]), int(u["kbeg"]) + int(u["ld"]) - int(u["ncol"])
                         assert a1 <= b0 or b1 <= a0 or a0 == a1 or b0 == b1
+                seen_flags.append((int(t["px"]), int(t["px"]) + nbk + nbk * G))
+            for t in heads:
                 ld, nc, first = int(t["ld"]), int(t["ncol"]), int(t["first"])
                 rows = p["sn_rows"][int(t["aux2"]):int(t["aux2"]) + ld]
                 nb = (nc + TS - 1) // TS
@@ -212,4 +221,6 @@ class Emulator:
                 else:
                     y = x[first:first + nc] - (Lfull[nc:ld].T @ x[rows[nc:ld]] if ld > nc else 0.0)
                     x[first:first + nc] = np.linalg.solve(Ld.T, y)
+        fl = sorted(set(seen_flags))                # flag areas of different supernodes disjoint
+        assert all(fl[i][1] <= fl[i + 1][0] for i in range(len(fl) - 1))
         return x
