@@ -64,6 +64,12 @@ struct SupernodeOptions {
     Index relax_small = 16;
     double relax_frac = 0.30;
     Index max_cols = 1 << 30;
+    // A child of more than `serial_child` columns is not merged into a parent when a sibling
+    // subtree carries at least `serial_frac` of the child's subtree work: the merge would move
+    // the child's whole factorization behind that sibling's (one etree level up), where it runs
+    // alone instead of beside it.
+    Index serial_child = 32;
+    double serial_frac = 0.5;
 };
 
 void compute_supernodal_structure(const core::CSRMatrix& A, const std::vector<Index>& P,

@@ -267,9 +267,19 @@ void compute_supernodal_structure(const core::CSRMatrix& A, const std::vector<In
         sp[s] = pj == -1 ? -1 : colsn[pj];
     }
 
-    // 4. relaxed amalgamation: top-down, merge the contiguous preceding child into its parent.
+    // 4. relaxed amalgamation: top-down, merge the contiguous preceding child into its parent
+    //    (never a wide child into a parent with other children: see serial_child).
     auto stored = [](int64_t n, int64_t l) { return n * l - n * (n - 1) / 2; };
     std::vector<char> alive(static_cast<size_t>(ns), 1);
+    std::vector<std::vector<Index>> skids(static_cast<size_t>(ns));
+    std::vector<double> subw(static_cast<size_t>(ns), 0.0);   // subtree work, sum of cc^2
+    for (Index s = 0; s < ns; ++s) {                          // children precede parents
+        for (Index j = first[s]; j < first[s + 1]; ++j) subw[s] += static_cast<double>(cc[j]) * cc[j];
+        if (sp[s] != -1) {
+            skids[sp[s]].push_back(s);
+            subw[sp[s]] += subw[s];
+        }
+    }
     std::vector<Index> absorbed_into(static_cast<size_t>(ns));
     std::iota(absorbed_into.begin(), absorbed_into.end(), 0);
     std::vector<Index> sfirst(first.begin(), first.end() - 1);
@@ -286,6 +296,12 @@ void compute_supernodal_structure(const core::CSRMatrix& A, const std::vector<In
             if (cp != p) break;
             const int64_t n = ncol[c] + ncol[p];
             if (n > opt.max_cols) break;
+            if (ncol[c] > opt.serial_child) {
+                double other = 0.0;
+                for (const Index k : skids[p])
+                    if (k != c) other = std::max(other, subw[k]);
+                if (other >= opt.serial_frac * subw[c]) break;
+            }
             const int64_t l = ncol[c] + ld[p];
             const int64_t st = stored(n, l);
             const int64_t zeros = st - (exact[c] + exact[p]);
@@ -294,6 +310,8 @@ void compute_supernodal_structure(const core::CSRMatrix& A, const std::vector<In
             if (!ok) break;
             alive[c] = 0;
             absorbed_into[c] = p;
+            skids[p].erase(std::find(skids[p].begin(), skids[p].end(), c));
+            skids[p].insert(skids[p].end(), skids[c].begin(), skids[c].end());
             sfirst[p] = sfirst[c];
             ncol[p] = n;
             ld[p] = l;
