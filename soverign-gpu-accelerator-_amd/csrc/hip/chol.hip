@@ -427,6 +427,9 @@ void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T*
     const int k0 = tk.k0, kb = tk.kb, kbm = kb - 1;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15;
     const int ra = tk.c0;
+    // the tile that factors the next diagonal block is the launch's critical path: its waves
+    // get issue priority over the other workgroup sharing the compute unit
+    if (tk.nsplit) __builtin_amdgcn_s_setprio(3);
     Acc<T> acc[4][2];
     zero_acc<T, 4, 2>(acc);
     gemm_tn<T>(P + tk.r0, ld, ra, P + k0, ld, kb, tk.kbeg, tk.kend, sm, acc);
@@ -494,19 +497,60 @@ void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T*
                 if (c < kb) P[(int64_t)(k0 + c) * ld + row[b]] = y[b][p][e];
             }
     }
-    // ---- rank-kb updates of the diagonal blocks inside the tile (rows below ncol only)
+    // ---- rank-kb updates of the diagonal blocks inside the tile (rows below ncol only): waves
+    //      2h, 2h+1 read back the rows of block h they have just stored (each thread its own
+    //      elements, so program order makes them visible) into LDS (Ys[c * 65 + r] = Y[r][c]),
+    //      and C(d, d) -= Y Y^T is formed on MFMA, wave w computing the 16-row stripe w, lower
+    //      blocks only
     for (int h = 0; h < 2; ++h) {
         const int d0 = tk.r0 + 64 * h;                    // first row = first column of block
         if (64 * h >= ra || d0 >= tk.ncol) break;         // uniform
         const int db = min(64, min(ra - 64 * h, tk.ncol - d0));
-        __syncthreads();                                  // the solved rows are visible
-        Acc<T> d[2][4];
-        zero_acc<T, 2, 4>(d);
-        gemm_nt<T>(P + d0, ld, db, P + d0, ld, db, k0, k0 + kb, sm, d);
-        if (w < 2)
-            for_acc<T>(d, [&](int r, int c, T v) {
-                if (r < db && c <= r) P[(int64_t)(d0 + c) * ld + d0 + r] -= v;
-            });
+        T* Ys = S;
+        __syncthreads();                                  // the staged factor / last stripe is read
+        if ((w >> 1) == h) {
+            T v[2][4][4];
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int p = 0; p < 4; ++p)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {           // clamped address times a 0/1 mask
+                        const int rl = 32 * (w & 1) + 16 * b + li, c = 16 * p + R::row(l, e);
+                        v[b][p][e] = P[(int64_t)(k0 + min(c, kbm)) * ld + d0 + min(rl, db - 1)] *
+                                     ((c < kb && rl < db) ? T(1) : T(0));
+                    }
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int p = 0; p < 4; ++p)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        Ys[(16 * p + R::row(l, e)) * DP + 32 * (w & 1) + 16 * b + li] = v[b][p][e];
+        }
+        __syncthreads();
+        Acc<T> dq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dq[q] = Acc<T>{0, 0, 0, 0};
+#pragma unroll 1
+        for (int kk = 0; kk < TS; kk += 16)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int k = kk + R::row(l, s);
+                const T a = Ys[k * DP + 16 * w + li];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q <= w) dq[q] = R::mma(a, Ys[k * DP + 16 * q + li], dq[q]);
+            }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (q > w) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * w + R::row(l, e), c = 16 * q + li;
+                if (r < db && c <= r) P[(int64_t)(d0 + c) * ld + d0 + r] -= dq[q][e];
+            }
+        }
     }
     if (!tk.nsplit) return;
     // ---- the next diagonal block (this tile) is complete: factor and publish it
