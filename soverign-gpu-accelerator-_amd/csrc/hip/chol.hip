@@ -238,17 +238,17 @@ __device__ __forceinline__ bool potrf16_wave(T* Ds, int o, T* Lv, T* cb) {
         __builtin_amdgcn_sched_barrier(0);   // keep each column's loads next to their use
     }
     // column c = r of L^{-1}: x[i] = (delta_ic - sum_{k<i} L[i][k] x[k]) / L[i][i], with
-    // L[i][k] = cb[16 k + i] (column k as published above) and 1 / L[i][i] = cb[256 + i]: LDS
-    // broadcast reads that do not depend on x, so only one fma and one multiply per i are on
-    // the dependency chain (x[i] = 0 for i < c follows from the zero start)
-    __builtin_amdgcn_wave_barrier();
+    // L[i][k] = lane i's a[k] broadcast by readlane (scalar registers: few VGPRs, which matters
+    // where this is inlined into the step kernel)
     T x[16];
+    const T invd = cb[256 + r];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         T acc = i == r ? T(1) : T(0);
 #pragma unroll
-        for (int k = 0; k < i; ++k) acc = fma(-cb[16 * k + i], x[k], acc);
-        x[i] = acc * cb[256 + i];
+        for (int k = 0; k < 16; ++k)
+            if (k < i) acc = fma(-bcast_lane(a[k], i), x[k], acc);
+        x[i] = acc * bcast_lane(invd, i);
     }
     if (l < 16) {
 #pragma unroll
@@ -457,13 +457,23 @@ void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T*
         row[b] = (int64_t)tk.r0 + rl;
         valid[b] = rl < ra;
         const int64_t rc = valid[b] ? row[b] : tk.r0;     // a row inside the panel
+        if (kb == TS) {
+            // full block: one per-lane base, the element offsets are wave-uniform multiples of ld
+            const T* pb = P + (int64_t)(k0 + R::row(l, 0)) * ld + rc;
 #pragma unroll
-        for (int p = 0; p < 4; ++p)
+            for (int p = 0; p < 4; ++p)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int c = min(16 * p + R::row(l, e), kbm);   // columns past kb: padding
-                y[b][p][e] = P[(int64_t)(k0 + c) * ld + rc] - acc[p][b][e];
-            }
+                for (int e = 0; e < 4; ++e)
+                    y[b][p][e] = pb[(int64_t)(16 * p + R::row(l, e) - R::row(l, 0)) * ld] - acc[p][b][e];
+        } else {
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int c = min(16 * p + R::row(l, e), kbm);   // columns past kb: padding
+                    y[b][p][e] = P[(int64_t)(k0 + c) * ld + rc] - acc[p][b][e];
+                }
+        }
     }
     __syncthreads();
 #pragma unroll
