@@ -78,9 +78,11 @@ __device__ __forceinline__ void zero_acc(Acc<T> (&acc)[X][Y]) {
 
 // Double-buffered K loop shared by the GEMM variants: chunks of KC columns of
 // A (rows 0..ra of pa, column-major, lda) and B (rows 0..rb of pb, ldb) are staged k-major in
-// LDS (rows past ra / rb and columns past kend read as zero); the next chunk is loaded into
-// registers while compute(As, Bs, k) consumes the current one, for k = 0, 4, .., KC-4.
-// One barrier per chunk; the LDS is free again when it returns.
+// LDS (rows past ra / rb and columns past kend read as zero); compute(As, Bs, k) consumes the
+// staged chunk for k = 0, 4, .., KC-4.  Global loads run two chunks ahead in two register
+// sets (chunk j in set j % 2): while chunk c is computed, chunks c + 1 and c + 2 are in
+// flight, and chunk c + 1 goes to LDS right after.  One barrier per chunk; the LDS is free
+// again when it returns.
 template <typename T, typename F>
 __device__ __forceinline__ void gemm_loop(const T* __restrict__ pa, int64_t lda, int ra,
                                           const T* __restrict__ pb, int64_t ldb, int rb,
@@ -91,37 +93,45 @@ __device__ __forceinline__ void gemm_loop(const T* __restrict__ pa, int64_t lda,
     const int ar = t & (MT - 1), ak = t >> 7;
     const int br = t & (NC - 1), bk = t >> 6;
     const bool arow = ar < ra, brow = br < rb;
-    T va[NA], vb[NB];
-    auto load = [&](int kc) {
+    T va[2][NA], vb[2][NB];
+    auto load = [&](int set, int kc) {
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
             const int k = kc + ak + 2 * q;
-            va[q] = (arow && k < kend) ? pa[(int64_t)k * lda + ar] : T(0);
+            va[set][q] = (arow && k < kend) ? pa[(int64_t)k * lda + ar] : T(0);
         }
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             const int k = kc + bk + 4 * q;
-            vb[q] = (brow && k < kend) ? pb[(int64_t)k * ldb + br] : T(0);
+            vb[set][q] = (brow && k < kend) ? pb[(int64_t)k * ldb + br] : T(0);
         }
     };
-    auto store = [&](int buf) {
+    auto store = [&](int set, int buf) {
 #pragma unroll
-        for (int q = 0; q < NA; ++q) sm.a[buf][(ak + 2 * q) * PA + ar] = va[q];
+        for (int q = 0; q < NA; ++q) sm.a[buf][(ak + 2 * q) * PA + ar] = va[set][q];
 #pragma unroll
-        for (int q = 0; q < NB; ++q) sm.b[buf][(bk + 4 * q) * PB + br] = vb[q];
+        for (int q = 0; q < NB; ++q) sm.b[buf][(bk + 4 * q) * PB + br] = vb[set][q];
     };
     const int nch = (kend - kbeg + KC - 1) / KC;
     if (nch <= 0) return;
-    load(kbeg);
-    store(0);
+    load(0, kbeg);
+    store(0, 0);
+    if (nch > 1) load(1, kbeg + KC);
+    if (nch > 2) load(0, kbeg + 2 * KC);
     __syncthreads();
-    for (int c = 0; c < nch; ++c) {
-        const int buf = c & 1;
-        if (c + 1 < nch) load(kbeg + (c + 1) * KC);
+    // unrolled by two so that register sets and LDS buffers are compile-time indices
+    for (int c = 0; c < nch; c += 2) {
 #pragma unroll
-        for (int k = 0; k < KC; k += 4) compute(sm.a[buf], sm.b[buf], k);
-        if (c + 1 < nch) store(buf ^ 1);
+        for (int k = 0; k < KC; k += 4) compute(sm.a[0], sm.b[0], k);
+        if (c + 1 < nch) store(1, 1);
         __syncthreads();
+        if (c + 3 < nch) load(1, kbeg + (c + 3) * KC);
+        if (c + 1 >= nch) break;
+#pragma unroll
+        for (int k = 0; k < KC; k += 4) compute(sm.a[1], sm.b[1], k);
+        if (c + 2 < nch) store(0, 0);
+        __syncthreads();
+        if (c + 4 < nch) load(0, kbeg + (c + 4) * KC);
     }
 }
 
