@@ -98,29 +98,46 @@ typedef struct {
 } lkd_long;
 
 /* ---------------------------------------------------------------- normal-equations assembly */
-/* w[p] = a[p] * theta[col(p)] over the CSC entries of A (one thread per column).
- * Replaces the Theta factor inside compute_M_numerics_kernel (src/cuda/kkt.cu:177-215). */
-int lkd_assemble_prep(int32_t n, const int32_t* colptr, const double* aval, const double* theta,
-                      double* w, void* stream);
-/* Diagonal of M0 per permuted row from its pair-map segment (same fixed order as
+/* The pair map: every product a_ik * a_jk (i >= j, permuted rows, k a column of A) that adds
+ * to the lower triangle of M0 = A Theta A^T, grouped by destination entry (segments, in panel
+ * order).  Pair q stores the static product pp[q] = a_ik * a_jk and its column pk[q] = k, so an
+ * entry is  M0_ij = sum_q pp[q] * theta[pk[q]]  and theta is the only value gathered per pair.
+ * This replaces the Theta factor and row-pair loop of compute_M_numerics_kernel
+ * (src/cuda/kkt.cu:177-215). */
+
+/* A segment is summed in a fixed order that depends only on its length L:
+ *   L <= LKD_SHORT_SEG: one thread, acc = fma(pp[q], theta[pk[q]], acc) over q in order;
+ *   L <= LKD_LONG_SEG:  16 lanes, lane l sums q = beg+l, beg+l+16, ... in order, then a fixed
+ *                       xor butterfly over the 16 lanes;
+ *   longer:             a 256-thread block, thread-strided fma partials, then a fixed tree.
+ * The mid and long classes are listed once at set-up (segment indices for the panel pass, row
+ * indices for the diagonal pass). */
+#define LKD_SHORT_SEG 4
+
+/* Diagonal of M0 per permuted row from its pair-map segment (the same fixed order as
  * lkd_assemble_pairs, so the two agree bitwise): dseg[i] = segment of (i,i) or -1 (empty row);
- * long_diag lists the rows whose diagonal segment has more than LKD_LONG_SEG pairs.
+ * mid_diag / long_diag list the rows whose diagonal segment is in the mid / long class.
  * D_i = 1/sqrt(max(|M0_ii|, 1e-12)) when equilibrate != 0, else 1 (src/cuda/kkt.cu:65-79). */
-int lkd_assemble_diag(int32_t m, const int64_t* dseg, const int64_t* segptr, const int32_t* pa,
-                      const int32_t* pb, const double* w, const double* aval, int equilibrate,
-                      double* D, double* m0diag, lkd_long long_diag, void* stream);
-/* Segmented, fixed-order sums over the pair map: for segment t,
- *   acc = sum_{q in [segptr[t], segptr[t+1])} fma(w[pa[q]], aval[pb[q]], acc)
- *   panel[segdst[t]] = acc * D[segi[t]] * D[segj[t]] (+ delta on the diagonal of non-empty rows)
- * sequentially for segments of up to LKD_LONG_SEG pairs; the segments in long_segs (all longer
- * ones) are summed by a block: per-thread strided fma partials, then a fixed tree.
+int lkd_assemble_diag(int32_t m, const int64_t* dseg, const int64_t* segptr, const double* pp,
+                      const int32_t* pk, const double* theta, int equilibrate, double* D,
+                      double* m0diag, lkd_long mid_diag, lkd_long long_diag, void* stream);
+/* The whole panel (panel_elems entries, precision prec) from the M0 entries of the lower
+ * triangle:
+ *   mval[t] = M0(segment t) * D[segi[t]] * D[segj[t]] (+ delta on the diagonal of non-empty
+ *             rows), each segment summed as above (mid_segs / long_segs list the segments of the
+ *             mid / long class; mval: nseg device doubles of scratch);
+ *   then panel chunk c (entries [c, c+1) * LKD_PANEL_CHUNK) is built in LDS, zeros plus the
+ *   entries of segments chunk_seg[c] .. chunk_seg[c+1]-1 at segdst[t], and stored whole.
+ * Writing each chunk once replaces a separate zero fill and scattered writes into HBM.
  * Replaces compute_M_numerics_kernel + build_equilibrated_M_kernel / add_regularization_kernel
- * (src/cuda/kkt.cu:81-94,157-171,177-215).  The panel must be zeroed beforehand. */
+ * (src/cuda/kkt.cu:81-94,157-171,177-215). */
+#define LKD_PANEL_CHUNK 4096
 int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int64_t* segdst,
-                       const int32_t* segi, const int32_t* segj, const int32_t* pa,
-                       const int32_t* pb, const double* w, const double* aval, const double* D,
-                       const uint8_t* empty_row, double delta, void* panel, lkd_long long_segs,
-                       void* stream);
+                       const int32_t* segi, const int32_t* segj, const double* pp,
+                       const int32_t* pk, const double* theta, const double* D,
+                       const uint8_t* empty_row, double delta, lkd_long mid_segs,
+                       lkd_long long_segs, double* mval, int64_t panel_elems,
+                       const int64_t* chunk_seg, void* panel, void* stream);
 
 /* ---------------------------------------------------------------- supernodal Cholesky */
 /* Replaces cholesky_level_kernel and its level loop (src/cuda/kkt.cu:217-285,582-596).

@@ -82,7 +82,7 @@ int lkd_graph_end(void* stream, void** exec) {
     if (e) return e;
     hipGraphExec_t x;
     e = (int)hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     *exec = x;
     return e;
 }

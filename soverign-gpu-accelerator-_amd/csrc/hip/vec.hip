@@ -89,103 +89,161 @@ __device__ __forceinline__ void for_rows(int64_t m, const int32_t* __restrict__ 
 }
 
 // ------------------------------------------------------------------------------------------
-// Assembly
-__global__ __launch_bounds__(NT) void k_prep(int32_t n, const int32_t* __restrict__ colptr,
-                                             const double* __restrict__ aval,
-                                             const double* __restrict__ theta,
-                                             double* __restrict__ w) {
-    for (int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x; j < n; j += (int64_t)gridDim.x * NT) {
-        const double t = theta[j];
-        for (int32_t p = colptr[j]; p < colptr[j + 1]; ++p) w[p] = aval[p] * t;
-    }
-}
+// Assembly: M0_ij = sum over the pairs q of entry (i,j) of pp[q] * theta[pk[q]], each segment
+// summed in the fixed order of its length class (short: one thread; mid: 16 lanes; long: one
+// block).  The diagonal pass and the panel pass run the same code, so they agree bitwise.
+constexpr int SEG_LANES = 16;
 
-__device__ __forceinline__ double seg_sum(const int64_t* __restrict__ segptr,
-                                          const int32_t* __restrict__ pa,
-                                          const int32_t* __restrict__ pb,
-                                          const double* __restrict__ w,
-                                          const double* __restrict__ aval, int64_t t) {
+__device__ __forceinline__ double seg_short(int64_t b, int64_t e, const double* __restrict__ pp,
+                                            const int32_t* __restrict__ pk,
+                                            const double* __restrict__ theta) {
     double acc = 0.0;
-    for (int64_t q = segptr[t]; q < segptr[t + 1]; ++q) acc = fma(w[pa[q]], aval[pb[q]], acc);
+#pragma unroll
+    for (int u = 0; u < LKD_SHORT_SEG; ++u) {       // clamped loads, masked product: no branch
+        const int64_t q = b + u < e ? b + u : e - 1;
+        const double v = b + u < e ? pp[q] : 0.0;
+        acc = fma(v, theta[pk[q]], acc);
+    }
     return acc;
 }
 
-// Long segments (more than LKD_LONG_SEG pairs): thread-stride fma partials, then the fixed tree
-// of block_reduce; the result is valid in thread 0.  All threads of the block must call it.
-__device__ __forceinline__ double seg_sum_block(const int64_t* __restrict__ segptr,
-                                                const int32_t* __restrict__ pa,
-                                                const int32_t* __restrict__ pb,
-                                                const double* __restrict__ w,
-                                                const double* __restrict__ aval, int64_t t,
-                                                double* sh) {
+// Valid in every lane of the 16-lane group; all 16 lanes must call it.
+__device__ __forceinline__ double seg_mid(int64_t b, int64_t e, int l,
+                                          const double* __restrict__ pp,
+                                          const int32_t* __restrict__ pk,
+                                          const double* __restrict__ theta) {
     double acc = 0.0;
-    for (int64_t q = segptr[t] + threadIdx.x; q < segptr[t + 1]; q += NT)
-        acc = fma(w[pa[q]], aval[pb[q]], acc);
+#pragma unroll
+    for (int u = 0; u < LKD_LONG_SEG / SEG_LANES; ++u) {
+        const int64_t q0 = b + l + u * SEG_LANES;
+        const int64_t q = q0 < e ? q0 : e - 1;
+        const double v = q0 < e ? pp[q] : 0.0;
+        acc = fma(v, theta[pk[q]], acc);
+    }
+    return group_sum<SEG_LANES>(acc);
+}
+
+// Result valid in thread 0; all threads of the block must call it.
+__device__ __forceinline__ double seg_block(int64_t b, int64_t e, const double* __restrict__ pp,
+                                            const int32_t* __restrict__ pk,
+                                            const double* __restrict__ theta, double* sh) {
+    double acc = 0.0;
+    for (int64_t q = b + threadIdx.x; q < e; q += NT) acc = fma(pp[q], theta[pk[q]], acc);
     return block_reduce(acc, OP_SUM, sh);
 }
 
-__device__ __forceinline__ bool seg_long(const int64_t* __restrict__ segptr, int64_t t) {
-    return segptr[t + 1] - segptr[t] > LKD_LONG_SEG;
+struct SegOut {              // where a segment's sum goes
+    // diagonal pass (D != nullptr and panel == nullptr)
+    int equil;
+    double* D;
+    double* m0d;
+    // entry pass
+    const int32_t* segi;
+    const int32_t* segj;
+    const double* Dp;
+    const uint8_t* empty;
+    double delta;
+    double* mval;
+};
+
+template <bool DIAG>
+__device__ __forceinline__ void seg_store(const SegOut& o, int64_t r, double acc) {
+    if constexpr (DIAG) {                   // r: row
+        if (o.m0d) o.m0d[r] = acc;
+        double a = fabs(acc);
+        if (a < 1e-12) a = 1e-12;
+        o.D[r] = o.equil ? 1.0 / sqrt(a) : 1.0;
+    } else {                                // r: segment
+        const int32_t i = o.segi[r], j = o.segj[r];
+        double v = acc * o.Dp[i] * o.Dp[j];
+        if (i == j && !o.empty[i]) v += o.delta;
+        o.mval[r] = v;
+    }
 }
 
-__device__ __forceinline__ void store_diag(int64_t i, double mii, int equil, double* D,
-                                           double* m0d) {
-    if (m0d) m0d[i] = mii;
-    double a = fabs(mii);
-    if (a < 1e-12) a = 1e-12;
-    D[i] = equil ? 1.0 / sqrt(a) : 1.0;
-}
-
-// ld lists the rows whose diagonal segment is long; blocks take those first.
-__global__ __launch_bounds__(NT) void k_diag(int32_t m, const int64_t* __restrict__ dseg,
+// Blocks [0, nbl) take the long list, [nbl, nbl + nbm) the mid list (16 groups of 16 lanes per
+// block), the rest every item of the short class.  Items are rows (DIAG, segment dseg[r]) or
+// segments.
+template <bool DIAG>
+__global__ __launch_bounds__(NT) void k_segs(int64_t count, const int64_t* __restrict__ dseg,
                                              const int64_t* __restrict__ segptr,
-                                             const int32_t* __restrict__ pa,
-                                             const int32_t* __restrict__ pb,
-                                             const double* __restrict__ w,
-                                             const double* __restrict__ aval, int equil,
-                                             double* __restrict__ D, double* __restrict__ m0d,
-                                             lkd_long ld) {
+                                             const double* __restrict__ pp,
+                                             const int32_t* __restrict__ pk,
+                                             const double* __restrict__ theta, lkd_long mid,
+                                             lkd_long lng, int nbl, int nbm, SegOut o) {
     __shared__ double sh[4];
-    for (int32_t k = blockIdx.x; k < ld.n; k += gridDim.x) {
-        const int32_t i = ld.idx[k];
-        const double mii = seg_sum_block(segptr, pa, pb, w, aval, dseg[i], sh);
-        if (threadIdx.x == 0) store_diag(i, mii, equil, D, m0d);
+    auto seg_of = [&](int64_t r) -> int64_t { return DIAG ? dseg[r] : r; };
+    const int bx = blockIdx.x;
+    if (bx < nbl) {
+        for (int32_t k = bx; k < lng.n; k += nbl) {
+            const int64_t r = lng.idx[k], t = seg_of(r);
+            const double acc = seg_block(segptr[t], segptr[t + 1], pp, pk, theta, sh);
+            if (threadIdx.x == 0) seg_store<DIAG>(o, r, acc);
+        }
+        return;
     }
-    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < m; i += (int64_t)gridDim.x * NT) {
-        const int64_t t = dseg[i];
-        if (t >= 0 && seg_long(segptr, t)) continue;
-        store_diag(i, t >= 0 ? seg_sum(segptr, pa, pb, w, aval, t) : 0.0, equil, D, m0d);
+    if (bx < nbl + nbm) {
+        const int l = threadIdx.x % SEG_LANES;
+        for (int32_t k = (bx - nbl) * (NT / SEG_LANES) + threadIdx.x / SEG_LANES; k < mid.n;
+             k += nbm * (NT / SEG_LANES)) {
+            const int64_t r = mid.idx[k], t = seg_of(r);
+            const double acc = seg_mid(segptr[t], segptr[t + 1], l, pp, pk, theta);
+            if (l == 0) seg_store<DIAG>(o, r, acc);
+        }
+        return;
+    }
+    const int64_t nbs = gridDim.x - nbl - nbm;
+    for (int64_t r = (bx - nbl - nbm) * (int64_t)NT + threadIdx.x; r < count; r += nbs * NT) {
+        const int64_t t = seg_of(r);
+        if (t < 0) {                        // empty row (DIAG only)
+            seg_store<DIAG>(o, r, 0.0);
+            continue;
+        }
+        const int64_t b = segptr[t], e = segptr[t + 1];
+        if (e - b > LKD_SHORT_SEG) continue;
+        seg_store<DIAG>(o, r, seg_short(b, e, pp, pk, theta));
     }
 }
 
+template <bool DIAG>
+void launch_segs(int64_t count, const int64_t* dseg, const int64_t* segptr, const double* pp,
+                const int32_t* pk, const double* theta, lkd_long mid, lkd_long lng,
+                const SegOut& o, void* stream) {
+    const int nbl = lng.n < 1024 ? lng.n : 1024;
+    const int64_t gm = (mid.n + NT / SEG_LANES - 1) / (NT / SEG_LANES);
+    const int nbm = (int)(gm < 4096 ? gm : 4096);
+    int64_t gs = (count + NT - 1) / NT;
+    gs = gs < 1 ? 1 : (gs > 8192 ? 8192 : gs);
+    LK_LAUNCH(k_segs<DIAG>, (int)(nbl + nbm + gs), NT, 0, stream, count, dseg, segptr, pp, pk,
+              theta, mid, lng, nbl, nbm, o);
+}
+
+// One block per panel chunk: zeros and the chunk's entries in LDS, then whole-line stores, so
+// every HBM line of the panel is written once and in full.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_pairs(int64_t nseg, const int64_t* __restrict__ segptr,
-                                              const int64_t* __restrict__ segdst,
-                                              const int32_t* __restrict__ segi,
-                                              const int32_t* __restrict__ segj,
-                                              const int32_t* __restrict__ pa,
-                                              const int32_t* __restrict__ pb,
-                                              const double* __restrict__ w,
-                                              const double* __restrict__ aval,
-                                              const double* __restrict__ D,
-                                              const uint8_t* __restrict__ empty, double delta,
-                                              T* __restrict__ panel, lkd_long ls) {
-    __shared__ double sh[4];
-    auto store = [&](int64_t t, double acc) {
-        const int32_t i = segi[t], j = segj[t];
-        double v = acc * D[i] * D[j];
-        if (i == j && !empty[i]) v += delta;
-        panel[segdst[t]] = (T)v;
-    };
-    for (int32_t k = blockIdx.x; k < ls.n; k += gridDim.x) {
-        const int32_t t = ls.idx[k];
-        const double acc = seg_sum_block(segptr, pa, pb, w, aval, t, sh);
-        if (threadIdx.x == 0) store(t, acc);
-    }
-    for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < nseg;
-         t += (int64_t)gridDim.x * NT) {
-        if (seg_long(segptr, t)) continue;
-        store(t, seg_sum(segptr, pa, pb, w, aval, t));
+__global__ __launch_bounds__(NT) void k_panel_chunks(int64_t panel_elems,
+                                                     const int64_t* __restrict__ chunk_seg,
+                                                     const int64_t* __restrict__ segdst,
+                                                     const double* __restrict__ mval,
+                                                     T* __restrict__ panel) {
+    constexpr int CH = LKD_PANEL_CHUNK;
+    constexpr int V = 16 / sizeof(T);                 // elements per 16-byte store
+    typedef T vec __attribute__((ext_vector_type(V)));
+    __shared__ __align__(16) T buf[CH];
+    const int64_t c = blockIdx.x, base = c * CH;
+    vec* bv = reinterpret_cast<vec*>(buf);
+    for (int e = threadIdx.x; e < CH / V; e += NT) bv[e] = vec{};
+    __syncthreads();
+    const int64_t t1 = chunk_seg[c + 1];
+    for (int64_t t = chunk_seg[c] + threadIdx.x; t < t1; t += NT)
+        buf[segdst[t] - base] = (T)mval[t];
+    __syncthreads();
+    const int64_t rem = panel_elems - base;
+    if (rem >= CH) {
+        vec* pv = reinterpret_cast<vec*>(panel + base);
+        for (int e = threadIdx.x; e < CH / V; e += NT) pv[e] = bv[e];
+    } else {
+        for (int e = threadIdx.x; e < rem; e += NT) panel[base + e] = buf[e];
     }
 }
 
@@ -634,32 +692,42 @@ template <int G> void l_pcg_rows(int32_t m, const int32_t* prp, const int32_t* p
 
 extern "C" {
 
-int lkd_assemble_prep(int32_t n, const int32_t* colptr, const double* aval, const double* theta,
-                      double* w, void* stream) {
-    LK_LAUNCH(k_prep, grid_for(n), NT, 0, stream, n, colptr, aval, theta, w);
-    return lk_last_error();
-}
-
-int lkd_assemble_diag(int32_t m, const int64_t* dseg, const int64_t* segptr, const int32_t* pa,
-                      const int32_t* pb, const double* w, const double* aval, int equilibrate,
-                      double* D, double* m0diag, lkd_long long_diag, void* stream) {
-    LK_LAUNCH(k_diag, grid_for(m), NT, 0, stream, m, dseg, segptr, pa, pb, w, aval, equilibrate,
-              D, m0diag, long_diag);
+int lkd_assemble_diag(int32_t m, const int64_t* dseg, const int64_t* segptr, const double* pp,
+                      const int32_t* pk, const double* theta, int equilibrate, double* D,
+                      double* m0diag, lkd_long mid_diag, lkd_long long_diag, void* stream) {
+    if (m <= 0) return 0;
+    SegOut o{};
+    o.equil = equilibrate;
+    o.D = D;
+    o.m0d = m0diag;
+    launch_segs<true>(m, dseg, segptr, pp, pk, theta, mid_diag, long_diag, o, stream);
     return lk_last_error();
 }
 
 int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int64_t* segdst,
-                       const int32_t* segi, const int32_t* segj, const int32_t* pa,
-                       const int32_t* pb, const double* w, const double* aval, const double* D,
-                       const uint8_t* empty_row, double delta, void* panel, lkd_long long_segs,
-                       void* stream) {
-    if (nseg <= 0) return 0;
+                       const int32_t* segi, const int32_t* segj, const double* pp,
+                       const int32_t* pk, const double* theta, const double* D,
+                       const uint8_t* empty_row, double delta, lkd_long mid_segs,
+                       lkd_long long_segs, double* mval, int64_t panel_elems,
+                       const int64_t* chunk_seg, void* panel, void* stream) {
+    if (nseg > 0) {
+        SegOut o{};
+        o.segi = segi;
+        o.segj = segj;
+        o.Dp = D;
+        o.empty = empty_row;
+        o.delta = delta;
+        o.mval = mval;
+        launch_segs<false>(nseg, nullptr, segptr, pp, pk, theta, mid_segs, long_segs, o, stream);
+    }
+    const int64_t nchunk = (panel_elems + LKD_PANEL_CHUNK - 1) / LKD_PANEL_CHUNK;
+    if (nchunk >= (int64_t(1) << 31)) return (int)hipErrorInvalidValue;
     if (prec == LKD_F32)
-        LK_LAUNCH(k_pairs<float>, grid_for(nseg, 16384), NT, 0, stream, nseg, segptr, segdst, segi,
-                  segj, pa, pb, w, aval, D, empty_row, delta, (float*)panel, long_segs);
+        LK_LAUNCH(k_panel_chunks<float>, (int)nchunk, NT, 0, stream, panel_elems, chunk_seg,
+                  segdst, mval, (float*)panel);
     else
-        LK_LAUNCH(k_pairs<double>, grid_for(nseg, 16384), NT, 0, stream, nseg, segptr, segdst, segi,
-                  segj, pa, pb, w, aval, D, empty_row, delta, (double*)panel, long_segs);
+        LK_LAUNCH(k_panel_chunks<double>, (int)nchunk, NT, 0, stream, panel_elems, chunk_seg,
+                  segdst, mval, (double*)panel);
     return lk_last_error();
 }
 

@@ -109,10 +109,14 @@ struct GPUKKTCholeskySolver::Impl {
             *d_pci = nullptr;
     double *d_cval = nullptr, *d_pval = nullptr;
     int64_t *d_segptr = nullptr, *d_segdst = nullptr, *d_dseg = nullptr;
-    int32_t *d_segi = nullptr, *d_segj = nullptr, *d_pa = nullptr, *d_pb = nullptr;
+    int32_t *d_segi = nullptr, *d_segj = nullptr, *d_pk = nullptr;
+    double* d_pp = nullptr;
     uint8_t* d_empty = nullptr;
     // rows / columns / pair-map segments handled by whole blocks (LKD_LONG_ROW / LKD_LONG_SEG)
     lkd_long long_prows{0, nullptr}, long_cols{0, nullptr}, long_diag{0, nullptr}, long_segs{0, nullptr};
+    lkd_long mid_diag{0, nullptr}, mid_segs{0, nullptr};   // 16-lane class of the pair map
+    int64_t* d_chunk_seg = nullptr;   // first segment of each LKD_PANEL_CHUNK panel chunk
+    double* d_mval = nullptr;         // per-segment M0 entries (assembly scratch)
     int32_t *d_sn_first = nullptr, *d_sn_rows = nullptr, *d_col_to_sn = nullptr;
     int64_t *d_sn_rowptr = nullptr, *d_sn_panel = nullptr;
     lkd_task *d_ftasks = nullptr, *d_stasks = nullptr;
@@ -132,7 +136,7 @@ struct GPUKKTCholeskySolver::Impl {
     double* d_pbuf = nullptr;      // level solves: z exchange / backward partial sums
     uint32_t* d_flags = nullptr;   // level solves: hand-off flags, then the epoch
     int32_t* d_err = nullptr;
-    double *d_w = nullptr, *d_D = nullptr, *d_m0d = nullptr, *d_theta = nullptr;
+    double *d_D = nullptr, *d_m0d = nullptr, *d_theta = nullptr;
     double *d_r = nullptr, *d_req = nullptr, *d_z = nullptr, *d_dy = nullptr, *d_bak = nullptr,
            *d_rhsp = nullptr,
            *d_u = nullptr, *d_rinf = nullptr, *d_csr = nullptr;
@@ -253,7 +257,7 @@ struct GPUKKTCholeskySolver::Impl {
             std::copy(A.values.begin() + A.row_ptrs[i], A.values.begin() + A.row_ptrs[i + 1],
                       pval.begin() + prp[k]);
         }
-        h_colptr = colptr; h_prow = prow; h_cval_n = nz;
+        h_colptr = colptr; h_prow = prow; h_cval = cval;
         long_prows = long_list(prp, m, LKD_LONG_ROW, "long_prows");
         long_cols = long_list(colptr, n, LKD_LONG_ROW, "long_cols");
         d_colptr = up(colptr, "colptr");
@@ -266,7 +270,7 @@ struct GPUKKTCholeskySolver::Impl {
         d_empty = up(S.empty_row, "empty_row");
     }
     std::vector<Index> h_colptr, h_prow;
-    int64_t h_cval_n = 0;
+    std::vector<double> h_cval;
 
     // indices k < count with ptr[k+1] - ptr[k] > thr, uploaded
     template <typename P>
@@ -289,7 +293,7 @@ struct GPUKKTCholeskySolver::Impl {
     }
 
     void build_pairs() {
-        struct Pair { int64_t dst; int32_t a, b; };
+        struct Pair { int64_t dst; int32_t a, b, k; };
         std::vector<Pair> pairs;
         int64_t total = 0;
         for (Index k = 0; k < n; ++k) {
@@ -302,13 +306,14 @@ struct GPUKKTCholeskySolver::Impl {
                 for (Index b = h_colptr[k]; b < h_colptr[k + 1]; ++b) {
                     const Index i = h_prow[a], j = h_prow[b];
                     if (i < j) continue;
-                    pairs.push_back({panel_index(i, j), a, b});
+                    pairs.push_back({panel_index(i, j), a, b, k});
                 }
         std::stable_sort(pairs.begin(), pairs.end(),
                          [](const Pair& x, const Pair& y) { return x.dst < y.dst; });
         npairs = static_cast<int64_t>(pairs.size());
         std::vector<int64_t> segptr, segdst;
-        std::vector<int32_t> segi, segj, pa(pairs.size()), pb(pairs.size());
+        std::vector<int32_t> segi, segj, pk(pairs.size());
+        std::vector<double> pp(pairs.size());
         std::vector<int64_t> dseg(static_cast<size_t>(m), -1);
         for (size_t q = 0; q < pairs.size(); ++q) {
             if (q == 0 || pairs[q].dst != pairs[q - 1].dst) {
@@ -319,8 +324,8 @@ struct GPUKKTCholeskySolver::Impl {
                 segj.push_back(j);
                 if (i == j) dseg[i] = static_cast<int64_t>(segdst.size()) - 1;
             }
-            pa[q] = pairs[q].a;
-            pb[q] = pairs[q].b;
+            pp[q] = h_cval[pairs[q].a] * h_cval[pairs[q].b];
+            pk[q] = pairs[q].k;
         }
         segptr.push_back(static_cast<int64_t>(pairs.size()));
         nseg = static_cast<int64_t>(segdst.size());
@@ -329,17 +334,36 @@ struct GPUKKTCholeskySolver::Impl {
         if (nseg >= (int64_t(1) << 31)) throw std::length_error("KKT plan: more than 2^31 M0 entries");
         long_segs = long_list(segptr, nseg, LKD_LONG_SEG, "long_segs");
         {
-            std::vector<int32_t> ld;
-            for (Index i = 0; i < m; ++i)
-                if (dseg[i] >= 0 && segptr[dseg[i] + 1] - segptr[dseg[i]] > LKD_LONG_SEG) ld.push_back(i);
+            std::vector<int32_t> ms;
+            for (int64_t t = 0; t < nseg; ++t) {
+                const int64_t len = segptr[t + 1] - segptr[t];
+                if (len > LKD_SHORT_SEG && len <= LKD_LONG_SEG) ms.push_back(static_cast<int32_t>(t));
+            }
+            mid_segs = {static_cast<int32_t>(ms.size()), up(ms, "mid_segs")};
+            std::vector<int32_t> ld, md;
+            for (Index i = 0; i < m; ++i) {
+                if (dseg[i] < 0) continue;
+                const int64_t len = segptr[dseg[i] + 1] - segptr[dseg[i]];
+                if (len > LKD_LONG_SEG) ld.push_back(i);
+                else if (len > LKD_SHORT_SEG) md.push_back(i);
+            }
             long_diag = {static_cast<int32_t>(ld.size()), up(ld, "long_diag")};
+            mid_diag = {static_cast<int32_t>(md.size()), up(md, "mid_diag")};
+        }
+        {
+            const int64_t pe = S.sn_panel[S.nsuper];
+            const int64_t nchunk = (pe + LKD_PANEL_CHUNK - 1) / LKD_PANEL_CHUNK;
+            std::vector<int64_t> cs(static_cast<size_t>(nchunk) + 1);
+            for (int64_t c = 0; c <= nchunk; ++c)
+                cs[c] = std::lower_bound(segdst.begin(), segdst.end(), c * LKD_PANEL_CHUNK) - segdst.begin();
+            d_chunk_seg = up(cs, "chunk_seg");
         }
         d_segptr = up(segptr, "segptr");
         d_segdst = up(segdst, "segdst");
         d_segi = up(segi, "segi");
         d_segj = up(segj, "segj");
-        d_pa = up(pa, "pa");
-        d_pb = up(pb, "pb");
+        d_pp = up(pp, "pp");
+        d_pk = up(pk, "pk");
         d_dseg = up(dseg, "dseg");
     }
 
@@ -741,9 +765,9 @@ struct GPUKKTCholeskySolver::Impl {
         };
         d_err = static_cast<int32_t*>(arena_p->allocate(sizeof(int32_t) * 4));
         owned.push_back(d_err);
-        d_w = dv(std::max<int64_t>(h_cval_n, 1));
         d_D = dv(m);
         d_m0d = dv(m);
+        d_mval = dv(nseg);
         d_theta = dv(n);
         d_r = dv(m);
         d_req = dv(m);
@@ -894,15 +918,14 @@ struct GPUKKTCholeskySolver::Impl {
         check(lkd_event_create(&e1), "event");
         check(lkd_event_create(&a0), "event");
         check(lkd_event_create(&a1), "event");
-        const size_t esz = prec == LKD_F32 ? sizeof(float) : sizeof(double);
         check(lkd_event_record(a0, stream), "event");
         check(lkd_event_record(e0, stream), "event");
-        check(lkd_assemble_prep(n, d_colptr, d_cval, d_theta, d_w, stream), "prep");
-        check(lkd_assemble_diag(m, d_dseg, d_segptr, d_pa, d_pb, d_w, d_cval, 1, d_D, d_m0d, long_diag, stream), "diag");
-        check(lkd_memset(d_panel, 0, static_cast<size_t>(S.sn_panel[S.nsuper]) * esz, stream), "zero");
+        check(lkd_assemble_diag(m, d_dseg, d_segptr, d_pp, d_pk, d_theta, 1, d_D, d_m0d, mid_diag, long_diag,
+                                stream), "diag");
         check(lkd_memset(d_err, 0, sizeof(int32_t), stream), "zero");
-        check(lkd_assemble_pairs(prec, nseg, d_segptr, d_segdst, d_segi, d_segj, d_pa, d_pb, d_w,
-                                 d_cval, d_D, d_empty, 1e-9, d_panel, long_segs, stream), "assemble");
+        check(lkd_assemble_pairs(prec, nseg, d_segptr, d_segdst, d_segi, d_segj, d_pp, d_pk, d_theta,
+                                 d_D, d_empty, 1e-9, mid_segs, long_segs, d_mval, S.sn_panel[S.nsuper],
+                                 d_chunk_seg, d_panel, stream), "assemble");
         check(lkd_event_record(e1, stream), "event");
         {
             float t_ms = 0;
@@ -1011,16 +1034,14 @@ struct GPUKKTCholeskySolver::Impl {
     // assemble (D, delta) and factor; returns true when every pivot was positive
     bool assemble_and_factor(const double* theta, bool equil, double delta, int prec,
                              bool need_diag) {
-        const size_t esz = prec == LKD_F32 ? sizeof(float) : sizeof(double);
         if (need_diag) {
-            check(lkd_assemble_prep(n, d_colptr, d_cval, theta, d_w, stream), "prep");
-            check(lkd_assemble_diag(m, d_dseg, d_segptr, d_pa, d_pb, d_w, d_cval, equil ? 1 : 0,
-                                    d_D, d_m0d, long_diag, stream), "diag");
+            check(lkd_assemble_diag(m, d_dseg, d_segptr, d_pp, d_pk, theta, equil ? 1 : 0, d_D, d_m0d,
+                                    mid_diag, long_diag, stream), "diag");
         }
-        check(lkd_memset(d_panel, 0, static_cast<size_t>(S.sn_panel[S.nsuper]) * esz, stream), "panel zero");
         check(lkd_memset(d_err, 0, sizeof(int32_t), stream), "err zero");
-        check(lkd_assemble_pairs(prec, nseg, d_segptr, d_segdst, d_segi, d_segj, d_pa, d_pb, d_w,
-                                 d_cval, d_D, d_empty, delta, d_panel, long_segs, stream), "assemble");
+        check(lkd_assemble_pairs(prec, nseg, d_segptr, d_segdst, d_segi, d_segj, d_pp, d_pk, theta,
+                                 d_D, d_empty, delta, mid_segs, long_segs, d_mval, S.sn_panel[S.nsuper],
+                                 d_chunk_seg, d_panel, stream), "assemble");
         factor_numeric(prec);
         int32_t err = 0;
         check(lkd_memcpy_dtoh(&err, d_err, sizeof(int32_t), stream), "err read");
@@ -1236,7 +1257,7 @@ std::size_t GPUKKTCholeskySolver::estimate_bytes(const core::CSRMatrix& A,
         const int64_t bc = (nc + 63) / 64, br = (ld + 63) / 64;
         tasks += 4 + bc * (br + 4) + bc * br * (br + 1) / 2 + br * br;
     }
-    const int64_t bytes = 8 * S.sn_panel[S.nsuper] + 8 * diag + 8 * pairs + 36 * pairs +
+    const int64_t bytes = 8 * S.sn_panel[S.nsuper] + 8 * diag + 20 * pairs + 36 * pairs +
                           2 * (16 * nz + 8 * (m + n)) + 8 * nz + 8 * 24 * (m + n) +
                           48 * 2 * tasks + 8 * 4096 + static_cast<int64_t>(S.sn_rows.size()) * 4 +
                           24 * static_cast<int64_t>(S.nsuper) + 8 * m + 256 * 96;

@@ -398,10 +398,12 @@ PLAN_DTYPES = {"perm": np.int32, "sn_rowptr": np.int64, "sn_rows": np.int32,
                "flaunch": np.int64, "slaunch": np.int64, "diag_elems": np.int64,
                "colptr": np.int32, "prow": np.int32, "cval": np.float64, "prp": np.int32,
                "pci": np.int32, "pval": np.float64, "empty_row": np.uint8, "segptr": np.int64,
-               "segdst": np.int64, "segi": np.int32, "segj": np.int32, "pa": np.int32,
-               "pb": np.int32, "dseg": np.int64, "ftasks": TASK_DTYPE, "stasks": TASK_DTYPE,
+               "segdst": np.int64, "segi": np.int32, "segj": np.int32, "pp": np.float64,
+               "pk": np.int32, "dseg": np.int64, "ftasks": TASK_DTYPE, "stasks": TASK_DTYPE,
                "csr_index": np.int64, "stage_elems": np.int64, "long_prows": np.int32,
-               "long_cols": np.int32, "long_segs": np.int32, "long_diag": np.int32}
+               "long_cols": np.int32, "long_segs": np.int32, "long_diag": np.int32,
+               "mid_segs": np.int32, "mid_diag": np.int32,
+               "chunk_seg": np.int64}
 
 
 def plan_export(rp, ci, val, n, P=None, relax_small=-1, relax_frac=-1.0):

@@ -27,8 +27,7 @@ class Emulator:
 
     def assemble(self, theta, equil=False, delta=1e-9):
         p = self.p
-        w = p["cval"] * np.repeat(theta, np.diff(p["colptr"]))
-        prod = w[p["pa"]] * p["cval"][p["pb"]]
+        prod = p["pp"] * theta[p["pk"]]
         seg = np.add.reduceat(prod, p["segptr"][:-1]) if len(prod) else np.zeros(0)
         m = len(p["perm"])
         D = np.ones(m)

@@ -52,3 +52,26 @@ def test_schedule_reproduces_dense_cholesky(native, name):
     got = em.solve(x.copy())
     exp = np.linalg.solve(L @ L.T, x)
     assert np.abs(got - exp).max() <= 1e-9 * max(1.0, np.abs(exp).max())
+
+
+@pytest.mark.parametrize("name", ["block_angular", "dense"])
+def test_pair_map_length_classes(native, name):
+    """Every M0 segment is summed by exactly one class: short (<= 4 pairs, one thread), mid
+    (<= 64, listed, 16 lanes) or long (listed, one block); the diagonal lists follow dseg."""
+    lp = _cases()[name]()
+    rp, ci, va = lp.csr()
+    plan = native.plan_export(rp, ci, va, lp.n, P=native.order_amd(rp, ci, lp.n))
+    L = np.diff(plan["segptr"])
+    assert L.min() >= 1 and plan["pp"].shape == plan["pk"].shape == (plan["segptr"][-1],)
+    np.testing.assert_array_equal(plan["mid_segs"], np.flatnonzero((L > 4) & (L <= 64)))
+    np.testing.assert_array_equal(plan["long_segs"], np.flatnonzero(L > 64))
+    dseg = plan["dseg"]
+    Ld = np.where(dseg >= 0, L[np.maximum(dseg, 0)], 0)
+    np.testing.assert_array_equal(plan["mid_diag"], np.flatnonzero((Ld > 4) & (Ld <= 64)))
+    np.testing.assert_array_equal(plan["long_diag"], np.flatnonzero(Ld > 64))
+    cs = plan["chunk_seg"]                # panel chunks of 4096 entries and their segments
+    nchunk = -(-int(plan["sn_panel"][-1]) // 4096)
+    np.testing.assert_array_equal(cs, np.searchsorted(plan["segdst"], np.arange(nchunk + 1) * 4096))
+    assert cs[-1] == len(L) and np.all(np.diff(plan["segdst"]) > 0)
+    if name == "block_angular":           # the linking rows' diagonals are long
+        assert len(plan["mid_segs"]) and len(plan["long_segs"]) and len(plan["long_diag"])
