@@ -64,7 +64,8 @@ int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream);
  *   sn     supernode id, first = first (permuted) column of the supernode
  *   np/px  STEP: number of partial products (from the PART launch before it) added to the
  *          tile's own GEMM, stored at work + px + s * LKD_PART (s < np); PART: px = where its
- *          product goes                                                                    */
+ *          product goes
+ *   prio   STEP: 1 on tiles of the critical path (their wavefronts get issue priority)     */
 typedef struct {
     int64_t off;
     int64_t aux;     /* diagonal-copy offset (DIAG / STEP next block / DIAGWB / TRSV) */
@@ -77,6 +78,7 @@ typedef struct {
     int32_t nsplit, sn;
     int32_t first, np;
     int64_t px;
+    int32_t prio, pad;
 } lkd_task;
 
 typedef struct {
@@ -149,10 +151,13 @@ int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int6
  *           partial products) + solve against the staged factor; the tile holding the next
  *           diagonal block (nsplit = 1, c0 rows) also factors it (staging slot aux2, diag
  *           copy aux)
+ *   STEPS   (in place of the STEP / PART launches) every STEP tile of the level in one
+ *           launch: a tile runs once the tiles it depends on are done (see lkd_chol_step_dag)
  *   UPDATE  ancestors (split K over kbeg..kend)
  * and DIAGWB (diagonal blocks into the panel) after the last level.
  * err: device int, bit 0 set when a pivot is not positive, bit 1 when an update falls outside
- * the L structure; kernels return early once set.
+ * the L structure, bit 5 when a STEPS wait exceeded its poll limit; kernels return early once
+ * set.
  * diag: row-major copies of the factored diagonal blocks (element offsets), with the
  * reciprocal on the diagonal, read by TRSV.  piv: the pivots L_jj per permuted column.
  * stage: staged diagonal factors, LKD_STAGE elements per slot (all panel precision). */
@@ -164,6 +169,26 @@ int lkd_chol_part(int prec, int32_t ntask, const lkd_task* tasks, const void* pa
                   int32_t* err, void* stream);
 int lkd_chol_step(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
                   void* piv, void* stage, const void* work, int32_t* err, void* stream);
+/* STEPS: the STEP tiles of one level (np = 0).  deps: 4 task indices per task (-1 = none):
+ * the tiles of the previous block column whose rows overlap its rows, and the tile that
+ * factored its diagonal block (whose rows its GEMM reads); sdeps: that last tile (-1: block
+ * staged before the launch).  Every supernode's tasks belong to one of nx queues (one per XCD):
+ * queue x = queue[qoff[x] .. qoff[x+1]) (task indices, every task after its dependencies).
+ * grid workgroups (at least nx times the longest queue when workgroups go to the XCDs in turn)
+ * each read the XCD they run on (XCC_ID mod nx) and take tasks of its queue from counter
+ * ctl[1 + q nx + x] until it is empty.  A task waits until flags[dep] == ctl[0] for each
+ * dependency, runs its GEMM, waits until flags2[sdep] == ctl[0], solves, then sets
+ * flags[task] = ctl[0]; a tile that factors the next diagonal block sets flags[task] once its
+ * rows are stored and flags2[task] once the factor is staged.  Staging slots must not be
+ * shared by two block columns of the level.  lkd_chol_dag_begin(ctl, nq) starts a
+ * factorization: ctl[0] += 1 and the nq counters ctl[1 .. nq] = 0 (a kernel, so it can be
+ * captured in a graph). */
+int lkd_chol_step_dag(int prec, int32_t ntask, const lkd_task* tasks, const int32_t* deps,
+                      const int32_t* sdeps, const int32_t* queue, const int32_t* qoff,
+                      int32_t nx, int32_t grid, uint32_t* flags, uint32_t* flags2, uint32_t* ctl,
+                      int32_t q, void* panel, void* diag, void* piv, void* stage, int32_t* err,
+                      void* stream);
+int lkd_chol_dag_begin(uint32_t* ctl, int32_t nq, void* stream);
 int lkd_chol_update(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
                     lkd_snplan plan, int32_t* err, void* stream);
 int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel, const void* diag,

@@ -281,7 +281,9 @@ __device__ __forceinline__ bool potrf64(T* Ds, T* Lv, int* s_ok) {
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15;
     using R = Mfma<T>;
     __shared__ T cb[256 + 16];
-#pragma unroll
+    // not unrolled: one copy of the 16-column code keeps the kernels that inline this small
+    // enough for the instruction cache
+#pragma unroll 1
     for (int p = 0; p < 4; ++p) {
         const int o = 16 * p;
         if (w == 0) {
@@ -304,9 +306,9 @@ __device__ __forceinline__ bool potrf64(T* Ds, T* Lv, int* s_ok) {
         __syncthreads();
         // trailing blocks (i, jj), p < jj <= i <= 3, round-robin over the waves
         int b = 0;
-#pragma unroll
+#pragma unroll 1
         for (int i = p + 1; i < 4; ++i)
-#pragma unroll
+#pragma unroll 1
             for (int jj = p + 1; jj <= i; ++jj, ++b) {
                 if ((b & 3) != w) continue;
                 Acc<T> d = Acc<T>{0, 0, 0, 0};
@@ -420,17 +422,40 @@ void k_part(const lkd_task* __restrict__ tasks, int ntask, const T* __restrict__
             for (int e = 0; e < 4; ++e) out[((p * 2 + b) * 4 + e) * 256] = acc[p][b][e];
 }
 
+// Wait (one thread) until flag == ep; false on an error flag or after kSpinLimit polls (then
+// bit 0x20 of err is set).
+constexpr long long kSpinLimit = 1ll << 22;
+__device__ __forceinline__ bool flag_wait(const uint32_t* flag, uint32_t ep, int32_t* err) {
+    long long n = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != ep) {
+        if ((n & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return false;
+        if (++n >= kSpinLimit) {
+            atomicOr(err, 0x20);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+// Hand-offs of a tile run inside a STEPS launch (absent for the one-launch-per-step form):
+// the tile waits for flags2[sdep] (its diagonal block staged) only after its GEMM, and a tile
+// that factors the next diagonal block sets flags[ti] as soon as its own rows are stored (the
+// next step's GEMMs read them) and flags2[ti] once the factor is staged.
+struct DagSync {
+    uint32_t* flags;
+    uint32_t* flags2;
+    int sdep, ti;
+    uint32_t ep;
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T* __restrict__ diag,
-            T* __restrict__ piv, T* __restrict__ stage, const T* __restrict__ work, int32_t* err) {
-    const int ti = task_index();
-    if (ti >= ntask) return;
-    __shared__ int s_err;
-    if (wg_error(err, &s_err)) return;
-    const lkd_task tk = tasks[ti];
-    __shared__ GemmLds<T> sm;
-    __shared__ int s_ok;
+__device__ __forceinline__ void step_tile(const lkd_task& tk, T* __restrict__ L,
+                                          T* __restrict__ diag, T* __restrict__ piv,
+                                          T* __restrict__ stage, const T* __restrict__ work,
+                                          int32_t* err, GemmLds<T>& sm, int* s_ok,
+                                          const DagSync* ds = nullptr) {
     using R = Mfma<T>;
     T* P = L + tk.off;
     const int64_t ld = tk.ld;
@@ -439,7 +464,7 @@ void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T*
     const int ra = tk.c0;
     // the tile that factors the next diagonal block is the launch's critical path: its waves
     // get issue priority over the other workgroup sharing the compute unit
-    if (tk.nsplit) __builtin_amdgcn_s_setprio(3);
+    if (tk.nsplit || tk.prio) __builtin_amdgcn_s_setprio(3);
     Acc<T> acc[4][2];
     zero_acc<T, 4, 2>(acc);
     gemm_tn<T>(P + tk.r0, ld, ra, P + k0, ld, kb, tk.kbeg, tk.kend, sm, acc);
@@ -451,6 +476,12 @@ void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T*
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) acc[p][b][e] += in[((p * 2 + b) * 4 + e) * 256];
+    }
+    if (ds && ds->sdep >= 0) {                        // the staged factor of block k
+        if (t == 0) *s_ok = flag_wait(ds->flags2 + ds->sdep, ds->ep, err) ? 1 : 0;
+        __syncthreads();
+        if (!*s_ok) return;
+        asm volatile("buffer_inv sc0" ::: "memory");
     }
     T* S = reinterpret_cast<T*>(&sm);
     {
@@ -517,11 +548,15 @@ void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T*
                 if (c < kb) P[(int64_t)(k0 + c) * ld + row[b]] = y[b][p][e];
             }
     }
+    if (ds && tk.nsplit) {                            // rows stored: the next step may read them
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (t == 0) __hip_atomic_store(ds->flags + ds->ti, ds->ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     // ---- rank-kb updates of the diagonal blocks inside the tile (rows below ncol only): waves
-    //      2h, 2h+1 read back the rows of block h they have just stored (each thread its own
-    //      elements, so program order makes them visible) into LDS (Ys[c * 65 + r] = Y[r][c]),
-    //      and C(d, d) -= Y Y^T is formed on MFMA, wave w computing the 16-row stripe w, lower
-    //      blocks only
+    //      2h, 2h+1 hold the rows of block h in y; they go to LDS (Ys[c * 65 + r] = Y[r][c],
+    //      zero past kb columns and db rows), and C(d, d) -= Y Y^T is formed on MFMA, wave w
+    //      computing the 16-row stripe w, lower blocks only
     for (int h = 0; h < 2; ++h) {
         const int d0 = tk.r0 + 64 * h;                    // first row = first column of block
         if (64 * h >= ra || d0 >= tk.ncol) break;         // uniform
@@ -529,24 +564,15 @@ void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T*
         T* Ys = S;
         __syncthreads();                                  // the staged factor / last stripe is read
         if ((w >> 1) == h) {
-            T v[2][4][4];
 #pragma unroll
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int p = 0; p < 4; ++p)
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {           // clamped address times a 0/1 mask
+                    for (int e = 0; e < 4; ++e) {
                         const int rl = 32 * (w & 1) + 16 * b + li, c = 16 * p + R::row(l, e);
-                        v[b][p][e] = P[(int64_t)(k0 + min(c, kbm)) * ld + d0 + min(rl, db - 1)] *
-                                     ((c < kb && rl < db) ? T(1) : T(0));
+                        Ys[c * DP + rl] = (c < kb && rl < db) ? y[b][p][e] : T(0);
                     }
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int p = 0; p < 4; ++p)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        Ys[(16 * p + R::row(l, e)) * DP + 32 * (w & 1) + 16 * b + li] = v[b][p][e];
         }
         __syncthreads();
         Acc<T> dq[4];
@@ -585,11 +611,79 @@ void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T*
         Ds[c * DP + r] = v;
     }
     __syncthreads();
-    if (!potrf64<T>(Ds, Lv, &s_ok)) {
+    if (!potrf64<T>(Ds, Lv, s_ok)) {
         if (t == 0) atomicOr(err, 1);
         return;
     }
     publish_diag<T>(Ds, Lv, kb1, stage + tk.aux2, piv + tk.first + k1, diag + tk.aux);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T* __restrict__ diag,
+            T* __restrict__ piv, T* __restrict__ stage, const T* __restrict__ work, int32_t* err) {
+    const int ti = task_index();
+    if (ti >= ntask) return;
+    __shared__ int s_err;
+    if (wg_error(err, &s_err)) return;
+    __shared__ GemmLds<T> sm;
+    __shared__ int s_ok;
+    step_tile<T>(tasks[ti], L, diag, piv, stage, work, err, sm, &s_ok);
+}
+
+// All STEP tiles of one etree level in one launch.  Every supernode's tiles are assigned to one
+// XCD (its L2): queue x lists them in an order where every task's dependencies come first
+// (queue[qoff[x] .. qoff[x+1]), task indices).  A workgroup reads the XCD it runs on
+// (XCC_ID mod nx), takes the next ticket of that queue from counter cnt[x] (none left: it
+// exits), waits until the tasks deps[4 i .. 4 i + 3] (-1 = none) carry this factorization's
+// epoch ep_ctl[0] in flags, runs task i as a STEP tile and sets flags[i].  Tickets go to
+// running workgroups in order and a task waits only for tasks earlier in its queue, so the
+// earliest unfinished task of a queue can always run.  Workgroups are dealt to the XCDs in
+// turn, so a grid of nx times the longest queue gives every queue a workgroup per ticket; if
+// one were left without, its waiters end at the poll limit with error bit 0x20.  Producer
+// and consumer share an L2, so a release is the wait for the wavefront's stores (they reach
+// the L2) and an acquire is an L1 invalidate.
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_step_dag(const lkd_task* __restrict__ tasks, const int32_t* __restrict__ deps,
+                const int32_t* __restrict__ sdeps, const int32_t* __restrict__ queue,
+                const int32_t* __restrict__ qoff, int nx, uint32_t* __restrict__ flags,
+                uint32_t* __restrict__ flags2, const uint32_t* __restrict__ ep_ctl,
+                uint32_t* __restrict__ cnt, T* __restrict__ L, T* __restrict__ diag,
+                T* __restrict__ piv, T* __restrict__ stage, int32_t* err) {
+    __shared__ GemmLds<T> sm;
+    __shared__ int s_ok, s_task;
+    const uint32_t ep = ep_ctl[0];
+    if (threadIdx.x == 0) {
+        unsigned xcc;
+        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+        const int x = (int)(xcc % (unsigned)nx);
+        const int qb = qoff[x], qn = qoff[x + 1] - qb;
+        const int t = (int)atomicAdd(cnt + x, 1u);
+        int ti = t < qn ? queue[qb + t] : -1;
+        if (ti >= 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ti = -1;
+        for (int d = 0; d < 4 && ti >= 0; ++d) {
+            const int j = deps[4 * ti + d];
+            if (j >= 0 && !flag_wait(flags + j, ep, err)) ti = -1;
+        }
+        s_task = ti;
+    }
+    __syncthreads();
+    const int ti = s_task;
+    if (ti < 0) return;
+    asm volatile("buffer_inv sc0" ::: "memory");        // this CU's L1
+    const lkd_task tk = tasks[ti];
+    const DagSync ds{flags, flags2, sdeps[ti], ti, ep};
+    step_tile<T>(tk, L, diag, piv, stage, nullptr, err, sm, &s_ok, &ds);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");    // this wavefront's stores are in the L2
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store((tk.nsplit ? flags2 : flags) + ti, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ctl[0] += 1 (the new epoch), ctl[1 .. 1 + nq) = 0 (queue counters)
+__global__ void k_dag_begin(uint32_t* ctl, int nq) {
+    for (int i = threadIdx.x; i <= nq; i += blockDim.x) ctl[i] = i == 0 ? ctl[0] + 1 : 0u;
 }
 
 // Lkk of each task's block column into the panel: strictly lower part from the row-major copy,
@@ -786,6 +880,29 @@ int lkd_chol_part(int prec, int32_t ntask, const lkd_task* tasks, const void* pa
 int lkd_chol_step(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
                   void* piv, void* stage, const void* work, int32_t* err, void* stream) {
     return dispatch(prec, 5, ntask, tasks, {panel, diag, piv, stage, nullptr, err, const_cast<void*>(work)}, stream);
+}
+int lkd_chol_step_dag(int prec, int32_t ntask, const lkd_task* tasks, const int32_t* deps,
+                      const int32_t* sdeps, const int32_t* queue, const int32_t* qoff,
+                      int32_t nx, int32_t grid, uint32_t* flags, uint32_t* flags2, uint32_t* ctl,
+                      int32_t q, void* panel, void* diag, void* piv, void* stage, int32_t* err,
+                      void* stream) {
+    if (ntask <= 0) return 0;
+    if (q < 0 || nx < 1 || grid < 1) return (int)hipErrorInvalidValue;
+    uint32_t* cnt = ctl + 1 + (int64_t)q * nx;
+    if (prec == LKD_F32)
+        LK_LAUNCH(k_step_dag<float>, grid, 256, 0, stream, tasks, deps, sdeps, queue, qoff, nx,
+                  flags, flags2, ctl, cnt, (float*)panel, (float*)diag, (float*)piv,
+                  (float*)stage, err);
+    else
+        LK_LAUNCH(k_step_dag<double>, grid, 256, 0, stream, tasks, deps, sdeps, queue, qoff, nx,
+                  flags, flags2, ctl, cnt, (double*)panel, (double*)diag, (double*)piv,
+                  (double*)stage, err);
+    return lk_last_error();
+}
+int lkd_chol_dag_begin(uint32_t* ctl, int32_t nq, void* stream) {
+    if (nq < 0) return (int)hipErrorInvalidValue;
+    LK_LAUNCH(k_dag_begin, 1, 256, 0, stream, ctl, nq);
+    return lk_last_error();
 }
 int lkd_chol_update(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
                     lkd_snplan plan, int32_t* err, void* stream) {

@@ -62,11 +62,13 @@ void order_for_dispatch(std::vector<lkd_task>& v, const std::vector<char>& crit)
 }
 
 struct Launch {
-    int kind;        // 1 diag 2 update 3 diagwb 5 step 6 part | 20 forward level 21 backward level
+    int kind;        // 1 diag 2 update 3 diagwb 5 step 6 part 7 steps | 20 forward / 21 backward level
     int64_t begin;
     int32_t count;
     int32_t nw = 0;  // level solves: wavefronts per workgroup
     int32_t g = 1;   // level solves: workgroups per supernode
+    int32_t q = -1;  // STEPS: launch number (its queue counters and offsets)
+    int32_t grid = 0;  // STEPS: workgroups
 };
 
 template <typename T>
@@ -120,6 +122,16 @@ struct GPUKKTCholeskySolver::Impl {
     int32_t *d_sn_first = nullptr, *d_sn_rows = nullptr, *d_col_to_sn = nullptr;
     int64_t *d_sn_rowptr = nullptr, *d_sn_panel = nullptr;
     lkd_task *d_ftasks = nullptr, *d_stasks = nullptr;
+    std::vector<int32_t> fdeps;            // STEPS dependencies, 4 per factor task (-1 = none)
+    std::vector<int32_t> fsdep;            // STEPS: the task staging the tile's diagonal block
+    int32_t *d_fdeps = nullptr, *d_fsdep = nullptr;
+    int32_t n_dag = 0;                     // STEPS launches
+    int32_t n_xcd = 8;                     // STEPS queues per launch: one per XCD
+    std::vector<int32_t> fqueue, fqoff;    // STEPS queue order (task indices per launch), offsets
+    int32_t *d_fqueue = nullptr, *d_fqoff = nullptr;
+    uint32_t* d_dag_flags = nullptr;       // per factor task: epoch of its completion (rows)
+    uint32_t* d_dag_flags2 = nullptr;      // per factor task: epoch of its staged factor
+    uint32_t* d_dag_ctl = nullptr;         // [0] epoch, [1 + q n_xcd + x] queue counters
     int64_t nseg = 0, npairs = 0;
     std::vector<Launch> flaunch, slaunch;
 
@@ -176,6 +188,10 @@ struct GPUKKTCholeskySolver::Impl {
         for (Index i = 0; i < m; ++i) pinv[perm[i]] = i;
         build_matrices(A);
         build_pairs();
+        if (arena_p) {   // XCDs of the device: 32 compute units each (1 .. 8)
+            int cus = 0;
+            if (lkd_device_cus(&cus) == 0 && cus > 0) n_xcd = std::max(1, std::min(8, (cus + 16) / 32));
+        }
         build_factor_plan();
         build_solve_plan();
         if (sym) build_csr_index(*sym);
@@ -423,16 +439,31 @@ struct GPUKKTCholeskySolver::Impl {
             return static_cast<int32_t>((static_cast<int64_t>(K) * j / ns) / 16 * 16);
         };
         std::vector<lkd_task> diag_tasks;   // one per (supernode, block column): DIAGWB
+        std::vector<int32_t> sidx(static_cast<size_t>(ns), -1);   // supernode -> index in its level
         for (const auto& lv : by_level) {
+            for (size_t si = 0; si < lv.size(); ++si) sidx[lv[si]] = static_cast<int32_t>(si);
             int maxblk = 0;
             for (Index s : lv) {
                 const Index nc = S.sn_first[s + 1] - S.sn_first[s];
                 maxblk = std::max<int>(maxblk, (nc + TILE - 1) / TILE);
             }
-            // two staging slots per supernode of the level: block k is read from slot k % 2
-            // while block k + 1 is written to the other one
-            auto slot = [&](size_t si, int k) { return (2 * static_cast<int64_t>(si) + (k & 1)) * LKD_STAGE; };
-            stage_elems = std::max<int64_t>(stage_elems, 2 * static_cast<int64_t>(lv.size()) * LKD_STAGE);
+            // staging slots: with one launch per step, two per supernode of the level (block k
+            // is read from slot k % 2 while block k + 1 is written to the other one); with the
+            // STEPS launch, where steps overlap, one per block column of the level
+            const bool dag = opt.step_dag;
+            std::vector<int64_t> blkbase(lv.size() + 1, 0);
+            for (size_t si = 0; si < lv.size(); ++si) {
+                const Index nc = S.sn_first[lv[si] + 1] - S.sn_first[lv[si]];
+                blkbase[si + 1] = blkbase[si] + (dag ? (nc + TILE - 1) / TILE : 2);
+            }
+            auto slot = [&](size_t si, int k) {
+                return (blkbase[si] + (dag ? k : (k & 1))) * static_cast<int64_t>(LKD_STAGE);
+            };
+            stage_elems = std::max<int64_t>(stage_elems, blkbase[lv.size()] * LKD_STAGE);
+            // STEPS: the tiles of all steps in step order, with, per tile, the step and rows
+            struct DagTile { size_t si; int k; int r0, r1; bool nsplit; };
+            std::vector<lkd_task> dag_tasks;
+            std::vector<DagTile> dag_meta;
             for (Index s : lv) {
                 lkd_task t = base(s);
                 for (int k = 0; k * TILE < t.ncol; ++k) {
@@ -483,6 +514,28 @@ struct GPUKKTCholeskySolver::Impl {
                     }
                 }
                 if (tl.empty()) continue;
+                if (dag) {   // whole K; in the queue, the step's next-diagonal tiles first, then
+                             // the tiles right below them (the next one waits for those), then
+                             // the others
+                    std::vector<int> cls(tl.size(), 2);
+                    for (size_t i = 0; i < tl.size(); ++i)
+                        if (tl[i].nsplit) {
+                            cls[i] = 0;
+                            if (i + 1 < tl.size() && tl[i + 1].sn == tl[i].sn) cls[i + 1] = 1;
+                        }
+                    for (int pass = 0; pass < 3; ++pass)
+                        for (size_t i = 0; i < tl.size(); ++i) {
+                            if (cls[i] != pass) continue;
+                            lkd_task u = tl[i];
+                            u.kbeg = 0;
+                            u.kend = u.k0;
+                            u.np = 0;
+                            u.px = 0;
+                            dag_meta.push_back({static_cast<size_t>(sidx[u.sn]), k, u.r0, u.r0 + u.c0, u.nsplit != 0});
+                            dag_tasks.push_back(u);
+                        }
+                    continue;
+                }
                 // few tiles: split every tile's K = k0 columns into S parts computed by a PART
                 // launch (the STEP then only adds them); the launch count per step is one more
                 int kmax = 0;
@@ -530,6 +583,69 @@ struct GPUKKTCholeskySolver::Impl {
                 tasks.insert(tasks.end(), tl.begin(), tl.end());
                 emit(5, b0);
             }
+            if (!dag_tasks.empty()) {
+                // a tile waits for the tiles of the previous step of its supernode whose rows
+                // overlap its rows, and for the one that factored its diagonal block
+                std::vector<std::vector<int32_t>> prev(lv.size()), cur(lv.size());
+                int cur_k = -1;
+                b0 = static_cast<int64_t>(tasks.size());
+                fdeps.resize(static_cast<size_t>(4 * b0), -1);
+                fsdep.resize(static_cast<size_t>(b0), -1);
+                for (size_t i = 0; i < dag_tasks.size(); ++i) {
+                    const DagTile& d = dag_meta[i];
+                    if (d.k != cur_k) {
+                        for (size_t si = 0; si < lv.size(); ++si) {
+                            prev[si].swap(cur[si]);
+                            cur[si].clear();
+                        }
+                        cur_k = d.k;
+                    }
+                    std::vector<int32_t> dep;
+                    int32_t sdep = -1;
+                    for (int32_t j : prev[d.si]) {
+                        const DagTile& e = dag_meta[static_cast<size_t>(j)];
+                        if (e.k != d.k - 1) continue;
+                        if (e.nsplit || (e.r0 < d.r1 && d.r0 < e.r1)) dep.push_back(j);
+                        if (e.nsplit) sdep = j;
+                    }
+                    fsdep.push_back(sdep);
+                    if (dep.size() > 4) throw std::logic_error("KKT plan: STEPS tile with more than 4 dependencies");
+                    for (size_t q = 0; q < 4; ++q) fdeps.push_back(q < dep.size() ? dep[q] : -1);
+                    if (d.nsplit)   // the chain of next-diagonal tiles: what it waits for is critical
+                        for (int32_t j : dep) dag_tasks[static_cast<size_t>(j)].prio = 1;
+                    cur[d.si].push_back(static_cast<int32_t>(i));
+                }
+                // queues: every supernode's tiles on one XCD, supernodes dealt by work to the
+                // least loaded XCD; each queue keeps the list order
+                std::vector<double> sw(lv.size(), 0.0), xw(static_cast<size_t>(n_xcd), 0.0);
+                for (const auto& u : dag_tasks) sw[static_cast<size_t>(sidx[u.sn])] += task_flops(u, 5) + 1e6;
+                std::vector<size_t> ord(lv.size());
+                for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+                std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return sw[a] > sw[b]; });
+                std::vector<int> sx(lv.size(), 0);
+                for (size_t i : ord) {
+                    const int x = static_cast<int>(std::min_element(xw.begin(), xw.end()) - xw.begin());
+                    sx[i] = x;
+                    xw[static_cast<size_t>(x)] += sw[i];
+                }
+                fqueue.resize(static_cast<size_t>(b0), -1);
+                int32_t longest = 0;
+                for (int x = 0; x < n_xcd; ++x) {
+                    fqoff.push_back(static_cast<int32_t>(fqueue.size() - static_cast<size_t>(b0)));
+                    int32_t len = 0;
+                    for (size_t i = 0; i < dag_tasks.size(); ++i)
+                        if (sx[dag_meta[i].si] == x) {
+                            fqueue.push_back(static_cast<int32_t>(i));
+                            ++len;
+                        }
+                    longest = std::max(longest, len);
+                }
+                fqoff.push_back(static_cast<int32_t>(dag_tasks.size()));
+                tasks.insert(tasks.end(), dag_tasks.begin(), dag_tasks.end());
+                emit(7, b0);
+                flaunch.back().q = n_dag++;
+                flaunch.back().grid = n_xcd * longest;
+            }
             // UPDATE: L21 L21^T of every supernode into its ancestors, split K
             int64_t tiles = 0, maxK = 0;
             for (Index s : lv) {
@@ -564,9 +680,54 @@ struct GPUKKTCholeskySolver::Impl {
             tasks.insert(tasks.end(), diag_tasks.begin(), diag_tasks.end());
             emit(3, b0);
         }
+        fdeps.resize(4 * tasks.size(), -1);
+        fsdep.resize(tasks.size(), -1);
+        fqueue.resize(tasks.size(), -1);
         for (const auto& L : flaunch)
-            for (int64_t q = L.begin; q < L.begin + L.count; ++q) validate(tasks[q], L.kind);
+            for (int64_t q = L.begin; q < L.begin + L.count; ++q) {
+                validate(tasks[q], L.kind == 7 ? 5 : L.kind);
+                if (L.kind != 7) continue;
+                if (tasks[q].np != 0) throw std::logic_error("KKT plan: STEPS tile with partial products");
+                for (int d = 0; d < 5; ++d) {   // earlier tiles of the same supernode
+                    const int32_t j = d < 4 ? fdeps[4 * q + d] : fsdep[q];
+                    if (j < -1 || j >= q - L.begin || (j >= 0 && tasks[L.begin + j].sn != tasks[q].sn))
+                        throw std::logic_error("KKT plan: invalid STEPS dependency");
+                    if (d == 4 && j >= 0 && !tasks[L.begin + j].nsplit)
+                        throw std::logic_error("KKT plan: STEPS staging dependency is not a factoring tile");
+                }
+            }
+        for (const auto& L : flaunch) {   // STEPS queues: a permutation, dependencies first
+            if (L.kind != 7) continue;
+            const int32_t* off = fqoff.data() + static_cast<size_t>(L.q) * (n_xcd + 1);
+            std::vector<int32_t> pos(static_cast<size_t>(L.count), -1), xq(static_cast<size_t>(L.count), -1);
+            int32_t longest = 0;
+            for (int x = 0; x < n_xcd; ++x) {
+                longest = std::max(longest, off[x + 1] - off[x]);
+                for (int32_t p = off[x]; p < off[x + 1]; ++p) {
+                    const int32_t i = fqueue[static_cast<size_t>(L.begin + p)];
+                    if (i < 0 || i >= L.count || pos[i] >= 0) throw std::logic_error("KKT plan: invalid STEPS queue");
+                    pos[i] = p;
+                    xq[i] = x;
+                }
+            }
+            if (off[0] != 0 || off[n_xcd] != L.count || L.grid < n_xcd * longest)
+                throw std::logic_error("KKT plan: invalid STEPS queue offsets");
+            for (int32_t i = 0; i < L.count; ++i)
+                for (int d = 0; d < 4; ++d) {
+                    const int32_t j = fdeps[4 * (L.begin + i) + d];
+                    if (j >= 0 && (xq[j] != xq[i] || pos[j] >= pos[i]))
+                        throw std::logic_error("KKT plan: STEPS dependency outside its queue order");
+                }
+        }
         d_ftasks = up(tasks, "ftasks");
+        d_fdeps = up(fdeps, "fdeps");
+        d_fsdep = up(fsdep, "fsdep");
+        d_fqueue = up(fqueue, "fqueue");
+        d_fqoff = up(fqoff, "fqoff");
+        if (keep) {
+            std::vector<int32_t> nx{n_xcd};
+            up(nx, "n_xcd");
+        }
         h_ftasks = tasks;
         n_ftasks = static_cast<int64_t>(tasks.size());
     }
@@ -752,6 +913,14 @@ struct GPUKKTCholeskySolver::Impl {
         owned.push_back(d_stage);
         d_work = arena_p->allocate_uninitialized(static_cast<size_t>(std::max<int64_t>(work_elems, 1)) * sizeof(double));
         owned.push_back(d_work);
+        if (n_dag > 0) {   // zero-filled: epoch 0 is never current
+            d_dag_flags = static_cast<uint32_t*>(arena_p->allocate(static_cast<size_t>(n_ftasks) * sizeof(uint32_t)));
+            owned.push_back(d_dag_flags);
+            d_dag_flags2 = static_cast<uint32_t*>(arena_p->allocate(static_cast<size_t>(n_ftasks) * sizeof(uint32_t)));
+            owned.push_back(d_dag_flags2);
+            d_dag_ctl = static_cast<uint32_t*>(arena_p->allocate(static_cast<size_t>(1 + n_dag * n_xcd) * sizeof(uint32_t)));
+            owned.push_back(d_dag_ctl);
+        }
         d_below = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(solve_below, 1)) * sizeof(double)));
         owned.push_back(d_below);
         d_pbuf = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(solve_pbuf, 1)) * sizeof(double)));
@@ -802,6 +971,7 @@ struct GPUKKTCholeskySolver::Impl {
     // -------------------------------------------------------------------- numeric phases
     void run_factor_schedule(int prec) {
         const lkd_snplan pl = plan();
+        if (n_dag > 0) check(lkd_chol_dag_begin(d_dag_ctl, n_dag * n_xcd, stream), "steps begin");
         if (!timer.on) {
             for (const auto& L : flaunch) check(launch_factor(prec, L, pl), "factor kernel");
             return;
@@ -838,7 +1008,7 @@ struct GPUKKTCholeskySolver::Impl {
 
     // 0 STEP, 1 DIAG + DIAGWB, 2 UPDATE, 5 PART (3 assembly and 4 totals are not launch kinds)
     static int slot_of(const Launch& L) {
-        return L.kind == 5 ? 0 : L.kind == 6 ? 5 : L.kind == 2 ? 2 : 1;
+        return L.kind == 5 || L.kind == 7 ? 0 : L.kind == 6 ? 5 : L.kind == 2 ? 2 : 1;
     }
 
     // Live launch timing: HIP events on the solver's stream at every change of launch kind in
@@ -901,6 +1071,10 @@ struct GPUKKTCholeskySolver::Impl {
             case 2: return lkd_chol_update(prec, L.count, t, d_panel, pl, d_err, stream);
             case 3: return lkd_chol_diagwb(prec, L.count, t, d_panel, d_diag, d_piv, stream);
             case 6: return lkd_chol_part(prec, L.count, t, d_panel, d_work, d_err, stream);
+            case 7: return lkd_chol_step_dag(prec, L.count, t, d_fdeps + 4 * L.begin, d_fsdep + L.begin,
+                                             d_fqueue + L.begin, d_fqoff + static_cast<int64_t>(L.q) * (n_xcd + 1),
+                                             n_xcd, L.grid, d_dag_flags + L.begin, d_dag_flags2 + L.begin,
+                                             d_dag_ctl, L.q, d_panel, d_diag, d_piv, d_stage, d_err, stream);
             default: return lkd_chol_step(prec, L.count, t, d_panel, d_diag, d_piv, d_stage, d_work, d_err, stream);
         }
     }
@@ -960,7 +1134,8 @@ struct GPUKKTCholeskySolver::Impl {
         const double kb = t.kb;
         switch (kind) {
             case 6: return 2.0 * t.c0 * kb * (t.kend - t.kbeg);
-            case 5: {   // GEMM + solve of the tile; diagonal-block updates; next block's factor
+            case 5:
+            case 7: {   // GEMM + solve of the tile; diagonal-block updates; next block's factor
                 const double ra = t.c0;
                 double f = 2.0 * ra * kb * (t.kend - t.kbeg) + ra * kb * kb;
                 for (int h = 0; h < 2; ++h) {
@@ -1047,6 +1222,7 @@ struct GPUKKTCholeskySolver::Impl {
         check(lkd_memcpy_dtoh(&err, d_err, sizeof(int32_t), stream), "err read");
         st.factorizations++;
         if (err & 2) throw std::logic_error("KKT factorization: update outside the L structure");
+        if (err & 0x20) throw std::runtime_error("KKT factorization: a STEPS wait exceeded its poll limit");
         return err == 0;
     }
 
@@ -1249,17 +1425,19 @@ std::size_t GPUKKTCholeskySolver::estimate_bytes(const core::CSRMatrix& A,
     for (int64_t p = 0; p < nz; ++p) cnt[A.col_indices[p]]++;
     int64_t pairs = 0;
     for (int64_t c : cnt) pairs += c * (c + 1) / 2;
-    int64_t diag = 0, tasks = 0;
+    int64_t diag = 0, tasks = 0, slots = 0;
     for (Index s = 0; s < S.nsuper; ++s) {
         const int64_t nc = S.sn_first[s + 1] - S.sn_first[s];
         const int64_t ld = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
         diag += nc * 64;
+        slots += std::max<int64_t>(2, (nc + 63) / 64);   // staging slots, all levels at once
         const int64_t bc = (nc + 63) / 64, br = (ld + 63) / 64;
         tasks += 4 + bc * (br + 4) + bc * br * (br + 1) / 2 + br * br;
     }
     const int64_t bytes = 8 * S.sn_panel[S.nsuper] + 8 * diag + 20 * pairs + 36 * pairs +
                           2 * (16 * nz + 8 * (m + n)) + 8 * nz + 8 * 24 * (m + n) +
-                          48 * 2 * tasks + 8 * 4096 + static_cast<int64_t>(S.sn_rows.size()) * 4 +
+                          128 * tasks + 8 * 4096 + static_cast<int64_t>(S.sn_rows.size()) * 4 +
+                          8 * LKD_STAGE * slots + 8 * int64_t(STEP_TARGET_WG) * LKD_PART +
                           24 * static_cast<int64_t>(S.nsuper) + 8 * m + 256 * 96;
     return static_cast<std::size_t>(bytes);
 }

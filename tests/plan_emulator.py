@@ -56,9 +56,34 @@ class Emulator:
         Lr[np.arange(kb), np.arange(kb)] = 1.0 / np.diag(L)
         self.diag[off:off + kb * kb] = Lr.reshape(-1)
 
-    def factor(self):
-        """Left-looking schedule: DIAG (kind 1, block column 0), PART (6), STEP (5), UPDATE (2),
-        DIAGWB (3).
+    def dag_order(self, b, c, rng=None):
+        """An execution order of STEPS launch tasks b .. b+c that respects their dependencies:
+        list order, or (rng given) a random choice among the ready tasks at every step."""
+        if rng is None:
+            return list(range(b, b + c))
+        deps = self.p["fdeps"][4 * b:4 * (b + c)].reshape(c, 4)
+        indeg = (deps >= 0).sum(axis=1)
+        kids = [[] for _ in range(c)]
+        for i in range(c):
+            for d in deps[i]:
+                if d >= 0:
+                    kids[d].append(i)
+        ready = [i for i in range(c) if indeg[i] == 0]
+        out = []
+        while ready:
+            i = ready.pop(int(rng.integers(len(ready))))
+            out.append(b + i)
+            for j in kids[i]:
+                indeg[j] -= 1
+                if indeg[j] == 0:
+                    ready.append(j)
+        assert len(out) == c, "STEPS dependencies have a cycle"
+        return out
+
+    def factor(self, rng=None):
+        """Left-looking schedule: DIAG (kind 1, block column 0), PART (6), STEP (5), STEPS (7:
+        the STEP tiles of a level, run in list order or, with rng, in a random order that
+        respects their dependencies), UPDATE (2), DIAGWB (3).
         Diagonal factors live in staging slots between launches; the panel's diagonal blocks
         are only written by DIAGWB."""
         p = self.p
@@ -67,8 +92,11 @@ class Emulator:
         self.stage = {}
         self.work = {}
         for kind, b, c in self.fl:
-            assert kind in (1, 2, 3, 5, 6)
-            for t in T[b:b + c]:
+            assert kind in (1, 2, 3, 5, 6, 7)
+            order = self.dag_order(b, c, rng) if kind == 7 else range(b, b + c)
+            kind = 5 if kind == 7 else kind
+            for q in order:
+                t = T[q]
                 if self.err:
                     return False
                 k0, kb, r0, c0, ld, nc = (int(t[x]) for x in ("k0", "kb", "r0", "c0", "ld", "ncol"))

@@ -17,10 +17,12 @@ def _cases():
         "block_angular": lambda: gen.block_angular(4, 60, 150, 5, 5, 0.2, seed=3),
         "transport": lambda: gen.transport(8, 8, 6, 3, seed=2),
         "dense": lambda: gen.dense_small(420, 900, 0.02, seed=5),
+        "block_angular_wide": lambda: gen.block_angular(3, 150, 400, 6, 8, 0.2, seed=3),
     }
 
 
-@pytest.mark.parametrize("name", ["random_sparse", "block_angular", "transport", "dense"])
+@pytest.mark.parametrize("name", ["random_sparse", "block_angular", "transport", "dense",
+                                  "block_angular_wide"])
 def test_schedule_reproduces_dense_cholesky(native, name):
     lp = _cases()[name]()
     rp, ci, va = lp.csr()
@@ -30,13 +32,16 @@ def test_schedule_reproduces_dense_cholesky(native, name):
     P = native.order_amd(rp, ci, n)
     plan = native.plan_export(rp, ci, va, n, P=P)
     kinds = set(plan["flaunch"].reshape(-1, 3)[:, 0].tolist())
-    assert kinds <= {1, 2, 3, 5, 6}
-    if name == "dense":
-        assert 6 in kinds                 # wide supernodes with few tiles split K
+    assert kinds <= {1, 2, 3, 7}          # STEP tiles of a level in one STEPS launch
     em = Emulator(plan)
     D = em.assemble(theta, equil=True, delta=1e-9)
     assert em.factor(), f"emulated factorization failed (err {em.err})"
     L = em.dense_L()
+    for seed in range(3):                 # any order the STEPS dependencies allow
+        em2 = Emulator(plan)
+        em2.assemble(theta, equil=True, delta=1e-9)
+        assert em2.factor(np.random.default_rng(seed))
+        assert np.abs(em2.dense_L() - L).max() <= 1e-12 * np.abs(L).max()
 
     A = np.zeros((m, n))
     for i in range(m):
@@ -75,3 +80,82 @@ def test_pair_map_length_classes(native, name):
     assert cs[-1] == len(L) and np.all(np.diff(plan["segdst"]) > 0)
     if name == "block_angular":           # the linking rows' diagonals are long
         assert len(plan["mid_segs"]) and len(plan["long_segs"]) and len(plan["long_diag"])
+
+
+def _tile_phases(t):
+    """Regions a STEPS tile touches, per phase: P (GEMM operands), S (solve, store, diagonal
+    blocks of a tile that does not factor), B (a factoring tile: its diagonal block, factor,
+    staging).  Region: (space, key, lo0, hi0, lo1, hi1, writes)."""
+    s, k0, kb, r0, ra = (int(t[x]) for x in ("sn", "k0", "kb", "r0", "c0"))
+    nc, first, nsplit = int(t["ncol"]), int(t["first"]), int(t["nsplit"])
+    P = [("panel", s, 0, k0, r0, r0 + ra, False),             # L(R, 0:k0)
+         ("panel", s, 0, k0, k0, k0 + kb, False)]             # L(k, 0:k0)
+    S = [("panel", s, k0, k0 + kb, r0, r0 + ra, True),         # C(R, k) -> L(R, k)
+         ("stage", int(t["ws"]), 0, 1, 0, 1, False)]
+    D = []
+    for h in range(2):                                         # diagonal blocks in the tile
+        d0 = r0 + 64 * h
+        if 64 * h >= ra or d0 >= nc:
+            break
+        db = min(64, ra - 64 * h, nc - d0)
+        D.append(("panel", s, d0, d0 + db, d0, d0 + db, True))
+    if not nsplit:
+        return [P, S + D]
+    B = D + [("stage", int(t["aux2"]), 0, 1, 0, 1, True),
+             ("diag", 0, int(t["aux"]), int(t["aux"]) + ra * ra, 0, 1, True),
+             ("piv", 0, first + r0, first + r0 + ra, 0, 1, True)]
+    return [P, S, B]
+
+
+@pytest.mark.parametrize("name", ["random_sparse", "dense", "block_angular_wide"])
+def test_steps_dependencies_order_every_conflict(native, name):
+    """Inside a STEPS launch, two phases of tiles that touch the same data with at least one
+    write must be ordered: by program order inside a tile, or through the hand-offs (a
+    dependency's rows flag before the GEMM, the staging flag before the solve)."""
+    lp = _cases()[name]()
+    rp, ci, va = lp.csr()
+    plan = native.plan_export(rp, ci, va, lp.n, P=native.order_amd(rp, ci, lp.n))
+    T, deps, sdep = plan["ftasks"], plan["fdeps"].reshape(-1, 4), plan["fsdep"]
+    nsteps = 0
+    for kind, b, c in plan["flaunch"].reshape(-1, 3):
+        if kind != 7:
+            continue
+        nsteps += c
+        ph = [_tile_phases(T[b + i]) for i in range(c)]
+        node = {}                                   # (task, phase) -> node id
+        for i in range(c):
+            for p_ in range(len(ph[i])):
+                node[(i, p_)] = len(node)
+        anc = [0] * len(node)                       # nodes in task order are topological
+        for i in range(c):
+            for p_ in range(len(ph[i])):
+                n_ = node[(i, p_)]
+                a = 0
+                if p_ > 0:                          # program order
+                    a |= anc[node[(i, p_ - 1)]] | (1 << node[(i, p_ - 1)])
+                if p_ == 0:                         # flags[j]: after j's S phase
+                    for j in deps[b + i]:
+                        if j >= 0:
+                            assert j < i
+                            m = node[(int(j), 1)]
+                            a |= anc[m] | (1 << m)
+                if p_ == 1 and sdep[b + i] >= 0:    # flags2[j]: after j's B phase
+                    j = int(sdep[b + i])
+                    assert j < i and len(ph[j]) == 3
+                    m = node[(j, 2)]
+                    a |= anc[m] | (1 << m)
+                anc[n_] = a
+        items = [(node[(i, p_)], i, reg) for i in range(c) for p_ in range(len(ph[i])) for reg in ph[i][p_]]
+        by_key = {}
+        for it in items:
+            by_key.setdefault((it[2][0], it[2][1]), []).append(it)
+        for lst in by_key.values():
+            for u in range(len(lst)):
+                for v in range(u + 1, len(lst)):
+                    (nx, i, x), (ny, j, y) = lst[u], lst[v]
+                    if i == j or not (x[6] or y[6]):
+                        continue
+                    if not (x[2] < y[3] and y[2] < x[3] and x[4] < y[5] and y[4] < x[5]):
+                        continue
+                    assert (anc[ny] >> nx) & 1 or (anc[nx] >> ny) & 1, (name, b, i, j, x, y)
+    assert nsteps > 0
