@@ -65,7 +65,8 @@ int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream);
  *   np/px  STEP: number of partial products (from the PART launch before it) added to the
  *          tile's own GEMM, stored at work + px + s * LKD_PART (s < np); PART: px = where its
  *          product goes
- *   prio   STEP: 1 on tiles of the critical path (their wavefronts get issue priority)     */
+ *   prio   STEP: 1 on tiles of the critical path (their wavefronts get issue priority)
+ *   part   STEPS: 1 on a PART task (product of K range kbeg..kend into work + px)          */
 typedef struct {
     int64_t off;
     int64_t aux;     /* diagonal-copy offset (DIAG / STEP next block / DIAGWB / TRSV) */
@@ -78,7 +79,7 @@ typedef struct {
     int32_t nsplit, sn;
     int32_t first, np;
     int64_t px;
-    int32_t prio, pad;
+    int32_t prio, part;
 } lkd_task;
 
 typedef struct {
@@ -179,15 +180,17 @@ int lkd_chol_step(int prec, int32_t ntask, const lkd_task* tasks, void* panel, v
  * ctl[1 + q nx + x] until it is empty.  A task waits until flags[dep] == ctl[0] for each
  * dependency, runs its GEMM, waits until flags2[sdep] == ctl[0], solves, then sets
  * flags[task] = ctl[0]; a tile that factors the next diagonal block sets flags[task] once its
- * rows are stored and flags2[task] once the factor is staged.  Staging slots must not be
+ * rows are stored and flags2[task] once the factor is staged.  A task with part = 1 is a PART
+ * (its product over kbeg..kend into work + px, then flags[task]); a tile with np = 1 adds the
+ * product at work + px (its PART is among its dependencies).  Staging slots must not be
  * shared by two block columns of the level.  lkd_chol_dag_begin(ctl, nq) starts a
  * factorization: ctl[0] += 1 and the nq counters ctl[1 .. nq] = 0 (a kernel, so it can be
  * captured in a graph). */
 int lkd_chol_step_dag(int prec, int32_t ntask, const lkd_task* tasks, const int32_t* deps,
                       const int32_t* sdeps, const int32_t* queue, const int32_t* qoff,
                       int32_t nx, int32_t grid, uint32_t* flags, uint32_t* flags2, uint32_t* ctl,
-                      int32_t q, void* panel, void* diag, void* piv, void* stage, int32_t* err,
-                      void* stream);
+                      int32_t q, void* panel, void* diag, void* piv, void* stage, void* work,
+                      int32_t* err, void* stream);
 int lkd_chol_dag_begin(uint32_t* ctl, int32_t nq, void* stream);
 int lkd_chol_update(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
                     lkd_snplan plan, int32_t* err, void* stream);

@@ -400,15 +400,8 @@ __global__ __launch_bounds__(256) void k_diag(const lkd_task* __restrict__ tasks
 // stored as this thread's accumulator elements: work[px + i * 256 + thread], i < 32.  The STEP
 // launch that follows reads them back in the same layout (coalesced) and adds them in order.
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-void k_part(const lkd_task* __restrict__ tasks, int ntask, const T* __restrict__ L,
-            T* __restrict__ work, int32_t* err) {
-    const int ti = task_index();
-    if (ti >= ntask) return;
-    __shared__ int s_err;
-    if (wg_error(err, &s_err)) return;
-    const lkd_task tk = tasks[ti];
-    __shared__ GemmLds<T> sm;
+__device__ __forceinline__ void part_tile(const lkd_task& tk, const T* __restrict__ L,
+                                          T* __restrict__ work, GemmLds<T>& sm) {
     const T* P = L + tk.off;
     Acc<T> acc[4][2];
     zero_acc<T, 4, 2>(acc);
@@ -420,6 +413,18 @@ void k_part(const lkd_task* __restrict__ tasks, int ntask, const T* __restrict__
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int e = 0; e < 4; ++e) out[((p * 2 + b) * 4 + e) * 256] = acc[p][b][e];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_part(const lkd_task* __restrict__ tasks, int ntask, const T* __restrict__ L,
+            T* __restrict__ work, int32_t* err) {
+    const int ti = task_index();
+    if (ti >= ntask) return;
+    __shared__ int s_err;
+    if (wg_error(err, &s_err)) return;
+    __shared__ GemmLds<T> sm;
+    part_tile<T>(tasks[ti], L, work, sm);
 }
 
 // Wait (one thread) until flag == ep; false on an error flag or after kSpinLimit polls (then
@@ -650,7 +655,7 @@ void k_step_dag(const lkd_task* __restrict__ tasks, const int32_t* __restrict__ 
                 const int32_t* __restrict__ qoff, int nx, uint32_t* __restrict__ flags,
                 uint32_t* __restrict__ flags2, const uint32_t* __restrict__ ep_ctl,
                 uint32_t* __restrict__ cnt, T* __restrict__ L, T* __restrict__ diag,
-                T* __restrict__ piv, T* __restrict__ stage, int32_t* err) {
+                T* __restrict__ piv, T* __restrict__ stage, T* __restrict__ work, int32_t* err) {
     __shared__ GemmLds<T> sm;
     __shared__ int s_ok, s_task;
     const uint32_t ep = ep_ctl[0];
@@ -673,8 +678,12 @@ void k_step_dag(const lkd_task* __restrict__ tasks, const int32_t* __restrict__ 
     if (ti < 0) return;
     asm volatile("buffer_inv sc0" ::: "memory");        // this CU's L1
     const lkd_task tk = tasks[ti];
-    const DagSync ds{flags, flags2, sdeps[ti], ti, ep};
-    step_tile<T>(tk, L, diag, piv, stage, nullptr, err, sm, &s_ok, &ds);
+    if (tk.part) {
+        part_tile<T>(tk, L, work, sm);
+    } else {
+        const DagSync ds{flags, flags2, sdeps[ti], ti, ep};
+        step_tile<T>(tk, L, diag, piv, stage, work, err, sm, &s_ok, &ds);
+    }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");    // this wavefront's stores are in the L2
     __syncthreads();
     if (threadIdx.x == 0)
@@ -884,19 +893,19 @@ int lkd_chol_step(int prec, int32_t ntask, const lkd_task* tasks, void* panel, v
 int lkd_chol_step_dag(int prec, int32_t ntask, const lkd_task* tasks, const int32_t* deps,
                       const int32_t* sdeps, const int32_t* queue, const int32_t* qoff,
                       int32_t nx, int32_t grid, uint32_t* flags, uint32_t* flags2, uint32_t* ctl,
-                      int32_t q, void* panel, void* diag, void* piv, void* stage, int32_t* err,
-                      void* stream) {
+                      int32_t q, void* panel, void* diag, void* piv, void* stage, void* work,
+                      int32_t* err, void* stream) {
     if (ntask <= 0) return 0;
     if (q < 0 || nx < 1 || grid < 1) return (int)hipErrorInvalidValue;
     uint32_t* cnt = ctl + 1 + (int64_t)q * nx;
     if (prec == LKD_F32)
         LK_LAUNCH(k_step_dag<float>, grid, 256, 0, stream, tasks, deps, sdeps, queue, qoff, nx,
                   flags, flags2, ctl, cnt, (float*)panel, (float*)diag, (float*)piv,
-                  (float*)stage, err);
+                  (float*)stage, (float*)work, err);
     else
         LK_LAUNCH(k_step_dag<double>, grid, 256, 0, stream, tasks, deps, sdeps, queue, qoff, nx,
                   flags, flags2, ctl, cnt, (double*)panel, (double*)diag, (double*)piv,
-                  (double*)stage, err);
+                  (double*)stage, (double*)work, err);
     return lk_last_error();
 }
 int lkd_chol_dag_begin(uint32_t* ctl, int32_t nq, void* stream) {

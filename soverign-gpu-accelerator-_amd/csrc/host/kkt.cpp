@@ -39,6 +39,7 @@ constexpr int GEMM_TARGET_WG = 1024;   // UPDATE K splitting aims at this many w
 constexpr int KSPLIT_MIN = 128;     // minimum K columns per split part
 constexpr int STEP_TARGET_WG = 512; // a STEP launch with fewer tiles splits K (PART launch) ...
 constexpr int STEP_KMIN = 64;       // ... into parts of at least this many columns
+constexpr int STEPS_RECENT = 256;   // STEPS: K columns a chain tile computes itself
 constexpr int SOLVE_WIDE_TILES = 8;   // level solves: 8 wavefronts per supernode above this
 constexpr int SOLVE_MAX_G = 4;        // forward level solves: workgroups per supernode (all resident)
 constexpr int SOLVE_G_LEVEL = 4;      // ... only for levels of at most this many supernodes
@@ -461,7 +462,8 @@ struct GPUKKTCholeskySolver::Impl {
             };
             stage_elems = std::max<int64_t>(stage_elems, blkbase[lv.size()] * LKD_STAGE);
             // STEPS: the tiles of all steps in step order, with, per tile, the step and rows
-            struct DagTile { size_t si; int k; int r0, r1; bool nsplit; };
+            // after: PART tasks, the step after whose tiles they are queued
+            struct DagTile { size_t si; int k; int r0, r1; bool nsplit; int cls; bool part; int after; };
             std::vector<lkd_task> dag_tasks;
             std::vector<DagTile> dag_meta;
             for (Index s : lv) {
@@ -531,7 +533,8 @@ struct GPUKKTCholeskySolver::Impl {
                             u.kend = u.k0;
                             u.np = 0;
                             u.px = 0;
-                            dag_meta.push_back({static_cast<size_t>(sidx[u.sn]), k, u.r0, u.r0 + u.c0, u.nsplit != 0});
+                            dag_meta.push_back({static_cast<size_t>(sidx[u.sn]), k, u.r0, u.r0 + u.c0,
+                                                u.nsplit != 0, cls[i], false, k});
                             dag_tasks.push_back(u);
                         }
                     continue;
@@ -584,36 +587,99 @@ struct GPUKKTCholeskySolver::Impl {
                 emit(5, b0);
             }
             if (!dag_tasks.empty()) {
+                // Tiles on the chain that limits the level (the next-diagonal tiles and the tiles
+                // right below them) leave all but their last STEPS_RECENT columns of K to a PART
+                // task queued right after the step that completes those columns, so that
+                // product is ready early and the chain only computes the recent columns.
+                {
+                    const size_t nt = dag_tasks.size();
+                    int64_t px = 0;
+                    for (size_t i = 0; i < nt; ++i) {
+                        lkd_task& u = dag_tasks[i];
+                        if (dag_meta[i].cls > 1 || u.k0 < STEPS_RECENT + 2 * TILE) continue;
+                        const int kp = (u.k0 - STEPS_RECENT) / TILE * TILE;
+                        lkd_task q = u;
+                        q.kbeg = 0;
+                        q.kend = kp;
+                        q.np = 0;
+                        q.px = px;
+                        q.nsplit = 0;
+                        q.prio = 0;
+                        q.part = 1;
+                        u.kbeg = kp;
+                        u.np = 1;
+                        u.px = px;
+                        px += LKD_PART;
+                        DagTile d = dag_meta[i];
+                        d.nsplit = false;
+                        d.part = true;
+                        d.after = kp / TILE - 1;
+                        dag_meta.push_back(d);
+                        dag_tasks.push_back(q);
+                    }
+                    work_elems = std::max(work_elems, px);
+                    // queue order: the tiles of step k, then the PART tasks queued after it
+                    std::vector<size_t> ord;
+                    for (int k = 0, kmax = dag_meta[nt - 1].k; k <= kmax; ++k) {
+                        for (size_t i = 0; i < nt; ++i)
+                            if (dag_meta[i].k == k) ord.push_back(i);
+                        for (size_t i = nt; i < dag_tasks.size(); ++i)
+                            if (dag_meta[i].after == k) ord.push_back(i);
+                    }
+                    if (ord.size() != dag_tasks.size()) throw std::logic_error("KKT plan: STEPS order");
+                    std::vector<lkd_task> t2;
+                    std::vector<DagTile> m2;
+                    for (size_t i : ord) {
+                        t2.push_back(dag_tasks[i]);
+                        m2.push_back(dag_meta[i]);
+                    }
+                    dag_tasks.swap(t2);
+                    dag_meta.swap(m2);
+                }
                 // a tile waits for the tiles of the previous step of its supernode whose rows
-                // overlap its rows, and for the one that factored its diagonal block
-                std::vector<std::vector<int32_t>> prev(lv.size()), cur(lv.size());
-                int cur_k = -1;
+                // overlap its rows, for the one that factored its diagonal block, and for its PART;
+                // a PART waits for the tiles of the step it follows that overlap its rows or the
+                // rows of its diagonal block
                 b0 = static_cast<int64_t>(tasks.size());
                 fdeps.resize(static_cast<size_t>(4 * b0), -1);
                 fsdep.resize(static_cast<size_t>(b0), -1);
+                std::vector<std::vector<int32_t>> tiles_of(lv.size());   // STEP tiles so far, per supernode
+                std::map<std::pair<size_t, int64_t>, int32_t> part_of;   // (supernode, px) -> PART
+                for (size_t i = 0; i < dag_tasks.size(); ++i)
+                    if (dag_meta[i].part) part_of[{dag_meta[i].si, dag_tasks[i].px}] = static_cast<int32_t>(i);
                 for (size_t i = 0; i < dag_tasks.size(); ++i) {
                     const DagTile& d = dag_meta[i];
-                    if (d.k != cur_k) {
-                        for (size_t si = 0; si < lv.size(); ++si) {
-                            prev[si].swap(cur[si]);
-                            cur[si].clear();
-                        }
-                        cur_k = d.k;
-                    }
+                    const lkd_task& u = dag_tasks[i];
                     std::vector<int32_t> dep;
                     int32_t sdep = -1;
-                    for (int32_t j : prev[d.si]) {
-                        const DagTile& e = dag_meta[static_cast<size_t>(j)];
-                        if (e.k != d.k - 1) continue;
-                        if (e.nsplit || (e.r0 < d.r1 && d.r0 < e.r1)) dep.push_back(j);
-                        if (e.nsplit) sdep = j;
+                    auto overl = [&](int step, int a0, int a1) {
+                        for (int32_t j : tiles_of[d.si]) {
+                            const DagTile& e = dag_meta[static_cast<size_t>(j)];
+                            if (e.k == step && e.r0 < a1 && a0 < e.r1 &&
+                                std::find(dep.begin(), dep.end(), j) == dep.end())
+                                dep.push_back(j);
+                        }
+                    };
+                    if (d.part) {
+                        overl(d.after, d.r0, d.r1);
+                        overl(d.after, u.k0, u.k0 + u.kb);
+                    } else {
+                        for (int32_t j : tiles_of[d.si]) {
+                            const DagTile& e = dag_meta[static_cast<size_t>(j)];
+                            if (e.k == d.k - 1 && e.nsplit) {
+                                dep.push_back(j);
+                                sdep = j;
+                            }
+                        }
+                        overl(d.k - 1, d.r0, d.r1);
+                        if (u.np > 0) dep.push_back(part_of.at({d.si, u.px}));
+                        tiles_of[d.si].push_back(static_cast<int32_t>(i));
                     }
                     fsdep.push_back(sdep);
-                    if (dep.size() > 4) throw std::logic_error("KKT plan: STEPS tile with more than 4 dependencies");
+                    if (dep.size() > 4) throw std::logic_error("KKT plan: STEPS task with more than 4 dependencies");
                     for (size_t q = 0; q < 4; ++q) fdeps.push_back(q < dep.size() ? dep[q] : -1);
                     if (d.nsplit)   // the chain of next-diagonal tiles: what it waits for is critical
                         for (int32_t j : dep) dag_tasks[static_cast<size_t>(j)].prio = 1;
-                    cur[d.si].push_back(static_cast<int32_t>(i));
                 }
                 // queues: every supernode's tiles on one XCD, supernodes dealt by work to the
                 // least loaded XCD; each queue keeps the list order
@@ -685,9 +751,17 @@ struct GPUKKTCholeskySolver::Impl {
         fqueue.resize(tasks.size(), -1);
         for (const auto& L : flaunch)
             for (int64_t q = L.begin; q < L.begin + L.count; ++q) {
-                validate(tasks[q], L.kind == 7 ? 5 : L.kind);
+                validate(tasks[q], L.kind == 7 ? (tasks[q].part ? 6 : 5) : L.kind);
                 if (L.kind != 7) continue;
-                if (tasks[q].np != 0) throw std::logic_error("KKT plan: STEPS tile with partial products");
+                const lkd_task& t = tasks[q];
+                bool has_part = t.np == 0;   // a tile with a product waits for its PART
+                for (int d = 0; d < 4 && !has_part; ++d) {
+                    const int32_t j = fdeps[4 * q + d];
+                    has_part = j >= 0 && tasks[L.begin + j].part && tasks[L.begin + j].px == t.px &&
+                               tasks[L.begin + j].kend == t.kbeg;
+                }
+                if (t.part ? t.np != 0 || t.nsplit || t.kbeg != 0 : t.np > 1 || !has_part)
+                    throw std::logic_error("KKT plan: STEPS tile with partial products");
                 for (int d = 0; d < 5; ++d) {   // earlier tiles of the same supernode
                     const int32_t j = d < 4 ? fdeps[4 * q + d] : fsdep[q];
                     if (j < -1 || j >= q - L.begin || (j >= 0 && tasks[L.begin + j].sn != tasks[q].sn))
@@ -1074,7 +1148,8 @@ struct GPUKKTCholeskySolver::Impl {
             case 7: return lkd_chol_step_dag(prec, L.count, t, d_fdeps + 4 * L.begin, d_fsdep + L.begin,
                                              d_fqueue + L.begin, d_fqoff + static_cast<int64_t>(L.q) * (n_xcd + 1),
                                              n_xcd, L.grid, d_dag_flags + L.begin, d_dag_flags2 + L.begin,
-                                             d_dag_ctl, L.q, d_panel, d_diag, d_piv, d_stage, d_err, stream);
+                                             d_dag_ctl, L.q, d_panel, d_diag, d_piv, d_stage, d_work, d_err,
+                                             stream);
             default: return lkd_chol_step(prec, L.count, t, d_panel, d_diag, d_piv, d_stage, d_work, d_err, stream);
         }
     }
@@ -1438,6 +1513,7 @@ std::size_t GPUKKTCholeskySolver::estimate_bytes(const core::CSRMatrix& A,
                           2 * (16 * nz + 8 * (m + n)) + 8 * nz + 8 * 24 * (m + n) +
                           128 * tasks + 8 * 4096 + static_cast<int64_t>(S.sn_rows.size()) * 4 +
                           8 * LKD_STAGE * slots + 8 * int64_t(STEP_TARGET_WG) * LKD_PART +
+                          8 * int64_t(LKD_PART) * 2 * slots +   // STEPS: two PART products per block column
                           24 * static_cast<int64_t>(S.nsuper) + 8 * m + 256 * 96;
     return static_cast<std::size_t>(bytes);
 }

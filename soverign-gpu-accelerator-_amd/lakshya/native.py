@@ -392,7 +392,7 @@ TASK_DTYPE = np.dtype([("off", "<i8"), ("aux", "<i8"), ("aux2", "<i8"), ("ws", "
                        ("ld", "<i4"), ("ncol", "<i4"), ("k0", "<i4"), ("kb", "<i4"),
                        ("r0", "<i4"), ("c0", "<i4"), ("kbeg", "<i4"), ("kend", "<i4"),
                        ("nsplit", "<i4"), ("sn", "<i4"), ("first", "<i4"), ("np", "<i4"),
-                       ("px", "<i8"), ("prio", "<i4"), ("pad", "<i4")])
+                       ("px", "<i8"), ("prio", "<i4"), ("part", "<i4")])
 PLAN_DTYPES = {"perm": np.int32, "sn_rowptr": np.int64, "sn_rows": np.int32,
                "sn_panel": np.int64, "col_to_sn": np.int32, "sn_first": np.int32,
                "flaunch": np.int64, "slaunch": np.int64, "diag_elems": np.int64,

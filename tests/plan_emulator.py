@@ -94,9 +94,11 @@ class Emulator:
         for kind, b, c in self.fl:
             assert kind in (1, 2, 3, 5, 6, 7)
             order = self.dag_order(b, c, rng) if kind == 7 else range(b, b + c)
-            kind = 5 if kind == 7 else kind
+            steps = kind == 7
             for q in order:
                 t = T[q]
+                if steps:                         # STEPS: PART tasks and STEP tiles
+                    kind = 6 if int(t["part"]) else 5
                 if self.err:
                     return False
                 k0, kb, r0, c0, ld, nc = (int(t[x]) for x in ("k0", "kb", "r0", "c0", "ld", "ncol"))

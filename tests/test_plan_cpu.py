@@ -18,11 +18,12 @@ def _cases():
         "transport": lambda: gen.transport(8, 8, 6, 3, seed=2),
         "dense": lambda: gen.dense_small(420, 900, 0.02, seed=5),
         "block_angular_wide": lambda: gen.block_angular(3, 150, 400, 6, 8, 0.2, seed=3),
+        "dense_wide": lambda: gen.dense_small(700, 1400, 0.02, seed=6),
     }
 
 
 @pytest.mark.parametrize("name", ["random_sparse", "block_angular", "transport", "dense",
-                                  "block_angular_wide"])
+                                  "block_angular_wide", "dense_wide"])
 def test_schedule_reproduces_dense_cholesky(native, name):
     lp = _cases()[name]()
     rp, ci, va = lp.csr()
@@ -88,8 +89,13 @@ def _tile_phases(t):
     staging).  Region: (space, key, lo0, hi0, lo1, hi1, writes)."""
     s, k0, kb, r0, ra = (int(t[x]) for x in ("sn", "k0", "kb", "r0", "c0"))
     nc, first, nsplit = int(t["ncol"]), int(t["first"]), int(t["nsplit"])
-    P = [("panel", s, 0, k0, r0, r0 + ra, False),             # L(R, 0:k0)
-         ("panel", s, 0, k0, k0, k0 + kb, False)]             # L(k, 0:k0)
+    kbeg, kend, px = int(t["kbeg"]), int(t["kend"]), int(t["px"])
+    P = [("panel", s, kbeg, kend, r0, r0 + ra, False),         # L(R, kbeg:kend)
+         ("panel", s, kbeg, kend, k0, k0 + kb, False)]         # L(k, kbeg:kend)
+    if int(t["part"]):                                         # PART: product into work
+        return [P + [("work", 0, px, px + 128 * 64, 0, 1, True)]]
+    if int(t["np"]):                                           # its PART's product
+        P.append(("work", 0, px, px + 128 * 64, 0, 1, False))
     S = [("panel", s, k0, k0 + kb, r0, r0 + ra, True),         # C(R, k) -> L(R, k)
          ("stage", int(t["ws"]), 0, 1, 0, 1, False)]
     D = []
@@ -107,7 +113,7 @@ def _tile_phases(t):
     return [P, S, B]
 
 
-@pytest.mark.parametrize("name", ["random_sparse", "dense", "block_angular_wide"])
+@pytest.mark.parametrize("name", ["random_sparse", "dense", "block_angular_wide", "dense_wide"])
 def test_steps_dependencies_order_every_conflict(native, name):
     """Inside a STEPS launch, two phases of tiles that touch the same data with at least one
     write must be ordered: by program order inside a tile, or through the hand-offs (a
@@ -133,11 +139,11 @@ def test_steps_dependencies_order_every_conflict(native, name):
                 a = 0
                 if p_ > 0:                          # program order
                     a |= anc[node[(i, p_ - 1)]] | (1 << node[(i, p_ - 1)])
-                if p_ == 0:                         # flags[j]: after j's S phase
+                if p_ == 0:                         # flags[j]: after j's S phase (a PART: its only one)
                     for j in deps[b + i]:
                         if j >= 0:
                             assert j < i
-                            m = node[(int(j), 1)]
+                            m = node[(int(j), min(1, len(ph[int(j)]) - 1))]
                             a |= anc[m] | (1 << m)
                 if p_ == 1 and sdep[b + i] >= 0:    # flags2[j]: after j's B phase
                     j = int(sdep[b + i])
