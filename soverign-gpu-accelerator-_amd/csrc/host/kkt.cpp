@@ -40,6 +40,7 @@ constexpr int KSPLIT_MIN = 128;     // minimum K columns per split part
 constexpr int STEP_TARGET_WG = 512; // a STEP launch with fewer tiles splits K (PART launch) ...
 constexpr int STEP_KMIN = 64;       // ... into parts of at least this many columns
 constexpr int STEPS_RECENT = 256;   // STEPS: K columns a chain tile computes itself
+constexpr int STEPS_SPLIT_CLS = 2;  // STEPS: tiles of class <= this (0 next-diagonal, 1 below it, 2 others) split
 constexpr int SOLVE_WIDE_TILES = 8;   // level solves: 8 wavefronts per supernode above this
 constexpr int SOLVE_MAX_G = 4;        // forward level solves: workgroups per supernode (all resident)
 constexpr int SOLVE_G_LEVEL = 4;      // ... only for levels of at most this many supernodes
@@ -587,16 +588,16 @@ struct GPUKKTCholeskySolver::Impl {
                 emit(5, b0);
             }
             if (!dag_tasks.empty()) {
-                // Tiles on the chain that limits the level (the next-diagonal tiles and the tiles
-                // right below them) leave all but their last STEPS_RECENT columns of K to a PART
-                // task queued right after the step that completes those columns, so that
-                // product is ready early and the chain only computes the recent columns.
+                // Tiles (of class <= STEPS_SPLIT_CLS) leave all but their last STEPS_RECENT
+                // columns of K to a PART task queued right after the step that completes those
+                // columns, so that product is ready early and the chains of tiles through the
+                // steps only compute the recent columns.
                 {
                     const size_t nt = dag_tasks.size();
                     int64_t px = 0;
                     for (size_t i = 0; i < nt; ++i) {
                         lkd_task& u = dag_tasks[i];
-                        if (dag_meta[i].cls > 1 || u.k0 < STEPS_RECENT + 2 * TILE) continue;
+                        if (dag_meta[i].cls > STEPS_SPLIT_CLS || u.k0 < STEPS_RECENT + 2 * TILE) continue;
                         const int kp = (u.k0 - STEPS_RECENT) / TILE * TILE;
                         lkd_task q = u;
                         q.kbeg = 0;
@@ -1500,20 +1501,21 @@ std::size_t GPUKKTCholeskySolver::estimate_bytes(const core::CSRMatrix& A,
     for (int64_t p = 0; p < nz; ++p) cnt[A.col_indices[p]]++;
     int64_t pairs = 0;
     for (int64_t c : cnt) pairs += c * (c + 1) / 2;
-    int64_t diag = 0, tasks = 0, slots = 0;
+    int64_t diag = 0, tasks = 0, slots = 0, step_tiles = 0;
     for (Index s = 0; s < S.nsuper; ++s) {
         const int64_t nc = S.sn_first[s + 1] - S.sn_first[s];
         const int64_t ld = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
         diag += nc * 64;
         slots += std::max<int64_t>(2, (nc + 63) / 64);   // staging slots, all levels at once
         const int64_t bc = (nc + 63) / 64, br = (ld + 63) / 64;
+        step_tiles += bc * (br + 2);
         tasks += 4 + bc * (br + 4) + bc * br * (br + 1) / 2 + br * br;
     }
     const int64_t bytes = 8 * S.sn_panel[S.nsuper] + 8 * diag + 20 * pairs + 36 * pairs +
                           2 * (16 * nz + 8 * (m + n)) + 8 * nz + 8 * 24 * (m + n) +
                           128 * tasks + 8 * 4096 + static_cast<int64_t>(S.sn_rows.size()) * 4 +
                           8 * LKD_STAGE * slots + 8 * int64_t(STEP_TARGET_WG) * LKD_PART +
-                          8 * int64_t(LKD_PART) * 2 * slots +   // STEPS: two PART products per block column
+                          8 * int64_t(LKD_PART) * step_tiles +   // STEPS: PART products
                           24 * static_cast<int64_t>(S.nsuper) + 8 * m + 256 * 96;
     return static_cast<std::size_t>(bytes);
 }
