@@ -230,6 +230,7 @@ int lkd_kkt_update(int32_t m, const double* D, const double* z, double* dy, doub
  * (src/cuda/kkt.cu:110-123, computed without storing M0). */
 #define LKD_KS_SLOTS 16
 #define LKD_KS_PART 16
+#define LKD_MAX_NBLK 4096   /* blocks of a reducing pass: partials need q * LKD_MAX_NBLK doubles */
 int lkd_kkt_residual(int32_t m, int32_t n, const int32_t* colptr, const int32_t* prow,
                      const double* cval, const int32_t* prp, const int32_t* pci,
                      const double* pval, const double* theta, const double* dy,

@@ -58,6 +58,8 @@ int finalize(const double* part, int nblk, int q, const int* ops, const int* slo
     return lk_last_error();
 }
 
+constexpr int LONG_U = 8;
+
 // Sparse row pass y_i = sum_p val[p] x[ci[p]] with the load balanced: rows longer than
 // LKD_LONG_ROW entries (listed in `longs`, e.g. the linking rows of a block-angular matrix) take
 // a whole block each, in a fixed thread-stride + tree order; the others take G lanes.  f(i, y_i)
@@ -70,8 +72,20 @@ __device__ __forceinline__ void for_rows(int64_t m, const int32_t* __restrict__ 
                                          const int32_t* __restrict__ longs, double* sh, F&& f) {
     for (int32_t k = blockIdx.x; k < nlong; k += gridDim.x) {
         const int32_t i = longs[k];
+        const int32_t e = rp[i + 1];
         double s = 0.0;
-        for (int32_t p = rp[i] + threadIdx.x; p < rp[i + 1]; p += NT) s += val[p] * x[ci[p]];
+        // LONG_U entries per thread and round, their loads all in flight together; summed in
+        // entry order
+        for (int32_t p = rp[i] + threadIdx.x; p < e; p += NT * LONG_U) {
+            double v[LONG_U];
+#pragma unroll
+            for (int u = 0; u < LONG_U; ++u) {
+                const int32_t q = p + u * NT < e ? p + u * NT : e - 1;
+                v[u] = p + u * NT < e ? val[q] * x[ci[q]] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < LONG_U; ++u) s += v[u];
+        }
         s = block_reduce(s, OP_SUM, sh);
         if (threadIdx.x == 0) f((int64_t)i, s);
     }
@@ -563,6 +577,14 @@ __global__ __launch_bounds__(NT) void k_muaff(lkd_ipm P, const double* __restric
 }
 
 // All backtracking trials in one pass (rp_t = rp + ap_t A dx, rd_t = rd + ad_t (A^T dy + ds)).
+// Per trial, every wavefront reduces its lanes' values at once (fixed butterfly) and lane 0
+// folds them into the wavefront's LDS row; one barrier at the end combines the four rows in a
+// fixed order.  No per-thread arrays indexed by the trial.
+__device__ __forceinline__ void trial_fold(double* row, int q, double v, int op, bool first) {
+    v = op == OP_SUM ? wave_sum(v) : wave_min(v);
+    if ((threadIdx.x & 63) == 0) row[q] = first ? v : op_apply(op, row[q], v);
+}
+
 __global__ __launch_bounds__(NT) void k_trials_cols(lkd_ipm P, const double* __restrict__ x,
                                                     const double* __restrict__ s,
                                                     const double* __restrict__ dx,
@@ -570,29 +592,34 @@ __global__ __launch_bounds__(NT) void k_trials_cols(lkd_ipm P, const double* __r
                                                     const double* __restrict__ rd,
                                                     const double* __restrict__ qd, double ap,
                                                     double ad, int nt) {
-    __shared__ double sh[4];
-    double rd2[LKD_MAX_TRIALS], xs[LKD_MAX_TRIALS], mx[LKD_MAX_TRIALS], ms[LKD_MAX_TRIALS];
-    for (int t = 0; t < nt; ++t) { rd2[t] = 0; xs[t] = 0; mx[t] = 1e30; ms[t] = 1e30; }
-    for (int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x; j < P.n; j += (int64_t)gridDim.x * NT) {
-        const double xj = x[j], sj = s[j], dxj = dx[j], dsj = ds[j], rdj = rd[j], qdj = qd[j];
+    __shared__ double acc[4][4 * LKD_MAX_TRIALS];
+    double* row = acc[threadIdx.x >> 6];
+    bool first = true;
+    const int64_t stride = (int64_t)gridDim.x * NT;
+    for (int64_t j0 = blockIdx.x * (int64_t)NT; j0 < P.n; j0 += stride) {   // uniform per block
+        const int64_t j = j0 + threadIdx.x;
+        const bool ok = j < P.n;
+        const int64_t jc = ok ? j : P.n - 1;
+        const double xj = x[jc], sj = s[jc], dxj = dx[jc], dsj = ds[jc], rdj = rd[jc], qdj = qd[jc];
         double a = ap, b = ad;
         for (int t = 0; t < nt; ++t) {
             const double tx = xj + a * dxj, ts = sj + b * dsj, tr = rdj + b * qdj;
-            rd2[t] += tr * tr;
-            xs[t] += tx * ts;
-            mx[t] = fmin(mx[t], tx);
-            ms[t] = fmin(ms[t], ts);
+            trial_fold(row, 4 * t + 0, ok ? tr * tr : 0.0, OP_SUM, first);
+            trial_fold(row, 4 * t + 1, ok ? tx * ts : 0.0, OP_SUM, first);
+            trial_fold(row, 4 * t + 2, ok ? tx : 1e30, OP_MIN, first);
+            trial_fold(row, 4 * t + 3, ok ? ts : 1e30, OP_MIN, first);
             a *= 0.5;
             b *= 0.5;
         }
+        first = false;
     }
+    __syncthreads();
     const int nb = gridDim.x;
-    for (int t = 0; t < nt; ++t) {
-        double r;
-        r = block_reduce(rd2[t], OP_SUM, sh); if (threadIdx.x == 0) P.part[(4 * t + 0) * nb + blockIdx.x] = r;
-        r = block_reduce(xs[t], OP_SUM, sh); if (threadIdx.x == 0) P.part[(4 * t + 1) * nb + blockIdx.x] = r;
-        r = block_reduce(mx[t], OP_MIN, sh); if (threadIdx.x == 0) P.part[(4 * t + 2) * nb + blockIdx.x] = r;
-        r = block_reduce(ms[t], OP_MIN, sh); if (threadIdx.x == 0) P.part[(4 * t + 3) * nb + blockIdx.x] = r;
+    for (int q = threadIdx.x; q < 4 * nt; q += NT) {
+        const int op = (q & 3) < 2 ? OP_SUM : OP_MIN;
+        const double r = first ? op_ident(op)
+                               : op_apply(op, op_apply(op, acc[0][q], acc[1][q]), op_apply(op, acc[2][q], acc[3][q]));
+        P.part[(int64_t)q * nb + blockIdx.x] = r;
     }
 }
 
@@ -601,26 +628,31 @@ __global__ __launch_bounds__(NT) void k_trials_rows(lkd_ipm P, const double* __r
                                                     const double* __restrict__ y,
                                                     const double* __restrict__ dy, double ap,
                                                     double ad, int nt, int qoff) {
-    __shared__ double sh[4];
-    double rp2[LKD_MAX_TRIALS], y2[LKD_MAX_TRIALS];
-    for (int t = 0; t < nt; ++t) { rp2[t] = 0; y2[t] = 0; }
-    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < P.m; i += (int64_t)gridDim.x * NT) {
-        const double rpi = rp[i], qpi = qp[i], yi = y[i], dyi = dy[i];
+    __shared__ double acc[4][2 * LKD_MAX_TRIALS];
+    double* row = acc[threadIdx.x >> 6];
+    bool first = true;
+    const int64_t stride = (int64_t)gridDim.x * NT;
+    for (int64_t i0 = blockIdx.x * (int64_t)NT; i0 < P.m; i0 += stride) {   // uniform per block
+        const int64_t i = i0 + threadIdx.x;
+        const bool ok = i < P.m;
+        const int64_t ic = ok ? i : P.m - 1;
+        const double rpi = rp[ic], qpi = qp[ic], yi = y[ic], dyi = dy[ic];
         double a = ap, b = ad;
         for (int t = 0; t < nt; ++t) {
             const double tr = rpi + a * qpi, ty = yi + b * dyi;
-            rp2[t] += tr * tr;
-            y2[t] += ty * ty;
+            trial_fold(row, 2 * t + 0, ok ? tr * tr : 0.0, OP_SUM, first);
+            trial_fold(row, 2 * t + 1, ok ? ty * ty : 0.0, OP_SUM, first);
             a *= 0.5;
             b *= 0.5;
         }
+        first = false;
     }
+    __syncthreads();
     const int nb = gridDim.x;
     double* part = P.part + (int64_t)qoff * nb;
-    for (int t = 0; t < nt; ++t) {
-        double r;
-        r = block_reduce(rp2[t], OP_SUM, sh); if (threadIdx.x == 0) part[(2 * t + 0) * nb + blockIdx.x] = r;
-        r = block_reduce(y2[t], OP_SUM, sh); if (threadIdx.x == 0) part[(2 * t + 1) * nb + blockIdx.x] = r;
+    for (int q = threadIdx.x; q < 2 * nt; q += NT) {
+        const double r = first ? 0.0 : (acc[0][q] + acc[1][q]) + (acc[2][q] + acc[3][q]);
+        part[(int64_t)q * nb + blockIdx.x] = r;
     }
 }
 
@@ -750,7 +782,7 @@ int lkd_kkt_residual(int32_t m, int32_t n, const int32_t* colptr, const int32_t*
                      const double* D, double* req, double* kS, lkd_long long_rows,
                      lkd_long long_cols, void* stream) {
     LK_LAUNCH(k_res_cols<4>, grid_for(4 * (int64_t)n), NT, 0, stream, n, colptr, prow, cval, theta, dy, u, long_cols);
-    const int nblk = grid_for(m, 1024);
+    const int nblk = grid_for(8 * (int64_t)m, LKD_MAX_NBLK);   // 8 lanes per row
     double* part = kS + LKD_KS_PART;
     l_res_rows<8>(m, prp, pci, pval, u, rhs_perm, dy, m0d, r, D, req, part, nblk, long_rows, stream);
     int ops[4] = {OP_MAX, OP_MAX, OP_MAX, OP_MAX}, slots[4] = {0, 1, 2, 3};
@@ -763,7 +795,7 @@ int lkd_pcg_matvec(int32_t m, int32_t n, const int32_t* colptr, const int32_t* p
                    double* q, double* kS, int32_t slot_pq, lkd_long long_rows, lkd_long long_cols,
                    void* stream) {
     LK_LAUNCH(k_res_cols<4>, grid_for(4 * (int64_t)n), NT, 0, stream, n, colptr, prow, cval, theta, p, u, long_cols);
-    const int nblk = grid_for(m, 1024);
+    const int nblk = grid_for(8 * (int64_t)m, LKD_MAX_NBLK);   // 8 lanes per row
     double* part = kS + LKD_KS_PART;
     l_pcg_rows<8>(m, prp, pci, pval, u, p, q, part, nblk, long_rows, stream);
     int op = OP_SUM;
@@ -773,7 +805,7 @@ int lkd_pcg_matvec(int32_t m, int32_t n, const int32_t* colptr, const int32_t* p
 int lkd_pcg_update(int32_t m, double* kS, int32_t slot_rz, int32_t slot_pq, const double* p,
                    const double* q, double* dy, double* r, const double* D, double* req,
                    void* stream) {
-    const int nblk = grid_for(m, 1024);
+    const int nblk = grid_for(m, LKD_MAX_NBLK);
     double* part = kS + LKD_KS_PART;
     LK_LAUNCH(k_pcg_update, nblk, NT, 0, stream, m, kS, slot_rz, slot_pq, p, q, dy, r, D, req, part);
     int ops[2] = {OP_MAX, OP_MAX}, slots[2] = {4, 2};
@@ -782,7 +814,7 @@ int lkd_pcg_update(int32_t m, double* kS, int32_t slot_rz, int32_t slot_pq, cons
 
 int lkd_pcg_precdot(int32_t m, const double* D, double* z, const double* r, double* kS,
                     int32_t slot_rz, void* stream) {
-    const int nblk = grid_for(m, 1024);
+    const int nblk = grid_for(m, LKD_MAX_NBLK);
     double* part = kS + LKD_KS_PART;
     LK_LAUNCH(k_pcg_precdot, nblk, NT, 0, stream, m, D, z, r, part);
     int op = OP_SUM;

@@ -39,7 +39,7 @@ constexpr int GEMM_TARGET_WG = 1024;   // UPDATE K splitting aims at this many w
 constexpr int KSPLIT_MIN = 128;     // minimum K columns per split part
 constexpr int STEP_TARGET_WG = 512; // a STEP launch with fewer tiles splits K (PART launch) ...
 constexpr int STEP_KMIN = 64;       // ... into parts of at least this many columns
-constexpr int STEPS_RECENT = 256;   // STEPS: K columns a chain tile computes itself
+constexpr int STEPS_RECENT = 192;   // STEPS: K columns a chain tile computes itself
 constexpr int STEPS_SPLIT_CLS = 2;  // STEPS: tiles of class <= this (0 next-diagonal, 1 below it, 2 others) split
 constexpr int SOLVE_WIDE_TILES = 8;   // level solves: 8 wavefronts per supernode above this
 constexpr int SOLVE_MAX_G = 4;        // forward level solves: workgroups per supernode (all resident)
@@ -1020,7 +1020,7 @@ struct GPUKKTCholeskySolver::Impl {
         d_bak = dv(m);
         d_rhsp = dv(m);
         d_u = dv(n);
-        d_rinf = dv(LKD_KS_PART + 8 * 1024);
+        d_rinf = dv(LKD_KS_PART + 8 * LKD_MAX_NBLK);
         d_pcg = dv(4 * static_cast<int64_t>(m));
         if (nnz_csr > 0) d_csr = dv(nnz_csr);
         std::vector<int32_t> sn_first(S.sn_first.begin(), S.sn_first.end());

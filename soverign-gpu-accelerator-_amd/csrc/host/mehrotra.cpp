@@ -89,11 +89,11 @@ public:
         compute_supernodal_structure(A_, P, SupernodeOptions{}, S);
         const size_t kkt_bytes = gpu::GPUKKTCholeskySolver::estimate_bytes(A_, S);
         const int64_t nz = A_.row_ptrs[m_];
-        const int nb_rows = 1024, nb_cols = 1024;
+        const int nb_rows = LKD_MAX_NBLK, nb_cols = LKD_MAX_NBLK;
         const size_t ipm_bytes = sizeof(double) * (20 * static_cast<size_t>(m_ + n_) +
                                                    2 * static_cast<size_t>(nz)) +
                                  sizeof(int32_t) * (2 * static_cast<size_t>(nz) + m_ + n_ + 2) +
-                                 sizeof(double) * (LKS_TOTAL + 256 * 1024) + (1u << 20);
+                                 sizeof(double) * (LKS_TOTAL + 256 * LKD_MAX_NBLK) + (1u << 20);
         arena_.reset(new gpu::VRAMArena(kkt_bytes + ipm_bytes + (64u << 20)));
         check(lkd_stream_create(&stream_), "stream");
         gpu::KKTOptions ko;
@@ -169,7 +169,7 @@ public:
         auto dv = [&](int64_t len) {
             return static_cast<double*>(arena_->allocate(static_cast<size_t>(std::max<int64_t>(len, 1)) * sizeof(double)));
         };
-        part_ = dv(256 * 1024);
+        part_ = dv(256 * LKD_MAX_NBLK);   // up to 6 LKD_MAX_TRIALS + 1 quantities per pass
         S_ = dv(LKS_TOTAL);
         P_.part = part_;
         P_.S = S_;
