@@ -482,18 +482,6 @@ __device__ __forceinline__ void step_tile(const lkd_task& tk, T* __restrict__ L,
 #pragma unroll
                 for (int e = 0; e < 4; ++e) acc[p][b][e] += in[((p * 2 + b) * 4 + e) * 256];
     }
-    if (ds && ds->sdep >= 0) {                        // the staged factor of block k
-        if (t == 0) *s_ok = flag_wait(ds->flags2 + ds->sdep, ds->ep, err) ? 1 : 0;
-        __syncthreads();
-        if (!*s_ok) return;
-        asm volatile("buffer_inv sc0" ::: "memory");
-    }
-    T* S = reinterpret_cast<T*>(&sm);
-    {
-        const T* Sg = stage + tk.ws;
-#pragma unroll 4
-        for (int idx = t; idx < STAGE; idx += 256) S[idx] = Sg[idx];
-    }
     int64_t row[2];
     bool valid[2];
     Acc<T> y[2][4];
@@ -520,6 +508,20 @@ __device__ __forceinline__ void step_tile(const lkd_task& tk, T* __restrict__ L,
                     y[b][p][e] = P[(int64_t)(k0 + c) * ld + rc] - acc[p][b][e];
                 }
         }
+    }
+    // the staged factor of block k (in a STEPS launch: once its tile has published it); the
+    // tile's own loads above are already in flight
+    if (ds && ds->sdep >= 0) {
+        if (t == 0) *s_ok = flag_wait(ds->flags2 + ds->sdep, ds->ep, err) ? 1 : 0;
+        __syncthreads();
+        if (!*s_ok) return;
+        asm volatile("buffer_inv sc0" ::: "memory");
+    }
+    T* S = reinterpret_cast<T*>(&sm);
+    {
+        const T* Sg = stage + tk.ws;
+#pragma unroll 4
+        for (int idx = t; idx < STAGE; idx += 256) S[idx] = Sg[idx];
     }
     __syncthreads();
 #pragma unroll
@@ -567,6 +569,15 @@ __device__ __forceinline__ void step_tile(const lkd_task& tk, T* __restrict__ L,
         if (64 * h >= ra || d0 >= tk.ncol) break;         // uniform
         const int db = min(64, min(ra - 64 * h, tk.ncol - d0));
         T* Ys = S;
+        // this wave's stripe of C(d, d) (clamped addresses), in flight during the product
+        T cold[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = min(16 * w + R::row(l, e), db - 1), c = min(16 * q + li, db - 1);
+                cold[q][e] = q <= w ? P[(int64_t)(d0 + c) * ld + d0 + r] : T(0);
+            }
         __syncthreads();                                  // the staged factor / last stripe is read
         if ((w >> 1) == h) {
 #pragma unroll
@@ -599,7 +610,7 @@ __device__ __forceinline__ void step_tile(const lkd_task& tk, T* __restrict__ L,
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int r = 16 * w + R::row(l, e), c = 16 * q + li;
-                if (r < db && c <= r) P[(int64_t)(d0 + c) * ld + d0 + r] -= dq[q][e];
+                if (r < db && c <= r) P[(int64_t)(d0 + c) * ld + d0 + r] = cold[q][e] - dq[q][e];
             }
         }
     }

@@ -40,6 +40,7 @@ constexpr int KSPLIT_MIN = 128;     // minimum K columns per split part
 constexpr int STEP_TARGET_WG = 512; // a STEP launch with fewer tiles splits K (PART launch) ...
 constexpr int STEP_KMIN = 64;       // ... into parts of at least this many columns
 constexpr int STEPS_RECENT = 192;   // STEPS: K columns a chain tile computes itself
+constexpr int STEPS_PART_LAG = 1;   // STEPS: steps between a PART's inputs and its queue place
 constexpr int STEPS_SPLIT_CLS = 2;  // STEPS: tiles of class <= this (0 next-diagonal, 1 below it, 2 others) split
 constexpr int SOLVE_WIDE_TILES = 8;   // level solves: 8 wavefronts per supernode above this
 constexpr int SOLVE_MAX_G = 4;        // forward level solves: workgroups per supernode (all resident)
@@ -463,8 +464,9 @@ struct GPUKKTCholeskySolver::Impl {
             };
             stage_elems = std::max<int64_t>(stage_elems, blkbase[lv.size()] * LKD_STAGE);
             // STEPS: the tiles of all steps in step order, with, per tile, the step and rows
-            // after: PART tasks, the step after whose tiles they are queued
-            struct DagTile { size_t si; int k; int r0, r1; bool nsplit; int cls; bool part; int after; };
+            // PART tasks: dstep = the step whose tiles complete their K range, after = the step
+            // after whose tiles they are queued (dstep + STEPS_PART_LAG, before their tile's step)
+            struct DagTile { size_t si; int k; int r0, r1; bool nsplit; int cls; bool part; int after, dstep; };
             std::vector<lkd_task> dag_tasks;
             std::vector<DagTile> dag_meta;
             for (Index s : lv) {
@@ -535,7 +537,7 @@ struct GPUKKTCholeskySolver::Impl {
                             u.np = 0;
                             u.px = 0;
                             dag_meta.push_back({static_cast<size_t>(sidx[u.sn]), k, u.r0, u.r0 + u.c0,
-                                                u.nsplit != 0, cls[i], false, k});
+                                                u.nsplit != 0, cls[i], false, k, k});
                             dag_tasks.push_back(u);
                         }
                     continue;
@@ -614,7 +616,8 @@ struct GPUKKTCholeskySolver::Impl {
                         DagTile d = dag_meta[i];
                         d.nsplit = false;
                         d.part = true;
-                        d.after = kp / TILE - 1;
+                        d.dstep = kp / TILE - 1;
+                        d.after = std::min(d.k - 1, d.dstep + STEPS_PART_LAG);
                         dag_meta.push_back(d);
                         dag_tasks.push_back(q);
                     }
@@ -662,8 +665,8 @@ struct GPUKKTCholeskySolver::Impl {
                         }
                     };
                     if (d.part) {
-                        overl(d.after, d.r0, d.r1);
-                        overl(d.after, u.k0, u.k0 + u.kb);
+                        overl(d.dstep, d.r0, d.r1);
+                        overl(d.dstep, u.k0, u.k0 + u.kb);
                     } else {
                         for (int32_t j : tiles_of[d.si]) {
                             const DagTile& e = dag_meta[static_cast<size_t>(j)];
