@@ -111,13 +111,11 @@ constexpr int SEG_LANES = 16;
 __device__ __forceinline__ double seg_short(int64_t b, int64_t e, const double* __restrict__ pp,
                                             const int32_t* __restrict__ pk,
                                             const double* __restrict__ theta) {
+    // lanes past their segment's end issue no loads (most segments hold a single pair)
     double acc = 0.0;
 #pragma unroll
-    for (int u = 0; u < LKD_SHORT_SEG; ++u) {       // clamped loads, masked product: no branch
-        const int64_t q = b + u < e ? b + u : e - 1;
-        const double v = b + u < e ? pp[q] : 0.0;
-        acc = fma(v, theta[pk[q]], acc);
-    }
+    for (int u = 0; u < LKD_SHORT_SEG; ++u)
+        if (b + u < e) acc = fma(pp[b + u], theta[pk[b + u]], acc);
     return acc;
 }
 
@@ -206,16 +204,19 @@ __global__ __launch_bounds__(NT) void k_segs(int64_t count, const int64_t* __res
         }
         return;
     }
-    const int64_t nbs = gridDim.x - nbl - nbm;
-    for (int64_t r = (bx - nbl - nbm) * (int64_t)NT + threadIdx.x; r < count; r += nbs * NT) {
-        const int64_t t = seg_of(r);
-        if (t < 0) {                        // empty row (DIAG only)
-            seg_store<DIAG>(o, r, 0.0);
-            continue;
-        }
-        const int64_t b = segptr[t], e = segptr[t + 1];
-        if (e - b > LKD_SHORT_SEG) continue;
-        seg_store<DIAG>(o, r, seg_short(b, e, pp, pk, theta));
+    // two items per thread and round, written out so that both items' loads are in flight
+    const int64_t stride = (int64_t)(gridDim.x - nbl - nbm) * NT;
+    for (int64_t r0 = (bx - nbl - nbm) * (int64_t)NT + threadIdx.x; r0 < count; r0 += 2 * stride) {
+        const int64_t r1 = r0 + stride;
+        const bool has1 = r1 < count;
+        const int64_t t0 = seg_of(r0), t1 = has1 ? seg_of(r1) : -1;
+        const int64_t b0 = t0 >= 0 ? segptr[t0] : 0, e0 = t0 >= 0 ? segptr[t0 + 1] : 0;
+        const int64_t b1 = t1 >= 0 ? segptr[t1] : 0, e1 = t1 >= 0 ? segptr[t1 + 1] : 0;
+        const bool s0 = t0 >= 0 && e0 - b0 <= LKD_SHORT_SEG, s1 = t1 >= 0 && e1 - b1 <= LKD_SHORT_SEG;
+        const double v0 = s0 ? seg_short(b0, e0, pp, pk, theta) : 0.0;
+        const double v1 = s1 ? seg_short(b1, e1, pp, pk, theta) : 0.0;
+        if (s0 || t0 < 0) seg_store<DIAG>(o, r0, v0);      // t < 0: empty row (DIAG only)
+        if (has1 && (s1 || t1 < 0)) seg_store<DIAG>(o, r1, v1);
     }
 }
 
