@@ -40,6 +40,8 @@ void check(int e, const char* what) {
         throw std::runtime_error(std::string("HIP call failed (") + what + "): " +
                                  lkd_error_string(e));
 }
+constexpr double LANE_DIV = 4.0;   // lanes per row of a sparse pass: mean entries / LANE_DIV
+
 int pow2_group(double avg) {
     int g = 1;
     while (g < 64 && g < avg) g <<= 1;
@@ -162,8 +164,8 @@ public:
         P_.c = up_d(c_);
         P_.R = up_d(R_);
         P_.C = up_d(C_);
-        P_.grp_rows = pow2_group(mean_row);
-        P_.grp_cols = pow2_group(mean_col);
+        P_.grp_rows = pow2_group(mean_row / LANE_DIV);
+        P_.grp_cols = pow2_group(mean_col / LANE_DIV);
         P_.nblk_rows = std::max(1, std::min<int>(nb_rows, static_cast<int>((static_cast<int64_t>(m_) * P_.grp_rows + 255) / 256)));
         P_.nblk_cols = std::max(1, std::min<int>(nb_cols, static_cast<int>((static_cast<int64_t>(n_) * P_.grp_cols + 255) / 256)));
         auto dv = [&](int64_t len) {
