@@ -201,20 +201,25 @@ int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
 /* Replaces cusparseSpSV (L and L^T) (src/cuda/kkt.cu:731-734,788-789).  x is the permuted
  * vector, solved in place: L z = x (dir 0, levels leaves first) or L^T x = z (dir 1, root
  * first), one launch per etree level, g = task.c0 workgroups of nw (4 or 8) wavefronts per
- * supernode (task.r0 = workgroup index; g = 1, or 4 for forward launches with nw = 8, when every
- * workgroup of the launch must be resident at once).
+ * supernode (task.r0 = workgroup index; g = 1, or 4 for forward launches with nw = 8).  With
+ * g = 4, grid workgroups each read the XCD they run on (XCC_ID mod nx) and take their task from
+ * that XCD's queue, queue[qoff[x] .. qoff[x+1]) (task indices; a supernode's g tasks
+ * consecutive), through counter cnt[x]; grid >= nx times the longest queue.  The workgroups of
+ * a supernode then share an L2 for their hand-offs.
  * task.aux = offset of the supernode's first diagonal copy, task.aux2 = row-list offset,
  * task.kbeg = offset of its rows below in `below` (scratch, one double per row below a
  * supernode), task.ws = offset of its area in `pbuf` (nb * G * nw * 64 doubles, nb = 64-column
  * blocks of the supernode: forward z exchange, backward partial sums), task.px = offset of its
  * nb + nb * G flags in `flags`.  *epoch is the solve number, advanced by lkd_trsv_epoch before
- * every solve (flags hold it, so they are never reset).  Waits are bounded: a timeout sets err
- * bit 0x10 and the result is invalid. */
-int lkd_trsv_epoch(uint32_t* epoch, void* stream);
+ * every solve (flags hold it, so they are never reset); lkd_trsv_epoch also zeroes the ncnt
+ * queue counters cnt.  Waits are bounded: a timeout sets err bit 0x10 and the result is
+ * invalid. */
+int lkd_trsv_epoch(uint32_t* epoch, uint32_t* cnt, int32_t ncnt, void* stream);
 int lkd_trsv_level(int prec, int dir, int nw, int g, int32_t ntask, const lkd_task* tasks,
                    const void* panel, const void* diag, const int32_t* sn_rows, double* x,
                    double* below, double* pbuf, uint32_t* flags, const uint32_t* epoch,
-                   int32_t* err, void* stream);
+                   const int32_t* queue, const int32_t* qoff, int32_t nx, uint32_t* cnt,
+                   int32_t grid, int32_t* err, void* stream);
 
 /* ---------------------------------------------------------------- KKT solve vector kernels */
 /* r[i] = rhs[P[i]]; req[i] = D[i] * r[i] */

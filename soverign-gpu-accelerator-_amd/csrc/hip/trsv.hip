@@ -215,11 +215,13 @@ struct Own {
     __device__ __forceinline__ int wg_of(int t) const { return (t % nwt) / NW; }
 };
 
-// Device-scope hand-off between the workgroups of one supernode: data first, then a release
-// at agent scope, then the flag (relaxed); the reader spins on the flag with relaxed loads and
-// acquires once.  Flags hold the solve's epoch, so they never need resetting.
+// Hand-off between the workgroups of one supernode, which all run on one XCD (they take their
+// tasks from that XCD's queue, see xcd_task): the writing wavefront waits until its data
+// stores have reached the shared L2, then sets the flag (relaxed); the reader spins on the flag
+// and invalidates its compute unit's L1 before reading.  Flags hold the solve's epoch, so they
+// never need resetting.
 __device__ __forceinline__ void dev_publish(uint32_t* flag, uint32_t ep) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if ((threadIdx.x & 63) == 0) __hip_atomic_store(flag, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ bool dev_wait(const uint32_t* flag, uint32_t ep, int32_t* err) {
@@ -231,8 +233,31 @@ __device__ __forceinline__ bool dev_wait(const uint32_t* flag, uint32_t ep, int3
         }
         __builtin_amdgcn_s_sleep(2);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    asm volatile("buffer_inv sc0" ::: "memory");
     return true;
+}
+
+// Task of this workgroup in a launch with several workgroups per supernode: the next ticket of
+// the queue of the XCD it runs on (XCC_ID mod nx), queue[qoff[x] .. qoff[x+1]); -1 when that
+// queue is empty.  A supernode's workgroups are consecutive in one queue, and tickets go to
+// running workgroups in order, so the earliest unfinished supernode has all its workgroups
+// running (the XCD holds at least G of them at once).
+struct XcdQueue {
+    const int32_t* queue;
+    const int32_t* qoff;
+    uint32_t* cnt;
+    int nx;
+};
+__device__ __forceinline__ int xcd_task(const XcdQueue& q, int* s_ti) {
+    if (threadIdx.x == 0) {
+        unsigned xcc;
+        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+        const int x = (int)(xcc % (unsigned)q.nx);
+        const int t = (int)atomicAdd(q.cnt + x, 1u);
+        *s_ti = t < q.qoff[x + 1] - q.qoff[x] ? q.queue[q.qoff[x] + t] : -1;
+    }
+    __syncthreads();
+    return *s_ti;
 }
 
 // Ring slot of block jb may be (re)written once every wavefront of the workgroup finished
@@ -281,8 +306,11 @@ __global__ __launch_bounds__(64 * NW) void k_fwd_sn(const lkd_task* __restrict__
                                                     double* __restrict__ x,
                                                     double* __restrict__ below, double* zbuf,
                                                     uint32_t* flags, const uint32_t* epoch,
-                                                    int32_t* err) {
-    const lkd_task tk = tasks[blockIdx.x];
+                                                    int32_t* err, XcdQueue xq) {
+    __shared__ int s_ti;
+    const int ti = G > 1 ? xcd_task(xq, &s_ti) : (int)blockIdx.x;
+    if (ti < 0) return;
+    const lkd_task tk = tasks[ti];
     __shared__ double zs[ZR * TS];                    // ring of solved blocks (zero past kb)
     __shared__ double Ds[2][TS * 65];                 // staged diagonal blocks (block jb: jb & 1)
     __shared__ int done;                              // blocks available in the ring
@@ -504,30 +532,33 @@ __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__
 template <typename T, int NW, int G>
 int level_solve(int dir, int32_t ntask, const lkd_task* tasks, const void* panel, const void* diag,
                 const int32_t* rows, double* x, double* below, double* pbuf, uint32_t* flags,
-                const uint32_t* epoch, int32_t* err, void* stream) {
+                const uint32_t* epoch, int32_t* err, const XcdQueue& xq, int grid, void* stream) {
     if (dir == 0)
-        LK_LAUNCH((k_fwd_sn<T, NW, G>), ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
-                  (const T*)diag, rows, x, below, pbuf, flags, epoch, err);
+        LK_LAUNCH((k_fwd_sn<T, NW, G>), G > 1 ? grid : ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
+                  (const T*)diag, rows, x, below, pbuf, flags, epoch, err, xq);
     else
         LK_LAUNCH((k_bwd_sn<T, NW, G>), ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
                   (const T*)diag, rows, x, below, pbuf, flags, epoch, err);
     return lk_last_error();
 }
 
-__global__ void k_epoch(uint32_t* epoch) {
+// a new solve: the epoch advances and the queue counters restart
+__global__ void k_epoch(uint32_t* epoch, uint32_t* cnt, int ncnt) {
     if (threadIdx.x == 0) atomicAdd(epoch, 1u);
+    for (int i = threadIdx.x; i < ncnt; i += blockDim.x) cnt[i] = 0u;
 }
 
 // wavefronts per workgroup 4 or 8, workgroups per supernode 1 or 4 (the backward uses 1)
 template <typename T>
 int level_solve_cfg(int nw, int g, int dir, int32_t ntask, const lkd_task* tasks, const void* panel,
                     const void* diag, const int32_t* rows, double* x, double* below, double* pbuf,
-                    uint32_t* flags, const uint32_t* epoch, int32_t* err, void* stream) {
+                    uint32_t* flags, const uint32_t* epoch, int32_t* err, const XcdQueue& xq,
+                    int grid, void* stream) {
     if (nw == 8 && g == 4 && dir == 0)
-        return level_solve<T, 8, 4>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, stream);
+        return level_solve<T, 8, 4>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, xq, grid, stream);
     if (nw == 8)
-        return level_solve<T, 8, 1>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, stream);
-    return level_solve<T, 4, 1>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, stream);
+        return level_solve<T, 8, 1>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, xq, grid, stream);
+    return level_solve<T, 4, 1>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, xq, grid, stream);
 }
 
 }  // namespace
@@ -537,16 +568,20 @@ extern "C" {
 int lkd_trsv_level(int prec, int dir, int nw, int g, int32_t ntask, const lkd_task* tasks,
                    const void* panel, const void* diag, const int32_t* sn_rows, double* x,
                    double* below, double* pbuf, uint32_t* flags, const uint32_t* epoch,
-                   int32_t* err, void* stream) {
+                   const int32_t* queue, const int32_t* qoff, int32_t nx, uint32_t* cnt,
+                   int32_t grid, int32_t* err, void* stream) {
     if (ntask <= 0) return 0;
     if ((nw != 4 && nw != 8) || (g != 1 && !(g == 4 && nw == 8 && dir == 0))) return (int)hipErrorInvalidValue;
+    if (g > 1 && (!queue || !qoff || !cnt || nx < 1 || grid < 1)) return (int)hipErrorInvalidValue;
+    const XcdQueue xq{queue, qoff, cnt, nx};
     return prec == LKD_F32
-               ? level_solve_cfg<float>(nw, g, dir, ntask, tasks, panel, diag, sn_rows, x, below, pbuf, flags, epoch, err, stream)
-               : level_solve_cfg<double>(nw, g, dir, ntask, tasks, panel, diag, sn_rows, x, below, pbuf, flags, epoch, err, stream);
+               ? level_solve_cfg<float>(nw, g, dir, ntask, tasks, panel, diag, sn_rows, x, below, pbuf, flags, epoch, err, xq, grid, stream)
+               : level_solve_cfg<double>(nw, g, dir, ntask, tasks, panel, diag, sn_rows, x, below, pbuf, flags, epoch, err, xq, grid, stream);
 }
 
-int lkd_trsv_epoch(uint32_t* epoch, void* stream) {
-    LK_LAUNCH(k_epoch, 1, 64, 0, stream, epoch);
+int lkd_trsv_epoch(uint32_t* epoch, uint32_t* cnt, int32_t ncnt, void* stream) {
+    if (ncnt < 0) return (int)hipErrorInvalidValue;
+    LK_LAUNCH(k_epoch, 1, 256, 0, stream, epoch, cnt, ncnt);
     return lk_last_error();
 }
 

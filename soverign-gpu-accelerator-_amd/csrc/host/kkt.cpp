@@ -43,8 +43,7 @@ constexpr int STEPS_RECENT = 192;   // STEPS: K columns a chain tile computes it
 constexpr int STEPS_PART_LAG = 1;   // STEPS: steps between a PART's inputs and its queue place
 constexpr int STEPS_SPLIT_CLS = 2;  // STEPS: tiles of class <= this (0 next-diagonal, 1 below it, 2 others) split
 constexpr int SOLVE_WIDE_TILES = 8;   // level solves: 8 wavefronts per supernode above this
-constexpr int SOLVE_MAX_G = 4;        // forward level solves: workgroups per supernode (all resident)
-constexpr int SOLVE_G_LEVEL = 4;      // ... only for levels of at most this many supernodes
+constexpr int SOLVE_MAX_G = 4;        // forward level solves: workgroups per wide supernode
 constexpr int FWD_ROWS = 256;
 
 // Order a launch's tasks for dispatch.  Workgroup b runs task (b % 8) * per + b / 8 with
@@ -879,11 +878,10 @@ struct GPUKKTCholeskySolver::Impl {
 
     // Solve plan: per etree level one launch per direction.  Wavefronts per workgroup: 8 when
     // some supernode of the level has more than SOLVE_WIDE_TILES row tiles, else 4.  Forward
-    // launches of wide levels with at most SOLVE_G_LEVEL supernodes spread each supernode over
-    // G <= SOLVE_MAX_G workgroups (all workgroups of the launch fit on the compute units at once:
-    // they wait on each other).  Device-scope hand-offs cost an L2 write-back / invalidate each,
-    // which outweighs the extra bandwidth when many supernodes run side by side, and the
-    // backward direction needs one cross-workgroup reduction per block: those keep G = 1.
+    // launches of wide levels whose G = SOLVE_MAX_G workgroups per supernode all fit on the
+    // compute units (one each) spread every supernode over G workgroups; they take their tasks
+    // from per-XCD queues, so a supernode's workgroups share an L2 and hand off without L2
+    // write-backs.  The backward direction keeps G = 1.
     // task.nsplit = wavefronts, c0 = G, r0 = g, ws = exchange offset, px = flag offset,
     // kbeg = scratch offset.
     void build_solve_plan() {
@@ -901,12 +899,12 @@ struct GPUKKTCholeskySolver::Impl {
                 maxnt = std::max(maxnt, nt);
                 if (nt > SOLVE_WIDE_TILES) nw = 8;
             }
+            // forward: SOLVE_MAX_G workgroups per wide supernode when the level's workgroups all
+            // fit on the compute units (one each); kernels exist for G = 1 and 4
             int G = 1;
-            if (nw == 8 && cus > 0 && static_cast<int64_t>(lv.size()) <= SOLVE_G_LEVEL) {
-                const int64_t fit = cus / static_cast<int64_t>(lv.size());
-                G = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({SOLVE_MAX_G, fit, (maxnt + nw - 1) / nw})));
-                if (G != SOLVE_MAX_G) G = 1;                      // kernels exist for G = 1 and 4
-            }
+            if (nw == 8 && cus > 0 && static_cast<int64_t>(lv.size()) * SOLVE_MAX_G <= cus &&
+                maxnt >= 2 * nw)
+                G = SOLVE_MAX_G;
             std::vector<lkd_task> lt, lb;
             for (const Index s : lv) {
                 lkd_task t{};
@@ -941,19 +939,62 @@ struct GPUKKTCholeskySolver::Impl {
         auto emit = [&](int kind, const std::vector<lkd_task>& lt) {
             slaunch.push_back({kind, static_cast<int64_t>(tasks.size()), static_cast<int32_t>(lt.size()),
                                lt.empty() ? 4 : lt[0].nsplit, lt.empty() ? 1 : lt[0].c0});
+            const int G = slaunch.back().g;
+            squeue.resize(tasks.size(), -1);
+            if (G > 1) {   // XCD queues: supernodes dealt in turn, each one's G tasks together
+                Launch& L = slaunch.back();
+                L.q = n_sq++;
+                const int nsn = static_cast<int>(lt.size()) / G;
+                int32_t longest = 0;
+                for (int x = 0; x < n_xcd; ++x) {
+                    sqoff.push_back(static_cast<int32_t>(squeue.size() - tasks.size()));
+                    int32_t len = 0;
+                    for (int sn = x; sn < nsn; sn += n_xcd)
+                        for (int g = 0; g < G; ++g, ++len) squeue.push_back(sn * G + g);
+                    longest = std::max(longest, len);
+                }
+                sqoff.push_back(static_cast<int32_t>(lt.size()));
+                L.grid = n_xcd * longest;
+            }
             tasks.insert(tasks.end(), lt.begin(), lt.end());
+            squeue.resize(tasks.size(), -1);
         };
         for (const auto& lt : per_level) emit(20, lt);                                       // forward
         for (auto it = per_level_bwd.rbegin(); it != per_level_bwd.rend(); ++it) emit(21, *it);   // backward
         solve_pbuf = npbuf;
         solve_flags = nflags;
         solve_below = nbelow;
-        for (const auto& L : slaunch)
+        for (const auto& L : slaunch) {
             for (int64_t q = L.begin; q < L.begin + L.count; ++q) validate(tasks[q], L.kind);
+            if (L.g == 1) continue;
+            // queues: every task once; a supernode's tasks consecutive, in one queue, g in order
+            const int32_t* off = sqoff.data() + static_cast<size_t>(L.q) * (n_xcd + 1);
+            std::vector<char> seen(static_cast<size_t>(L.count), 0);
+            int32_t longest = 0;
+            for (int x = 0; x < n_xcd; ++x) {
+                longest = std::max(longest, off[x + 1] - off[x]);
+                for (int32_t p = off[x]; p < off[x + 1]; ++p) {
+                    const int32_t i = squeue[static_cast<size_t>(L.begin + p)];
+                    if (i < 0 || i >= L.count || seen[i]) throw std::logic_error("KKT plan: invalid solve queue");
+                    seen[i] = 1;
+                    const lkd_task& t = tasks[L.begin + i];
+                    if (t.c0 != L.g || t.r0 != (p - off[x]) % L.g ||
+                        (t.r0 > 0 && tasks[L.begin + squeue[L.begin + p - 1]].sn != t.sn))
+                        throw std::logic_error("KKT plan: solve queue splits a supernode");
+                }
+            }
+            if (off[0] != 0 || off[n_xcd] != L.count || L.grid < n_xcd * longest)
+                throw std::logic_error("KKT plan: invalid solve queue offsets");
+        }
         d_stasks = up(tasks, "stasks");
+        d_squeue = up(squeue, "squeue");
+        d_sqoff = up(sqoff, "sqoff");
         n_stasks = static_cast<int64_t>(tasks.size());
     }
     int64_t solve_pbuf = 0, solve_below = 0, solve_flags = 0;
+    std::vector<int32_t> squeue, sqoff;    // XCD queues of the multi-workgroup level solves
+    int32_t *d_squeue = nullptr, *d_sqoff = nullptr;
+    int32_t n_sq = 0;                      // level solves with XCD queues
 
     // reference CSR layout of L (sym.L_pattern rows + diagonal last) -> panel indices
     void build_csr_index(const ipm::SymbolicFactorization& sym) {
@@ -1003,8 +1044,8 @@ struct GPUKKTCholeskySolver::Impl {
         owned.push_back(d_below);
         d_pbuf = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(solve_pbuf, 1)) * sizeof(double)));
         owned.push_back(d_pbuf);
-        d_flags = static_cast<uint32_t*>(arena_p->allocate(static_cast<size_t>(solve_flags + 1) * sizeof(uint32_t)));
-        owned.push_back(d_flags);   // flags [0, solve_flags), epoch [solve_flags]; zero-filled
+        d_flags = static_cast<uint32_t*>(arena_p->allocate(static_cast<size_t>(solve_flags + 1 + n_sq * n_xcd) * sizeof(uint32_t)));
+        owned.push_back(d_flags);   // flags [0, solve_flags), epoch [solve_flags], queue counters; zero-filled
         auto dv = [&](int64_t len) {
             double* p = static_cast<double*>(arena_p->allocate(static_cast<size_t>(std::max<int64_t>(len, 1)) * sizeof(double)));
             owned.push_back(p);
@@ -1239,11 +1280,15 @@ struct GPUKKTCholeskySolver::Impl {
 
     void run_solve_schedule(int prec, double* x) {
         uint32_t* epoch = d_flags + solve_flags;
-        check(lkd_trsv_epoch(epoch, stream), "solve epoch");
+        check(lkd_trsv_epoch(epoch, epoch + 1, n_sq * n_xcd, stream), "solve epoch");
         for (const auto& L : slaunch) {
             const int dir = L.kind == 20 ? 0 : 1;
+            const bool q = L.g > 1;
             check(lkd_trsv_level(prec, dir, L.nw, L.g, L.count, d_stasks + L.begin, d_panel, d_diag, d_sn_rows,
-                                 x, d_below, d_pbuf, d_flags, epoch, d_err + 1, stream), "solve kernel");
+                                 x, d_below, d_pbuf, d_flags, epoch, q ? d_squeue + L.begin : nullptr,
+                                 q ? d_sqoff + static_cast<int64_t>(L.q) * (n_xcd + 1) : nullptr, n_xcd,
+                                 q ? epoch + 1 + static_cast<int64_t>(L.q) * n_xcd : nullptr, L.grid,
+                                 d_err + 1, stream), "solve kernel");
         }
     }
 

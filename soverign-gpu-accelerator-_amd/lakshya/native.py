@@ -404,7 +404,8 @@ PLAN_DTYPES = {"perm": np.int32, "sn_rowptr": np.int64, "sn_rows": np.int32,
                "long_cols": np.int32, "long_segs": np.int32, "long_diag": np.int32,
                "mid_segs": np.int32, "mid_diag": np.int32,
                "chunk_seg": np.int64, "fdeps": np.int32, "fsdep": np.int32,
-               "fqueue": np.int32, "fqoff": np.int32, "n_xcd": np.int32}
+               "fqueue": np.int32, "fqoff": np.int32, "n_xcd": np.int32,
+               "squeue": np.int32, "sqoff": np.int32}
 
 
 def plan_export(rp, ci, val, n, P=None, relax_small=-1, relax_frac=-1.0):

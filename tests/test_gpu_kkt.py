@@ -126,3 +126,26 @@ def test_direct_solve_multi_supernode(gpu, oracle, idx, mode):
     # the factor is of D M0 D + 1e-9 I: the direct solve is accurate to ~1e-9 * cond(D M0 D)
     assert rel <= (1e-7 if mode == 1 else 1e-4), rel
     kk.close(); arena.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_direct_solve_wide_supernodes(gpu, mode):
+    """Wide supernodes (about 26 row tiles each): the forward level solve spreads every
+    supernode over four workgroups of one XCD.  Residual of the direct solve with sparse
+    products, M0 x = A (theta o (A^T x))."""
+    import scipy.sparse as sp
+    from lakshya import gen
+    lp = gen.block_angular(4, 1500, 4000, 8, 50, 0.05, seed=0)
+    rp, ci, va = lp.csr()
+    m, n = lp.m, lp.n
+    A = sp.csr_matrix((va, ci, rp), shape=(m, n))
+    theta = np.exp(np.random.default_rng(3).uniform(-2, 2, n))
+    arena = gpu.Arena(2 << 30)
+    kk = gpu.KKT(arena, rp, ci, va, n, reference_symbolic=False)
+    assert kk.factorize(theta, mode=mode)
+    rhs = np.random.default_rng(6).standard_normal(m)
+    for _ in range(3):                    # repeated solves: flags and queues of every solve
+        x, info = kk.solve(rhs, tol=1e-300, max_refine=0)
+        rel = np.abs(A @ (theta * (A.T @ x)) - rhs).max() / np.abs(rhs).max()
+        assert rel <= (1e-7 if mode == 1 else 1e-4), rel
+    kk.close(); arena.close()
