@@ -8,13 +8,13 @@ set up once (Ruiz scaling, AMD ordering, supernodal plan and data resident in HB
 timed region).  Every timed solve is checked: status Optimal, the 1e-8 criteria, the FP64
 certificate, and the objective against the HiGHS golden value when one is recorded.
 
-roofline: the dominant kernel, k_step (one launch = one 64-column block-column step of the
-supernodal FP64 factorization, MFMA-bound).  Its algorithmic flops per factorization are
+roofline: the dominant kernel, k_step_dag (one launch = every 64-column block-column STEP tile
+and its split-off partial products for one etree level of the supernodal FP64 factorization,
+run in dependency order; MFMA-bound).  Its algorithmic flops per factorization are
 F_chol = sum_j c_j^2 (exact column counts of L) minus the panel flops of the other launch kinds
-(first diagonal blocks, ancestor updates, split-K partial products); achieved = those flops x factorizations timed / the
+(first diagonal blocks, ancestor updates); achieved = those flops x factorizations timed / the
 summed device time of its launches, measured live over the timed solves with HIP events on the
-solver's stream at every change of launch kind (runs of consecutive k_step launches are timed
-as a whole, so the short gaps between them count against the kernel).  traffic: HBM bytes per k_step launch from
+solver's stream at every change of launch kind.  traffic: HBM bytes per k_step_dag launch from
 the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), scaled by the same counters
 on a copy kernel of known size.
 cpu_baseline: the C oracle (same algorithm, dense FP64 Cholesky, 1 thread) on one block of the
@@ -115,7 +115,7 @@ def step_roofline(live, st, fp32):
     fac = live["factorization"]
     return {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": traffic,
-            "kernel": "k_step (supernodal block-column step: MFMA GEMM + solve + next diagonal)",
+            "kernel": "k_step_dag (all block-column steps of a level, dependency-driven: MFMA GEMM + solve + next diagonal)",
             "launches": step["launches"], "avg_launch_ms": per_launch,
             "algorithmic_flops_per_launch": f_step * nfact / max(step["launches"], 1),
             "stored_flops_per_launch": step["flops"] / max(step["launches"], 1),

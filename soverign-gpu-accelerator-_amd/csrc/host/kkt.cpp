@@ -1254,8 +1254,10 @@ struct GPUKKTCholeskySolver::Impl {
         const double kb = t.kb;
         switch (kind) {
             case 6: return 2.0 * t.c0 * kb * (t.kend - t.kbeg);
-            case 5:
-            case 7: {   // GEMM + solve of the tile; diagonal-block updates; next block's factor
+            case 7:
+                if (t.part) return 2.0 * t.c0 * kb * (t.kend - t.kbeg);
+                [[fallthrough]];
+            case 5: {   // GEMM + solve of the tile; diagonal-block updates; next block's factor
                 const double ra = t.c0;
                 double f = 2.0 * ra * kb * (t.kend - t.kbeg) + ra * kb * kb;
                 for (int h = 0; h < 2; ++h) {
