@@ -35,7 +35,7 @@ void check(int e, const char* what) {
 constexpr int TILE = 64;           // block-column width (diagonal blocks, substitution blocks)
 constexpr int GEMM_MT = 128;        // COLUPD / UPDATE tile rows
 constexpr int GEMM_NC = 64;         // COLUPD / UPDATE tile columns
-constexpr int GEMM_TARGET_WG = 1024;   // UPDATE K splitting aims at this many workgroups
+constexpr int GEMM_TARGET_WG = 256;   // UPDATE K splitting aims at this many workgroups
 constexpr int KSPLIT_MIN = 128;     // minimum K columns per split part
 constexpr int STEP_TARGET_WG = 512; // a STEP launch with fewer tiles splits K (PART launch) ...
 constexpr int STEP_KMIN = 64;       // ... into parts of at least this many columns
