@@ -165,7 +165,6 @@ def main():
     ap.add_argument("--blocks", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--graphs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (traffic-counter reference)")
@@ -188,8 +187,7 @@ def main():
     log(f"workload {name}: m={lp.m} n={lp.n} nnz={lp.nnz} golden={gold}")
 
     t = time.perf_counter()
-    solver = native.IPM(lp, relative=True, precision=1 if args.precision == "mixed" else 0,
-                        use_graphs=args.graphs)
+    solver = native.IPM(lp, relative=True, precision=1 if args.precision == "mixed" else 0)
     setup_ms = (time.perf_counter() - t) * 1e3
     st = solver.kkt_stats()
     log(f"setup {setup_ms:.0f} ms; supernodes={int(st['nsuper'])} nnz(L)={st['nnz_L']:.3e} "

@@ -35,7 +35,7 @@ enum class FactorPrecision { FP64 = 0, FP32 = 1 };
 
 struct KKTOptions {
     ipm::SupernodeOptions supernodes;
-    bool use_graphs = false;        // capture factorization / solve launch sequences in hipGraphs
+    bool use_graphs = false;        // launch-graph capture: not supported (true is rejected)
     bool step_dag = true;           // all STEP tiles of a level in one dependency-driven launch
     void* stream = nullptr;         // hipStream_t to run on (nullptr: the solver creates one)
     // factor_equilibrated regularization: delta_r = delta0 * 10^r, r < delta_retries

@@ -57,7 +57,7 @@ struct IpmOptions {
     Float mixed_fp64_spread = 1e12;   // max(Theta)/min(Theta) beyond which FP64 factors are used
     int mixed_max_refine = 60;        // PCG budget with an FP32 factor before the FP64 retry
     Ordering ordering = Ordering::Amd;
-    bool use_graphs = false;
+    bool use_graphs = false;          // launch-graph capture: not supported (true is rejected)
     bool verbose = false;
     const Float* warm_x = nullptr;    // optional original-space warm start (all three or none)
     const Float* warm_y = nullptr;

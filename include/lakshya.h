@@ -75,7 +75,7 @@ int lk_plan_destroy(lk_plan* p);
  * rule is used and L is exposed in the reference CSR layout by lk_kkt_get_L_csr. */
 int lk_kkt_create(lk_arena* arena, int32_t m, int32_t n, const int32_t* rowptr,
                   const int32_t* colidx, const double* vals, const int32_t* P,
-                  int32_t reference_symbolic, int32_t use_graphs, lk_kkt** out);
+                  int32_t reference_symbolic, int32_t use_graphs, lk_kkt** out);   /* use_graphs must be 0 */
 /* mode 0: gpu_cholesky_factorize (M0 + 1e-9 I, unequilibrated; *not_spd set instead of throwing)
  * mode 1: equilibrated FP64 factor (delta retries), mode 2: equilibrated FP32 factor.
  * theta: n values, device pointer when on_device != 0. */
@@ -105,7 +105,7 @@ typedef struct {
     int32_t mixed_max_refine;
     double mixed_fp64_spread;
     int32_t ordering;           /* 0 AMD, 1 reference minimum degree */
-    int32_t use_graphs;
+    int32_t use_graphs;         /* must be 0: launch-graph capture is not supported */
     int32_t verbose;
     int32_t pad;
     double warm_floor;

@@ -45,10 +45,6 @@ int lkd_event_create(void** ev);
 int lkd_event_destroy(void* ev);
 int lkd_event_record(void* ev, void* stream);
 int lkd_event_elapsed_ms(float* ms, void* start, void* stop);
-int lkd_graph_begin(void* stream);
-int lkd_graph_end(void* stream, void** exec);
-int lkd_graph_launch(void* exec, void* stream);
-int lkd_graph_destroy(void* exec);
 int lkd_fill_f64(double* dst, double value, int64_t n, void* stream);   /* kernel fill */
 /* streaming copy kernel (n even, 16-byte aligned): calibrates HBM traffic counters */
 int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream);
@@ -184,8 +180,7 @@ int lkd_chol_step(int prec, int32_t ntask, const lkd_task* tasks, void* panel, v
  * (its product over kbeg..kend into work + px, then flags[task]); a tile with np = 1 adds the
  * product at work + px (its PART is among its dependencies).  Staging slots must not be
  * shared by two block columns of the level.  lkd_chol_dag_begin(ctl, nq) starts a
- * factorization: ctl[0] += 1 and the nq counters ctl[1 .. nq] = 0 (a kernel, so it can be
- * captured in a graph). */
+ * factorization: ctl[0] += 1 and the nq counters ctl[1 .. nq] = 0. */
 int lkd_chol_step_dag(int prec, int32_t ntask, const lkd_task* tasks, const int32_t* deps,
                       const int32_t* sdeps, const int32_t* queue, const int32_t* qoff,
                       int32_t nx, int32_t grid, uint32_t* flags, uint32_t* flags2, uint32_t* ctl,

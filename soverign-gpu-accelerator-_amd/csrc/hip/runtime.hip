@@ -73,23 +73,6 @@ int lkd_event_elapsed_ms(float* ms, void* start, void* stop) {
     if (e) return e;
     return (int)hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop);
 }
-int lkd_graph_begin(void* stream) {
-    return (int)hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal);
-}
-int lkd_graph_end(void* stream, void** exec) {
-    hipGraph_t g;
-    int e = (int)hipStreamEndCapture((hipStream_t)stream, &g);
-    if (e) return e;
-    hipGraphExec_t x;
-    e = (int)hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    *exec = x;
-    return e;
-}
-int lkd_graph_launch(void* exec, void* stream) {
-    return (int)hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)stream);
-}
-int lkd_graph_destroy(void* exec) { return exec ? (int)hipGraphExecDestroy((hipGraphExec_t)exec) : 0; }
 int lkd_fill_f64(double* dst, double value, int64_t n, void* stream) {
     if (n <= 0) return 0;
     const int64_t blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;

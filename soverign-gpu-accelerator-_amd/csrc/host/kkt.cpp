@@ -164,13 +164,13 @@ struct GPUKKTCholeskySolver::Impl {
     FactorPrecision cached_prec = FactorPrecision::FP64;
     double cached_delta = 0;
     int factor_kind = 0;           // 0 none, 1 unequilibrated (reference factorize), 2 equilibrated
-    void* fgraph[2] = {nullptr, nullptr};   // factor graph per precision
-    void* sgraph[2] = {nullptr, nullptr};   // solve graph per precision (on d_z)
 
     Impl(VRAMArena* a, const core::CSRMatrix& A, const ipm::SymbolicFactorization* sym,
          ipm::SupernodalStructure* pre, const KKTOptions& o,
          std::map<std::string, std::vector<char>>* keep_host = nullptr)
         : arena_p(a), keep(keep_host), m(A.rows), n(A.cols), opt(o) {
+        if (opt.use_graphs)
+            throw std::invalid_argument("GPUKKTCholeskySolver: launch-graph capture is not supported");
         auto t0 = std::chrono::steady_clock::now();
         stream = opt.stream;
         if (!stream && arena_p) {
@@ -236,8 +236,6 @@ struct GPUKKTCholeskySolver::Impl {
 
     ~Impl() {
         for (int p = 0; p < 2; ++p) {
-            if (fgraph[p]) lkd_graph_destroy(fgraph[p]);
-            if (sgraph[p]) lkd_graph_destroy(sgraph[p]);
         }
         if (stream) lkd_stream_sync(stream);
         for (void* e : timer.events) lkd_event_destroy(e);
@@ -1305,30 +1303,10 @@ struct GPUKKTCholeskySolver::Impl {
     }
 
     void trsv(int prec, double* x) {
-        if (opt.use_graphs && x == d_z) {
-            void*& g = sgraph[prec];
-            if (!g) {
-                check(lkd_graph_begin(stream), "graph begin");
-                run_solve_schedule(prec, x);
-                check(lkd_graph_end(stream, &g), "graph end");
-            }
-            check(lkd_graph_launch(g, stream), "graph launch");
-            return;
-        }
         run_solve_schedule(prec, x);
     }
 
     void factor_numeric(int prec) {
-        if (opt.use_graphs) {
-            void*& g = fgraph[prec];
-            if (!g) {
-                check(lkd_graph_begin(stream), "graph begin");
-                run_factor_schedule(prec);
-                check(lkd_graph_end(stream, &g), "graph end");
-            }
-            check(lkd_graph_launch(g, stream), "graph launch");
-            return;
-        }
         run_factor_schedule(prec);
     }
 

@@ -149,3 +149,11 @@ def test_direct_solve_wide_supernodes(gpu, mode):
         rel = np.abs(A @ (theta * (A.T @ x)) - rhs).max() / np.abs(rhs).max()
         assert rel <= (1e-7 if mode == 1 else 1e-4), rel
     kk.close(); arena.close()
+
+
+def test_graph_capture_refused(gpu):
+    """Launch-graph capture is not offered: asking for it is an invalid argument."""
+    arena = gpu.Arena(1 << 20)
+    with pytest.raises(ValueError):
+        gpu.KKT(arena, RP, CI, VA, 5, use_graphs=True)
+    arena.close()
