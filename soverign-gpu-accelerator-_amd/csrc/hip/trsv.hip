@@ -97,10 +97,11 @@ __device__ __forceinline__ double bsub_regs(double r, const T (&col)[TS], double
 //            (diagonal tiles in x, rows below in a scratch vector that is added to the
 //            ancestors' entries with atomics at the end).  A ring slot is reused only after
 //            every wavefront of the workgroup finished the iteration that read it (`prog`).
-//  backward  for j = nb-1 .. 0: every wavefront forms part_v = sum over its tiles t > j of
-//            L(t, j)^T x_t (LDS-transposed CW-column chunks, fixed reduction order), stores it
-//            in `pbuf` and publishes; the owner of j waits for all parts, sums them in
-//            wavefront order (deterministic), solves its diagonal block and keeps x_j.
+//  backward  (G = 1) for j = nb-1 .. 0: every wavefront forms part_v = sum over its tiles
+//            t > j of L(t, j)^T x_t (LDS-transposed CW-column chunks, fixed reduction order),
+//            stores it in `pbuf` (the owner of j + 1: in LDS) and publishes; the owner of j
+//            waits for all parts, sums them in wavefront order (deterministic), solves its
+//            diagonal block and keeps x_j.
 // Task fields: aux = diagonal-copy offset of block 0 (block j at aux + 4096 j), aux2 = row-list
 // offset, kbeg = offset of the rows below in `below`, r0 = g, c0 = G, ws = offset of the
 // supernode's area in `pbuf`, px = offset of its flags.  Waits are bounded; a timeout sets
@@ -379,47 +380,47 @@ __global__ __launch_bounds__(64 * NW) void k_fwd_sn(const lkd_task* __restrict__
     }
 }
 
-// Backward level solve.  Task: r0 = g, c0 = G, ws = partial sums in pbuf (nb * G * NW * 64
-// doubles: [block][global wavefront][64]), px = flag area (forward: nb flags, then backward:
-// nb * G flags, one per block and workgroup).  In a workgroup that does not own block j, the
-// last wavefront to store its part (an LDS counter per block) publishes the workgroup's parts.
-template <typename T, int NW, int G>
+// Backward level solve, one workgroup per supernode (task r0 = 0, c0 = 1).  ws = partial sums
+// in pbuf (nb * NW * 64 doubles: [block][wavefront][64]).  Block j's owner (wavefront j % NW)
+// sums the parts of all wavefronts in wavefront order.  The part of the owner of block j + 1
+// is on the critical path (it needs x_{j+1}): that wavefront computes everything else early
+// (look-ahead), keeps x_{j+1} in registers after its solve, has the first half of tile
+// (j + 1, j) in registers already (FP32 panels; FP64 ones have no registers to spare), and hands
+// the part over in LDS (`cp`, two slots: slot j & 1
+// is rewritten only after block j + 2's owner read it, which precedes the solve of j + 1).
+// The other parts go through pbuf.
+template <typename T, int NW>
 __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__ tasks,
                                                     const T* __restrict__ L,
                                                     const T* __restrict__ diag,
                                                     const int32_t* __restrict__ sn_rows,
                                                     double* __restrict__ x,
                                                     double* __restrict__ below,
-                                                    double* pbuf, uint32_t* flags,
-                                                    const uint32_t* epoch, int32_t* err) {
+                                                    double* pbuf, int32_t* err) {
     constexpr int CW = 16;                            // columns per transposed chunk
     constexpr int NQ = TS / CW;
-    constexpr int NC = 64;                            // part counters (blocks in flight < NC)
     const lkd_task tk = tasks[blockIdx.x];
     __shared__ double Ts[NW][CW * 65];                // per wavefront: CW columns x 64 rows
+    __shared__ double cp[2][TS];                      // critical-path parts, slot j & 1
     __shared__ int pub[NW];                           // lowest block each wavefront published
-    __shared__ int cnt[NC];                           // parts stored per block (mod NC)
     const T* P = L + tk.off;
     const int64_t ld = tk.ld;
     const Tiles tl(ld, tk.ncol);
     const int nb = tl.nb, nt = tl.nt, nc = tk.ncol;
     const int v = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const Own<G, NW> own{tk.r0 * NW + v, tk.r0};
-    const uint32_t ep = G > 1 ? *epoch : 0u;
-    uint32_t* fl = flags + tk.px + nb;                // [block][workgroup]
+    const Own<1, NW> own{v, 0};
     const int cl = lane % CW, rg = lane / CW;         // column of a chunk, row group
     double* bl = below + tk.kbeg;
-    double* pb = pbuf + tk.ws;                        // [nb][G * NW][64]
+    double* pb = pbuf + tk.ws;                        // [nb][NW][64]
     double* Tw = Ts[v];
-    for (int t = own.first(nb); t < nt; t += own.nwt) {
+    for (int t = own.first(nb); t < nt; t += NW) {
         const int64_t row = tl.beg(t) + lane;
         if (row < tl.end(t)) bl[row - nc] = x[sn_rows[tk.aux2 + row]];
     }
     if (lane == 0) pub[v] = nb;
-    for (int i = threadIdx.x; i < NC; i += 64 * NW) cnt[i] = 0;
     __syncthreads();
     // part[q] (lanes cl): += sum over the rows of tile t of L(t, jb)^T x_t, columns CW q + cl,
-    // for the tiles t = t0, t0 + G NW, ... < tlim; software-pipelined by halves (32 columns): the
+    // for the tiles t = t0, t0 + NW, ... < tlim; software-pipelined by halves (32 columns): the
     // next half's loads are in flight while the current one goes through the LDS transpose.
     auto chunk = [&](double& acc, const T* h, double xr) {
 #pragma unroll
@@ -437,64 +438,74 @@ __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
+    auto rowp = [&](const T* Pb, int u) { return Pb + min(tl.beg(u) + lane, tl.end(u) - 1); };
+    auto rok = [&](int u) { return tl.beg(u) + lane < tl.end(u); };
     auto contrib_list = [&, tl](double (&part)[NQ], int t0, int jb, int tlim) {
         if (t0 >= tlim) return;
         const int kb = min(TS, nc - jb * TS);
         const T* Pb = P + (int64_t)jb * TS * ld;
-        auto rowp = [&](int u) { return Pb + min(tl.beg(u) + lane, tl.end(u) - 1); };
-        auto rok = [&](int u) { return tl.beg(u) + lane < tl.end(u); };
         auto xval = [&](int u) {
             const int64_t r = tl.beg(u) + lane;
             return r < tl.end(u) ? (u < nb ? x[tk.first + r] : bl[r - nc]) : 0.0;
         };
         T h0[TS / 2], h1[TS / 2];
-        load_half<T>(h0, rowp(t0), ld, 0, kb, rok(t0));
-        load_half<T>(h1, rowp(t0), ld, TS / 2, kb, rok(t0));
+        load_half<T>(h0, rowp(Pb, t0), ld, 0, kb, rok(t0));
+        load_half<T>(h1, rowp(Pb, t0), ld, TS / 2, kb, rok(t0));
 #pragma unroll 1
-        for (int t = t0; t < tlim; t += own.nwt) {
-            const int tn = t + own.nwt < tlim ? t + own.nwt : t;    // past the end: reload, unused
+        for (int t = t0; t < tlim; t += NW) {
+            const int tn = t + NW < tlim ? t + NW : t;    // past the end: reload, unused
             const double xr = xval(t);
             chunk(part[0], h0, xr);
             chunk(part[1], h0 + CW, xr);
-            load_half<T>(h0, rowp(tn), ld, 0, kb, rok(tn));
+            load_half<T>(h0, rowp(Pb, tn), ld, 0, kb, rok(tn));
             chunk(part[2], h1, xr);
             chunk(part[3], h1 + CW, xr);
-            load_half<T>(h1, rowp(tn), ld, TS / 2, kb, rok(tn));
+            load_half<T>(h1, rowp(Pb, tn), ld, TS / 2, kb, rok(tn));
         }
     };
-    double pre[NQ];          // look-ahead contributions to block pre_blk (tiles t > pre_blk + 1)
+    double pre[NQ];          // look-ahead part for block pre_blk (tiles t > pre_blk + 1)
     int pre_blk = -1;
+    double xlast = 0.0;      // x of the block this wavefront solved last (lane = row; 0 past kb)
+    constexpr bool kEarly = sizeof(T) == 4;
+    T hc[TS / 2];            // first half of tile (j, j - 1), loaded before the solve of block j
     for (int j = nb - 1; j >= 0; --j) {
-        double part[NQ];
-        const bool have = pre_blk == j;
+        if (pre_blk == j) {
+            // critical path: tile j + 1 of block j with x_{j+1} from registers, into LDS
+            const int kb = min(TS, nc - j * TS);
+            const T* Pb = P + (int64_t)j * TS * ld;
+            T h1[TS / 2];
+            if (!kEarly) load_half<T>(hc, rowp(Pb, j + 1), ld, 0, kb, rok(j + 1));
+            load_half<T>(h1, rowp(Pb, j + 1), ld, TS / 2, kb, rok(j + 1));
+            chunk(pre[0], hc, xlast);
+            chunk(pre[1], hc + CW, xlast);
+            chunk(pre[2], h1, xlast);
+            chunk(pre[3], h1 + CW, xlast);
+            if (lane < CW) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) part[q] = have ? pre[q] : 0.0;
-        if (have) contrib_list(part, j + 1, j, j + 2);   // the tile solved last is the only one left
-        else contrib_list(part, own.first(j + 1), j, nt);
-        double* pj = pb + ((int64_t)j * own.nwt + own.gw) * TS;
-        if (lane < CW) {
+                for (int q = 0; q < NQ; ++q) cp[j & 1][CW * q + lane] = pre[q];
+            }
+        } else {
+            double part[NQ];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) pj[CW * q + lane] = part[q];
-        }
-        lds_publish(&pub[v], j);
-        const int ow = own.wg_of(j);
-        if (G > 1 && ow != own.g) {
-            // the last of this workgroup's wavefronts to store its part publishes them all
-            int last = 0;
-            if (lane == 0) last = (atomicAdd(&cnt[j % NC], 1) + 1) % NW == 0;
-            if (__builtin_amdgcn_readfirstlane(last)) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                dev_publish(fl + (int64_t)j * G + own.g, ep);
+            for (int q = 0; q < NQ; ++q) part[q] = 0.0;
+            contrib_list(part, own.first(j + 1), j, nt);
+            double* pj = pb + ((int64_t)j * NW + v) * TS;
+            if (lane < CW) {
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) pj[CW * q + lane] = part[q];
             }
         }
-        if ((j % own.nwt) != own.gw) continue;
+        lds_publish(&pub[v], j);
+        if ((j % NW) != v) continue;
         // owner of block j: while the other parts arrive, start block j - 1 with the tiles
-        // that do not need x_j
+        // that do not need x_j, and load the first half of tile (j, j - 1)
         if (j > 0) {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) pre[q] = 0.0;
             contrib_list(pre, own.first(j + 1), j - 1, nt);
             pre_blk = j - 1;
+            if (kEarly)
+                load_half<T>(hc, rowp(P + (int64_t)(j - 1) * TS * ld, j), ld, 0, min(TS, nc - (j - 1) * TS), rok(j));
         }
         __builtin_amdgcn_sched_barrier(0);
         // the diagonal block's columns load while the other parts arrive
@@ -509,23 +520,31 @@ __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__
             for (int i = 0; i < TS; ++i)       // clamped loads times a 0/1 mask: no branches
                 col[i] = Lr[min(i, kbj - 1) * kbj + min(lane, kbj - 1)] * ((i < kbj && lane < kbj) ? T(1) : T(0));
         }
-        const int ld_ = min(lane, kbj - 1);
-        const double rd = lane < kbj ? (double)Lr[ld_ * kbj + ld_] : 0.0;
 #pragma unroll
         for (int i = 0; i < TS; ++i) col[i] = i == lane ? T(0) : col[i];
+        const int ld_ = min(lane, kbj - 1);
+        const double rd = lane < kbj ? (double)Lr[ld_ * kbj + ld_] : 0.0;
+        const double rhs = lane < kbj ? x[tk.first + j * TS + lane] : 0.0;
+        // the parts in pbuf load while the critical-path part is still being formed; all
+        // parts are then summed in wavefront order
+        const int prod = j + 1 < nb ? (j + 1) % NW : -1;    // wavefront whose part is in cp
         for (int u = 0; u < NW; ++u)
-            if (u != v && !lds_wait(&pub[u], [&](int p) { return p <= j; }, err)) return;
-        for (int h = 0; h < G; ++h)
-            if (h != own.g && !dev_wait(fl + (int64_t)j * G + h, ep, err)) return;
+            if (u != v && u != prod && !lds_wait(&pub[u], [&](int p) { return p <= j; }, err)) return;
+        double pv[NW];
+        const double* p0 = pb + (int64_t)j * NW * TS + lane;
+#pragma unroll
+        for (int u = 0; u < NW; ++u) pv[u] = u == prod ? 0.0 : p0[(int64_t)u * TS];
+        if (prod >= 0 && !lds_wait(&pub[prod], [&](int p) { return p <= j; }, err)) return;
         double val = 0.0;
         if (lane < kbj) {
             double s = 0.0;
-            const double* p0 = pb + (int64_t)j * own.nwt * TS + lane;
-            for (int u = 0; u < own.nwt; ++u) s += p0[(int64_t)u * TS];
-            val = x[tk.first + j * TS + lane] - s;
+#pragma unroll
+            for (int u = 0; u < NW; ++u) s += u == prod ? cp[j & 1][lane] : pv[u];
+            val = rhs - s;
         }
         const double sol = bsub_regs<T>(val, col, rd);
         if (lane < kbj) x[tk.first + j * TS + lane] = sol;
+        xlast = lane < kbj ? sol : 0.0;
     }
 }
 
@@ -536,9 +555,11 @@ int level_solve(int dir, int32_t ntask, const lkd_task* tasks, const void* panel
     if (dir == 0)
         LK_LAUNCH((k_fwd_sn<T, NW, G>), G > 1 ? grid : ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
                   (const T*)diag, rows, x, below, pbuf, flags, epoch, err, xq);
+    else if constexpr (G == 1)
+        LK_LAUNCH((k_bwd_sn<T, NW>), ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
+                  (const T*)diag, rows, x, below, pbuf, err);
     else
-        LK_LAUNCH((k_bwd_sn<T, NW, G>), ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
-                  (const T*)diag, rows, x, below, pbuf, flags, epoch, err);
+        return (int)hipErrorInvalidValue;
     return lk_last_error();
 }
 
@@ -548,13 +569,14 @@ __global__ void k_epoch(uint32_t* epoch, uint32_t* cnt, int ncnt) {
     for (int i = threadIdx.x; i < ncnt; i += blockDim.x) cnt[i] = 0u;
 }
 
-// wavefronts per workgroup 4 or 8, workgroups per supernode 1 or 4 (the backward uses 1)
+// wavefronts per workgroup 4 or 8, workgroups per supernode 1 or 4 (forward, 8 wavefronts; the
+// backward with 4 measured slower than with 1 and is not offered)
 template <typename T>
 int level_solve_cfg(int nw, int g, int dir, int32_t ntask, const lkd_task* tasks, const void* panel,
                     const void* diag, const int32_t* rows, double* x, double* below, double* pbuf,
                     uint32_t* flags, const uint32_t* epoch, int32_t* err, const XcdQueue& xq,
                     int grid, void* stream) {
-    if (nw == 8 && g == 4 && dir == 0)
+    if (nw == 8 && g == 4)
         return level_solve<T, 8, 4>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, xq, grid, stream);
     if (nw == 8)
         return level_solve<T, 8, 1>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, xq, grid, stream);
