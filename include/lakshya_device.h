@@ -88,12 +88,19 @@ typedef struct {
 } lkd_snplan;
 
 /* Load balance of sparse row passes: rows (or pair-map segments) longer than the threshold are
- * listed once at set-up and each handled by a whole 256-thread block, in a fixed order. */
+ * listed once at set-up and each handled by a whole 256-thread block, in a fixed order.  Row
+ * passes may split listed rows further: with split > 1, a pre-pass sums each listed row k in
+ * `split` pieces of equal length (one block each) into part[k * split + p], and the pass adds
+ * the pieces in order (about LKD_LONG_PIECE entries per piece, at most LKD_LONG_MAX_SPLIT). */
 #define LKD_LONG_ROW 512
 #define LKD_LONG_SEG 64
+#define LKD_LONG_PIECE 1024
+#define LKD_LONG_MAX_SPLIT 64
 typedef struct {
     int32_t n;              /* number of listed rows / segments */
     const int32_t* idx;     /* their indices (device) */
+    int32_t split;          /* row passes: pieces per listed row (0 or 1: none) */
+    double* part;           /* row passes, split > 1: n * split piece sums (device scratch) */
 } lkd_long;
 
 /* ---------------------------------------------------------------- normal-equations assembly */

@@ -60,33 +60,66 @@ int finalize(const double* part, int nblk, int q, const int* ops, const int* slo
 
 constexpr int LONG_U = 8;
 
+// sum of val[p] x[ci[p]] over p in [b, e) by one block (valid in thread 0): LONG_U entries per
+// thread and round, their loads all in flight together, summed in entry order, then the tree
+__device__ __forceinline__ double long_sum(int32_t b, int32_t e, const int32_t* __restrict__ ci,
+                                           const double* __restrict__ val,
+                                           const double* __restrict__ x, double* sh) {
+    double s = 0.0;
+    for (int32_t p = b + threadIdx.x; p < e; p += NT * LONG_U) {
+        double v[LONG_U];
+#pragma unroll
+        for (int u = 0; u < LONG_U; ++u) {
+            const int32_t q = p + u * NT < e ? p + u * NT : e - 1;
+            v[u] = p + u * NT < e ? val[q] * x[ci[q]] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < LONG_U; ++u) s += v[u];
+    }
+    return block_reduce(s, OP_SUM, sh);
+}
+
+// Pieces of the listed long rows of a row pass: block k * split + p sums the entries
+// [b + len p / split, b + len (p + 1) / split) of listed row k.
+__global__ __launch_bounds__(NT) void k_long_part(const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ ci,
+                                                  const double* __restrict__ val,
+                                                  const double* __restrict__ x, lkd_long lg) {
+    __shared__ double sh[4];
+    const int32_t k = blockIdx.x / lg.split, p = blockIdx.x % lg.split;
+    const int32_t i = lg.idx[k];
+    const int64_t b0 = rp[i], len = rp[i + 1] - b0;
+    const double s = long_sum((int32_t)(b0 + len * p / lg.split), (int32_t)(b0 + len * (p + 1) / lg.split),
+                              ci, val, x, sh);
+    if (threadIdx.x == 0) lg.part[blockIdx.x] = s;
+}
+
+// the pre-pass of a row pass over (rp, ci, val) with vector x, when its long rows are split
+void long_pre(const lkd_long& lg, const int32_t* rp, const int32_t* ci, const double* val,
+              const double* x, void* st) {
+    if (lg.n > 0 && lg.split > 1) LK_LAUNCH(k_long_part, lg.n * lg.split, NT, 0, st, rp, ci, val, x, lg);
+}
+
 // Sparse row pass y_i = sum_p val[p] x[ci[p]] with the load balanced: rows longer than
-// LKD_LONG_ROW entries (listed in `longs`, e.g. the linking rows of a block-angular matrix) take
-// a whole block each, in a fixed thread-stride + tree order; the others take G lanes.  f(i, y_i)
-// runs once per row, in thread 0 of the block (long rows) or lane 0 of the group.
+// LKD_LONG_ROW entries (listed in `lg`, e.g. the linking rows of a block-angular matrix) take
+// a whole block each, in a fixed thread-stride + tree order, or, split, the pieces of the
+// pre-pass (long_pre, launched before the pass) added in order; the others take G lanes.
+// f(i, y_i) runs once per row, in thread 0 of the block (long rows) or lane 0 of the group.
 template <int G, typename F>
 __device__ __forceinline__ void for_rows(int64_t m, const int32_t* __restrict__ rp,
                                          const int32_t* __restrict__ ci,
                                          const double* __restrict__ val,
-                                         const double* __restrict__ x, int32_t nlong,
-                                         const int32_t* __restrict__ longs, double* sh, F&& f) {
-    for (int32_t k = blockIdx.x; k < nlong; k += gridDim.x) {
-        const int32_t i = longs[k];
-        const int32_t e = rp[i + 1];
+                                         const double* __restrict__ x, const lkd_long& lg,
+                                         double* sh, F&& f) {
+    for (int32_t k = blockIdx.x; k < lg.n; k += gridDim.x) {
+        const int32_t i = lg.idx[k];
         double s = 0.0;
-        // LONG_U entries per thread and round, their loads all in flight together; summed in
-        // entry order
-        for (int32_t p = rp[i] + threadIdx.x; p < e; p += NT * LONG_U) {
-            double v[LONG_U];
-#pragma unroll
-            for (int u = 0; u < LONG_U; ++u) {
-                const int32_t q = p + u * NT < e ? p + u * NT : e - 1;
-                v[u] = p + u * NT < e ? val[q] * x[ci[q]] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < LONG_U; ++u) s += v[u];
+        if (lg.split > 1) {
+            if (threadIdx.x == 0)
+                for (int p = 0; p < lg.split; ++p) s += lg.part[(int64_t)k * lg.split + p];
+        } else {
+            s = long_sum(rp[i], rp[i + 1], ci, val, x, sh);
         }
-        s = block_reduce(s, OP_SUM, sh);
         if (threadIdx.x == 0) f((int64_t)i, s);
     }
     const int lane = threadIdx.x & (G - 1);
@@ -306,7 +339,7 @@ __global__ __launch_bounds__(NT) void k_res_cols(int32_t n, const int32_t* __res
                                                  const double* __restrict__ dy,
                                                  double* __restrict__ u, lkd_long lc) {
     __shared__ double sh[4];
-    for_rows<G>(n, colptr, prow, cval, dy, lc.n, lc.idx, sh,
+    for_rows<G>(n, colptr, prow, cval, dy, lc, sh,
                 [&](int64_t j, double s) { u[j] = theta[j] * s; });
 }
 
@@ -325,7 +358,7 @@ __global__ __launch_bounds__(NT) void k_res_rows(int32_t m, const int32_t* __res
                                                  double* __restrict__ part, lkd_long lr) {
     __shared__ double sh[4];
     double mr = 0.0, mb = 0.0, md = 0.0, mm = 0.0;
-    for_rows<G>(m, prp, pci, pval, u, lr.n, lr.idx, sh, [&](int64_t i, double s) {
+    for_rows<G>(m, prp, pci, pval, u, lr, sh, [&](int64_t i, double s) {
         const double v = rhs[i] - s;
         r[i] = v;
         req[i] = v * D[i];
@@ -353,7 +386,7 @@ __global__ __launch_bounds__(NT) void k_pcg_rows(int32_t m, const int32_t* __res
                                                  double* __restrict__ part, lkd_long lr) {
     __shared__ double sh[4];
     double acc = 0.0;
-    for_rows<G>(m, prp, pci, pval, u, lr.n, lr.idx, sh, [&](int64_t i, double s) {
+    for_rows<G>(m, prp, pci, pval, u, lr, sh, [&](int64_t i, double s) {
         q[i] = s;
         acc += p[i] * s;
     });
@@ -436,7 +469,7 @@ __global__ __launch_bounds__(NT) void k_resid_rows(lkd_ipm P, const double* __re
                                                    double* __restrict__ rp) {
     __shared__ double sh[4];
     double s2 = 0.0, inf = 0.0, by = 0.0;
-    for_rows<G>(P.m, P.arp, P.aci, P.aval, x, P.long_rows.n, P.long_rows.idx, sh, [&](int64_t i, double ax) {
+    for_rows<G>(P.m, P.arp, P.aci, P.aval, x, P.long_rows, sh, [&](int64_t i, double ax) {
         const double v = ax + (-P.b[i]);
         rp[i] = v;
         s2 += v * v;
@@ -458,7 +491,7 @@ __global__ __launch_bounds__(NT) void k_resid_cols(lkd_ipm P, const double* __re
                                                    double* __restrict__ theta, int qoff) {
     __shared__ double sh[4];
     double s2 = 0.0, xs = 0.0, cx = 0.0, inf = 0.0, xsmax = 0.0;
-    for_rows<G>(P.n, P.trp, P.tci, P.tval, y, P.long_cols.n, P.long_cols.idx, sh, [&](int64_t j, double aty) {
+    for_rows<G>(P.n, P.trp, P.tci, P.tval, y, P.long_cols, sh, [&](int64_t j, double aty) {
         const double v = aty + (s[j] - P.c[j]);
         rd[j] = v;
         if (theta) theta[j] = x[j] / s[j];
@@ -499,7 +532,7 @@ __global__ __launch_bounds__(NT) void k_rkkt_rows(lkd_ipm P, const double* __res
                                                   const double* __restrict__ rp,
                                                   double* __restrict__ rkkt) {
     __shared__ double sh[4];
-    for_rows<G>(P.m, P.arp, P.aci, P.aval, v, P.long_rows.n, P.long_rows.idx, sh,
+    for_rows<G>(P.m, P.arp, P.aci, P.aval, v, P.long_rows, sh,
                 [&](int64_t i, double av) { rkkt[i] = av + (-rp[i]); });
 }
 
@@ -514,7 +547,7 @@ __global__ __launch_bounds__(NT) void k_dir_cols(lkd_ipm P, const double* __rest
                                                  double* __restrict__ qd) {
     __shared__ double sh[4];
     double r2 = 0, r3 = 0, rx = 0, apm = 1e30, adm = 1e30, dx2 = 0, ds2 = 0;
-    for_rows<G>(P.n, P.trp, P.tci, P.tval, dy, P.long_cols.n, P.long_cols.idx, sh, [&](int64_t j, double t) {
+    for_rows<G>(P.n, P.trp, P.tci, P.tval, dy, P.long_cols, sh, [&](int64_t j, double t) {
         const double dsj = (-t) + (-rd[j]);
         const double dxj = -theta[j] * dsj + rxs[j] / s[j];
         ds[j] = dsj;
@@ -548,7 +581,7 @@ __global__ __launch_bounds__(NT) void k_dir_rows(lkd_ipm P, const double* __rest
                                                  double* __restrict__ qp, int qoff) {
     __shared__ double sh[4];
     double r1 = 0, dy2 = 0;
-    for_rows<G>(P.m, P.arp, P.aci, P.aval, dx, P.long_rows.n, P.long_rows.idx, sh, [&](int64_t i, double adx) {
+    for_rows<G>(P.m, P.arp, P.aci, P.aval, dx, P.long_rows, sh, [&](int64_t i, double adx) {
         qp[i] = adx;
         const double e1 = adx + rp[i];
         r1 += e1 * e1;
@@ -691,33 +724,40 @@ __global__ __launch_bounds__(NT) void k_init(int32_t m, int32_t n, double* __res
     }
 
 template <int G> void l_resid_rows(const lkd_ipm* P, const double* x, const double* y, double* rp, void* st) {
+    long_pre(P->long_rows, P->arp, P->aci, P->aval, x, st);
     LK_LAUNCH(k_resid_rows<G>, P->nblk_rows, NT, 0, st, *P, x, y, rp);
 }
 template <int G> void l_resid_cols(const lkd_ipm* P, const double* x, const double* y, const double* s,
                                    double* rd, double* th, int qoff, void* st) {
+    long_pre(P->long_cols, P->trp, P->tci, P->tval, y, st);
     LK_LAUNCH(k_resid_cols<G>, P->nblk_cols, NT, 0, st, *P, x, y, s, rd, th, qoff);
 }
 template <int G> void l_rkkt(const lkd_ipm* P, const double* v, const double* rp, double* rk, void* st) {
+    long_pre(P->long_rows, P->arp, P->aci, P->aval, v, st);
     LK_LAUNCH(k_rkkt_rows<G>, P->nblk_rows, NT, 0, st, *P, v, rp, rk);
 }
 template <int G> void l_dir_cols(const lkd_ipm* P, const double* x, const double* s, const double* th,
                                  const double* rd, const double* rxs, const double* dy, double* dx,
                                  double* ds, double* qd, void* st) {
+    long_pre(P->long_cols, P->trp, P->tci, P->tval, dy, st);
     LK_LAUNCH(k_dir_cols<G>, P->nblk_cols, NT, 0, st, *P, x, s, th, rd, rxs, dy, dx, ds, qd);
 }
 template <int G> void l_dir_rows(const lkd_ipm* P, const double* dx, const double* rp, const double* dy,
                                  double* qp, int qoff, void* st) {
+    long_pre(P->long_rows, P->arp, P->aci, P->aval, dx, st);
     LK_LAUNCH(k_dir_rows<G>, P->nblk_rows, NT, 0, st, *P, dx, rp, dy, qp, qoff);
 }
 template <int G> void l_res_rows(int32_t m, const int32_t* prp, const int32_t* pci, const double* pval,
                                  const double* u, const double* rhs, const double* dy, const double* m0d,
                                  double* r, const double* D, double* req, double* part, int nblk,
                                  lkd_long lr, void* st) {
+    long_pre(lr, prp, pci, pval, u, st);
     LK_LAUNCH(k_res_rows<G>, nblk, NT, 0, st, m, prp, pci, pval, u, rhs, dy, m0d, r, D, req, part, lr);
 }
 template <int G> void l_pcg_rows(int32_t m, const int32_t* prp, const int32_t* pci, const double* pval,
                                  const double* u, const double* p, double* q, double* part, int nblk,
                                  lkd_long lr, void* st) {
+    long_pre(lr, prp, pci, pval, u, st);
     LK_LAUNCH(k_pcg_rows<G>, nblk, NT, 0, st, m, prp, pci, pval, u, p, q, part, lr);
 }
 
@@ -782,6 +822,7 @@ int lkd_kkt_residual(int32_t m, int32_t n, const int32_t* colptr, const int32_t*
                      const double* rhs_perm, const double* m0d, double* u, double* r,
                      const double* D, double* req, double* kS, lkd_long long_rows,
                      lkd_long long_cols, void* stream) {
+    long_pre(long_cols, colptr, prow, cval, dy, stream);
     LK_LAUNCH(k_res_cols<4>, grid_for(4 * (int64_t)n), NT, 0, stream, n, colptr, prow, cval, theta, dy, u, long_cols);
     const int nblk = grid_for(8 * (int64_t)m, LKD_MAX_NBLK);   // 8 lanes per row
     double* part = kS + LKD_KS_PART;
@@ -795,6 +836,7 @@ int lkd_pcg_matvec(int32_t m, int32_t n, const int32_t* colptr, const int32_t* p
                    const double* pval, const double* theta, const double* p, double* u,
                    double* q, double* kS, int32_t slot_pq, lkd_long long_rows, lkd_long long_cols,
                    void* stream) {
+    long_pre(long_cols, colptr, prow, cval, p, stream);
     LK_LAUNCH(k_res_cols<4>, grid_for(4 * (int64_t)n), NT, 0, stream, n, colptr, prow, cval, theta, p, u, long_cols);
     const int nblk = grid_for(8 * (int64_t)m, LKD_MAX_NBLK);   // 8 lanes per row
     double* part = kS + LKD_KS_PART;

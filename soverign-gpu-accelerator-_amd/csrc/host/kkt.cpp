@@ -117,8 +117,8 @@ struct GPUKKTCholeskySolver::Impl {
     double* d_pp = nullptr;
     uint8_t* d_empty = nullptr;
     // rows / columns / pair-map segments handled by whole blocks (LKD_LONG_ROW / LKD_LONG_SEG)
-    lkd_long long_prows{0, nullptr}, long_cols{0, nullptr}, long_diag{0, nullptr}, long_segs{0, nullptr};
-    lkd_long mid_diag{0, nullptr}, mid_segs{0, nullptr};   // 16-lane class of the pair map
+    lkd_long long_prows{0, nullptr, 0, nullptr}, long_cols{0, nullptr, 0, nullptr}, long_diag{0, nullptr, 0, nullptr}, long_segs{0, nullptr, 0, nullptr};
+    lkd_long mid_diag{0, nullptr, 0, nullptr}, mid_segs{0, nullptr, 0, nullptr};   // 16-lane class of the pair map
     int64_t* d_chunk_seg = nullptr;   // first segment of each LKD_PANEL_CHUNK panel chunk
     double* d_mval = nullptr;         // per-segment M0 entries (assembly scratch)
     int32_t *d_sn_first = nullptr, *d_sn_rows = nullptr, *d_col_to_sn = nullptr;
@@ -276,6 +276,8 @@ struct GPUKKTCholeskySolver::Impl {
         h_colptr = colptr; h_prow = prow; h_cval = cval;
         long_prows = long_list(prp, m, LKD_LONG_ROW, "long_prows");
         long_cols = long_list(colptr, n, LKD_LONG_ROW, "long_cols");
+        split_long(long_prows, prp, "long_prows_part");
+        split_long(long_cols, colptr, "long_cols_part");
         d_colptr = up(colptr, "colptr");
         d_prow = up(prow, "prow");
         d_cval = up(cval, "cval");
@@ -295,7 +297,20 @@ struct GPUKKTCholeskySolver::Impl {
         for (int64_t k = 0; k < count; ++k)
             if (static_cast<int64_t>(ptr[k + 1]) - static_cast<int64_t>(ptr[k]) > thr)
                 idx.push_back(static_cast<int32_t>(k));
-        return {static_cast<int32_t>(idx.size()), up(idx, name)};
+        return {static_cast<int32_t>(idx.size()), up(idx, name), 0, nullptr};
+    }
+
+    // row passes: the listed rows are summed in pieces of about LKD_LONG_PIECE entries by a
+    // pre-pass; scratch for the piece sums
+    template <typename P>
+    void split_long(lkd_long& lg, const std::vector<P>& ptr, const char* name) {
+        int64_t longest = 0;
+        for (size_t k = 0; k + 1 < ptr.size(); ++k)
+            longest = std::max(longest, static_cast<int64_t>(ptr[k + 1]) - static_cast<int64_t>(ptr[k]));
+        lg.split = static_cast<int32_t>(std::min<int64_t>(LKD_LONG_MAX_SPLIT, (longest + LKD_LONG_PIECE - 1) / LKD_LONG_PIECE));
+        lg.part = nullptr;
+        if (lg.n > 0 && lg.split > 1)
+            lg.part = up(std::vector<double>(static_cast<size_t>(lg.n) * lg.split, 0.0), name);
     }
 
     int64_t panel_index(Index i, Index j) const {   // i >= j, permuted
@@ -355,7 +370,7 @@ struct GPUKKTCholeskySolver::Impl {
                 const int64_t len = segptr[t + 1] - segptr[t];
                 if (len > LKD_SHORT_SEG && len <= LKD_LONG_SEG) ms.push_back(static_cast<int32_t>(t));
             }
-            mid_segs = {static_cast<int32_t>(ms.size()), up(ms, "mid_segs")};
+            mid_segs = {static_cast<int32_t>(ms.size()), up(ms, "mid_segs"), 0, nullptr};
             std::vector<int32_t> ld, md;
             for (Index i = 0; i < m; ++i) {
                 if (dseg[i] < 0) continue;
@@ -363,8 +378,8 @@ struct GPUKKTCholeskySolver::Impl {
                 if (len > LKD_LONG_SEG) ld.push_back(i);
                 else if (len > LKD_SHORT_SEG) md.push_back(i);
             }
-            long_diag = {static_cast<int32_t>(ld.size()), up(ld, "long_diag")};
-            mid_diag = {static_cast<int32_t>(md.size()), up(md, "mid_diag")};
+            long_diag = {static_cast<int32_t>(ld.size()), up(ld, "long_diag"), 0, nullptr};
+            mid_diag = {static_cast<int32_t>(md.size()), up(md, "mid_diag"), 0, nullptr};
         }
         {
             const int64_t pe = S.sn_panel[S.nsuper];

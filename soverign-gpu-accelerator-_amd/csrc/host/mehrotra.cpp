@@ -123,20 +123,30 @@ public:
         };
         // rows / columns longer than LKD_LONG_ROW go to whole blocks; the lane-group width of the
         // other rows follows their mean length
-        auto long_list = [&](const std::vector<Index>& ptr, Index count, int32_t& nl, double& mean) {
+        // (listed rows are summed in pieces of about LKD_LONG_PIECE entries by a pre-pass)
+        auto long_list = [&](const std::vector<Index>& ptr, Index count, lkd_long& lg, double& mean) {
             std::vector<Index> idx;
-            int64_t short_nz = 0;
+            int64_t short_nz = 0, longest = 0;
             for (Index k = 0; k < count; ++k) {
                 const int64_t len = ptr[k + 1] - ptr[k];
-                if (len > LKD_LONG_ROW) idx.push_back(k);
-                else short_nz += len;
+                if (len > LKD_LONG_ROW) {
+                    idx.push_back(k);
+                    longest = std::max(longest, len);
+                } else {
+                    short_nz += len;
+                }
             }
-            nl = static_cast<int32_t>(idx.size());
-            mean = static_cast<double>(short_nz) / std::max<int64_t>(1, count - nl);
-            return idx.empty() ? nullptr : up_i(idx);
+            lg.n = static_cast<int32_t>(idx.size());
+            mean = static_cast<double>(short_nz) / std::max<int64_t>(1, count - lg.n);
+            lg.idx = idx.empty() ? nullptr : up_i(idx);
+            lg.split = static_cast<int32_t>(std::min<int64_t>(LKD_LONG_MAX_SPLIT, (longest + LKD_LONG_PIECE - 1) / LKD_LONG_PIECE));
+            lg.part = nullptr;
+            if (lg.n > 0 && lg.split > 1)
+                lg.part = static_cast<double*>(arena_->allocate_uninitialized(
+                    static_cast<size_t>(lg.n) * static_cast<size_t>(lg.split) * sizeof(double)));
         };
         double mean_row = 0, mean_col = 0;
-        P_.long_rows.idx = long_list(A_.row_ptrs, m_, P_.long_rows.n, mean_row);
+        long_list(A_.row_ptrs, m_, P_.long_rows, mean_row);
         core::CSCMatrix Acsc(m_, n_, {}, {}, {});
         {   // A^T in CSR = CSC of the scaled A
             std::vector<Index> cp(static_cast<size_t>(n_) + 1, 0), ri(static_cast<size_t>(nz));
@@ -150,7 +160,7 @@ public:
                     ri[q] = i;
                     va[q] = A_.values[p];
                 }
-            P_.long_cols.idx = long_list(cp, n_, P_.long_cols.n, mean_col);
+            long_list(cp, n_, P_.long_cols, mean_col);
             P_.trp = up_i(cp);
             P_.tci = up_i(ri);
             P_.tval = up_d(va);
