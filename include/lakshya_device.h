@@ -314,6 +314,10 @@ int lkd_ipm_direction(const lkd_ipm* P, const double* x, const double* s, const 
 /* slot MUAFF = (x + ap dxa)^T (s + ad dsa)  (mehrotra_device.cu:831-835) */
 int lkd_ipm_mu_aff(const lkd_ipm* P, const double* x, const double* s, const double* dxa,
                    const double* dsa, double ap, double ad, void* stream);
+/* the same with ap = min(1, slot APMAX), ad = min(1, slot ADMAX) taken on the device, so the
+ * affine step needs no host round trip (mehrotra_device.cu:824-835 computes them on the host) */
+int lkd_ipm_mu_aff_step(const lkd_ipm* P, const double* x, const double* s, const double* dxa,
+                        const double* dsa, void* stream);
 /* All backtracking trials t = 0..ntrial-1 with (ap, ad) * 0.5^t evaluated in one pass:
  * per trial ||rp + ap_t qp||^2, ||rd + ad_t qd||^2, (x + ap_t dx)^T (s + ad_t ds),
  * min(x + ap_t dx), min(s + ad_t ds), ||y + ad_t dy||^2   (mehrotra_device.cu:920-995) */

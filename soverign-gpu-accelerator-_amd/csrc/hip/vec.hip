@@ -598,8 +598,13 @@ __global__ __launch_bounds__(NT) void k_muaff(lkd_ipm P, const double* __restric
                                               const double* __restrict__ s,
                                               const double* __restrict__ dxa,
                                               const double* __restrict__ dsa, double ap,
-                                              double ad) {
+                                              double ad, int from_slots) {
     __shared__ double sh[4];
+    if (from_slots) {   // the affine step lengths min(1, APMAX), min(1, ADMAX) from the slots
+        const double a = P.S[LKS_APMAX], b = P.S[LKS_ADMAX];
+        ap = a < 1.0 ? a : 1.0;
+        ad = b < 1.0 ? b : 1.0;
+    }
     double acc = 0.0;
     for (int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x; j < P.n; j += (int64_t)gridDim.x * NT) {
         const double tx = x[j] + ap * dxa[j];
@@ -930,7 +935,14 @@ int lkd_ipm_direction(const lkd_ipm* P, const double* x, const double* s, const 
 
 int lkd_ipm_mu_aff(const lkd_ipm* P, const double* x, const double* s, const double* dxa,
                    const double* dsa, double ap, double ad, void* stream) {
-    LK_LAUNCH(k_muaff, P->nblk_cols, NT, 0, stream, *P, x, s, dxa, dsa, ap, ad);
+    LK_LAUNCH(k_muaff, P->nblk_cols, NT, 0, stream, *P, x, s, dxa, dsa, ap, ad, 0);
+    int op = OP_SUM, slot = LKS_MUAFF;
+    return finalize(P->part, P->nblk_cols, 1, &op, &slot, P->S, stream);
+}
+
+int lkd_ipm_mu_aff_step(const lkd_ipm* P, const double* x, const double* s, const double* dxa,
+                        const double* dsa, void* stream) {
+    LK_LAUNCH(k_muaff, P->nblk_cols, NT, 0, stream, *P, x, s, dxa, dsa, 0.0, 0.0, 1);
     int op = OP_SUM, slot = LKS_MUAFF;
     return finalize(P->part, P->nblk_cols, 1, &op, &slot, P->S, stream);
 }

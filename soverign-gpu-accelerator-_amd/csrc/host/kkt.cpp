@@ -150,6 +150,7 @@ struct GPUKKTCholeskySolver::Impl {
     double* d_pbuf = nullptr;      // level solves: z exchange / backward partial sums
     uint32_t* d_flags = nullptr;   // level solves: hand-off flags, then the epoch
     int32_t* d_err = nullptr;
+    int32_t* d_serr = nullptr;     // level-solve status (bit 0x10: a bounded wait timed out)
     double *d_D = nullptr, *d_m0d = nullptr, *d_theta = nullptr;
     double *d_r = nullptr, *d_req = nullptr, *d_z = nullptr, *d_dy = nullptr, *d_bak = nullptr,
            *d_rhsp = nullptr,
@@ -1077,7 +1078,10 @@ struct GPUKKTCholeskySolver::Impl {
         d_bak = dv(m);
         d_rhsp = dv(m);
         d_u = dv(n);
-        d_rinf = dv(LKD_KS_PART + 8 * LKD_MAX_NBLK);
+        // the level-solve status word sits just before the kS slots, so one copy reads both
+        double* ks_block = dv(1 + LKD_KS_PART + 8 * LKD_MAX_NBLK);
+        d_serr = reinterpret_cast<int32_t*>(ks_block);
+        d_rinf = ks_block + 1;
         d_pcg = dv(4 * static_cast<int64_t>(m));
         if (nnz_csr > 0) d_csr = dv(nnz_csr);
         std::vector<int32_t> sn_first(S.sn_first.begin(), S.sn_first.end());
@@ -1303,22 +1307,27 @@ struct GPUKKTCholeskySolver::Impl {
                                  x, d_below, d_pbuf, d_flags, epoch, q ? d_squeue + L.begin : nullptr,
                                  q ? d_sqoff + static_cast<int64_t>(L.q) * (n_xcd + 1) : nullptr, n_xcd,
                                  q ? epoch + 1 + static_cast<int64_t>(L.q) * n_xcd : nullptr, L.grid,
-                                 d_err + 1, stream), "solve kernel");
+                                 d_serr, stream), "solve kernel");
         }
     }
 
-    // A level solve whose waits timed out produced an invalid vector: report it.
+    // A level solve whose waits timed out produced an invalid vector: report it.  The status
+    // comes with the kS read when no level solve ran after it, else it is read here.
     void check_solve_sync() {
-        int32_t e = 0;
-        check(lkd_memcpy_dtoh(&e, d_err + 1, sizeof(int32_t), stream), "solve status");
+        int32_t e = h_serr;
+        if (solve_unread) check(lkd_memcpy_dtoh(&e, d_serr, sizeof(int32_t), stream), "solve status");
         if (e == 0) return;
-        check(lkd_memset(d_err + 1, 0, sizeof(int32_t), stream), "reset");
+        h_serr = 0;
+        check(lkd_memset(d_serr, 0, sizeof(int32_t), stream), "reset");
         check(lkd_stream_sync(stream), "reset");
         throw std::runtime_error("KKT solve: level synchronisation timed out");
     }
 
+    int32_t h_serr = 0;            // status word as of the last read_ks
+    bool solve_unread = false;     // a level solve ran after the last read_ks
     void trsv(int prec, double* x) {
         run_solve_schedule(prec, x);
+        solve_unread = true;
     }
 
     void factor_numeric(int prec) {
@@ -1380,8 +1389,12 @@ struct GPUKKTCholeskySolver::Impl {
                                stream), "residual");
     }
     double hk[LKD_KS_SLOTS];
-    void read_ks() {
-        check(lkd_memcpy_dtoh(hk, d_rinf, sizeof(hk), stream), "kkt scalars");
+    void read_ks() {   // the status word and the kS slots in one copy
+        double blk[1 + LKD_KS_SLOTS];
+        check(lkd_memcpy_dtoh(blk, d_rinf - 1, sizeof(blk), stream), "kkt scalars");
+        std::memcpy(&h_serr, blk, sizeof(int32_t));
+        std::copy(blk + 1, blk + 1 + LKD_KS_SLOTS, hk);
+        solve_unread = false;
     }
     // backward-error acceptance (relative-gap mode): ||r|| <= 1e-12 (||rhs|| + max M0_ii ||dy||)
     bool backward_ok(double rinf) const { return rinf <= 1e-12 * (hk[1] + hk[3] * hk[2]); }
@@ -1641,7 +1654,7 @@ bool GPUKKTCholeskySolver::solve_pcg(Float* d_rhs, Float tol, int max_iter, KKTS
 void GPUKKTCholeskySolver::triangular_solve_permuted(Float* d_x) {
     auto& I = *impl_;
     const int prec = I.cached_prec == FactorPrecision::FP32 && I.factor_kind == 2 ? LKD_F32 : LKD_F64;
-    I.run_solve_schedule(prec, d_x);
+    I.trsv(prec, d_x);
     check(lkd_stream_sync(I.stream), "sync");
     I.check_solve_sync();
 }

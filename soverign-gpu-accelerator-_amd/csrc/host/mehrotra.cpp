@@ -293,6 +293,9 @@ public:
                         info.refinement_steps, info.residual_inf, info.delta, tol);
         if (!ok) return false;
         check(lkd_ipm_direction(&P_, x_, s_, y_, th_, rd_, rp_, rxs_, dy_, dx, ds, qd_, qp_, stream_), "direction");
+        // predictor: mu_aff at the affine step lengths (taken from the direction's slots on the
+        // device) comes with the same scalar read
+        if (!dxa) check(lkd_ipm_mu_aff_step(&P_, x_, s_, dx, ds, stream_), "mu_aff");
         read_scalars(LKS_NSLOTS);
         if (opt_.verbose)
             std::printf("    dir: e1 %.3e e2 %.3e e3 %.3e apmax %.3e admax %.3e\n",
@@ -422,9 +425,8 @@ public:
                 broke = true;
                 break;
             }
-            const double ap_aff = std::min(1.0, d.apmax), ad_aff = std::min(1.0, d.admax);
-            check(lkd_ipm_mu_aff(&P_, x_, s_, dxa_, dsa_, ap_aff, ad_aff, stream_), "mu_aff");
-            read_scalars(LKS_NSLOTS);
+            // MUAFF = (x + ap dxa)^T (s + ad dsa) at ap = min(1, apmax), ad = min(1, admax), read
+            // with the predictor's direction scalars
             const double mu_aff = hS_[LKS_MUAFF] / n_;
             const double sigma = std::min(1.0, std::pow(std::max(0.0, mu_aff) / std::max(1e-16, nr.mu), 3.0));
 
