@@ -85,11 +85,12 @@ __device__ __forceinline__ double bsub_regs(double r, const T (&col)[TS], double
 // Per-level solves: one launch per level, G workgroups of NW wavefronts per supernode (G = 1
 // unless the level has very few supernodes).  The row tiles of the panel (t < nb: the rows of
 // diagonal block t; t >= nb: 64-row groups of the rows below the supernode, owned by ancestors)
-// belong to wavefront gw = t % (G NW) (workgroup gw / NW).  Inside a workgroup, wavefronts hand
-// results to each other through workgroup-scope release / acquire on LDS progress words, so
-// nobody waits at a full barrier; between workgroups a flag in device memory is released and
-// acquired at agent scope, once per hand-off (each such fence writes back or invalidates an
-// L2, which is why G stays 1 when many supernodes run side by side).
+// belong to the wavefront with residue t % (G NW), where wavefront v of workgroup g has residue
+// g + G v (see Own).  Inside a workgroup, wavefronts hand results to each other through
+// workgroup-scope release / acquire on LDS progress words, so nobody waits at a full barrier;
+// between the workgroups of a supernode, which share one XCD's L2, a flag in device memory is
+// set after the data stores completed and the reader invalidates its L1 (dev_publish /
+// dev_wait), once per hand-off.
 //  forward   for j = 0 .. nb-1: the owner of tile j solves its diagonal block (it applied all
 //            updates of tile j itself), publishes z_j in its workgroup's LDS ring of ZR blocks
 //            (and, G > 1, in device memory, where wavefront 0 of every other workgroup picks it
@@ -205,15 +206,16 @@ __device__ __forceinline__ void fwd_apply(const Tiles tl, int t, const T (&lv)[T
     else bl[row - tl.nc] -= a;
 }
 
-// Tile ownership among the G workgroups of a supernode (G * NW wavefronts, global wavefront
-// index gw = g * NW + v): tile t belongs to gw = t % (G * NW), so runs of NW consecutive
-// blocks live in one workgroup and most hand-offs on the forward critical path stay in LDS.
+// Tile ownership among the G workgroups of a supernode (G * NW wavefronts; wavefront v of
+// workgroup g has the residue gw = g + G v): tile t belongs to the wavefront with residue
+// t % (G * NW), so consecutive tiles go to consecutive workgroups and the rows of a supernode
+// are spread evenly over them (every forward hand-off then crosses workgroups, in one L2).
 template <int G, int NW>
 struct Own {
     static constexpr int nwt = G * NW;
     int gw, g;
     __device__ __forceinline__ int first(int t0) const { return t0 + ((gw - t0 % nwt + nwt) % nwt); }
-    __device__ __forceinline__ int wg_of(int t) const { return (t % nwt) / NW; }
+    __device__ __forceinline__ int wg_of(int t) const { return (t % nwt) % G; }
 };
 
 // Hand-off between the workgroups of one supernode, which all run on one XCD (they take their
@@ -321,7 +323,7 @@ __global__ __launch_bounds__(64 * NW) void k_fwd_sn(const lkd_task* __restrict__
     const Tiles tl(ld, tk.ncol);
     const int nb = tl.nb, nt = tl.nt, nc = tk.ncol;
     const int v = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const Own<G, NW> own{tk.r0 * NW + v, tk.r0};
+    const Own<G, NW> own{tk.r0 + G * v, tk.r0};
     const uint32_t ep = G > 1 ? *epoch : 0u;
     double* zg = zbuf + tk.ws;
     uint32_t* fl = flags + tk.px;
