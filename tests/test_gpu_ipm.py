@@ -33,17 +33,25 @@ def test_matches_oracle(gpu, oracle, m, n, k, seed):
     assert ok
 
 
-@pytest.mark.parametrize("case", ["linking_rows", "dense_columns"])
+@pytest.mark.parametrize("case", ["linking_rows", "dense_columns", "linking_rows_split",
+                                  "dense_columns_split"])
 def test_long_rows_and_columns_match_oracle(gpu, oracle, case):
     """Rows / columns above the whole-block threshold (512 entries) and M0 entries summed from more
-    than 64 column pairs go through the block-per-row / block-per-segment paths."""
+    than 64 column pairs go through the block-per-row / block-per-segment paths; above 1,024
+    entries the row passes sum them in pieces (pre-pass)."""
     from lakshya import gen
     if case == "linking_rows":
         lp = gen.block_angular(4, 60, 200, 4, 5, 0.9, seed=7)      # 5 linking rows of ~720 entries
         assert np.diff(lp.csr()[0]).max() > 512
-    else:
+    elif case == "dense_columns":
         lp = gen.dense_columns(600, 1500, 4, 3, seed=8)            # 3 columns of 600 entries
         assert np.diff(lp.indptr).max() > 512
+    elif case == "linking_rows_split":
+        lp = gen.block_angular(4, 60, 800, 4, 5, 0.9, seed=9)      # 5 linking rows of ~2,900 entries
+        assert np.diff(lp.csr()[0]).max() > 2048
+    else:
+        lp = gen.dense_columns(1100, 2200, 4, 3, seed=10)          # 3 columns of 1,100 entries
+        assert np.diff(lp.indptr).max() > 1024
     g = gpu.ipm_solve(lp, relative=True)
     o = oracle.ipm_solve(lp.csr(), lp.b, lp.c, lp.n, stop_rule=1)
     assert g["status_name"] == "Optimal" and o["status"] == 0
