@@ -1205,7 +1205,9 @@ struct GPUKKTCholeskySolver::Impl {
         switch (L.kind) {
             case 1: return lkd_chol_diag(prec, L.count, t, d_panel, d_diag, d_piv, d_stage, d_err, stream);
             case 2: return lkd_chol_update(prec, L.count, t, d_panel, pl, d_err, stream);
-            case 3: return lkd_chol_diagwb(prec, L.count, t, d_panel, d_diag, d_piv, stream);
+            case 3:   // only the reference-layout export reads the panel's diagonal blocks (the
+                      // solves use the row-major copies)
+                return d_csr ? lkd_chol_diagwb(prec, L.count, t, d_panel, d_diag, d_piv, stream) : 0;
             case 6: return lkd_chol_part(prec, L.count, t, d_panel, d_work, d_err, stream);
             case 7: return lkd_chol_step_dag(prec, L.count, t, d_fdeps + 4 * L.begin, d_fsdep + L.begin,
                                              d_fqueue + L.begin, d_fqoff + static_cast<int64_t>(L.q) * (n_xcd + 1),
