@@ -203,7 +203,8 @@ int lkd_chol_diagwb(int prec, int32_t ntask, const lkd_task* tasks, void* panel,
 /* Replaces cusparseSpSV (L and L^T) (src/cuda/kkt.cu:731-734,788-789).  x is the permuted
  * vector, solved in place: L z = x (dir 0, levels leaves first) or L^T x = z (dir 1, root
  * first), one launch per etree level, g = task.c0 workgroups of nw (4 or 8) wavefronts per
- * supernode (task.r0 = workgroup index; g = 1, or 4 for forward launches with nw = 8).  With
+ * supernode (task.r0 = workgroup index; g = 1, or with nw = 8: 4 for forward launches, 2 for
+ * backward ones).  With
  * g = 4, grid workgroups each read the XCD they run on (XCC_ID mod nx) and take their task from
  * that XCD's queue, queue[qoff[x] .. qoff[x+1]) (task indices; a supernode's g tasks
  * consecutive), through counter cnt[x]; grid >= nx times the longest queue.  The workgroups of

@@ -391,31 +391,43 @@ __global__ __launch_bounds__(64 * NW) void k_fwd_sn(const lkd_task* __restrict__
 // the part over in LDS (`cp`, two slots: slot j & 1
 // is rewritten only after block j + 2's owner read it, which precedes the solve of j + 1).
 // The other parts go through pbuf.
-template <typename T, int NW>
+template <typename T, int NW, int G>
 __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__ tasks,
                                                     const T* __restrict__ L,
                                                     const T* __restrict__ diag,
                                                     const int32_t* __restrict__ sn_rows,
                                                     double* __restrict__ x,
                                                     double* __restrict__ below,
-                                                    double* pbuf, int32_t* err) {
+                                                    double* pbuf, uint32_t* flags,
+                                                    const uint32_t* epoch, int32_t* err,
+                                                    XcdQueue xq) {
     constexpr int CW = 16;                            // columns per transposed chunk
     constexpr int NQ = TS / CW;
-    const lkd_task tk = tasks[blockIdx.x];
+    constexpr int NWT = G * NW;                       // wavefronts of the supernode
+    constexpr int NC = 64;                            // part counters (blocks in flight < NC)
+    __shared__ int s_ti;
+    const int ti = G > 1 ? xcd_task(xq, &s_ti) : (int)blockIdx.x;
+    if (ti < 0) return;
+    const lkd_task tk = tasks[ti];
     __shared__ double Ts[NW][CW * 65];                // per wavefront: CW columns x 64 rows
-    __shared__ double cp[2][TS];                      // critical-path parts, slot j & 1
+    __shared__ double cp[2][TS];                      // critical-path parts, slot j & 1 (G = 1)
     __shared__ int pub[NW];                           // lowest block each wavefront published
+    __shared__ int cnt[NC];                           // G > 1: parts stored per block (mod NC)
     const T* P = L + tk.off;
     const int64_t ld = tk.ld;
     const Tiles tl(ld, tk.ncol);
     const int nb = tl.nb, nt = tl.nt, nc = tk.ncol;
     const int v = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const Own<1, NW> own{v, 0};
+    const Own<G, NW> own{tk.r0 + G * v, tk.r0};
+    const uint32_t ep = G > 1 ? *epoch : 0u;
+    uint32_t* fl = flags + tk.px + nb;                // G > 1: [block][workgroup]
     const int cl = lane % CW, rg = lane / CW;         // column of a chunk, row group
     double* bl = below + tk.kbeg;
-    double* pb = pbuf + tk.ws;                        // [nb][NW][64]
+    double* pb = pbuf + tk.ws;                        // [nb][NWT][64], by wavefront residue
     double* Tw = Ts[v];
-    for (int t = own.first(nb); t < nt; t += NW) {
+    if (G > 1)
+        for (int i = threadIdx.x; i < NC; i += 64 * NW) cnt[i] = 0;
+    for (int t = own.first(nb); t < nt; t += NWT) {
         const int64_t row = tl.beg(t) + lane;
         if (row < tl.end(t)) bl[row - nc] = x[sn_rows[tk.aux2 + row]];
     }
@@ -454,8 +466,8 @@ __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__
         load_half<T>(h0, rowp(Pb, t0), ld, 0, kb, rok(t0));
         load_half<T>(h1, rowp(Pb, t0), ld, TS / 2, kb, rok(t0));
 #pragma unroll 1
-        for (int t = t0; t < tlim; t += NW) {
-            const int tn = t + NW < tlim ? t + NW : t;    // past the end: reload, unused
+        for (int t = t0; t < tlim; t += NWT) {
+            const int tn = t + NWT < tlim ? t + NWT : t;  // past the end: reload, unused
             const double xr = xval(t);
             chunk(part[0], h0, xr);
             chunk(part[1], h0 + CW, xr);
@@ -482,23 +494,32 @@ __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__
             chunk(pre[1], hc + CW, xlast);
             chunk(pre[2], h1, xlast);
             chunk(pre[3], h1 + CW, xlast);
+            double* dst = G == 1 ? cp[j & 1] : pb + ((int64_t)j * NWT + own.gw) * TS;
             if (lane < CW) {
 #pragma unroll
-                for (int q = 0; q < NQ; ++q) cp[j & 1][CW * q + lane] = pre[q];
+                for (int q = 0; q < NQ; ++q) dst[CW * q + lane] = pre[q];
             }
         } else {
             double part[NQ];
 #pragma unroll
             for (int q = 0; q < NQ; ++q) part[q] = 0.0;
             contrib_list(part, own.first(j + 1), j, nt);
-            double* pj = pb + ((int64_t)j * NW + v) * TS;
+            double* pj = pb + ((int64_t)j * NWT + own.gw) * TS;
             if (lane < CW) {
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) pj[CW * q + lane] = part[q];
             }
         }
         lds_publish(&pub[v], j);
-        if ((j % NW) != v) continue;
+        if (G > 1 && own.wg_of(j) != own.g) {
+            // every wavefront's part has reached the L2 before it counts itself; the last one
+            // of the workgroup publishes them to the owner's workgroup
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            int last = 0;
+            if (lane == 0) last = (atomicAdd(&cnt[j % NC], 1) + 1) % NW == 0;
+            if (__builtin_amdgcn_readfirstlane(last)) dev_publish(fl + (int64_t)j * G + own.g, ep);
+        }
+        if ((j % NWT) != own.gw) continue;
         // owner of block j: while the other parts arrive, start block j - 1 with the tiles
         // that do not need x_j, and load the first half of tile (j, j - 1)
         if (j > 0) {
@@ -529,20 +550,33 @@ __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__
         const double rhs = lane < kbj ? x[tk.first + j * TS + lane] : 0.0;
         // the parts in pbuf load while the critical-path part is still being formed; all
         // parts are then summed in wavefront order
-        const int prod = j + 1 < nb ? (j + 1) % NW : -1;    // wavefront whose part is in cp
-        for (int u = 0; u < NW; ++u)
-            if (u != v && u != prod && !lds_wait(&pub[u], [&](int p) { return p <= j; }, err)) return;
-        double pv[NW];
-        const double* p0 = pb + (int64_t)j * NW * TS + lane;
-#pragma unroll
-        for (int u = 0; u < NW; ++u) pv[u] = u == prod ? 0.0 : p0[(int64_t)u * TS];
-        if (prod >= 0 && !lds_wait(&pub[prod], [&](int p) { return p <= j; }, err)) return;
         double val = 0.0;
-        if (lane < kbj) {
-            double s = 0.0;
+        if constexpr (G == 1) {
+            const int prod = j + 1 < nb ? (j + 1) % NW : -1;    // wavefront whose part is in cp
+            for (int u = 0; u < NW; ++u)
+                if (u != v && u != prod && !lds_wait(&pub[u], [&](int p) { return p <= j; }, err)) return;
+            double pv[NW];
+            const double* p0 = pb + (int64_t)j * NW * TS + lane;
 #pragma unroll
-            for (int u = 0; u < NW; ++u) s += u == prod ? cp[j & 1][lane] : pv[u];
-            val = rhs - s;
+            for (int u = 0; u < NW; ++u) pv[u] = u == prod ? 0.0 : p0[(int64_t)u * TS];
+            if (prod >= 0 && !lds_wait(&pub[prod], [&](int p) { return p <= j; }, err)) return;
+            if (lane < kbj) {
+                double s = 0.0;
+#pragma unroll
+                for (int u = 0; u < NW; ++u) s += u == prod ? cp[j & 1][lane] : pv[u];
+                val = rhs - s;
+            }
+        } else {
+            for (int u = 0; u < NW; ++u)
+                if (u != v && !lds_wait(&pub[u], [&](int p) { return p <= j; }, err)) return;
+            for (int h = 0; h < G; ++h)
+                if (h != own.g && !dev_wait(fl + (int64_t)j * G + h, ep, err)) return;
+            if (lane < kbj) {
+                double s = 0.0;
+                const double* p0 = pb + (int64_t)j * NWT * TS + lane;
+                for (int r = 0; r < NWT; ++r) s += p0[(int64_t)r * TS];
+                val = rhs - s;
+            }
         }
         const double sol = bsub_regs<T>(val, col, rd);
         if (lane < kbj) x[tk.first + j * TS + lane] = sol;
@@ -554,14 +588,19 @@ template <typename T, int NW, int G>
 int level_solve(int dir, int32_t ntask, const lkd_task* tasks, const void* panel, const void* diag,
                 const int32_t* rows, double* x, double* below, double* pbuf, uint32_t* flags,
                 const uint32_t* epoch, int32_t* err, const XcdQueue& xq, int grid, void* stream) {
-    if (dir == 0)
-        LK_LAUNCH((k_fwd_sn<T, NW, G>), G > 1 ? grid : ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
-                  (const T*)diag, rows, x, below, pbuf, flags, epoch, err, xq);
-    else if constexpr (G == 1)
-        LK_LAUNCH((k_bwd_sn<T, NW>), ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
-                  (const T*)diag, rows, x, below, pbuf, err);
-    else
-        return (int)hipErrorInvalidValue;
+    if (dir == 0) {
+        if constexpr (G == 1 || G == 4)
+            LK_LAUNCH((k_fwd_sn<T, NW, G>), G > 1 ? grid : ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
+                      (const T*)diag, rows, x, below, pbuf, flags, epoch, err, xq);
+        else
+            return (int)hipErrorInvalidValue;
+    } else {
+        if constexpr (G == 1 || G == 2)
+            LK_LAUNCH((k_bwd_sn<T, NW, G>), G > 1 ? grid : ntask, 64 * NW, 0, stream, tasks, (const T*)panel,
+                      (const T*)diag, rows, x, below, pbuf, flags, epoch, err, xq);
+        else
+            return (int)hipErrorInvalidValue;
+    }
     return lk_last_error();
 }
 
@@ -578,6 +617,8 @@ int level_solve_cfg(int nw, int g, int dir, int32_t ntask, const lkd_task* tasks
                     const void* diag, const int32_t* rows, double* x, double* below, double* pbuf,
                     uint32_t* flags, const uint32_t* epoch, int32_t* err, const XcdQueue& xq,
                     int grid, void* stream) {
+    if (nw == 8 && g == 2)
+        return level_solve<T, 8, 2>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, xq, grid, stream);
     if (nw == 8 && g == 4)
         return level_solve<T, 8, 4>(dir, ntask, tasks, panel, diag, rows, x, below, pbuf, flags, epoch, err, xq, grid, stream);
     if (nw == 8)
@@ -595,7 +636,8 @@ int lkd_trsv_level(int prec, int dir, int nw, int g, int32_t ntask, const lkd_ta
                    const int32_t* queue, const int32_t* qoff, int32_t nx, uint32_t* cnt,
                    int32_t grid, int32_t* err, void* stream) {
     if (ntask <= 0) return 0;
-    if ((nw != 4 && nw != 8) || (g != 1 && !(g == 4 && nw == 8 && dir == 0))) return (int)hipErrorInvalidValue;
+    if ((nw != 4 && nw != 8) || (g != 1 && !(g == 4 && nw == 8 && dir == 0) && !(g == 2 && nw == 8 && dir == 1)))
+        return (int)hipErrorInvalidValue;
     if (g > 1 && (!queue || !qoff || !cnt || nx < 1 || grid < 1)) return (int)hipErrorInvalidValue;
     const XcdQueue xq{queue, qoff, cnt, nx};
     return prec == LKD_F32

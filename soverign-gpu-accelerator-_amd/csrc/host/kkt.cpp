@@ -44,6 +44,7 @@ constexpr int STEPS_PART_LAG = 1;   // STEPS: steps between a PART's inputs and 
 constexpr int STEPS_SPLIT_CLS = 2;  // STEPS: tiles of class <= this (0 next-diagonal, 1 below it, 2 others) split
 constexpr int SOLVE_WIDE_TILES = 8;   // level solves: 8 wavefronts per supernode above this
 constexpr int SOLVE_MAX_G = 4;        // forward level solves: workgroups per wide supernode
+constexpr int SOLVE_BWD_G = 2;        // backward level solves: workgroups per wide supernode
 constexpr int FWD_ROWS = 256;
 
 // Order a launch's tasks for dispatch.  Workgroup b runs task (b % 8) * per + b / 8 with
@@ -877,7 +878,8 @@ struct GPUKKTCholeskySolver::Impl {
             case 21: {  // level solves: workgroups, wavefronts, scratch / exchange / flag offsets
                 const int nb = (nc + TILE - 1) / TILE;
                 if (t.nsplit != 4 && t.nsplit != 8) bad("solve wavefronts");
-                if ((t.c0 != 1 && !(t.c0 == SOLVE_MAX_G && t.nsplit == 8 && kind == 20)) || t.r0 < 0 || t.r0 >= t.c0)
+                if ((t.c0 != 1 && !(t.c0 == (kind == 20 ? SOLVE_MAX_G : SOLVE_BWD_G) && t.nsplit == 8)) ||
+                    t.r0 < 0 || t.r0 >= t.c0)
                     bad("solve workgroups");
                 if (t.ws < 0 || t.ws + int64_t(nb) * t.c0 * t.nsplit * TILE > solve_pbuf || t.kbeg < 0 ||
                     t.kbeg + (ld - nc) > solve_below || t.px < 0 || t.px + nb + int64_t(nb) * t.c0 > solve_flags)
@@ -895,7 +897,8 @@ struct GPUKKTCholeskySolver::Impl {
     // launches of wide levels whose G = SOLVE_MAX_G workgroups per supernode all fit on the
     // compute units (one each) spread every supernode over G workgroups; they take their tasks
     // from per-XCD queues, so a supernode's workgroups share an L2 and hand off without L2
-    // write-backs.  The backward direction keeps G = 1.
+    // write-backs.  The backward direction uses SOLVE_BWD_G workgroups on those levels (its
+    // hand-offs between workgroups are per block, so fewer of them pay).
     // task.nsplit = wavefronts, c0 = G, r0 = g, ws = exchange offset, px = flag offset,
     // kbeg = scratch offset.
     void build_solve_plan() {
@@ -943,9 +946,13 @@ struct GPUKKTCholeskySolver::Impl {
                     u.r0 = g;
                     lt.push_back(u);
                 }
-                lkd_task u = t;
-                u.c0 = 1;
-                lb.push_back(u);
+                const int Gb = G > 1 ? SOLVE_BWD_G : 1;
+                for (int g = 0; g < Gb; ++g) {
+                    lkd_task u = t;
+                    u.c0 = Gb;
+                    u.r0 = g;
+                    lb.push_back(u);
+                }
             }
             per_level.push_back(std::move(lt));
             per_level_bwd.push_back(std::move(lb));
