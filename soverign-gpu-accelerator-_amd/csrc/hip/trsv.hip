@@ -98,11 +98,11 @@ __device__ __forceinline__ double bsub_regs(double r, const T (&col)[TS], double
 //            (diagonal tiles in x, rows below in a scratch vector that is added to the
 //            ancestors' entries with atomics at the end).  A ring slot is reused only after
 //            every wavefront of the workgroup finished the iteration that read it (`prog`).
-//  backward  (G = 1) for j = nb-1 .. 0: every wavefront forms part_v = sum over its tiles
-//            t > j of L(t, j)^T x_t (LDS-transposed CW-column chunks, fixed reduction order),
-//            stores it in `pbuf` (the owner of j + 1: in LDS) and publishes; the owner of j
-//            waits for all parts, sums them in wavefront order (deterministic), solves its
-//            diagonal block and keeps x_j.
+//  backward  (G = 1 or 2) for j = nb-1 .. 0: every wavefront forms part_v = sum over its
+//            tiles t > j of L(t, j)^T x_t (LDS-transposed CW-column chunks, fixed reduction
+//            order), stores it in `pbuf` (G = 1, the owner of j + 1: in LDS) and publishes; the
+//            owner of j waits for all parts, sums them in residue order (deterministic), solves
+//            its diagonal block and keeps x_j.  Four workgroups measured slower than two.
 // Task fields: aux = diagonal-copy offset of block 0 (block j at aux + 4096 j), aux2 = row-list
 // offset, kbeg = offset of the rows below in `below`, r0 = g, c0 = G, ws = offset of the
 // supernode's area in `pbuf`, px = offset of its flags.  Waits are bounded; a timeout sets
@@ -382,15 +382,17 @@ __global__ __launch_bounds__(64 * NW) void k_fwd_sn(const lkd_task* __restrict__
     }
 }
 
-// Backward level solve, one workgroup per supernode (task r0 = 0, c0 = 1).  ws = partial sums
-// in pbuf (nb * NW * 64 doubles: [block][wavefront][64]).  Block j's owner (wavefront j % NW)
-// sums the parts of all wavefronts in wavefront order.  The part of the owner of block j + 1
-// is on the critical path (it needs x_{j+1}): that wavefront computes everything else early
-// (look-ahead), keeps x_{j+1} in registers after its solve, has the first half of tile
-// (j + 1, j) in registers already (FP32 panels; FP64 ones have no registers to spare), and hands
-// the part over in LDS (`cp`, two slots: slot j & 1
-// is rewritten only after block j + 2's owner read it, which precedes the solve of j + 1).
-// The other parts go through pbuf.
+// Backward level solve, G = 1 or 2 workgroups per supernode (task r0 = g, c0 = G; with G = 2
+// from per-XCD queues, like the forward).  ws = partial sums in pbuf (nb * G * NW * 64 doubles:
+// [block][wavefront residue][64], see Own).  Block j's owner sums the parts of all wavefronts
+// in residue order.  The part of the owner of block j + 1 is on the critical path (it needs
+// x_{j+1}): that wavefront computes everything else early (look-ahead), keeps x_{j+1} in
+// registers after its solve, has the first half of tile (j + 1, j) in registers already (FP32
+// panels; FP64 ones have no registers to spare) and, with G = 1, hands the part over in LDS
+// (`cp`, two slots: slot j & 1 is rewritten only after block j + 2's owner read it, which
+// precedes the solve of j + 1).  The other parts go through pbuf; with G = 2 the wavefronts of
+// the workgroup that does not own block j publish theirs with one flag per block and workgroup
+// (each waits for its own stores first, the last one to count itself sets the flag).
 template <typename T, int NW, int G>
 __global__ __launch_bounds__(64 * NW) void k_bwd_sn(const lkd_task* __restrict__ tasks,
                                                     const T* __restrict__ L,
