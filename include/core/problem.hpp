@@ -1,7 +1,9 @@
 // LP model in equality form: min/max c^T x  s.t.  A x = b,  lb <= x <= ub.
-// Interface of src/core/problem.hpp:40-119 (reference). The MINLP bilinear-term list of the
-// reference model is not carried: nothing on the interior-point path reads it.
+// Interface and member layout of src/core/problem.hpp:26-119 (reference), so reference code
+// built against this header (the MILP / McCormick units included) sees the same class.  The
+// bilinear-term list is carried but nothing on the interior-point path reads it.
 #pragma once
+#include <array>
 #include <stdexcept>
 #include <vector>
 #include "sankhya/types.hpp"
@@ -9,6 +11,14 @@
 
 namespace sankhya {
 namespace core {
+
+// w = x * y with its four McCormick envelope rows (src/core/problem.hpp:26-31)
+struct BilinearTerm {
+    Index x_col;
+    Index y_col;
+    Index w_col;
+    std::array<Index, 4> mccormick_rows;
+};
 
 class Model {
 public:
@@ -19,6 +29,7 @@ public:
     std::vector<VariableType> vtype;
     std::vector<Float> rhs;
     CSCMatrix A;
+    std::vector<BilinearTerm> bilinear_terms;
     std::vector<Float> row_scale;   // optional presolve scaling (empty = unscaled)
     std::vector<Float> col_scale;
 

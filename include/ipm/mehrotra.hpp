@@ -10,8 +10,8 @@
 // 30 trials), 200 iterations; numerical failures end in SimplexStatus::IterationLimit.
 //
 // Extensions (IpmOptions): a 1e-8 relative-gap stop rule evaluated in the original space,
-// FP32-factor mixed precision with FP64 refinement and an FP64 fallback, a warm start, the
-// ordering choice and hipGraph capture of the factorization / solve launch sequences.
+// FP32-factor mixed precision with FP64 refinement and an FP64 fallback, a warm start and the
+// ordering choice.
 #pragma once
 #include <vector>
 #include "core/problem.hpp"
@@ -64,6 +64,14 @@ struct IpmOptions {
     const Float* warm_s = nullptr;
     Float warm_floor = 1e-3;          // lower clamp of warm-start x and s (scaled space)
 };
+
+// The solver's equilibration, run once in its constructor: 10 Jacobi Ruiz passes
+// (src/ipm/mehrotra_device.cu:286-323).  Each pass takes every row and column max-abs from the
+// same snapshot of A, then scales row i by 1/sqrt(rmax_i) and column j by 1/sqrt(cmax_j) (a
+// factor 1 where the max is <= 1e-8); b and c follow their rows / columns.  On return A holds
+// R A C, b holds R b, c holds C c, and R (m) and C (n) the accumulated factors.
+void ruiz_equilibrate(core::CSRMatrix& A, std::vector<Float>& b, std::vector<Float>& c,
+                      std::vector<Float>& R, std::vector<Float>& C);
 
 class MehrotraSolver {
 public:

@@ -124,6 +124,11 @@ typedef struct {
 } lk_ipm_result;
 
 int lk_ipm_default_options(lk_ipm_options* opt);
+/* The solver's equilibration (host only): 10 Jacobi Ruiz passes of the MehrotraSolver
+ * constructor (src/ipm/mehrotra_device.cu:286-323).  A in CSR; vals (nnz), b (m), c (n) are
+ * scaled in place; R (m) and C (n) receive the factors (scaled A = R A C). */
+int lk_ruiz_scaling(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                    double* vals, double* b, double* c, double* R, double* C);
 /* One LP in CSC (rows ascending per column): construct + solve.  lb/ub may be NULL (0, +inf);
  * warm_* (original space) optional; x (n), y (m), s (n) outputs optional. */
 int lk_ipm_solve(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,

@@ -48,6 +48,11 @@ int lkd_event_elapsed_ms(float* ms, void* start, void* stop);
 int lkd_fill_f64(double* dst, double value, int64_t n, void* stream);   /* kernel fill */
 /* streaming copy kernel (n even, 16-byte aligned): calibrates HBM traffic counters */
 int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream);
+/* Queue completeness after launches that deal tasks from per-XCD queues: for every queue q < nq
+ * and XCD x < nx, counter cnt[q nx + x] (tickets taken) must have reached the queue's length
+ * qoff[q (nx + 1) + x + 1] - qoff[q (nx + 1) + x]; otherwise `bit` is OR-ed into *err. */
+int lkd_queue_check(const uint32_t* cnt, const int32_t* qoff, int32_t nq, int32_t nx,
+                    int32_t bit, int32_t* err, void* stream);
 
 /* ---------------------------------------------------------------- supernodal plan (device) */
 /* One work item of the factorization / triangular-solve schedule.  Field meaning per kernel:
@@ -275,6 +280,7 @@ enum {
     LKS_R1, LKS_R2, LKS_R3, LKS_RXS2, LKS_APMAX, LKS_ADMAX, LKS_DX2, LKS_DS2, /* direction */
     LKS_MUAFF, LKS_SIGMAMU, LKS_AP, LKS_AD, LKS_DY2, LKS_KKTR,
     LKS_XSMAX,                                          /* max_j |x_j (s_j - rd_j)| */
+    LKS_THMAX, LKS_THMIN,                               /* max_j, min_j x_j / s_j */
     LKS_NSLOTS
 };
 #define LKD_MAX_TRIALS 32

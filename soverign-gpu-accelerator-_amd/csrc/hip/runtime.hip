@@ -13,6 +13,16 @@ __global__ void k_copy(double2* __restrict__ d, const double2* __restrict__ s, i
          i += (int64_t)gridDim.x * blockDim.x)
         d[i] = s[i];
 }
+// One thread per (queue, XCD) counter: a queue whose counter stayed below its length had
+// tickets no workgroup took, so those tasks never ran.
+__global__ void k_queue_check(const uint32_t* __restrict__ cnt, const int32_t* __restrict__ qoff,
+                              int nq, int nx, int bit, int32_t* err) {
+    for (int i = threadIdx.x; i < nq * nx; i += blockDim.x) {
+        const int q = i / nx, x = i % nx;
+        const int64_t len = qoff[(int64_t)q * (nx + 1) + x + 1] - qoff[(int64_t)q * (nx + 1) + x];
+        if ((int64_t)cnt[i] < len) atomicOr(err, bit);
+    }
+}
 }  // namespace
 
 extern "C" {
@@ -85,6 +95,14 @@ int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream) {
     const int64_t n2 = n / 2;
     const int64_t blocks = (n2 + 255) / 256 < 8192 ? (n2 + 255) / 256 : 8192;
     LK_LAUNCH(k_copy, (int)blocks, 256, 0, stream, (double2*)dst, (const double2*)src, n2);
+    return lk_last_error();
+}
+
+int lkd_queue_check(const uint32_t* cnt, const int32_t* qoff, int32_t nq, int32_t nx,
+                    int32_t bit, int32_t* err, void* stream) {
+    if (nq <= 0) return 0;
+    if (nx < 1 || !cnt || !qoff || !err) return (int)hipErrorInvalidValue;
+    LK_LAUNCH(k_queue_check, 1, 256, 0, stream, cnt, qoff, nq, nx, bit, err);
     return lk_last_error();
 }
 

@@ -490,11 +490,14 @@ __global__ __launch_bounds__(NT) void k_resid_cols(lkd_ipm P, const double* __re
                                                    double* __restrict__ rd,
                                                    double* __restrict__ theta, int qoff) {
     __shared__ double sh[4];
-    double s2 = 0.0, xs = 0.0, cx = 0.0, inf = 0.0, xsmax = 0.0;
+    double s2 = 0.0, xs = 0.0, cx = 0.0, inf = 0.0, xsmax = 0.0, thmax = -1e300, thmin = 1e300;
     for_rows<G>(P.n, P.trp, P.tci, P.tval, y, P.long_cols, sh, [&](int64_t j, double aty) {
         const double v = aty + (s[j] - P.c[j]);
+        const double t = x[j] / s[j];
         rd[j] = v;
-        if (theta) theta[j] = x[j] / s[j];
+        if (theta) theta[j] = t;
+        thmax = fmax(thmax, t);
+        thmin = fmin(thmin, t);
         s2 += v * v;
         xs += x[j] * s[j];
         cx += P.c[j] * x[j];
@@ -509,6 +512,8 @@ __global__ __launch_bounds__(NT) void k_resid_cols(lkd_ipm P, const double* __re
     r = block_reduce(cx, OP_SUM, sh); if (threadIdx.x == 0) part[2 * nb + blockIdx.x] = r;
     r = block_reduce(inf, OP_MAX, sh); if (threadIdx.x == 0) part[3 * nb + blockIdx.x] = r;
     r = block_reduce(xsmax, OP_MAX, sh); if (threadIdx.x == 0) part[4 * nb + blockIdx.x] = r;
+    r = block_reduce(thmax, OP_MAX, sh); if (threadIdx.x == 0) part[5 * nb + blockIdx.x] = r;
+    r = block_reduce(thmin, OP_MIN, sh); if (threadIdx.x == 0) part[6 * nb + blockIdx.x] = r;
 }
 
 __global__ __launch_bounds__(NT) void k_rhs_cols(lkd_ipm P, const double* __restrict__ x,
@@ -904,9 +909,9 @@ int lkd_ipm_residuals(const lkd_ipm* P, const double* x, const double* y, const 
     // column pass writes its partials after the three row quantities
     const int qoff = (3 * P->nblk_rows + P->nblk_cols - 1) / P->nblk_cols;
     LK_GSWITCH(P->grp_cols, l_resid_cols, P, x, y, s, rd, theta, qoff, stream);
-    int ops_c[5] = {OP_SUM, OP_SUM, OP_SUM, OP_MAX, OP_MAX};
-    int slots_c[5] = {LKS_RD2, LKS_XS, LKS_CX, LKS_RDINF, LKS_XSMAX};
-    return finalize(P->part + (int64_t)qoff * P->nblk_cols, P->nblk_cols, 5, ops_c, slots_c, P->S, stream);
+    int ops_c[7] = {OP_SUM, OP_SUM, OP_SUM, OP_MAX, OP_MAX, OP_MAX, OP_MIN};
+    int slots_c[7] = {LKS_RD2, LKS_XS, LKS_CX, LKS_RDINF, LKS_XSMAX, LKS_THMAX, LKS_THMIN};
+    return finalize(P->part + (int64_t)qoff * P->nblk_cols, P->nblk_cols, 7, ops_c, slots_c, P->S, stream);
 }
 
 int lkd_ipm_rhs(const lkd_ipm* P, const double* x, const double* s, const double* theta,

@@ -43,7 +43,11 @@ void* VRAMArena::allocate_impl(std::size_t bytes, bool zero) {
         live_[off] = need;
         char* p = base_ + off;
         if (zero) {
-            const int e = lkd_memset(p, 0, need, nullptr);
+            // The fill is finished before the pointer is handed out: the callers' streams are
+            // non-blocking, so a fill left queued on the null stream would be ordered with
+            // none of their work and could land after (and wipe) their first upload.
+            int e = lkd_memset(p, 0, need, nullptr);
+            if (e == 0) e = lkd_stream_sync(nullptr);
             if (e != 0) throw std::runtime_error(std::string("VRAMArena: zero fill failed: ") +
                                                  lkd_error_string(e));
         }
@@ -57,11 +61,11 @@ void VRAMArena::free(void* ptr) {
     std::lock_guard<std::mutex> lock(mu_);
     const char* p = static_cast<const char*>(ptr);
     if (p < base_ || p >= base_ + capacity_)
-        throw std::invalid_argument("VRAMArena::free - Pointer is outside the persistent device buffer.");
+        throw std::invalid_argument("VRAMArena::free: the pointer does not lie in this arena's slab");
     const std::size_t off = static_cast<std::size_t>(p - base_);
     auto it = live_.find(off);
     if (it == live_.end())
-        throw std::invalid_argument("VRAMArena::free - Invalid or double-free pointer.");
+        throw std::invalid_argument("VRAMArena::free: no live allocation starts at this pointer (freed twice, or interior)");
     const std::size_t size = it->second;
     live_.erase(it);
     in_use_ -= size;

@@ -415,6 +415,21 @@ int lk_ipm_default_options(lk_ipm_options* o) {
     });
 }
 
+int lk_ruiz_scaling(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
+                    double* vals, double* b, double* c, double* R, double* C) {
+    return guarded([&] {
+        if (!vals || !b || !c || !R || !C) throw std::invalid_argument("lk_ruiz_scaling: null argument");
+        auto A = make_csr(m, n, rowptr, colidx, vals);
+        std::vector<Float> bv(b, b + m), cv(c, c + n), Rv, Cv;
+        ipm::ruiz_equilibrate(A, bv, cv, Rv, Cv);
+        std::copy(A.values.begin(), A.values.end(), vals);
+        std::copy(bv.begin(), bv.end(), b);
+        std::copy(cv.begin(), cv.end(), c);
+        std::copy(Rv.begin(), Rv.end(), R);
+        std::copy(Cv.begin(), Cv.end(), C);
+    });
+}
+
 int lk_ipm_solve(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
                  const double* vals, const double* c, const double* b, const double* lb,
                  const double* ub, const lk_ipm_options* opt, const double* warm_x,

@@ -63,5 +63,10 @@ bool verify_optimal(const core::Model& model, const std::vector<Float>& x,
     return true;
 }
 
+bool verify_optimal(const core::Model& model, const std::vector<Float>& x,
+                    const std::vector<Float>& pi, Float* out_primal_residual) {
+    return verify_optimal(model, x, pi, out_primal_residual, 1e-6);
+}
+
 }  // namespace verifier
 }  // namespace sankhya

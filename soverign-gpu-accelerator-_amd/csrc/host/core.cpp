@@ -3,6 +3,7 @@
 // finalize() buckets rows into columns in row order; to_csr() is a stable counting sort.
 #include <algorithm>
 #include <stdexcept>
+#include <string>
 #include <utility>
 #include "core/problem.hpp"
 #include "core/sparse_matrix.hpp"
@@ -58,10 +59,10 @@ void Model::add_variable(Float obj_coeff, Float lower, Float upper, VariableType
 static void check_row(const std::vector<Index>& cols, const std::vector<Float>& vals,
                       Index nvars, const char* who) {
     if (cols.size() != vals.size())
-        throw std::invalid_argument(std::string(who) + ": cols and vals must have the same size");
+        throw std::invalid_argument(std::string(who) + ": " + std::to_string(cols.size()) + " column indices but " + std::to_string(vals.size()) + " values");
     for (Index c : cols)
         if (c < 0 || c >= nvars)
-            throw std::invalid_argument(std::string(who) + ": column index out of range");
+            throw std::invalid_argument(std::string(who) + ": column " + std::to_string(c) + " is not a variable");
 }
 
 void Model::add_constraint(const std::vector<Index>& cols, const std::vector<Float>& vals,
@@ -75,7 +76,7 @@ void Model::add_constraint(const std::vector<Index>& cols, const std::vector<Flo
 void Model::update_constraint(Index row, const std::vector<Index>& cols,
                               const std::vector<Float>& vals, Float b) {
     if (row < 0 || static_cast<size_t>(row) >= rhs.size())
-        throw std::out_of_range("update_constraint: row index out of bounds");
+        throw std::out_of_range("Model::update_constraint: no row " + std::to_string(row));
     check_row(cols, vals, static_cast<Index>(vtype.size()), "update_constraint");
     constraint_cols_[static_cast<size_t>(row)] = cols;
     constraint_vals_[static_cast<size_t>(row)] = vals;
@@ -84,13 +85,13 @@ void Model::update_constraint(Index row, const std::vector<Index>& cols,
 
 const std::vector<Index>& Model::get_constraint_cols(Index row) const {
     if (row < 0 || static_cast<size_t>(row) >= constraint_cols_.size())
-        throw std::out_of_range("get_constraint_cols: row index out of bounds");
+        throw std::out_of_range("Model::get_constraint_cols: no row " + std::to_string(row));
     return constraint_cols_[static_cast<size_t>(row)];
 }
 
 const std::vector<Float>& Model::get_constraint_vals(Index row) const {
     if (row < 0 || static_cast<size_t>(row) >= constraint_vals_.size())
-        throw std::out_of_range("get_constraint_vals: row index out of bounds");
+        throw std::out_of_range("Model::get_constraint_vals: no row " + std::to_string(row));
     return constraint_vals_[static_cast<size_t>(row)];
 }
 

@@ -1117,6 +1117,7 @@ struct GPUKKTCholeskySolver::Impl {
         if (n_dag > 0) check(lkd_chol_dag_begin(d_dag_ctl, n_dag * n_xcd, stream), "steps begin");
         if (!timer.on) {
             for (const auto& L : flaunch) check(launch_factor(prec, L, pl), "factor kernel");
+            check_factor_queues();
             return;
         }
         // one event at every change of launch kind: a segment of consecutive launches of one
@@ -1147,6 +1148,14 @@ struct GPUKKTCholeskySolver::Impl {
             }
         }
         timer.pending.push_back({4, first, timer.used - 1, 1, S.flops_exact});
+        check_factor_queues();
+    }
+
+    // Every ticket of every STEPS queue must have been taken (error bit 0x40 otherwise): a task
+    // nobody waits on would otherwise be skipped without a trace.
+    void check_factor_queues() {
+        if (n_dag > 0)
+            check(lkd_queue_check(d_dag_ctl + 1, d_fqoff, n_dag, n_xcd, 0x40, d_err, stream), "queue check");
     }
 
     // 0 STEP, 1 DIAG + DIAGWB, 2 UPDATE, 5 PART (3 assembly and 4 totals are not launch kinds)
@@ -1254,6 +1263,8 @@ struct GPUKKTCholeskySolver::Impl {
             launches[3] = 4;
             flops[3] = 2.0 * static_cast<double>(npairs);
         }
+        // a new STEPS epoch and fresh queue counters, as in every factorization
+        if (n_dag > 0) check(lkd_chol_dag_begin(d_dag_ctl, n_dag * n_xcd, stream), "steps begin");
         for (const auto& L : flaunch) {
             check(lkd_event_record(e0, stream), "event");
             check(launch_factor(prec, L, pl), "factor kernel");
@@ -1318,6 +1329,9 @@ struct GPUKKTCholeskySolver::Impl {
                                  q ? epoch + 1 + static_cast<int64_t>(L.q) * n_xcd : nullptr, L.grid,
                                  d_serr, stream), "solve kernel");
         }
+        // every ticket of the multi-workgroup level queues was taken (bit 0x40 otherwise)
+        if (n_sq > 0)
+            check(lkd_queue_check(epoch + 1, d_sqoff, n_sq, n_xcd, 0x40, d_serr, stream), "queue check");
     }
 
     // A level solve whose waits timed out produced an invalid vector: report it.  The status
@@ -1329,6 +1343,7 @@ struct GPUKKTCholeskySolver::Impl {
         h_serr = 0;
         check(lkd_memset(d_serr, 0, sizeof(int32_t), stream), "reset");
         check(lkd_stream_sync(stream), "reset");
+        if (e & 0x40) throw std::runtime_error("KKT solve: a level-solve queue was not drained");
         throw std::runtime_error("KKT solve: level synchronisation timed out");
     }
 
@@ -1360,6 +1375,7 @@ struct GPUKKTCholeskySolver::Impl {
         st.factorizations++;
         if (err & 2) throw std::logic_error("KKT factorization: update outside the L structure");
         if (err & 0x20) throw std::runtime_error("KKT factorization: a STEPS wait exceeded its poll limit");
+        if (err & 0x40) throw std::runtime_error("KKT factorization: a STEPS queue was not drained");
         return err == 0;
     }
 

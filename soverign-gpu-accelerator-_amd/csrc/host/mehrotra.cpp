@@ -53,6 +53,40 @@ double ms_since(clk::time_point t) {
 }
 }  // namespace
 
+void ruiz_equilibrate(core::CSRMatrix& A, std::vector<Float>& b, std::vector<Float>& c,
+                      std::vector<Float>& R, std::vector<Float>& C) {
+    const Index m = A.rows, n = A.cols;
+    if (static_cast<Index>(b.size()) != m || static_cast<Index>(c.size()) != n)
+        throw std::invalid_argument("ruiz_equilibrate: dimension mismatch");
+    R.assign(static_cast<size_t>(m), 1.0);
+    C.assign(static_cast<size_t>(n), 1.0);
+    // per-pass factors: 1/sqrt(max |a|) over the row / column, 1 for (near) empty ones
+    std::vector<Float> fr(static_cast<size_t>(m)), fc(static_cast<size_t>(n));
+    for (int pass = 0; pass < 10; ++pass) {
+        std::fill(fr.begin(), fr.end(), 0.0);
+        std::fill(fc.begin(), fc.end(), 0.0);
+        for (Index i = 0; i < m; ++i)
+            for (Index p = A.row_ptrs[i]; p < A.row_ptrs[i + 1]; ++p) {
+                const Float a = std::abs(A.values[p]);
+                const Index j = A.col_indices[p];
+                if (a > fr[i]) fr[i] = a;
+                if (a > fc[j]) fc[j] = a;
+            }
+        for (Float& f : fr) f = f > 1e-8 ? 1.0 / std::sqrt(f) : 1.0;
+        for (Float& f : fc) f = f > 1e-8 ? 1.0 / std::sqrt(f) : 1.0;
+        for (Index i = 0; i < m; ++i) {
+            R[i] *= fr[i];
+            b[i] *= fr[i];
+            for (Index p = A.row_ptrs[i]; p < A.row_ptrs[i + 1]; ++p)
+                A.values[p] *= fr[i] * fc[A.col_indices[p]];
+        }
+        for (Index j = 0; j < n; ++j) {
+            C[j] *= fc[j];
+            c[j] *= fc[j];
+        }
+    }
+}
+
 class MehrotraSolver::Impl {
 public:
     const core::Model& model_;
@@ -200,39 +234,7 @@ public:
         if (stream_) lkd_stream_destroy(stream_);
     }
 
-    // 10 Jacobi Ruiz passes (src/ipm/mehrotra_device.cu:286-323)
-    void ruiz() {
-        R_.assign(static_cast<size_t>(m_), 1.0);
-        C_.assign(static_cast<size_t>(n_), 1.0);
-        std::vector<Float> rmax(static_cast<size_t>(m_)), cmax(static_cast<size_t>(n_));
-        for (int it = 0; it < 10; ++it) {
-            std::fill(rmax.begin(), rmax.end(), 0.0);
-            std::fill(cmax.begin(), cmax.end(), 0.0);
-            for (Index i = 0; i < m_; ++i)
-                for (Index p = A_.row_ptrs[i]; p < A_.row_ptrs[i + 1]; ++p) {
-                    const Float a = std::abs(A_.values[p]);
-                    rmax[i] = std::max(rmax[i], a);
-                    cmax[A_.col_indices[p]] = std::max(cmax[A_.col_indices[p]], a);
-                }
-            for (Index i = 0; i < m_; ++i) {
-                const Float f = rmax[i] > 1e-8 ? 1.0 / std::sqrt(rmax[i]) : 1.0;
-                R_[i] *= f;
-                b_[i] *= f;
-            }
-            for (Index j = 0; j < n_; ++j) {
-                const Float f = cmax[j] > 1e-8 ? 1.0 / std::sqrt(cmax[j]) : 1.0;
-                C_[j] *= f;
-                c_[j] *= f;
-            }
-            for (Index i = 0; i < m_; ++i)
-                for (Index p = A_.row_ptrs[i]; p < A_.row_ptrs[i + 1]; ++p) {
-                    const Float fr = rmax[i] > 1e-8 ? 1.0 / std::sqrt(rmax[i]) : 1.0;
-                    const Index j = A_.col_indices[p];
-                    const Float fc = cmax[j] > 1e-8 ? 1.0 / std::sqrt(cmax[j]) : 1.0;
-                    A_.values[p] *= fr * fc;
-                }
-        }
-    }
+    void ruiz() { ruiz_equilibrate(A_, b_, c_, R_, C_); }
 
     void read_scalars(int count) {
         check(lkd_memcpy_dtoh(hS_.data(), S_, sizeof(double) * count, stream_), "scalars");
@@ -528,12 +530,10 @@ public:
     }
     void clear_warm() { opt_.warm_x = opt_.warm_y = opt_.warm_s = nullptr; }
 
-    double theta_spread() {
-        // max(Theta) / min(Theta) from a device copy (small; only used in mixed precision)
-        std::vector<double> th(static_cast<size_t>(n_));
-        check(lkd_memcpy_dtoh(th.data(), th_, sizeof(double) * n_, stream_), "theta");
-        double lo = 1e300, hi = 0;
-        for (double t : th) { lo = std::min(lo, t); hi = std::max(hi, t); }
+    // max(Theta) / min(Theta) of the current iterate: both extremes come with the residual
+    // scalars read at the top of the iteration (the pass that forms Theta reduces them)
+    double theta_spread() const {
+        const double hi = hS_[LKS_THMAX], lo = hS_[LKS_THMIN];
         return lo > 0 ? hi / lo : 1e300;
     }
 };

@@ -92,6 +92,8 @@ def lib():
         "lk_kkt_stream": [_vp, C.POINTER(_vp)],
         "lk_kkt_destroy": [_vp],
         "lk_ipm_default_options": [C.POINTER(IpmOptions)],
+        "lk_ruiz_scaling": [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                            _f64p],
         "lk_ipm_solve": [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _vp, _vp,
                          C.POINTER(IpmOptions), _vp, _vp, _vp, C.POINTER(IpmResult), _vp, _vp,
                          _vp],
@@ -374,6 +376,18 @@ class IPM:
             self.close()
         except Exception:
             pass
+
+
+def ruiz_scaling(rowptr, colidx, vals, b, c):
+    """The solver's 10-pass Jacobi Ruiz equilibration (host code of the library).
+    Returns (scaled values, scaled b, scaled c, R, C)."""
+    rp, ci = _i32(rowptr), _i32(colidx)
+    m, n = len(rp) - 1, len(c)
+    v = np.array(vals, dtype=np.float64)
+    bb, cc = np.array(b, dtype=np.float64), np.array(c, dtype=np.float64)
+    R, Cs = np.empty(m), np.empty(n)
+    _ck(lib().lk_ruiz_scaling(m, n, rp, ci, v, bb, cc, R, Cs))
+    return v, bb, cc, R, Cs
 
 
 def verify_optimal(lp, x, pi, tol=1e-6, lb=None, ub=None, maximize=False):

@@ -1,0 +1,150 @@
+// A reference-side caller compiled against this repository's include/ and linked to
+// liblakshya.so: it drives the C++ classes exactly as the reference's own callers do.
+//   kkt   -- GPUKKTCholeskySolver through the four-argument constructor with an explicit
+//            M_pattern, host factorize, get_device_L_values() read back in the reference CSR
+//            layout, host solve (the steps of tests/unit/test_gpu_kkt.cu:21-121);
+//   ipm   -- MehrotraSolver(model).solve() on the 2 x 4 textbook LP built with add_variable /
+//            add_constraint / finalize (tests/unit/test_mehrotra.cu:29-78), then the
+//            reference-signature verify_optimal(model, x, pi, &res);
+//   model -- the Model layout: bilinear_terms is a member as in src/core/problem.hpp:49.
+// Exit status 0 when every check holds; each failed check prints one line.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "core/problem.hpp"
+#include "gpu/kkt.hpp"
+#include "gpu/vram_arena.hpp"
+#include "ipm/mehrotra.hpp"
+#include "ipm/symbolic.hpp"
+#include "verifier/certificate.hpp"
+
+using namespace sankhya;
+
+static int g_fail = 0;
+#define EXPECT(cond)                                                         \
+    do {                                                                     \
+        if (!(cond)) {                                                       \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);      \
+            ++g_fail;                                                        \
+        }                                                                    \
+    } while (0)
+
+static int run_kkt() {
+    // A (3 x 5) whose normal matrix with Theta = 1 is [[6,1,1],[1,5,0],[1,0,5]]
+    core::CSRMatrix A;
+    A.rows = 3;
+    A.cols = 5;
+    A.row_ptrs = {0, 3, 5, 7};
+    A.col_indices = {0, 3, 4, 1, 3, 2, 4};
+    A.values = {2.0, 1.0, 1.0, 2.0, 1.0, 2.0, 1.0};
+    const std::vector<Float> theta(5, 1.0);
+
+    const std::vector<Index> P = ipm::compute_amd_ordering(A);
+    EXPECT((P == std::vector<Index>{1, 0, 2}));
+    ipm::SymbolicFactorization sym;
+    ipm::compute_symbolic_factorization(A, P, sym);
+
+    // the full permuted pattern of M, as the reference's callers pass it
+    core::CSRMatrix Mp;
+    Mp.rows = Mp.cols = 3;
+    Mp.row_ptrs = {0, 2, 5, 7};
+    Mp.col_indices = {0, 1, 0, 1, 2, 1, 2};
+    Mp.values.assign(7, 0.0);
+
+    gpu::VRAMArena arena(1 << 20);
+    gpu::GPUKKTCholeskySolver kkt(arena, A, sym, Mp);
+    kkt.gpu_cholesky_factorize(A, theta);
+
+    const size_t nnzL = sym.L_pattern.values.size() + 3;   // strictly lower + one diagonal per row
+    std::vector<Float> Lv(nnzL, 0.0);
+    EXPECT(hipMemcpy(Lv.data(), kkt.get_device_L_values(), nnzL * sizeof(Float),
+                     hipMemcpyDeviceToHost) == hipSuccess);
+    // rows of L in CSR, the diagonal last in each row
+    const int rp[4] = {0, 1, 3, 5}, ci[5] = {0, 0, 1, 1, 2};
+    double L[3][3] = {};
+    for (int i = 0; i < 3; ++i)
+        for (int p = rp[i]; p < rp[i + 1]; ++p) L[i][ci[p]] = Lv[p];
+    const double Mperm[3][3] = {{5, 1, 0}, {1, 6, 1}, {0, 1, 5}};
+    double fro = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0;
+            for (int k = 0; k < 3; ++k) s += L[i][k] * L[j][k];
+            fro += (s - Mperm[i][j]) * (s - Mperm[i][j]);
+        }
+    EXPECT(std::sqrt(fro) < 1e-6);
+
+    std::vector<Float> rhs = {11.0, 11.0, 16.0};
+    kkt.gpu_cholesky_solve(rhs);
+    EXPECT(std::fabs(rhs[0] - 1.0) < 1e-6 && std::fabs(rhs[1] - 2.0) < 1e-6 &&
+           std::fabs(rhs[2] - 3.0) < 1e-6);
+
+    // a Theta of the wrong length is std::invalid_argument
+    bool threw = false;
+    try {
+        kkt.gpu_cholesky_factorize(A, std::vector<Float>(4, 1.0));
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    EXPECT(threw);
+    std::printf("kkt: factor residual %.3e, x = (%.9f, %.9f, %.9f)\n", std::sqrt(fro), rhs[0],
+                rhs[1], rhs[2]);
+    return 0;
+}
+
+static int run_ipm() {
+    // min -3 x1 - 5 x2  s.t.  x1 + x2 + x3 = 4,  2 x1 + 3 x2 + x4 = 12,  x >= 0  (optimum -20)
+    core::Model model;
+    model.sense = OptimizationSense::Minimize;
+    for (double c : {-3.0, -5.0, 0.0, 0.0}) model.add_variable(c);
+    model.add_constraint({0, 1, 2}, {1.0, 1.0, 1.0}, 4.0);
+    model.add_constraint({0, 1, 3}, {2.0, 3.0, 1.0}, 12.0);
+    model.finalize();
+
+    ipm::MehrotraSolver solver(model);
+    const ipm::MehrotraResult r = solver.solve();
+    EXPECT(r.status == simplex::SimplexStatus::Optimal);
+    EXPECT(r.iterations > 0);
+    EXPECT(r.primal_residual < 1e-6 && r.dual_residual < 1e-6 && r.duality_gap < 1e-6);
+    EXPECT(std::fabs(r.objective_value + 20.0) <= 1e-4);
+    EXPECT(r.x.size() == 4 && r.pi.size() == 2);
+    for (Float v : r.x) EXPECT(std::isfinite(v));
+    for (Float v : r.pi) EXPECT(std::isfinite(v));
+    Float res = -1;
+    EXPECT(verifier::verify_optimal(model, r.x, r.pi, &res));
+    EXPECT(res >= 0 && res < 1e-6);
+    std::printf("ipm: status %d, %d iterations, objective %.9f, |Ax-b| %.2e\n",
+                static_cast<int>(r.status), r.iterations, r.objective_value, res);
+    return 0;
+}
+
+static int run_model() {
+    core::Model model;
+    model.add_variable(1.0);
+    model.add_variable(1.0);
+    model.add_variable(0.0);
+    core::BilinearTerm t{0, 1, 2, {0, 1, 2, 3}};
+    model.bilinear_terms.push_back(t);
+    EXPECT(model.bilinear_terms.size() == 1 && model.bilinear_terms[0].w_col == 2 &&
+           model.bilinear_terms[0].mccormick_rows[3] == 3);
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    const char* what = argc > 1 ? argv[1] : "all";
+    try {
+        if (!std::strcmp(what, "model") || !std::strcmp(what, "all")) run_model();
+        if (!std::strcmp(what, "kkt") || !std::strcmp(what, "all")) run_kkt();
+        if (!std::strcmp(what, "ipm") || !std::strcmp(what, "all")) run_ipm();
+    } catch (const std::exception& e) {
+        std::printf("FAIL: exception %s\n", e.what());
+        return 2;
+    }
+    std::printf("%s\n", g_fail ? "FAILED" : "OK");
+    return g_fail ? 1 : 0;
+}

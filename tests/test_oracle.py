@@ -101,6 +101,64 @@ def test_ruiz_invariant(oracle):
     assert np.all(np.abs(np.abs(As).max(1) - 1) < 1e-2) and np.all(np.abs(np.abs(As).max(0) - 1) < 1e-2)
 
 
+def _gauss_seidel_ruiz(A, passes=10):
+    """A planted mistake for the KAT below: rows scaled first, then the columns of the
+    row-scaled matrix (the ordering of src/presolve/scaling.cpp:39-115, not the IPM's)."""
+    A = A.copy()
+    R, Cs = np.ones(A.shape[0]), np.ones(A.shape[1])
+    for _ in range(passes):
+        fr = 1 / np.sqrt(np.abs(A).max(1))
+        A *= fr[:, None]
+        R *= fr
+        fc = 1 / np.sqrt(np.abs(A).max(0))
+        A *= fc[None, :]
+        Cs *= fc
+    return R, Cs
+
+
+def test_ruiz_jacobi_kat(oracle, native):
+    """Hand-computed Jacobi Ruiz (src/ipm/mehrotra_device.cu:293-323) on A = [[4,1],[1,1]]:
+    pass 1 takes row max (4, 1) and column max (4, 1) from the same snapshot, so
+    R = C = (1/2, 1) and the scaled A = [[1, 1/2], [1/2, 1]]; every later pass sees unit maxima
+    and changes nothing.  A Gauss-Seidel Ruiz gives C_0 = 1/sqrt(2) after its first pass."""
+    rp = np.array([0, 2, 4], np.int32)
+    ci = np.array([0, 1, 0, 1], np.int32)
+    va = np.array([4.0, 1.0, 1.0, 1.0])
+    b, c = np.array([3.0, 5.0]), np.array([7.0, 11.0])
+    for impl in (oracle.ruiz, native.ruiz_scaling):
+        v2, b2, c2, R, Cs = impl(rp, ci, va.copy(), b.copy(), c.copy())
+        assert R.tolist() == [0.5, 1.0] and Cs.tolist() == [0.5, 1.0]
+        assert v2.tolist() == [1.0, 0.5, 0.5, 1.0]
+        assert b2.tolist() == [1.5, 5.0] and c2.tolist() == [3.5, 11.0]
+    Rg, Cg = _gauss_seidel_ruiz(np.array([[4.0, 1.0], [1.0, 1.0]]), passes=1)
+    assert abs(Cg[0] - 2 ** -0.5) < 1e-15 and Cg[0] != 0.5
+
+
+def test_ruiz_gauss_seidel_is_rejected(oracle):
+    """The invariant check below also holds for Gauss-Seidel; this one separates the two on a
+    random matrix: the library-order (Jacobi) R and C differ from Gauss-Seidel's."""
+    from lakshya import gen
+    lp = gen.random_sparse(50, 120, 4, seed=9)
+    rp, ci, va = lp.csr()
+    _, _, _, R, Cs = oracle.ruiz(rp, ci, va.copy(), lp.b.copy(), lp.c.copy())
+    Rg, Cg = _gauss_seidel_ruiz(lp.to_scipy().toarray())
+    assert np.max(np.abs(R / Rg - 1)) > 1e-3 or np.max(np.abs(Cs / Cg - 1)) > 1e-3
+
+
+@pytest.mark.parametrize("seed", [9, 21])
+def test_ruiz_library_matches_oracle_bitwise(oracle, native, seed):
+    """The library's host equilibration (run in the MehrotraSolver constructor) gives the
+    oracle's R, C, scaled A, b and c bit for bit."""
+    from lakshya import gen
+    lp = gen.random_sparse(300, 700, 5, seed=seed)
+    rp, ci, va = lp.csr()
+    va = va * np.exp(np.random.default_rng(seed).uniform(-6, 6, len(va)))   # badly scaled
+    o = oracle.ruiz(rp, ci, va.copy(), lp.b.copy(), lp.c.copy())
+    g = native.ruiz_scaling(rp, ci, va.copy(), lp.b.copy(), lp.c.copy())
+    for a, b in zip(o, g):
+        assert np.array_equal(a, b)
+
+
 def test_textbook_lp(oracle):
     """tests/unit/test_mehrotra.cu:29-78: Optimal, obj -20 +- 1e-4, residuals and mu < 1e-6."""
     lp = lp_from_rows([[1, 1, 1, 0], [2, 3, 0, 1]], [4, 12], [-3, -5, 0, 0], 4)
