@@ -160,6 +160,44 @@ int lk_ipm_read_launch_timing(lk_ipm* h, double* ms, double* flops, double* laun
  * factor launches, solve launches, symbolic ms */
 int lk_ipm_kkt_stats(lk_ipm* h, double* out12);
 
+/* ---- batched interior-point solver: ipm::BatchedMehrotraSolver (include/ipm/batch.hpp) ----
+ * Extension with no reference counterpart (the reference solves one LP per MehrotraSolver; its
+ * batch pattern, one block per problem, is src/gpu/batched_dual_simplex.cu:7,55-58).  count LPs
+ * in CSC, concatenated: LP i's column pointers are n[i] + 1 entries of colptr (starting at 0)
+ * following those of LP i-1; its rows / values follow in rowidx / vals, b and c likewise. */
+typedef struct lk_batch lk_batch;
+typedef struct {
+    int32_t stop_rule;          /* 0 reference (scaled norms < tol), 1 relative gap (default) */
+    int32_t precision;          /* 0 FP64 factor, 1 FP32 factor + FP64 PCG */
+    int32_t max_iter, mixed_max_refine;
+    double tol, gap_tol, mixed_fp64_spread;
+    int32_t wg_per_cu;          /* persistent workgroups per CU (0: 2, or the autotuned value) */
+    int32_t iters_per_launch;   /* IPM iterations per LP visit (0: all, or the autotuned value) */
+    int32_t use_graph;          /* replay the launch sequence from a captured hipGraph */
+    int32_t order;              /* 0 largest m first, 1 input order */
+} lk_batch_options;
+typedef struct {
+    int32_t count, optimal, launches, graph_replays;
+    double ms, flops_chol;      /* device ms of the solve; sum over LPs of factorizations m^3/3 */
+    int32_t wg_per_cu, iters_per_launch, grid, pad;
+} lk_batch_summary;
+typedef struct {
+    int32_t status, iterations, fail_code, factorizations;
+    int32_t fp32_factorizations, solves, pcg_steps, pad;
+    double objective, dual_objective, rel_gap, rel_primal, rel_dual;
+    double primal_residual, dual_residual, mu;
+} lk_batch_result;
+int lk_batch_default_options(lk_batch_options* opt);
+int lk_batch_create(int32_t count, const int32_t* m, const int32_t* n, const int32_t* colptr,
+                    const int32_t* rowidx, const double* vals, const double* b, const double* c,
+                    const lk_batch_options* opt, lk_batch** out);
+int lk_batch_solve(lk_batch* h, lk_batch_summary* sum);
+/* time the launch shapes on `sample` LPs and keep the fastest for later solves */
+int lk_batch_autotune(lk_batch* h, int32_t sample, lk_batch_summary* chosen);
+int lk_batch_results(lk_batch* h, lk_batch_result* out);          /* count entries */
+int lk_batch_solution(lk_batch* h, int32_t i, double* x, double* y, double* s);
+int lk_batch_destroy(lk_batch* h);
+
 /* ---- verifier::verify_optimal (src/verifier/certificate.hpp:26-31) with a tolerance ---- */
 int lk_verify_optimal(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
                       const double* vals, const double* c, const double* b, const double* lb,

@@ -338,6 +338,75 @@ int lkd_ipm_commit(const lkd_ipm* P, double* x, double* s, double* y, const doub
 /* x = 1, s = 1, y = 0 (init_vars_kernel, mehrotra_device.cu:93-102) */
 int lkd_ipm_init(int32_t m, int32_t n, double* x, double* y, double* s, void* stream);
 
+/* ---------------------------------------------------------------- batched dense IPM */
+/* Thousands of small LPs (min c^T x, A x = b, x >= 0) in one persistent launch: every
+ * workgroup takes the next LP from a ticket counter (order[] gives the visit order) and runs
+ * up to iters_per_visit IPM iterations on it -- Ruiz scaling, dense A Theta A^T assembly,
+ * MFMA Cholesky, triangular solves, PCG and the Mehrotra step -- with all vectors of the LP
+ * in its own work area.  LP state survives between launches in lkb_state.
+ *
+ * Layout of one LP (lkb_lp):
+ *   idx[a ..]  CSR row pointers (m + 1), CSR columns (nnz), CSC position of each CSR entry
+ *              (nnz), CSC column pointers (n + 1), CSC rows (nnz)             [int32]
+ *   val0[v ..] original CSC values (nnz)
+ *   valw[2 v ..] scaled CSR values (nnz), then scaled CSC values (nnz)
+ *   rhs[bc ..] original b (m) then c (n)
+ *   work[w ..] LKB_NVN n-vectors, LKB_NVM m-vectors, one staged 64x64 factor (LKB_STAGE)
+ *   fac[f ..]  the dense m x ld factor (column-major, lower; doubles, or floats in the same
+ *              bytes for an FP32 factor)
+ *   x_out/s_out[on ..], y_out[om ..]  original-space solution */
+#define LKB_MAX_M 1600
+#define LKB_NVN 14
+#define LKB_NVM 14
+#define LKB_STAGE (64 * 65 + 1024)
+typedef struct {
+    int32_t m, n, nnz, ld;
+    int64_t a, v, bc, w, f, on, om;
+} lkb_lp;
+
+typedef struct {
+    int32_t phase;                 /* 0 not started, 1 iterating, 2 finished */
+    int32_t status;                /* 0 Optimal, 3 IterationLimit */
+    int32_t iterations, fail_code; /* fail_code as lk_ipm_result */
+    int32_t factorizations, fp32_factorizations, solves, pcg_steps;
+    double rp0, rd0, mu0, binf, cinf;
+    double objective, dual_objective, rel_gap, rel_primal, rel_dual;
+    double primal_residual, dual_residual, mu;
+} lkb_state;
+
+typedef struct {
+    int32_t count, stop_rule, precision, max_iter;
+    int32_t iters_per_visit, mixed_max_refine;     /* PCG budget with an FP32 factor */
+    double tol, gap_tol, mixed_spread;
+    const lkb_lp* lps;
+    const int32_t* order;
+    const int32_t* idx;
+    const double* val0;
+    double* valw;
+    const double* rhs;
+    double* work;
+    void* fac;
+    lkb_state* state;
+    double *x_out, *y_out, *s_out;
+    uint32_t* ctl;                 /* [0] ticket counter, [1] LPs finished */
+} lkb_args;
+
+/* ctl[0] = 0; with `restart` also ctl[1] = 0 and the phase of LPs order[0 .. count) back to 0
+ * (a new solve).  A kernel, not a memset, so a captured graph may hold it. */
+int lkd_batch_reset(uint32_t* ctl, lkb_state* state, const int32_t* order, int32_t count,
+                    int32_t restart, void* stream);
+/* the persistent launch: `grid` workgroups of 256 threads; args (host struct) are passed to
+ * the kernel by value */
+int lkd_batch_solve(const lkb_args* args, int32_t grid, void* stream);
+
+/* ---------------------------------------------------------------- launch graphs */
+/* Stream capture of a launch sequence into an executable graph (kernels only: no memset or
+ * copy nodes), its replay and destruction. */
+int lkd_graph_begin(void* stream);
+int lkd_graph_end(void* stream, void** exec);
+int lkd_graph_launch(void* exec, void* stream);
+int lkd_graph_destroy(void* exec);
+
 /* ---------------------------------------------------------------- probes (tests) */
 /* One 16x16x4 MFMA (FP64 or FP32) on 64 lanes: lane l supplies a[l], b[l]; out[l*4+e] = D[e]. */
 int lkd_probe_mfma(int prec, const void* a, const void* b, void* out, void* stream);

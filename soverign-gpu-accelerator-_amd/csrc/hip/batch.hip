@@ -1,0 +1,1033 @@
+// Batched dense interior-point kernels for gfx950: thousands of small LPs in one persistent
+// launch, one 256-thread workgroup per LP at a time.
+//
+// Per LP the workgroup runs the Mehrotra predictor-corrector loop of the single-LP solver (the
+// reference loop src/ipm/mehrotra_device.cu:695-1047 with the same constants, in the
+// relative-gap or the reference stop mode), entirely on the device:
+//   * 10 Jacobi Ruiz passes (mehrotra_device.cu:286-323) over both the CSR and the CSC copy,
+//     with the same per-entry products, so the two copies stay bitwise equal;
+//   * fused sparse passes (G lanes per row of A or of A^T) with workgroup reductions in a
+//     fixed order: residuals, right-hand sides, directions, Newton-direction checks;
+//   * M = D (A Theta A^T) D + delta I assembled densely, one column per wavefront: the column
+//     is accumulated in LDS from the CSC segments at or below the diagonal of every column of
+//     A that row i touches (flattened over the lanes, so loads stay coalesced);
+//   * a left-looking blocked Cholesky on MFMA (64-column block columns, 128-row tiles, the
+//     GEMM / potrf64 primitives of dense_tile.hpp), FP64 or FP32;
+//   * blocked forward / backward substitution through LDS;
+//   * PCG on M0 dy = rhs (preconditioned by the factor) or the reference's refinement.
+// The launch is persistent: a workgroup takes tickets from ctl[0] until all LPs are visited;
+// every visit runs up to iters_per_visit iterations and leaves the LP's state in lkb_state, so
+// the same launch can be repeated (for instance from a captured graph) until all LPs finish.
+#include "dense_tile.hpp"
+
+namespace {
+
+constexpr int BT = 256;
+constexpr int GR = 16;                 // lanes per row of A in the row passes
+constexpr int GC = 8;                  // lanes per column of A in the column passes
+constexpr int LDS_D = 6656;            // doubles in the workgroup's shared union
+static_assert(sizeof(GemmLds<double>) == LDS_D * sizeof(double), "GEMM staging fits the union");
+static_assert(sizeof(GemmLds<float>) <= LDS_D * sizeof(double), "GEMM staging fits the union");
+static_assert(64 * DP + 1024 <= LDS_D, "diagonal block and inverses fit the union");
+static_assert(4 * LKB_MAX_M <= LDS_D, "one assembly column per wavefront");
+static_assert(LKB_MAX_M + 64 * DP + 64 + 4 * 64 <= LDS_D, "substitution buffers");
+static_assert(LKB_STAGE == 64 * DP + 1024, "staged diagonal block");
+
+// The workgroup's shared memory, at file scope so that the large phases can be separate
+// (non-inlined) functions with their own register allocation and still address it as LDS:
+// one union (GEMM staging / diagonal block / assembly columns / substitution buffers), the
+// reduction scratch and a flag word.
+__shared__ double g_lds[LDS_D];
+__shared__ double g_red[4 * 10];
+__shared__ int g_flag;
+
+// n-vectors and m-vectors of one LP in its work area
+enum { VX = 0, VS, VC, VCS, VRD, VTH, VRXS, VDXA, VDSA, VDX, VDS, VQD, VTN, VV };
+enum { WY = 0, WB, WR, WRP, WDY, WRHS, WD, WQP, WPR, WPZ, WPP, WPQ, WBEST, WTM };
+static_assert(VV + 1 == LKB_NVN && WTM + 1 == LKB_NVM, "work vector counts");
+
+enum { RSUM = 0, RMIN = 1, RMAX = 2 };
+__device__ __forceinline__ double rop(int op, double a, double b) {
+    return op == RSUM ? a + b : (op == RMIN ? fmin(a, b) : fmax(a, b));
+}
+
+struct Lp {
+    int m, n, nnz, ld;
+    const int32_t *rp, *ci, *pos, *cp, *ri;
+    const double* v0;
+    double *av, *cv;
+    const double *b0, *c0;
+    double *vn, *vm, *stage;
+    double* fac;
+    __device__ __forceinline__ double* N(int k) const { return vn + (int64_t)k * n; }
+    __device__ __forceinline__ double* M(int k) const { return vm + (int64_t)k * m; }
+};
+
+__device__ __forceinline__ Lp make_lp(const lkb_args& A, const lkb_lp& d) {
+    Lp L;
+    L.m = d.m;
+    L.n = d.n;
+    L.nnz = d.nnz;
+    L.ld = d.ld;
+    const int32_t* ip = A.idx + d.a;
+    L.rp = ip;
+    L.ci = ip + d.m + 1;
+    L.pos = L.ci + d.nnz;
+    L.cp = L.pos + d.nnz;
+    L.ri = L.cp + d.n + 1;
+    L.v0 = A.val0 + d.v;
+    L.av = A.valw + 2 * d.v;
+    L.cv = L.av + d.nnz;
+    L.b0 = A.rhs + d.bc;
+    L.c0 = L.b0 + d.m;
+    L.vn = A.work + d.w;
+    L.vm = L.vn + (int64_t)LKB_NVN * d.n;
+    L.stage = L.vm + (int64_t)LKB_NVM * d.m;
+    L.fac = static_cast<double*>(A.fac) + d.f;
+    return L;
+}
+
+// K values reduced over the workgroup in a fixed order; every thread receives the results.
+template <int K>
+__device__ __forceinline__ void wg_reduce(double (&v)[K], const int (&op)[K], double* red) {
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[q] = rop(op[q], v[q], __shfl_xor(v[q], o, 64));
+    if (l == 0) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) red[w * K + q] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+        v[q] = rop(op[q], rop(op[q], red[q], red[K + q]), rop(op[q], red[2 * K + q], red[3 * K + q]));
+    __syncthreads();
+}
+
+template <int G>
+__device__ __forceinline__ double group_op(int op, double v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v = rop(op, v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Segments [ptr[r], ptr[r+1]) of a compressed matrix, G lanes per segment: the lanes combine
+// term(p) with `op` and the first lane of the group calls f(r, value).  Every thread runs the
+// same number of rounds (the group reductions are wave-wide shuffles).
+template <int G, typename Term, typename F>
+__device__ __forceinline__ void seg_pass(int nseg, const int32_t* __restrict__ ptr, int op,
+                                         Term&& term, F&& f) {
+    const int g = threadIdx.x & (G - 1);
+    const double ident = op == RSUM ? 0.0 : (op == RMIN ? 1e300 : 0.0);
+    for (int r0 = 0; r0 < nseg; r0 += BT / G) {
+        const int r = r0 + (int)threadIdx.x / G;
+        double s = ident;
+        if (r < nseg) {
+            const int e = ptr[r + 1];
+            for (int p = ptr[r] + g; p < e; p += G) s = rop(op, s, term(p));
+        }
+        s = group_op<G>(op, s);
+        if (r < nseg && g == 0) f(r, s);
+    }
+}
+
+// f(r, p) for every entry p of every segment r, G lanes per segment
+template <int G, typename F>
+__device__ __forceinline__ void seg_for(int nseg, const int32_t* __restrict__ ptr, F&& f) {
+    const int g = threadIdx.x & (G - 1);
+    for (int r = (int)threadIdx.x / G; r < nseg; r += BT / G) {
+        const int e = ptr[r + 1];
+        for (int p = ptr[r] + g; p < e; p += G) f(r, p);
+    }
+}
+
+__device__ __forceinline__ double step_len(double v, double dv) {
+    return dv >= 0.0 ? 1e30 : -v / dv;
+}
+
+// Wave-uniform copies of values that arrive in vector registers at a call boundary (function
+// arguments are passed in VGPRs): in scalar registers the address arithmetic derived from them
+// stays scalar instead of occupying (and spilling) vector registers.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <typename P>
+__device__ __forceinline__ P* uni(P* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<P*>(((uint64_t)hi << 32) | lo);
+}
+
+// potrf64 on the diagonal block at the start of the shared union (Ds, then Lv), as a call of
+// its own: it takes the whole register file, which the tile code around it then keeps
+template <typename T>
+__device__ __attribute__((noinline)) bool potrf_lds() {
+    T* Ds = reinterpret_cast<T*>(g_lds);
+    return potrf64<T>(Ds, Ds + 64 * DP, &g_flag);
+}
+
+// ------------------------------------------------------------------ dense factor
+// Left-looking blocked Cholesky of the m x m lower matrix at F (column-major, ld), in place,
+// by 64-column block columns k = columns k0 .. k0+kb:
+//   diagonal block   C(k, k) - L(k, 0:k0) L(k, 0:k0)^T on MFMA (waves 0 and 1 hold its rows),
+//                    factored in LDS (potrf64), stored, and staged in the global slot `stage`
+//                    (the Ds + inverses layout of the shared union)
+//   128-row tiles R  below it: X = C(R, k) - L(R, 0:k0) L(k, 0:k0)^T (gemm_tn: accumulators in
+//                    the solve layout), then X <- X Lkk^{-T} on MFMA, 16 columns at a time
+// Returns false (uniformly) when a pivot is not positive.
+template <typename T>
+__device__ __attribute__((noinline)) bool dense_chol(T* __restrict__ F_, int m_, int ld_,
+                                                     T* __restrict__ stage_) {
+    using R = Mfma<T>;
+    T* __restrict__ F = uni(F_);
+    T* __restrict__ stage = uni(stage_);
+    const int m = uni(m_), ld = uni(ld_);
+    GemmLds<T>& sm = *reinterpret_cast<GemmLds<T>*>(g_lds);
+    T* Ds = reinterpret_cast<T*>(g_lds);
+    T* Lv = Ds + 64 * DP;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15;
+    const int rl0 = R::row(l, 0);
+    for (int k0 = 0; k0 < m; k0 += TS) {
+        const int kb = min(TS, m - k0), kbm = kb - 1;
+        {   // ---- the diagonal block
+            Acc<T> acc[4][2];
+            zero_acc<T, 4, 2>(acc);
+            gemm_tn<T>(F + k0, ld, kb, F + k0, ld, kb, 0, k0, sm, acc);
+            __syncthreads();
+            if (w < 2) {
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const int rl = 32 * w + 16 * b + li;
+                    const int rc = min(rl, kbm);
+#pragma unroll
+                    for (int p = 0; p < 4; ++p)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int c = 16 * p + R::row(l, e);
+                            const T cv = F[(int64_t)(k0 + min(c, kbm)) * ld + k0 + rc] - acc[p][b][e];
+                            T v = rl == c ? T(1) : T(0);
+                            if (rl < kb && c < kb) v = rl >= c ? cv : T(0);
+                            Ds[c * DP + rl] = v;
+                        }
+                }
+            }
+            __syncthreads();
+            if (!potrf_lds<T>()) return false;
+            for (int idx = t; idx < LKB_STAGE; idx += BT) stage[idx] = Ds[idx];
+            for (int idx = t; idx < TS * TS; idx += BT) {
+                const int r = idx & 63, c = idx >> 6;
+                if (r < kb && c <= r) F[(int64_t)(k0 + c) * ld + k0 + r] = Ds[c * DP + r];
+            }
+            __syncthreads();
+        }
+        for (int r0 = k0 + kb; r0 < m; r0 += MT) {   // ---- the tiles below it
+            const int ra = min(MT, m - r0);
+            Acc<T> acc[4][2];
+            zero_acc<T, 4, 2>(acc);
+            gemm_tn<T>(F + r0, ld, ra, F + k0, ld, kb, 0, k0, sm, acc);
+            Acc<T> y[2][4];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int rl = 32 * w + 16 * b + li;
+                const int64_t rc = rl < ra ? (int64_t)r0 + rl : (int64_t)r0;
+                if (kb == TS) {
+                    // full block column: one base per lane, wave-uniform element offsets
+                    const T* pb = F + (int64_t)(k0 + rl0) * ld + rc;
+#pragma unroll
+                    for (int p = 0; p < 4; ++p)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            y[b][p][e] = pb[(int64_t)(16 * p + R::row(l, e) - rl0) * ld] - acc[p][b][e];
+                } else {
+#pragma unroll
+                    for (int p = 0; p < 4; ++p)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int c = min(16 * p + R::row(l, e), kbm);   // padding columns
+                            y[b][p][e] = F[(int64_t)(k0 + c) * ld + rc] - acc[p][b][e];
+                        }
+                }
+            }
+            // the staged factor of block k into LDS (the GEMM used the space)
+            for (int idx = t; idx < LKB_STAGE; idx += BT) Ds[idx] = stage[idx];
+            __syncthreads();
+            // Y_p = Linv_pp (X_p^T - sum_{q<p} L_pq Y_q)
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                Acc<T> tp[2] = {y[0][p], y[1][p]};
+#pragma unroll
+                for (int q = 0; q < p; ++q)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const T a = -Ds[(16 * q + R::row(l, s)) * DP + 16 * p + li];
+#pragma unroll
+                        for (int b = 0; b < 2; ++b) tp[b] = R::mma(a, y[b][q][s], tp[b]);
+                    }
+#pragma unroll
+                for (int b = 0; b < 2; ++b) y[b][p] = Acc<T>{0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const T a = Lv[256 * p + R::row(l, s) * 16 + li];
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) y[b][p] = R::mma(a, tp[b][s], y[b][p]);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int rl = 32 * w + 16 * b + li;
+                if (rl >= ra) continue;
+                T* out = F + (int64_t)(k0 + rl0) * ld + r0 + rl;
+#pragma unroll
+                for (int p = 0; p < 4; ++p)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int c = 16 * p + R::row(l, e);
+                        if (c < kb) out[(int64_t)(c - rl0) * ld] = y[b][p][e];
+                    }
+            }
+            __syncthreads();   // LDS reads done before the next tile's GEMM; stores issued
+        }
+    }
+    return true;
+}
+
+// x <- (L L^T)^{-1} x for the factor at F (lower, column-major, ld), x an m-vector in global
+// memory; blocked by 64 through LDS (z, one diagonal block, reciprocal pivots, partial sums).
+template <typename T>
+__device__ __attribute__((noinline)) void dense_solve(const T* __restrict__ F_, int m_, int ld_,
+                                                      double* __restrict__ x_) {
+    const T* __restrict__ F = uni(F_);
+    double* __restrict__ x = uni(x_);
+    const int m = uni(m_), ld = uni(ld_);
+    double* z = g_lds;
+    double* Db = g_lds + LKB_MAX_M;
+    double* rinv = Db + 64 * DP;
+    double* part = rinv + 64;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    for (int i = t; i < m; i += BT) z[i] = x[i];
+    __syncthreads();
+    auto stage_block = [&](int jb, int kb) {
+        for (int idx = t; idx < 64 * 64; idx += BT) {
+            const int r = idx & 63, c = idx >> 6;
+            if (r < kb && c <= r) Db[c * DP + r] = (double)F[(int64_t)(jb + c) * ld + jb + r];
+        }
+        __syncthreads();
+        if (t < kb) rinv[t] = 1.0 / Db[t * DP + t];
+    };
+    // forward: L z = x
+    for (int jb = 0; jb < m; jb += 64) {
+        const int kb = min(64, m - jb);
+        {   // part[g][r] = sum over columns c = g, g + 4, .. < jb of L(jb + r, c) z[c]
+            const int r = l, g = w;
+            double s = 0.0;
+            if (r < kb)
+                for (int c = g; c < jb; c += 4) s += (double)F[(int64_t)c * ld + jb + r] * z[c];
+            part[g * 64 + r] = s;
+        }
+        stage_block(jb, kb);
+        __syncthreads();
+        if (w == 0) {
+            double v = l < kb ? z[jb + l] - ((part[l] + part[64 + l]) + (part[128 + l] + part[192 + l])) : 0.0;
+            for (int j = 0; j < kb; ++j) {
+                if (l == j) v *= rinv[j];
+                const double zj = __shfl(v, j, 64);
+                if (l > j && l < kb) v -= Db[j * DP + l] * zj;
+            }
+            if (l < kb) z[jb + l] = v;
+        }
+        __syncthreads();
+    }
+    // backward: L^T x = z
+    const int nblk = (m + 63) / 64;
+    for (int bi = nblk - 1; bi >= 0; --bi) {
+        const int jb = 64 * bi, kb = min(64, m - jb), r1 = jb + kb;
+        // part[c] = sum over rows r >= r1 of L(r, jb + c) x[r], column c on wave c % 4
+        for (int c = w; c < kb; c += 4) {
+            double s = 0.0;
+            for (int r = r1 + l; r < m; r += 64) s += (double)F[(int64_t)(jb + c) * ld + r] * z[r];
+            s = wave_sum(s);
+            if (l == 0) part[c] = s;
+        }
+        stage_block(jb, kb);
+        __syncthreads();
+        if (w == 0) {
+            double v = l < kb ? z[jb + l] - part[l] : 0.0;
+            for (int j = kb - 1; j >= 0; --j) {
+                if (l == j) v *= rinv[j];
+                const double xj = __shfl(v, j, 64);
+                if (l < j) v -= Db[l * DP + j] * xj;
+            }
+            if (l < kb) z[jb + l] = v;
+        }
+        __syncthreads();
+    }
+    for (int i = t; i < m; i += BT) x[i] = z[i];
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ the per-LP solver
+struct Ctx {
+    const lkb_args& A;
+    Lp L;
+    lkb_state st;
+    double* red;
+    bool rel;
+    bool fp32;        // the current factor is FP32
+    double m0max;     // max_i M0_ii of the current Theta
+};
+
+// D_i = 1 / sqrt(max(|M0_ii|, 1e-12)), M0_ii = sum_k a_ik^2 theta_k (src/cuda/kkt.cu:75 floor)
+__device__ void diag_scaling(Ctx& c) {
+    const Lp& L = c.L;
+    const double* th = L.N(VTH);
+    double* D = L.M(WD);
+    double mx = 0.0;
+    seg_pass<GR>(L.m, L.rp, RSUM,
+                 [&](int p) { const double a = L.av[p]; return a * a * th[L.ci[p]]; },
+                 [&](int i, double d) {
+                     mx = fmax(mx, d);
+                     D[i] = 1.0 / sqrt(fmax(fabs(d), 1e-12));
+                 });
+    double v[1] = {mx};
+    const int op[1] = {RMAX};
+    wg_reduce<1>(v, op, c.red);
+    c.m0max = v[0];
+}
+
+// The lower part of D M0 D + delta I (delta on rows that are not empty) into the factor
+// storage, one column per wavefront accumulated in LDS.
+template <typename T>
+__device__ __attribute__((noinline)) void assemble(Ctx& c, double delta) {
+    const Lp& L = c.L;
+    const double* th = L.N(VTH);
+    const double* D = L.M(WD);
+    T* F = reinterpret_cast<T*>(L.fac);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    double* buf = g_lds + w * LKB_MAX_M;
+    for (int i = w; i < L.m; i += 4) {
+        for (int r = i + l; r < L.m; r += 64) buf[r] = 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const int e0 = L.rp[i], e1 = L.rp[i + 1];
+        for (int eb = e0; eb < e1; eb += 64) {
+            const int e = eb + l;
+            int p0 = 0, len = 0;
+            double coef = 0.0;
+            if (e < e1) {
+                const int k = L.ci[e];
+                p0 = L.pos[e];
+                len = L.cp[k + 1] - p0;
+                coef = L.av[e] * th[k];
+            }
+            int incl = len;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(incl, o, 64);
+                if (l >= o) incl += u;
+            }
+            const int excl = incl - len;
+            const int total = __shfl(incl, 63, 64);
+            for (int f0 = 0; f0 < total; f0 += 64) {
+                const int f = f0 + l;
+                const int fq = min(f, total - 1);
+                int j = 0;
+#pragma unroll
+                for (int b = 32; b > 0; b >>= 1)
+                    if (__shfl(incl, j + b - 1, 64) <= fq) j += b;
+                const int q = __shfl(p0, j, 64) + (fq - __shfl(excl, j, 64));
+                const double cj = __shfl(coef, j, 64);
+                if (f < total) atomicAdd(&buf[L.ri[q]], L.cv[q] * cj);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const bool has_diag = e1 > e0;
+        const double di = D[i];
+        for (int r = i + l; r < L.m; r += 64) {
+            double v = buf[r] * D[r] * di;
+            if (r == i && has_diag) v += delta;
+            F[(int64_t)i * L.ld + r] = (T)v;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+}
+
+// Factor D M0 D + delta I for the current Theta in precision T: delta = delta0 * 10^r for the
+// first r that succeeds (src/cuda/kkt.cu:652-690).  false when every delta fails.
+template <typename T>
+__device__ bool factor(Ctx& c, double delta0, int retries) {
+    double delta = delta0;
+    for (int r = 0; r < retries; ++r) {
+        if (r > 0) delta *= 10.0;
+        assemble<T>(c, delta);
+        if (dense_chol<T>(reinterpret_cast<T*>(c.L.fac), c.L.m, c.L.ld,
+                          reinterpret_cast<T*>(c.L.stage)))
+            return true;
+        __syncthreads();
+    }
+    return false;
+}
+
+__device__ bool factor_any(Ctx& c, bool fp32) {
+    c.st.factorizations++;
+    if (fp32) {
+        c.st.fp32_factorizations++;
+        c.fp32 = true;
+        if (factor<float>(c, c.rel ? 1e-6 : 1e-9, c.rel ? 10 : 5)) return true;
+    }
+    c.fp32 = false;
+    return factor<double>(c, c.rel ? 1e-14 : 1e-9, c.rel ? 10 : 5);
+}
+
+// x <- D (L L^T)^{-1} D x  (the preconditioner / direct solve, permutation-free)
+__device__ void precond(Ctx& c, double* x) {
+    const Lp& L = c.L;
+    const double* D = L.M(WD);
+    for (int i = threadIdx.x; i < L.m; i += BT) x[i] *= D[i];
+    __syncthreads();
+    if (c.fp32) dense_solve<float>(reinterpret_cast<const float*>(L.fac), L.m, L.ld, x);
+    else dense_solve<double>(L.fac, L.m, L.ld, x);
+    for (int i = threadIdx.x; i < L.m; i += BT) x[i] *= D[i];
+    __syncthreads();
+}
+
+// r = rhs - M0 v with M0 v = A (Theta (A^T v)); returns ||r||_inf
+__device__ double m0_residual(Ctx& c, const double* v, const double* rhs, double* r) {
+    const Lp& L = c.L;
+    const double* th = L.N(VTH);
+    double* tn = L.N(VTN);
+    seg_pass<GC>(L.n, L.cp, RSUM, [&](int q) { return L.cv[q] * v[L.ri[q]]; },
+                 [&](int j, double s) { tn[j] = th[j] * s; });
+    __syncthreads();
+    double mx = 0.0;
+    seg_pass<GR>(L.m, L.rp, RSUM, [&](int p) { return L.av[p] * tn[L.ci[p]]; },
+                 [&](int i, double s) {
+                     const double ri = rhs[i] - s;
+                     r[i] = ri;
+                     mx = fmax(mx, fabs(ri));
+                 });
+    double vv[1] = {mx};
+    const int op[1] = {RMAX};
+    wg_reduce<1>(vv, op, c.red);
+    return vv[0];
+}
+
+// q = M0 p; returns p^T q
+__device__ double m0_apply(Ctx& c, const double* p, double* q) {
+    const Lp& L = c.L;
+    const double* th = L.N(VTH);
+    double* tn = L.N(VTN);
+    seg_pass<GC>(L.n, L.cp, RSUM, [&](int k) { return L.cv[k] * p[L.ri[k]]; },
+                 [&](int j, double s) { tn[j] = th[j] * s; });
+    __syncthreads();
+    double pq = 0.0;
+    seg_pass<GR>(L.m, L.rp, RSUM, [&](int k) { return L.av[k] * tn[L.ci[k]]; },
+                 [&](int i, double s) {
+                     q[i] = s;
+                     pq += p[i] * s;
+                 });
+    double vv[1] = {pq};
+    const int op[1] = {RSUM};
+    wg_reduce<1>(vv, op, c.red);
+    return vv[0];
+}
+
+__device__ void copy_m(const Ctx& c, double* dst, const double* src) {
+    for (int i = threadIdx.x; i < c.L.m; i += BT) dst[i] = src[i];
+    __syncthreads();
+}
+
+// KKT solve M0 dy = rhs (src/cuda/kkt.cu:620-870 contract: success iff ||rhs - M0 dy||_inf <=
+// tol; relative-gap mode and FP32 factors: PCG from the direct solve with backward-error
+// acceptance in relative-gap mode; reference mode: refinement, <= max_refine steps with
+// rollback on stagnation).  dy overwritten even on failure.
+__device__ bool kkt_solve(Ctx& c, double tol, int max_it) {
+    const Lp& L = c.L;
+    const double* rhs = L.M(WRHS);
+    double* dy = L.M(WDY);
+    double* r = L.M(WPR);
+    double* z = L.M(WPZ);
+    double* p = L.M(WPP);
+    double* q = L.M(WPQ);
+    double* best = L.M(WBEST);
+    copy_m(c, dy, rhs);
+    precond(c, dy);
+    double before = m0_residual(c, dy, rhs, r);
+    c.st.solves++;
+    if (before <= tol) return true;
+    if (!(c.rel || c.fp32)) {
+        for (int it = 0; it < max_it; ++it) {
+            copy_m(c, best, dy);
+            copy_m(c, z, r);
+            precond(c, z);
+            for (int i = threadIdx.x; i < L.m; i += BT) dy[i] += z[i];
+            __syncthreads();
+            c.st.pcg_steps++;
+            const double after = m0_residual(c, dy, rhs, r);
+            if (after <= tol) return true;
+            if (after >= before) {
+                copy_m(c, dy, best);
+                m0_residual(c, dy, rhs, r);
+                return false;
+            }
+            before = after;
+        }
+        return false;
+    }
+    // PCG, the best iterate (smallest true ||r||_inf) kept
+    double rbest = before;
+    copy_m(c, best, dy);
+    bool ok = false;
+    int stall = 0;
+    copy_m(c, z, r);
+    precond(c, z);
+    double rz;
+    {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < L.m; i += BT) {
+            p[i] = z[i];
+            s += r[i] * z[i];
+        }
+        double v[1] = {s};
+        const int op[1] = {RSUM};
+        wg_reduce<1>(v, op, c.red);
+        rz = v[0];
+    }
+    for (int it = 0; it < max_it && !ok; ++it) {
+        const double pq = m0_apply(c, p, q);
+        if (!(pq > 0.0)) break;
+        const double alpha = rz / pq;
+        double mx = 0.0;
+        for (int i = threadIdx.x; i < L.m; i += BT) {
+            dy[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+            mx = fmax(mx, fabs(r[i]));
+        }
+        double v1[1] = {mx};
+        const int opm[1] = {RMAX};
+        wg_reduce<1>(v1, opm, c.red);
+        double rr = v1[0];
+        c.st.pcg_steps++;
+        if (rr <= tol || it == max_it - 1 || (it + 1) % 5 == 0) {
+            rr = m0_residual(c, dy, rhs, r);
+            stall = rr < 0.5 * rbest ? 0 : stall + 1;
+            if (rr < rbest) {
+                rbest = rr;
+                copy_m(c, best, dy);
+            }
+            if (rr <= tol) { ok = true; break; }
+            if (stall >= 2) break;
+        }
+        copy_m(c, z, r);
+        precond(c, z);
+        double s = 0.0;
+        for (int i = threadIdx.x; i < L.m; i += BT) s += r[i] * z[i];
+        double v2[1] = {s};
+        const int ops[1] = {RSUM};
+        wg_reduce<1>(v2, ops, c.red);
+        const double beta = v2[0] / rz;
+        rz = v2[0];
+        for (int i = threadIdx.x; i < L.m; i += BT) p[i] = z[i] + beta * p[i];
+        __syncthreads();
+    }
+    if (!ok) {
+        copy_m(c, dy, best);
+        if (c.rel) {
+            // accepted at the level a backward-stable solve reaches:
+            // ||r||_inf <= 1e-12 (||rhs||_inf + max_i M0_ii ||dy||_inf)
+            double a = 0.0, b = 0.0;
+            for (int i = threadIdx.x; i < L.m; i += BT) {
+                a = fmax(a, fabs(rhs[i]));
+                b = fmax(b, fabs(dy[i]));
+            }
+            double v[2] = {a, b};
+            const int op[2] = {RMAX, RMAX};
+            wg_reduce<2>(v, op, c.red);
+            ok = rbest <= 1e-12 * (v[0] + c.m0max * v[1]);
+        }
+    }
+    return ok;
+}
+
+struct Res {
+    double nrp, nrd, mu, cx, by, rpinf, rdinf, xsmax, thmax, thmin;
+};
+
+// rp = A x - b, rd = A^T y + s - c, Theta = x / s and their norms (compute_residuals,
+// mehrotra_device.cu:589-624)
+__device__ Res residuals(Ctx& c) {
+    const Lp& L = c.L;
+    const double *x = L.N(VX), *s = L.N(VS), *cc = L.N(VC), *Cs = L.N(VCS);
+    const double *y = L.M(WY), *b = L.M(WB), *R = L.M(WR);
+    double *rp = L.M(WRP), *rd = L.N(VRD), *th = L.N(VTH);
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, -1e300, 1e300};
+    seg_pass<GR>(L.m, L.rp, RSUM, [&](int p) { return L.av[p] * x[L.ci[p]]; },
+                 [&](int i, double ax) {
+                     const double r = ax + (-b[i]);
+                     rp[i] = r;
+                     v[0] += r * r;
+                     v[1] = fmax(v[1], fabs(r / R[i]));
+                     v[2] += b[i] * y[i];
+                 });
+    seg_pass<GC>(L.n, L.cp, RSUM, [&](int q) { return L.cv[q] * y[L.ri[q]]; },
+                 [&](int j, double aty) {
+                     const double r = aty + (s[j] - cc[j]);
+                     const double tj = x[j] / s[j];
+                     rd[j] = r;
+                     th[j] = tj;
+                     v[3] += r * r;
+                     v[4] += x[j] * s[j];
+                     v[5] += cc[j] * x[j];
+                     v[6] = fmax(v[6], fabs(r / Cs[j]));
+                     v[7] = fmax(v[7], fabs(x[j] * (s[j] - r)));
+                     v[8] = fmax(v[8], tj);
+                     v[9] = fmin(v[9], tj);
+                 });
+    const int op[10] = {RSUM, RMAX, RSUM, RSUM, RSUM, RSUM, RMAX, RMAX, RMAX, RMIN};
+    wg_reduce<10>(v, op, c.red);
+    Res r;
+    r.nrp = sqrt(v[0]);
+    r.rpinf = v[1];
+    r.by = v[2];
+    r.nrd = sqrt(v[3]);
+    r.mu = v[4] / L.n;
+    r.cx = v[5];
+    r.rdinf = v[6];
+    r.xsmax = v[7];
+    r.thmax = v[8];
+    r.thmin = v[9];
+    return r;
+}
+
+struct Dir { double e1, e2, e3, apmax, admax; };
+
+// v = -Theta rd - rxs / s ; rhs = -rp + A v ; KKT solve ; ds = -rd - A^T dy ;
+// dx = -Theta ds + rxs / s (mehrotra_device.cu:111-176,627-660).  false when the solve failed.
+__device__ bool direction(Ctx& c, double* dx, double* ds, double tol) {
+    const Lp& L = c.L;
+    const double *x = L.N(VX), *s = L.N(VS), *th = L.N(VTH), *rd = L.N(VRD), *rxs = L.N(VRXS);
+    const double* rp = L.M(WRP);
+    double *v = L.N(VV), *rhs = L.M(WRHS), *dy = L.M(WDY);
+    (void)x;
+    for (int j = threadIdx.x; j < L.n; j += BT) v[j] = -th[j] * rd[j] - rxs[j] / s[j];
+    __syncthreads();
+    seg_pass<GR>(L.m, L.rp, RSUM, [&](int p) { return L.av[p] * v[L.ci[p]]; },
+                 [&](int i, double av) { rhs[i] = -rp[i] + av; });
+    __syncthreads();
+    const int budget = c.fp32 ? c.A.mixed_max_refine : (c.rel ? 60 : 10);
+    bool ok = kkt_solve(c, tol, budget);
+    if (!ok && c.fp32) {
+        // the FP32 factor could not deliver: FP64 factor of the same matrix, same right-hand side
+        if (!factor_any(c, false)) return false;
+        ok = kkt_solve(c, tol, c.rel ? 60 : 10);
+    }
+    if (!ok) return false;
+    seg_pass<GC>(L.n, L.cp, RSUM, [&](int q) { return L.cv[q] * dy[L.ri[q]]; },
+                 [&](int j, double aty) {
+                     const double d = -rd[j] - aty;
+                     ds[j] = d;
+                     dx[j] = -th[j] * d + rxs[j] / s[j];
+                 });
+    __syncthreads();
+    return true;
+}
+
+// verify_newton_direction (mehrotra_device.cu:494-587) with the line-search products:
+// qp = A dx, qd = A^T dy + ds; e1 = ||rp + qp|| / max(1, ||rp||) (0 in relative-gap mode),
+// e2 = ||rd + qd|| / max(1, ||rd||), e3 = ||s dx + x ds - rxs|| / max(1, ||rxs||); step bounds
+__device__ Dir verify(Ctx& c, const double* dx, const double* ds, const Res& nr) {
+    const Lp& L = c.L;
+    const double *x = L.N(VX), *s = L.N(VS), *rd = L.N(VRD), *rxs = L.N(VRXS);
+    const double *rp = L.M(WRP), *dy = L.M(WDY);
+    double *qp = L.M(WQP), *qd = L.N(VQD);
+    double v[6] = {0, 0, 0, 0, 1e30, 1e30};
+    seg_pass<GR>(L.m, L.rp, RSUM, [&](int p) { return L.av[p] * dx[L.ci[p]]; },
+                 [&](int i, double a) {
+                     qp[i] = a;
+                     const double r1 = rp[i] + a;
+                     v[0] += r1 * r1;
+                 });
+    seg_pass<GC>(L.n, L.cp, RSUM, [&](int q) { return L.cv[q] * dy[L.ri[q]]; },
+                 [&](int j, double a) {
+                     const double qj = a + ds[j];
+                     qd[j] = qj;
+                     const double r2 = rd[j] + qj;
+                     const double r3 = s[j] * dx[j] + x[j] * ds[j] - rxs[j];
+                     v[1] += r2 * r2;
+                     v[2] += r3 * r3;
+                     v[3] += rxs[j] * rxs[j];
+                     v[4] = fmin(v[4], step_len(x[j], dx[j]));
+                     v[5] = fmin(v[5], step_len(s[j], ds[j]));
+                 });
+    const int op[6] = {RSUM, RSUM, RSUM, RSUM, RMIN, RMIN};
+    wg_reduce<6>(v, op, c.red);
+    Dir d;
+    d.e1 = c.rel ? 0.0 : sqrt(v[0]) / fmax(1.0, nr.nrp);
+    d.e2 = sqrt(v[1]) / fmax(1.0, nr.nrd);
+    d.e3 = sqrt(v[2]) / fmax(1.0, sqrt(v[3]));
+    d.apmax = v[4];
+    d.admax = v[5];
+    return d;
+}
+
+__device__ __forceinline__ bool dir_ok(const Dir& d) {
+    return !(d.e1 > 1e-8 || d.e2 > 1e-8 || d.e3 > 1e-8);
+}
+
+
+// Scaling and start point (mehrotra_device.cu:93-102,286-323): the working CSR and CSC values
+// from the original ones, 10 Jacobi Ruiz passes, x = s = 1, y = 0.
+__device__ void init_lp(Ctx& c) {
+    const Lp& L = c.L;
+    double *b = L.M(WB), *R = L.M(WR), *fr = L.M(WTM), *y = L.M(WY);
+    double *cc = L.N(VC), *Cs = L.N(VCS), *fc = L.N(VTN), *x = L.N(VX), *s = L.N(VS);
+    double bi = 0.0, ci = 0.0;
+    for (int p = threadIdx.x; p < L.nnz; p += BT) {
+        L.cv[p] = L.v0[p];
+        L.av[p] = L.v0[L.pos[p]];
+    }
+    for (int i = threadIdx.x; i < L.m; i += BT) {
+        b[i] = L.b0[i];
+        R[i] = 1.0;
+        y[i] = 0.0;
+        bi = fmax(bi, fabs(L.b0[i]));
+    }
+    for (int j = threadIdx.x; j < L.n; j += BT) {
+        cc[j] = L.c0[j];
+        Cs[j] = 1.0;
+        x[j] = 1.0;
+        s[j] = 1.0;
+        ci = fmax(ci, fabs(L.c0[j]));
+    }
+    {
+        double v[2] = {bi, ci};
+        const int op[2] = {RMAX, RMAX};
+        wg_reduce<2>(v, op, c.red);   // its barriers also order the copies above
+        c.st.binf = v[0];
+        c.st.cinf = v[1];
+    }
+    for (int pass = 0; pass < 10; ++pass) {
+        seg_pass<GR>(L.m, L.rp, RMAX, [&](int p) { return fabs(L.av[p]); },
+                     [&](int i, double mx) { fr[i] = mx > 1e-8 ? 1.0 / sqrt(mx) : 1.0; });
+        seg_pass<GC>(L.n, L.cp, RMAX, [&](int q) { return fabs(L.cv[q]); },
+                     [&](int j, double mx) { fc[j] = mx > 1e-8 ? 1.0 / sqrt(mx) : 1.0; });
+        __syncthreads();
+        for (int i = threadIdx.x; i < L.m; i += BT) {
+            R[i] *= fr[i];
+            b[i] *= fr[i];
+        }
+        for (int j = threadIdx.x; j < L.n; j += BT) {
+            Cs[j] *= fc[j];
+            cc[j] *= fc[j];
+        }
+        // the same product fr_i fc_j scales the CSR and the CSC copy of every entry
+        seg_for<GR>(L.m, L.rp, [&](int i, int p) { L.av[p] *= fr[i] * fc[L.ci[p]]; });
+        seg_for<GC>(L.n, L.cp, [&](int j, int q) { L.cv[q] *= fr[L.ri[q]] * fc[j]; });
+        __syncthreads();
+    }
+}
+
+// Final figures of the LP: objective c_s^T x_s (= c^T x), b_s^T y_s, the relative gap and the
+// original-space relative residuals (the solver's result fields), then the solution unscaled:
+// x = C x_s, y = R y_s, s = s_s / C.
+__device__ void finish(Ctx& c, const lkb_lp& d, const Res& nr, int status) {
+    const Lp& L = c.L;
+    if (threadIdx.x == 0) atomicAdd(c.A.ctl + 1, 1u);   // LPs finished in this solve
+    c.st.phase = 2;
+    c.st.status = status;
+    c.st.objective = nr.cx;
+    c.st.dual_objective = nr.by;
+    c.st.rel_gap = fabs(nr.cx - nr.by) / fmax(1.0, fmax(fabs(nr.cx), fabs(nr.by)));
+    c.st.rel_primal = nr.rpinf / (1.0 + c.st.binf);
+    c.st.rel_dual = nr.rdinf / (1.0 + c.st.cinf);
+    c.st.primal_residual = nr.nrp;
+    c.st.dual_residual = nr.nrd;
+    c.st.mu = nr.mu;
+    const double *x = L.N(VX), *s = L.N(VS), *Cs = L.N(VCS), *y = L.M(WY), *R = L.M(WR);
+    for (int j = threadIdx.x; j < L.n; j += BT) {
+        c.A.x_out[d.on + j] = x[j] * Cs[j];
+        c.A.s_out[d.on + j] = s[j] / Cs[j];
+    }
+    for (int i = threadIdx.x; i < L.m; i += BT) c.A.y_out[d.om + i] = y[i] * R[i];
+}
+
+__device__ bool stop_rule(const Ctx& c, const Res& nr) {
+    if (!c.rel) return nr.nrp < c.A.tol && nr.nrd < c.A.tol && nr.mu < c.A.tol;
+    const double g = c.A.gap_tol;
+    const double big = fmax(1.0, fmax(fabs(nr.cx), fabs(nr.by)));
+    const double gap = fabs(nr.cx - nr.by) / big;
+    return gap <= g && nr.rpinf / (1.0 + c.st.binf) <= g && nr.rdinf / (1.0 + c.st.cinf) <= g &&
+           nr.rpinf <= fmax(0.5 * g, 1e-12 * (1.0 + c.st.binf)) &&
+           nr.rdinf <= fmax(0.5 * g, 1e-12 * (1.0 + c.st.cinf)) &&
+           nr.xsmax <= fmax(0.5 * g, 1e-14 * big);
+}
+
+// One Mehrotra iteration (mehrotra_device.cu:695-1047).  Returns true when the LP finished.
+__device__ bool iterate(Ctx& c, const lkb_lp& d) {
+    const Lp& L = c.L;
+    double *x = L.N(VX), *s = L.N(VS), *rxs = L.N(VRXS), *dxa = L.N(VDXA), *dsa = L.N(VDSA);
+    double *dx = L.N(VDX), *ds = L.N(VDS), *qd = L.N(VQD), *rd = L.N(VRD);
+    double *y = L.M(WY), *dy = L.M(WDY), *qp = L.M(WQP), *rp = L.M(WRP);
+    const int iter = c.st.iterations;
+    const Res nr = residuals(c);
+    if (iter == 0) {
+        c.st.rp0 = fmax(1.0, nr.nrp);
+        c.st.rd0 = fmax(1.0, nr.nrd);
+        c.st.mu0 = fmax(1.0, nr.mu);
+    }
+    if (iter >= c.A.max_iter) { finish(c, d, nr, 3); return true; }
+    if (stop_rule(c, nr)) { finish(c, d, nr, 0); return true; }
+    auto fail = [&](int code) { c.st.fail_code = code; finish(c, d, nr, 3); return true; };
+
+    // ---- factorization of this iteration (FP32 while the Theta spread allows it)
+    const bool want32 = c.A.precision == 1 && nr.thmin > 0.0 && nr.thmax / nr.thmin <= c.A.mixed_spread;
+    diag_scaling(c);
+    if (!factor_any(c, want32)) return fail(1);
+    const double tol = 1e-9 * fmax(1.0, nr.nrp);
+
+    // ---- predictor
+    for (int j = threadIdx.x; j < L.n; j += BT) rxs[j] = -x[j] * s[j] + 0.0;
+    __syncthreads();
+    if (!direction(c, dxa, dsa, tol)) return fail(2);
+    const Dir dp = verify(c, dxa, dsa, nr);
+    if (!dir_ok(dp)) return fail(3);
+    double mu_aff;
+    {
+        const double ap = fmin(1.0, dp.apmax), ad = fmin(1.0, dp.admax);
+        double v[1] = {0.0};
+        for (int j = threadIdx.x; j < L.n; j += BT) v[0] += (x[j] + ap * dxa[j]) * (s[j] + ad * dsa[j]);
+        const int op[1] = {RSUM};
+        wg_reduce<1>(v, op, c.red);
+        mu_aff = v[0] / L.n;
+    }
+    const double sigma = fmin(1.0, pow(fmax(0.0, mu_aff) / fmax(1e-16, nr.mu), 3.0));
+
+    // ---- corrector
+    for (int j = threadIdx.x; j < L.n; j += BT) rxs[j] = -x[j] * s[j] + sigma * nr.mu - dxa[j] * dsa[j];
+    __syncthreads();
+    if (!direction(c, dx, ds, tol)) return fail(4);
+    const Dir dc = verify(c, dx, ds, nr);
+    double ap = fmin(1.0, 0.995 * dc.apmax), ad = fmin(1.0, 0.995 * dc.admax);
+    const double merit0 = nr.nrp / c.st.rp0 + nr.nrd / c.st.rd0 + nr.mu / c.st.mu0;
+    bool accepted = false;
+    for (int dir = dir_ok(dc) ? 0 : 1; dir < 2 && !accepted; ++dir) {
+        if (dir == 1) {
+            // centred fallback: sigma = 0.5, common step (mehrotra_device.cu:905-995)
+            for (int j = threadIdx.x; j < L.n; j += BT) rxs[j] = -x[j] * s[j] + 0.5 * nr.mu;
+            __syncthreads();
+            if (!direction(c, dx, ds, tol)) break;
+            const Dir dz = verify(c, dx, ds, nr);
+            if (!dir_ok(dz)) break;
+            ap = ad = fmin(1.0, 0.995 * fmin(dz.apmax, dz.admax));
+        }
+        const int ntr = dir == 0 ? 15 : 30;
+        double a = ap, b = ad, fa = -1.0, fb = -1.0;
+        for (int t = 0; t < ntr; ++t) {
+            double v[6] = {0, 0, 0, 1e300, 1e300, 0};
+            for (int j = threadIdx.x; j < L.n; j += BT) {
+                const double xn = x[j] + a * dx[j], sn = s[j] + b * ds[j], r2 = rd[j] + b * qd[j];
+                v[1] += r2 * r2;
+                v[2] += xn * sn;
+                v[3] = fmin(v[3], xn);
+                v[4] = fmin(v[4], sn);
+            }
+            for (int i = threadIdx.x; i < L.m; i += BT) {
+                const double r1 = rp[i] + a * qp[i], yn = y[i] + b * dy[i];
+                v[0] += r1 * r1;
+                v[5] += yn * yn;
+            }
+            const int op[6] = {RSUM, RSUM, RSUM, RMIN, RMIN, RSUM};
+            wg_reduce<6>(v, op, c.red);
+            const double nrp2 = sqrt(v[0]), nrd2 = sqrt(v[1]), mu2 = v[2] / L.n, ny = sqrt(v[5]);
+            const double merit = nrp2 / c.st.rp0 + nrd2 / c.st.rd0 + mu2 / c.st.mu0;
+            const double stp = fmin(a, b);
+            const bool finite = isfinite(nrp2) && isfinite(nrd2) && isfinite(mu2) && isfinite(merit) &&
+                                isfinite(merit0) && isfinite(v[3]) && isfinite(v[4]) && isfinite(a) &&
+                                isfinite(b) && isfinite(ny);
+            const bool positive = v[3] > 0.0 && v[4] > 0.0 && mu2 > 0.0 && stp > 0.0;
+            const bool progress = merit < merit0 && merit <= (1.0 - 1e-4 * stp) * merit0;
+            if (finite && positive && progress) { accepted = true; break; }
+            if (c.rel && dir == 1 && finite && positive && fa < 0) { fa = a; fb = b; }
+            a *= 0.5;
+            b *= 0.5;
+        }
+        if (!accepted && c.rel && dir == 1 && fa > 0) {
+            // relative-gap mode: the longest admissible centred step (non-monotone)
+            a = fa;
+            b = fb;
+            accepted = true;
+        }
+        if (accepted) {
+            for (int j = threadIdx.x; j < L.n; j += BT) {
+                x[j] += a * dx[j];
+                s[j] += b * ds[j];
+            }
+            for (int i = threadIdx.x; i < L.m; i += BT) y[i] += b * dy[i];
+            __syncthreads();
+        }
+    }
+    if (!accepted) return fail(5);
+    c.st.iterations = iter + 1;
+    return false;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_batch(lkb_args A) {
+    __shared__ int s_ticket;
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = (int)atomicAdd(A.ctl, 1u);
+        __syncthreads();
+        const int ticket = s_ticket;
+        __syncthreads();
+        if (ticket >= A.count) break;
+        const int li = A.order[ticket];
+        const lkb_lp d = A.lps[li];
+        Ctx c{A, make_lp(A, d), A.state[li], g_red, A.stop_rule == 1, false, 0.0};
+        if (c.st.phase == 2) continue;
+        if (c.st.phase == 0) {
+            c.st.phase = 1;
+            c.st.iterations = 0;
+            c.st.fail_code = 0;
+            c.st.factorizations = c.st.fp32_factorizations = c.st.solves = c.st.pcg_steps = 0;
+            init_lp(c);
+        }
+        for (int k = 0; k < A.iters_per_visit; ++k)
+            if (iterate(c, d)) break;
+        __syncthreads();
+        if (threadIdx.x == 0) A.state[li] = c.st;
+        __syncthreads();
+    }
+}
+
+__global__ void k_batch_reset(uint32_t* ctl, lkb_state* state, const int32_t* order, int count,
+                              int restart) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctl[0] = 0u;
+        if (restart) ctl[1] = 0u;
+    }
+    if (restart && i < count) state[order[i]].phase = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lkd_batch_reset(uint32_t* ctl, lkb_state* state, const int32_t* order, int32_t count,
+                    int32_t restart, void* stream) {
+    if (!ctl || count < 0 || (restart && (!state || !order))) return (int)hipErrorInvalidValue;
+    const int grid = count > 0 ? (count + 255) / 256 : 1;
+    LK_LAUNCH(k_batch_reset, grid, 256, 0, stream, ctl, state, order, count, restart);
+    return lk_last_error();
+}
+
+int lkd_batch_solve(const lkb_args* args, int32_t grid, void* stream) {
+    if (!args || grid < 1 || args->count < 0 || args->iters_per_visit < 1) return (int)hipErrorInvalidValue;
+    if (args->count == 0) return 0;
+    LK_LAUNCH(k_batch, grid, 256, 0, stream, *args);
+    return lk_last_error();
+}
+
+}  // extern "C"

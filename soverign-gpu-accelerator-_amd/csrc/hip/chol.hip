@@ -23,31 +23,15 @@
 // All workgroups are 256 threads (4 wavefronts of 64).  Task i of a launch runs on workgroup
 // (i % per) * 8 + i / per (per = grid / 8): consecutive tasks -- the tiles of one supernode --
 // share an XCD and its L2.
-#include "common.hpp"
+#include "dense_tile.hpp"
 
 namespace {
-
-constexpr int TS = 64;        // block-column width, diagonal block edge
-constexpr int DP = 65;        // LDS pitch of a 64x64 block
-constexpr int MT = 128;       // GEMM tile rows
-constexpr int NC = 64;        // GEMM tile columns
-constexpr int PA = MT + 8;    // LDS pitch of the A chunk (k-major)
-constexpr int PB = NC + 8;    // LDS pitch of the B chunk
-
 // Staging slot of a factored diagonal block: L (64x64 column-major, pitch 64, identity past kb)
 // followed by the inverses of its four 16x16 diagonal blocks (column-major, pitch 16).
 constexpr int STAGE = TS * TS + 4 * 256;
 static_assert(STAGE == LKD_STAGE, "staging slot size");
 static_assert(LKD_PART == 32 * 256, "partial product: 32 accumulator elements per thread");
 
-template <typename T> struct KChunk;
-template <> struct KChunk<double> { static constexpr int v = 16; };
-template <> struct KChunk<float> { static constexpr int v = 32; };
-
-template <typename T> struct GemmLds {
-    T a[2][KChunk<T>::v * PA];
-    T b[2][KChunk<T>::v * PB];
-};
 // the GEMM buffers are reused afterwards: 64 x DP staging tile (UPDATE), staged factor (STEP),
 // diagonal block + inverses (STEP, next diagonal)
 static_assert(sizeof(GemmLds<double>) >= STAGE * sizeof(double), "staged factor");
@@ -64,265 +48,6 @@ __device__ __forceinline__ bool wg_error(const int32_t* err, int* s) {
     if (threadIdx.x == 0) *s = *(volatile const int32_t*)err;
     __syncthreads();
     return *s != 0;
-}
-
-template <typename T> using Acc = typename Mfma<T>::acc;
-
-template <typename T, int X, int Y>
-__device__ __forceinline__ void zero_acc(Acc<T> (&acc)[X][Y]) {
-#pragma unroll
-    for (int x = 0; x < X; ++x)
-#pragma unroll
-        for (int y = 0; y < Y; ++y) acc[x][y] = Acc<T>{0, 0, 0, 0};
-}
-
-// Double-buffered K loop shared by the GEMM variants: chunks of KC columns of
-// A (rows 0..ra of pa, column-major, lda) and B (rows 0..rb of pb, ldb) are staged k-major in
-// LDS (rows past ra / rb and columns past kend read as zero); compute(As, Bs, k) consumes the
-// staged chunk for k = 0, 4, .., KC-4.  Global loads run two chunks ahead in two register
-// sets (chunk j in set j % 2): while chunk c is computed, chunks c + 1 and c + 2 are in
-// flight, and chunk c + 1 goes to LDS right after.  One barrier per chunk; the LDS is free
-// again when it returns.
-template <typename T, typename F>
-__device__ __forceinline__ void gemm_loop(const T* __restrict__ pa, int64_t lda, int ra,
-                                          const T* __restrict__ pb, int64_t ldb, int rb,
-                                          int kbeg, int kend, GemmLds<T>& sm, F&& compute) {
-    constexpr int KC = KChunk<T>::v;
-    constexpr int NA = KC * MT / 256, NB = KC * NC / 256;
-    const int t = threadIdx.x;
-    const int ar = t & (MT - 1), ak = t >> 7;
-    const int br = t & (NC - 1), bk = t >> 6;
-    const bool arow = ar < ra, brow = br < rb;
-    T va[2][NA], vb[2][NB];
-    auto load = [&](int set, int kc) {
-#pragma unroll
-        for (int q = 0; q < NA; ++q) {
-            const int k = kc + ak + 2 * q;
-            va[set][q] = (arow && k < kend) ? pa[(int64_t)k * lda + ar] : T(0);
-        }
-#pragma unroll
-        for (int q = 0; q < NB; ++q) {
-            const int k = kc + bk + 4 * q;
-            vb[set][q] = (brow && k < kend) ? pb[(int64_t)k * ldb + br] : T(0);
-        }
-    };
-    auto store = [&](int set, int buf) {
-#pragma unroll
-        for (int q = 0; q < NA; ++q) sm.a[buf][(ak + 2 * q) * PA + ar] = va[set][q];
-#pragma unroll
-        for (int q = 0; q < NB; ++q) sm.b[buf][(bk + 4 * q) * PB + br] = vb[set][q];
-    };
-    const int nch = (kend - kbeg + KC - 1) / KC;
-    if (nch <= 0) return;
-    load(0, kbeg);
-    store(0, 0);
-    if (nch > 1) load(1, kbeg + KC);
-    if (nch > 2) load(0, kbeg + 2 * KC);
-    __syncthreads();
-    // unrolled by two so that register sets and LDS buffers are compile-time indices
-    for (int c = 0; c < nch; c += 2) {
-#pragma unroll
-        for (int k = 0; k < KC; k += 4) compute(sm.a[0], sm.b[0], k);
-        if (c + 1 < nch) store(1, 1);
-        __syncthreads();
-        if (c + 3 < nch) load(1, kbeg + (c + 3) * KC);
-        if (c + 1 >= nch) break;
-#pragma unroll
-        for (int k = 0; k < KC; k += 4) compute(sm.a[1], sm.b[1], k);
-        if (c + 2 < nch) store(0, 0);
-        __syncthreads();
-        if (c + 4 < nch) load(0, kbeg + (c + 4) * KC);
-    }
-}
-
-// NT: acc[x][y] (wave w) = tile rows 32w + 16x .., columns 16y ..:
-//   acc += sum_k A(r, k) B(c, k);  element e of lane l: row 32w + 16x + row(l,e), col 16y + l%16
-template <typename T>
-__device__ __forceinline__ void gemm_nt(const T* __restrict__ pa, int64_t lda, int ra,
-                                        const T* __restrict__ pb, int64_t ldb, int rb, int kbeg,
-                                        int kend, GemmLds<T>& sm, Acc<T> (&acc)[2][4]) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
-    gemm_loop<T>(pa, lda, ra, pb, ldb, rb, kbeg, kend, sm, [&](const T* As, const T* Bs, int k) {
-        const T a0 = As[(k + lk) * PA + 32 * w + li];
-        const T a1 = As[(k + lk) * PA + 32 * w + 16 + li];
-        T b[4];
-#pragma unroll
-        for (int y = 0; y < 4; ++y) b[y] = Bs[(k + lk) * PB + 16 * y + li];
-#pragma unroll
-        for (int y = 0; y < 4; ++y) {
-            acc[0][y] = Mfma<T>::mma(a0, b[y], acc[0][y]);
-            acc[1][y] = Mfma<T>::mma(a1, b[y], acc[1][y]);
-        }
-    });
-}
-
-// TN (the transposed product): acc[p][b] (wave w) holds D = B A^T for B columns 16p .. and tile
-// rows 32w + 16b ..:  element e of lane l = sum_k A(32w + 16b + l%16, k) B(16p + row(l,e), k),
-// i.e. tile row from the lane, column from the element -- the B-operand layout of the solve.
-template <typename T>
-__device__ __forceinline__ void gemm_tn(const T* __restrict__ pa, int64_t lda, int ra,
-                                        const T* __restrict__ pb, int64_t ldb, int rb, int kbeg,
-                                        int kend, GemmLds<T>& sm, Acc<T> (&acc)[4][2]) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
-    gemm_loop<T>(pa, lda, ra, pb, ldb, rb, kbeg, kend, sm, [&](const T* As, const T* Bs, int k) {
-        const T a0 = As[(k + lk) * PA + 32 * w + li];
-        const T a1 = As[(k + lk) * PA + 32 * w + 16 + li];
-        T b[4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p) b[p] = Bs[(k + lk) * PB + 16 * p + li];
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            acc[p][0] = Mfma<T>::mma(b[p], a0, acc[p][0]);
-            acc[p][1] = Mfma<T>::mma(b[p], a1, acc[p][1]);
-        }
-    });
-}
-
-// f(row, col, value) for every NT accumulator element of this thread (tile coordinates).
-template <typename T, typename F>
-__device__ __forceinline__ void for_acc(Acc<T> (&acc)[2][4], F&& f) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y)
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                f(32 * w + 16 * x + Mfma<T>::row(lane, e), 16 * y + (lane & 15), acc[x][y][e]);
-}
-
-// Wave-uniform broadcast of lane `src`'s value (src is a compile-time constant at every use).
-__device__ __forceinline__ double bcast_lane(double v, int src) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ float bcast_lane(float v, int src) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
-}
-
-// 1/sqrt(d) for d > 0: hardware estimate refined by two Newton steps (each squares the relative
-// error), avoiding the long correctly-rounded sqrt + divide sequences on the critical path.
-__device__ __forceinline__ double rsqrt_nr(double d) {
-    double y = __builtin_amdgcn_rsq(d);
-    y = y * fma(-0.5 * d * y, y, 1.5);
-    y = y * fma(-0.5 * d * y, y, 1.5);
-    return y;
-}
-__device__ __forceinline__ float rsqrt_nr(float d) {
-    float y = __builtin_amdgcn_rsqf(d);
-    y = y * fmaf(-0.5f * d * y, y, 1.5f);
-    return y;
-}
-
-// One wavefront factors the 16x16 block at Ds(o, o) and inverts its factor.  Lane r (mod 16)
-// holds row r in registers; each finished column goes to LDS (cb, 16 values) and comes back as
-// broadcast reads, in order within the wavefront, so no barriers are needed.  Writes L (lower)
-// back to Ds and L^{-1} to Lv (column-major, pitch 16, upper part zero).  Returns false
-// (wave-uniform) when a pivot is not positive.
-template <typename T>
-__device__ __forceinline__ bool potrf16_wave(T* Ds, int o, T* Lv, T* cb) {
-    const int l = threadIdx.x & 63, r = l & 15;
-    T a[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) a[k] = Ds[(o + k) * DP + o + r];
-    bool ok = true;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const T d = bcast_lane(a[j], j);
-        ok = ok && (d > T(0));
-        const T is = rsqrt_nr(d);                 // 1 / L[j][j]
-        a[j] = r == j ? d * is : a[j] * is;       // r > j: L[r][j]; r < j: unused upper part
-        if (l < 16) {
-            cb[16 * j + r] = a[j];                // column j (diagonal entry included)
-            if (r == j) cb[256 + j] = is;
-        }
-        __builtin_amdgcn_wave_barrier();
-        T lkj[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (k > j) lkj[k] = cb[16 * j + k];
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (k > j) a[k] = fma(-a[j], lkj[k], a[k]);
-        __builtin_amdgcn_sched_barrier(0);   // keep each column's loads next to their use
-    }
-    // column c = r of L^{-1}: x[i] = (delta_ic - sum_{k<i} L[i][k] x[k]) / L[i][i], with
-    // L[i][k] = lane i's a[k] broadcast by readlane (scalar registers: few VGPRs, which matters
-    // where this is inlined into the step kernel)
-    T x[16];
-    const T invd = cb[256 + r];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        T acc = i == r ? T(1) : T(0);
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (k < i) acc = fma(-bcast_lane(a[k], i), x[k], acc);
-        x[i] = acc * bcast_lane(invd, i);
-    }
-    if (l < 16) {
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (k <= r) Ds[(o + k) * DP + o + r] = a[k];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) Lv[r * 16 + i] = i >= r ? x[i] : T(0);
-    }
-    return ok;
-}
-
-// In-place Cholesky of the 64x64 block Ds[c*DP + r] (lower part; rows/columns past the valid
-// block must hold the identity), blocked by 16: wave 0 factors and inverts the diagonal block,
-// the 16-row panel below is solved on MFMA (L_ip^T = Linv_pp A_ip^T) and the trailing blocks
-// are updated on MFMA (A_ij -= L_ip L_jp^T).  Lv receives the four diagonal inverses
-// (Lv + 256 p, see potrf16_wave).  Three barriers per 16 columns.  Returns false, uniformly
-// across the workgroup, when a pivot is not positive.
-template <typename T>
-__device__ __forceinline__ bool potrf64(T* Ds, T* Lv, int* s_ok) {
-    const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15;
-    using R = Mfma<T>;
-    __shared__ T cb[256 + 16];
-    // not unrolled: one copy of the 16-column code keeps the kernels that inline this small
-    // enough for the instruction cache
-#pragma unroll 1
-    for (int p = 0; p < 4; ++p) {
-        const int o = 16 * p;
-        if (w == 0) {
-            const bool ok = potrf16_wave<T>(Ds, o, Lv + 256 * p, cb);
-            if (l == 0) *s_ok = ok ? 1 : 0;
-        }
-        __syncthreads();
-        if (!*s_ok) return false;
-        if (w < 3 - p) {                          // panel block i: L_ip^T = Linv_pp * A_ip^T
-            const int i = p + 1 + w;
-            Acc<T> d = Acc<T>{0, 0, 0, 0};
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int k = R::row(l, s);
-                d = R::mma(Lv[256 * p + k * 16 + li], Ds[(o + k) * DP + 16 * i + li], d);
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) Ds[(o + R::row(l, e)) * DP + 16 * i + li] = d[e];
-        }
-        __syncthreads();
-        // trailing blocks (i, jj), p < jj <= i <= 3, round-robin over the waves
-        int b = 0;
-#pragma unroll 1
-        for (int i = p + 1; i < 4; ++i)
-#pragma unroll 1
-            for (int jj = p + 1; jj <= i; ++jj, ++b) {
-                if ((b & 3) != w) continue;
-                Acc<T> d = Acc<T>{0, 0, 0, 0};
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int k = R::row(l, s);
-                    d = R::mma(Ds[(o + k) * DP + 16 * i + li], Ds[(o + k) * DP + 16 * jj + li], d);
-                }
-#pragma unroll
-                for (int e = 0; e < 4; ++e) Ds[(16 * jj + li) * DP + 16 * i + R::row(l, e)] -= d[e];
-            }
-        __syncthreads();
-    }
-    return true;
 }
 
 // Row-major copy of a factored diagonal block for the substitution kernels:

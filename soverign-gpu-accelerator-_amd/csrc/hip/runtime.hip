@@ -98,6 +98,25 @@ int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream) {
     return lk_last_error();
 }
 
+int lkd_graph_begin(void* stream) {
+    return (int)hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal);
+}
+int lkd_graph_end(void* stream, void** exec) {
+    hipGraph_t g = nullptr;
+    int e = (int)hipStreamEndCapture((hipStream_t)stream, &g);
+    if (e) return e;
+    hipGraphExec_t x = nullptr;
+    e = (int)hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    const int e2 = (int)hipGraphDestroy(g);
+    if (e) return e;
+    *exec = x;
+    return e2;
+}
+int lkd_graph_launch(void* exec, void* stream) {
+    return (int)hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)stream);
+}
+int lkd_graph_destroy(void* exec) { return exec ? (int)hipGraphExecDestroy((hipGraphExec_t)exec) : 0; }
+
 int lkd_queue_check(const uint32_t* cnt, const int32_t* qoff, int32_t nq, int32_t nx,
                     int32_t bit, int32_t* err, void* stream) {
     if (nq <= 0) return 0;

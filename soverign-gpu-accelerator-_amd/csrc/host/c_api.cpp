@@ -10,6 +10,7 @@
 #include "core/problem.hpp"
 #include "gpu/kkt.hpp"
 #include "gpu/vram_arena.hpp"
+#include "ipm/batch.hpp"
 #include "ipm/mehrotra.hpp"
 #include "ipm/symbolic.hpp"
 #include "lakshya.h"
@@ -171,6 +172,41 @@ struct lk_ipm {
     core::Model model;
     std::unique_ptr<ipm::MehrotraSolver> solver;
 };
+struct lk_batch {
+    std::unique_ptr<ipm::BatchedMehrotraSolver> solver;
+};
+
+namespace {
+ipm::BatchOptions to_batch_options(const lk_batch_options* o) {
+    ipm::BatchOptions r;
+    if (!o) return r;
+    r.stop_rule = o->stop_rule;
+    r.precision = o->precision;
+    r.max_iter = o->max_iter;
+    r.mixed_max_refine = o->mixed_max_refine;
+    r.tol = o->tol;
+    r.gap_tol = o->gap_tol;
+    r.mixed_fp64_spread = o->mixed_fp64_spread;
+    r.wg_per_cu = o->wg_per_cu;
+    r.iters_per_launch = o->iters_per_launch;
+    r.use_graph = o->use_graph;
+    r.order = o->order;
+    return r;
+}
+void copy_summary(const ipm::BatchSummary& s, lk_batch_summary* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->count = s.count;
+    o->optimal = s.optimal;
+    o->launches = s.launches;
+    o->graph_replays = s.graph_replays;
+    o->ms = s.ms;
+    o->flops_chol = s.flops_chol;
+    o->wg_per_cu = s.wg_per_cu;
+    o->iters_per_launch = s.iters_per_launch;
+    o->grid = s.grid;
+}
+}  // namespace
 
 extern "C" {
 
@@ -489,6 +525,68 @@ int lk_ipm_read_launch_timing(lk_ipm* h, double* ms, double* flops, double* laun
 int lk_ipm_kkt_stats(lk_ipm* h, double* out12) {
     return guarded([&] { h->solver->kkt_statistics(out12); });
 }
+
+int lk_batch_default_options(lk_batch_options* o) {
+    return guarded([&] {
+        ipm::BatchOptions d;
+        std::memset(o, 0, sizeof(*o));
+        o->stop_rule = d.stop_rule;
+        o->precision = d.precision;
+        o->max_iter = d.max_iter;
+        o->mixed_max_refine = d.mixed_max_refine;
+        o->tol = d.tol;
+        o->gap_tol = d.gap_tol;
+        o->mixed_fp64_spread = d.mixed_fp64_spread;
+        o->wg_per_cu = d.wg_per_cu;
+        o->iters_per_launch = d.iters_per_launch;
+        o->use_graph = d.use_graph;
+        o->order = d.order;
+    });
+}
+int lk_batch_create(int32_t count, const int32_t* m, const int32_t* n, const int32_t* colptr,
+                    const int32_t* rowidx, const double* vals, const double* b, const double* c,
+                    const lk_batch_options* opt, lk_batch** out) {
+    return guarded([&] {
+        std::unique_ptr<lk_batch> h(new lk_batch);
+        h->solver.reset(new ipm::BatchedMehrotraSolver(count, m, n, colptr, rowidx, vals, b, c,
+                                                       to_batch_options(opt)));
+        *out = h.release();
+    });
+}
+int lk_batch_solve(lk_batch* h, lk_batch_summary* sum) {
+    return guarded([&] { copy_summary(h->solver->solve(), sum); });
+}
+int lk_batch_autotune(lk_batch* h, int32_t sample, lk_batch_summary* chosen) {
+    return guarded([&] { copy_summary(h->solver->autotune(sample), chosen); });
+}
+int lk_batch_results(lk_batch* h, lk_batch_result* out) {
+    return guarded([&] {
+        const auto& r = h->solver->results();
+        for (size_t i = 0; i < r.size(); ++i) {
+            lk_batch_result& o = out[i];
+            std::memset(&o, 0, sizeof(o));
+            o.status = r[i].status;
+            o.iterations = r[i].iterations;
+            o.fail_code = r[i].fail_code;
+            o.factorizations = r[i].factorizations;
+            o.fp32_factorizations = r[i].fp32_factorizations;
+            o.solves = r[i].solves;
+            o.pcg_steps = r[i].pcg_steps;
+            o.objective = r[i].objective;
+            o.dual_objective = r[i].dual_objective;
+            o.rel_gap = r[i].rel_gap;
+            o.rel_primal = r[i].rel_primal;
+            o.rel_dual = r[i].rel_dual;
+            o.primal_residual = r[i].primal_residual;
+            o.dual_residual = r[i].dual_residual;
+            o.mu = r[i].mu;
+        }
+    });
+}
+int lk_batch_solution(lk_batch* h, int32_t i, double* x, double* y, double* s) {
+    return guarded([&] { h->solver->solution(i, x, y, s); });
+}
+int lk_batch_destroy(lk_batch* h) { return guarded([&] { delete h; }); }
 
 int lk_verify_optimal(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
                       const double* vals, const double* c, const double* b, const double* lb,

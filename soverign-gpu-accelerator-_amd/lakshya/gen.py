@@ -216,10 +216,33 @@ def batch_sizes(count=4096, lo=50, hi=800, seed=0) -> np.ndarray:
     return rng.integers(lo, hi + 1, size=count)
 
 
-def batch(count=4096, lo=50, hi=800, seed=0):
-    """The batch family: LP i has m_i ~ U{lo..hi}; instance seed = seed * 1_000_003 + i."""
+def batch_lp(m: int, seed: int, density=0.05) -> LP:
+    """One LP of the batch family: n = round(2.5 m), every entry present with probability
+    `density` (the mask drawn column by column), N(0,1) values, standard construction."""
+    import scipy.sparse as sp
+    n = int(round(2.5 * m))
+    for attempt in range(1000):
+        rng = _rng("batch", seed, attempt)
+        mask = rng.random((n, m), dtype=np.float32) < density   # line j = column j of A
+        cols, rows = np.nonzero(mask)                              # column-major, rows ascending
+        if np.bincount(rows, minlength=m).min() == 0:
+            continue
+        indptr = np.zeros(n + 1, np.int64)
+        np.cumsum(np.bincount(cols, minlength=n), out=indptr[1:])
+        data = rng.standard_normal(rows.size)
+        A = sp.csc_matrix((data, rows.astype(np.int32), indptr.astype(np.int32)), shape=(m, n))
+        b, c = _standard_rhs(rng, A, m, n)
+        return LP(m, n, indptr.astype(np.int32), rows.astype(np.int32), data, b, c,
+                  f"batch_m{m}_s{seed}", {"attempt": attempt})
+    raise RuntimeError("batch_lp: no instance without empty rows")
+
+
+def batch(count=4096, lo=50, hi=800, seed=0, indices=None):
+    """The batch family: LP i has m_i ~ U{lo..hi} and instance seed seed * 1_000_003 + i
+    (`indices`: only those members, in that order)."""
     sizes = batch_sizes(count, lo, hi, seed)
-    return [dense_small(int(mi), seed=seed * 1_000_003 + i) for i, mi in enumerate(sizes)]
+    idx = range(count) if indices is None else indices
+    return [batch_lp(int(sizes[i]), seed * 1_000_003 + i) for i in idx]
 
 
 def parametric_base(m=20000, n=50000, k=8, seed=0) -> LP:

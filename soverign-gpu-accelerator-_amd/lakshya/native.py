@@ -55,6 +55,29 @@ class IpmResult(C.Structure):
                 ("nnz_L", C.c_double), ("flops_per_factorization", C.c_double)]
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [("stop_rule", C.c_int32), ("precision", C.c_int32), ("max_iter", C.c_int32),
+                ("mixed_max_refine", C.c_int32), ("tol", C.c_double), ("gap_tol", C.c_double),
+                ("mixed_fp64_spread", C.c_double), ("wg_per_cu", C.c_int32),
+                ("iters_per_launch", C.c_int32), ("use_graph", C.c_int32), ("order", C.c_int32)]
+
+
+class BatchSummary(C.Structure):
+    _fields_ = [("count", C.c_int32), ("optimal", C.c_int32), ("launches", C.c_int32),
+                ("graph_replays", C.c_int32), ("ms", C.c_double), ("flops_chol", C.c_double),
+                ("wg_per_cu", C.c_int32), ("iters_per_launch", C.c_int32), ("grid", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("fail_code", C.c_int32),
+                ("factorizations", C.c_int32), ("fp32_factorizations", C.c_int32),
+                ("solves", C.c_int32), ("pcg_steps", C.c_int32), ("pad", C.c_int32),
+                ("objective", C.c_double), ("dual_objective", C.c_double),
+                ("rel_gap", C.c_double), ("rel_primal", C.c_double), ("rel_dual", C.c_double),
+                ("primal_residual", C.c_double), ("dual_residual", C.c_double), ("mu", C.c_double)]
+
+
 STATUS = {0: "Optimal", 1: "Infeasible", 2: "Unbounded", 3: "IterationLimit"}
 
 _lib = None
@@ -101,6 +124,14 @@ def lib():
                           C.POINTER(IpmOptions), C.POINTER(_vp)],
         "lk_ipm_run": [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(IpmResult), _vp, _vp, _vp],
         "lk_ipm_destroy": [_vp],
+        "lk_batch_default_options": [C.POINTER(BatchOptions)],
+        "lk_batch_create": [C.c_int32, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p,
+                            C.POINTER(BatchOptions), C.POINTER(_vp)],
+        "lk_batch_solve": [_vp, C.POINTER(BatchSummary)],
+        "lk_batch_autotune": [_vp, C.c_int32, C.POINTER(BatchSummary)],
+        "lk_batch_results": [_vp, _vp],
+        "lk_batch_solution": [_vp, C.c_int32, _f64p, _f64p, _f64p],
+        "lk_batch_destroy": [_vp],
         "lk_verify_optimal": [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _vp, _vp,
                               C.c_int32, _f64p, _f64p, C.c_double, C.POINTER(C.c_double),
                               C.POINTER(C.c_int32)],
@@ -369,6 +400,73 @@ class IPM:
     def close(self):
         if self.h:
             _ck(lib().lk_ipm_destroy(self.h))
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ batched IPM
+def batch_options(**kw) -> BatchOptions:
+    o = BatchOptions()
+    _ck(lib().lk_batch_default_options(C.byref(o)))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _summary(s: BatchSummary) -> dict:
+    return {f: getattr(s, f) for f, _ in BatchSummary._fields_ if f != "pad"}
+
+
+class Batch:
+    """ipm::BatchedMehrotraSolver over a list of CSC LPs (lakshya.gen.LP)."""
+
+    def __init__(self, lps, **opts):
+        self.count = len(lps)
+        self.m = np.array([lp.m for lp in lps], np.int32)
+        self.n = np.array([lp.n for lp in lps], np.int32)
+        colptr = np.concatenate([np.asarray(lp.indptr, np.int32) for lp in lps])
+        rowidx = np.concatenate([np.asarray(lp.indices, np.int32) for lp in lps])
+        vals = np.concatenate([np.asarray(lp.data, np.float64) for lp in lps])
+        b = np.concatenate([np.asarray(lp.b, np.float64) for lp in lps])
+        c = np.concatenate([np.asarray(lp.c, np.float64) for lp in lps])
+        o = batch_options(**opts)
+        self.h = _vp()
+        _ck(lib().lk_batch_create(self.count, self.m, self.n, colptr, rowidx, vals, b, c,
+                                  C.byref(o), C.byref(self.h)))
+
+    def solve(self) -> dict:
+        s = BatchSummary()
+        _ck(lib().lk_batch_solve(self.h, C.byref(s)))
+        return _summary(s)
+
+    def autotune(self, sample=512) -> dict:
+        s = BatchSummary()
+        _ck(lib().lk_batch_autotune(self.h, sample, C.byref(s)))
+        return _summary(s)
+
+    def results(self) -> list:
+        arr = (BatchResult * self.count)()
+        _ck(lib().lk_batch_results(self.h, C.cast(arr, _vp)))
+        out = []
+        for r in arr:
+            d = {f: getattr(r, f) for f, _ in BatchResult._fields_ if f != "pad"}
+            d["status_name"] = STATUS.get(r.status, str(r.status))
+            out.append(d)
+        return out
+
+    def solution(self, i):
+        x, y, s = np.zeros(self.n[i]), np.zeros(self.m[i]), np.zeros(self.n[i])
+        _ck(lib().lk_batch_solution(self.h, i, x, y, s))
+        return x, y, s
+
+    def close(self):
+        if self.h:
+            _ck(lib().lk_batch_destroy(self.h))
             self.h = _vp()
 
     def __del__(self):
