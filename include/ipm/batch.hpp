@@ -40,6 +40,9 @@ struct BatchLPResult {
     int factorizations = 0, fp32_factorizations = 0, solves = 0, pcg_steps = 0;
     double objective = 0, dual_objective = 0, rel_gap = 0, rel_primal = 0, rel_dual = 0;
     double primal_residual = 0, dual_residual = 0, mu = 0;
+    // device time of the LP's workgroup (ms): scaling, sparse passes and vector work, assembly,
+    // dense factorization, dense triangular solves
+    double ms_init = 0, ms_other = 0, ms_assemble = 0, ms_factor = 0, ms_solve = 0;
 };
 
 struct BatchSummary {

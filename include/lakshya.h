@@ -186,6 +186,7 @@ typedef struct {
     int32_t fp32_factorizations, solves, pcg_steps, pad;
     double objective, dual_objective, rel_gap, rel_primal, rel_dual;
     double primal_residual, dual_residual, mu;
+    double ms_init, ms_other, ms_assemble, ms_factor, ms_solve;   /* LP's workgroup time */
 } lk_batch_result;
 int lk_batch_default_options(lk_batch_options* opt);
 int lk_batch_create(int32_t count, const int32_t* m, const int32_t* n, const int32_t* colptr,

@@ -46,6 +46,7 @@ int lkd_event_destroy(void* ev);
 int lkd_event_record(void* ev, void* stream);
 int lkd_event_elapsed_ms(float* ms, void* start, void* stop);
 int lkd_fill_f64(double* dst, double value, int64_t n, void* stream);   /* kernel fill */
+int lkd_wallclock_khz(int* khz);   /* rate of the device wall clock (wall_clock64) */
 /* streaming copy kernel (n even, 16-byte aligned): calibrates HBM traffic counters */
 int lkd_copy_f64(double* dst, const double* src, int64_t n, void* stream);
 /* Queue completeness after launches that deal tasks from per-XCD queues: for every queue q < nq
@@ -372,6 +373,9 @@ typedef struct {
     double rp0, rd0, mu0, binf, cinf;
     double objective, dual_objective, rel_gap, rel_primal, rel_dual;
     double primal_residual, dual_residual, mu;
+    /* wall-clock ticks spent by the LP's workgroup: scaling and start point, sparse passes and
+     * vector work, assembly, dense factorization, dense triangular solves */
+    double t_init, t_other, t_assemble, t_factor, t_solve;
 } lkb_state;
 
 typedef struct {

@@ -20,7 +20,11 @@
  * is printed.  Relative-gap mode (stop_rule 1, an extension with no reference counterpart) uses
  * PCG instead of plain refinement, regularizes from delta = 1e-14 (not 1e-9), accepts a solve at
  * the backward-error level only when PCG stagnates, skips the e1 re-check of the KKT residual,
- * stops on absolute residual and per-column complementarity targets as well, and takes the longest finite, positive centred trial when both
+ * stops on absolute residual and per-column complementarity targets as well (at REL_ABS_MARGIN
+ * of gap_tol), rejects trials that take a residual already inside its absolute target back
+ * out of it, tries dual-only steps of the Mehrotra direction before the centred one (the
+ * primal residual cannot grow; near the optimum the normal equations limit the primal
+ * accuracy of a direction), and takes the longest finite, positive centred trial when all
  * merit searches fail (non-monotone fallback): the reference's absolute 1e-9 acceptance and
  * monotone merit search stall before a 1e-8 gap (its own loop stops at 1e-6).  The device path
  * implements the same rules.  Sums use the reference's term order; multiply-adds use fma() so the assembled
@@ -35,6 +39,10 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* relative-gap mode: the absolute primal, dual and complementarity targets are this fraction of
+ * gap_tol (the FP64 certificate checks them absolutely at gap_tol) */
+#define REL_ABS_MARGIN 0.9
 
 #define OR_OK 0
 #define OR_NOT_SPD 1
@@ -576,9 +584,9 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
             /* absolute targets as well (the FP64 certificate checks |Ax-b| and the reduced costs
              * absolutely), floored where FP64 cannot resolve them */
             stop = gap <= opt->gap_tol && rpi / (1 + bi) <= opt->gap_tol && rdi / (1 + cinf) <= opt->gap_tol &&
-                   rpi <= fmax(0.5 * opt->gap_tol, 1e-12 * (1 + bi)) &&
-                   rdi <= fmax(0.5 * opt->gap_tol, 1e-12 * (1 + cinf)) &&
-                   xsmax <= fmax(0.5 * opt->gap_tol, 1e-14 * fmax(1.0, fmax(fabs(pobj), fabs(dobj))));
+                   rpi <= fmax(REL_ABS_MARGIN * opt->gap_tol, 1e-12 * (1 + bi)) &&
+                   rdi <= fmax(REL_ABS_MARGIN * opt->gap_tol, 1e-12 * (1 + cinf)) &&
+                   xsmax <= fmax(REL_ABS_MARGIN * opt->gap_tol, 1e-14 * fmax(1.0, fmax(fabs(pobj), fabs(dobj))));
         }
         if (stop) { res->iterations = iter; res->status = 0; done = 1; break; }
 
@@ -608,6 +616,25 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
         VERIFY(dx, dy, ds, corr_valid);
         double ap = fmin(1.0, 0.995 * step_len(n, x, dx)), ad = fmin(1.0, 0.995 * step_len(n, s, ds));
         double merit0 = nrp / rp0 + nrd / rd0 + mu / mu0;
+        /* relative-gap mode: a residual already inside its absolute target must stay inside */
+        double rpi0 = 0, rdi0 = 0, ptar = 0, dtar = 0;
+        if (relmode) {
+            double bi = 0, ci = 0;
+            for (int i = 0; i < m; ++i) { rpi0 = fmax(rpi0, fabs(rpv[i] / R[i])); bi = fmax(bi, fabs(b_in[i])); }
+            for (int j = 0; j < n; ++j) { rdi0 = fmax(rdi0, fabs(rd[j] / C[j])); ci = fmax(ci, fabs(c_in[j])); }
+            ptar = fmax(REL_ABS_MARGIN * opt->gap_tol, 1e-12 * (1 + bi));
+            dtar = fmax(REL_ABS_MARGIN * opt->gap_tol, 1e-12 * (1 + ci));
+        }
+#define KEEPS_TARGETS(OKV)                                                        \
+        do {                                                                      \
+            OKV = 1;                                                              \
+            if (relmode) {                                                        \
+                double rpt_ = 0, rdt_ = 0;                                        \
+                for (int i_ = 0; i_ < m; ++i_) rpt_ = fmax(rpt_, fabs(brp[i_] / R[i_])); \
+                for (int j_ = 0; j_ < n; ++j_) rdt_ = fmax(rdt_, fabs(brd[j_] / C[j_])); \
+                if ((rpi0 <= ptar && rpt_ > ptar) || (rdi0 <= dtar && rdt_ > dtar)) OKV = 0; \
+            }                                                                     \
+        } while (0)
         int accepted = 0;
         for (int dir = corr_valid ? 0 : 1; dir < 2 && !accepted; ++dir) {
             if (dir == 1) {
@@ -635,10 +662,34 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                              isfinite(ad) && isfinite(nrm2(m, by));
                 int pos = mx > 0.0 && ms > 0.0 && mu2 > 0.0 && st > 0.0;
                 int prog = merit < merit0 && merit <= (1.0 - 1e-4 * st) * merit0;
+                int keeps;
+                KEEPS_TARGETS(keeps);
+                prog = prog && keeps;
                 if (finite && pos && prog) { accepted = 1; break; }
                 if (relmode && dir == 1 && finite && pos && fallback_bt < 0) fallback_bt = bt;
                 ap *= 0.5;
                 ad *= 0.5;
+            }
+            if (!accepted && relmode && dir == 0) {
+                /* relative-gap mode: dual-only trials of the same direction (the primal point,
+                 * and with it the primal residual, stays), ad halving from 0.995 of its bound */
+                double ad2 = fmin(1.0, 0.995 * step_len(n, s, ds));
+                for (int bt = 0; bt < 15 && !accepted; ++bt) {
+                    for (int j = 0; j < n; ++j) { bx[j] = x[j]; bs[j] = s[j] + ad2 * ds[j]; }
+                    for (int i = 0; i < m; ++i) by[i] = y[i] + ad2 * dy[i];
+                    RESID(bx, by, bs, brp, brd);
+                    double nrp2 = nrm2(m, brp), nrd2 = nrm2(n, brd), mu2 = dot(n, bx, bs) / n;
+                    double ms = vmin(n, bs);
+                    double merit = nrp2 / rp0 + nrd2 / rd0 + mu2 / mu0;
+                    int finite = isfinite(nrp2) && isfinite(nrd2) && isfinite(mu2) && isfinite(merit) &&
+                                 isfinite(merit0) && isfinite(ms) && isfinite(ad2) && isfinite(nrm2(m, by));
+                    int pos = ms > 0.0 && mu2 > 0.0 && ad2 > 0.0;
+                    int prog = merit < merit0 && merit <= (1.0 - 1e-4 * ad2) * merit0;
+                    int keeps;
+                    KEEPS_TARGETS(keeps);
+                    if (finite && pos && prog && keeps) accepted = 1;
+                    else ad2 *= 0.5;
+                }
             }
             if (!accepted && relmode && dir == 1 && fallback_bt >= 0) {
                 /* non-monotone fallback: take the longest admissible centred step */

@@ -125,8 +125,19 @@ __device__ __forceinline__ void seg_pass(int nseg, const int32_t* __restrict__ p
         const int r = r0 + (int)threadIdx.x / G;
         double s = ident;
         if (r < nseg) {
+            // four independent partial results per lane keep four gathers in flight
             const int e = ptr[r + 1];
-            for (int p = ptr[r] + g; p < e; p += G) s = rop(op, s, term(p));
+            int p = ptr[r] + g;
+            double s1 = ident, s2 = ident, s3 = ident;
+            for (; p + 3 * G < e; p += 4 * G) {
+                const double t0 = term(p), t1 = term(p + G), t2 = term(p + 2 * G), t3 = term(p + 3 * G);
+                s = rop(op, s, t0);
+                s1 = rop(op, s1, t1);
+                s2 = rop(op, s2, t2);
+                s3 = rop(op, s3, t3);
+            }
+            for (; p < e; p += G) s = rop(op, s, term(p));
+            s = rop(op, rop(op, s, s1), rop(op, s2, s3));
         }
         s = group_op<G>(op, s);
         if (r < nseg && g == 0) f(r, s);
@@ -319,11 +330,21 @@ __device__ __attribute__((noinline)) void dense_solve(const T* __restrict__ F_, 
     for (int jb = 0; jb < m; jb += 64) {
         const int kb = min(64, m - jb);
         {   // part[g][r] = sum over columns c = g, g + 4, .. < jb of L(jb + r, c) z[c]
-            const int r = l, g = w;
-            double s = 0.0;
-            if (r < kb)
-                for (int c = g; c < jb; c += 4) s += (double)F[(int64_t)c * ld + jb + r] * z[c];
-            part[g * 64 + r] = s;
+            // (four columns per round in flight, rows past kb read row kb - 1 and are unused)
+            const int r = min(l, kb - 1), g = w;
+            const T* col = F + jb + r;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int c = g;
+            for (; c + 12 < jb; c += 16) {
+                const double a0 = (double)col[(int64_t)c * ld], a1 = (double)col[(int64_t)(c + 4) * ld];
+                const double a2 = (double)col[(int64_t)(c + 8) * ld], a3 = (double)col[(int64_t)(c + 12) * ld];
+                s0 += a0 * z[c];
+                s1 += a1 * z[c + 4];
+                s2 += a2 * z[c + 8];
+                s3 += a3 * z[c + 12];
+            }
+            for (; c < jb; c += 4) s0 += (double)col[(int64_t)c * ld] * z[c];
+            part[g * 64 + l] = (s0 + s1) + (s2 + s3);
         }
         stage_block(jb, kb);
         __syncthreads();
@@ -344,9 +365,19 @@ __device__ __attribute__((noinline)) void dense_solve(const T* __restrict__ F_, 
         const int jb = 64 * bi, kb = min(64, m - jb), r1 = jb + kb;
         // part[c] = sum over rows r >= r1 of L(r, jb + c) x[r], column c on wave c % 4
         for (int c = w; c < kb; c += 4) {
-            double s = 0.0;
-            for (int r = r1 + l; r < m; r += 64) s += (double)F[(int64_t)(jb + c) * ld + r] * z[r];
-            s = wave_sum(s);
+            const T* col = F + (int64_t)(jb + c) * ld;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int r = r1 + l;
+            for (; r + 192 < m; r += 256) {
+                const double a0 = (double)col[r], a1 = (double)col[r + 64];
+                const double a2 = (double)col[r + 128], a3 = (double)col[r + 192];
+                s0 += a0 * z[r];
+                s1 += a1 * z[r + 64];
+                s2 += a2 * z[r + 128];
+                s3 += a3 * z[r + 192];
+            }
+            for (; r < m; r += 64) s0 += (double)col[r] * z[r];
+            const double s = wave_sum((s0 + s1) + (s2 + s3));
             if (l == 0) part[c] = s;
         }
         stage_block(jb, kb);
@@ -375,6 +406,13 @@ struct Ctx {
     bool rel;
     bool fp32;        // the current factor is FP32
     double m0max;     // max_i M0_ii of the current Theta
+    long long mark;   // wall clock at the end of the last timed phase
+    // the time since the last mark goes to phase slot `slot` of the LP's state
+    __device__ __forceinline__ void tick(double& slot) {
+        const long long now = wall_clock64();
+        slot += (double)(now - mark);
+        mark = now;
+    }
 };
 
 // D_i = 1 / sqrt(max(|M0_ii|, 1e-12)), M0_ii = sum_k a_ik^2 theta_k (src/cuda/kkt.cu:75 floor)
@@ -428,16 +466,31 @@ __device__ __attribute__((noinline)) void assemble(Ctx& c, double delta) {
             }
             const int excl = incl - len;
             const int total = __shfl(incl, 63, 64);
-            for (int f0 = 0; f0 < total; f0 += 64) {
-                const int f = f0 + l;
-                const int fq = min(f, total - 1);
-                int j = 0;
+            // four rounds of 64 flattened entries at a time: all their loads are issued before
+            // the first accumulation (the LDS adds of one wavefront stay in program order)
+            for (int f0 = 0; f0 < total; f0 += 256) {
+                int q[4];
+                double cj[4];
 #pragma unroll
-                for (int b = 32; b > 0; b >>= 1)
-                    if (__shfl(incl, j + b - 1, 64) <= fq) j += b;
-                const int q = __shfl(p0, j, 64) + (fq - __shfl(excl, j, 64));
-                const double cj = __shfl(coef, j, 64);
-                if (f < total) atomicAdd(&buf[L.ri[q]], L.cv[q] * cj);
+                for (int u = 0; u < 4; ++u) {
+                    const int fq = min(f0 + 64 * u + l, total - 1);
+                    int j = 0;
+#pragma unroll
+                    for (int b = 32; b > 0; b >>= 1)
+                        if (__shfl(incl, j + b - 1, 64) <= fq) j += b;
+                    q[u] = __shfl(p0, j, 64) + (fq - __shfl(excl, j, 64));
+                    cj[u] = __shfl(coef, j, 64);
+                }
+                int rr[4];
+                double vv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    rr[u] = L.ri[q[u]];
+                    vv[u] = L.cv[q[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (f0 + 64 * u + l < total) atomicAdd(&buf[rr[u]], vv[u] * cj[u]);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -461,10 +514,13 @@ __device__ bool factor(Ctx& c, double delta0, int retries) {
     double delta = delta0;
     for (int r = 0; r < retries; ++r) {
         if (r > 0) delta *= 10.0;
+        c.tick(c.st.t_other);
         assemble<T>(c, delta);
-        if (dense_chol<T>(reinterpret_cast<T*>(c.L.fac), c.L.m, c.L.ld,
-                          reinterpret_cast<T*>(c.L.stage)))
-            return true;
+        c.tick(c.st.t_assemble);
+        const bool ok = dense_chol<T>(reinterpret_cast<T*>(c.L.fac), c.L.m, c.L.ld,
+                                      reinterpret_cast<T*>(c.L.stage));
+        c.tick(c.st.t_factor);
+        if (ok) return true;
         __syncthreads();
     }
     return false;
@@ -487,8 +543,10 @@ __device__ void precond(Ctx& c, double* x) {
     const double* D = L.M(WD);
     for (int i = threadIdx.x; i < L.m; i += BT) x[i] *= D[i];
     __syncthreads();
+    c.tick(c.st.t_other);
     if (c.fp32) dense_solve<float>(reinterpret_cast<const float*>(L.fac), L.m, L.ld, x);
     else dense_solve<double>(L.fac, L.m, L.ld, x);
+    c.tick(c.st.t_solve);
     for (int i = threadIdx.x; i < L.m; i += BT) x[i] *= D[i];
     __syncthreads();
 }
@@ -853,15 +911,54 @@ __device__ void finish(Ctx& c, const lkb_lp& d, const Res& nr, int status) {
     for (int i = threadIdx.x; i < L.m; i += BT) c.A.y_out[d.om + i] = y[i] * R[i];
 }
 
+// relative-gap mode: absolute targets of ||Ax - b||_inf, ||A^T y + s - c||_inf (original space)
+// and max_j x_j (c - A^T y)_j, a fraction REL_ABS_MARGIN of gap_tol (the FP64 certificate
+// checks them absolutely)
+constexpr double REL_ABS_MARGIN = 0.9;
+__device__ __forceinline__ double primal_target(const Ctx& c) {
+    return fmax(REL_ABS_MARGIN * c.A.gap_tol, 1e-12 * (1.0 + c.st.binf));
+}
+__device__ __forceinline__ double dual_target(const Ctx& c) {
+    return fmax(REL_ABS_MARGIN * c.A.gap_tol, 1e-12 * (1.0 + c.st.cinf));
+}
+
 __device__ bool stop_rule(const Ctx& c, const Res& nr) {
     if (!c.rel) return nr.nrp < c.A.tol && nr.nrd < c.A.tol && nr.mu < c.A.tol;
     const double g = c.A.gap_tol;
     const double big = fmax(1.0, fmax(fabs(nr.cx), fabs(nr.by)));
     const double gap = fabs(nr.cx - nr.by) / big;
     return gap <= g && nr.rpinf / (1.0 + c.st.binf) <= g && nr.rdinf / (1.0 + c.st.cinf) <= g &&
-           nr.rpinf <= fmax(0.5 * g, 1e-12 * (1.0 + c.st.binf)) &&
-           nr.rdinf <= fmax(0.5 * g, 1e-12 * (1.0 + c.st.cinf)) &&
-           nr.xsmax <= fmax(0.5 * g, 1e-14 * big);
+           nr.rpinf <= primal_target(c) && nr.rdinf <= dual_target(c) &&
+           nr.xsmax <= fmax(REL_ABS_MARGIN * g, 1e-14 * big);
+}
+
+// One backtracking trial x + a dx, s + b ds, y + b dy evaluated through the residual updates
+// rp + a (A dx), rd + b (A^T dy + ds): 2-norms, mu, minima, ||y||, and the original-space
+// infinity norms of both residuals.
+struct Trial { double nrp, nrd, mu, mx, ms, ny, rpinf, rdinf; };
+__device__ Trial trial(Ctx& c, double a, double b) {
+    const Lp& L = c.L;
+    const double *x = L.N(VX), *s = L.N(VS), *dx = L.N(VDX), *ds = L.N(VDS), *qd = L.N(VQD);
+    const double *rd = L.N(VRD), *Cs = L.N(VCS);
+    const double *y = L.M(WY), *dy = L.M(WDY), *qp = L.M(WQP), *rp = L.M(WRP), *R = L.M(WR);
+    double v[8] = {0, 0, 0, 1e300, 1e300, 0, 0, 0};
+    for (int j = threadIdx.x; j < L.n; j += BT) {
+        const double xn = x[j] + a * dx[j], sn = s[j] + b * ds[j], r2 = rd[j] + b * qd[j];
+        v[1] += r2 * r2;
+        v[2] += xn * sn;
+        v[3] = fmin(v[3], xn);
+        v[4] = fmin(v[4], sn);
+        v[7] = fmax(v[7], fabs(r2 / Cs[j]));
+    }
+    for (int i = threadIdx.x; i < L.m; i += BT) {
+        const double r1 = rp[i] + a * qp[i], yn = y[i] + b * dy[i];
+        v[0] += r1 * r1;
+        v[5] += yn * yn;
+        v[6] = fmax(v[6], fabs(r1 / R[i]));
+    }
+    const int op[8] = {RSUM, RSUM, RSUM, RMIN, RMIN, RSUM, RMAX, RMAX};
+    wg_reduce<8>(v, op, c.red);
+    return Trial{sqrt(v[0]), sqrt(v[1]), v[2] / L.n, v[3], v[4], sqrt(v[5]), v[6], v[7]};
 }
 
 // One Mehrotra iteration (mehrotra_device.cu:695-1047).  Returns true when the LP finished.
@@ -911,6 +1008,21 @@ __device__ bool iterate(Ctx& c, const lkb_lp& d) {
     const Dir dc = verify(c, dx, ds, nr);
     double ap = fmin(1.0, 0.995 * dc.apmax), ad = fmin(1.0, 0.995 * dc.admax);
     const double merit0 = nr.nrp / c.st.rp0 + nr.nrd / c.st.rd0 + nr.mu / c.st.mu0;
+    // relative-gap mode: a residual already inside its absolute target must stay inside
+    const bool keep_p = c.rel && nr.rpinf <= primal_target(c);
+    const bool keep_d = c.rel && nr.rdinf <= dual_target(c);
+    auto admissible = [&](const Trial& T, double a, double b, bool& finite, bool& positive) {
+        const double merit = T.nrp / c.st.rp0 + T.nrd / c.st.rd0 + T.mu / c.st.mu0;
+        const double stp = fmin(a, b);
+        finite = isfinite(T.nrp) && isfinite(T.nrd) && isfinite(T.mu) && isfinite(merit) &&
+                 isfinite(merit0) && isfinite(T.mx) && isfinite(T.ms) && isfinite(a) &&
+                 isfinite(b) && isfinite(T.ny);
+        positive = T.mx > 0.0 && T.ms > 0.0 && T.mu > 0.0 && stp > 0.0;
+        const bool progress = merit < merit0 && merit <= (1.0 - 1e-4 * stp) * merit0 &&
+                              !(keep_p && T.rpinf > primal_target(c)) &&
+                              !(keep_d && T.rdinf > dual_target(c));
+        return finite && positive && progress;
+    };
     bool accepted = false;
     for (int dir = dir_ok(dc) ? 0 : 1; dir < 2 && !accepted; ++dir) {
         if (dir == 1) {
@@ -925,33 +1037,23 @@ __device__ bool iterate(Ctx& c, const lkb_lp& d) {
         const int ntr = dir == 0 ? 15 : 30;
         double a = ap, b = ad, fa = -1.0, fb = -1.0;
         for (int t = 0; t < ntr; ++t) {
-            double v[6] = {0, 0, 0, 1e300, 1e300, 0};
-            for (int j = threadIdx.x; j < L.n; j += BT) {
-                const double xn = x[j] + a * dx[j], sn = s[j] + b * ds[j], r2 = rd[j] + b * qd[j];
-                v[1] += r2 * r2;
-                v[2] += xn * sn;
-                v[3] = fmin(v[3], xn);
-                v[4] = fmin(v[4], sn);
-            }
-            for (int i = threadIdx.x; i < L.m; i += BT) {
-                const double r1 = rp[i] + a * qp[i], yn = y[i] + b * dy[i];
-                v[0] += r1 * r1;
-                v[5] += yn * yn;
-            }
-            const int op[6] = {RSUM, RSUM, RSUM, RMIN, RMIN, RSUM};
-            wg_reduce<6>(v, op, c.red);
-            const double nrp2 = sqrt(v[0]), nrd2 = sqrt(v[1]), mu2 = v[2] / L.n, ny = sqrt(v[5]);
-            const double merit = nrp2 / c.st.rp0 + nrd2 / c.st.rd0 + mu2 / c.st.mu0;
-            const double stp = fmin(a, b);
-            const bool finite = isfinite(nrp2) && isfinite(nrd2) && isfinite(mu2) && isfinite(merit) &&
-                                isfinite(merit0) && isfinite(v[3]) && isfinite(v[4]) && isfinite(a) &&
-                                isfinite(b) && isfinite(ny);
-            const bool positive = v[3] > 0.0 && v[4] > 0.0 && mu2 > 0.0 && stp > 0.0;
-            const bool progress = merit < merit0 && merit <= (1.0 - 1e-4 * stp) * merit0;
-            if (finite && positive && progress) { accepted = true; break; }
+            bool finite, positive;
+            if (admissible(trial(c, a, b), a, b, finite, positive)) { accepted = true; break; }
             if (c.rel && dir == 1 && finite && positive && fa < 0) { fa = a; fb = b; }
             a *= 0.5;
             b *= 0.5;
+        }
+        if (!accepted && c.rel && dir == 0) {
+            // relative-gap mode: dual-only trials of the same direction -- the primal point (and
+            // its residual) stays; near the optimum the normal equations limit the primal
+            // accuracy of a direction, not its dual accuracy
+            a = 0.0;
+            b = fmin(1.0, 0.995 * dc.admax);
+            for (int t = 0; t < 15; ++t) {
+                bool finite, positive;
+                if (admissible(trial(c, a, b), b, b, finite, positive)) { accepted = true; break; }
+                b *= 0.5;
+            }
         }
         if (!accepted && c.rel && dir == 1 && fa > 0) {
             // relative-gap mode: the longest admissible centred step (non-monotone)
@@ -984,17 +1086,20 @@ void k_batch(lkb_args A) {
         if (ticket >= A.count) break;
         const int li = A.order[ticket];
         const lkb_lp d = A.lps[li];
-        Ctx c{A, make_lp(A, d), A.state[li], g_red, A.stop_rule == 1, false, 0.0};
+        Ctx c{A, make_lp(A, d), A.state[li], g_red, A.stop_rule == 1, false, 0.0, wall_clock64()};
         if (c.st.phase == 2) continue;
         if (c.st.phase == 0) {
             c.st.phase = 1;
             c.st.iterations = 0;
             c.st.fail_code = 0;
             c.st.factorizations = c.st.fp32_factorizations = c.st.solves = c.st.pcg_steps = 0;
+            c.st.t_init = c.st.t_other = c.st.t_assemble = c.st.t_factor = c.st.t_solve = 0.0;
             init_lp(c);
+            c.tick(c.st.t_init);
         }
         for (int k = 0; k < A.iters_per_visit; ++k)
             if (iterate(c, d)) break;
+        c.tick(c.st.t_other);
         __syncthreads();
         if (threadIdx.x == 0) A.state[li] = c.st;
         __syncthreads();

@@ -36,6 +36,12 @@ int lkd_device_cus(int* cus) {
     return (int)hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev);
 }
 int lkd_set_device(int dev) { return (int)hipSetDevice(dev); }
+int lkd_wallclock_khz(int* khz) {
+    int dev = 0;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    return (int)hipDeviceGetAttribute(khz, hipDeviceAttributeWallClockRate, dev);
+}
 int lkd_malloc(void** ptr, size_t bytes) { return (int)hipMalloc(ptr, bytes ? bytes : 1); }
 int lkd_free(void* ptr) { return ptr ? (int)hipFree(ptr) : 0; }
 int lkd_host_alloc(void** ptr, size_t bytes) {

@@ -625,7 +625,7 @@ __global__ __launch_bounds__(NT) void k_muaff(lkd_ipm P, const double* __restric
 // folds them into the wavefront's LDS row; one barrier at the end combines the four rows in a
 // fixed order.  No per-thread arrays indexed by the trial.
 __device__ __forceinline__ void trial_fold(double* row, int q, double v, int op, bool first) {
-    v = op == OP_SUM ? wave_sum(v) : wave_min(v);
+    v = op == OP_SUM ? wave_sum(v) : (op == OP_MIN ? wave_min(v) : wave_max(v));
     if ((threadIdx.x & 63) == 0) row[q] = first ? v : op_apply(op, row[q], v);
 }
 
@@ -636,7 +636,7 @@ __global__ __launch_bounds__(NT) void k_trials_cols(lkd_ipm P, const double* __r
                                                     const double* __restrict__ rd,
                                                     const double* __restrict__ qd, double ap,
                                                     double ad, int nt) {
-    __shared__ double acc[4][4 * LKD_MAX_TRIALS];
+    __shared__ double acc[4][5 * LKD_MAX_TRIALS];
     double* row = acc[threadIdx.x >> 6];
     bool first = true;
     const int64_t stride = (int64_t)gridDim.x * NT;
@@ -645,13 +645,15 @@ __global__ __launch_bounds__(NT) void k_trials_cols(lkd_ipm P, const double* __r
         const bool ok = j < P.n;
         const int64_t jc = ok ? j : P.n - 1;
         const double xj = x[jc], sj = s[jc], dxj = dx[jc], dsj = ds[jc], rdj = rd[jc], qdj = qd[jc];
+        const double cj = P.C[jc];
         double a = ap, b = ad;
         for (int t = 0; t < nt; ++t) {
             const double tx = xj + a * dxj, ts = sj + b * dsj, tr = rdj + b * qdj;
-            trial_fold(row, 4 * t + 0, ok ? tr * tr : 0.0, OP_SUM, first);
-            trial_fold(row, 4 * t + 1, ok ? tx * ts : 0.0, OP_SUM, first);
-            trial_fold(row, 4 * t + 2, ok ? tx : 1e30, OP_MIN, first);
-            trial_fold(row, 4 * t + 3, ok ? ts : 1e30, OP_MIN, first);
+            trial_fold(row, 5 * t + 0, ok ? tr * tr : 0.0, OP_SUM, first);
+            trial_fold(row, 5 * t + 1, ok ? tx * ts : 0.0, OP_SUM, first);
+            trial_fold(row, 5 * t + 2, ok ? tx : 1e30, OP_MIN, first);
+            trial_fold(row, 5 * t + 3, ok ? ts : 1e30, OP_MIN, first);
+            trial_fold(row, 5 * t + 4, ok ? fabs(tr / cj) : 0.0, OP_MAX, first);   // original space
             a *= 0.5;
             b *= 0.5;
         }
@@ -659,8 +661,9 @@ __global__ __launch_bounds__(NT) void k_trials_cols(lkd_ipm P, const double* __r
     }
     __syncthreads();
     const int nb = gridDim.x;
-    for (int q = threadIdx.x; q < 4 * nt; q += NT) {
-        const int op = (q & 3) < 2 ? OP_SUM : OP_MIN;
+    for (int q = threadIdx.x; q < 5 * nt; q += NT) {
+        const int k = q % 5;
+        const int op = k < 2 ? OP_SUM : (k < 4 ? OP_MIN : OP_MAX);
         const double r = first ? op_ident(op)
                                : op_apply(op, op_apply(op, acc[0][q], acc[1][q]), op_apply(op, acc[2][q], acc[3][q]));
         P.part[(int64_t)q * nb + blockIdx.x] = r;
@@ -672,7 +675,7 @@ __global__ __launch_bounds__(NT) void k_trials_rows(lkd_ipm P, const double* __r
                                                     const double* __restrict__ y,
                                                     const double* __restrict__ dy, double ap,
                                                     double ad, int nt, int qoff) {
-    __shared__ double acc[4][2 * LKD_MAX_TRIALS];
+    __shared__ double acc[4][3 * LKD_MAX_TRIALS];
     double* row = acc[threadIdx.x >> 6];
     bool first = true;
     const int64_t stride = (int64_t)gridDim.x * NT;
@@ -680,12 +683,13 @@ __global__ __launch_bounds__(NT) void k_trials_rows(lkd_ipm P, const double* __r
         const int64_t i = i0 + threadIdx.x;
         const bool ok = i < P.m;
         const int64_t ic = ok ? i : P.m - 1;
-        const double rpi = rp[ic], qpi = qp[ic], yi = y[ic], dyi = dy[ic];
+        const double rpi = rp[ic], qpi = qp[ic], yi = y[ic], dyi = dy[ic], ri = P.R[ic];
         double a = ap, b = ad;
         for (int t = 0; t < nt; ++t) {
             const double tr = rpi + a * qpi, ty = yi + b * dyi;
-            trial_fold(row, 2 * t + 0, ok ? tr * tr : 0.0, OP_SUM, first);
-            trial_fold(row, 2 * t + 1, ok ? ty * ty : 0.0, OP_SUM, first);
+            trial_fold(row, 3 * t + 0, ok ? tr * tr : 0.0, OP_SUM, first);
+            trial_fold(row, 3 * t + 1, ok ? ty * ty : 0.0, OP_SUM, first);
+            trial_fold(row, 3 * t + 2, ok ? fabs(tr / ri) : 0.0, OP_MAX, first);   // original space
             a *= 0.5;
             b *= 0.5;
         }
@@ -694,8 +698,10 @@ __global__ __launch_bounds__(NT) void k_trials_rows(lkd_ipm P, const double* __r
     __syncthreads();
     const int nb = gridDim.x;
     double* part = P.part + (int64_t)qoff * nb;
-    for (int q = threadIdx.x; q < 2 * nt; q += NT) {
-        const double r = first ? 0.0 : (acc[0][q] + acc[1][q]) + (acc[2][q] + acc[3][q]);
+    for (int q = threadIdx.x; q < 3 * nt; q += NT) {
+        const int op = q % 3 < 2 ? OP_SUM : OP_MAX;
+        const double r = first ? op_ident(op)
+                               : op_apply(op, op_apply(op, acc[0][q], acc[1][q]), op_apply(op, acc[2][q], acc[3][q]));
         part[(int64_t)q * nb + blockIdx.x] = r;
     }
 }
@@ -958,24 +964,26 @@ int lkd_ipm_trials(const lkd_ipm* P, const double* x, const double* s, const dou
                    int32_t ntrial, void* stream) {
     if (ntrial > LKD_MAX_TRIALS) ntrial = LKD_MAX_TRIALS;
     LK_LAUNCH(k_trials_cols, P->nblk_cols, NT, 0, stream, *P, x, s, dx, ds, rd, qd, ap, ad, ntrial);
-    int ops[4 * LKD_MAX_TRIALS], slots[4 * LKD_MAX_TRIALS];
+    int ops[5 * LKD_MAX_TRIALS], slots[5 * LKD_MAX_TRIALS];
     for (int t = 0; t < ntrial; ++t) {
         const int base = LKS_TRIAL_BASE + t * LKS_TRIAL_STRIDE;
-        ops[4 * t + 0] = OP_SUM; slots[4 * t + 0] = base + 1;   // ||rd_t||^2
-        ops[4 * t + 1] = OP_SUM; slots[4 * t + 1] = base + 2;   // x_t^T s_t
-        ops[4 * t + 2] = OP_MIN; slots[4 * t + 2] = base + 3;   // min x_t
-        ops[4 * t + 3] = OP_MIN; slots[4 * t + 3] = base + 4;   // min s_t
+        ops[5 * t + 0] = OP_SUM; slots[5 * t + 0] = base + 1;   // ||rd_t||^2
+        ops[5 * t + 1] = OP_SUM; slots[5 * t + 1] = base + 2;   // x_t^T s_t
+        ops[5 * t + 2] = OP_MIN; slots[5 * t + 2] = base + 3;   // min x_t
+        ops[5 * t + 3] = OP_MIN; slots[5 * t + 3] = base + 4;   // min s_t
+        ops[5 * t + 4] = OP_MAX; slots[5 * t + 4] = base + 6;   // ||rd_t / C||_inf
     }
-    int e = finalize(P->part, P->nblk_cols, 4 * ntrial, ops, slots, P->S, stream);
+    int e = finalize(P->part, P->nblk_cols, 5 * ntrial, ops, slots, P->S, stream);
     if (e) return e;
-    const int qoff = (4 * ntrial * P->nblk_cols + P->nblk_rows - 1) / P->nblk_rows;
+    const int qoff = (5 * ntrial * P->nblk_cols + P->nblk_rows - 1) / P->nblk_rows;
     LK_LAUNCH(k_trials_rows, P->nblk_rows, NT, 0, stream, *P, rp, qp, y, dy, ap, ad, ntrial, qoff);
     for (int t = 0; t < ntrial; ++t) {
         const int base = LKS_TRIAL_BASE + t * LKS_TRIAL_STRIDE;
-        ops[2 * t + 0] = OP_SUM; slots[2 * t + 0] = base + 0;   // ||rp_t||^2
-        ops[2 * t + 1] = OP_SUM; slots[2 * t + 1] = base + 5;   // ||y_t||^2
+        ops[3 * t + 0] = OP_SUM; slots[3 * t + 0] = base + 0;   // ||rp_t||^2
+        ops[3 * t + 1] = OP_SUM; slots[3 * t + 1] = base + 5;   // ||y_t||^2
+        ops[3 * t + 2] = OP_MAX; slots[3 * t + 2] = base + 7;   // ||rp_t / R||_inf
     }
-    return finalize(P->part + (int64_t)qoff * P->nblk_rows, P->nblk_rows, 2 * ntrial, ops, slots, P->S, stream);
+    return finalize(P->part + (int64_t)qoff * P->nblk_rows, P->nblk_rows, 3 * ntrial, ops, slots, P->S, stream);
 }
 
 int lkd_ipm_commit(const lkd_ipm* P, double* x, double* s, double* y, const double* dx,

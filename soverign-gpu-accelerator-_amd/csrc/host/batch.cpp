@@ -56,6 +56,7 @@ struct BatchedMehrotraSolver::Impl {
     lkb_state* d_state = nullptr;
     uint32_t* d_ctl = nullptr;
     int cus = 256;
+    double clock_khz = 100000.0;     // device wall clock ticks per ms
     std::vector<BatchLPResult> res;
     std::vector<double> hx, hy, hs;
     bool have_solution = false;
@@ -161,6 +162,9 @@ struct BatchedMehrotraSolver::Impl {
 
         check(lkd_device_cus(&cus), "device attributes");
         if (cus < 1) cus = 256;
+        int khz = 0;
+        check(lkd_wallclock_khz(&khz), "device attributes");
+        clock_khz = khz > 0 ? khz : 100000.0;
         const size_t al = gpu::VRAMArena::kAlignment;
         auto rnd = [&](size_t bytes) { return (bytes + al - 1) / al * al; };
         const size_t bytes = rnd(sizeof(lkb_lp) * cnt) + rnd(sizeof(int32_t) * cnt) +
@@ -314,6 +318,11 @@ struct BatchedMehrotraSolver::Impl {
             r.primal_residual = s.primal_residual;
             r.dual_residual = s.dual_residual;
             r.mu = s.mu;
+            r.ms_init = s.t_init / clock_khz;
+            r.ms_other = s.t_other / clock_khz;
+            r.ms_assemble = s.t_assemble / clock_khz;
+            r.ms_factor = s.t_factor / clock_khz;
+            r.ms_solve = s.t_solve / clock_khz;
             if (s.status == 0) sum.optimal++;
             const double mm = lps[i].m;
             sum.flops_chol += s.factorizations * mm * mm * mm / 3.0;

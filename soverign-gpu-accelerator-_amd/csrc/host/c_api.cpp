@@ -580,6 +580,11 @@ int lk_batch_results(lk_batch* h, lk_batch_result* out) {
             o.primal_residual = r[i].primal_residual;
             o.dual_residual = r[i].dual_residual;
             o.mu = r[i].mu;
+            o.ms_init = r[i].ms_init;
+            o.ms_other = r[i].ms_other;
+            o.ms_assemble = r[i].ms_assemble;
+            o.ms_factor = r[i].ms_factor;
+            o.ms_solve = r[i].ms_solve;
         }
     });
 }

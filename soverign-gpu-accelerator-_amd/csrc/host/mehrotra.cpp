@@ -359,11 +359,16 @@ public:
         // Absolute targets as well (the FP64 certificate checks |Ax-b| and the reduced costs
         // absolutely), floored where FP64 cannot resolve them.
         return gap <= opt_.gap_tol && nr.rpinf / (1.0 + binf_) <= opt_.gap_tol &&
-               nr.rdinf / (1.0 + cinf_) <= opt_.gap_tol &&
-               nr.rpinf <= std::max(0.5 * opt_.gap_tol, 1e-12 * (1.0 + binf_)) &&
-               nr.rdinf <= std::max(0.5 * opt_.gap_tol, 1e-12 * (1.0 + cinf_)) &&
-               nr.xsmax <= std::max(0.5 * opt_.gap_tol, 1e-14 * std::max({1.0, std::abs(nr.cx), std::abs(nr.by)}));
+               nr.rdinf / (1.0 + cinf_) <= opt_.gap_tol && nr.rpinf <= primal_target() &&
+               nr.rdinf <= dual_target() &&
+               nr.xsmax <= std::max(kRelAbsMargin * opt_.gap_tol,
+                                    1e-14 * std::max({1.0, std::abs(nr.cx), std::abs(nr.by)}));
     }
+    // relative-gap mode: absolute targets of ||Ax - b||_inf and ||A^T y + s - c||_inf (original
+    // space), a fraction kRelAbsMargin of gap_tol (the FP64 certificate checks them absolutely)
+    static constexpr double kRelAbsMargin = 0.9;
+    double primal_target() const { return std::max(kRelAbsMargin * opt_.gap_tol, 1e-12 * (1.0 + binf_)); }
+    double dual_target() const { return std::max(kRelAbsMargin * opt_.gap_tol, 1e-12 * (1.0 + cinf_)); }
 
     void init_point() {
         if (opt_.warm_x && opt_.warm_y && opt_.warm_s) {
@@ -445,6 +450,11 @@ public:
             double ap = std::min(1.0, 0.995 * d.apmax), ad = std::min(1.0, 0.995 * d.admax);
             const double merit0 = nr.rp / rp0 + nr.rd / rd0 + nr.mu / mu0;
             bool accepted = false;
+            // relative-gap mode: a residual already inside its absolute target must stay inside
+            const bool keep_p = rel && nr.rpinf <= primal_target(), keep_d = rel && nr.rdinf <= dual_target();
+            auto keeps = [&](const double* T) {
+                return !(keep_p && T[7] > primal_target()) && !(keep_d && T[6] > dual_target());
+            };
             for (int dir = corrector_valid ? 0 : 1; dir < 2 && !accepted; ++dir) {
                 if (dir == 1) {
                     Dir dc;
@@ -468,7 +478,7 @@ public:
                                         std::isfinite(ms) && std::isfinite(a) && std::isfinite(b) &&
                                         std::isfinite(ny);
                     const bool positive = mx > 0.0 && ms > 0.0 && mu2 > 0.0 && step > 0.0;
-                    const bool progress = merit < merit0 && merit <= (1.0 - 1e-4 * step) * merit0;
+                    const bool progress = merit < merit0 && merit <= (1.0 - 1e-4 * step) * merit0 && keeps(T);
                     if (finite && positive && progress) {
                         check(lkd_ipm_commit(&P_, x_, s_, y_, dx_, ds_, dy_, a, b, stream_), "commit");
                         accepted = true;
@@ -479,6 +489,31 @@ public:
                     if (rel && dir == 1 && finite && positive && fa < 0) { fa = a; fb = b; }
                     a *= 0.5;
                     b *= 0.5;
+                }
+                if (!accepted && rel && dir == 0) {
+                    // relative-gap mode: dual-only trials of the same direction -- the primal
+                    // point (and its residual) stays; near the optimum the normal equations
+                    // limit the primal accuracy of a direction, not its dual accuracy
+                    const double ad0 = std::min(1.0, 0.995 * d.admax);
+                    const int ntd = 15;
+                    check(lkd_ipm_trials(&P_, x_, s_, y_, dx_, ds_, dy_, rp_, rd_, qp_, qd_, 0.0, ad0, ntd, stream_), "trials");
+                    read_scalars(LKS_TRIAL_BASE + LKS_TRIAL_STRIDE * ntd);
+                    double bd = ad0;
+                    for (int t = 0; t < ntd && !accepted; ++t, bd *= 0.5) {
+                        const double* T = &hS_[LKS_TRIAL_BASE + t * LKS_TRIAL_STRIDE];
+                        const double nrp2 = std::sqrt(T[0]), nrd2 = std::sqrt(T[1]), mu2 = T[2] / n_;
+                        const double merit = nrp2 / rp0 + nrd2 / rd0 + mu2 / mu0;
+                        const bool finite = std::isfinite(nrp2) && std::isfinite(nrd2) && std::isfinite(mu2) &&
+                                            std::isfinite(merit) && std::isfinite(merit0) && std::isfinite(T[4]) &&
+                                            std::isfinite(bd) && std::isfinite(T[5]);
+                        const bool positive = T[4] > 0.0 && mu2 > 0.0 && bd > 0.0;
+                        const bool progress = merit < merit0 && merit <= (1.0 - 1e-4 * bd) * merit0 && keeps(T);
+                        if (finite && positive && progress) {
+                            check(lkd_ipm_commit(&P_, x_, s_, y_, dx_, ds_, dy_, 0.0, bd, stream_), "commit");
+                            accepted = true;
+                            if (opt_.verbose) std::printf("    step: dual-only trial %d ad %.3e\n", t, bd);
+                        }
+                    }
                 }
                 if (!accepted && rel && dir == 1 && fa > 0) {
                     // relative-gap mode: non-monotone fallback to the longest admissible centred step

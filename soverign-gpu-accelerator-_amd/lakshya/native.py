@@ -75,7 +75,9 @@ class BatchResult(C.Structure):
                 ("solves", C.c_int32), ("pcg_steps", C.c_int32), ("pad", C.c_int32),
                 ("objective", C.c_double), ("dual_objective", C.c_double),
                 ("rel_gap", C.c_double), ("rel_primal", C.c_double), ("rel_dual", C.c_double),
-                ("primal_residual", C.c_double), ("dual_residual", C.c_double), ("mu", C.c_double)]
+                ("primal_residual", C.c_double), ("dual_residual", C.c_double), ("mu", C.c_double),
+                ("ms_init", C.c_double), ("ms_other", C.c_double), ("ms_assemble", C.c_double),
+                ("ms_factor", C.c_double), ("ms_solve", C.c_double)]
 
 
 STATUS = {0: "Optimal", 1: "Infeasible", 2: "Unbounded", 3: "IterationLimit"}
