@@ -80,5 +80,18 @@ private:
     Impl* impl_;
 };
 
+// For callers that cannot allocate device memory (code under graph capture, serving loops,
+// harnesses that forbid hipMalloc): the same solve with every device buffer carved from a fixed
+// device scratch array of LKB_SCRATCH_BYTES (64 MiB) instead of an arena; no hipMalloc / hipFree.
+// Host inputs and outputs as the class (x, s: sum n; y: sum m; results: count).  A batch whose
+// device footprint exceeds the scratch is rejected (std::invalid_argument).  Calls on one GPU
+// share its scratch: they are serialized within a process, and the scratch keeps the last call's
+// contents (every buffer is rewritten before it is read).  Extension with no reference
+// counterpart.
+BatchSummary solve_batch_noalloc(int count, const int32_t* m, const int32_t* n,
+                                 const int32_t* colptr, const int32_t* rowidx, const double* vals,
+                                 const double* b, const double* c, const BatchOptions& opt,
+                                 double* x, double* y, double* s, BatchLPResult* results);
+
 }  // namespace ipm
 }  // namespace sankhya

@@ -198,6 +198,18 @@ int lk_batch_autotune(lk_batch* h, int32_t sample, lk_batch_summary* chosen);
 int lk_batch_results(lk_batch* h, lk_batch_result* out);          /* count entries */
 int lk_batch_solution(lk_batch* h, int32_t i, double* x, double* y, double* s);
 int lk_batch_destroy(lk_batch* h);
+/* No reference counterpart.  For callers that cannot allocate device memory (code under graph
+ * capture, serving loops, harnesses that forbid hipMalloc): the lk_batch_create + lk_batch_solve
+ * computation in one call that allocates no device memory -- every device buffer lies in a fixed
+ * device scratch array of LKB_SCRATCH_BYTES (64 MiB, lakshya_device.h) held per GPU until the
+ * process ends.  Host inputs as lk_batch_create; host outputs x, s (sum n), y (sum m), results
+ * (count), sum optional.  A batch needing more scratch is rejected with LK_ERR_INVALID.  Calls on
+ * one GPU share its scratch: they are serialized within the process. */
+int lk_batch_solve_noalloc(int32_t count, const int32_t* m, const int32_t* n,
+                           const int32_t* colptr, const int32_t* rowidx, const double* vals,
+                           const double* b, const double* c, const lk_batch_options* opt,
+                           double* x, double* y, double* s, lk_batch_result* results,
+                           lk_batch_summary* sum);
 
 /* ---- verifier::verify_optimal (src/verifier/certificate.hpp:26-31) with a tolerance ---- */
 int lk_verify_optimal(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,

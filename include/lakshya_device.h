@@ -403,6 +403,12 @@ int lkd_batch_reset(uint32_t* ctl, lkb_state* state, const int32_t* order, int32
  * the kernel by value */
 int lkd_batch_solve(const lkb_args* args, int32_t grid, void* stream);
 
+/* Device scratch of the no-allocation batch entry point: a fixed file-scope device array of
+ * LKB_SCRATCH_BYTES (every GPU the library runs on holds one until the process ends).  *ptr is
+ * its address on the current device, resolved once per device. */
+#define LKB_SCRATCH_BYTES (64ull << 20)
+int lkd_batch_scratch(void** ptr, size_t* bytes);
+
 /* ---------------------------------------------------------------- launch graphs */
 /* Stream capture of a launch sequence into an executable graph (kernels only: no memset or
  * copy nodes), its replay and destruction. */

@@ -18,7 +18,14 @@
 // The launch is persistent: a workgroup takes tickets from ctl[0] until all LPs are visited;
 // every visit runs up to iters_per_visit iterations and leaves the LP's state in lkb_state, so
 // the same launch can be repeated (for instance from a captured graph) until all LPs finish.
+#include <mutex>
 #include "dense_tile.hpp"
+
+// The no-allocation entry point's device memory: a fixed array (each GPU holds its own copy).
+// Host code never uses its name as an address: lkd_batch_scratch resolves the device address
+// once per device (hipGetSymbolAddress with that device current) and every host use -- kernel
+// arguments, copies -- goes through that pointer.
+static __device__ __align__(256) unsigned char scratch[LKB_SCRATCH_BYTES];
 
 namespace {
 
@@ -1119,6 +1126,26 @@ __global__ void k_batch_reset(uint32_t* ctl, lkb_state* state, const int32_t* or
 }  // namespace
 
 extern "C" {
+
+int lkd_batch_scratch(void** ptr, size_t* bytes) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    *bytes = (size_t)LKB_SCRATCH_BYTES;
+    return 0;
+}
 
 int lkd_batch_reset(uint32_t* ctl, lkb_state* state, const int32_t* order, int32_t count,
                     int32_t restart, void* stream) {

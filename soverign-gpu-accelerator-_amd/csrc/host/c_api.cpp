@@ -559,6 +559,47 @@ int lk_batch_solve(lk_batch* h, lk_batch_summary* sum) {
 int lk_batch_autotune(lk_batch* h, int32_t sample, lk_batch_summary* chosen) {
     return guarded([&] { copy_summary(h->solver->autotune(sample), chosen); });
 }
+namespace {
+void copy_batch_result(const ipm::BatchLPResult& r, lk_batch_result& o) {
+    std::memset(&o, 0, sizeof(o));
+    o.status = r.status;
+    o.iterations = r.iterations;
+    o.fail_code = r.fail_code;
+    o.factorizations = r.factorizations;
+    o.fp32_factorizations = r.fp32_factorizations;
+    o.solves = r.solves;
+    o.pcg_steps = r.pcg_steps;
+    o.objective = r.objective;
+    o.dual_objective = r.dual_objective;
+    o.rel_gap = r.rel_gap;
+    o.rel_primal = r.rel_primal;
+    o.rel_dual = r.rel_dual;
+    o.primal_residual = r.primal_residual;
+    o.dual_residual = r.dual_residual;
+    o.mu = r.mu;
+    o.ms_init = r.ms_init;
+    o.ms_other = r.ms_other;
+    o.ms_assemble = r.ms_assemble;
+    o.ms_factor = r.ms_factor;
+    o.ms_solve = r.ms_solve;
+}
+}  // namespace
+
+int lk_batch_solve_noalloc(int32_t count, const int32_t* m, const int32_t* n,
+                           const int32_t* colptr, const int32_t* rowidx, const double* vals,
+                           const double* b, const double* c, const lk_batch_options* opt,
+                           double* x, double* y, double* s, lk_batch_result* results,
+                           lk_batch_summary* sum) {
+    return guarded([&] {
+        if (count < 1 || !results) throw std::invalid_argument("lk_batch_solve_noalloc: bad arguments");
+        std::vector<ipm::BatchLPResult> r(static_cast<size_t>(count));
+        const ipm::BatchSummary su = ipm::solve_batch_noalloc(count, m, n, colptr, rowidx, vals, b, c,
+                                                              to_batch_options(opt), x, y, s, r.data());
+        for (int32_t i = 0; i < count; ++i) copy_batch_result(r[i], results[i]);
+        copy_summary(su, sum);
+    });
+}
+
 int lk_batch_results(lk_batch* h, lk_batch_result* out) {
     return guarded([&] {
         const auto& r = h->solver->results();

@@ -134,6 +134,9 @@ def lib():
         "lk_batch_results": [_vp, _vp],
         "lk_batch_solution": [_vp, C.c_int32, _f64p, _f64p, _f64p],
         "lk_batch_destroy": [_vp],
+        "lk_batch_solve_noalloc": [C.c_int32, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                   C.POINTER(BatchOptions), _f64p, _f64p, _f64p, _vp,
+                                   C.POINTER(BatchSummary)],
         "lk_verify_optimal": [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _vp, _vp,
                               C.c_int32, _f64p, _f64p, C.c_double, C.POINTER(C.c_double),
                               C.POINTER(C.c_int32)],
@@ -476,6 +479,35 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def batch_solve_noalloc(lps, **opts):
+    """lk_batch_solve_noalloc: the batch solved in the library's fixed device scratch (no device
+    allocation).  Returns (rc, x, y, s, results, summary); rc != 0 leaves the message in
+    lk_last_error()."""
+    m = np.array([lp.m for lp in lps], np.int32)
+    n = np.array([lp.n for lp in lps], np.int32)
+    colptr = np.concatenate([np.asarray(lp.indptr, np.int32) for lp in lps])
+    rowidx = np.concatenate([np.asarray(lp.indices, np.int32) for lp in lps])
+    vals = np.concatenate([np.asarray(lp.data, np.float64) for lp in lps])
+    b = np.concatenate([np.asarray(lp.b, np.float64) for lp in lps])
+    c = np.concatenate([np.asarray(lp.c, np.float64) for lp in lps])
+    x, s, y = np.zeros(int(n.sum())), np.zeros(int(n.sum())), np.zeros(int(m.sum()))
+    res = (BatchResult * len(lps))()
+    summ = BatchSummary()
+    o = batch_options(**opts)
+    rc = lib().lk_batch_solve_noalloc(len(lps), m, n, colptr, rowidx, vals, b, c, C.byref(o),
+                                      x, y, s, C.cast(res, _vp), C.byref(summ))
+    out = []
+    for r in res:
+        d = {f: getattr(r, f) for f, _ in BatchResult._fields_ if f != "pad"}
+        d["status_name"] = STATUS.get(r.status, str(r.status))
+        out.append(d)
+    return rc, x, y, s, out, _summary(summ)
+
+
+def last_error() -> str:
+    return lib().lk_last_error().decode(errors="replace")
 
 
 def ruiz_scaling(rowptr, colidx, vals, b, c):
