@@ -32,6 +32,7 @@ struct BatchOptions {
     int iters_per_launch = 0;       // IPM iterations per LP visit (0: autotuned or all)
     int use_graph = 1;              // replay the launch sequence from a captured hipGraph
     int order = 0;                  // 0 largest m first, 1 input order
+    double warm_floor = 1e-3;       // lower clamp of warm-start x and s (scaled space)
 };
 
 struct BatchLPResult {
@@ -70,6 +71,10 @@ public:
     // fastest; returns the chosen wg_per_cu and iters_per_launch in the summary.
     BatchSummary autotune(int sample);
     const std::vector<BatchLPResult>& results() const;
+    // Original-space start points for the next solves (x, s: sum n; y: sum m, the layout of
+    // solution()); LP i uses its point when flags[i] != 0 (flags null: all).  x0 null: every LP
+    // back to the standard start.
+    void set_warm_start(const double* x0, const double* y0, const double* s0, const int32_t* flags);
     // original-space solution of LP i (after solve())
     void solution(int i, double* x, double* y, double* s) const;
     int count() const;

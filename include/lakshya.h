@@ -175,6 +175,7 @@ typedef struct {
     int32_t iters_per_launch;   /* IPM iterations per LP visit (0: all, or the autotuned value) */
     int32_t use_graph;          /* replay the launch sequence from a captured hipGraph */
     int32_t order;              /* 0 largest m first, 1 input order */
+    double warm_floor;          /* lower clamp of warm-start x and s (scaled space) */
 } lk_batch_options;
 typedef struct {
     int32_t count, optimal, launches, graph_replays;
@@ -197,6 +198,11 @@ int lk_batch_solve(lk_batch* h, lk_batch_summary* sum);
 int lk_batch_autotune(lk_batch* h, int32_t sample, lk_batch_summary* chosen);
 int lk_batch_results(lk_batch* h, lk_batch_result* out);          /* count entries */
 int lk_batch_solution(lk_batch* h, int32_t i, double* x, double* y, double* s);
+/* Warm start for the next solves: original-space x, s (sum n) and y (sum m) in the layout of
+ * the inputs; LP i uses it when flags[i] != 0 (flags NULL: every LP); x NULL clears it.
+ * Extension (the single-LP counterpart is lk_ipm_run's warm_* arguments). */
+int lk_batch_set_warm_start(lk_batch* h, const double* x, const double* y, const double* s,
+                            const int32_t* flags);
 int lk_batch_destroy(lk_batch* h);
 /* No reference counterpart.  For callers that cannot allocate device memory (code under graph
  * capture, serving loops, harnesses that forbid hipMalloc): the lk_batch_create + lk_batch_solve
@@ -210,6 +216,27 @@ int lk_batch_solve_noalloc(int32_t count, const int32_t* m, const int32_t* n,
                            const double* b, const double* c, const lk_batch_options* opt,
                            double* x, double* y, double* s, lk_batch_result* results,
                            lk_batch_summary* sum);
+
+/* ---- warm-start network: learn::WarmStartMLP (include/learn/mlp.hpp) ----
+ * Extension with no reference counterpart (the start point it replaces is x = s = 1, y = 0,
+ * src/ipm/mehrotra_device.cu:93-102).  Three-layer perceptron d_in -> h1 -> h2 -> d_out with
+ * ReLU, BF16 MFMA products with FP32 accumulation, FP32 master weights and Adam.  Host arrays,
+ * row-major; parameters concatenated W1 (h1 x d_in), b1, W2 (h2 x h1), b2, W3 (d_out x h2), b3. */
+typedef struct lk_mlp lk_mlp;
+int lk_mlp_create(int32_t d_in, int32_t h1, int32_t h2, int32_t d_out, int32_t max_batch,
+                  uint64_t seed, lk_mlp** out);
+int lk_mlp_param_count(lk_mlp* h, int64_t* count);
+int lk_mlp_set_params(lk_mlp* h, const float* params);
+int lk_mlp_get_params(lk_mlp* h, float* params);
+int lk_mlp_forward(lk_mlp* h, const float* u, int32_t batch, float* y);
+/* gradient of the mean squared error (1 / (batch d_out)) sum (y - t)^2; *loss = that error */
+int lk_mlp_gradients(lk_mlp* h, const float* u, const float* t, int32_t batch, float* grad,
+                     double* loss);
+int lk_mlp_set_dataset(lk_mlp* h, const float* u, const float* t, int32_t n);
+/* Adam over `epochs` shuffled passes of full mini-batches; epoch_loss[epochs] optional */
+int lk_mlp_train(lk_mlp* h, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                 double* epoch_loss);
+int lk_mlp_destroy(lk_mlp* h);
 
 /* ---- verifier::verify_optimal (src/verifier/certificate.hpp:26-31) with a tolerance ---- */
 int lk_verify_optimal(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,

@@ -393,6 +393,12 @@ typedef struct {
     lkb_state* state;
     double *x_out, *y_out, *s_out;
     uint32_t* ctl;                 /* [0] ticket counter, [1] LPs finished */
+    /* optional warm start (original space, the layout of x_out / y_out / s_out): LP i starts
+     * from it when warm_flag[i] != 0 -- x = max(x0 / C, warm_floor), s = max(s0 C, warm_floor),
+     * y = y0 / R after scaling -- else from x = s = 1, y = 0 */
+    const double *x0, *y0, *s0;
+    const int32_t* warm_flag;
+    double warm_floor;
 } lkb_args;
 
 /* ctl[0] = 0; with `restart` also ctl[1] = 0 and the phase of LPs order[0 .. count) back to 0
@@ -416,6 +422,29 @@ int lkd_graph_begin(void* stream);
 int lkd_graph_end(void* stream, void** exec);
 int lkd_graph_launch(void* exec, void* stream);
 int lkd_graph_destroy(void* exec);
+
+/* ---------------------------------------------------------------- warm-start network */
+/* C(m, n) = alpha sum_k A(m, k) B(n, k) + beta C(m, n) (+ bias[n]; then 0 where
+ * mask[m ldc + n] <= 0), A(m, k) = A[m sam + k sak], B(n, k) = B[n sbn + k sbk], C(m, n) =
+ * C[m ldc + n]; relu_a / relu_b read that operand as max(., 0).  Operands are rounded to BF16
+ * and multiplied on MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation. */
+int lkd_mlp_gemm(int M, int N, int K, const float* A, int64_t sam, int64_t sak, const float* B,
+                 int64_t sbn, int64_t sbk, float* C, int64_t ldc, float alpha, float beta,
+                 const float* bias, const float* mask, int relu_a, int relu_b, void* stream);
+/* out[n] = beta out[n] + sum_m G[m ld + n] (rows in order) */
+int lkd_mlp_colsum(const float* G, int M, int N, int64_t ld, float beta, float* out, void* stream);
+/* g = scale (Y - T) over `count` entries; *loss = sum (Y - T)^2 (block partials in `part`,
+ * at most nparts, added in order) */
+int lkd_mlp_mse(const float* Y, const float* T, int64_t count, float scale, float* g, float* part,
+                int32_t nparts, float* loss, void* stream);
+/* dst[r, :] = src[idx[r], :] (rows of `cols` floats) */
+int lkd_mlp_gather(const float* src, const int32_t* idx, int rows, int cols, float* dst, void* stream);
+/* Adam step `step` (>= 1) on n FP32 weights w with gradient g and moments m, v */
+int lkd_mlp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                 float b2, float eps, int64_t step, void* stream);
+/* one v_mfma_f32_16x16x16_bf16 on 64 lanes: lane l supplies a[4l..4l+3], b[4l..4l+3] (rounded
+ * to BF16); out[4 l + e] = element e of its result */
+int lkd_probe_mfma_bf16(const float* a, const float* b, float* out, void* stream);
 
 /* ---------------------------------------------------------------- probes (tests) */
 /* One 16x16x4 MFMA (FP64 or FP32) on 64 lanes: lane l supplies a[l], b[l]; out[l*4+e] = D[e]. */

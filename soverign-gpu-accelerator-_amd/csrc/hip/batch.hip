@@ -843,8 +843,9 @@ __device__ __forceinline__ bool dir_ok(const Dir& d) {
 
 
 // Scaling and start point (mehrotra_device.cu:93-102,286-323): the working CSR and CSC values
-// from the original ones, 10 Jacobi Ruiz passes, x = s = 1, y = 0.
-__device__ void init_lp(Ctx& c) {
+// from the original ones, 10 Jacobi Ruiz passes, x = s = 1, y = 0 -- or, for an LP with a warm
+// start, the given original-space point scaled (x / C, y / R, s C) with x and s floored.
+__device__ void init_lp(Ctx& c, const lkb_lp& d, bool warm) {
     const Lp& L = c.L;
     double *b = L.M(WB), *R = L.M(WR), *fr = L.M(WTM), *y = L.M(WY);
     double *cc = L.N(VC), *Cs = L.N(VCS), *fc = L.N(VTN), *x = L.N(VX), *s = L.N(VS);
@@ -890,6 +891,15 @@ __device__ void init_lp(Ctx& c) {
         // the same product fr_i fc_j scales the CSR and the CSC copy of every entry
         seg_for<GR>(L.m, L.rp, [&](int i, int p) { L.av[p] *= fr[i] * fc[L.ci[p]]; });
         seg_for<GC>(L.n, L.cp, [&](int j, int q) { L.cv[q] *= fr[L.ri[q]] * fc[j]; });
+        __syncthreads();
+    }
+    if (warm) {
+        const double fl = c.A.warm_floor;
+        for (int j = threadIdx.x; j < L.n; j += BT) {
+            x[j] = fmax(c.A.x0[d.on + j] / Cs[j], fl);
+            s[j] = fmax(c.A.s0[d.on + j] * Cs[j], fl);
+        }
+        for (int i = threadIdx.x; i < L.m; i += BT) y[i] = c.A.y0[d.om + i] / R[i];
         __syncthreads();
     }
 }
@@ -1101,7 +1111,7 @@ void k_batch(lkb_args A) {
             c.st.fail_code = 0;
             c.st.factorizations = c.st.fp32_factorizations = c.st.solves = c.st.pcg_steps = 0;
             c.st.t_init = c.st.t_other = c.st.t_assemble = c.st.t_factor = c.st.t_solve = 0.0;
-            init_lp(c);
+            init_lp(c, d, A.warm_flag && A.warm_flag[li] && A.x0 && A.y0 && A.s0);
             c.tick(c.st.t_init);
         }
         for (int k = 0; k < A.iters_per_visit; ++k)

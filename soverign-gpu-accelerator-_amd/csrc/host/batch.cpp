@@ -65,6 +65,10 @@ struct BatchImage {
                rnd(sizeof(lkb_state) * count) + 2 * rnd(sizeof(double) * n_xn) +
                rnd(sizeof(double) * n_ym) + rnd(64);
     }
+    // the optional warm-start buffers (x0, s0, y0, flags)
+    size_t warm_bytes() const {
+        return 2 * rnd(sizeof(double) * n_xn) + rnd(sizeof(double) * n_ym) + rnd(sizeof(int32_t) * count);
+    }
 };
 
 BatchImage pack_batch(int cnt, const int32_t* m, const int32_t* n, const int32_t* colptr,
@@ -285,6 +289,9 @@ struct BatchedMehrotraSolver::Impl {
     std::vector<BatchLPResult> res;
     std::vector<double> hx, hy, hs;
     bool have_solution = false;
+    // warm start (original space) and per-LP flags, allocated on first use
+    double *d_x0 = nullptr, *d_y0 = nullptr, *d_s0 = nullptr;
+    int32_t* d_wflag = nullptr;
     // cached graph of `graph_len` launches for one (grid, iterations, count) shape
     void* graph = nullptr;
     int graph_grid = 0, graph_iters = 0, graph_len = 0, graph_count = -1;
@@ -297,7 +304,7 @@ struct BatchedMehrotraSolver::Impl {
         img = pack_batch(cnt, m, n, colptr, rowidx, vals, b, c, opt.order);
         cus = device_cus();
         clock_khz = device_clock_khz();
-        arena.reset(new gpu::VRAMArena(img.device_bytes() + (1u << 20)));
+        arena.reset(new gpu::VRAMArena(img.device_bytes() + img.warm_bytes() + (1u << 20)));
         check(lkd_stream_create(&stream), "stream");
         dev = place_batch(img, [&](size_t nb) { return arena->allocate_uninitialized(nb); }, stream);
         check(lkd_stream_sync(stream), "upload");
@@ -318,7 +325,12 @@ struct BatchedMehrotraSolver::Impl {
         sum.wg_per_cu = wg;
         sum.iters_per_launch = iters;
         sum.grid = wg * cus;
-        const lkb_args a = make_args(opt, dev, ord, cnt, iters);
+        lkb_args a = make_args(opt, dev, ord, cnt, iters);
+        a.x0 = d_x0;
+        a.y0 = d_y0;
+        a.s0 = d_s0;
+        a.warm_flag = d_wflag;
+        a.warm_floor = opt.warm_floor;
         const int max_launches = (opt.max_iter + 1 + iters - 1) / iters + 1;
         void *e0 = nullptr, *e1 = nullptr;
         check(lkd_event_create(&e0), "event");
@@ -427,6 +439,28 @@ struct BatchedMehrotraSolver::Impl {
         return best;
     }
 
+    void set_warm(const double* x0, const double* y0, const double* s0, const int32_t* flags) {
+        if (!d_x0) {
+            d_x0 = static_cast<double*>(arena->allocate_uninitialized(sizeof(double) * img.n_xn));
+            d_s0 = static_cast<double*>(arena->allocate_uninitialized(sizeof(double) * img.n_xn));
+            d_y0 = static_cast<double*>(arena->allocate_uninitialized(sizeof(double) * img.n_ym));
+            d_wflag = static_cast<int32_t*>(arena->allocate(sizeof(int32_t) * count));
+        }
+        std::vector<int32_t> f(count, 0);
+        if (flags) std::copy(flags, flags + count, f.begin());
+        else if (x0) std::fill(f.begin(), f.end(), 1);
+        if (x0) {
+            if (!y0 || !s0) throw std::invalid_argument("set_warm_start: x, y and s together");
+            check(lkd_memcpy_htod(d_x0, x0, sizeof(double) * img.n_xn, stream), "warm x");
+            check(lkd_memcpy_htod(d_s0, s0, sizeof(double) * img.n_xn, stream), "warm s");
+            check(lkd_memcpy_htod(d_y0, y0, sizeof(double) * img.n_ym, stream), "warm y");
+        } else {
+            std::fill(f.begin(), f.end(), 0);
+        }
+        check(lkd_memcpy_htod(d_wflag, f.data(), sizeof(int32_t) * count, stream), "warm flags");
+        check(lkd_stream_sync(stream), "warm");
+    }
+
     void fetch_solution() {
         if (have_solution) return;
         hx.resize(static_cast<size_t>(img.n_xn));
@@ -449,6 +483,10 @@ BatchSummary BatchedMehrotraSolver::solve() { return impl_->solve(); }
 BatchSummary BatchedMehrotraSolver::autotune(int sample) { return impl_->autotune(sample); }
 const std::vector<BatchLPResult>& BatchedMehrotraSolver::results() const { return impl_->res; }
 int BatchedMehrotraSolver::count() const { return impl_->count; }
+void BatchedMehrotraSolver::set_warm_start(const double* x0, const double* y0, const double* s0,
+                                           const int32_t* flags) {
+    impl_->set_warm(x0, y0, s0, flags);
+}
 void BatchedMehrotraSolver::solution(int i, double* x, double* y, double* s) const {
     if (i < 0 || i >= impl_->count) throw std::invalid_argument("solution: LP index out of range");
     impl_->fetch_solution();

@@ -14,6 +14,7 @@
 #include "ipm/mehrotra.hpp"
 #include "ipm/symbolic.hpp"
 #include "lakshya.h"
+#include "learn/mlp.hpp"
 #include "lakshya_device.h"
 #include "verifier/certificate.hpp"
 
@@ -175,6 +176,9 @@ struct lk_ipm {
 struct lk_batch {
     std::unique_ptr<ipm::BatchedMehrotraSolver> solver;
 };
+struct lk_mlp {
+    std::unique_ptr<learn::WarmStartMLP> net;
+};
 
 namespace {
 ipm::BatchOptions to_batch_options(const lk_batch_options* o) {
@@ -191,6 +195,7 @@ ipm::BatchOptions to_batch_options(const lk_batch_options* o) {
     r.iters_per_launch = o->iters_per_launch;
     r.use_graph = o->use_graph;
     r.order = o->order;
+    r.warm_floor = o->warm_floor;
     return r;
 }
 void copy_summary(const ipm::BatchSummary& s, lk_batch_summary* o) {
@@ -541,6 +546,7 @@ int lk_batch_default_options(lk_batch_options* o) {
         o->iters_per_launch = d.iters_per_launch;
         o->use_graph = d.use_graph;
         o->order = d.order;
+        o->warm_floor = d.warm_floor;
     });
 }
 int lk_batch_create(int32_t count, const int32_t* m, const int32_t* n, const int32_t* colptr,
@@ -633,6 +639,49 @@ int lk_batch_solution(lk_batch* h, int32_t i, double* x, double* y, double* s) {
     return guarded([&] { h->solver->solution(i, x, y, s); });
 }
 int lk_batch_destroy(lk_batch* h) { return guarded([&] { delete h; }); }
+int lk_batch_set_warm_start(lk_batch* h, const double* x, const double* y, const double* s,
+                            const int32_t* flags) {
+    return guarded([&] { h->solver->set_warm_start(x, y, s, flags); });
+}
+
+int lk_mlp_create(int32_t d_in, int32_t h1, int32_t h2, int32_t d_out, int32_t max_batch,
+                  uint64_t seed, lk_mlp** out) {
+    return guarded([&] {
+        std::unique_ptr<lk_mlp> h(new lk_mlp);
+        h->net.reset(new learn::WarmStartMLP(d_in, h1, h2, d_out, max_batch, seed));
+        *out = h.release();
+    });
+}
+int lk_mlp_param_count(lk_mlp* h, int64_t* count) {
+    return guarded([&] { *count = h->net->param_count(); });
+}
+int lk_mlp_set_params(lk_mlp* h, const float* params) {
+    return guarded([&] { h->net->set_params(params); });
+}
+int lk_mlp_get_params(lk_mlp* h, float* params) {
+    return guarded([&] { h->net->get_params(params); });
+}
+int lk_mlp_forward(lk_mlp* h, const float* u, int32_t batch, float* y) {
+    return guarded([&] { h->net->forward(u, batch, y); });
+}
+int lk_mlp_gradients(lk_mlp* h, const float* u, const float* t, int32_t batch, float* grad,
+                     double* loss) {
+    return guarded([&] {
+        const double l = h->net->gradients(u, t, batch, grad);
+        if (loss) *loss = l;
+    });
+}
+int lk_mlp_set_dataset(lk_mlp* h, const float* u, const float* t, int32_t n) {
+    return guarded([&] { h->net->set_dataset(u, t, n); });
+}
+int lk_mlp_train(lk_mlp* h, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                 double* epoch_loss) {
+    return guarded([&] {
+        const auto l = h->net->train(epochs, batch, lr, seed);
+        if (epoch_loss) std::copy(l.begin(), l.end(), epoch_loss);
+    });
+}
+int lk_mlp_destroy(lk_mlp* h) { return guarded([&] { delete h; }); }
 
 int lk_verify_optimal(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
                       const double* vals, const double* c, const double* b, const double* lb,

@@ -59,7 +59,8 @@ class BatchOptions(C.Structure):
     _fields_ = [("stop_rule", C.c_int32), ("precision", C.c_int32), ("max_iter", C.c_int32),
                 ("mixed_max_refine", C.c_int32), ("tol", C.c_double), ("gap_tol", C.c_double),
                 ("mixed_fp64_spread", C.c_double), ("wg_per_cu", C.c_int32),
-                ("iters_per_launch", C.c_int32), ("use_graph", C.c_int32), ("order", C.c_int32)]
+                ("iters_per_launch", C.c_int32), ("use_graph", C.c_int32), ("order", C.c_int32),
+                ("warm_floor", C.c_double)]
 
 
 class BatchSummary(C.Structure):
@@ -134,9 +135,20 @@ def lib():
         "lk_batch_results": [_vp, _vp],
         "lk_batch_solution": [_vp, C.c_int32, _f64p, _f64p, _f64p],
         "lk_batch_destroy": [_vp],
+        "lk_batch_set_warm_start": [_vp, _vp, _vp, _vp, _vp],
         "lk_batch_solve_noalloc": [C.c_int32, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p,
                                    C.POINTER(BatchOptions), _f64p, _f64p, _f64p, _vp,
                                    C.POINTER(BatchSummary)],
+        "lk_mlp_create": [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                          C.POINTER(_vp)],
+        "lk_mlp_param_count": [_vp, C.POINTER(C.c_int64)],
+        "lk_mlp_set_params": [_vp, _f32p],
+        "lk_mlp_get_params": [_vp, _f32p],
+        "lk_mlp_forward": [_vp, _f32p, C.c_int32, _f32p],
+        "lk_mlp_gradients": [_vp, _f32p, _f32p, C.c_int32, _f32p, C.POINTER(C.c_double)],
+        "lk_mlp_set_dataset": [_vp, _f32p, _f32p, C.c_int32],
+        "lk_mlp_train": [_vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _f64p],
+        "lk_mlp_destroy": [_vp],
         "lk_verify_optimal": [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _vp, _vp,
                               C.c_int32, _f64p, _f64p, C.c_double, C.POINTER(C.c_double),
                               C.POINTER(C.c_int32)],
@@ -464,6 +476,21 @@ class Batch:
             out.append(d)
         return out
 
+    def set_warm_start(self, x=None, y=None, s=None, flags=None):
+        """Original-space start points (concatenated like the inputs); None clears them."""
+        if x is None:
+            _ck(lib().lk_batch_set_warm_start(self.h, None, None, None, None))
+            return
+        x, y, s = _f64(x), _f64(y), _f64(s)
+        f = None if flags is None else _i32(flags)
+        _ck(lib().lk_batch_set_warm_start(self.h, x.ctypes.data, y.ctypes.data, s.ctypes.data,
+                                          None if f is None else f.ctypes.data))
+
+    def solutions(self):
+        """All solutions, concatenated (x, y, s)."""
+        xs, ys, ss = zip(*[self.solution(i) for i in range(self.count)])
+        return np.concatenate(xs), np.concatenate(ys), np.concatenate(ss)
+
     def solution(self, i):
         x, y, s = np.zeros(self.n[i]), np.zeros(self.m[i]), np.zeros(self.n[i])
         _ck(lib().lk_batch_solution(self.h, i, x, y, s))
@@ -508,6 +535,69 @@ def batch_solve_noalloc(lps, **opts):
 
 def last_error() -> str:
     return lib().lk_last_error().decode(errors="replace")
+
+
+# ------------------------------------------------------------------ warm-start network
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class MLP:
+    """learn::WarmStartMLP: d_in -> h1 -> h2 -> d_out, ReLU, BF16 MFMA / FP32 accumulation."""
+
+    def __init__(self, d_in, h1, h2, d_out, max_batch=256, seed=0):
+        self.dims = (d_in, h1, h2, d_out)
+        self.max_batch = max_batch
+        self.h = _vp()
+        _ck(lib().lk_mlp_create(d_in, h1, h2, d_out, max_batch, seed, C.byref(self.h)))
+        n = C.c_int64()
+        _ck(lib().lk_mlp_param_count(self.h, C.byref(n)))
+        self.nparams = n.value
+
+    def params(self):
+        p = np.zeros(self.nparams, np.float32)
+        _ck(lib().lk_mlp_get_params(self.h, p))
+        return p
+
+    def set_params(self, p):
+        _ck(lib().lk_mlp_set_params(self.h, _f32(p)))
+
+    def forward(self, u):
+        u = _f32(u)
+        out = np.zeros((u.shape[0], self.dims[3]), np.float32)
+        for b0 in range(0, u.shape[0], self.max_batch):
+            ub = np.ascontiguousarray(u[b0:b0 + self.max_batch])
+            yb = np.zeros((ub.shape[0], self.dims[3]), np.float32)
+            _ck(lib().lk_mlp_forward(self.h, ub, ub.shape[0], yb))
+            out[b0:b0 + ub.shape[0]] = yb
+        return out
+
+    def gradients(self, u, t):
+        u, t = _f32(u), _f32(t)
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_double()
+        _ck(lib().lk_mlp_gradients(self.h, u, t, u.shape[0], g, C.byref(loss)))
+        return g, loss.value
+
+    def set_dataset(self, u, t):
+        u, t = _f32(u), _f32(t)
+        _ck(lib().lk_mlp_set_dataset(self.h, u, t, u.shape[0]))
+
+    def train(self, epochs, batch, lr, seed=0):
+        out = np.zeros(epochs)
+        _ck(lib().lk_mlp_train(self.h, epochs, batch, lr, seed, out))
+        return out
+
+    def close(self):
+        if self.h:
+            _ck(lib().lk_mlp_destroy(self.h))
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def ruiz_scaling(rowptr, colidx, vals, b, c):

@@ -62,7 +62,7 @@ def test_long_rows_and_columns_match_oracle(gpu, oracle, case):
 
 GOLDEN_GPU = ["rs_60_150_3_s1", "rs_200_500_6_s2", "ba_4x60x150_L5_s3", "tr_12x12_side30_s4",
               "dense_80_s5", "tr_100x100_side2000_s0", "rs_2000_5000_6_s0",
-              "ba_8x1500x4000_L50_s0", "ba_64x1500x4000_L50_s0"]
+              "ba_8x1500x4000_L50_s0", "ba_64x1500x4000_L50_s0", "tr_700x700_side20000_s0"]
 
 
 def _golden(name):
@@ -84,14 +84,14 @@ def _check_certified(gpu, lp, r, gold):
 def test_golden_objective_fp64(gpu, name):
     """The plain FP64 device IPM on the seeded instances whose HiGHS objectives are committed
     (rs_2000_5000: the 2,000 x 5,000 random LP; ba_8 / ba_64: the block-angular LP with 8 / 64
-    blocks of 1,500 x 4,000)."""
+    blocks of 1,500 x 4,000; tr_700x700: the 700 x 700 transport LP with 20,000 side rows)."""
     from golden_cases import build_case
     lp = build_case(name)
     _check_certified(gpu, lp, gpu.ipm_solve(lp, relative=True), _golden(name))
 
 
 @pytest.mark.parametrize("name", ["rs_200_500_6_s2", "tr_100x100_side2000_s0", "rs_2000_5000_6_s0",
-                                  "ba_8x1500x4000_L50_s0"])
+                                  "ba_8x1500x4000_L50_s0", "tr_700x700_side20000_s0"])
 def test_golden_objective_mixed(gpu, name):
     """FP32 factorization with FP64 PCG / refinement reaches the same certified optimum."""
     from golden_cases import build_case
