@@ -33,6 +33,10 @@ struct BatchOptions {
     int use_graph = 1;              // replay the launch sequence from a captured hipGraph
     int order = 0;                  // 0 largest m first, 1 input order
     double warm_floor = 1e-3;       // lower clamp of warm-start x and s (scaled space)
+    double warm_mu = 0.0;           // lift warm pairs to x_j s_j >= warm_mu (scaled space; 0: none)
+    // a warm point whose relative infeasibility max(|Ax-b|_inf / (1+|b|_inf),
+    // |A^T y+s-c|_inf / (1+|c|_inf)) exceeds this is rejected for the standard start
+    double warm_max_infeas = 1e-2;
 };
 
 struct BatchLPResult {
@@ -44,6 +48,8 @@ struct BatchLPResult {
     // device time of the LP's workgroup (ms): scaling, sparse passes and vector work, assembly,
     // dense factorization, dense triangular solves
     double ms_init = 0, ms_other = 0, ms_assemble = 0, ms_factor = 0, ms_solve = 0;
+    int warm = 0;                   // 0 standard start, 1 warm point used, 2 warm point rejected
+    double warm_infeas = 0;         // relative infeasibility of the warm point offered
 };
 
 struct BatchSummary {

@@ -176,6 +176,9 @@ typedef struct {
     int32_t use_graph;          /* replay the launch sequence from a captured hipGraph */
     int32_t order;              /* 0 largest m first, 1 input order */
     double warm_floor;          /* lower clamp of warm-start x and s (scaled space) */
+    double warm_mu;             /* lift of warm-start pairs to x_j s_j >= warm_mu (0: none) */
+    double warm_max_infeas;     /* warm point rejected (standard start) above this relative
+                                   infeasibility, max(|Ax-b|_inf/(1+|b|_inf), |A'y+s-c|_inf/(1+|c|_inf)) */
 } lk_batch_options;
 typedef struct {
     int32_t count, optimal, launches, graph_replays;
@@ -188,6 +191,8 @@ typedef struct {
     double objective, dual_objective, rel_gap, rel_primal, rel_dual;
     double primal_residual, dual_residual, mu;
     double ms_init, ms_other, ms_assemble, ms_factor, ms_solve;   /* LP's workgroup time */
+    double warm_infeas;         /* relative infeasibility of the warm point offered */
+    int32_t warm, pad2;         /* 0 standard start, 1 warm point used, 2 warm point rejected */
 } lk_batch_result;
 int lk_batch_default_options(lk_batch_options* opt);
 int lk_batch_create(int32_t count, const int32_t* m, const int32_t* n, const int32_t* colptr,

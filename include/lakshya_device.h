@@ -376,6 +376,10 @@ typedef struct {
     /* wall-clock ticks spent by the LP's workgroup: scaling and start point, sparse passes and
      * vector work, assembly, dense factorization, dense triangular solves */
     double t_init, t_other, t_assemble, t_factor, t_solve;
+    /* start point: 0 standard, 1 the warm point, 2 the warm point rejected (its original-space
+     * relative infeasibility, warm_infeas, above warm_max_infeas) and the standard start taken */
+    double warm_infeas;
+    int32_t warm, warm_pad;
 } lkb_state;
 
 typedef struct {
@@ -394,11 +398,14 @@ typedef struct {
     double *x_out, *y_out, *s_out;
     uint32_t* ctl;                 /* [0] ticket counter, [1] LPs finished */
     /* optional warm start (original space, the layout of x_out / y_out / s_out): LP i starts
-     * from it when warm_flag[i] != 0 -- x = max(x0 / C, warm_floor), s = max(s0 C, warm_floor),
-     * y = y0 / R after scaling -- else from x = s = 1, y = 0 */
+     * from it when warm_flag[i] != 0 and its relative infeasibility
+     * max(||A x0 - b||_inf / (1 + ||b||_inf), ||A^T y0 + s0 - c||_inf / (1 + ||c||_inf)) is at
+     * most warm_max_infeas -- x = max(x0 / C, warm_floor), s = max(s0 C, warm_floor),
+     * y = y0 / R after scaling, then, with warm_mu > 0, the smaller of x_j and s_j raised so
+     * that x_j s_j >= warm_mu -- else from x = s = 1, y = 0 */
     const double *x0, *y0, *s0;
     const int32_t* warm_flag;
-    double warm_floor;
+    double warm_floor, warm_mu, warm_max_infeas;
 } lkb_args;
 
 /* ctl[0] = 0; with `restart` also ctl[1] = 0 and the phase of LPs order[0 .. count) back to 0

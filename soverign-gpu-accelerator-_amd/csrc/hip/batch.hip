@@ -842,9 +842,27 @@ __device__ __forceinline__ bool dir_ok(const Dir& d) {
 }
 
 
+// Original-space relative infeasibility of the warm point of LP d:
+// max(||A x0 - b||_inf / (1 + ||b||_inf), ||A^T y0 + s0 - c||_inf / (1 + ||c||_inf)), with the
+// working values still equal to the original ones.
+__device__ double warm_infeasibility(Ctx& c, const lkb_lp& d) {
+    const Lp& L = c.L;
+    const double *x0 = c.A.x0 + d.on, *s0 = c.A.s0 + d.on, *y0 = c.A.y0 + d.om;
+    double v[2] = {0.0, 0.0};
+    seg_pass<GR>(L.m, L.rp, RSUM, [&](int p) { return L.av[p] * x0[L.ci[p]]; },
+                 [&](int i, double ax) { v[0] = fmax(v[0], fabs(ax - L.b0[i])); });
+    seg_pass<GC>(L.n, L.cp, RSUM, [&](int q) { return L.cv[q] * y0[L.ri[q]]; },
+                 [&](int j, double aty) { v[1] = fmax(v[1], fabs(aty + s0[j] - L.c0[j])); });
+    const int op[2] = {RMAX, RMAX};
+    wg_reduce<2>(v, op, c.red);
+    return fmax(v[0] / (1.0 + c.st.binf), v[1] / (1.0 + c.st.cinf));
+}
+
 // Scaling and start point (mehrotra_device.cu:93-102,286-323): the working CSR and CSC values
-// from the original ones, 10 Jacobi Ruiz passes, x = s = 1, y = 0 -- or, for an LP with a warm
-// start, the given original-space point scaled (x / C, y / R, s C) with x and s floored.
+// from the original ones, 10 Jacobi Ruiz passes, x = s = 1, y = 0 -- or, for an LP with an
+// accepted warm start, the given original-space point scaled (x / C, y / R, s C), x and s
+// floored, then each pair lifted to x_j s_j >= warm_mu by raising its smaller member (the
+// central-path point at mu of a strictly complementary pair is about (x + mu / s, s + mu / x)).
 __device__ void init_lp(Ctx& c, const lkb_lp& d, bool warm) {
     const Lp& L = c.L;
     double *b = L.M(WB), *R = L.M(WR), *fr = L.M(WTM), *y = L.M(WY);
@@ -874,6 +892,13 @@ __device__ void init_lp(Ctx& c, const lkb_lp& d, bool warm) {
         c.st.binf = v[0];
         c.st.cinf = v[1];
     }
+    c.st.warm = 0;
+    c.st.warm_infeas = 0.0;
+    if (warm) {
+        c.st.warm_infeas = warm_infeasibility(c, d);
+        warm = c.st.warm_infeas <= c.A.warm_max_infeas;
+        c.st.warm = warm ? 1 : 2;
+    }
     for (int pass = 0; pass < 10; ++pass) {
         seg_pass<GR>(L.m, L.rp, RMAX, [&](int p) { return fabs(L.av[p]); },
                      [&](int i, double mx) { fr[i] = mx > 1e-8 ? 1.0 / sqrt(mx) : 1.0; });
@@ -894,10 +919,16 @@ __device__ void init_lp(Ctx& c, const lkb_lp& d, bool warm) {
         __syncthreads();
     }
     if (warm) {
-        const double fl = c.A.warm_floor;
+        const double fl = c.A.warm_floor, mu = c.A.warm_mu;
         for (int j = threadIdx.x; j < L.n; j += BT) {
-            x[j] = fmax(c.A.x0[d.on + j] / Cs[j], fl);
-            s[j] = fmax(c.A.s0[d.on + j] * Cs[j], fl);
+            double xj = fmax(c.A.x0[d.on + j] / Cs[j], fl);
+            double sj = fmax(c.A.s0[d.on + j] * Cs[j], fl);
+            if (xj * sj < mu) {
+                if (xj >= sj) sj = mu / xj;
+                else xj = mu / sj;
+            }
+            x[j] = xj;
+            s[j] = sj;
         }
         for (int i = threadIdx.x; i < L.m; i += BT) y[i] = c.A.y0[d.om + i] / R[i];
         __syncthreads();

@@ -260,6 +260,8 @@ BatchLPResult to_result(const lkb_state& s, double clock_khz) {
     r.ms_assemble = s.t_assemble / clock_khz;
     r.ms_factor = s.t_factor / clock_khz;
     r.ms_solve = s.t_solve / clock_khz;
+    r.warm = s.warm;
+    r.warm_infeas = s.warm_infeas;
     return r;
 }
 
@@ -331,6 +333,8 @@ struct BatchedMehrotraSolver::Impl {
         a.s0 = d_s0;
         a.warm_flag = d_wflag;
         a.warm_floor = opt.warm_floor;
+        a.warm_mu = opt.warm_mu;
+        a.warm_max_infeas = opt.warm_max_infeas;
         const int max_launches = (opt.max_iter + 1 + iters - 1) / iters + 1;
         void *e0 = nullptr, *e1 = nullptr;
         check(lkd_event_create(&e0), "event");

@@ -196,6 +196,8 @@ ipm::BatchOptions to_batch_options(const lk_batch_options* o) {
     r.use_graph = o->use_graph;
     r.order = o->order;
     r.warm_floor = o->warm_floor;
+    r.warm_mu = o->warm_mu;
+    r.warm_max_infeas = o->warm_max_infeas;
     return r;
 }
 void copy_summary(const ipm::BatchSummary& s, lk_batch_summary* o) {
@@ -547,6 +549,8 @@ int lk_batch_default_options(lk_batch_options* o) {
         o->use_graph = d.use_graph;
         o->order = d.order;
         o->warm_floor = d.warm_floor;
+        o->warm_mu = d.warm_mu;
+        o->warm_max_infeas = d.warm_max_infeas;
     });
 }
 int lk_batch_create(int32_t count, const int32_t* m, const int32_t* n, const int32_t* colptr,
@@ -588,6 +592,8 @@ void copy_batch_result(const ipm::BatchLPResult& r, lk_batch_result& o) {
     o.ms_assemble = r.ms_assemble;
     o.ms_factor = r.ms_factor;
     o.ms_solve = r.ms_solve;
+    o.warm = r.warm;
+    o.warm_infeas = r.warm_infeas;
 }
 }  // namespace
 
@@ -609,30 +615,7 @@ int lk_batch_solve_noalloc(int32_t count, const int32_t* m, const int32_t* n,
 int lk_batch_results(lk_batch* h, lk_batch_result* out) {
     return guarded([&] {
         const auto& r = h->solver->results();
-        for (size_t i = 0; i < r.size(); ++i) {
-            lk_batch_result& o = out[i];
-            std::memset(&o, 0, sizeof(o));
-            o.status = r[i].status;
-            o.iterations = r[i].iterations;
-            o.fail_code = r[i].fail_code;
-            o.factorizations = r[i].factorizations;
-            o.fp32_factorizations = r[i].fp32_factorizations;
-            o.solves = r[i].solves;
-            o.pcg_steps = r[i].pcg_steps;
-            o.objective = r[i].objective;
-            o.dual_objective = r[i].dual_objective;
-            o.rel_gap = r[i].rel_gap;
-            o.rel_primal = r[i].rel_primal;
-            o.rel_dual = r[i].rel_dual;
-            o.primal_residual = r[i].primal_residual;
-            o.dual_residual = r[i].dual_residual;
-            o.mu = r[i].mu;
-            o.ms_init = r[i].ms_init;
-            o.ms_other = r[i].ms_other;
-            o.ms_assemble = r[i].ms_assemble;
-            o.ms_factor = r[i].ms_factor;
-            o.ms_solve = r[i].ms_solve;
-        }
+        for (size_t i = 0; i < r.size(); ++i) copy_batch_result(r[i], out[i]);
     });
 }
 int lk_batch_solution(lk_batch* h, int32_t i, double* x, double* y, double* s) {

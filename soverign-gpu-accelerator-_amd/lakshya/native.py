@@ -60,7 +60,8 @@ class BatchOptions(C.Structure):
                 ("mixed_max_refine", C.c_int32), ("tol", C.c_double), ("gap_tol", C.c_double),
                 ("mixed_fp64_spread", C.c_double), ("wg_per_cu", C.c_int32),
                 ("iters_per_launch", C.c_int32), ("use_graph", C.c_int32), ("order", C.c_int32),
-                ("warm_floor", C.c_double)]
+                ("warm_floor", C.c_double), ("warm_mu", C.c_double),
+                ("warm_max_infeas", C.c_double)]
 
 
 class BatchSummary(C.Structure):
@@ -78,7 +79,8 @@ class BatchResult(C.Structure):
                 ("rel_gap", C.c_double), ("rel_primal", C.c_double), ("rel_dual", C.c_double),
                 ("primal_residual", C.c_double), ("dual_residual", C.c_double), ("mu", C.c_double),
                 ("ms_init", C.c_double), ("ms_other", C.c_double), ("ms_assemble", C.c_double),
-                ("ms_factor", C.c_double), ("ms_solve", C.c_double)]
+                ("ms_factor", C.c_double), ("ms_solve", C.c_double),
+                ("warm_infeas", C.c_double), ("warm", C.c_int32), ("pad2", C.c_int32)]
 
 
 STATUS = {0: "Optimal", 1: "Infeasible", 2: "Unbounded", 3: "IterationLimit"}
@@ -443,16 +445,27 @@ class Batch:
     """ipm::BatchedMehrotraSolver over a list of CSC LPs (lakshya.gen.LP)."""
 
     def __init__(self, lps, **opts):
-        self.count = len(lps)
-        self.m = np.array([lp.m for lp in lps], np.int32)
-        self.n = np.array([lp.n for lp in lps], np.int32)
+        m = np.array([lp.m for lp in lps], np.int32)
+        n = np.array([lp.n for lp in lps], np.int32)
         colptr = np.concatenate([np.asarray(lp.indptr, np.int32) for lp in lps])
         rowidx = np.concatenate([np.asarray(lp.indices, np.int32) for lp in lps])
         vals = np.concatenate([np.asarray(lp.data, np.float64) for lp in lps])
         b = np.concatenate([np.asarray(lp.b, np.float64) for lp in lps])
         c = np.concatenate([np.asarray(lp.c, np.float64) for lp in lps])
-        o = batch_options(**opts)
+        self._create(m, n, colptr, rowidx, vals, b, c, opts)
+
+    @classmethod
+    def from_arrays(cls, m, n, colptr, rowidx, vals, b, c, **opts):
+        """The batch from its concatenated arrays (the lk_batch_create layout)."""
+        self = cls.__new__(cls)
+        self._create(_i32(m), _i32(n), _i32(colptr), _i32(rowidx), _f64(vals), _f64(b), _f64(c), opts)
+        return self
+
+    def _create(self, m, n, colptr, rowidx, vals, b, c, opts):
         self.h = _vp()
+        self.count = len(m)
+        self.m, self.n = m, n
+        o = batch_options(**opts)
         _ck(lib().lk_batch_create(self.count, self.m, self.n, colptr, rowidx, vals, b, c,
                                   C.byref(o), C.byref(self.h)))
 
