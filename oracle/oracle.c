@@ -36,6 +36,7 @@
  * reference's own symbolic/verifier sources compiled into oracle/_ref where available.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -525,6 +526,9 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
     int max_iter = opt ? opt->max_iter : 200;
     int relmode = opt && opt->stop_rule == 1;
     int done = 0;
+    /* OR_TRACE set: one line per iteration on stderr (route: 0 corrector, 1 centred, 3 dual-only,
+     * 4 non-monotone centred fallback) */
+    const int trace = getenv("OR_TRACE") != NULL;
 
 #define RESID(X, Y, S, RP, RD)                                                   \
     do {                                                                          \
@@ -588,12 +592,21 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                    rdi <= fmax(REL_ABS_MARGIN * opt->gap_tol, 1e-12 * (1 + cinf)) &&
                    xsmax <= fmax(REL_ABS_MARGIN * opt->gap_tol, 1e-14 * fmax(1.0, fmax(fabs(pobj), fabs(dobj))));
         }
+        if (trace && relmode) {
+            double pobj = dot(n, c, x), dobj = dot(m, b, y), xsm = 0;
+            for (int j = 0; j < n; ++j) xsm = fmax(xsm, fabs(x[j] * (s[j] - rd[j])));
+            fprintf(stderr, "or_ipm %3d gap %.3e xsmax %.3e\n", iter,
+                    fabs(pobj - dobj) / fmax(1.0, fmax(fabs(pobj), fabs(dobj))), xsm);
+        }
         if (stop) { res->iterations = iter; res->status = 0; done = 1; break; }
 
         for (int j = 0; j < n; ++j) th[j] = x[j] / s[j];
         or_assemble_dense(m, rp, ci, A, th, P, k.M0);
         res->factorizations++;
-        if (kkt_factor_check(&k) != OR_OK) { rc = OR_NOT_SPD; res->status = -1; res->iterations = iter; res->fail_code = 1; break; }
+        /* the reference's unequilibrated factorization of M0 + 1e-9 I only decides whether its
+         * solve throws; the relative-gap mode (an extension) does without it, like the device */
+        if (relmode) k.have_eq = 0;
+        else if (kkt_factor_check(&k) != OR_OK) { rc = OR_NOT_SPD; res->status = -1; res->iterations = iter; res->fail_code = 1; break; }
         double tol_kkt = 1e-9 * fmax(1.0, nrp);
 
         /* predictor */
@@ -635,7 +648,8 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                 if ((rpi0 <= ptar && rpt_ > ptar) || (rdi0 <= dtar && rdt_ > dtar)) OKV = 0; \
             }                                                                     \
         } while (0)
-        int accepted = 0;
+        int accepted = 0, route = -1;
+        double tr_ap = 0, tr_ad = 0;
         for (int dir = corr_valid ? 0 : 1; dir < 2 && !accepted; ++dir) {
             if (dir == 1) {
                 for (int j = 0; j < n; ++j) rxs[j] = -x[j] * s[j] + 0.5 * mu;
@@ -648,7 +662,8 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                 ap = ad = fmin(1.0, 0.995 * bnd);
             }
             int maxbt = dir == 0 ? 15 : 30;
-            int fallback_bt = -1;   /* relative mode: first finite, positive centred trial */
+            int fallback_bt = -1;   /* relative mode: first finite, positive centred trial that
+                                       keeps the met residual targets */
             for (int bt = 0; bt < maxbt; ++bt) {
                 for (int j = 0; j < n; ++j) { bx[j] = x[j] + ap * dx[j]; bs[j] = s[j] + ad * ds[j]; }
                 for (int i = 0; i < m; ++i) by[i] = y[i] + ad * dy[i];
@@ -665,8 +680,8 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                 int keeps;
                 KEEPS_TARGETS(keeps);
                 prog = prog && keeps;
-                if (finite && pos && prog) { accepted = 1; break; }
-                if (relmode && dir == 1 && finite && pos && fallback_bt < 0) fallback_bt = bt;
+                if (finite && pos && prog) { accepted = 1; route = dir; tr_ap = ap; tr_ad = ad; break; }
+                if (relmode && dir == 1 && finite && pos && keeps && fallback_bt < 0) fallback_bt = bt;
                 ap *= 0.5;
                 ad *= 0.5;
             }
@@ -687,7 +702,7 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                     int prog = merit < merit0 && merit <= (1.0 - 1e-4 * ad2) * merit0;
                     int keeps;
                     KEEPS_TARGETS(keeps);
-                    if (finite && pos && prog && keeps) accepted = 1;
+                    if (finite && pos && prog && keeps) { accepted = 1; route = 3; tr_ad = ad2; }
                     else ad2 *= 0.5;
                 }
             }
@@ -698,8 +713,13 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                 for (int j = 0; j < n; ++j) { bx[j] = x[j] + a2 * dx[j]; bs[j] = s[j] + a2 * ds[j]; }
                 for (int i = 0; i < m; ++i) by[i] = y[i] + a2 * dy[i];
                 accepted = 1;
+                route = 4;
+                tr_ap = tr_ad = a2;
             }
         }
+        if (trace)
+            fprintf(stderr, "or_ipm %3d |rp| %.3e |rd| %.3e mu %.3e rp_inf %.3e sigma %.3e route %d ap %.3e ad %.3e\n",
+                    iter, nrp, nrd, mu, rpi0, sigma, route, tr_ap, tr_ad);
         if (!accepted) { res->status = 3; res->iterations = iter; res->fail_code = 5; break; }
         memcpy(x, bx, sizeof(double) * n);
         memcpy(y, by, sizeof(double) * m);

@@ -1059,17 +1059,18 @@ __device__ bool iterate(Ctx& c, const lkb_lp& d) {
     // relative-gap mode: a residual already inside its absolute target must stay inside
     const bool keep_p = c.rel && nr.rpinf <= primal_target(c);
     const bool keep_d = c.rel && nr.rdinf <= dual_target(c);
-    auto admissible = [&](const Trial& T, double a, double b, bool& finite, bool& positive) {
+    // `usable`: finite, positive and keeping the met residual targets (merit aside)
+    auto admissible = [&](const Trial& T, double a, double b, bool& usable) {
         const double merit = T.nrp / c.st.rp0 + T.nrd / c.st.rd0 + T.mu / c.st.mu0;
         const double stp = fmin(a, b);
-        finite = isfinite(T.nrp) && isfinite(T.nrd) && isfinite(T.mu) && isfinite(merit) &&
-                 isfinite(merit0) && isfinite(T.mx) && isfinite(T.ms) && isfinite(a) &&
-                 isfinite(b) && isfinite(T.ny);
-        positive = T.mx > 0.0 && T.ms > 0.0 && T.mu > 0.0 && stp > 0.0;
-        const bool progress = merit < merit0 && merit <= (1.0 - 1e-4 * stp) * merit0 &&
-                              !(keep_p && T.rpinf > primal_target(c)) &&
-                              !(keep_d && T.rdinf > dual_target(c));
-        return finite && positive && progress;
+        const bool finite = isfinite(T.nrp) && isfinite(T.nrd) && isfinite(T.mu) &&
+                            isfinite(merit) && isfinite(merit0) && isfinite(T.mx) &&
+                            isfinite(T.ms) && isfinite(a) && isfinite(b) && isfinite(T.ny);
+        const bool positive = T.mx > 0.0 && T.ms > 0.0 && T.mu > 0.0 && stp > 0.0;
+        const bool keeps = !(keep_p && T.rpinf > primal_target(c)) &&
+                           !(keep_d && T.rdinf > dual_target(c));
+        usable = finite && positive && keeps;
+        return usable && merit < merit0 && merit <= (1.0 - 1e-4 * stp) * merit0;
     };
     bool accepted = false;
     for (int dir = dir_ok(dc) ? 0 : 1; dir < 2 && !accepted; ++dir) {
@@ -1085,9 +1086,9 @@ __device__ bool iterate(Ctx& c, const lkb_lp& d) {
         const int ntr = dir == 0 ? 15 : 30;
         double a = ap, b = ad, fa = -1.0, fb = -1.0;
         for (int t = 0; t < ntr; ++t) {
-            bool finite, positive;
-            if (admissible(trial(c, a, b), a, b, finite, positive)) { accepted = true; break; }
-            if (c.rel && dir == 1 && finite && positive && fa < 0) { fa = a; fb = b; }
+            bool usable;
+            if (admissible(trial(c, a, b), a, b, usable)) { accepted = true; break; }
+            if (c.rel && dir == 1 && usable && fa < 0) { fa = a; fb = b; }
             a *= 0.5;
             b *= 0.5;
         }
@@ -1098,13 +1099,14 @@ __device__ bool iterate(Ctx& c, const lkb_lp& d) {
             a = 0.0;
             b = fmin(1.0, 0.995 * dc.admax);
             for (int t = 0; t < 15; ++t) {
-                bool finite, positive;
-                if (admissible(trial(c, a, b), b, b, finite, positive)) { accepted = true; break; }
+                bool usable;
+                if (admissible(trial(c, a, b), b, b, usable)) { accepted = true; break; }
                 b *= 0.5;
             }
         }
         if (!accepted && c.rel && dir == 1 && fa > 0) {
-            // relative-gap mode: the longest admissible centred step (non-monotone)
+            // relative-gap mode: the longest usable centred step (non-monotone; it may not break
+            // a met residual target)
             a = fa;
             b = fb;
             accepted = true;

@@ -466,7 +466,8 @@ public:
                 check(lkd_ipm_trials(&P_, x_, s_, y_, dx_, ds_, dy_, rp_, rd_, qp_, qd_, ap, ad, ntr, stream_), "trials");
                 read_scalars(LKS_TRIAL_BASE + LKS_TRIAL_STRIDE * ntr);
                 double a = ap, b = ad;
-                double fa = -1.0, fb = -1.0;   // relative mode: first finite, positive centred trial
+                // relative mode: first finite, positive centred trial that keeps the met targets
+                double fa = -1.0, fb = -1.0;
                 for (int t = 0; t < ntr; ++t) {
                     const double* T = &hS_[LKS_TRIAL_BASE + t * LKS_TRIAL_STRIDE];
                     const double nrp2 = std::sqrt(T[0]), nrd2 = std::sqrt(T[1]), mu2 = T[2] / n_;
@@ -486,7 +487,7 @@ public:
                             std::printf("    step: dir %d trial %d ap %.3e ad %.3e sigma %.3e pcg %d\n", dir, t, a, b, sigma, res.refinement_steps);
                         break;
                     }
-                    if (rel && dir == 1 && finite && positive && fa < 0) { fa = a; fb = b; }
+                    if (rel && dir == 1 && finite && positive && keeps(T) && fa < 0) { fa = a; fb = b; }
                     a *= 0.5;
                     b *= 0.5;
                 }
