@@ -17,9 +17,20 @@ summed device time of its launches, measured live over the timed solves with HIP
 solver's stream at every change of launch kind.  traffic: HBM bytes per k_step_dag launch from
 the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), scaled by the same counters
 on a copy kernel of known size.
-cpu_baseline: the C oracle (same algorithm, dense FP64 Cholesky, 1 thread) on one block of the
-workload (1,550 x 4,000 incl. the 50 linking rows) for CPU_ITERS IPM iterations, scaled to 64
-blocks x the GPU's iteration count.
+cpu_baseline: the reference's own CPU route -- solve_with_phase1 (Phase I + Phase II primal
+simplex over the sparse LU basis factorization), compiled from the reference sources into
+oracle/_ref/libref.so -- timed on one pinned core on the largest member of the 2,000 x 5,000
+random-LP family it solves within the sample budget (m = 500, n = 1,250); on the workload itself
+its Phase I needs >= m = 96,050 pivots and stops at the reference's 10,000-pivot limit.  The
+C oracle IPM (same algorithm as the device, dense FP64 Cholesky, 1 thread) on one block of the
+workload, scaled to 64 blocks x the GPU's iteration count, is reported beside it ("port_ipm").
+
+chosen_forms: each chosen form against plain FP64 on the same instances, in the same run --
+mixed precision (FP32 MFMA factor + FP64 PCG) on this workload and on the 700 x 700 transport LP
+with 20,000 side rows; the batched persistent solver on the 4,096-LP batch (m ~ U{50..800}),
+FP64 and mixed, against a sequential loop of the plain single-LP device IPM on a sample of the
+same LPs; the learned warm start on the parametric family at 400 x 1,000 (the 20,000 x 50,000
+family scaled down so its training set can be generated in the run).
 """
 import argparse
 import json
@@ -97,6 +108,59 @@ def cpu_baseline(gpu_iters, blocks, seed):
                        f"blocks x {gpu_iters} iterations = {solve_s:.0f} s per solve")}
 
 
+def _ref_route_child(q, cpu, m, n, seed):
+    """Child process: the reference CPU route on one random LP, pinned to one core."""
+    import ctypes as C
+    try:
+        os.sched_setaffinity(0, {cpu})
+    except (AttributeError, OSError):
+        pass
+    from lakshya import gen
+    L = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref.so"))
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C")
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    L.ref_phase1_solve.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), dp, C.POINTER(C.c_double)]
+    lp = gen.random_sparse(m, n, 6, seed=seed)
+    st, obj, ms = C.c_int(), C.c_double(), C.c_double()
+    x = np.zeros(lp.n)
+    rc = L.ref_phase1_solve(lp.m, lp.n, lp.indptr.astype(np.int32), lp.indices.astype(np.int32),
+                            lp.data.astype(np.float64), lp.c.astype(np.float64),
+                            lp.b.astype(np.float64), C.byref(st), C.byref(obj), x, C.byref(ms))
+    q.put({"rc": rc, "status": st.value, "objective": obj.value, "ms": ms.value})
+
+
+def cpu_reference_route(m=500, n=1250, seed=1, timeout_s=120):
+    """The reference's CPU simplex route on one pinned core (see the module docstring)."""
+    import multiprocessing as mp
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libref.so")):
+        return None
+    ctx = mp.get_context("spawn")
+    q = ctx.Queue()
+    cpu = sorted(os.sched_getaffinity(0))[-1] if hasattr(os, "sched_getaffinity") else 0
+    p = ctx.Process(target=_ref_route_child, args=(q, cpu, m, n, seed))
+    p.start()
+    p.join(timeout_s)
+    if p.is_alive():
+        p.kill()
+        p.join()
+        return {"value": 0.0, "unit": "solves/s", "cores": 1, "kind": "reference",
+                "sample": f"reference solve_with_phase1 + SparseLU on rs_{m}_{n}_6_s{seed}: "
+                          f"no result within {timeout_s} s"}
+    r = q.get()
+    names = {0: "Optimal", 1: "Infeasible", 2: "Unbounded", 3: "IterationLimit"}
+    solved = r["rc"] == 0 and r["status"] == 0
+    return {"value": (1e3 / r["ms"]) if solved else 0.0, "unit": "solves/s", "cores": 1,
+            "kind": "reference",
+            "sample": (f"reference CPU route (solve_with_phase1: Phase I + Phase II primal simplex, "
+                       f"SparseLUFactorization; oracle/_ref built from the reference sources, -O2, "
+                       f"no fast-math) on one pinned core, random LP m={m} n={n} (6 nnz/col, seed "
+                       f"{seed}): {names.get(r['status'], r['status'])} in {r['ms'] / 1e3:.1f} s, "
+                       f"objective {r['objective']:.10g}.  The workload (m=96,050) needs >= 96,050 "
+                       f"Phase-I pivots against the route's 10,000-pivot limit: IterationLimit, "
+                       f"no solve")}
+
+
 def step_roofline(live, st, fp32):
     peak = PEAK_FP32 if fp32 else PEAK_FP64
     nfact = max(live["factorization"]["launches"], 1)
@@ -155,6 +219,112 @@ def copy_calibration(mib, reps):
                       "launches": reps + 1, "avg_ms": per, "GB_per_s": 2 * nbytes / (per * 1e-3) / 1e9}))
 
 
+# ------------------------------------------------------------------ chosen forms
+def _certified(r):
+    return (r["status_name"] == "Optimal" and r["rel_gap"] <= 1e-8 and
+            r["rel_primal_residual"] <= 1e-8 and r["rel_dual_residual"] <= 1e-8)
+
+
+def _timed_solves(native, lp, gold, steps, **opts):
+    """Set up one solver, one untimed solve, `steps` timed solves; returns a summary and the
+    last result."""
+    t = time.perf_counter()
+    solver = native.IPM(lp, relative=True, **opts)
+    setup = time.perf_counter() - t
+    solver.run()
+    rs = []
+    t = time.perf_counter()
+    for _ in range(steps):
+        rs.append(solver.run())
+    dt = time.perf_counter() - t
+    solver.close()
+    ok = all(_certified(r) for r in rs)
+    errs = [abs(r["objective"] - gold) / max(1.0, abs(gold)) for r in rs] if gold is not None else None
+    return {"solves_per_s": steps / dt, "ms_per_solve": dt * 1e3 / steps, "setup_s": setup,
+            "iterations": [r["iterations"] for r in rs],
+            "pcg_steps": [r["refinement_steps"] for r in rs],
+            "certified": ok, "max_obj_err_vs_golden": max(errs) if errs else None}, rs[-1]
+
+
+def _vs(a, b):
+    """Agreement of two certified solutions of one LP: relative objective difference and the
+    largest relative difference of x."""
+    return {"rel_objective_diff": abs(a["objective"] - b["objective"]) / max(1.0, abs(b["objective"])),
+            "max_rel_x_diff": float(np.max(np.abs(a["x"] - b["x"]) / (1.0 + np.abs(b["x"]))))}
+
+
+def form_mixed(native, lp, gold, steps, fp64_summary, fp64_last):
+    mx, last = _timed_solves(native, lp, gold, steps, precision=1)
+    mx["vs_fp64"] = _vs(last, fp64_last)
+    return {"fp64": fp64_summary, "mixed": mx,
+            "mixed_over_fp64_solves_per_s": mx["solves_per_s"] / fp64_summary["solves_per_s"]}
+
+
+def form_transport(native, steps):
+    from lakshya import gen
+    lp = gen.transport(seed=0)
+    g = golden("tr_700x700_side20000_s0")
+    gold = g["objective"] if g else None
+    f64, last64 = _timed_solves(native, lp, gold, steps)
+    out = form_mixed(native, lp, gold, steps, f64, last64)
+    out["workload"] = {"name": "tr_700x700_side20000_s0", "m": lp.m, "n": lp.n, "nnz": lp.nnz}
+    return out
+
+
+def _batch_lp(args):
+    from lakshya import gen
+    return gen.batch_lp(*args)
+
+
+def form_batch(native, count=4096, sample=48, workers=8):
+    """The 4,096-LP batch: batched FP64 and mixed against a sequential loop of the plain
+    single-LP device IPM (set-up + solve per LP, as a caller with distinct LPs pays it) on
+    `sample` LPs spread over the batch, extrapolated to the batch."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    from lakshya import gen
+    sizes = gen.batch_sizes(count)
+    t = time.perf_counter()
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        lps = list(ex.map(_batch_lp, [(int(sizes[i]), i) for i in range(count)], chunksize=64))
+    gen_s = time.perf_counter() - t
+    out = {"workload": {"count": count, "m_range": [int(sizes.min()), int(sizes.max())],
+                        "generate_s": gen_s}}
+    for name, prec in (("batched_fp64", 0), ("batched_mixed", 1)):
+        bt = native.Batch(lps, precision=prec)
+        bt.solve()                                     # first solve: graph capture, warm caches
+        sm = bt.solve()
+        res = bt.results()
+        bt.close()
+        it = np.array([r["iterations"] for r in res])
+        cert = sum(1 for r in res if r["status"] == 0 and r["rel_gap"] <= 1e-8 and
+                   r["rel_primal"] <= 1e-8 and r["rel_dual"] <= 1e-8)
+        out[name] = {"lps_per_s": count / (sm["ms"] / 1e3), "ms": sm["ms"], "certified": cert,
+                     "iterations_mean": float(it.mean()), "iterations_max": int(it.max()),
+                     "wg_per_cu": sm["wg_per_cu"], "graph_replays": sm["graph_replays"]}
+        if name == "batched_fp64":
+            objs = np.array([r["objective"] for r in res])
+    idx = np.linspace(0, count - 1, sample).astype(int)
+    t = time.perf_counter()
+    seq = [native.ipm_solve(lps[i], relative=True) for i in idx]
+    dt = time.perf_counter() - t
+    scert = sum(1 for r in seq if _certified(r))
+    sobj = np.array([r["objective"] for r in seq])
+    seq_rate = sample / dt
+    out["sequential_plain_fp64"] = {"lps_per_s": seq_rate, "sample": int(sample),
+                                    "certified": scert, "seconds": dt,
+                                    "max_rel_obj_diff_vs_batched":
+                                        float(np.max(np.abs(sobj - objs[idx]) / np.maximum(1.0, np.abs(objs[idx]))))}
+    out["batched_fp64_over_sequential"] = out["batched_fp64"]["lps_per_s"] / seq_rate
+    out["batched_mixed_over_sequential"] = out["batched_mixed"]["lps_per_s"] / seq_rate
+    return out
+
+
+def form_warm(ntrain=20_000, ntest=1000):
+    from lakshya import warmstart
+    return warmstart.experiment(400, 1000, 0.1, ntrain, ntest, target_iter=2, epochs=20)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +336,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="all",
+                    help="comma list of chosen-form comparisons: mixed,transport,batch,warm (all / none)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (traffic-counter reference)")
     args = ap.parse_args()
@@ -218,9 +390,40 @@ def main():
     value = args.steps / elapsed if all_ok else 0.0
     live = solver.launch_timing()
     solver.set_launch_timing(False)
+    solver.close()
     log("launch timing " + json.dumps(live))
     roofline = step_roofline(live, st, args.precision == "mixed")
-    cpu = None if args.no_cpu else cpu_baseline(int(np.median(iters)), args.blocks, args.seed)
+
+    forms = {}
+    want = {"mixed", "transport", "batch", "warm"} if args.forms == "all" else \
+        set() if args.forms == "none" else set(args.forms.split(","))
+    if "mixed" in want and args.precision == "fp64":
+        log("chosen form: mixed precision on the workload")
+        f64 = {"solves_per_s": value, "ms_per_solve": elapsed * 1e3 / args.steps,
+               "setup_s": setup_ms / 1e3, "iterations": iters,
+               "pcg_steps": [r["refinement_steps"] for r in results], "certified": bool(all_ok),
+               "max_obj_err_vs_golden": None}
+        forms["workload_mixed"] = form_mixed(native, lp, gold, args.steps, f64, results[-1])
+        log(json.dumps(forms["workload_mixed"]))
+    if "transport" in want:
+        log("chosen form: mixed precision on the transport LP")
+        forms["transport_mixed"] = form_transport(native, 1)
+        log(json.dumps(forms["transport_mixed"]))
+    if "batch" in want:
+        log("chosen form: batched solver on the 4,096-LP batch")
+        forms["batch_4096"] = form_batch(native)
+        log(json.dumps(forms["batch_4096"]))
+    if "warm" in want:
+        log("chosen form: learned warm start (parametric family 400 x 1,000)")
+        forms["warm_start"] = form_warm()
+        log(json.dumps(forms["warm_start"]))
+
+    cpu = None
+    if not args.no_cpu:
+        port = cpu_baseline(int(np.median(iters)), args.blocks, args.seed)
+        cpu = cpu_reference_route() or port
+        if cpu is not port:
+            cpu["port_ipm"] = port
 
     out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
@@ -232,7 +435,7 @@ def main():
                       "stop": "rel gap, primal, dual residual <= 1e-8",
                       "iterations": iters, "setup_ms": round(setup_ms, 1),
                       "all_checks_passed": bool(all_ok), "golden_objective": gold},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu, "chosen_forms": forms}
     print(json.dumps(out))
 
 
