@@ -211,7 +211,7 @@ __device__ __attribute__((noinline)) bool dense_chol(T* __restrict__ F_, int m_,
         {   // ---- the diagonal block
             Acc<T> acc[4][2];
             zero_acc<T, 4, 2>(acc);
-            gemm_tn<T>(F + k0, ld, kb, F + k0, ld, kb, 0, k0, sm, acc);
+            gemm_tn<T, true>(F + k0, ld, kb, F + k0, ld, kb, 0, k0, sm, acc);
             __syncthreads();
             if (w < 2) {
 #pragma unroll
@@ -243,7 +243,7 @@ __device__ __attribute__((noinline)) bool dense_chol(T* __restrict__ F_, int m_,
             const int ra = min(MT, m - r0);
             Acc<T> acc[4][2];
             zero_acc<T, 4, 2>(acc);
-            gemm_tn<T>(F + r0, ld, ra, F + k0, ld, kb, 0, k0, sm, acc);
+            gemm_tn<T, true>(F + r0, ld, ra, F + k0, ld, kb, 0, k0, sm, acc);
             Acc<T> y[2][4];
 #pragma unroll
             for (int b = 0; b < 2; ++b) {

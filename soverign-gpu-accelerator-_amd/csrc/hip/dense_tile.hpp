@@ -41,7 +41,11 @@ __device__ __forceinline__ void zero_acc(Acc<T> (&acc)[X][Y]) {
 // sets (chunk j in set j % 2): while chunk c is computed, chunks c + 1 and c + 2 are in
 // flight, and chunk c + 1 goes to LDS right after.  One barrier per chunk; the LDS is free
 // again when it returns.
-template <typename T, typename F>
+// Clamp = false: out-of-range elements are not loaded (exec-masked loads; the fewest memory
+// instructions, for the supernodal kernels).  Clamp = true: every lane loads from a clamped,
+// always valid address and a select zeroes the element (no divergent control flow, fewer live
+// registers around the loads: the batched kernel, which runs at its register limit).
+template <typename T, bool Clamp, typename F>
 __device__ __forceinline__ void gemm_loop(const T* __restrict__ pa, int64_t lda, int ra,
                                           const T* __restrict__ pb, int64_t ldb, int rb,
                                           int kbeg, int kend, GemmLds<T>& sm, F&& compute) {
@@ -52,21 +56,28 @@ __device__ __forceinline__ void gemm_loop(const T* __restrict__ pa, int64_t lda,
     const int br = t & (NC - 1), bk = t >> 6;
     const bool arow = ar < ra, brow = br < rb;
     T va[2][NA], vb[2][NB];
-    // loads from clamped (always valid) addresses, zeroed by a select: no divergent branches
     const T* qa = pa + (arow ? ar : 0);
     const T* qb = pb + (brow ? br : 0);
     auto load = [&](int set, int kc) {
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
             const int k = kc + ak + 2 * q;
-            const T v = qa[(int64_t)min(k, kend - 1) * lda];
-            va[set][q] = (arow && k < kend) ? v : T(0);
+            if (Clamp) {
+                const T v = qa[(int64_t)min(k, kend - 1) * lda];
+                va[set][q] = (arow && k < kend) ? v : T(0);
+            } else {
+                va[set][q] = (arow && k < kend) ? pa[(int64_t)k * lda + ar] : T(0);
+            }
         }
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             const int k = kc + bk + 4 * q;
-            const T v = qb[(int64_t)min(k, kend - 1) * ldb];
-            vb[set][q] = (brow && k < kend) ? v : T(0);
+            if (Clamp) {
+                const T v = qb[(int64_t)min(k, kend - 1) * ldb];
+                vb[set][q] = (brow && k < kend) ? v : T(0);
+            } else {
+                vb[set][q] = (brow && k < kend) ? pb[(int64_t)k * ldb + br] : T(0);
+            }
         }
     };
     auto store = [&](int set, int buf) {
@@ -100,12 +111,12 @@ __device__ __forceinline__ void gemm_loop(const T* __restrict__ pa, int64_t lda,
 
 // NT: acc[x][y] (wave w) = tile rows 32w + 16x .., columns 16y ..:
 //   acc += sum_k A(r, k) B(c, k);  element e of lane l: row 32w + 16x + row(l,e), col 16y + l%16
-template <typename T>
+template <typename T, bool Clamp = false>
 __device__ __forceinline__ void gemm_nt(const T* __restrict__ pa, int64_t lda, int ra,
                                         const T* __restrict__ pb, int64_t ldb, int rb, int kbeg,
                                         int kend, GemmLds<T>& sm, Acc<T> (&acc)[2][4]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
-    gemm_loop<T>(pa, lda, ra, pb, ldb, rb, kbeg, kend, sm, [&](const T* As, const T* Bs, int k) {
+    gemm_loop<T, Clamp>(pa, lda, ra, pb, ldb, rb, kbeg, kend, sm, [&](const T* As, const T* Bs, int k) {
         const T a0 = As[(k + lk) * PA + 32 * w + li];
         const T a1 = As[(k + lk) * PA + 32 * w + 16 + li];
         T b[4];
@@ -122,12 +133,12 @@ __device__ __forceinline__ void gemm_nt(const T* __restrict__ pa, int64_t lda, i
 // TN (the transposed product): acc[p][b] (wave w) holds D = B A^T for B columns 16p .. and tile
 // rows 32w + 16b ..:  element e of lane l = sum_k A(32w + 16b + l%16, k) B(16p + row(l,e), k),
 // i.e. tile row from the lane, column from the element -- the B-operand layout of the solve.
-template <typename T>
+template <typename T, bool Clamp = false>
 __device__ __forceinline__ void gemm_tn(const T* __restrict__ pa, int64_t lda, int ra,
                                         const T* __restrict__ pb, int64_t ldb, int rb, int kbeg,
                                         int kend, GemmLds<T>& sm, Acc<T> (&acc)[4][2]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
-    gemm_loop<T>(pa, lda, ra, pb, ldb, rb, kbeg, kend, sm, [&](const T* As, const T* Bs, int k) {
+    gemm_loop<T, Clamp>(pa, lda, ra, pb, ldb, rb, kbeg, kend, sm, [&](const T* As, const T* Bs, int k) {
         const T a0 = As[(k + lk) * PA + 32 * w + li];
         const T a1 = As[(k + lk) * PA + 32 * w + 16 + li];
         T b[4];
