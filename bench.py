@@ -300,6 +300,7 @@ def form_batch(native, count=4096, sample=48, workers=8):
         cert = sum(1 for r in res if r["status"] == 0 and r["rel_gap"] <= 1e-8 and
                    r["rel_primal"] <= 1e-8 and r["rel_dual"] <= 1e-8)
         out[name] = {"lps_per_s": count / (sm["ms"] / 1e3), "ms": sm["ms"], "certified": cert,
+                     "factor_tflops": sm["flops_chol"] / (sm["ms"] * 1e-3) / 1e12,
                      "iterations_mean": float(it.mean()), "iterations_max": int(it.max()),
                      "wg_per_cu": sm["wg_per_cu"], "graph_replays": sm["graph_replays"]}
         if name == "batched_fp64":

@@ -110,14 +110,23 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
             }
 }
 
-// out[n] = beta out[n] + sum_m G[m * ld + n], rows in order (one thread per column)
+// out[n] = beta out[n] + sum_m G[m * ld + n]: a workgroup takes 32 columns, its 8 row groups
+// sum rows g, g + 8, .. in order and the 8 partials are added in order (fixed summation order)
 __global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ G, int M, int N, int64_t ld,
                                                 float beta, float* __restrict__ out) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n >= N) return;
+    __shared__ float part[8][32];
+    const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int n = blockIdx.x * 32 + c;
     float s = 0.0f;
-    for (int m = 0; m < M; ++m) s += G[m * ld + n];
-    out[n] = beta * out[n] + s;
+    if (n < N)
+        for (int m = g; m < M; m += 8) s += G[m * ld + n];
+    part[g][c] = s;
+    __syncthreads();
+    if (g == 0 && n < N) {
+        float t = part[0][c];
+        for (int k = 1; k < 8; ++k) t += part[k][c];
+        out[n] = beta * out[n] + t;
+    }
 }
 
 // g = scale (Y - T); per-block partial sums of (Y - T)^2 (block order fixed)
@@ -137,12 +146,19 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ Y, const 
     if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
-__global__ void k_sum_parts(const float* __restrict__ part, int n, float* __restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += part[i];
-        out[0] = (float)s;
+// out[0] = sum of part[0 .. n): 256 threads sum strided slices in FP64, then a fixed tree
+__global__ __launch_bounds__(256) void k_sum_parts(const float* __restrict__ part, int n,
+                                                   float* __restrict__ out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[0] = (float)sh[0];
 }
 
 // dst[r, :] = src[idx[r], :]
@@ -204,7 +220,7 @@ int lkd_mlp_gemm(int M, int N, int K, const float* A, int64_t sam, int64_t sak, 
 
 int lkd_mlp_colsum(const float* G, int M, int N, int64_t ld, float beta, float* out, void* stream) {
     if (N <= 0) return 0;
-    LK_LAUNCH(k_colsum, (N + 255) / 256, 256, 0, stream, G, M, N, ld, beta, out);
+    LK_LAUNCH(k_colsum, (N + 31) / 32, 256, 0, stream, G, M, N, ld, beta, out);
     return lk_last_error();
 }
 
@@ -213,7 +229,7 @@ int lkd_mlp_mse(const float* Y, const float* T, int64_t count, float scale, floa
     if (count <= 0 || nparts < 1) return (int)hipErrorInvalidValue;
     const int nb = blocks_for(count) < nparts ? blocks_for(count) : nparts;
     LK_LAUNCH(k_mse, nb, 256, 0, stream, Y, T, count, scale, g, part);
-    LK_LAUNCH(k_sum_parts, 1, 64, 0, stream, part, nb, loss);
+    LK_LAUNCH(k_sum_parts, 1, 256, 0, stream, part, nb, loss);
     return lk_last_error();
 }
 
