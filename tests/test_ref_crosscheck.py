@@ -1,7 +1,9 @@
 """Cross-check against the reference's own CPU sources (oracle/_ref/libref.so, built from
 src/ipm/symbolic.cpp, src/verifier/certificate.cpp and src/core by oracle/build_ref.sh where the
 reference tree is present; skipped elsewhere): the exact minimum-degree ordering rule, the
-elimination tree / L pattern, and verify_optimal, for both the C oracle and the library."""
+elimination tree / L pattern, and verify_optimal, for both the C oracle and the library; and
+the CPU simplex route timed as the benchmark's CPU baseline (solve_with_phase1 over the sparse
+LU) against the oracle IPM's optimum."""
 import ctypes as C
 import os
 
@@ -25,6 +27,8 @@ def ref():
                                C.POINTER(C.c_longlong)]
     L.ref_verify.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, dp, dp, C.c_int, dp, dp,
                              C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.ref_phase1_solve.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), dp, C.POINTER(C.c_double)]
     return L
 
 
@@ -105,3 +109,23 @@ def test_verifier_matches_reference_on_random_certificates(ref, native, oracle):
         assert mine == bool(p.value) == orc, trial
         agree += bool(p.value)
     assert 0 < agree < 300      # both outcomes exercised
+
+
+@pytest.mark.parametrize("m,n,seed", [(40, 100, 3), (120, 300, 4)])
+def test_reference_simplex_route_reaches_the_ipm_optimum(ref, oracle, m, n, seed):
+    """The CPU baseline's route (Phase I + Phase II primal simplex, sparse LU) and the oracle
+    IPM at a 1e-8 gap agree on the objective of a feasible, bounded random LP; the simplex
+    point is feasible."""
+    from lakshya import gen
+    lp = gen.random_sparse(m, n, 6, seed=seed)
+    st, obj, ms = C.c_int(), C.c_double(), C.c_double()
+    x = np.zeros(n)
+    assert ref.ref_phase1_solve(m, n, lp.indptr.astype(np.int32), lp.indices.astype(np.int32),
+                                lp.data, lp.c, lp.b, C.byref(st), C.byref(obj), x, C.byref(ms)) == 0
+    assert st.value == 0                                     # Optimal
+    r = oracle.ipm_solve(lp.csr(), lp.b, lp.c, n, stop_rule=1)
+    assert r["status"] == 0
+    assert abs(obj.value - r["objective"]) <= 1e-8 * max(1.0, abs(r["objective"])) * 10
+    A = lp.to_scipy()
+    assert np.abs(A @ x - lp.b).max() <= 1e-7 * (1 + np.abs(lp.b).max())
+    assert x.min() >= -1e-9
