@@ -243,6 +243,18 @@ int lk_mlp_train(lk_mlp* h, int32_t epochs, int32_t batch, float lr, uint64_t se
                  double* epoch_loss);
 int lk_mlp_destroy(lk_mlp* h);
 
+/* ---- MPS input: parsers::read_mps (src/parsers/mps_reader.hpp:8-20, mps_reader.cpp:65-355) ----
+ * lk_mps_open parses the file (L / G rows get slack / surplus columns after the structural
+ * ones; the model is in equality form, Minimize) and returns its sizes; lk_mps_get copies it
+ * out -- A in CSC (colptr n + 1, rowidx and vals nnz), obj, lb, ub and vtype (0 Continuous,
+ * 1 Integer, 2 Binary) n, rhs m; lk_mps_close frees it.  Parse errors: LK_ERR_RUNTIME with
+ * the message in lk_last_error(). */
+typedef struct lk_mps lk_mps;
+int lk_mps_open(const char* path, lk_mps** out, int32_t* m, int32_t* n, int64_t* nnz);
+int lk_mps_get(lk_mps* h, int32_t* colptr, int32_t* rowidx, double* vals, double* obj,
+               double* rhs, double* lb, double* ub, int32_t* vtype);
+int lk_mps_close(lk_mps* h);
+
 /* ---- verifier::verify_optimal (src/verifier/certificate.hpp:26-31) with a tolerance ---- */
 int lk_verify_optimal(int32_t m, int32_t n, const int32_t* colptr, const int32_t* rowidx,
                       const double* vals, const double* c, const double* b, const double* lb,

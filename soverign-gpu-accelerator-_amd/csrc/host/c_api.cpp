@@ -1,5 +1,6 @@
 // C ABI over the C++ solver classes (include/lakshya.h).  Exceptions never cross the boundary:
 // they become error codes with the message kept per thread for lk_last_error().
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -15,6 +16,7 @@
 #include "ipm/symbolic.hpp"
 #include "lakshya.h"
 #include "learn/mlp.hpp"
+#include "parsers/mps_reader.hpp"
 #include "lakshya_device.h"
 #include "verifier/certificate.hpp"
 
@@ -680,6 +682,43 @@ int lk_verify_optimal(int32_t m, int32_t n, const int32_t* colptr, const int32_t
                     ? 1 : 0;
         if (primal_residual) *primal_residual = res;
     });
+}
+
+struct lk_mps {
+    core::Model model;
+};
+
+int lk_mps_open(const char* path, lk_mps** out, int32_t* m, int32_t* n, int64_t* nnz) {
+    return guarded([&] {
+        if (!path || !out) throw std::invalid_argument("lk_mps_open: null argument");
+        std::unique_ptr<lk_mps> h(new lk_mps{parsers::read_mps(path)});
+        if (m) *m = h->model.A.rows;
+        if (n) *n = h->model.A.cols;
+        if (nnz) *nnz = static_cast<int64_t>(h->model.A.values.size());
+        *out = h.release();
+    });
+}
+
+int lk_mps_get(lk_mps* h, int32_t* colptr, int32_t* rowidx, double* vals, double* obj,
+               double* rhs, double* lb, double* ub, int32_t* vtype) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("lk_mps_get: null handle");
+        const core::Model& md = h->model;
+        const auto& A = md.A;
+        if (colptr) std::copy(A.col_ptrs.begin(), A.col_ptrs.end(), colptr);
+        if (rowidx) std::copy(A.row_indices.begin(), A.row_indices.end(), rowidx);
+        if (vals) std::copy(A.values.begin(), A.values.end(), vals);
+        if (obj) std::copy(md.obj.begin(), md.obj.end(), obj);
+        if (rhs) std::copy(md.rhs.begin(), md.rhs.end(), rhs);
+        if (lb) std::copy(md.lb.begin(), md.lb.end(), lb);
+        if (ub) std::copy(md.ub.begin(), md.ub.end(), ub);
+        if (vtype)
+            for (size_t j = 0; j < md.vtype.size(); ++j) vtype[j] = static_cast<int32_t>(md.vtype[j]);
+    });
+}
+
+int lk_mps_close(lk_mps* h) {
+    return guarded([&] { delete h; });
 }
 
 }  // extern "C"

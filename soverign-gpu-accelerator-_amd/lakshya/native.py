@@ -151,6 +151,10 @@ def lib():
         "lk_mlp_set_dataset": [_vp, _f32p, _f32p, C.c_int32],
         "lk_mlp_train": [_vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _f64p],
         "lk_mlp_destroy": [_vp],
+        "lk_mps_open": [C.c_char_p, C.POINTER(_vp), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                        C.POINTER(C.c_int64)],
+        "lk_mps_get": [_vp, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _i32p],
+        "lk_mps_close": [_vp],
         "lk_verify_optimal": [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _vp, _vp,
                               C.c_int32, _f64p, _f64p, C.c_double, C.POINTER(C.c_double),
                               C.POINTER(C.c_int32)],
@@ -548,6 +552,26 @@ def batch_solve_noalloc(lps, **opts):
 
 def last_error() -> str:
     return lib().lk_last_error().decode(errors="replace")
+
+
+# ------------------------------------------------------------------ MPS input
+def read_mps(path):
+    """parsers::read_mps: the file as an equality-form LP (lakshya.gen.LP, CSC); meta holds
+    lb, ub and vtype (0 Continuous, 1 Integer, 2 Binary).  Parse errors raise RuntimeError."""
+    from .gen import LP
+    h = _vp()
+    m, n, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+    _ck(lib().lk_mps_open(str(path).encode(), C.byref(h), C.byref(m), C.byref(n), C.byref(nnz)))
+    try:
+        m, n, nnz = m.value, n.value, nnz.value
+        colptr, rowidx = np.zeros(n + 1, np.int32), np.zeros(max(nnz, 1), np.int32)
+        vals, obj, rhs = np.zeros(max(nnz, 1)), np.zeros(max(n, 1)), np.zeros(max(m, 1))
+        lb, ub, vt = np.zeros(max(n, 1)), np.zeros(max(n, 1)), np.zeros(max(n, 1), np.int32)
+        _ck(lib().lk_mps_get(h, colptr, rowidx, vals, obj, rhs, lb, ub, vt))
+    finally:
+        lib().lk_mps_close(h)
+    return LP(m, n, colptr, rowidx[:nnz], vals[:nnz], rhs[:m], obj[:n], str(path),
+              {"lb": lb[:n], "ub": ub[:n], "vtype": vt[:n]})
 
 
 # ------------------------------------------------------------------ warm-start network
