@@ -434,10 +434,13 @@ int lkd_graph_destroy(void* exec);
 /* C(m, n) = alpha sum_k A(m, k) B(n, k) + beta C(m, n) (+ bias[n]; then 0 where
  * mask[m ldc + n] <= 0), A(m, k) = A[m sam + k sak], B(n, k) = B[n sbn + k sbk], C(m, n) =
  * C[m ldc + n]; relu_a / relu_b read that operand as max(., 0).  Operands are rounded to BF16
- * and multiplied on MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation. */
+ * and multiplied on MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation.  work (optional,
+ * work_elems floats): a product with few output tiles and a long K is split over K into it and
+ * the splits are added in a fixed order (deterministic). */
 int lkd_mlp_gemm(int M, int N, int K, const float* A, int64_t sam, int64_t sak, const float* B,
                  int64_t sbn, int64_t sbk, float* C, int64_t ldc, float alpha, float beta,
-                 const float* bias, const float* mask, int relu_a, int relu_b, void* stream);
+                 const float* bias, const float* mask, int relu_a, int relu_b, float* work,
+                 int64_t work_elems, void* stream);
 /* out[n] = beta out[n] + sum_m G[m ld + n] (rows in order) */
 int lkd_mlp_colsum(const float* G, int M, int N, int64_t ld, float beta, float* out, void* stream);
 /* g = scale (Y - T) over `count` entries; *loss = sum (Y - T)^2 (block partials in `part`,

@@ -6,11 +6,13 @@
 //   * column sums (bias gradients), the squared-error loss and its gradient, row gathers of a
 //     device-resident data set, and the Adam update on FP32 master weights.
 // All reductions run in a fixed order, so a training step is deterministic.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace {
 
-constexpr int BM = 64, BN = 64, BK = 32, KP = BK + 8;   // tile, K chunk, LDS pitch (bf16)
+constexpr int BM = 64, BN = 64, BK = 64, KP = BK + 8;   // tile, K chunk, LDS pitch (bf16)
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -33,43 +35,56 @@ struct GemmArgs {
     const float* bias;         // bias[n] added (nullable)
     const float* mask;         // C(m, n) zeroed where mask[m * ldc + n] <= 0 (nullable)
     int relu_a, relu_b;        // operand read as max(., 0)
+    float* part;               // split K (gridDim.z > 1): raw sums of split z at part[z M N + m N + n]
+    int kchunk;                // K columns per split (a multiple of BK)
 };
 
-// One 64 x 64 tile of C per workgroup; wave w computes rows 32 (w / 2) .., columns 32 (w % 2) ..
-// as 2 x 2 blocks of 16 x 16.  Operand element e of a 64 x 32 chunk goes to thread e % 256:
-// along k when k is the unit stride, along the row otherwise, so global loads coalesce.
+// One 64 x 64 tile of C per workgroup (split z of K: columns z kchunk ..); wave w computes rows
+// 32 (w / 2) .., columns 32 (w % 2) .. as 2 x 2 blocks of 16 x 16.  Operand element e of a
+// 64 x 64 chunk goes to thread e % 256: along k when k is the unit stride, along the row
+// otherwise, so global loads coalesce; all 16 loads of an operand are issued before the LDS
+// stores.  Without a split the epilogue (alpha, beta, bias, mask) is applied here.
 __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
     __shared__ unsigned short As[BM * KP];
     __shared__ unsigned short Bs[BN * KP];
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
     const int wm = 32 * (w >> 1), wn = 32 * (w & 1);
+    const int kb0 = blockIdx.z * g.kchunk;
+    const int kb1 = min(g.K, kb0 + g.kchunk);
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
     const bool a_k = g.sak == 1, b_k = g.sbk == 1;
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
+    for (int k0 = kb0; k0 < kb1; k0 += BK) {
+        float va[(BM * BK) / 256], vb[(BN * BK) / 256];
 #pragma unroll
         for (int j = 0; j < (BM * BK) / 256; ++j) {
             const int e = t + 256 * j;
             const int r = a_k ? e / BK : e % BM, k = a_k ? e % BK : e / BM;
             const int gm = m0 + r, gk = k0 + k;
-            float v = 0.0f;
-            if (gm < g.M && gk < g.K) v = g.A[gm * g.sam + gk * g.sak];
-            if (g.relu_a) v = fmaxf(v, 0.0f);
-            As[r * KP + k] = to_bf16(v);
+            va[j] = (gm < g.M && gk < kb1) ? g.A[gm * g.sam + gk * g.sak] : 0.0f;
         }
 #pragma unroll
         for (int j = 0; j < (BN * BK) / 256; ++j) {
             const int e = t + 256 * j;
             const int r = b_k ? e / BK : e % BN, k = b_k ? e % BK : e / BN;
             const int gn = n0 + r, gk = k0 + k;
-            float v = 0.0f;
-            if (gn < g.N && gk < g.K) v = g.B[gn * g.sbn + gk * g.sbk];
-            if (g.relu_b) v = fmaxf(v, 0.0f);
-            Bs[r * KP + k] = to_bf16(v);
+            vb[j] = (gn < g.N && gk < kb1) ? g.B[gn * g.sbn + gk * g.sbk] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < (BM * BK) / 256; ++j) {
+            const int e = t + 256 * j;
+            const int r = a_k ? e / BK : e % BM, k = a_k ? e % BK : e / BM;
+            As[r * KP + k] = to_bf16(g.relu_a ? fmaxf(va[j], 0.0f) : va[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < (BN * BK) / 256; ++j) {
+            const int e = t + 256 * j;
+            const int r = b_k ? e / BK : e % BN, k = b_k ? e % BK : e / BN;
+            Bs[r * KP + k] = to_bf16(g.relu_b ? fmaxf(vb[j], 0.0f) : vb[j]);
         }
         __syncthreads();
 #pragma unroll
@@ -101,6 +116,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
                 const int m = m0 + wm + 16 * i + 4 * (l >> 4) + e;
                 const int n = n0 + wn + 16 * j + (l & 15);
                 if (m >= g.M || n >= g.N) continue;
+                if (gridDim.z > 1) {
+                    g.part[((int64_t)blockIdx.z * g.M + m) * g.N + n] = acc[i][j][e];
+                    continue;
+                }
                 float* cp = g.C + m * g.ldc + n;
                 float v = g.alpha * acc[i][j][e];
                 if (g.beta != 0.0f) v += g.beta * *cp;
@@ -108,6 +127,22 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
                 if (g.mask && !(g.mask[m * g.ldc + n] > 0.0f)) v = 0.0f;
                 *cp = v;
             }
+}
+
+// split-K epilogue: C(m, n) = alpha sum_z part[z][m][n] (z in order) + beta C + bias, mask
+__global__ __launch_bounds__(256) void k_gemm_splitk_epi(GemmArgs g, int nsplit) {
+    const int64_t total = (int64_t)g.M * g.N;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < total; i += (int64_t)gridDim.x * 256) {
+        const int m = (int)(i / g.N), n = (int)(i - (int64_t)m * g.N);
+        float s = g.part[i];
+        for (int z = 1; z < nsplit; ++z) s += g.part[(int64_t)z * total + i];
+        float* cp = g.C + m * g.ldc + n;
+        float v = g.alpha * s;
+        if (g.beta != 0.0f) v += g.beta * *cp;
+        if (g.bias) v += g.bias[n];
+        if (g.mask && !(g.mask[m * g.ldc + n] > 0.0f)) v = 0.0f;
+        *cp = v;
+    }
 }
 
 // out[n] = beta out[n] + sum_m G[m * ld + n]: a workgroup takes 32 columns, its 8 row groups
@@ -209,12 +244,29 @@ extern "C" {
 
 int lkd_mlp_gemm(int M, int N, int K, const float* A, int64_t sam, int64_t sak, const float* B,
                  int64_t sbn, int64_t sbk, float* C, int64_t ldc, float alpha, float beta,
-                 const float* bias, const float* mask, int relu_a, int relu_b, void* stream) {
+                 const float* bias, const float* mask, int relu_a, int relu_b, float* work,
+                 int64_t work_elems, void* stream) {
     if (M <= 0 || N <= 0) return 0;
     if (K < 0 || !A || !B || !C) return (int)hipErrorInvalidValue;
-    const GemmArgs g{M, N, K, A, sam, sak, B, sbn, sbk, C, ldc, alpha, beta, bias, mask, relu_a, relu_b};
-    hipLaunchKernelGGL(k_gemm_bf16, dim3((N + BN - 1) / BN, (M + BM - 1) / BM), dim3(256), 0,
+    const int tiles = ((N + BN - 1) / BN) * ((M + BM - 1) / BM);
+    // few tiles and a long K: split K so that about 256 workgroups run, at least 2 chunks each,
+    // within the workspace; the partial sums are added in split order by the epilogue kernel
+    int ns = 1;
+    if (work && tiles < 192 && K >= 4 * BK) {
+        ns = std::min({(256 + tiles - 1) / tiles, K / (2 * BK), 16});
+        while (ns > 1 && (int64_t)ns * M * N > work_elems) --ns;
+    }
+    const int kchunk = ns > 1 ? ((K + ns - 1) / ns + BK - 1) / BK * BK : (K > 0 ? K : 1);
+    ns = ns > 1 ? (K + kchunk - 1) / kchunk : 1;
+    const GemmArgs g{M, N, K, A, sam, sak, B, sbn, sbk, C, ldc, alpha, beta, bias, mask, relu_a, relu_b,
+                     work, kchunk};
+    hipLaunchKernelGGL(k_gemm_bf16, dim3((N + BN - 1) / BN, (M + BM - 1) / BM, ns), dim3(256), 0,
                        (hipStream_t)stream, g);
+    if (ns > 1) {
+        const int64_t total = (int64_t)M * N;
+        const int blocks = (int)std::min<int64_t>((total + 255) / 256, 2048);
+        hipLaunchKernelGGL(k_gemm_splitk_epi, dim3(blocks), dim3(256), 0, (hipStream_t)stream, g, ns);
+    }
     return lk_last_error();
 }
 

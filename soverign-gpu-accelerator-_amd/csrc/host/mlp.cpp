@@ -39,6 +39,8 @@ struct WarmStartMLP::Impl {
     float *P = nullptr, *G = nullptr, *Mo = nullptr, *Vo = nullptr;
     float *U = nullptr, *Z1 = nullptr, *Z2 = nullptr, *Y = nullptr, *T = nullptr, *GY = nullptr;
     float *G2 = nullptr, *G1 = nullptr, *part = nullptr, *loss = nullptr;
+    float* W = nullptr;                      // split-K partial sums of the products
+    int64_t work_elems = 0;
     int32_t* idx = nullptr;
     float *DU = nullptr, *DT = nullptr;      // device-resident training set
     int ndata = 0;
@@ -56,7 +58,11 @@ struct WarmStartMLP::Impl {
         ob3 = oW3 + int64_t(d_out) * h2;
         np = ob3 + d_out;
         const size_t act = size_t(mb) * (size_t(d_in) + 2 * h1 + 2 * h2 + 3 * size_t(d_out));
-        const size_t bytes = sizeof(float) * (4 * size_t(np) + act + kParts + 64) +
+        // split-K workspace: up to 16 splits of the largest product output
+        const int64_t big = std::max({int64_t(mb) * h1, int64_t(mb) * h2, int64_t(mb) * dout,
+                                      int64_t(dout) * h2, int64_t(h2) * h1, int64_t(h1) * di});
+        work_elems = std::min<int64_t>(16 * big, int64_t(1) << 26);
+        const size_t bytes = sizeof(float) * (4 * size_t(np) + act + kParts + 64 + size_t(work_elems)) +
                              sizeof(int32_t) * mb + (16u << 10);
         arena.reset(new gpu::VRAMArena(bytes + (1u << 20)));
         check(lkd_stream_create(&stream), "stream");
@@ -66,6 +72,7 @@ struct WarmStartMLP::Impl {
         Y = f(int64_t(mb) * d_out); T = f(int64_t(mb) * d_out); GY = f(int64_t(mb) * d_out);
         G2 = f(int64_t(mb) * h2); G1 = f(int64_t(mb) * h1); part = f(kParts); loss = f(16);
         idx = static_cast<int32_t*>(arena->allocate(sizeof(int32_t) * mb));
+        W = static_cast<float*>(arena->allocate_uninitialized(sizeof(float) * size_t(work_elems)));
         // He initialization (normal, std sqrt(2 / fan_in)), biases zero
         std::vector<float> p(static_cast<size_t>(np), 0.0f);
         std::mt19937_64 rng(seed);
@@ -98,7 +105,7 @@ struct WarmStartMLP::Impl {
               int64_t sbn, int64_t sbk, float* C, int64_t ldc, const float* bias, const float* mask,
               int relu_a, int relu_b) {
         check(lkd_mlp_gemm(M, N, K, A, sam, sak, B, sbn, sbk, C, ldc, 1.0f, 0.0f, bias, mask, relu_a,
-                           relu_b, stream), "gemm");
+                           relu_b, W, work_elems, stream), "gemm");
     }
 
     // forward on the batch in U (rows B)
