@@ -13,6 +13,7 @@
 #include <numeric>
 #include <vector>
 #include "ipm/symbolic.hpp"
+#include "parallel.hpp"
 
 namespace sankhya {
 namespace ipm {
@@ -23,29 +24,39 @@ std::vector<Index> compute_fill_reducing_ordering(const core::CSRMatrix& A) {
     order.reserve(static_cast<size_t>(m));
     if (m == 0) return order;
 
-    // ---- graph of A*A^T (no self loops) ----
-    std::vector<std::vector<Index>> by_col(static_cast<size_t>(A.cols));
-    for (Index i = 0; i < m; ++i)
-        for (Index p = A.row_ptrs[i]; p < A.row_ptrs[i + 1]; ++p)
-            by_col[static_cast<size_t>(A.col_indices[p])].push_back(i);
+    // ---- graph of A*A^T (no self loops): row i's neighbours are the rows of the columns it
+    //      touches, deduplicated with a per-thread marker and sorted (rows in parallel) ----
     std::vector<std::vector<Index>> avar(static_cast<size_t>(m));   // variable adjacency
     {
-        std::vector<int64_t> cnt(static_cast<size_t>(m), 0);
-        for (const auto& r : by_col)
-            for (Index i : r) cnt[i] += static_cast<int64_t>(r.size()) - 1;
-        for (Index i = 0; i < m; ++i) avar[i].reserve(static_cast<size_t>(cnt[i]));
-        for (const auto& r : by_col)
-            for (size_t a = 0; a < r.size(); ++a)
-                for (size_t b = 0; b < r.size(); ++b)
-                    if (a != b) avar[static_cast<size_t>(r[a])].push_back(r[b]);
-        for (auto& v : avar) {
-            std::sort(v.begin(), v.end());
-            v.erase(std::unique(v.begin(), v.end()), v.end());
-            v.shrink_to_fit();
+        const Index nc = A.cols;
+        std::vector<Index> cptr(static_cast<size_t>(nc) + 1, 0);
+        for (Index p = 0; p < A.row_ptrs[m]; ++p) ++cptr[static_cast<size_t>(A.col_indices[p]) + 1];
+        for (Index j = 0; j < nc; ++j) cptr[static_cast<size_t>(j) + 1] += cptr[static_cast<size_t>(j)];
+        std::vector<Index> crow(static_cast<size_t>(A.row_ptrs[m]));
+        {
+            std::vector<Index> fill(cptr.begin(), cptr.end() - 1);
+            for (Index i = 0; i < m; ++i)
+                for (Index p = A.row_ptrs[i]; p < A.row_ptrs[i + 1]; ++p)
+                    crow[static_cast<size_t>(fill[static_cast<size_t>(A.col_indices[p])]++)] = i;
         }
+        host::parallel_rows(m, [&] {
+            return [&, mark = std::vector<Index>(static_cast<size_t>(m), -1)](int64_t ii) mutable {
+                const Index i = static_cast<Index>(ii);
+                auto& v = avar[static_cast<size_t>(i)];
+                for (Index p = A.row_ptrs[i]; p < A.row_ptrs[i + 1]; ++p) {
+                    const Index j = A.col_indices[p];
+                    for (Index q = cptr[static_cast<size_t>(j)]; q < cptr[static_cast<size_t>(j) + 1]; ++q) {
+                        const Index r = crow[static_cast<size_t>(q)];
+                        if (r == i || mark[static_cast<size_t>(r)] == i) continue;
+                        mark[static_cast<size_t>(r)] = i;
+                        v.push_back(r);
+                    }
+                }
+                std::sort(v.begin(), v.end());
+                v.shrink_to_fit();
+            };
+        });
     }
-    by_col.clear();
-    by_col.shrink_to_fit();
 
     // ---- state ----
     enum : char { VAR = 0, ELEMENT = 1, DEAD = 2, MERGED = 3, DENSE = 4 };

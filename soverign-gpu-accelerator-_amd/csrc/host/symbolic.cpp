@@ -12,6 +12,7 @@
 #include <numeric>
 #include <stdexcept>
 #include "ipm/symbolic.hpp"
+#include "parallel.hpp"
 
 namespace sankhya {
 namespace ipm {
@@ -31,14 +32,14 @@ std::vector<std::vector<Index>> rows_by_column(const core::CSRMatrix& A) {
 void permuted_lower_graph(const core::CSRMatrix& A, const std::vector<Index>& pinv,
                           std::vector<Index>& ptr, std::vector<Index>& idx) {
     const Index m = A.rows;
-    // Count with duplicates first, then compact; duplicates come from several shared columns.
-    std::vector<int64_t> cnt(static_cast<size_t>(m) + 1, 0);
-    std::vector<Index> cstart(static_cast<size_t>(A.cols) + 1, 0);
-    std::vector<Index> crow;
+    // column lists (original rows), then per permuted row r (original row i = P[r]) the permuted
+    // rows below r that share a column, deduplicated with a per-thread marker and sorted; rows
+    // in parallel, concatenated in order
+    std::vector<Index> cstart(static_cast<size_t>(A.cols) + 1, 0), crow, P(static_cast<size_t>(m));
+    for (Index i = 0; i < m; ++i) P[static_cast<size_t>(pinv[static_cast<size_t>(i)])] = i;
     {
-        std::vector<Index> c(static_cast<size_t>(A.cols), 0);
-        for (Index p = 0; p < A.row_ptrs[m]; ++p) c[static_cast<size_t>(A.col_indices[p])]++;
-        for (Index k = 0; k < A.cols; ++k) cstart[k + 1] = cstart[k] + c[k];
+        for (Index p = 0; p < A.row_ptrs[m]; ++p) cstart[static_cast<size_t>(A.col_indices[p]) + 1]++;
+        for (Index k = 0; k < A.cols; ++k) cstart[k + 1] += cstart[k];
         crow.resize(static_cast<size_t>(cstart[A.cols]));
         std::vector<Index> nxt(cstart.begin(), cstart.end() - 1);
         for (Index i = 0; i < m; ++i)
@@ -46,33 +47,33 @@ void permuted_lower_graph(const core::CSRMatrix& A, const std::vector<Index>& pi
                 crow[static_cast<size_t>(nxt[static_cast<size_t>(A.col_indices[p])]++)] =
                     pinv[static_cast<size_t>(i)];
     }
-    for (Index k = 0; k < A.cols; ++k)
-        for (Index a = cstart[k]; a < cstart[k + 1]; ++a)
-            for (Index b = cstart[k]; b < cstart[k + 1]; ++b) {
-                const Index ra = crow[a], rb = crow[b];
-                if (rb < ra) cnt[static_cast<size_t>(ra) + 1]++;
-            }
-    for (Index i = 0; i < m; ++i) cnt[i + 1] += cnt[i];
-    std::vector<Index> raw(static_cast<size_t>(cnt[m]));
-    {
-        std::vector<int64_t> nxt(cnt.begin(), cnt.end() - 1);
-        for (Index k = 0; k < A.cols; ++k)
-            for (Index a = cstart[k]; a < cstart[k + 1]; ++a)
-                for (Index b = cstart[k]; b < cstart[k + 1]; ++b) {
-                    const Index ra = crow[a], rb = crow[b];
-                    if (rb < ra) raw[static_cast<size_t>(nxt[static_cast<size_t>(ra)]++)] = rb;
+    std::vector<std::vector<Index>> rows(static_cast<size_t>(m));
+    host::parallel_rows(m, [&] {
+        return [&, mark = std::vector<Index>(static_cast<size_t>(m), -1)](int64_t rr) mutable {
+            const Index r = static_cast<Index>(rr), i = P[static_cast<size_t>(r)];
+            auto& v = rows[static_cast<size_t>(r)];
+            for (Index p = A.row_ptrs[i]; p < A.row_ptrs[i + 1]; ++p) {
+                const Index k = A.col_indices[p];
+                for (Index a = cstart[k]; a < cstart[k + 1]; ++a) {
+                    const Index rb = crow[static_cast<size_t>(a)];
+                    if (rb >= r || mark[static_cast<size_t>(rb)] == r) continue;
+                    mark[static_cast<size_t>(rb)] = r;
+                    v.push_back(rb);
                 }
-    }
+            }
+            std::sort(v.begin(), v.end());
+        };
+    });
     ptr.assign(static_cast<size_t>(m) + 1, 0);
-    idx.clear();
-    idx.reserve(raw.size());
-    for (Index i = 0; i < m; ++i) {
-        auto b = raw.begin() + cnt[i], e = raw.begin() + cnt[i + 1];
-        std::sort(b, e);
-        auto u = std::unique(b, e);
-        idx.insert(idx.end(), b, u);
-        ptr[i + 1] = static_cast<Index>(idx.size());
-    }
+    for (Index r = 0; r < m; ++r) ptr[r + 1] = ptr[r] + static_cast<Index>(rows[static_cast<size_t>(r)].size());
+    idx.resize(static_cast<size_t>(ptr[m]));
+    host::parallel_rows(m, [&] {
+        return [&](int64_t r) {
+            std::copy(rows[static_cast<size_t>(r)].begin(), rows[static_cast<size_t>(r)].end(),
+                      idx.begin() + ptr[static_cast<size_t>(r)]);
+            std::vector<Index>().swap(rows[static_cast<size_t>(r)]);
+        };
+    });
 }
 
 // Liu's elimination-tree algorithm with path compression over the lower rows.
