@@ -19,6 +19,7 @@
 #include <map>
 #include <string>
 #include "gpu/kkt.hpp"
+#include "parallel.hpp"
 #include "lakshya_device.h"
 
 namespace sankhya {
@@ -327,22 +328,51 @@ struct GPUKKTCholeskySolver::Impl {
 
     void build_pairs() {
         struct Pair { int64_t dst; int32_t a, b, k; };
-        std::vector<Pair> pairs;
-        int64_t total = 0;
+        // The pairs of every column k of A in (a, b) order, at off[k] (columns in parallel); then
+        // a stable counting sort by destination column of L and, per column, a stable sort by
+        // panel slot.  A column's slots form one contiguous range, increasing with the column,
+        // so the result is the stable order by slot of the generation order.
+        std::vector<int64_t> off(static_cast<size_t>(n) + 1, 0);
         for (Index k = 0; k < n; ++k) {
             const int64_t c = h_colptr[k + 1] - h_colptr[k];
-            total += c * (c + 1) / 2;
+            off[static_cast<size_t>(k) + 1] = off[static_cast<size_t>(k)] + c * (c + 1) / 2;
         }
-        pairs.reserve(static_cast<size_t>(total));
-        for (Index k = 0; k < n; ++k)
-            for (Index a = h_colptr[k]; a < h_colptr[k + 1]; ++a)
-                for (Index b = h_colptr[k]; b < h_colptr[k + 1]; ++b) {
-                    const Index i = h_prow[a], j = h_prow[b];
-                    if (i < j) continue;
-                    pairs.push_back({panel_index(i, j), a, b, k});
-                }
-        std::stable_sort(pairs.begin(), pairs.end(),
-                         [](const Pair& x, const Pair& y) { return x.dst < y.dst; });
+        const int64_t total = off[static_cast<size_t>(n)];
+        std::vector<Pair> gen(static_cast<size_t>(total));
+        std::vector<Index> dcol(static_cast<size_t>(total));
+        host::parallel_rows(n, [&] {
+            return [&](int64_t kk) {
+                const Index k = static_cast<Index>(kk);
+                int64_t q = off[static_cast<size_t>(k)];
+                for (Index a = h_colptr[k]; a < h_colptr[k + 1]; ++a)
+                    for (Index b = h_colptr[k]; b < h_colptr[k + 1]; ++b) {
+                        const Index i = h_prow[a], j = h_prow[b];
+                        if (i < j) continue;
+                        gen[static_cast<size_t>(q)] = {panel_index(i, j), a, b, k};
+                        dcol[static_cast<size_t>(q)] = j;
+                        ++q;
+                    }
+            };
+        });
+        std::vector<int64_t> cstart(static_cast<size_t>(m) + 1, 0);
+        for (Index j : dcol) ++cstart[static_cast<size_t>(j) + 1];
+        for (Index j = 0; j < m; ++j) cstart[static_cast<size_t>(j) + 1] += cstart[static_cast<size_t>(j)];
+        std::vector<Pair> pairs(static_cast<size_t>(total));
+        {
+            std::vector<int64_t> nxt(cstart.begin(), cstart.end() - 1);
+            for (int64_t q = 0; q < total; ++q)
+                pairs[static_cast<size_t>(nxt[static_cast<size_t>(dcol[static_cast<size_t>(q)])]++)] =
+                    gen[static_cast<size_t>(q)];
+        }
+        std::vector<Pair>().swap(gen);
+        std::vector<Index>().swap(dcol);
+        host::parallel_rows(m, [&] {
+            return [&](int64_t j) {
+                std::stable_sort(pairs.begin() + cstart[static_cast<size_t>(j)],
+                                 pairs.begin() + cstart[static_cast<size_t>(j) + 1],
+                                 [](const Pair& x, const Pair& y) { return x.dst < y.dst; });
+            };
+        });
         npairs = static_cast<int64_t>(pairs.size());
         std::vector<int64_t> segptr, segdst;
         std::vector<int32_t> segi, segj, pk(pairs.size());

@@ -1,11 +1,14 @@
 // Host-side row parallelism for the symbolic phase: rows 0 .. n-1 are dealt to up to 16 threads
 // in chunks from a shared counter.  make() runs once per thread and returns that thread's row
 // function (so per-thread scratch, such as a marker array, lives in its closure).  Results must
-// not depend on which thread handles a row.
+// not depend on which thread handles a row.  An exception in a row stops that thread; the first
+// one is rethrown after all threads have joined.
 #pragma once
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
+#include <exception>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -29,9 +32,21 @@ void parallel_rows(int64_t n, Make&& make, int64_t chunk = 256) {
         run();
         return;
     }
+    std::exception_ptr first;
+    std::mutex mu;
     std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t) th.emplace_back(run);
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            try {
+                run();
+            } catch (...) {
+                std::lock_guard<std::mutex> lock(mu);
+                if (!first) first = std::current_exception();
+                next.store(n);                      // the other threads stop at their next chunk
+            }
+        });
     for (auto& x : th) x.join();
+    if (first) std::rethrow_exception(first);
 }
 
 }  // namespace host
