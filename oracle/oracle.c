@@ -25,7 +25,9 @@
  * out of it, tries dual-only steps of the Mehrotra direction before the centred one (the
  * primal residual cannot grow; near the optimum the normal equations limit the primal
  * accuracy of a direction), and takes the longest finite, positive centred trial when all
- * merit searches fail (non-monotone fallback): the reference's absolute 1e-9 acceptance and
+ * merit searches fail (non-monotone fallback, never leaving a met residual target), and, once
+ * both residuals are inside their targets, lets the dual-only trials compete with a corrector
+ * step shorter than SHORT_STEP (better merit wins): the reference's absolute 1e-9 acceptance and
  * monotone merit search stall before a 1e-8 gap (its own loop stops at 1e-6).  The device path
  * implements the same rules.  Sums use the reference's term order; multiply-adds use fma() so the assembled
  * M0 is bit-identical to an fma-based device kernel with the same order.
@@ -44,6 +46,7 @@
 /* relative-gap mode: the absolute primal, dual and complementarity targets are this fraction of
  * gap_tol (the FP64 certificate checks them absolutely at gap_tol) */
 #define REL_ABS_MARGIN 0.9
+#define SHORT_STEP 1e-2
 
 #define OR_OK 0
 #define OR_NOT_SPD 1
@@ -649,7 +652,7 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
             }                                                                     \
         } while (0)
         int accepted = 0, route = -1;
-        double tr_ap = 0, tr_ad = 0;
+        double tr_ap = 0, tr_ad = 0, acc_merit = 0;
         for (int dir = corr_valid ? 0 : 1; dir < 2 && !accepted; ++dir) {
             if (dir == 1) {
                 for (int j = 0; j < n; ++j) rxs[j] = -x[j] * s[j] + 0.5 * mu;
@@ -680,16 +683,27 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                 int keeps;
                 KEEPS_TARGETS(keeps);
                 prog = prog && keeps;
-                if (finite && pos && prog) { accepted = 1; route = dir; tr_ap = ap; tr_ad = ad; break; }
+                if (finite && pos && prog) { accepted = 1; route = dir; tr_ap = ap; tr_ad = ad; acc_merit = merit; break; }
                 if (relmode && dir == 1 && finite && pos && keeps && fallback_bt < 0) fallback_bt = bt;
                 ap *= 0.5;
                 ad *= 0.5;
             }
-            if (!accepted && relmode && dir == 0) {
+            /* relative-gap mode: a corrector step shorter than SHORT_STEP taken while both
+             * residuals are inside their targets competes with the dual-only trials (the better
+             * merit wins) */
+            const int short_step = relmode && dir == 0 && accepted && fmin(tr_ap, tr_ad) < SHORT_STEP &&
+                                   rpi0 <= ptar && rdi0 <= dtar;
+            if (short_step) {        /* keep the corrector's point in case the dual-only loses */
+                memcpy(tx, bx, sizeof(double) * n);
+                memcpy(ts, bs, sizeof(double) * n);
+                memcpy(r1, by, sizeof(double) * m);
+            }
+            if ((!accepted || short_step) && relmode && dir == 0) {
                 /* relative-gap mode: dual-only trials of the same direction (the primal point,
                  * and with it the primal residual, stays), ad halving from 0.995 of its bound */
                 double ad2 = fmin(1.0, 0.995 * step_len(n, s, ds));
-                for (int bt = 0; bt < 15 && !accepted; ++bt) {
+                int took = 0;
+                for (int bt = 0; bt < 15 && !took; ++bt) {
                     for (int j = 0; j < n; ++j) { bx[j] = x[j]; bs[j] = s[j] + ad2 * ds[j]; }
                     for (int i = 0; i < m; ++i) by[i] = y[i] + ad2 * dy[i];
                     RESID(bx, by, bs, brp, brd);
@@ -702,8 +716,14 @@ int or_ipm_solve(int m, int n, const int* rp_in, const int* ci, const double* va
                     int prog = merit < merit0 && merit <= (1.0 - 1e-4 * ad2) * merit0;
                     int keeps;
                     KEEPS_TARGETS(keeps);
-                    if (finite && pos && prog && keeps) { accepted = 1; route = 3; tr_ad = ad2; }
-                    else ad2 *= 0.5;
+                    if (finite && pos && prog && keeps && (!short_step || merit < acc_merit)) {
+                        took = accepted = 1; route = 3; tr_ap = 0; tr_ad = ad2;
+                    } else ad2 *= 0.5;
+                }
+                if (short_step && !took) {
+                    memcpy(bx, tx, sizeof(double) * n);
+                    memcpy(bs, ts, sizeof(double) * n);
+                    memcpy(by, r1, sizeof(double) * m);
                 }
             }
             if (!accepted && relmode && dir == 1 && fallback_bt >= 0) {

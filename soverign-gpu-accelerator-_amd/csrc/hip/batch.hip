@@ -963,6 +963,7 @@ __device__ void finish(Ctx& c, const lkb_lp& d, const Res& nr, int status) {
 // and max_j x_j (c - A^T y)_j, a fraction REL_ABS_MARGIN of gap_tol (the FP64 certificate
 // checks them absolutely)
 constexpr double REL_ABS_MARGIN = 0.9;
+constexpr double SHORT_STEP = 1e-2;
 __device__ __forceinline__ double primal_target(const Ctx& c) {
     return fmax(REL_ABS_MARGIN * c.A.gap_tol, 1e-12 * (1.0 + c.st.binf));
 }
@@ -1059,9 +1060,10 @@ __device__ bool iterate(Ctx& c, const lkb_lp& d) {
     // relative-gap mode: a residual already inside its absolute target must stay inside
     const bool keep_p = c.rel && nr.rpinf <= primal_target(c);
     const bool keep_d = c.rel && nr.rdinf <= dual_target(c);
-    // `usable`: finite, positive and keeping the met residual targets (merit aside)
-    auto admissible = [&](const Trial& T, double a, double b, bool& usable) {
-        const double merit = T.nrp / c.st.rp0 + T.nrd / c.st.rd0 + T.mu / c.st.mu0;
+    // `usable`: finite, positive and keeping the met residual targets (merit aside); `merit`
+    // receives the trial's merit
+    auto admissible = [&](const Trial& T, double a, double b, bool& usable, double& merit) {
+        merit = T.nrp / c.st.rp0 + T.nrd / c.st.rd0 + T.mu / c.st.mu0;
         const double stp = fmin(a, b);
         const bool finite = isfinite(T.nrp) && isfinite(T.nrd) && isfinite(T.mu) &&
                             isfinite(merit) && isfinite(merit0) && isfinite(T.mx) &&
@@ -1084,24 +1086,33 @@ __device__ bool iterate(Ctx& c, const lkb_lp& d) {
             ap = ad = fmin(1.0, 0.995 * fmin(dz.apmax, dz.admax));
         }
         const int ntr = dir == 0 ? 15 : 30;
-        double a = ap, b = ad, fa = -1.0, fb = -1.0;
+        double a = ap, b = ad, fa = -1.0, fb = -1.0, acc_merit = 0.0;
         for (int t = 0; t < ntr; ++t) {
             bool usable;
-            if (admissible(trial(c, a, b), a, b, usable)) { accepted = true; break; }
+            double mt;
+            if (admissible(trial(c, a, b), a, b, usable, mt)) { accepted = true; acc_merit = mt; break; }
             if (c.rel && dir == 1 && usable && fa < 0) { fa = a; fb = b; }
             a *= 0.5;
             b *= 0.5;
         }
-        if (!accepted && c.rel && dir == 0) {
+        // relative-gap mode: a corrector step shorter than SHORT_STEP taken with both residuals
+        // inside their targets competes with the dual-only trials (the better merit wins)
+        const bool short_step = accepted && c.rel && dir == 0 && fmin(a, b) < SHORT_STEP && keep_p && keep_d;
+        if ((!accepted || short_step) && c.rel && dir == 0) {
             // relative-gap mode: dual-only trials of the same direction -- the primal point (and
             // its residual) stays; near the optimum the normal equations limit the primal
             // accuracy of a direction, not its dual accuracy
-            a = 0.0;
-            b = fmin(1.0, 0.995 * dc.admax);
+            double b2 = fmin(1.0, 0.995 * dc.admax);
             for (int t = 0; t < 15; ++t) {
                 bool usable;
-                if (admissible(trial(c, a, b), b, b, usable)) { accepted = true; break; }
-                b *= 0.5;
+                double mt;
+                if (admissible(trial(c, 0.0, b2), b2, b2, usable, mt) && (!short_step || mt < acc_merit)) {
+                    accepted = true;
+                    a = 0.0;
+                    b = b2;
+                    break;
+                }
+                b2 *= 0.5;
             }
         }
         if (!accepted && c.rel && dir == 1 && fa > 0) {

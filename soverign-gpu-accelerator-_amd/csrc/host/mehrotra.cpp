@@ -317,6 +317,34 @@ public:
         return true;
     }
 
+    // Relative-gap mode: dual-only trials (x stays; s, y move by b ds, b dy, b halving from
+    // 0.995 of the dual step bound, 15 trials) of the current direction; the first that is finite,
+    // positive, makes Armijo progress on merit0, keeps the met targets and has a merit below
+    // `below` is returned (its index, b in *bd), else -1.
+    template <typename Keeps>
+    int dual_only_trial(double admax, double merit0, double below, const Keeps& keeps, double& bd) {
+        const double ad0 = std::min(1.0, 0.995 * admax);
+        const int ntd = 15;
+        check(lkd_ipm_trials(&P_, x_, s_, y_, dx_, ds_, dy_, rp_, rd_, qp_, qd_, 0.0, ad0, ntd, stream_), "trials");
+        read_scalars(LKS_TRIAL_BASE + LKS_TRIAL_STRIDE * ntd);
+        double b = ad0;
+        for (int t = 0; t < ntd; ++t, b *= 0.5) {
+            const double* T = &hS_[LKS_TRIAL_BASE + t * LKS_TRIAL_STRIDE];
+            const double nrp2 = std::sqrt(T[0]), nrd2 = std::sqrt(T[1]), mu2 = T[2] / n_;
+            const double merit = nrp2 / rp0_ + nrd2 / rd0_ + mu2 / mu0_;
+            const bool finite = std::isfinite(nrp2) && std::isfinite(nrd2) && std::isfinite(mu2) &&
+                                std::isfinite(merit) && std::isfinite(merit0) && std::isfinite(T[4]) &&
+                                std::isfinite(b) && std::isfinite(T[5]);
+            const bool positive = T[4] > 0.0 && mu2 > 0.0 && b > 0.0;
+            const bool progress = merit < merit0 && merit <= (1.0 - 1e-4 * b) * merit0 && keeps(T);
+            if (finite && positive && progress && merit < below) {
+                bd = b;
+                return t;
+            }
+        }
+        return -1;
+    }
+
     bool fp64_fail_ = false;
 
     // factor for the current Theta; fp32 selects the FP32 panel (mixed precision)
@@ -370,6 +398,9 @@ public:
     // relative-gap mode: absolute targets of ||Ax - b||_inf and ||A^T y + s - c||_inf (original
     // space), a fraction kRelAbsMargin of gap_tol (the FP64 certificate checks them absolutely)
     static constexpr double kRelAbsMargin = 0.9;
+    // relative-gap mode: a corrector step below this competes with the dual-only trials
+    static constexpr double kShortStep = 1e-2;
+    double rp0_ = 1, rd0_ = 1, mu0_ = 1;   // merit normalisation of the current solve
     double primal_target() const { return std::max(kRelAbsMargin * opt_.gap_tol, 1e-12 * (1.0 + binf_)); }
     double dual_target() const { return std::max(kRelAbsMargin * opt_.gap_tol, 1e-12 * (1.0 + cinf_)); }
 
@@ -406,6 +437,9 @@ public:
                 rp0 = std::max(1.0, nr.rp);
                 rd0 = std::max(1.0, nr.rd);
                 mu0 = std::max(1.0, nr.mu);
+                rp0_ = rp0;
+                rd0_ = rd0;
+                mu0_ = mu0;
             }
             if (opt_.verbose)
                 std::printf("  Iter %d | rp: %.3e | rd: %.3e | mu: %.3e | obj: %.12e\n", iter,
@@ -484,10 +518,22 @@ public:
                     const bool positive = mx > 0.0 && ms > 0.0 && mu2 > 0.0 && step > 0.0;
                     const bool progress = merit < merit0 && merit <= (1.0 - 1e-4 * step) * merit0 && keeps(T);
                     if (finite && positive && progress) {
-                        check(lkd_ipm_commit(&P_, x_, s_, y_, dx_, ds_, dy_, a, b, stream_), "commit");
+                        // relative-gap mode: a short corrector step with both residuals inside
+                        // their targets competes with the dual-only trials (better merit wins)
+                        double ca = a, cb = b;
+                        if (rel && dir == 0 && step < kShortStep && keep_p && keep_d) {
+                            const int t2 = dual_only_trial(d.admax, merit0, merit, keeps, cb);
+                            if (t2 >= 0) {
+                                ca = 0.0;
+                                if (opt_.verbose) std::printf("    step: dual-only trial %d ad %.3e beats the corrector\n", t2, cb);
+                            } else {
+                                cb = b;
+                            }
+                        }
+                        check(lkd_ipm_commit(&P_, x_, s_, y_, dx_, ds_, dy_, ca, cb, stream_), "commit");
                         accepted = true;
                         if (opt_.verbose)
-                            std::printf("    step: dir %d trial %d ap %.3e ad %.3e sigma %.3e pcg %d\n", dir, t, a, b, sigma, res.refinement_steps);
+                            std::printf("    step: dir %d trial %d ap %.3e ad %.3e sigma %.3e pcg %d\n", dir, t, ca, cb, sigma, res.refinement_steps);
                         break;
                     }
                     if (rel && dir == 1 && finite && positive && keeps(T) && fa < 0) { fa = a; fb = b; }
@@ -498,25 +544,12 @@ public:
                     // relative-gap mode: dual-only trials of the same direction -- the primal
                     // point (and its residual) stays; near the optimum the normal equations
                     // limit the primal accuracy of a direction, not its dual accuracy
-                    const double ad0 = std::min(1.0, 0.995 * d.admax);
-                    const int ntd = 15;
-                    check(lkd_ipm_trials(&P_, x_, s_, y_, dx_, ds_, dy_, rp_, rd_, qp_, qd_, 0.0, ad0, ntd, stream_), "trials");
-                    read_scalars(LKS_TRIAL_BASE + LKS_TRIAL_STRIDE * ntd);
-                    double bd = ad0;
-                    for (int t = 0; t < ntd && !accepted; ++t, bd *= 0.5) {
-                        const double* T = &hS_[LKS_TRIAL_BASE + t * LKS_TRIAL_STRIDE];
-                        const double nrp2 = std::sqrt(T[0]), nrd2 = std::sqrt(T[1]), mu2 = T[2] / n_;
-                        const double merit = nrp2 / rp0 + nrd2 / rd0 + mu2 / mu0;
-                        const bool finite = std::isfinite(nrp2) && std::isfinite(nrd2) && std::isfinite(mu2) &&
-                                            std::isfinite(merit) && std::isfinite(merit0) && std::isfinite(T[4]) &&
-                                            std::isfinite(bd) && std::isfinite(T[5]);
-                        const bool positive = T[4] > 0.0 && mu2 > 0.0 && bd > 0.0;
-                        const bool progress = merit < merit0 && merit <= (1.0 - 1e-4 * bd) * merit0 && keeps(T);
-                        if (finite && positive && progress) {
-                            check(lkd_ipm_commit(&P_, x_, s_, y_, dx_, ds_, dy_, 0.0, bd, stream_), "commit");
-                            accepted = true;
-                            if (opt_.verbose) std::printf("    step: dual-only trial %d ad %.3e\n", t, bd);
-                        }
+                    double bd = 0.0;
+                    const int t = dual_only_trial(d.admax, merit0, HUGE_VAL, keeps, bd);
+                    if (t >= 0) {
+                        check(lkd_ipm_commit(&P_, x_, s_, y_, dx_, ds_, dy_, 0.0, bd, stream_), "commit");
+                        accepted = true;
+                        if (opt_.verbose) std::printf("    step: dual-only trial %d ad %.3e\n", t, bd);
                     }
                 }
                 if (!accepted && rel && dir == 1 && fa > 0) {
