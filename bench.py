@@ -295,6 +295,13 @@ def form_batch(native, count=4096, sample=48, workers=8):
         bt.solve()                                     # first solve: graph capture, warm caches
         sm = bt.solve()
         res = bt.results()
+        if name == "batched_fp64":
+            t = time.perf_counter()
+            xo = bt.crossover()                        # launch + host copies of every output
+            xs = time.perf_counter() - t
+            out["crossover_fp64"] = {"lps_per_s": count / xs, "seconds": xs,
+                                     "optimal_vertices": sum(1 for o in xo if o["status"] == 0),
+                                     "pivots_mean": float(np.mean([o["pivots"] for o in xo]))}
         bt.close()
         it = np.array([r["iterations"] for r in res])
         cert = sum(1 for r in res if r["status"] == 0 and r["rel_gap"] <= 1e-8 and

@@ -83,6 +83,14 @@ public:
     void set_warm_start(const double* x0, const double* y0, const double* s0, const int32_t* flags);
     // original-space solution of LP i (after solve())
     void solution(int i, double* x, double* y, double* s) const;
+    // Basic solutions from the last solve's x, one workgroup per LP (the reference's counterpart
+    // for one LP is generate_crossover_basis, src/ipm/crossover.cu:322).  Host outputs in the
+    // layouts of the inputs: basis (sum m; the basic column of each row, -1 for a row left
+    // artificial), x (sum n; x_N = 0), and per LP status (0 optimal vertex, 1 primal feasible
+    // basis that is not dual feasible, 2 rank deficient, 3 unbounded purification direction,
+    // 4 basic solution infeasible) and pivots; any output may be null.  The solver's factor and
+    // work buffers serve as the crossover's workspace: the next solve() rebuilds them.
+    void crossover(int32_t* basis, double* x, int32_t* status, int32_t* pivots);
     int count() const;
 
     struct Impl;

@@ -203,6 +203,19 @@ int lk_batch_solve(lk_batch* h, lk_batch_summary* sum);
 int lk_batch_autotune(lk_batch* h, int32_t sample, lk_batch_summary* chosen);
 int lk_batch_results(lk_batch* h, lk_batch_result* out);          /* count entries */
 int lk_batch_solution(lk_batch* h, int32_t i, double* x, double* y, double* s);
+/* Basic solutions from the last lk_batch_solve's x, one workgroup per LP (the reference's
+ * counterpart for one LP: generate_crossover_basis, src/ipm/crossover.cu:322).  Host outputs:
+ * basis (sum m; basic column of each row, -1 for a row left artificial), x (sum n, x_N = 0),
+ * status and pivots (count; status 0 optimal vertex, 1 primal feasible basis that is not dual
+ * feasible, 2 rank deficient, 3 unbounded purification direction, 4 basic solution
+ * infeasible); any output may be NULL.  LK_ERR_INVALID before the first solve. */
+int lk_batch_crossover(lk_batch* h, int32_t* basis, double* x, int32_t* status, int32_t* pivots);
+/* The same crossover from caller-given points: count LPs in the lk_batch_create layout and one
+ * point per LP in x (sum n), outputs as lk_batch_crossover (ipm::crossover_points,
+ * include/ipm/crossover.hpp).  Device memory is allocated for the call only. */
+int lk_crossover(int32_t count, const int32_t* m, const int32_t* n, const int32_t* colptr,
+                 const int32_t* rowidx, const double* vals, const double* b, const double* c,
+                 const double* x, int32_t* basis, double* xv, int32_t* status, int32_t* pivots);
 /* Warm start for the next solves: original-space x, s (sum n) and y (sum m) in the layout of
  * the inputs; LP i uses it when flags[i] != 0 (flags NULL: every LP); x NULL clears it.
  * Extension (the single-LP counterpart is lk_ipm_run's warm_* arguments). */

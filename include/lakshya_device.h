@@ -416,6 +416,22 @@ int lkd_batch_reset(uint32_t* ctl, lkb_state* state, const int32_t* order, int32
  * the kernel by value */
 int lkd_batch_solve(const lkb_args* args, int32_t grid, void* stream);
 
+/* Crossover of every LP of a batch (one workgroup per LP, after a solve): from the interior
+ * point x (x_out layout) to a basic solution.  The basis is built from the support of x (largest
+ * values first; a Gauss-Jordan explicit inverse in the LP's factor slot), the remaining positive
+ * nonbasic columns are purified along directions that do not increase c^T x, and x_B = B^-1 b.
+ * Outputs: basis (y layout: the basic column of each row, -1 artificial), xv (x layout), status
+ * (0 optimal vertex, 1 primal-feasible vertex that is not dual feasible, 2 rank deficient,
+ * 3 unbounded direction, 4 basic solution infeasible) and pivots, per LP. */
+typedef struct {
+    const double* x;
+    int32_t* basis;
+    double* xv;
+    int32_t* status;
+    int32_t* pivots;
+} lkx_args;
+int lkd_batch_crossover(const lkb_args* args, const lkx_args* xo, void* stream);
+
 /* Device scratch of the no-allocation batch entry point: a fixed file-scope device array of
  * LKB_SCRATCH_BYTES (every GPU the library runs on holds one until the process ends).  *ptr is
  * its address on the current device, resolved once per device. */

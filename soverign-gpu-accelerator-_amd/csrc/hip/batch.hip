@@ -1177,6 +1177,269 @@ __global__ void k_batch_reset(uint32_t* ctl, lkb_state* state, const int32_t* or
     if (restart && i < count) state[order[i]].phase = 0;
 }
 
+// ----------------------------------------------------------------------------- crossover
+// One workgroup per LP (src/ipm/crossover.cu:322-460 is the reference's column-by-column
+// purification from an artificial basis; here the basis comes from the support first):
+//  1. xv = max(x, 0), tau = 1e-7 max(1, max xv); candidates: the support {xv > tau} by xv
+//     descending (ties: lower index), then the other columns by index;
+//  2. B^-1 = I with every row artificial (the LP's factor slot, column-major, leading dimension
+//     ld); a candidate q takes the artificial row r of largest |u_r| (lowest r on ties),
+//     u = B^-1 a_q, when |u_r| > 1e-9 max(1, max |u|): Gauss-Jordan update on pivot (r, u);
+//  3. purification: the lowest nonbasic q with xv_q > tau moves by theta and the basic values by
+//     -theta u, theta > 0 when its reduced cost d_q <= 1e-11 (1 + |c_q|) (ratio test over
+//     u_i > 1e-12 with the basic values floored at 0; none: unbounded, status 3), else
+//     theta = -min(xv_q, ratios over u_i < -1e-12); a blocking basic row (first minimum) pivots
+//     q in, x_q blocking ends at 0;
+//  4. x_B = B^-1 b, x_N = 0, y = B^-T c_B, reduced costs; the status.
+// Work vectors: N(0) values, N(1) candidate order (int), N(2) basic flags, M(0) u, M(1) c_B.
+constexpr int XO_T = 256;
+
+// thread-local (value, index) candidates reduced over the workgroup: the smallest key wins,
+// ties by the lower index; every thread receives the result
+struct KeyIdx { double key; int idx; };
+__device__ KeyIdx wg_argmin(KeyIdx v, KeyIdx* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = XO_T / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const KeyIdx a = sh[threadIdx.x], b = sh[threadIdx.x + w];
+            if (b.idx >= 0 && (a.idx < 0 || b.key < a.key || (b.key == a.key && b.idx < a.idx)))
+                sh[threadIdx.x] = b;
+        }
+        __syncthreads();
+    }
+    const KeyIdx r = sh[0];
+    __syncthreads();
+    return r;
+}
+__device__ double wg_sum_max(double v, bool is_max, double* shd) {
+    shd[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = XO_T / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            shd[threadIdx.x] = is_max ? fmax(shd[threadIdx.x], shd[threadIdx.x + w])
+                                      : shd[threadIdx.x] + shd[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = shd[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_crossover(lkb_args A, lkx_args X) {
+    __shared__ int brow[LKB_MAX_M];          // basic column of each row, -1 artificial
+    __shared__ double prow[LKB_MAX_M];       // pivot row of B^-1 (scaled)
+    __shared__ KeyIdx ski[XO_T];
+    __shared__ double sd[XO_T];
+    const int li = blockIdx.x;
+    if (li >= A.count) return;
+    const lkb_lp d = A.lps[li];
+    const Lp L = make_lp(A, d);
+    const int m = L.m, n = L.n, t = threadIdx.x;
+    const int64_t ld = L.ld;
+    double* B = L.fac;
+    double* xv = L.N(0);
+    int* ord = reinterpret_cast<int*>(L.N(1));
+    double* isb = L.N(2);
+    double* u = L.M(0);
+    double* cb = L.M(1);
+    const double* x = X.x + d.on;
+
+    // 1. values, threshold, candidate order
+    double mx = 0.0;
+    for (int j = t; j < n; j += XO_T) {
+        const double v = fmax(x[j], 0.0);
+        xv[j] = v;
+        isb[j] = 0.0;
+        mx = fmax(mx, v);
+    }
+    const double tau = 1e-7 * fmax(1.0, wg_sum_max(mx, true, sd));
+    for (int j = t; j < n; j += XO_T) {      // rank of a support column: larger values first
+        const double v = xv[j];
+        if (!(v > tau)) continue;
+        int rank = 0;
+        for (int k = 0; k < n; ++k) {
+            const double w = xv[k];
+            rank += (w > tau && (w > v || (w == v && k < j))) ? 1 : 0;
+        }
+        ord[rank] = j;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int p = 0;
+        for (int j = 0; j < n; ++j) p += xv[j] > tau ? 1 : 0;
+        for (int j = 0; j < n; ++j)
+            if (!(xv[j] > tau)) ord[p++] = j;
+    }
+    for (int i = t; i < m; i += XO_T) brow[i] = -1;
+    for (int64_t e = t; e < ld * m; e += XO_T) {
+        const int64_t j = e / ld, i = e - j * ld;
+        B[e] = i == j ? 1.0 : 0.0;
+    }
+    __syncthreads();
+
+    // u = B^-1 a_q
+    auto ftran = [&](int q) {
+        const int p0 = L.cp[q], p1 = L.cp[q + 1];
+        for (int i = t; i < m; i += XO_T) {
+            double s = 0.0;
+            for (int p = p0; p < p1; ++p) s += B[i + ld * L.ri[p]] * L.v0[p];
+            u[i] = s;
+        }
+        __syncthreads();
+    };
+    // Gauss-Jordan update of B^-1 on pivot (r, u)
+    auto pivot = [&](int r) {
+        const double ur = u[r];
+        for (int j = t; j < m; j += XO_T) prow[j] = B[r + ld * j] / ur;
+        __syncthreads();
+        for (int i = t; i < m; i += XO_T) {
+            const double ui = u[i];
+            if (i == r) {
+                for (int j = 0; j < m; ++j) B[i + ld * j] = prow[j];
+            } else if (ui != 0.0) {
+                for (int j = 0; j < m; ++j) B[i + ld * j] -= ui * prow[j];
+            }
+        }
+        __syncthreads();
+    };
+
+    // 2. basis from the candidates
+    int pivots = 0, nart = m;
+    for (int c = 0; c < n && nart > 0; ++c) {
+        const int q = ord[c];
+        ftran(q);
+        double um = 0.0;
+        KeyIdx best{0.0, -1};
+        for (int i = t; i < m; i += XO_T) {
+            const double a = fabs(u[i]);
+            um = fmax(um, a);
+            if (brow[i] < 0 && (best.idx < 0 || -a < best.key)) best = KeyIdx{-a, i};
+        }
+        um = wg_sum_max(um, true, sd);
+        best = wg_argmin(best, ski);
+        if (best.idx >= 0 && -best.key > 1e-9 * fmax(1.0, um)) {
+            pivot(best.idx);
+            if (t == 0) {
+                brow[best.idx] = q;
+                isb[q] = 1.0;
+            }
+            __syncthreads();
+            ++pivots;
+            --nart;
+        }
+    }
+
+    // 3. purification
+    int status = 0;
+    for (int guard = 0; guard < n + m; ++guard) {
+        KeyIdx qk{0.0, -1};
+        for (int j = t; j < n; j += XO_T)
+            if (isb[j] == 0.0 && xv[j] > tau && (qk.idx < 0 || j < qk.idx)) qk = KeyIdx{(double)j, j};
+        qk = wg_argmin(qk, ski);
+        if (qk.idx < 0) break;
+        const int q = qk.idx;
+        ftran(q);
+        double dsum = 0.0;
+        for (int i = t; i < m; i += XO_T) {
+            const double ci = brow[i] >= 0 ? L.c0[brow[i]] : 0.0;
+            cb[i] = ci;
+            dsum += ci * u[i];
+        }
+        const double dq = L.c0[q] - wg_sum_max(dsum, false, sd);
+        const bool up = dq <= 1e-11 * (1.0 + fabs(L.c0[q]));   // no cost increase moving up
+        KeyIdx rt{0.0, -1};
+        for (int i = t; i < m; i += XO_T) {
+            const double ui = u[i];
+            const double vi = brow[i] >= 0 ? fmax(xv[brow[i]], 0.0) : 0.0;
+            double ratio = -1.0;
+            if (up && ui > 1e-12) ratio = vi / ui;
+            if (!up && ui < -1e-12) ratio = vi / (-ui);
+            if (ratio >= 0.0 && (rt.idx < 0 || ratio < rt.key)) rt = KeyIdx{ratio, i};
+        }
+        rt = wg_argmin(rt, ski);
+        double theta;
+        int r = rt.idx;
+        if (up) {
+            if (r < 0) { status = 3; break; }
+            theta = rt.key;
+        } else {
+            theta = xv[q];
+            if (r >= 0 && rt.key < theta) theta = rt.key;
+            else r = -1;
+            theta = -theta;
+        }
+        for (int i = t; i < m; i += XO_T)
+            if (brow[i] >= 0) xv[brow[i]] -= theta * u[i];
+        __syncthreads();
+        if (t == 0) xv[q] += theta;
+        __syncthreads();
+        if (r >= 0) {
+            const int leave = brow[r];
+            pivot(r);
+            if (t == 0) {
+                if (leave >= 0) { isb[leave] = 0.0; xv[leave] = 0.0; }
+                brow[r] = q;
+                isb[q] = 1.0;
+            }
+            ++pivots;
+        } else if (t == 0) {
+            xv[q] = 0.0;
+        }
+        __syncthreads();
+    }
+
+    // 4. basic solution, dual check, outputs
+    double* xout = X.xv + d.on;
+    for (int j = t; j < n; j += XO_T) xout[j] = 0.0;
+    __syncthreads();
+    double xbmin = 1e300, xbabs = 0.0;
+    for (int i = t; i < m; i += XO_T) {
+        double s = 0.0;
+        for (int j = 0; j < m; ++j) s += B[i + ld * j] * L.b0[j];
+        u[i] = s;
+        cb[i] = brow[i] >= 0 ? L.c0[brow[i]] : 0.0;
+        if (brow[i] >= 0) {
+            xout[brow[i]] = s;
+            xbmin = fmin(xbmin, s);
+        }
+        xbabs = fmax(xbabs, fabs(s));
+    }
+    xbmin = -wg_sum_max(-xbmin, true, sd);
+    xbabs = wg_sum_max(xbabs, true, sd);
+    int nleft = 0;
+    for (int i = t; i < m; i += XO_T) nleft += brow[i] < 0 ? 1 : 0;
+    nleft = (int)wg_sum_max((double)nleft, false, sd);
+    if (status == 0) {
+        if (nleft > 0) status = 2;
+        else if (xbmin < -1e-9 * fmax(1.0, xbabs)) status = 4;
+    }
+    if (status == 0) {
+        double* y = prow;                    // y = B^-T c_B (m <= LKB_MAX_M)
+        for (int j = t; j < m; j += XO_T) {
+            double s = 0.0;
+            for (int i = 0; i < m; ++i) s += B[i + ld * j] * cb[i];
+            y[j] = s;
+        }
+        __syncthreads();
+        double rmin = 1e300, cabs = 0.0;
+        for (int j = t; j < n; j += XO_T) {
+            double s = 0.0;
+            for (int p = L.cp[j]; p < L.cp[j + 1]; ++p) s += L.v0[p] * y[L.ri[p]];
+            rmin = fmin(rmin, L.c0[j] - s);
+            cabs = fmax(cabs, fabs(L.c0[j]));
+        }
+        rmin = -wg_sum_max(-rmin, true, sd);
+        cabs = wg_sum_max(cabs, true, sd);
+        if (rmin < -1e-9 * fmax(1.0, cabs)) status = 1;
+    }
+    for (int i = t; i < m; i += XO_T) X.basis[d.om + i] = brow[i];
+    if (t == 0) {
+        X.status[li] = status;
+        X.pivots[li] = pivots;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1206,6 +1469,14 @@ int lkd_batch_reset(uint32_t* ctl, lkb_state* state, const int32_t* order, int32
     if (!ctl || count < 0 || (restart && (!state || !order))) return (int)hipErrorInvalidValue;
     const int grid = count > 0 ? (count + 255) / 256 : 1;
     LK_LAUNCH(k_batch_reset, grid, 256, 0, stream, ctl, state, order, count, restart);
+    return lk_last_error();
+}
+
+int lkd_batch_crossover(const lkb_args* args, const lkx_args* xo, void* stream) {
+    if (!args || !xo || !xo->x || !xo->basis || !xo->xv || !xo->status || !xo->pivots || args->count < 0)
+        return (int)hipErrorInvalidValue;
+    if (args->count == 0) return 0;
+    LK_LAUNCH(k_crossover, args->count, XO_T, 0, stream, *args, *xo);
     return lk_last_error();
 }
 

@@ -15,6 +15,7 @@
 #include <thread>
 #include "gpu/vram_arena.hpp"
 #include "ipm/batch.hpp"
+#include "ipm/crossover.hpp"
 #include "lakshya_device.h"
 
 namespace sankhya {
@@ -68,6 +69,10 @@ struct BatchImage {
     // the optional warm-start buffers (x0, s0, y0, flags)
     size_t warm_bytes() const {
         return 2 * rnd(sizeof(double) * n_xn) + rnd(sizeof(double) * n_ym) + rnd(sizeof(int32_t) * count);
+    }
+    // the optional crossover outputs (basis, values, status + pivots)
+    size_t crossover_bytes() const {
+        return rnd(sizeof(int32_t) * n_ym) + rnd(sizeof(double) * n_xn) + rnd(sizeof(int32_t) * 2 * count);
     }
 };
 
@@ -294,6 +299,10 @@ struct BatchedMehrotraSolver::Impl {
     // warm start (original space) and per-LP flags, allocated on first use
     double *d_x0 = nullptr, *d_y0 = nullptr, *d_s0 = nullptr;
     int32_t* d_wflag = nullptr;
+    // crossover outputs, allocated on first use
+    int32_t *d_basis = nullptr, *d_xstat = nullptr;
+    double* d_xv = nullptr;
+    bool solved = false;
     // cached graph of `graph_len` launches for one (grid, iterations, count) shape
     void* graph = nullptr;
     int graph_grid = 0, graph_iters = 0, graph_len = 0, graph_count = -1;
@@ -306,7 +315,8 @@ struct BatchedMehrotraSolver::Impl {
         img = pack_batch(cnt, m, n, colptr, rowidx, vals, b, c, opt.order);
         cus = device_cus();
         clock_khz = device_clock_khz();
-        arena.reset(new gpu::VRAMArena(img.device_bytes() + img.warm_bytes() + (1u << 20)));
+        arena.reset(new gpu::VRAMArena(img.device_bytes() + img.warm_bytes() + img.crossover_bytes() +
+                                       (1u << 20)));
         check(lkd_stream_create(&stream), "stream");
         dev = place_batch(img, [&](size_t nb) { return arena->allocate_uninitialized(nb); }, stream);
         check(lkd_stream_sync(stream), "upload");
@@ -409,7 +419,9 @@ struct BatchedMehrotraSolver::Impl {
     int default_iters() const { return opt.iters_per_launch > 0 ? opt.iters_per_launch : opt.max_iter + 1; }
 
     BatchSummary solve() {
-        return run(dev.order, count, default_wg(), default_iters(), opt.use_graph != 0);
+        BatchSummary sum = run(dev.order, count, default_wg(), default_iters(), opt.use_graph != 0);
+        solved = true;
+        return sum;
     }
 
     BatchSummary autotune(int sample) {
@@ -465,6 +477,28 @@ struct BatchedMehrotraSolver::Impl {
         check(lkd_stream_sync(stream), "warm");
     }
 
+    void crossover(int32_t* basis, double* x, int32_t* status, int32_t* pivots) {
+        if (!solved) throw std::invalid_argument("crossover: no solution yet (call solve() first)");
+        if (!d_basis) {
+            d_basis = static_cast<int32_t*>(arena->allocate_uninitialized(sizeof(int32_t) * img.n_ym));
+            d_xv = static_cast<double*>(arena->allocate_uninitialized(sizeof(double) * img.n_xn));
+            d_xstat = static_cast<int32_t*>(arena->allocate_uninitialized(sizeof(int32_t) * 2 * count));
+        }
+        const lkb_args a = make_args(opt, dev, dev.order, count, 1);
+        lkx_args xo;
+        xo.x = dev.x;
+        xo.basis = d_basis;
+        xo.xv = d_xv;
+        xo.status = d_xstat;
+        xo.pivots = d_xstat + count;
+        check(lkd_batch_crossover(&a, &xo, stream), "crossover launch");
+        if (basis) check(lkd_memcpy_dtoh(basis, d_basis, sizeof(int32_t) * img.n_ym, stream), "basis");
+        if (x) check(lkd_memcpy_dtoh(x, d_xv, sizeof(double) * img.n_xn, stream), "basic x");
+        if (status) check(lkd_memcpy_dtoh(status, d_xstat, sizeof(int32_t) * count, stream), "status");
+        if (pivots) check(lkd_memcpy_dtoh(pivots, d_xstat + count, sizeof(int32_t) * count, stream), "pivots");
+        check(lkd_stream_sync(stream), "crossover");
+    }
+
     void fetch_solution() {
         if (have_solution) return;
         hx.resize(static_cast<size_t>(img.n_xn));
@@ -491,6 +525,9 @@ void BatchedMehrotraSolver::set_warm_start(const double* x0, const double* y0, c
                                            const int32_t* flags) {
     impl_->set_warm(x0, y0, s0, flags);
 }
+void BatchedMehrotraSolver::crossover(int32_t* basis, double* x, int32_t* status, int32_t* pivots) {
+    impl_->crossover(basis, x, status, pivots);
+}
 void BatchedMehrotraSolver::solution(int i, double* x, double* y, double* s) const {
     if (i < 0 || i >= impl_->count) throw std::invalid_argument("solution: LP index out of range");
     impl_->fetch_solution();
@@ -498,6 +535,44 @@ void BatchedMehrotraSolver::solution(int i, double* x, double* y, double* s) con
     if (x) std::copy(impl_->hx.begin() + d.on, impl_->hx.begin() + d.on + d.n, x);
     if (s) std::copy(impl_->hs.begin() + d.on, impl_->hs.begin() + d.on + d.n, s);
     if (y) std::copy(impl_->hy.begin() + d.om, impl_->hy.begin() + d.om + d.m, y);
+}
+
+void crossover_points(int count, const int32_t* m, const int32_t* n, const int32_t* colptr,
+                      const int32_t* rowidx, const double* vals, const double* b, const double* c,
+                      const double* x, int32_t* basis, double* xv, int32_t* status,
+                      int32_t* pivots) {
+    if (!x) throw std::invalid_argument("crossover_points: null x");
+    const BatchImage img = pack_batch(count, m, n, colptr, rowidx, vals, b, c, 1);
+    gpu::VRAMArena arena(img.device_bytes() + img.crossover_bytes() + (1u << 20));
+    void* stream = nullptr;
+    check(lkd_stream_create(&stream), "stream");
+    try {
+        const BatchDevice dev =
+            place_batch(img, [&](size_t nb) { return arena.allocate_uninitialized(nb); }, stream);
+        check(lkd_memcpy_htod(dev.x, x, sizeof(double) * img.n_xn, stream), "points");
+        int32_t* d_basis = static_cast<int32_t*>(arena.allocate_uninitialized(sizeof(int32_t) * img.n_ym));
+        double* d_xv = static_cast<double*>(arena.allocate_uninitialized(sizeof(double) * img.n_xn));
+        int32_t* d_st = static_cast<int32_t*>(arena.allocate_uninitialized(sizeof(int32_t) * 2 * count));
+        BatchOptions opt;
+        const lkb_args a = make_args(opt, dev, dev.order, count, 1);
+        lkx_args xo;
+        xo.x = dev.x;
+        xo.basis = d_basis;
+        xo.xv = d_xv;
+        xo.status = d_st;
+        xo.pivots = d_st + count;
+        check(lkd_batch_crossover(&a, &xo, stream), "crossover launch");
+        if (basis) check(lkd_memcpy_dtoh(basis, d_basis, sizeof(int32_t) * img.n_ym, stream), "basis");
+        if (xv) check(lkd_memcpy_dtoh(xv, d_xv, sizeof(double) * img.n_xn, stream), "basic x");
+        if (status) check(lkd_memcpy_dtoh(status, d_st, sizeof(int32_t) * count, stream), "status");
+        if (pivots) check(lkd_memcpy_dtoh(pivots, d_st + count, sizeof(int32_t) * count, stream), "pivots");
+        check(lkd_stream_sync(stream), "crossover");
+    } catch (...) {
+        lkd_stream_sync(stream);
+        lkd_stream_destroy(stream);
+        throw;
+    }
+    check(lkd_stream_destroy(stream), "stream");
 }
 
 BatchSummary solve_batch_noalloc(int count, const int32_t* m, const int32_t* n,

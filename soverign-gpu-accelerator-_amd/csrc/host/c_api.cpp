@@ -12,6 +12,7 @@
 #include "gpu/kkt.hpp"
 #include "gpu/vram_arena.hpp"
 #include "ipm/batch.hpp"
+#include "ipm/crossover.hpp"
 #include "ipm/mehrotra.hpp"
 #include "ipm/symbolic.hpp"
 #include "lakshya.h"
@@ -622,6 +623,16 @@ int lk_batch_results(lk_batch* h, lk_batch_result* out) {
 }
 int lk_batch_solution(lk_batch* h, int32_t i, double* x, double* y, double* s) {
     return guarded([&] { h->solver->solution(i, x, y, s); });
+}
+int lk_batch_crossover(lk_batch* h, int32_t* basis, double* x, int32_t* status, int32_t* pivots) {
+    return guarded([&] { h->solver->crossover(basis, x, status, pivots); });
+}
+int lk_crossover(int32_t count, const int32_t* m, const int32_t* n, const int32_t* colptr,
+                 const int32_t* rowidx, const double* vals, const double* b, const double* c,
+                 const double* x, int32_t* basis, double* xv, int32_t* status, int32_t* pivots) {
+    return guarded([&] {
+        ipm::crossover_points(count, m, n, colptr, rowidx, vals, b, c, x, basis, xv, status, pivots);
+    });
 }
 int lk_batch_destroy(lk_batch* h) { return guarded([&] { delete h; }); }
 int lk_batch_set_warm_start(lk_batch* h, const double* x, const double* y, const double* s,

@@ -136,6 +136,9 @@ def lib():
         "lk_batch_autotune": [_vp, C.c_int32, C.POINTER(BatchSummary)],
         "lk_batch_results": [_vp, _vp],
         "lk_batch_solution": [_vp, C.c_int32, _f64p, _f64p, _f64p],
+        "lk_batch_crossover": [_vp, _i32p, _f64p, _i32p, _i32p],
+        "lk_crossover": [C.c_int32, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                         _i32p, _f64p, _i32p, _i32p],
         "lk_batch_destroy": [_vp],
         "lk_batch_set_warm_start": [_vp, _vp, _vp, _vp, _vp],
         "lk_batch_solve_noalloc": [C.c_int32, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p,
@@ -513,6 +516,19 @@ class Batch:
         _ck(lib().lk_batch_solution(self.h, i, x, y, s))
         return x, y, s
 
+    def crossover(self):
+        """lk_batch_crossover: basic solutions from the last solve, one list entry per LP:
+        {"basis": int32[m] (-1 artificial row), "x": float64[n], "status": int, "pivots": int}."""
+        mo = np.concatenate([[0], np.cumsum(self.m)]).astype(np.int64)
+        no = np.concatenate([[0], np.cumsum(self.n)]).astype(np.int64)
+        basis = np.zeros(int(mo[-1]), np.int32)
+        x = np.zeros(int(no[-1]))
+        st = np.zeros(self.count, np.int32)
+        pv = np.zeros(self.count, np.int32)
+        _ck(lib().lk_batch_crossover(self.h, basis, x, st, pv))
+        return [{"basis": basis[mo[i]:mo[i + 1]], "x": x[no[i]:no[i + 1]], "status": int(st[i]),
+                 "pivots": int(pv[i])} for i in range(self.count)]
+
     def close(self):
         if self.h:
             _ck(lib().lk_batch_destroy(self.h))
@@ -523,6 +539,30 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def crossover_points(lps, xs):
+    """lk_crossover: basic solutions of the LPs from the points xs (one per LP), in the form of
+    Batch.crossover()."""
+    m = np.array([lp.m for lp in lps], np.int32)
+    n = np.array([lp.n for lp in lps], np.int32)
+    colptr = np.concatenate([np.asarray(lp.indptr, np.int32) for lp in lps])
+    rowidx = np.concatenate([np.asarray(lp.indices, np.int32) for lp in lps])
+    vals = np.concatenate([np.asarray(lp.data, np.float64) for lp in lps])
+    b = np.concatenate([np.asarray(lp.b, np.float64) for lp in lps])
+    c = np.concatenate([np.asarray(lp.c, np.float64) for lp in lps])
+    x = np.concatenate([np.asarray(v, np.float64) for v in xs])
+    if x.size != int(n.sum()):
+        raise ValueError("crossover_points: one point of n entries per LP")
+    mo = np.concatenate([[0], np.cumsum(m)]).astype(np.int64)
+    no = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+    basis = np.zeros(int(mo[-1]), np.int32)
+    xv = np.zeros(int(no[-1]))
+    st = np.zeros(len(lps), np.int32)
+    pv = np.zeros(len(lps), np.int32)
+    _ck(lib().lk_crossover(len(lps), m, n, colptr, rowidx, vals, b, c, x, basis, xv, st, pv))
+    return [{"basis": basis[mo[i]:mo[i + 1]], "x": xv[no[i]:no[i + 1]], "status": int(st[i]),
+             "pivots": int(pv[i])} for i in range(len(lps))]
 
 
 def batch_solve_noalloc(lps, **opts):

@@ -6,6 +6,8 @@
 //   ipm   -- MehrotraSolver(model).solve() on the 2 x 4 textbook LP built with add_variable /
 //            add_constraint / finalize (tests/unit/test_mehrotra.cu:29-78), then the
 //            reference-signature verify_optimal(model, x, pi, &res);
+//   crossover -- generate_crossover_basis on a DeviceModel from upload_to_device, the two cases
+//            of tests/unit/test_crossover.cu:15-133;
 //   model -- the Model layout: bilinear_terms is a member as in src/core/problem.hpp:49.
 // Exit status 0 when every check holds; each failed check prints one line.
 #include <hip/hip_runtime_api.h>
@@ -17,8 +19,10 @@
 #include <vector>
 
 #include "core/problem.hpp"
+#include "gpu/device_model.hpp"
 #include "gpu/kkt.hpp"
 #include "gpu/vram_arena.hpp"
+#include "ipm/crossover.hpp"
 #include "ipm/mehrotra.hpp"
 #include "ipm/symbolic.hpp"
 #include "verifier/certificate.hpp"
@@ -123,6 +127,70 @@ static int run_ipm() {
     return 0;
 }
 
+static int run_crossover() {
+    // tests/unit/test_crossover.cu:15-114: min -x1 - x2  s.t.  x1 + 2 x2 + x3 = 4,
+    // 2 x1 + x2 + x4 = 4 from the interior point (1, 1, 1, 1), through a DeviceModel uploaded
+    // into an arena; the basis must be m valid, nonsingular columns whose basic solution is
+    // feasible with every nonbasic column AtLower
+    core::Model model;
+    model.sense = OptimizationSense::Minimize;
+    for (double c : {-1.0, -1.0, 0.0, 0.0}) model.add_variable(c);
+    model.add_constraint({0, 1, 2}, {1.0, 2.0, 1.0}, 4.0);
+    model.add_constraint({0, 1, 3}, {2.0, 1.0, 1.0}, 4.0);
+    model.finalize();
+    gpu::VRAMArena arena;
+    gpu::DeviceModel d_model = gpu::upload_to_device(model, arena);
+    const std::vector<Float> x_ipm = {1.0, 1.0, 1.0, 1.0};
+    simplex::Basis basis = ipm::generate_crossover_basis(d_model, x_ipm);
+    EXPECT(basis.basic_indices.size() == 2 && basis.col_status.size() == 4);
+    EXPECT(basis.basic_indices[0] < 4 && basis.basic_indices[1] < 4);
+    int num_basic = 0;
+    for (int j = 0; j < 4; ++j) num_basic += basis.col_status[j] == simplex::BasisStatus::Basic;
+    EXPECT(num_basic == 2);
+    auto col = [&](Index j) {
+        std::vector<Float> v(2, 0.0);
+        for (Index p = model.A.col_ptrs[j]; p < model.A.col_ptrs[j + 1]; ++p)
+            v[model.A.row_indices[p]] = model.A.values[p];
+        return v;
+    };
+    const Index b0 = basis.basic_indices[0], b1 = basis.basic_indices[1];
+    const std::vector<Float> A0 = col(b0), A1 = col(b1);
+    const Float det = A0[0] * A1[1] - A0[1] * A1[0];
+    EXPECT(std::fabs(det) > 1e-7);
+    std::vector<Float> x(4, 0.0);
+    x[b0] = (4.0 * A1[1] - 4.0 * A1[0]) / det;
+    x[b1] = (A0[0] * 4.0 - A0[1] * 4.0) / det;
+    EXPECT(std::fabs(x[0] + 2 * x[1] + x[2] - 4.0) <= 1e-6);
+    EXPECT(std::fabs(2 * x[0] + x[1] + x[3] - 4.0) <= 1e-6);
+    for (int j = 0; j < 4; ++j) {
+        EXPECT(x[j] >= -1e-6);
+        if (j != b0 && j != b1) EXPECT(basis.col_status[j] == simplex::BasisStatus::AtLower);
+    }
+    // the same call on the host model
+    simplex::Basis hb = ipm::generate_crossover_basis(model, x_ipm);
+    EXPECT(hb.basic_indices == basis.basic_indices && hb.col_status == basis.col_status);
+    std::printf("crossover: basis (%d, %d), x = (%.6f, %.6f, %.6f, %.6f), objective %.6f\n", b0, b1,
+                x[0], x[1], x[2], x[3], -x[0] - x[1]);
+
+    // tests/unit/test_crossover.cu:117-133: dependent rows throw the rank-deficiency message
+    core::Model dep;
+    dep.sense = OptimizationSense::Minimize;
+    dep.add_variable(-1.0);
+    dep.add_variable(0.0);
+    dep.add_constraint({0, 1}, {1.0, 1.0}, 2.0);
+    dep.add_constraint({0, 1}, {2.0, 2.0}, 4.0);
+    dep.finalize();
+    gpu::DeviceModel d_dep = gpu::upload_to_device(dep, arena);
+    bool threw = false;
+    try {
+        ipm::generate_crossover_basis(d_dep, std::vector<Float>{1.0, 1.0});
+    } catch (const std::runtime_error& e) {
+        threw = std::strstr(e.what(), "Crossover failed: Matrix is rank deficient") != nullptr;
+    }
+    EXPECT(threw);
+    return 0;
+}
+
 static int run_model() {
     core::Model model;
     model.add_variable(1.0);
@@ -141,6 +209,7 @@ int main(int argc, char** argv) {
         if (!std::strcmp(what, "model") || !std::strcmp(what, "all")) run_model();
         if (!std::strcmp(what, "kkt") || !std::strcmp(what, "all")) run_kkt();
         if (!std::strcmp(what, "ipm") || !std::strcmp(what, "all")) run_ipm();
+        if (!std::strcmp(what, "crossover") || !std::strcmp(what, "all")) run_crossover();
     } catch (const std::exception& e) {
         std::printf("FAIL: exception %s\n", e.what());
         return 2;

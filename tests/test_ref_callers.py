@@ -35,3 +35,11 @@ def test_mehrotra_model_solve_flow(gpu):
     """tests/unit/test_mehrotra.cu:29-78 through MehrotraSolver(model).solve(), then the
     reference-signature verify_optimal(model, x, pi, &res)."""
     _run("ipm")
+
+
+@pytest.mark.gpu
+def test_crossover_device_model_flow(gpu):
+    """tests/unit/test_crossover.cu:15-133 through upload_to_device and
+    generate_crossover_basis(DeviceModel, x): a feasible nonsingular basis from (1, 1, 1, 1), and
+    the rank-deficiency message for dependent rows."""
+    _run("crossover")
