@@ -19,15 +19,30 @@ support first and purifies what remains:
      0; none: unbounded), else theta = -min(xb_q, ratios over rows with u_i < -1e-12); a
      blocking basic row r (the first minimum) pivots q in (its variable leaves at 0); x_q
      blocking sets xb_q = 0;
-  4. x_B = B^-1 b, x_N = 0; y = B^-T c_B; reduced costs; status 0 optimal vertex, 1 feasible
+  4. clean-up when no artificial row is left, with x_B = B^-1 b, y = B^-T c_B, d = c - A'y
+     (0 on basic columns), ptol = 1e-9 max(1, max|x_B|), dtol = 1e-9 max(1, max|c|):
+     a. dual simplex, at most m pivots: while the most negative x_B(r) (lowest r on ties) is
+        below -ptol, the row alpha = (B^-1)_r A over nonbasic columns with
+        alpha_j < -1e-9 max(1, max|alpha|) chooses the entering q = argmin d'_j / -alpha_j,
+        d'_j = d_j if d_j > dtol else 0 (lowest j on ties); no such column ends this phase;
+     b. primal simplex with Bland's rule, at most m pivots: the lowest nonbasic q with
+        d_q < -dtol enters; u = B^-1 a_q; over rows with u_i > 1e-9 max(1, max|u|) the ratio
+        x'_i / u_i, x'_i = x_B(i) if x_B(i) > ptol else 0, is minimised, ties to the lowest basic
+        column; no such row: unbounded (status 3);
+     q replaces the basic column of the chosen row (Gauss-Jordan update as in 2);
+  5. x_B = B^-1 b, x_N = 0; y = B^-T c_B; reduced costs; status 0 optimal vertex, 1 feasible
      but not dual feasible, 2 rank deficient (an artificial row left), 3 unbounded direction,
      4 basic solution infeasible (min x_B < -1e-9 max(1, max|x_B|)).
+Step 4 acts at primal-degenerate optimal vertices (support below m): the columns that fill the
+last rows take B^-1 b values of the size of the point's residual, of either sign (4a), and the
+reduced costs of that basis need not all be >= 0 (4b).
 """
 import numpy as np
 
 
-def crossover(A, b, c, x):
-    """A dense (m x n).  Returns (basis[m] with -1 for artificial rows, xv[n], status, pivots)."""
+def crossover(A, b, c, x, cleanup=True):
+    """A dense (m x n).  Returns (basis[m] with -1 for artificial rows, xv[n], status, pivots).
+    cleanup=False skips step 4 (tests use it to show the step is needed)."""
     A = np.asarray(A, np.float64)
     m, n = A.shape
     xb = np.maximum(np.asarray(x, np.float64), 0.0)
@@ -99,6 +114,53 @@ def crossover(A, b, c, x):
             pivots += 1
         else:
             xv[q] = 0.0
+    if cleanup and status == 0 and not (basis < 0).any():
+        dtol = 1e-9 * max(1.0, float(np.abs(c).max()))
+
+        def state():
+            xB = Binv @ b
+            y = Binv.T @ np.array([c[j] for j in basis])
+            d = c - A.T @ y
+            d[isb] = 0.0
+            return xB, d, 1e-9 * max(1.0, float(np.abs(xB).max()))
+
+        def swap(r, q):
+            u = Binv @ A[:, q]
+            pivot(r, u)
+            isb[basis[r]] = False
+            basis[r] = q
+            isb[q] = True
+
+        for _ in range(m):                       # dual simplex: primal infeasibilities
+            xB, d, ptol = state()
+            r = int(np.argmin(xB))
+            if not xB[r] < -ptol:
+                break
+            alpha = Binv[r, :] @ A
+            alpha[isb] = 0.0
+            cand = np.where(alpha < -1e-9 * max(1.0, float(np.abs(alpha).max())))[0]
+            if cand.size == 0:
+                break
+            ratios = np.where(d[cand] > dtol, d[cand], 0.0) / (-alpha[cand])
+            swap(r, int(cand[int(np.argmin(ratios))]))
+            pivots += 1
+        for _ in range(m):                       # primal simplex (Bland): dual infeasibilities
+            xB, d, ptol = state()
+            neg = np.where(d < -dtol)[0]
+            if neg.size == 0:
+                break
+            q = int(neg[0])
+            u = Binv @ A[:, q]
+            rows = np.where(u > 1e-9 * max(1.0, float(np.abs(u).max())))[0]
+            if rows.size == 0:
+                status = 3
+                break
+            ratios = np.where(xB[rows] > ptol, xB[rows], 0.0) / u[rows]
+            best = ratios.min()
+            tied = rows[ratios == best]
+            r = int(tied[np.argmin(basis[tied])])
+            swap(r, q)
+            pivots += 1
     cb = np.array([c[j] if j >= 0 else 0.0 for j in basis])
     xB = Binv @ b
     out = np.zeros(n)

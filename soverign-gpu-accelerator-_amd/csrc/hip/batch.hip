@@ -1190,8 +1190,16 @@ __global__ void k_batch_reset(uint32_t* ctl, lkb_state* state, const int32_t* or
 //     u_i > 1e-12 with the basic values floored at 0; none: unbounded, status 3), else
 //     theta = -min(xv_q, ratios over u_i < -1e-12); a blocking basic row (first minimum) pivots
 //     q in, x_q blocking ends at 0;
-//  4. x_B = B^-1 b, x_N = 0, y = B^-T c_B, reduced costs; the status.
-// Work vectors: N(0) values, N(1) candidate order (int), N(2) basic flags, M(0) u, M(1) c_B.
+//  4. clean-up when no row is artificial (x_B = B^-1 b, y = B^-T c_B, d = c - A'y,
+//     ptol = 1e-9 max(1, max|x_B|), dtol = 1e-9 max(1, max|c|)): dual simplex while the lowest
+//     x_B(r) < -ptol (entering: alpha = (B^-1)_r A below -1e-9 max(1, max|alpha|), least
+//     d'_j / -alpha_j with d' = d above dtol else 0, lowest j on ties), then primal simplex
+//     with Bland's rule while some d_q < -dtol (lowest q; ratio x'_i / u_i over u_i above
+//     1e-9 max(1, max|u|), x' = x_B above ptol else 0, ties to the lowest basic column; no row:
+//     status 3); at most m pivots each;
+//  5. x_B = B^-1 b, x_N = 0, y = B^-T c_B, reduced costs; the status.
+// Work vectors: N(0) values, N(1) candidate order (int), N(2) basic flags, M(0) u, M(1) c_B,
+// M(2) x_B, M(3) y.
 constexpr int XO_T = 256;
 
 // thread-local (value, index) candidates reduced over the workgroup: the smallest key wins,
@@ -1389,7 +1397,117 @@ __global__ __launch_bounds__(256) void k_crossover(lkb_args A, lkx_args X) {
         __syncthreads();
     }
 
-    // 4. basic solution, dual check, outputs
+    // 4. clean-up at degenerate vertices: dual simplex on primal infeasibilities, then primal
+    //    simplex (Bland's rule) on dual infeasibilities; both only with no artificial row left
+    int nart_left = 0;
+    for (int i = t; i < m; i += XO_T) nart_left += brow[i] < 0 ? 1 : 0;
+    nart_left = (int)wg_sum_max((double)nart_left, false, sd);
+    if (status == 0 && nart_left == 0) {
+        __shared__ int s_row;
+        double* xB = L.M(2);
+        double* y = L.M(3);
+        double cmax = 0.0;
+        for (int j = t; j < n; j += XO_T) cmax = fmax(cmax, fabs(L.c0[j]));
+        const double dtol = 1e-9 * fmax(1.0, wg_sum_max(cmax, true, sd));
+        // x_B = B^-1 b and y = B^-T c_B; returns the primal tolerance 1e-9 max(1, max|x_B|)
+        auto basic_state = [&]() -> double {
+            double xa = 0.0;
+            for (int i = t; i < m; i += XO_T) {
+                double s = 0.0;
+                for (int j = 0; j < m; ++j) s += B[i + ld * j] * L.b0[j];
+                xB[i] = s;
+                cb[i] = L.c0[brow[i]];
+                xa = fmax(xa, fabs(s));
+            }
+            __syncthreads();
+            for (int j = t; j < m; j += XO_T) {
+                double s = 0.0;
+                for (int i = 0; i < m; ++i) s += B[i + ld * j] * cb[i];
+                y[j] = s;
+            }
+            return 1e-9 * fmax(1.0, wg_sum_max(xa, true, sd));
+        };
+        auto reduced_cost = [&](int j) {
+            double s = 0.0;
+            for (int p = L.cp[j]; p < L.cp[j + 1]; ++p) s += L.v0[p] * y[L.ri[p]];
+            return L.c0[j] - s;
+        };
+        auto row_alpha = [&](int j) {        // (B^-1)_r a_j with row r staged in prow
+            double a = 0.0;
+            for (int p = L.cp[j]; p < L.cp[j + 1]; ++p) a += prow[L.ri[p]] * L.v0[p];
+            return a;
+        };
+        auto swap_in = [&](int r, int q) {
+            ftran(q);
+            const int leave = brow[r];
+            pivot(r);
+            if (t == 0) {
+                isb[leave] = 0.0;
+                brow[r] = q;
+                isb[q] = 1.0;
+            }
+            __syncthreads();
+            ++pivots;
+        };
+        for (int it = 0; it < m; ++it) {            // dual simplex
+            const double ptol = basic_state();
+            KeyIdx lo{0.0, -1};
+            for (int i = t; i < m; i += XO_T)
+                if (lo.idx < 0 || xB[i] < lo.key) lo = KeyIdx{xB[i], i};
+            lo = wg_argmin(lo, ski);
+            if (!(lo.key < -ptol)) break;
+            const int r = lo.idx;
+            for (int j = t; j < m; j += XO_T) prow[j] = B[r + ld * j];
+            __syncthreads();
+            double amax = 0.0;
+            for (int j = t; j < n; j += XO_T)
+                if (isb[j] == 0.0) amax = fmax(amax, fabs(row_alpha(j)));
+            const double atol = 1e-9 * fmax(1.0, wg_sum_max(amax, true, sd));
+            KeyIdx best{0.0, -1};
+            for (int j = t; j < n; j += XO_T) {
+                if (isb[j] != 0.0) continue;
+                const double a = row_alpha(j);
+                if (!(a < -atol)) continue;
+                const double d = reduced_cost(j);
+                const double ratio = (d > dtol ? d : 0.0) / (-a);
+                if (best.idx < 0 || ratio < best.key) best = KeyIdx{ratio, j};
+            }
+            best = wg_argmin(best, ski);
+            if (best.idx < 0) break;
+            swap_in(r, best.idx);
+        }
+        for (int it = 0; it < m; ++it) {            // primal simplex, Bland's rule
+            const double ptol = basic_state();
+            __syncthreads();
+            KeyIdx qk{0.0, -1};
+            for (int j = t; j < n; j += XO_T)
+                if (isb[j] == 0.0 && reduced_cost(j) < -dtol) { qk = KeyIdx{(double)j, j}; break; }
+            qk = wg_argmin(qk, ski);
+            if (qk.idx < 0) break;
+            const int q = qk.idx;
+            ftran(q);
+            double um = 0.0;
+            for (int i = t; i < m; i += XO_T) um = fmax(um, fabs(u[i]));
+            const double utol = 1e-9 * fmax(1.0, wg_sum_max(um, true, sd));
+            KeyIdx rt{0.0, -1};                       // (ratio, basic column): ties to the lowest column
+            for (int i = t; i < m; i += XO_T) {
+                if (!(u[i] > utol)) continue;
+                const double ratio = (xB[i] > ptol ? xB[i] : 0.0) / u[i];
+                if (rt.idx < 0 || ratio < rt.key || (ratio == rt.key && brow[i] < rt.idx))
+                    rt = KeyIdx{ratio, brow[i]};
+            }
+            rt = wg_argmin(rt, ski);
+            if (rt.idx < 0) { status = 3; break; }
+            for (int i = t; i < m; i += XO_T)
+                if (brow[i] == rt.idx) s_row = i;
+            __syncthreads();
+            const int r = s_row;
+            __syncthreads();
+            swap_in(r, q);
+        }
+    }
+
+    // 5. basic solution, dual check, outputs
     double* xout = X.xv + d.on;
     for (int j = t; j < n; j += XO_T) xout[j] = 0.0;
     __syncthreads();

@@ -96,10 +96,13 @@ def test_status_unbounded_direction():
 
 
 def test_status_not_dual_feasible():
-    # min -x1  s.t.  x1 + x2 = 1 from the vertex (0, 1): feasible basis {x2}, reduced cost of x1 < 0
-    basis, xv, st, piv = X.crossover(np.array([[1.0, 1.0]]), np.array([1.0]), np.array([-1.0, 0.0]),
-                                     np.array([0.0, 1.0]))
+    # min -x1  s.t.  x1 + x2 = 1 from the vertex (0, 1): feasible basis {x2}, reduced cost of
+    # x1 < 0; without the clean-up that is status 1, with it one primal pivot reaches (1, 0)
+    A, b, c, x = np.array([[1.0, 1.0]]), np.array([1.0]), np.array([-1.0, 0.0]), np.array([0.0, 1.0])
+    basis, xv, st, piv = X.crossover(A, b, c, x, cleanup=False)
     assert st == 1 and basis.tolist() == [1] and xv.tolist() == [0.0, 1.0]
+    basis, xv, st, piv = X.crossover(A, b, c, x)
+    assert st == 0 and basis.tolist() == [0] and xv.tolist() == [1.0, 0.0] and piv == 2
 
 
 def test_status_basic_solution_infeasible():
@@ -116,3 +119,46 @@ def test_status_rank_deficient_rows():
     A2, b2 = np.vstack([A, A[1]]), np.append(b, b[1])
     basis, xv, st, piv = X.crossover(A2, b2, c, x)
     assert st == 2 and (basis < 0).sum() == 1 and piv >= 3
+
+
+def _degenerate_vertex_lp(seed, m=5, n=12, b_noise=0.0):
+    """An optimal vertex v with m - 1 positive entries (primal degenerate); c makes v optimal
+    with strictly positive reduced costs off its support.  b_noise perturbs b as an interior
+    point's residual would, so the fillers' basic values are small and of either sign."""
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(m, n))
+    S = rng.choice(n, m - 1, replace=False)
+    v = np.zeros(n)
+    v[S] = rng.uniform(0.5, 2.0, m - 1)
+    b = A @ v + rng.normal(scale=b_noise, size=m) if b_noise else A @ v
+    y = rng.normal(size=m)
+    s = rng.uniform(0.5, 1.5, n)
+    s[S] = 0.0
+    x = v.copy()
+    x[x == 0] = 1e-10
+    return A, b, A.T @ y + s, x
+
+
+@pytest.mark.parametrize("seed", [2, 4, 5, 8])
+def test_clean_up_restores_dual_feasibility(seed):
+    """At a degenerate vertex the basis completed by a filler column need not be dual feasible;
+    the primal simplex clean-up (Bland) pivots to an optimal basis of the same vertex."""
+    A, b, c, x = _degenerate_vertex_lp(seed)
+    assert X.crossover(A, b, c, x, cleanup=False)[2] == 1
+    basis, xv, st, piv = X.crossover(A, b, c, x)
+    best, _ = _brute_force(A, b, c)
+    assert st == 0 and abs(c @ xv - best) <= 1e-12 * max(1.0, abs(best))
+    y = np.linalg.solve(A[:, basis].T, c[basis])
+    assert (c - A.T @ y).min() >= -1e-9 * max(1.0, np.abs(c).max())
+
+
+@pytest.mark.parametrize("seed", [2, 3, 5, 7])
+def test_clean_up_restores_primal_feasibility(seed):
+    """With b off the degenerate vertex by 1e-7 a filler's basic value can be negative; the dual
+    simplex clean-up pivots it out and the result is the brute-force optimum."""
+    A, b, c, x = _degenerate_vertex_lp(seed, b_noise=1e-7)
+    assert X.crossover(A, b, c, x, cleanup=False)[2] == 4
+    basis, xv, st, piv = X.crossover(A, b, c, x)
+    best, _ = _brute_force(A, b, c)
+    assert st == 0 and abs(c @ xv - best) <= 1e-8 * max(1.0, abs(best))
+    assert xv.min() >= -1e-9 * max(1.0, np.abs(xv).max())

@@ -10,7 +10,9 @@ import pycrossover as X
 
 pytestmark = pytest.mark.gpu
 
-MEMBERS = list(range(0, 4096, 273))      # 16 LPs of the 4,096-LP batch, m from 50 to 800
+# 16 LPs of the 4,096-LP batch (m from 50 to 800), plus two whose support is m - 1 and whose
+# filler column comes out slightly negative before the dual simplex clean-up
+MEMBERS = list(range(0, 4096, 273)) + [1043, 3294]
 
 
 def _check_against_oracle(lp, x, out):
@@ -65,12 +67,13 @@ def test_given_points_match_the_batch_method(solved, gpu):
 
 
 def test_status_known_answers(gpu):
-    """One LP per status code from points whose outcome is fixed by hand (the oracle's
-    known-answer cases): unbounded ray, non-optimal vertex, infeasible basis, face midpoint."""
+    """Points whose outcome is fixed by hand (the oracle's known-answer cases): unbounded ray,
+    a non-optimal vertex that the primal clean-up moves to (1, 0), an infeasible basis, a face
+    midpoint."""
     from conftest import lp_from_rows
     cases = [
         (lp_from_rows([[1.0, -1.0]], [0.0], [-1.0, 0.0], 2), [1.0, 1.0], 3),
-        (lp_from_rows([[1.0, 1.0]], [1.0], [-1.0, 0.0], 2), [0.0, 1.0], 1),
+        (lp_from_rows([[1.0, 1.0]], [1.0], [-1.0, 0.0], 2), [0.0, 1.0], 0),
         (lp_from_rows([[1.0, 1.0], [1.0, -1.0]], [1.0, 3.0], [0.0, 0.0], 2), [1.0, 1.0], 4),
         (lp_from_rows([[1.0, 1, 1, 0, 0], [1, 0, 0, 1, 0], [0, 1, 0, 0, 1]], [2.0, 1.5, 1.5],
                       [-1.0, -1, 0, 0, 0], 5), [1.0, 1.0, 0.0, 0.5, 0.5], 0),
@@ -79,9 +82,26 @@ def test_status_known_answers(gpu):
     for (lp, x, want), out in zip(cases, outs):
         assert out["status"] == want, (x, out)
         _check_against_oracle(lp, np.array(x), out)
+    np.testing.assert_array_equal(outs[1]["x"], [1.0, 0.0])
     face = outs[3]["x"]
     assert face[0] + face[1] == pytest.approx(2.0, abs=1e-14)
     assert np.count_nonzero(face) <= 3
+
+
+def test_clean_up_at_degenerate_vertices(gpu):
+    """Degenerate optimal vertices (support m - 1) from the oracle's clean-up tests: exact b
+    (the filler's basis is not dual feasible: primal clean-up) and b off by 1e-7 (the filler's
+    value is negative: dual clean-up); device and oracle agree and end optimal."""
+    from conftest import lp_from_rows
+    from test_crossover_oracle import _degenerate_vertex_lp
+    cases = [_degenerate_vertex_lp(s) for s in (2, 4, 5, 8)]
+    cases += [_degenerate_vertex_lp(s, b_noise=1e-7) for s in (2, 3, 5, 7)]
+    lps = [lp_from_rows(A, b, c, A.shape[1]) for A, b, c, x in cases]
+    outs = gpu.crossover_points(lps, [x for A, b, c, x in cases])
+    for lp, (A, b, c, x), out in zip(lps, cases, outs):
+        assert X.crossover(A, b, c, x, cleanup=False)[2] in (1, 4)
+        _check_against_oracle(lp, x, out)
+        assert out["status"] == 0
 
 
 def test_batch_members_reach_optimal_vertices(solved):
