@@ -195,21 +195,13 @@ __device__ __forceinline__ void step_tile(const lkd_task& tk, T* __restrict__ L,
     // the tile that factors the next diagonal block is the launch's critical path: its waves
     // get issue priority over the other workgroup sharing the compute unit
     if (tk.nsplit || tk.prio) __builtin_amdgcn_s_setprio(3);
+    // The accumulators start at (sum of the PART products) - C(R, k): the PART tasks are
+    // dependencies (their products are final), and the tile's block of the assembled panel is
+    // modified by no task of the factorization, so both loads are issued ahead of the GEMM and
+    // their latency overlaps its first chunks.  After the GEMM, X = C - L L^T - parts = -acc.
     Acc<T> acc[4][2];
-    zero_acc<T, 4, 2>(acc);
-    gemm_tn<T>(P + tk.r0, ld, ra, P + k0, ld, kb, tk.kbeg, tk.kend, sm, acc);
-    for (int s = 0; s < tk.np; ++s) {                 // K ranges computed by the PART launch
-        const T* in = work + tk.px + (int64_t)s * LKD_PART + t;
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[p][b][e] += in[((p * 2 + b) * 4 + e) * 256];
-    }
     int64_t row[2];
     bool valid[2];
-    Acc<T> y[2][4];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
         const int rl = 32 * w + 16 * b + li;
@@ -223,19 +215,33 @@ __device__ __forceinline__ void step_tile(const lkd_task& tk, T* __restrict__ L,
             for (int p = 0; p < 4; ++p)
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    y[b][p][e] = pb[(int64_t)(16 * p + R::row(l, e) - R::row(l, 0)) * ld] - acc[p][b][e];
+                    acc[p][b][e] = -pb[(int64_t)(16 * p + R::row(l, e) - R::row(l, 0)) * ld];
         } else {
 #pragma unroll
             for (int p = 0; p < 4; ++p)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int c = min(16 * p + R::row(l, e), kbm);   // columns past kb: padding
-                    y[b][p][e] = P[(int64_t)(k0 + c) * ld + rc] - acc[p][b][e];
+                    acc[p][b][e] = -P[(int64_t)(k0 + c) * ld + rc];
                 }
         }
     }
-    // the staged factor of block k (in a STEPS launch: once its tile has published it); the
-    // tile's own loads above are already in flight
+    for (int s = 0; s < tk.np; ++s) {                 // K ranges computed by PART tasks
+        const T* in = work + tk.px + (int64_t)s * LKD_PART + t;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[p][b][e] += in[((p * 2 + b) * 4 + e) * 256];
+    }
+    gemm_tn<T>(P + tk.r0, ld, ra, P + k0, ld, kb, tk.kbeg, tk.kend, sm, acc);
+    Acc<T> y[2][4];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) y[b][p] = -acc[p][b];
+    // the staged factor of block k (in a STEPS launch: once its tile has published it)
     if (ds && ds->sdep >= 0) {
         if (t == 0) *s_ok = flag_wait(ds->flags2 + ds->sdep, ds->ep, err) ? 1 : 0;
         __syncthreads();
@@ -556,7 +562,9 @@ __global__ __launch_bounds__(256) void k_probe_gemm(const T* pa, int64_t lda, in
     Acc<T> acc[2][4];
     zero_acc<T, 2, 4>(acc);
     gemm_nt<T>(pa, lda, ra, pb, ldb, rb, 0, K, sm, acc);
-    for_acc<T>(acc, [&](int r, int c, T v) { out[c * MT + r] = v; });
+    // rows >= ra and columns >= rb of the product are unspecified (the callers discard them):
+    // written as zero here
+    for_acc<T>(acc, [&](int r, int c, T v) { out[c * MT + r] = r < ra && c < rb ? v : T(0); });
     // the transposed variant must produce the same product
     Acc<T> tr[4][2];
     zero_acc<T, 4, 2>(tr);
@@ -568,7 +576,10 @@ __global__ __launch_bounds__(256) void k_probe_gemm(const T* pa, int64_t lda, in
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                out[MT * NC + (16 * p + Mfma<T>::row(lane, e)) * MT + 32 * w + 16 * b + (lane & 15)] = tr[p][b][e];
+            {
+                const int c = 16 * p + Mfma<T>::row(lane, e), r = 32 * w + 16 * b + (lane & 15);
+                out[MT * NC + c * MT + r] = r < ra && c < rb ? tr[p][b][e] : T(0);
+            }
 }
 
 template <typename T>

@@ -36,15 +36,19 @@ __device__ __forceinline__ void zero_acc(Acc<T> (&acc)[X][Y]) {
 
 // Double-buffered K loop shared by the GEMM variants: chunks of KC columns of
 // A (rows 0..ra of pa, column-major, lda) and B (rows 0..rb of pb, ldb) are staged k-major in
-// LDS (rows past ra / rb and columns past kend read as zero); compute(As, Bs, k) consumes the
+// LDS (columns past kend read as zero, rows past ra / rb as below); compute(As, Bs, k) consumes the
 // staged chunk for k = 0, 4, .., KC-4.  Global loads run two chunks ahead in two register
 // sets (chunk j in set j % 2): while chunk c is computed, chunks c + 1 and c + 2 are in
 // flight, and chunk c + 1 goes to LDS right after.  One barrier per chunk; the LDS is free
 // again when it returns.
-// Clamp = false: out-of-range elements are not loaded (exec-masked loads; the fewest memory
-// instructions, for the supernodal kernels).  Clamp = true: every lane loads from a clamped,
-// always valid address and a select zeroes the element (no divergent control flow, fewer live
-// registers around the loads: the batched kernel, which runs at its register limit).
+// Clamp = false (the supernodal kernels): rows past ra / rb are loaded from row ra - 1 / rb - 1
+// (always valid addresses) and not zeroed: a row of A only feeds the same row of the product
+// and a row of B the same column, so they only reach product rows / columns the callers
+// discard.  Whole chunks (the only kind when K ranges are multiples of KC, as in the supernodal
+// plans) load without any per-element condition, so the loads carry no exec-mask branches and
+// the waits before the LDS stores count only the chunk being stored; a partial last chunk
+// zeroes its columns past kend.  Clamp = true (the batched kernel, at its register limit):
+// every element goes through a clamped column and a select that zeroes it outside the range.
 template <typename T, bool Clamp, typename F>
 __device__ __forceinline__ void gemm_loop(const T* __restrict__ pa, int64_t lda, int ra,
                                           const T* __restrict__ pb, int64_t ldb, int rb,
@@ -56,28 +60,27 @@ __device__ __forceinline__ void gemm_loop(const T* __restrict__ pa, int64_t lda,
     const int br = t & (NC - 1), bk = t >> 6;
     const bool arow = ar < ra, brow = br < rb;
     T va[2][NA], vb[2][NB];
-    const T* qa = pa + (arow ? ar : 0);
-    const T* qb = pb + (brow ? br : 0);
+    const T* qa = pa + (Clamp ? (arow ? ar : 0) : min(ar, max(ra, 1) - 1));
+    const T* qb = pb + (Clamp ? (brow ? br : 0) : min(br, max(rb, 1) - 1));
     auto load = [&](int set, int kc) {
+        if (!Clamp && kc + KC <= kend) {
+#pragma unroll
+            for (int q = 0; q < NA; ++q) va[set][q] = qa[(int64_t)(kc + ak + 2 * q) * lda];
+#pragma unroll
+            for (int q = 0; q < NB; ++q) vb[set][q] = qb[(int64_t)(kc + bk + 4 * q) * ldb];
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
             const int k = kc + ak + 2 * q;
-            if (Clamp) {
-                const T v = qa[(int64_t)min(k, kend - 1) * lda];
-                va[set][q] = (arow && k < kend) ? v : T(0);
-            } else {
-                va[set][q] = (arow && k < kend) ? pa[(int64_t)k * lda + ar] : T(0);
-            }
+            const T v = qa[(int64_t)min(k, kend - 1) * lda];
+            va[set][q] = ((Clamp ? arow : true) && k < kend) ? v : T(0);
         }
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             const int k = kc + bk + 4 * q;
-            if (Clamp) {
-                const T v = qb[(int64_t)min(k, kend - 1) * ldb];
-                vb[set][q] = (brow && k < kend) ? v : T(0);
-            } else {
-                vb[set][q] = (brow && k < kend) ? pb[(int64_t)k * ldb + br] : T(0);
-            }
+            const T v = qb[(int64_t)min(k, kend - 1) * ldb];
+            vb[set][q] = ((Clamp ? brow : true) && k < kend) ? v : T(0);
         }
     };
     auto store = [&](int set, int buf) {
@@ -133,21 +136,32 @@ __device__ __forceinline__ void gemm_nt(const T* __restrict__ pa, int64_t lda, i
 // TN (the transposed product): acc[p][b] (wave w) holds D = B A^T for B columns 16p .. and tile
 // rows 32w + 16b ..:  element e of lane l = sum_k A(32w + 16b + l%16, k) B(16p + row(l,e), k),
 // i.e. tile row from the lane, column from the element -- the B-operand layout of the solve.
+// Clamp = false (supernodal kernels): a wavefront skips the products of its 16-row blocks that
+// lie past row ra (product rows the callers discard; their accumulators keep their initial
+// values); the test is wave-uniform, so whole MFMA sequences are skipped, not masked.
 template <typename T, bool Clamp = false>
 __device__ __forceinline__ void gemm_tn(const T* __restrict__ pa, int64_t lda, int ra,
                                         const T* __restrict__ pb, int64_t ldb, int rb, int kbeg,
                                         int kend, GemmLds<T>& sm, Acc<T> (&acc)[4][2]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const bool v0 = Clamp || 32 * wu < ra, v1 = Clamp || 32 * wu + 16 < ra;
     gemm_loop<T, Clamp>(pa, lda, ra, pb, ldb, rb, kbeg, kend, sm, [&](const T* As, const T* Bs, int k) {
+        if (!v0) return;
         const T a0 = As[(k + lk) * PA + 32 * w + li];
-        const T a1 = As[(k + lk) * PA + 32 * w + 16 + li];
         T b[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) b[p] = Bs[(k + lk) * PB + 16 * p + li];
+        if (v1) {
+            const T a1 = As[(k + lk) * PA + 32 * w + 16 + li];
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            acc[p][0] = Mfma<T>::mma(b[p], a0, acc[p][0]);
-            acc[p][1] = Mfma<T>::mma(b[p], a1, acc[p][1]);
+            for (int p = 0; p < 4; ++p) {
+                acc[p][0] = Mfma<T>::mma(b[p], a0, acc[p][0]);
+                acc[p][1] = Mfma<T>::mma(b[p], a1, acc[p][1]);
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < 4; ++p) acc[p][0] = Mfma<T>::mma(b[p], a0, acc[p][0]);
         }
     });
 }
@@ -190,57 +204,92 @@ __device__ __forceinline__ float rsqrt_nr(float d) {
 }
 
 // One wavefront factors the 16x16 block at Ds(o, o) and inverts its factor.  Lane r (mod 16)
-// holds row r in registers; each finished column goes to LDS (cb, 16 values) and comes back as
-// broadcast reads, in order within the wavefront, so no barriers are needed.  Writes L (lower)
-// back to Ds and L^{-1} to Lv (column-major, pitch 16, upper part zero).  Returns false
-// (wave-uniform) when a pivot is not positive.
+// holds row r in registers.  Columns are factored four at a time: every lane forms the
+// Cholesky factor of the 4x4 diagonal block from broadcast (readlane) values and solves its
+// own row's four entries against it; the four finished columns go to LDS once (cb, column-
+// major, pitch 16) and are read back with wave-uniform (broadcast) addresses for the update of
+// the remaining columns -- four LDS round trips instead of sixteen.  Lane i then forms row i
+// of L^{-1} by substitution from the end, y L = e_i, reading L's columns from cb.  Every LDS
+// store puts consecutive lanes on consecutive addresses (no bank conflicts).  Writes L (lower)
+// back to Ds and L^{-1} to Lv (column-major, pitch 16, upper part zero); cb needs 272 values.
+// Returns false (wave-uniform) when a pivot is not positive.
 template <typename T>
 __device__ __forceinline__ bool potrf16_wave(T* Ds, int o, T* Lv, T* cb) {
     const int l = threadIdx.x & 63, r = l & 15;
+    T* isd = cb + 256;                            // 1 / L[j][j]
     T a[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) a[k] = Ds[(o + k) * DP + o + r];
     bool ok = true;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const T d = bcast_lane(a[j], j);
-        ok = ok && (d > T(0));
-        const T is = rsqrt_nr(d);                 // 1 / L[j][j]
-        a[j] = r == j ? d * is : a[j] * is;       // r > j: L[r][j]; r < j: unused upper part
-        if (l < 16) {
-            cb[16 * j + r] = a[j];                // column j (diagonal entry included)
-            if (r == j) cb[256 + j] = is;
+    for (int jb = 0; jb < 16; jb += 4) {
+        T m[4][4], L4[4][4], is4[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q <= p; ++q) m[p][q] = bcast_lane(a[jb + q], jb + p);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            T d = m[q][q];
+#pragma unroll
+            for (int s = 0; s < q; ++s) d = fma(-L4[q][s], L4[q][s], d);
+            ok = ok && (d > T(0));
+            is4[q] = rsqrt_nr(d);
+#pragma unroll
+            for (int p = q + 1; p < 4; ++p) {
+                T v = m[p][q];
+#pragma unroll
+                for (int s = 0; s < q; ++s) v = fma(-L4[p][s], L4[q][s], v);
+                L4[p][q] = v * is4[q];
+            }
         }
+        // row r in columns jb .. jb+3 (rows of the block reproduce L4, the diagonal included;
+        // entries right of a lane's diagonal are never used)
+        T lq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            T v = a[jb + q];
+#pragma unroll
+            for (int s = 0; s < q; ++s) v = fma(-lq[s], L4[q][s], v);
+            lq[q] = v * is4[q];
+        }
+        if (l < 16) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                cb[16 * (jb + q) + r] = lq[q];
+                if (jb + q <= r) Ds[(o + jb + q) * DP + o + r] = lq[q];
+            }
+        }
+        if (l < 4) isd[jb + l] = l == 0 ? is4[0] : l == 1 ? is4[1] : l == 2 ? is4[2] : is4[3];
         __builtin_amdgcn_wave_barrier();
-        T lkj[16];
 #pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (k > j) lkj[k] = cb[16 * j + k];
+        for (int k = jb + 4; k < 16; ++k) {
+            T v = a[k];
 #pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (k > j) a[k] = fma(-a[j], lkj[k], a[k]);
-        __builtin_amdgcn_sched_barrier(0);   // keep each column's loads next to their use
+            for (int q = 0; q < 4; ++q) v = fma(-lq[q], cb[16 * (jb + q) + k], v);
+            a[k] = v;
+        }
     }
-    // column c = r of L^{-1}: x[i] = (delta_ic - sum_{k<i} L[i][k] x[k]) / L[i][i], with
-    // L[i][k] = lane i's a[k] broadcast by readlane (scalar registers: few VGPRs, which matters
-    // where this is inlined into the step kernel)
-    T x[16];
-    const T invd = cb[256 + r];
+    __builtin_amdgcn_wave_barrier();
+    // row r of L^{-1}: y[k] = (delta_rk - sum_{j>k} y[j] L[j][k]) / L[k][k], k = 15 .. 0
+    // (zero for k > r)
+    T y[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        T acc = i == r ? T(1) : T(0);
+    for (int k = 15; k >= 0; --k) {
+        T s0 = k == r ? T(1) : T(0), s1 = T(0);
 #pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (k < i) acc = fma(-bcast_lane(a[k], i), x[k], acc);
-        x[i] = acc * bcast_lane(invd, i);
+        for (int j = k + 1; j < 16; ++j) {
+            const T ljk = cb[16 * k + j];
+            if (j & 1) s1 = fma(-y[j], ljk, s1);
+            else s0 = fma(-y[j], ljk, s0);
+        }
+        y[k] = (s0 + s1) * isd[k];
     }
     if (l < 16) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (k <= r) Ds[(o + k) * DP + o + r] = a[k];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) Lv[r * 16 + i] = i >= r ? x[i] : T(0);
+        for (int c = 0; c < 16; ++c) Lv[c * 16 + r] = c <= r ? y[c] : T(0);
     }
+    __builtin_amdgcn_wave_barrier();              // cb reads done before the caller reuses it
     return ok;
 }
 

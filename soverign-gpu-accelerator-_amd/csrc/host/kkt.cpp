@@ -684,6 +684,27 @@ struct GPUKKTCholeskySolver::Impl {
                     dag_tasks.swap(t2);
                     dag_meta.swap(m2);
                 }
+                // the dependencies below are taken from the tiles listed before a task: every
+                // tile of the previous step (for a PART: of its data step) that it reads must be
+                // listed before it
+                {
+                    std::vector<std::vector<size_t>> by_sn(lv.size());
+                    for (size_t i = 0; i < dag_tasks.size(); ++i)
+                        if (!dag_meta[i].part) by_sn[dag_meta[i].si].push_back(i);
+                    for (size_t i = 0; i < dag_tasks.size(); ++i) {
+                        const DagTile& d = dag_meta[i];
+                        const lkd_task& u = dag_tasks[i];
+                        const int step = d.part ? d.dstep : d.k - 1;
+                        for (size_t j : by_sn[d.si]) {
+                            const DagTile& e = dag_meta[j];
+                            if (e.k != step) continue;
+                            const bool reads = (e.r0 < d.r1 && d.r0 < e.r1) || (!d.part && e.nsplit) ||
+                                               (d.part && e.r0 < u.k0 + u.kb && u.k0 < e.r1);
+                            if (reads && j >= i)
+                                throw std::logic_error("KKT plan: STEPS task listed before a tile it reads");
+                        }
+                    }
+                }
                 // a tile waits for the tiles of the previous step of its supernode whose rows
                 // overlap its rows, for the one that factored its diagonal block, and for its PART;
                 // a PART waits for the tiles of the step it follows that overlap its rows or the
