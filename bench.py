@@ -30,7 +30,8 @@ mixed precision (FP32 MFMA factor + FP64 PCG) on this workload and on the 700 x 
 with 20,000 side rows; the batched persistent solver on the 4,096-LP batch (m ~ U{50..800}),
 FP64 and mixed, against a sequential loop of the plain single-LP device IPM on a sample of the
 same LPs; the learned warm start on the parametric family at 400 x 1,000 (the 20,000 x 50,000
-family scaled down so its training set can be generated in the run).
+family scaled down so its training set can be generated in the run); and the reference's simplex
+route itself run on the device (lk_simplex) on the LP the CPU baseline times, beside it.
 """
 import argparse
 import json
@@ -151,7 +152,8 @@ def cpu_reference_route(m=500, n=1250, seed=1, timeout_s=120):
     names = {0: "Optimal", 1: "Infeasible", 2: "Unbounded", 3: "IterationLimit"}
     solved = r["rc"] == 0 and r["status"] == 0
     return {"value": (1e3 / r["ms"]) if solved else 0.0, "unit": "solves/s", "cores": 1,
-            "kind": "reference",
+            "kind": "reference", "route_seconds": r["ms"] / 1e3, "route_objective": r["objective"],
+            "route_status": names.get(r["status"], r["status"]),
             "sample": (f"reference CPU route (solve_with_phase1: Phase I + Phase II primal simplex, "
                        f"SparseLUFactorization; oracle/_ref built from the reference sources, -O2, "
                        f"no fast-math) on one pinned core, random LP m={m} n={n} (6 nnz/col, seed "
@@ -328,6 +330,26 @@ def form_batch(native, count=4096, sample=48, workers=8):
     return out
 
 
+def form_simplex(native, cpu, m=500, n=1250, seed=1):
+    """The reference's CPU route (Phase I + Phase II simplex) run on the device (lk_simplex) on the
+    LP the CPU baseline times: wall time of the call (upload, launch, read-back), status and
+    objective, beside the reference build's time and objective on one pinned core."""
+    from lakshya import gen
+    lp = gen.random_sparse(m, n, 6, seed=seed)
+    native.simplex([lp])                               # first call: module and arena set-up
+    t = time.perf_counter()
+    r = native.simplex([lp])[0]
+    dt = time.perf_counter() - t
+    out = {"workload": f"rs_{m}_{n}_6_s{seed}", "device_seconds": dt, "status": r["status_name"],
+           "objective": r["objective"], "iterations": list(r["iterations"])}
+    if cpu and cpu.get("kind") == "reference" and cpu.get("route_status") == "Optimal":
+        out["reference_cpu_seconds"] = cpu["route_seconds"]
+        out["device_over_reference_cpu"] = cpu["route_seconds"] / dt
+        out["rel_objective_diff_vs_reference"] = abs(r["objective"] - cpu["route_objective"]) / \
+            max(1.0, abs(cpu["route_objective"]))
+    return out
+
+
 def form_warm(ntrain=20_000, ntest=1000):
     from lakshya import warmstart
     return warmstart.experiment(400, 1000, 0.1, ntrain, ntest, target_iter=2, epochs=20)
@@ -345,7 +367,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="all",
-                    help="comma list of chosen-form comparisons: mixed,transport,batch,warm (all / none)")
+                    help="comma list of chosen-form comparisons: mixed,transport,batch,warm,simplex (all / none)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (traffic-counter reference)")
     args = ap.parse_args()
@@ -403,7 +425,7 @@ def main():
     roofline = step_roofline(live, st, args.precision == "mixed")
 
     forms = {}
-    want = {"mixed", "transport", "batch", "warm"} if args.forms == "all" else \
+    want = {"mixed", "transport", "batch", "warm", "simplex"} if args.forms == "all" else \
         set() if args.forms == "none" else set(args.forms.split(","))
     if "mixed" in want and args.precision == "fp64":
         log("chosen form: mixed precision on the workload")
@@ -432,6 +454,10 @@ def main():
         cpu = cpu_reference_route() or port
         if cpu is not port:
             cpu["port_ipm"] = port
+    if "simplex" in want:
+        log("the reference's simplex route on the device, beside its CPU build")
+        forms["simplex_route"] = form_simplex(native, cpu)
+        log(json.dumps(forms["simplex_route"]))
 
     out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,

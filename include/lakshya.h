@@ -216,6 +216,19 @@ int lk_batch_crossover(lk_batch* h, int32_t* basis, double* x, int32_t* status, 
 int lk_crossover(int32_t count, const int32_t* m, const int32_t* n, const int32_t* colptr,
                  const int32_t* rowidx, const double* vals, const double* b, const double* c,
                  const double* x, int32_t* basis, double* xv, int32_t* status, int32_t* pivots);
+
+/* ---- device simplex route: the reference's CPU route solve_with_phase1 + primal_simplex_phase2
+ * (src/simplex/phase1.cpp:124-449, src/simplex/primal.cpp:27-423) on the device, one workgroup
+ * per LP (simplex::solve_batch, include/simplex/primal.hpp): count LPs min c^T x, A x = b,
+ * x >= 0 in the lk_batch_create layout (m <= 1,600 each); max_iter iterations per phase (the
+ * reference: 10,000).  Outputs (any may be NULL): basis (sum m; -1 for a row left artificial),
+ * x (sum n), y (sum m; B^-T c_B), status (count; 0 Optimal, 1 Infeasible, 2 Unbounded,
+ * 3 IterationLimit), iterations (2 count: Phase I, Phase II), objective (count).  Device memory
+ * is allocated for the call only. */
+int lk_simplex(int32_t count, const int32_t* m, const int32_t* n, const int32_t* colptr,
+               const int32_t* rowidx, const double* vals, const double* b, const double* c,
+               int32_t max_iter, int32_t* basis, double* x, double* y, int32_t* status,
+               int32_t* iterations, double* objective);
 /* Warm start for the next solves: original-space x, s (sum n) and y (sum m) in the layout of
  * the inputs; LP i uses it when flags[i] != 0 (flags NULL: every LP); x NULL clears it.
  * Extension (the single-LP counterpart is lk_ipm_run's warm_* arguments). */

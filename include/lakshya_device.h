@@ -432,6 +432,26 @@ typedef struct {
 } lkx_args;
 int lkd_batch_crossover(const lkb_args* args, const lkx_args* xo, void* stream);
 
+/* Phase I / Phase II primal simplex of every LP of a batch (one workgroup per LP; the
+ * reference's CPU route solve_with_phase1 + primal_simplex_phase2, src/simplex/phase1.cpp:124-449
+ * and primal.cpp:27-423, with a dense explicit basis inverse in the LP's factor slot): Devex
+ * pricing with Bland's rule after 50 consecutive degenerate steps, Harris two-pass ratio test,
+ * at most max_iter iterations per phase.  Outputs per LP: status (0 Optimal, 1 Infeasible,
+ * 2 Unbounded, 3 IterationLimit), basis (y layout; -1 for a row left artificial), x (x layout,
+ * nonbasic 0), duals y = B^-T c_B of the final basis (y layout), objective c^T x,
+ * iterations[2 i .. 2 i + 1] of Phase I and Phase II. */
+#define LKS_MAX_ITER 10000
+typedef struct {
+    int32_t* basis;
+    double* x;
+    double* y;
+    int32_t* status;
+    int32_t* iterations;
+    double* objective;
+    int32_t max_iter;
+} lks_args;
+int lkd_batch_simplex(const lkb_args* args, const lks_args* sx, void* stream);
+
 /* Device scratch of the no-allocation batch entry point: a fixed file-scope device array of
  * LKB_SCRATCH_BYTES (every GPU the library runs on holds one until the process ends).  *ptr is
  * its address on the current device, resolved once per device. */

@@ -9,11 +9,14 @@
 //
 // Flow: read_mps (a parse error prints {"error": "..."} and exits 0, as the reference does);
 // MehrotraSolver(model).solve(); an Optimal result is certified with verify_optimal.  On
-// IterationLimit the reference runs its CPU Phase-I simplex; that route is not part of this
-// library, so the recovery here re-solves on the device in the relative-gap mode (its stricter
-// end-game rules are the ones that carry stalled solves to a certificate) and reports
-// execution_path "GPU+GPU_RECOVERY".  solve_time_ms covers the solver set-up, the solves and
-// the certificate; iterations are those of the first solve, as in the reference.
+// IterationLimit the reference falls back to its CPU Phase-I / Phase-II simplex
+// (sankhya_benchmark_cli.cpp:48-75); here the same route runs on the device
+// (simplex::solve_with_phase1, execution_path "GPU+GPU_SIMPLEX") for models of up to 1,600 rows,
+// and its optimum is certified with pi = B^-T c_B as the reference does; a non-Optimal route
+// ends as "FailedRecovery".  Larger models re-solve on the device in the relative-gap mode
+// instead (its stricter end-game rules carry stalled solves to a certificate;
+// "GPU+GPU_RECOVERY").  solve_time_ms covers the solver set-up, the solves and the
+// certificate; iterations are those of the first solve, as in the reference.
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -24,6 +27,7 @@
 #include "core/problem.hpp"
 #include "ipm/mehrotra.hpp"
 #include "parsers/mps_reader.hpp"
+#include "simplex/primal.hpp"
 #include "verifier/certificate.hpp"
 
 using namespace sankhya;
@@ -67,6 +71,7 @@ int main(int argc, char** argv) {
     std::string status = "Failed";
     double objective = 0.0, primal_residual = 0.0;
     bool cert = false, fallback = false;
+    const char* path_name = "GPU_ONLY";
     long iterations = 0;
     try {
         ipm::IpmOptions opt;
@@ -85,7 +90,20 @@ int main(int argc, char** argv) {
         } else if (r.status == simplex::SimplexStatus::IterationLimit) {
             fallback = true;
             status = "FailedRecovery";
-            if (!relative) {
+            if (model.A.rows <= simplex::kDeviceSimplexMaxRows) {
+                path_name = "GPU+GPU_SIMPLEX";
+                simplex::Basis basis;
+                std::vector<Float> x, pi;
+                if (simplex::solve_with_phase1(model, basis, x, &pi) == simplex::SimplexStatus::Optimal) {
+                    status = "Optimal";
+                    objective = 0.0;
+                    for (std::size_t j = 0; j < x.size(); ++j) objective += x[j] * model.obj[j];
+                    Float res = 0.0;
+                    cert = verifier::verify_optimal(model, x, pi, &res);
+                    primal_residual = res;
+                }
+            } else if (!relative) {
+                path_name = "GPU+GPU_RECOVERY";
                 ipm::IpmOptions rec;
                 rec.stop_rule = ipm::IpmOptions::StopRule::RelativeGap;
                 ipm::MehrotraSolver solver(model, rec);
@@ -109,7 +127,7 @@ int main(int argc, char** argv) {
     std::printf("{\n");
     std::printf("  \"solver\": \"SANKHYA\",\n");
     std::printf("  \"status\": \"%s\",\n", status.c_str());
-    std::printf("  \"execution_path\": \"%s\",\n", fallback ? "GPU+GPU_RECOVERY" : "GPU_ONLY");
+    std::printf("  \"execution_path\": \"%s\",\n", path_name);
     std::printf("  \"fallback_triggered\": \"%s\",\n", fallback ? "YES" : "NO");
     std::printf("  \"solve_time_ms\": %.17g,\n", ms);
     std::printf("  \"iterations\": %ld,\n", iterations);

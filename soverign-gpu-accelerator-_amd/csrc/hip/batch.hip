@@ -1558,6 +1558,339 @@ __global__ __launch_bounds__(256) void k_crossover(lkb_args A, lkx_args X) {
     }
 }
 
+// ----------------------------------------------------------------------------- simplex
+// The reference's CPU route, solve_with_phase1 + primal_simplex_phase2 (src/simplex/phase1.cpp:
+// 124-449, src/simplex/primal.cpp:27-423), for every LP of a batch (min c^T x, A x = b, x >= 0),
+// one workgroup per LP, with a dense explicit basis inverse in the LP's factor slot (Gauss-Jordan
+// updates) in place of the sparse LU with eta updates:
+//  Phase I: rows with b_i < 0 negated (sign s_i), one artificial column per row (cost 1, the
+//    structural columns cost 0), the artificial basis: B^-1 = I, x_B = |b|.
+//  Each phase is the primal simplex, at most LKS_MAX_ITER iterations:
+//    duals y = B^-T c_B (updated by y += (r_q / u_p) rho_p after every pivot, rho_p the pivot
+//      row of the old inverse, and recomputed every 50 iterations); every 50 iterations and at
+//      the end x_B and y are also refined against the basis matrix itself (one step of
+//      x_B += B^-1 (b - B x_B), y += B^-T (c_B - B^T y)), since the Gauss-Jordan inverse drifts
+//      over thousands of pivots where the reference refactorizes its LU;
+//    entering column: Devex, the largest r_j^2 / gamma_j over nonbasic r_j < -1e-6 (lowest j
+//      on ties; gamma = 1 at the start and every ncols iterations, the leaving column's gamma =
+//      max(1, |u|^2)), or after more than 50 consecutive degenerate steps (theta < 1e-6) for the
+//      rest of the phase, Bland's rule: the lowest j with r_j < -1e-6; none: Optimal;
+//    u = B^-1 a_q; ratio test over u_i > 1e-7: Harris, Delta = min (x_i + 1e-6) / u_i, then the
+//      largest |u_i| among x_i / u_i <= Delta (lowest i on ties), theta = max(0, x_p / u_p); under
+//      Bland the smallest x_i / u_i (lowest i on ties); no eligible row: Unbounded;
+//    x_B -= theta u, x_p = theta, pivot.
+//  Between the phases: IterationLimit ends the route; artificial sum > 1e-6: Infeasible; every
+//  row still holding an artificial takes the lowest nonbasic structural column j with
+//  |(B^-1 a_j)_i| > 1e-7 (theta = 0; none: IterationLimit, the reference's report for a
+//  rank-deficient model); B^-1 of the unnegated rows = B^-1 diag(s); a basic value below -1e-6
+//  then ends as Infeasible.  Phase II runs on the structural columns with the costs c.
+// Outputs: status (0 Optimal, 1 Infeasible, 2 Unbounded, 3 IterationLimit), the basis (y
+// layout), x (x layout; nonbasic 0), c^T x, the duals B^-T c_B of the final basis (artificial
+// rows cost 0), and the iterations of each phase.
+// Work vectors: N(0) basic flags of the structural columns, N(1) their Devex weights, N(2) x of
+// the basic columns; M(0) u, M(1) x_B, M(2) y, M(3) row signs, M(4) basic flags of the
+// artificials, M(5) their weights, M(6) residuals.
+constexpr double kSxFeas = 1e-6;     // math::kDefaultFeasibilityTol
+constexpr double kSxPivot = 1e-7;    // math::kDefaultPivotTol
+constexpr int kSxBland = 50;         // consecutive degenerate steps before Bland's rule
+constexpr int kSxRefresh = 50;       // iterations between recomputations of y
+
+__global__ __launch_bounds__(256) void k_simplex(lkb_args A, lks_args S) {
+    __shared__ int brow[LKB_MAX_M];          // basic column of each row (n + i: artificial i)
+    __shared__ double rho[LKB_MAX_M];        // pivot row of B^-1
+    __shared__ KeyIdx ski[XO_T];
+    __shared__ double sd[XO_T];
+    __shared__ double s_rq;
+    const int li = blockIdx.x;
+    if (li >= A.count) return;
+    const lkb_lp d = A.lps[li];
+    const Lp L = make_lp(A, d);
+    const int m = L.m, n = L.n, t = threadIdx.x;
+    const int64_t ld = L.ld;
+    double* B = L.fac;
+    double* isb = L.N(0);
+    double* gam = L.N(1);
+    double* u = L.M(0);
+    double* xb = L.M(1);
+    double* y = L.M(2);
+    double* sg = L.M(3);
+    double* isa = L.M(4);
+    double* gma = L.M(5);
+    const int max_iter = S.max_iter;
+
+    // Phase I start
+    for (int j = t; j < n; j += XO_T) { isb[j] = 0.0; gam[j] = 1.0; }
+    for (int i = t; i < m; i += XO_T) {
+        const double bi = L.b0[i];
+        sg[i] = bi < 0.0 ? -1.0 : 1.0;
+        xb[i] = fabs(bi);
+        isa[i] = 1.0;
+        gma[i] = 1.0;
+        brow[i] = n + i;
+    }
+    for (int64_t e = t; e < ld * m; e += XO_T) {
+        const int64_t j = e / ld, i = e - j * ld;
+        B[e] = i == j ? 1.0 : 0.0;
+    }
+    __syncthreads();
+
+    auto cost = [&](int j, bool p1) { return j >= n ? (p1 ? 1.0 : 0.0) : (p1 ? 0.0 : L.c0[j]); };
+    auto dot_col = [&](int j, const double* v) {       // a_j^T v (current row signs)
+        if (j >= n) return v[j - n];
+        double s = 0.0;
+        for (int k = L.cp[j]; k < L.cp[j + 1]; ++k) s += L.v0[k] * sg[L.ri[k]] * v[L.ri[k]];
+        return s;
+    };
+    auto ftran = [&](int q) {                          // u = B^-1 a_q
+        for (int i = t; i < m; i += XO_T) {
+            double s = 0.0;
+            if (q >= n) {
+                s = B[i + ld * (q - n)];
+            } else {
+                for (int k = L.cp[q]; k < L.cp[q + 1]; ++k)
+                    s += B[i + ld * L.ri[k]] * (L.v0[k] * sg[L.ri[k]]);
+            }
+            u[i] = s;
+        }
+        __syncthreads();
+    };
+    auto load_rho = [&](int r) {
+        for (int j = t; j < m; j += XO_T) rho[j] = B[r + ld * j];
+        __syncthreads();
+    };
+    auto pivot = [&](int r) {                          // Gauss-Jordan on (r, u), rho = old row r
+        const double ur = u[r];
+        for (int i = t; i < m; i += XO_T) {
+            const double ui = u[i];
+            if (i == r) {
+                for (int j = 0; j < m; ++j) B[i + ld * j] = rho[j] / ur;
+            } else if (ui != 0.0) {
+                const double f = ui / ur;
+                for (int j = 0; j < m; ++j) B[i + ld * j] -= f * rho[j];
+            }
+        }
+        __syncthreads();
+    };
+    auto duals = [&](bool p1) {                        // y = B^-T c_B
+        for (int j = t; j < m; j += XO_T) {
+            double s = 0.0;
+            for (int i = 0; i < m; ++i) s += B[i + ld * j] * cost(brow[i], p1);
+            y[j] = s;
+        }
+        __syncthreads();
+    };
+    // x_B and y refined against the basis itself (the Gauss-Jordan inverse drifts over thousands
+    // of pivots): r = b' - B x_B, x_B += B^-1 r; r = c_B - B^T y, y += B^-T r
+    double* xcol = L.N(2);
+    double* rr = L.M(6);
+    auto refine = [&](bool p1) {
+        for (int j = t; j < n; j += XO_T) xcol[j] = 0.0;
+        __syncthreads();
+        for (int i = t; i < m; i += XO_T)
+            if (brow[i] < n) xcol[brow[i]] = xb[i];
+        __syncthreads();
+        for (int k = t; k < m; k += XO_T) {
+            double s = 0.0;
+            for (int e = L.rp[k]; e < L.rp[k + 1]; ++e) s += L.v0[L.pos[e]] * xcol[L.ci[e]];
+            rr[k] = sg[k] * (L.b0[k] - s);
+        }
+        __syncthreads();
+        for (int i = t; i < m; i += XO_T)
+            if (brow[i] >= n) rr[brow[i] - n] -= xb[i];   // artificial columns are unit vectors
+        __syncthreads();
+        for (int i = t; i < m; i += XO_T) {
+            double s = 0.0;
+            for (int j = 0; j < m; ++j) s += B[i + ld * j] * rr[j];
+            xb[i] += s;
+        }
+        __syncthreads();
+        for (int i = t; i < m; i += XO_T) {
+            const int j = brow[i];
+            double s = j >= n ? y[j - n] : 0.0;
+            if (j < n)
+                for (int k = L.cp[j]; k < L.cp[j + 1]; ++k) s += L.v0[k] * sg[L.ri[k]] * y[L.ri[k]];
+            rr[i] = cost(j, p1) - s;
+        }
+        __syncthreads();
+        for (int j = t; j < m; j += XO_T) {
+            double s = 0.0;
+            for (int i = 0; i < m; ++i) s += B[i + ld * j] * rr[i];
+            y[j] += s;
+        }
+        __syncthreads();
+    };
+    auto is_basic = [&](int j) { return j >= n ? isa[j - n] != 0.0 : isb[j] != 0.0; };
+    auto gamma_of = [&](int j) -> double& { return j >= n ? gma[j - n] : gam[j]; };
+
+    // the primal simplex on columns 0 .. ncols; returns its status, adds its iterations to *it
+    auto primal = [&](bool p1, int* it) -> int {
+        const int ncols = p1 ? n + m : n;
+        int deg = 0;
+        bool bland = false;
+        for (int iter = 0; iter < max_iter; ++iter) {
+            if (iter % kSxRefresh == 0) {
+                duals(p1);
+                refine(p1);
+            }
+            KeyIdx best{0.0, -1};
+            for (int j = t; j < ncols; j += XO_T) {
+                if (is_basic(j)) continue;
+                const double rc = cost(j, p1) - dot_col(j, y);
+                if (!(rc < -kSxFeas)) continue;
+                if (bland) {
+                    if (best.idx < 0) best = KeyIdx{(double)j, j};
+                } else {
+                    double g = gamma_of(j);
+                    if (g <= 0.0) { g = 1.0; gamma_of(j) = 1.0; }
+                    const double key = -(rc * rc) / g;
+                    if (best.idx < 0 || key < best.key) best = KeyIdx{key, j};
+                }
+            }
+            best = wg_argmin(best, ski);
+            if (best.idx < 0) {                                    // Optimal
+                refine(p1);
+                return 0;
+            }
+            const int q = best.idx;
+            if (t == 0) s_rq = cost(q, p1) - dot_col(q, y);
+            ftran(q);
+            int pr = -1;
+            double theta = 0.0;
+            if (bland) {
+                KeyIdx rt{0.0, -1};
+                for (int i = t; i < m; i += XO_T)
+                    if (u[i] > kSxPivot) {
+                        const double ratio = xb[i] / u[i];
+                        if (rt.idx < 0 || ratio < rt.key) rt = KeyIdx{ratio, i};
+                    }
+                rt = wg_argmin(rt, ski);
+                pr = rt.idx;
+                if (pr >= 0) theta = rt.key;
+            } else {
+                KeyIdx dl{0.0, -1};
+                for (int i = t; i < m; i += XO_T)
+                    if (u[i] > kSxPivot) {
+                        const double rr = (xb[i] + kSxFeas) / u[i];
+                        if (dl.idx < 0 || rr < dl.key) dl = KeyIdx{rr, i};
+                    }
+                dl = wg_argmin(dl, ski);
+                if (dl.idx >= 0) {
+                    const double Delta = dl.key;
+                    KeyIdx bp{0.0, -1};
+                    for (int i = t; i < m; i += XO_T)
+                        if (u[i] > kSxPivot && xb[i] / u[i] <= Delta) {
+                            const double key = -fabs(u[i]);
+                            if (bp.idx < 0 || key < bp.key) bp = KeyIdx{key, i};
+                        }
+                    bp = wg_argmin(bp, ski);
+                    pr = bp.idx;
+                    if (pr >= 0) theta = fmax(xb[pr] / u[pr], 0.0);
+                }
+            }
+            if (pr < 0) return 2;                                  // Unbounded
+            ++*it;
+            if (theta < kSxFeas) {
+                if (++deg > kSxBland) bland = true;
+            } else {
+                deg = 0;
+            }
+            const int leave = brow[pr];
+            double uu = 0.0;
+            for (int i = t; i < m; i += XO_T) uu += u[i] * u[i];
+            uu = wg_sum_max(uu, false, sd);
+            load_rho(pr);
+            const double f = s_rq / u[pr];
+            for (int j = t; j < m; j += XO_T) y[j] += f * rho[j];
+            for (int i = t; i < m; i += XO_T) xb[i] = i == pr ? theta : xb[i] - theta * u[i];
+            pivot(pr);
+            if (t == 0) {
+                gamma_of(leave) = uu > 1.0 ? uu : 1.0;
+                if (leave >= n) isa[leave - n] = 0.0; else isb[leave] = 0.0;
+                if (q >= n) isa[q - n] = 1.0; else isb[q] = 1.0;
+                brow[pr] = q;
+            }
+            __syncthreads();
+            if ((iter + 1) % ncols == 0) {
+                for (int j = t; j < n; j += XO_T) gam[j] = 1.0;
+                for (int i = t; i < m; i += XO_T) gma[i] = 1.0;
+                __syncthreads();
+            }
+        }
+        return 3;                                                  // IterationLimit
+    };
+
+    int it1 = 0, it2 = 0;
+    int status = primal(true, &it1);
+    if (status == 3) goto done;
+    {
+        double art = 0.0;
+        for (int i = t; i < m; i += XO_T)
+            if (brow[i] >= n && xb[i] > 0.0) art += xb[i];
+        art = wg_sum_max(art, false, sd);
+        if (art > kSxFeas) { status = 1; goto done; }
+    }
+    for (int r = 0; r < m; ++r) {                                  // artificials out
+        if (brow[r] < n) continue;
+        load_rho(r);
+        KeyIdx pc{0.0, -1};
+        for (int j = t; j < n; j += XO_T) {
+            if (isb[j] != 0.0 || pc.idx >= 0) continue;
+            if (fabs(dot_col(j, rho)) > kSxPivot) pc = KeyIdx{(double)j, j};
+        }
+        pc = wg_argmin(pc, ski);
+        if (pc.idx < 0) { status = 3; goto done; }
+        const int q = pc.idx;
+        ftran(q);
+        pivot(r);
+        if (t == 0) {
+            isa[brow[r] - n] = 0.0;
+            isb[q] = 1.0;
+            brow[r] = q;
+            xb[r] = 0.0;
+        }
+        __syncthreads();
+    }
+    // Phase II on the unnegated rows: B^-1 diag(s), then s = 1
+    for (int64_t e = t; e < ld * m; e += XO_T) {
+        const int64_t j = e / ld, i = e - j * ld;
+        if (i < m) B[e] *= sg[j];
+    }
+    __syncthreads();
+    {
+        double lo = 0.0;
+        for (int i = t; i < m; i += XO_T) { sg[i] = 1.0; lo = fmin(lo, xb[i]); }
+        for (int j = t; j < n; j += XO_T) gam[j] = 1.0;
+        lo = -wg_sum_max(-lo, true, sd);
+        if (lo < -kSxFeas) { status = 1; goto done; }
+    }
+    status = primal(false, &it2);
+
+done:
+    __syncthreads();
+    duals(false);
+    for (int pass = 0; pass < 2; ++pass) refine(false);
+    double* xo = S.x + d.on;
+    for (int j = t; j < n; j += XO_T) xo[j] = 0.0;
+    __syncthreads();
+    double obj = 0.0;
+    for (int i = t; i < m; i += XO_T) {
+        const int j = brow[i];
+        S.basis[d.om + i] = j < n ? j : -1;
+        if (j < n) {
+            xo[j] = xb[i];
+            obj += L.c0[j] * xb[i];
+        }
+    }
+    obj = wg_sum_max(obj, false, sd);
+    for (int i = t; i < m; i += XO_T) S.y[d.om + i] = y[i];   // B^-T c_B of the final basis
+    if (t == 0) {
+        S.status[li] = status;
+        S.iterations[2 * li] = it1;
+        S.iterations[2 * li + 1] = it2;
+        S.objective[li] = obj;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1595,6 +1928,15 @@ int lkd_batch_crossover(const lkb_args* args, const lkx_args* xo, void* stream) 
         return (int)hipErrorInvalidValue;
     if (args->count == 0) return 0;
     LK_LAUNCH(k_crossover, args->count, XO_T, 0, stream, *args, *xo);
+    return lk_last_error();
+}
+
+int lkd_batch_simplex(const lkb_args* args, const lks_args* sx, void* stream) {
+    if (!args || !sx || !sx->x || !sx->y || !sx->basis || !sx->status || !sx->iterations || !sx->objective ||
+        args->count < 0 || sx->max_iter < 0)
+        return (int)hipErrorInvalidValue;
+    if (args->count == 0) return 0;
+    LK_LAUNCH(k_simplex, args->count, XO_T, 0, stream, *args, *sx);
     return lk_last_error();
 }
 

@@ -16,6 +16,7 @@
 #include "gpu/vram_arena.hpp"
 #include "ipm/batch.hpp"
 #include "ipm/crossover.hpp"
+#include "simplex/primal.hpp"
 #include "lakshya_device.h"
 
 namespace sankhya {
@@ -634,4 +635,51 @@ BatchSummary solve_batch_noalloc(int count, const int32_t* m, const int32_t* n,
 }
 
 }  // namespace ipm
+
+namespace simplex {
+
+void solve_batch(int count, const int32_t* m, const int32_t* n, const int32_t* colptr,
+                 const int32_t* rowidx, const double* vals, const double* b, const double* c,
+                 int32_t max_iter, int32_t* basis, double* x, double* y, int32_t* status,
+                 int32_t* iterations, double* objective) {
+    using namespace ipm;
+    if (max_iter < 0) throw std::invalid_argument("simplex::solve_batch: negative iteration limit");
+    const BatchImage img = pack_batch(count, m, n, colptr, rowidx, vals, b, c, 1);
+    const size_t out = BatchImage::rnd(sizeof(int32_t) * img.n_ym) + BatchImage::rnd(sizeof(double) * img.n_xn) +
+                       BatchImage::rnd(sizeof(double) * img.n_ym) + BatchImage::rnd(sizeof(int32_t) * 3 * count) +
+                       BatchImage::rnd(sizeof(double) * count);
+    gpu::VRAMArena arena(img.device_bytes() + out + (1u << 20));
+    void* stream = nullptr;
+    check(lkd_stream_create(&stream), "stream");
+    try {
+        auto take = [&](size_t nb) { return arena.allocate_uninitialized(nb); };
+        const BatchDevice dev = place_batch(img, take, stream);
+        lks_args sx;
+        sx.basis = static_cast<int32_t*>(take(sizeof(int32_t) * img.n_ym));
+        sx.x = static_cast<double*>(take(sizeof(double) * img.n_xn));
+        sx.y = static_cast<double*>(take(sizeof(double) * img.n_ym));
+        sx.status = static_cast<int32_t*>(take(sizeof(int32_t) * 3 * count));
+        sx.iterations = sx.status + count;
+        sx.objective = static_cast<double*>(take(sizeof(double) * count));
+        sx.max_iter = max_iter;
+        BatchOptions opt;
+        const lkb_args a = make_args(opt, dev, dev.order, count, 1);
+        check(lkd_batch_simplex(&a, &sx, stream), "simplex launch");
+        if (basis) check(lkd_memcpy_dtoh(basis, sx.basis, sizeof(int32_t) * img.n_ym, stream), "basis");
+        if (x) check(lkd_memcpy_dtoh(x, sx.x, sizeof(double) * img.n_xn, stream), "x");
+        if (y) check(lkd_memcpy_dtoh(y, sx.y, sizeof(double) * img.n_ym, stream), "y");
+        if (status) check(lkd_memcpy_dtoh(status, sx.status, sizeof(int32_t) * count, stream), "status");
+        if (iterations)
+            check(lkd_memcpy_dtoh(iterations, sx.iterations, sizeof(int32_t) * 2 * count, stream), "iterations");
+        if (objective) check(lkd_memcpy_dtoh(objective, sx.objective, sizeof(double) * count, stream), "objective");
+        check(lkd_stream_sync(stream), "simplex");
+    } catch (...) {
+        lkd_stream_sync(stream);
+        lkd_stream_destroy(stream);
+        throw;
+    }
+    check(lkd_stream_destroy(stream), "stream");
+}
+
+}  // namespace simplex
 }  // namespace sankhya

@@ -18,6 +18,7 @@
 #include "lakshya.h"
 #include "learn/mlp.hpp"
 #include "parsers/mps_reader.hpp"
+#include "simplex/primal.hpp"
 #include "lakshya_device.h"
 #include "verifier/certificate.hpp"
 
@@ -632,6 +633,15 @@ int lk_crossover(int32_t count, const int32_t* m, const int32_t* n, const int32_
                  const double* x, int32_t* basis, double* xv, int32_t* status, int32_t* pivots) {
     return guarded([&] {
         ipm::crossover_points(count, m, n, colptr, rowidx, vals, b, c, x, basis, xv, status, pivots);
+    });
+}
+int lk_simplex(int32_t count, const int32_t* m, const int32_t* n, const int32_t* colptr,
+               const int32_t* rowidx, const double* vals, const double* b, const double* c,
+               int32_t max_iter, int32_t* basis, double* x, double* y, int32_t* status,
+               int32_t* iterations, double* objective) {
+    return guarded([&] {
+        simplex::solve_batch(count, m, n, colptr, rowidx, vals, b, c, max_iter, basis, x, y, status,
+                             iterations, objective);
     });
 }
 int lk_batch_destroy(lk_batch* h) { return guarded([&] { delete h; }); }

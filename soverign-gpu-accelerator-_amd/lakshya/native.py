@@ -139,6 +139,8 @@ def lib():
         "lk_batch_crossover": [_vp, _i32p, _f64p, _i32p, _i32p],
         "lk_crossover": [C.c_int32, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
                          _i32p, _f64p, _i32p, _i32p],
+        "lk_simplex": [C.c_int32, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int32,
+                       _i32p, _f64p, _f64p, _i32p, _i32p, _f64p],
         "lk_batch_destroy": [_vp],
         "lk_batch_set_warm_start": [_vp, _vp, _vp, _vp, _vp],
         "lk_batch_solve_noalloc": [C.c_int32, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p,
@@ -539,6 +541,34 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+SIMPLEX_STATUS = {0: "Optimal", 1: "Infeasible", 2: "Unbounded", 3: "IterationLimit"}
+
+
+def simplex(lps, max_iter=10000):
+    """lk_simplex: the device Phase I / Phase II simplex route on each LP, one dict per LP:
+    {"status", "status_name", "objective", "x" (n), "y" (m), "basis" (m, -1 artificial),
+    "iterations" (Phase I, Phase II)}."""
+    m = np.array([lp.m for lp in lps], np.int32)
+    n = np.array([lp.n for lp in lps], np.int32)
+    cp = np.concatenate([np.asarray(lp.indptr, np.int32) for lp in lps])
+    ri = np.concatenate([np.asarray(lp.indices, np.int32) for lp in lps])
+    va = np.concatenate([np.asarray(lp.data, np.float64) for lp in lps])
+    b = np.concatenate([np.asarray(lp.b, np.float64) for lp in lps])
+    c = np.concatenate([np.asarray(lp.c, np.float64) for lp in lps])
+    mo = np.concatenate([[0], np.cumsum(m)]).astype(np.int64)
+    no = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+    basis = np.zeros(int(mo[-1]), np.int32)
+    x = np.zeros(int(no[-1]))
+    y = np.zeros(int(mo[-1]))
+    st = np.zeros(len(lps), np.int32)
+    it = np.zeros(2 * len(lps), np.int32)
+    obj = np.zeros(len(lps))
+    _ck(lib().lk_simplex(len(lps), m, n, cp, ri, va, b, c, int(max_iter), basis, x, y, st, it, obj))
+    return [{"status": int(st[i]), "status_name": SIMPLEX_STATUS[int(st[i])], "objective": float(obj[i]),
+             "x": x[no[i]:no[i + 1]], "y": y[mo[i]:mo[i + 1]], "basis": basis[mo[i]:mo[i + 1]],
+             "iterations": (int(it[2 * i]), int(it[2 * i + 1]))} for i in range(len(lps))]
 
 
 def crossover_points(lps, xs):

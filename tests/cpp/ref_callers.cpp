@@ -8,6 +8,10 @@
 //            reference-signature verify_optimal(model, x, pi, &res);
 //   crossover -- generate_crossover_basis on a DeviceModel from upload_to_device, the two cases
 //            of tests/unit/test_crossover.cu:15-133;
+//   simplex -- simplex::solve_with_phase1(model, basis, x) (the device route of
+//            src/simplex/phase1.cpp:124) on the models of tests/unit/test_phase1.cpp:123-430
+//            built as that file's make_model does (direct CSC), then verify_optimal with
+//            pi = B^-T c_B as the reference CLI forms it (sankhya_benchmark_cli.cpp:62-71);
 //   model -- the Model layout: bilinear_terms is a member as in src/core/problem.hpp:49.
 // Exit status 0 when every check holds; each failed check prints one line.
 #include <hip/hip_runtime_api.h>
@@ -25,6 +29,7 @@
 #include "ipm/crossover.hpp"
 #include "ipm/mehrotra.hpp"
 #include "ipm/symbolic.hpp"
+#include "simplex/primal.hpp"
 #include "verifier/certificate.hpp"
 
 using namespace sankhya;
@@ -191,6 +196,63 @@ static int run_crossover() {
     return 0;
 }
 
+// tests/unit/test_phase1.cpp:44-70: a Model from CSC arrays, lb = 0, ub = +inf unless given
+static core::Model phase1_model(Index m, Index n, std::vector<Index> cp, std::vector<Index> ri,
+                                std::vector<Float> v, std::vector<Float> b, std::vector<Float> c,
+                                OptimizationSense sense = OptimizationSense::Minimize,
+                                std::vector<Float> lb = {}, std::vector<Float> ub = {}) {
+    core::Model mo;
+    mo.sense = sense;
+    mo.obj = c;
+    mo.rhs = b;
+    mo.lb = lb.empty() ? std::vector<Float>(n, 0.0) : lb;
+    mo.ub = ub.empty() ? std::vector<Float>(n, math::kInfinity) : ub;
+    mo.vtype.assign(n, VariableType::Continuous);
+    mo.A.rows = m;
+    mo.A.cols = n;
+    mo.A.col_ptrs = cp;
+    mo.A.row_indices = ri;
+    mo.A.values = v;
+    return mo;
+}
+
+static int run_simplex() {
+    struct Case { const char* name; core::Model model; simplex::SimplexStatus status; double obj; };
+    using S = simplex::SimplexStatus;
+    std::vector<Case> cases;
+    cases.push_back({"test1", phase1_model(2, 4, {0, 2, 3, 4, 5}, {0, 1, 0, 0, 1}, {1, 1, 1, 1, 1}, {4, 2}, {-1, -1, 0, 0}), S::Optimal, -4.0});
+    cases.push_back({"test2", phase1_model(1, 2, {0, 1, 2}, {0, 0}, {1, 1}, {3}, {1, 1}), S::Optimal, 3.0});
+    cases.push_back({"test3", phase1_model(1, 1, {0, 1}, {0}, {-1}, {-5}, {1}), S::Optimal, 5.0});
+    cases.push_back({"test4", phase1_model(2, 2, {0, 2, 4}, {0, 1, 0, 1}, {1, 1, 1, -1}, {1, 0}, {1, 2}), S::Optimal, 1.5});
+    cases.push_back({"test5", phase1_model(2, 2, {0, 2, 4}, {0, 1, 0, 1}, {1, 1, 1, 1}, {1, 2}, {0, 0}), S::Infeasible, 0.0});
+    cases.push_back({"test6", phase1_model(2, 2, {0, 2, 4}, {0, 1, 0, 1}, {1, 2, 1, 2}, {3, 6}, {1, 1}), S::IterationLimit, 0.0});
+    cases.push_back({"test9", phase1_model(2, 4, {0, 2, 3, 4, 5}, {0, 1, 0, 0, 1}, {1, 1, 1, 1, 1}, {4, 2}, {-3, -2, 0, 0}), S::Optimal, -10.0});
+    // a Maximize model (the route minimizes -c) and a shifted lower bound
+    cases.push_back({"max", phase1_model(2, 4, {0, 2, 3, 4, 5}, {0, 1, 0, 0, 1}, {1, 1, 1, 1, 1}, {4, 2}, {3, 2, 0, 0},
+                                         OptimizationSense::Maximize), S::Optimal, 10.0});
+    cases.push_back({"lb", phase1_model(1, 2, {0, 1, 2}, {0, 0}, {1, 1}, {5}, {1, 2}, OptimizationSense::Minimize,
+                                        {0.0, 1.0}), S::Optimal, 6.0});
+    // a finite upper bound: the reference's unsupported case (phase1.cpp:151-161)
+    cases.push_back({"ub", phase1_model(1, 2, {0, 1, 2}, {0, 0}, {1, 1}, {5}, {1, 2}, OptimizationSense::Minimize,
+                                        {}, {3.0, math::kInfinity}), S::IterationLimit, 0.0});
+    for (auto& cs : cases) {
+        simplex::Basis basis;
+        std::vector<Float> x, pi;
+        const S st = simplex::solve_with_phase1(cs.model, basis, x, &pi);
+        EXPECT(st == cs.status);
+        if (st != cs.status) std::printf("  case %s: status %d\n", cs.name, static_cast<int>(st));
+        if (st != S::Optimal) continue;
+        double obj = 0.0;
+        for (size_t j = 0; j < x.size(); ++j) obj += cs.model.obj[j] * x[j];
+        EXPECT(std::fabs(obj - cs.obj) <= 1e-9);
+        for (Index bi : basis.basic_indices) EXPECT(bi >= 0 && bi < cs.model.A.cols);
+        Float res = -1.0;
+        EXPECT(verifier::verify_optimal(cs.model, x, pi, &res));
+        std::printf("simplex %s: objective %.6f, residual %.2e\n", cs.name, obj, res);
+    }
+    return 0;
+}
+
 static int run_model() {
     core::Model model;
     model.add_variable(1.0);
@@ -210,6 +272,7 @@ int main(int argc, char** argv) {
         if (!std::strcmp(what, "kkt") || !std::strcmp(what, "all")) run_kkt();
         if (!std::strcmp(what, "ipm") || !std::strcmp(what, "all")) run_ipm();
         if (!std::strcmp(what, "crossover") || !std::strcmp(what, "all")) run_crossover();
+        if (!std::strcmp(what, "simplex") || !std::strcmp(what, "all")) run_simplex();
     } catch (const std::exception& e) {
         std::printf("FAIL: exception %s\n", e.what());
         return 2;

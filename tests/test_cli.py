@@ -92,3 +92,34 @@ def test_generated_lp_through_mps(tmp_path, native):
     ref = native.ipm_solve(lp, relative=True)
     # two certified solves of one LP (the factor's atomics may change the last bits of a path)
     assert abs(r["objective"] - ref["objective"]) <= 2e-8 * max(1.0, abs(ref["objective"]))
+
+
+# x1 + x2 = 1 and x1 + x2 = 2 (tests/unit/test_phase1.cpp:264-283): no feasible point
+INFEASIBLE = """NAME INFEAS
+ROWS
+ N  COST
+ E  R1
+ E  R2
+COLUMNS
+    X1  R1  1  R2  1
+    X2  R1  1  R2  1
+RHS
+    RHS  R1  1  R2  2
+ENDATA
+"""
+
+
+@pytest.mark.gpu
+def test_iteration_limit_takes_the_simplex_route(tmp_path):
+    """The interior-point solve of an infeasible LP ends in IterationLimit (the reference never
+    reports Infeasible from the IPM); the fallback then runs the Phase-I / Phase-II simplex on the
+    device, whose Phase I finds no feasible point: FailedRecovery, as the reference CLI reports
+    a non-Optimal fallback (sankhya_benchmark_cli.cpp:48-75)."""
+    f = tmp_path / "infeasible.mps"
+    f.write_text(INFEASIBLE)
+    rc, out, err = _run(str(f))
+    assert rc == 0, err
+    r = _last_object(out)
+    assert set(r) == KEYS
+    assert r["fallback_triggered"] == "YES" and r["execution_path"] == "GPU+GPU_SIMPLEX"
+    assert r["status"] == "FailedRecovery" and r["certificate_pass"] is False

@@ -43,3 +43,12 @@ def test_crossover_device_model_flow(gpu):
     generate_crossover_basis(DeviceModel, x): a feasible nonsingular basis from (1, 1, 1, 1), and
     the rank-deficiency message for dependent rows."""
     _run("crossover")
+
+
+@pytest.mark.gpu
+def test_simplex_route_flow(gpu):
+    """simplex::solve_with_phase1(model, basis, x) on the models of tests/unit/test_phase1.cpp:
+    123-430 (statuses and objectives), a Maximize model, a shifted lower bound and the finite
+    upper bound the reference reports as IterationLimit; each optimum certified by the
+    reference-signature verify_optimal with pi = B^-T c_B."""
+    _run("simplex")
