@@ -419,6 +419,69 @@ static int kkt_solve(or_kkt* k, double* rhs_orig, double tol, int relmode) {
 }
 
 /* ---- exported KKT helpers (dense, small m): factor + solve on an explicit A, Theta, P ---- */
+/* Sums of the pair-map segments in the device's documented fixed order (the summation contract
+ * of the assembly kernels, include/lakshya_device.h, for the normal-equations entries of
+ * src/cuda/kkt.cu:177-215 M0_ij = sum_k A_ik theta_k A_jk): segment t holds pairs
+ * q = segptr[t] .. segptr[t+1]-1 with products pp[q] and columns pk[q];
+ *   L <= 4:   acc = fma(pp[q], theta[pk[q]], acc) over q in order from 0;
+ *   L <= 64:  16 lanes, lane l: fma over q = b+l, b+l+16, b+l+32, b+l+48 (past the end the
+ *             product is 0 * theta[pk[e-1]]), then a xor butterfly v_l += v_{l^o}, o = 8,4,2,1;
+ *   L > 64:   256 threads, thread t: fma over q = b+t, b+t+256, ..; a xor butterfly over each
+ *             group of 64 (o = 32 .. 1), then (w0 + w1) + (w2 + w3).
+ * Test infrastructure: the checker of the device assembly, bit for bit. */
+static double seg_fixed(int64_t b, int64_t e, const double* pp, const int* pk, const double* theta) {
+    const int64_t L = e - b;
+    if (L <= 4) {
+        double acc = 0.0;
+        for (int64_t q = b; q < e; ++q) acc = fma(pp[q], theta[pk[q]], acc);
+        return acc;
+    }
+    if (L <= 64) {
+        double v[16];
+        for (int l = 0; l < 16; ++l) {
+            double acc = 0.0;
+            for (int u = 0; u < 4; ++u) {
+                const int64_t q0 = b + l + 16 * u;
+                const int64_t q = q0 < e ? q0 : e - 1;
+                acc = fma(q0 < e ? pp[q] : 0.0, theta[pk[q]], acc);
+            }
+            v[l] = acc;
+        }
+        for (int o = 8; o > 0; o >>= 1) {
+            double w[16];
+            for (int l = 0; l < 16; ++l) w[l] = v[l] + v[l ^ o];
+            for (int l = 0; l < 16; ++l) v[l] = w[l];
+        }
+        return v[0];
+    }
+    double v[256];
+    for (int t = 0; t < 256; ++t) {
+        double acc = 0.0;
+        for (int64_t q = b + t; q < e; q += 256) acc = fma(pp[q], theta[pk[q]], acc);
+        v[t] = acc;
+    }
+    double sw[4];
+    for (int wv = 0; wv < 4; ++wv) {
+        double* x = v + 64 * wv;
+        for (int o = 32; o > 0; o >>= 1) {
+            double w[64];
+            for (int l = 0; l < 64; ++l) w[l] = x[l] + x[l ^ o];
+            for (int l = 0; l < 64; ++l) x[l] = w[l];
+        }
+        sw[wv] = x[0];
+    }
+    return (sw[0] + sw[1]) + (sw[2] + sw[3]);
+}
+
+void or_fma_vec(int64_t n, const double* a, const double* b, const double* c, double* out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = fma(a[i], b[i], c[i]);
+}
+
+void or_segment_sums(int64_t nseg, const int64_t* segptr, const double* pp, const int* pk,
+                     const double* theta, double* out) {
+    for (int64_t t = 0; t < nseg; ++t) out[t] = seg_fixed(segptr[t], segptr[t + 1], pp, pk, theta);
+}
+
 int or_kkt_factor_solve(int m, const int* rp, const int* ci, const double* val,
                         const double* theta, const int* P, double* Lout_eq_free,
                         double* rhs, double tol, int* success) {

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 
 
@@ -50,6 +51,8 @@ def lib():
         L.or_symbolic.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p,
                                   C.c_int64]
         L.or_assemble_dense.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _i32p, _f64p]
+        L.or_segment_sums.argtypes = [C.c_int64, _i64p, _f64p, _i32p, _f64p, _f64p]
+        L.or_fma_vec.argtypes = [C.c_int64, _f64p, _f64p, _f64p, _f64p]
         L.or_kkt_factor_solve.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _i32p,
                                           C.c_void_p, C.c_void_p, C.c_double,
                                           C.POINTER(C.c_int)]
@@ -69,6 +72,23 @@ def _i32(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def segment_sums(segptr, pp, pk, theta):
+    """Pair-map segment sums in the device assembly's fixed order (exact fma)."""
+    nseg = len(segptr) - 1
+    out = np.zeros(nseg)
+    lib().or_segment_sums(nseg, np.ascontiguousarray(segptr, np.int64), _f64(pp),
+                          np.ascontiguousarray(pk, np.int32), _f64(theta), out)
+    return out
+
+
+def fma(a, b, c):
+    """Elementwise a * b + c with one rounding (C fma)."""
+    a, b, c = np.broadcast_arrays(np.asarray(a, np.float64), np.asarray(b, np.float64), np.asarray(c, np.float64))
+    out = np.zeros(a.shape)
+    lib().or_fma_vec(a.size, _f64(a.ravel()), _f64(b.ravel()), _f64(c.ravel()), out.reshape(-1))
+    return out
 
 
 def ruiz(rp, ci, val, b, c):
