@@ -36,7 +36,10 @@ enum class FactorPrecision { FP64 = 0, FP32 = 1 };
 struct KKTOptions {
     ipm::SupernodeOptions supernodes;
     bool use_graphs = false;        // launch-graph capture: not supported (true is rejected)
-    bool step_dag = true;           // all STEP tiles of a level in one dependency-driven launch
+    // STEP tiles: 0 one launch per block column; 1 all tiles of a level in one dependency-driven
+    // launch unless one supernode dominates the level (then one launch per block column);
+    // 2 one dependency-driven launch for every level
+    int step_dag = 1;
     void* stream = nullptr;         // hipStream_t to run on (nullptr: the solver creates one)
     // factor_equilibrated regularization: delta_r = delta0 * 10^r, r < delta_retries
     // (src/cuda/kkt.cu:652-690 uses 1e-9 and 5 for both precisions)
@@ -78,7 +81,8 @@ public:
     // Host-only construction of the complete device plan (structure, pair map, task lists) for
     // inspection and CPU emulation; keys name the arrays the device would receive.
     static std::map<std::string, std::vector<char>> export_plan(
-        const core::CSRMatrix& A, const std::vector<Index>& P, const ipm::SupernodeOptions& so);
+        const core::CSRMatrix& A, const std::vector<Index>& P, const ipm::SupernodeOptions& so,
+        int step_dag = 1);
     ~GPUKKTCholeskySolver();
     GPUKKTCholeskySolver(const GPUKKTCholeskySolver&) = delete;
     GPUKKTCholeskySolver& operator=(const GPUKKTCholeskySolver&) = delete;

@@ -62,11 +62,13 @@ int lk_supernodal(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* co
                   int32_t* perm_out, int32_t* sn_first_out, int32_t* nsuper_out);
 
 /* ---- plan inspection (host only): the device plan of the KKT solver for an ordering P
- * (NULL -> AMD).  lk_plan_array copies the named array (call with dst NULL for its size). */
+ * (NULL -> AMD).  step_dag: 0 one STEP launch per block column, 1 the solver's choice per level,
+ * 2 one dependency-driven launch per level.  lk_plan_array copies the named array (call with
+ * dst NULL for its size). */
 typedef struct lk_plan lk_plan;
 int lk_plan_create(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
                    const double* vals, const int32_t* P, int32_t relax_small, double relax_frac,
-                   lk_plan** out);
+                   int32_t step_dag, lk_plan** out);
 int lk_plan_array(lk_plan* p, const char* name, void* dst, size_t capacity, size_t* bytes);
 int lk_plan_destroy(lk_plan* p);
 

@@ -301,7 +301,7 @@ int lk_supernodal(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* co
 
 int lk_plan_create(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* colidx,
                    const double* vals, const int32_t* P, int32_t relax_small, double relax_frac,
-                   lk_plan** out) {
+                   int32_t step_dag, lk_plan** out) {
     return guarded([&] {
         auto A = make_csr(m, n, rowptr, colidx, vals);
         std::vector<Index> p = P ? std::vector<Index>(P, P + m) : ipm::compute_fill_reducing_ordering(A);
@@ -309,7 +309,8 @@ int lk_plan_create(int32_t m, int32_t n, const int32_t* rowptr, const int32_t* c
         if (relax_small >= 0) so.relax_small = relax_small;
         if (relax_frac >= 0) so.relax_frac = relax_frac;
         std::unique_ptr<lk_plan> h(new lk_plan);
-        h->arrays = gpu::GPUKKTCholeskySolver::export_plan(A, p, so);
+        if (step_dag < 0 || step_dag > 2) throw std::invalid_argument("lk_plan_create: step_dag must be 0, 1 or 2");
+        h->arrays = gpu::GPUKKTCholeskySolver::export_plan(A, p, so, step_dag);
         *out = h.release();
     });
 }

@@ -443,6 +443,28 @@ struct GPUKKTCholeskySolver::Impl {
         return off;
     }
 
+    // A STEPS launch keeps every supernode on one XCD (its tiles hand data to each other through
+    // that XCD's L2).  A level whose work one or a few supernodes dominate -- a dense top
+    // separator -- would then leave most XCDs idle; such a level takes one launch per step
+    // instead, whose tiles spread over all XCDs and whose hand-offs are the launch boundaries.
+    // Balanced: the busiest XCD of a largest-first assignment (dense-trapezoid flops per
+    // supernode) holds at most 1.5 times the mean.
+    bool xcd_balanced(const std::vector<Index>& lv) const {
+        std::vector<double> w;
+        double total = 0.0;
+        for (Index s : lv) {
+            const double nc = S.sn_first[s + 1] - S.sn_first[s];
+            const double ld = static_cast<double>(S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
+            const double f = nc * ld * ld - nc * nc * ld + nc * nc * nc / 3.0;
+            w.push_back(f);
+            total += f;
+        }
+        std::sort(w.begin(), w.end(), std::greater<double>());
+        std::vector<double> bin(static_cast<size_t>(n_xcd), 0.0);
+        for (double f : w) *std::min_element(bin.begin(), bin.end()) += f;
+        return *std::max_element(bin.begin(), bin.end()) <= 1.5 * total / n_xcd;
+    }
+
     void build_factor_plan() {
         const Index ns = S.nsuper;
         diag_off_start.assign(static_cast<size_t>(ns) + 1, 0);
@@ -497,7 +519,7 @@ struct GPUKKTCholeskySolver::Impl {
             // staging slots: with one launch per step, two per supernode of the level (block k
             // is read from slot k % 2 while block k + 1 is written to the other one); with the
             // STEPS launch, where steps overlap, one per block column of the level
-            const bool dag = opt.step_dag;
+            const bool dag = opt.step_dag == 2 || (opt.step_dag == 1 && xcd_balanced(lv));
             std::vector<int64_t> blkbase(lv.size() + 1, 0);
             for (size_t si = 0; si < lv.size(); ++si) {
                 const Index nc = S.sn_first[lv[si] + 1] - S.sn_first[lv[si]];
@@ -1616,12 +1638,14 @@ GPUKKTCholeskySolver::GPUKKTCholeskySolver(VRAMArena& arena, const core::CSRMatr
     : impl_(new Impl(&arena, A, nullptr, &structure, opt)) {}
 
 std::map<std::string, std::vector<char>> GPUKKTCholeskySolver::export_plan(
-    const core::CSRMatrix& A, const std::vector<Index>& P, const ipm::SupernodeOptions& so) {
+    const core::CSRMatrix& A, const std::vector<Index>& P, const ipm::SupernodeOptions& so,
+    int step_dag) {
     std::map<std::string, std::vector<char>> out;
     ipm::SymbolicFactorization sym;
     ipm::compute_symbolic_factorization(A, P, sym);
     KKTOptions ko;
     ko.supernodes = so;
+    ko.step_dag = step_dag;
     Impl impl(nullptr, A, &sym, nullptr, ko, &out);
     return out;
 }

@@ -751,18 +751,19 @@ PLAN_DTYPES = {"perm": np.int32, "sn_rowptr": np.int64, "sn_rows": np.int32,
                "squeue": np.int32, "sqoff": np.int32}
 
 
-def plan_export(rp, ci, val, n, P=None, relax_small=-1, relax_frac=-1.0):
-    """Host-only device plan of the KKT solver (dict of numpy arrays)."""
+def plan_export(rp, ci, val, n, P=None, relax_small=-1, relax_frac=-1.0, step_dag=1):
+    """Host-only device plan of the KKT solver (dict of numpy arrays).  step_dag: 0 one STEP
+    launch per block column, 1 the solver's choice per level, 2 one STEPS launch per level."""
     L = lib()
     L.lk_plan_create.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _vp, C.c_int32,
-                                 C.c_double, C.POINTER(_vp)]
+                                 C.c_double, C.c_int32, C.POINTER(_vp)]
     L.lk_plan_array.argtypes = [_vp, C.c_char_p, _vp, C.c_size_t, C.POINTER(C.c_size_t)]
     L.lk_plan_destroy.argtypes = [_vp]
     m = len(rp) - 1
     h = _vp()
     Pp = _i32(P) if P is not None else None
     _ck(L.lk_plan_create(m, n, _i32(rp), _i32(ci), _f64(val), _ptr(Pp), relax_small,
-                         relax_frac, C.byref(h)))
+                         relax_frac, int(step_dag), C.byref(h)))
     out = {}
     try:
         for name, dt in PLAN_DTYPES.items():

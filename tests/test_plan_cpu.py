@@ -22,18 +22,19 @@ def _cases():
     }
 
 
-@pytest.mark.parametrize("name", ["random_sparse", "block_angular", "transport", "dense",
-                                  "block_angular_wide", "dense_wide"])
-def test_schedule_reproduces_dense_cholesky(native, name):
+@pytest.mark.parametrize("name,steps", [(c, 1) for c in ("random_sparse", "block_angular", "transport",
+                                                         "dense", "block_angular_wide", "dense_wide")] +
+                         [("dense_wide", 0), ("dense_wide", 2), ("block_angular_wide", 2)])
+def test_schedule_reproduces_dense_cholesky(native, name, steps):
     lp = _cases()[name]()
     rp, ci, va = lp.csr()
     m, n = lp.m, lp.n
     rng = np.random.default_rng(7)
     theta = np.exp(rng.uniform(-2, 2, n))
     P = native.order_amd(rp, ci, n)
-    plan = native.plan_export(rp, ci, va, n, P=P)
+    plan = native.plan_export(rp, ci, va, n, P=P, step_dag=steps)
     kinds = set(plan["flaunch"].reshape(-1, 3)[:, 0].tolist())
-    assert kinds <= {1, 2, 3, 7}          # STEP tiles of a level in one STEPS launch
+    assert kinds <= {1, 2, 3, 5, 6, 7}    # STEPS launches, or PART / STEP launches per step
     em = Emulator(plan)
     D = em.assemble(theta, equil=True, delta=1e-9)
     assert em.factor(), f"emulated factorization failed (err {em.err})"
@@ -120,7 +121,9 @@ def test_steps_dependencies_order_every_conflict(native, name):
     dependency's rows flag before the GEMM, the staging flag before the solve)."""
     lp = _cases()[name]()
     rp, ci, va = lp.csr()
-    plan = native.plan_export(rp, ci, va, lp.n, P=native.order_amd(rp, ci, lp.n))
+    # every level in a STEPS launch (the solver's own choice keeps a dense separator's level
+    # in launches per block column)
+    plan = native.plan_export(rp, ci, va, lp.n, P=native.order_amd(rp, ci, lp.n), step_dag=2)
     T, deps, sdep = plan["ftasks"], plan["fdeps"].reshape(-1, 4), plan["fsdep"]
     nsteps = 0
     for kind, b, c in plan["flaunch"].reshape(-1, 3):
@@ -165,3 +168,36 @@ def test_steps_dependencies_order_every_conflict(native, name):
                         continue
                     assert (anc[ny] >> nx) & 1 or (anc[nx] >> ny) & 1, (name, b, i, j, x, y)
     assert nsteps > 0
+
+
+def _level_kinds(plan):
+    """Launch kinds of the factor plan between consecutive DIAG launches (one group per level)."""
+    groups, cur = [], []
+    for kind, b, c in plan["flaunch"].reshape(-1, 3):
+        if kind == 1 and cur:
+            groups.append(cur)
+            cur = []
+        cur.append((int(kind), int(c)))
+    groups.append(cur)
+    return groups
+
+
+def test_steps_launch_only_where_the_xcds_share_the_work(native):
+    """A STEPS launch keeps each supernode on one XCD: 16 equal diagonal blocks (two per XCD)
+    are factored in one STEPS launch; one dense supernode, which would occupy a single XCD,
+    gets a PART / STEP launch pair per block column (its tiles spread over every XCD)."""
+    from lakshya import gen
+    lp = gen.block_angular(16, 150, 400, 6, 8, 0.2, seed=3)
+    rp, ci, va = lp.csr()
+    plan = native.plan_export(rp, ci, va, lp.n, P=native.order_amd(rp, ci, lp.n))
+    sf, rpt = plan["sn_first"], plan["sn_rowptr"]
+    assert (np.diff(sf) >= 100).sum() == 16
+    lv0 = _level_kinds(plan)[0]
+    assert [k for k, _ in lv0 if k in (5, 6, 7)] == [7]
+    lp = gen.dense_small(700, 1400, 0.02, seed=6)
+    rp, ci, va = lp.csr()
+    plan = native.plan_export(rp, ci, va, lp.n, P=native.order_amd(rp, ci, lp.n))
+    ncol = np.diff(plan["sn_first"])
+    assert ncol.max() >= 0.9 * lp.m                      # one dense supernode
+    kinds = [k for g in _level_kinds(plan) for k, _ in g]
+    assert 7 not in kinds and kinds.count(5) >= ncol.max() // 64 - 1
