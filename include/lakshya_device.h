@@ -229,6 +229,15 @@ int lkd_trsv_level(int prec, int dir, int nw, int g, int32_t ntask, const lkd_ta
                    double* below, double* pbuf, uint32_t* flags, const uint32_t* epoch,
                    const int32_t* queue, const int32_t* qoff, int32_t nx, uint32_t* cnt,
                    int32_t grid, int32_t* err, void* stream);
+/* Wide level solve (dir 0 forward, 1 backward) for levels of few supernodes with many blocks:
+ * one workgroup per row tile (forward) or block column (backward) of each supernode, tasks
+ * taken in the order `order` from the ticket counter cnt (reset by lkd_trsv_epoch); task r0 =
+ * tile / block index, px = the supernode's flags (nb forward, then nb backward).  Blocks are
+ * handed between workgroups on any XCD through device-coherent stores and flags holding the
+ * epoch.  Same error bits as lkd_trsv_level. */
+int lkd_trsv_wide(int prec, int dir, int32_t ntask, const lkd_task* tasks, const int32_t* order,
+                  uint32_t* cnt, const void* panel, const void* diag, const int32_t* sn_rows,
+                  double* x, uint32_t* flags, const uint32_t* epoch, int32_t* err, void* stream);
 
 /* ---------------------------------------------------------------- KKT solve vector kernels */
 /* r[i] = rhs[P[i]]; req[i] = D[i] * r[i] */

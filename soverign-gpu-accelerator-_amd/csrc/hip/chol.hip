@@ -469,7 +469,9 @@ __device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t lo,
 
 // Contribution of the below-diagonal rows of a supernode to its ancestors, K range
 // [kbeg, kend): U = L21[r0 rows] * L21[c0 rows]^T subtracted (atomic add) from the ancestor
-// panels.  The tile is staged through LDS 64 rows at a time for the scatter.
+// panels.  The tile is staged through LDS 64 rows at a time for the scatter.  A target
+// supernode's row list starts with its own columns in order, so a row inside them sits at
+// position row - first; only rows below it are searched in the rest of the list.
 template <typename T>
 __global__ __launch_bounds__(256) void k_update(const lkd_task* __restrict__ tasks, int ntask,
                                                 T* __restrict__ L, lkd_snplan plan,
@@ -482,8 +484,10 @@ __global__ __launch_bounds__(256) void k_update(const lkd_task* __restrict__ tas
     __shared__ GemmLds<T> sm;
     __shared__ int64_t tbase[NC];
     __shared__ int64_t trow0[NC];
-    __shared__ int64_t tlo[NC];
+    __shared__ int32_t tlo[NC];
     __shared__ int32_t tld[NC];
+    __shared__ int32_t tfirst[NC];
+    __shared__ int32_t tnc[NC];
     T* P = L + tk.off;
     const int64_t ld = tk.ld;
     const int ra = min(MT, tk.ld - tk.r0), rb = min(NC, tk.ld - tk.c0);
@@ -502,6 +506,8 @@ __global__ __launch_bounds__(256) void k_update(const lkd_task* __restrict__ tas
         trow0[t] = rp;
         tlo[t] = gj - f;
         tld[t] = ldt;
+        tfirst[t] = f;
+        tnc[t] = plan.sn_first[s + 1] - f;
     }
     T* Cs = reinterpret_cast<T*>(&sm);   // 64 x DP staging tile (the GEMM buffers are free now)
     for (int half = 0; half < 2; ++half) {
@@ -515,11 +521,15 @@ __global__ __launch_bounds__(256) void k_update(const lkd_task* __restrict__ tas
         for (int q = 0; q < 16; ++q) {
             const int c = cg + 4 * q;
             if (c >= rb || tk.r0 + rr < tk.c0 + c) continue;
-            const int32_t* trows = plan.sn_rows + trow0[c];
-            const int64_t pos = lower_bound_i32(trows, tlo[c], tld[c], gi);
-            if (pos >= tld[c] || trows[pos] != gi) {   // structure violated: report, never write
-                atomicOr(err, 2);
-                continue;
+            const int32_t d = gi - tfirst[c];
+            int64_t pos = d;
+            if (d < tlo[c] || d >= tnc[c]) {              // below the target's columns: search
+                const int32_t* trows = plan.sn_rows + trow0[c];
+                pos = lower_bound_i32(trows, max(tlo[c], tnc[c]), tld[c], gi);
+                if (pos >= tld[c] || trows[pos] != gi) {   // structure violated: report, never write
+                    atomicOr(err, 2);
+                    continue;
+                }
             }
             atomic_add(L + tbase[c] + pos, -Cs[c * DP + rl]);
         }

@@ -606,6 +606,257 @@ int level_solve(int dir, int32_t ntask, const lkd_task* tasks, const void* panel
     return lk_last_error();
 }
 
+// ------------------------------------------------------------------------------------------
+// Wide level solves, for levels of few supernodes with many blocks (a dense top separator),
+// where a few workgroups per supernode would leave the device idle.  One 256-thread workgroup
+// per row tile (forward) or per block column (backward) of every supernode of the level; the
+// workgroups take their tasks from one ticket counter, in an order where every task's
+// producers come first, so the earliest unfinished task can always run.  The tasks of one
+// supernode run on all XCDs, so a solved block is handed over through the device-coherent
+// level: the producer writes it with device-scope relaxed atomic stores (written through its
+// XCD's L2), waits until they completed and sets the block's flag to the solve's epoch;
+// consumers poll the flag and read the block with device-scope atomic loads.  Everything else
+// a task reads is final before the launch (panel, diagonal copies, its own right-hand side, the
+// ancestors' entries).  Wavefront v handles columns 16 v .. 16 v + 15 of each 64-column block
+// (lane = row of a 64-row tile), so the four wavefronts follow the chain of blocks in step.
+//  forward   task (s, t): acc = sum over blocks j < min(t, nb) of L(t, j) z_j, each block's
+//            panel data loaded one block ahead of its wait; a diagonal tile solves
+//            z_t = L_tt^{-1} (b_t - acc) and publishes it, a tile of rows below the supernode
+//            adds -acc to the ancestors' entries (atomics).
+//  backward  task (s, j): acc = sum over tiles i > j of L(i, j)^T x_i: the rows below the
+//            supernode first (their x is final), then the diagonal tiles nb-1 .. j+1 as their
+//            blocks are published; x_j = L_jj^{-T} (z_j - acc) is published.
+// Per lane and column the products are summed in tile order, the wavefronts' or rows' partial
+// sums in a fixed order: results do not depend on timing.
+__device__ __forceinline__ bool wide_wait(const uint32_t* flag, uint32_t ep, int32_t* err) {
+    long long n = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != ep) {
+        if ((n & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        if (++n >= kSpinLimit) {
+            if ((threadIdx.x & 63) == 0) atomicOr(err, 0x10);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+__device__ __forceinline__ double wide_load(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void wide_store(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// after the wavefront's block stores: wait until they completed, then lane 0 sets the flag
+__device__ __forceinline__ void wide_publish(uint32_t* flag, uint32_t ep) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(flag, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// next ticket -> task index (-1: none left)
+__device__ __forceinline__ int wide_task(const int32_t* order, int ntask, uint32_t* cnt, int* s_ti) {
+    if (threadIdx.x == 0) {
+        const int k = (int)atomicAdd(cnt, 1u);
+        *s_ti = k < ntask ? order[k] : -1;
+    }
+    __syncthreads();
+    return *s_ti;
+}
+
+// 16 columns (16 v + c of block j) of the 64-row tile starting at `row` (clamped to a panel row
+// by the caller); columns >= kb and lanes with !rok read as zero (clamped unconditional loads)
+template <typename T>
+__device__ __forceinline__ void wide_cols(T (&h)[16], const T* __restrict__ P, int64_t ld, int j, int kb,
+                                          int64_t row, bool rok) {
+    const int v = threadIdx.x >> 6;
+    const T* base = P + (int64_t)j * TS * ld + row;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const int cc = 16 * v + c;
+        const T e = base[(int64_t)min(cc, kb - 1) * ld];
+        h[c] = (rok && cc < kb) ? e : T(0);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_wide(const lkd_task* __restrict__ tasks,
+                                                  const int32_t* __restrict__ order, int ntask,
+                                                  uint32_t* cnt, const T* __restrict__ L,
+                                                  const T* __restrict__ diag,
+                                                  const int32_t* __restrict__ sn_rows, double* x,
+                                                  uint32_t* flags, const uint32_t* epoch,
+                                                  int32_t* err) {
+    __shared__ int s_ti, s_ok;
+    __shared__ double Ds[TS * 65];
+    __shared__ double red[4][TS];
+    const int ti = wide_task(order, ntask, cnt, &s_ti);
+    if (ti < 0) return;
+    const lkd_task tk = tasks[ti];
+    const uint32_t ep = *epoch;
+    const T* P = L + tk.off;
+    const int64_t ld = tk.ld;
+    const Tiles tl(ld, tk.ncol);
+    const int nb = tl.nb, nc = tk.ncol, t = tk.r0;
+    const int v = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t rb = tl.beg(t);
+    const bool rok = rb + lane < tl.end(t);
+    const int64_t row = rok ? rb + lane : rb;
+    const int nj = min(t, nb);
+    const uint32_t* fl = flags + tk.px;
+    double* xs = x + tk.first;
+    const int kbt = t < nb ? min(TS, nc - t * TS) : 0;
+    if (threadIdx.x == 0) s_ok = 1;
+    if (t < nb && v == 0) stage_diag<T>(diag + tk.aux + (int64_t)t * TS * TS, kbt, Ds);
+    // the tile's right-hand side is final before the launch: loaded ahead of the chain
+    const double b = lane < kbt ? xs[(int64_t)t * TS + lane] : 0.0;
+    double acc = 0.0;
+    bool ok = true;
+    auto apply = [&](const T (&h)[16], int j) {
+        if (!ok || !wide_wait(fl + j, ep, err)) {
+            ok = false;
+            return;
+        }
+        const int kb = min(TS, nc - j * TS);
+        const int cc = 16 * v + (lane & 15);
+        const double zr = wide_load(xs + (int64_t)j * TS + min(cc, kb - 1));
+        const double z = cc < kb ? zr : 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc = fma((double)h[c], bcast_lane(z, c), acc);
+    };
+    T h0[16], h1[16];
+    if (nj > 0) wide_cols<T>(h0, P, ld, 0, min(TS, nc), row, rok);
+#pragma unroll 1
+    for (int j = 0; j < nj; j += 2) {
+        if (j + 1 < nj) wide_cols<T>(h1, P, ld, j + 1, min(TS, nc - (j + 1) * TS), row, rok);
+        apply(h0, j);
+        if (j + 1 >= nj) break;
+        if (j + 2 < nj) wide_cols<T>(h0, P, ld, j + 2, min(TS, nc - (j + 2) * TS), row, rok);
+        apply(h1, j + 1);
+    }
+    if (!ok) s_ok = 0;
+    red[v][lane] = acc;
+    __syncthreads();
+    if (!s_ok || v != 0) return;
+    const double s = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (t < nb) {
+        const double z = fsub_staged(b - s, Ds, kbt);
+        if (lane < kbt) wide_store(xs + (int64_t)t * TS + lane, z);
+        wide_publish(flags + tk.px + t, ep);
+    } else if (rok) {
+        atomic_add(x + sn_rows[tk.aux2 + row], -s);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_wide(const lkd_task* __restrict__ tasks,
+                                                  const int32_t* __restrict__ order, int ntask,
+                                                  uint32_t* cnt, const T* __restrict__ L,
+                                                  const T* __restrict__ diag,
+                                                  const int32_t* __restrict__ sn_rows, double* x,
+                                                  uint32_t* flags, const uint32_t* epoch,
+                                                  int32_t* err) {
+    __shared__ int s_ti, s_ok;
+    __shared__ double red[4][16][65];
+    const int ti = wide_task(order, ntask, cnt, &s_ti);
+    if (ti < 0) return;
+    const lkd_task tk = tasks[ti];
+    const uint32_t ep = *epoch;
+    const T* P = L + tk.off;
+    const int64_t ld = tk.ld;
+    const Tiles tl(ld, tk.ncol);
+    const int nb = tl.nb, nt = tl.nt, nc = tk.ncol, j = tk.r0;
+    const int v = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kbj = min(TS, nc - j * TS);
+    const uint32_t* fl = flags + tk.px + nb;
+    double* xs = x + tk.first;
+    if (threadIdx.x == 0) s_ok = 1;
+    // the block's right-hand side z_j is final before the launch: loaded ahead of the chain
+    const double rhs = lane < kbj ? xs[(int64_t)j * TS + lane] : 0.0;
+    double part[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) part[c] = 0.0;
+    bool ok = true;
+    // tile i's 16 columns of block j, and (rows below the supernode) the ancestors' x
+    auto load = [&](T (&h)[16], double& xr, int i) {
+        const int64_t rb = tl.beg(i);
+        const bool rok = rb + lane < tl.end(i);
+        const int64_t row = rok ? rb + lane : rb;
+        wide_cols<T>(h, P, ld, j, kbj, row, rok);
+        if (i >= nb) xr = rok ? x[sn_rows[tk.aux2 + row]] : 0.0;
+    };
+    auto apply = [&](const T (&h)[16], double xr, int i) {
+        if (!ok) return;
+        if (i < nb) {
+            if (!wide_wait(fl + i, ep, err)) {
+                ok = false;
+                return;
+            }
+            const int kb = min(TS, nc - i * TS);
+            const double xv = wide_load(xs + (int64_t)i * TS + min(lane, kb - 1));
+            xr = lane < kb ? xv : 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < 16; ++c) part[c] = fma((double)h[c], xr, part[c]);
+    };
+    // the diagonal block's columns (lane = column): row i of the row-major copy, diagonal
+    // entry zeroed; rd = 1 / L_jj (stored on the copy's diagonal)
+    T col[TS];
+    double rd = 0.0;
+    auto load_diag = [&]() {
+        const T* Lr = diag + tk.aux + (int64_t)j * TS * TS;
+        if (kbj == TS) {
+#pragma unroll
+            for (int i = 0; i < TS; ++i) col[i] = Lr[i * TS + lane];
+        } else {
+#pragma unroll
+            for (int i = 0; i < TS; ++i)       // clamped loads times a 0/1 mask: no branches
+                col[i] = Lr[min(i, kbj - 1) * kbj + min(lane, kbj - 1)] * ((i < kbj && lane < kbj) ? T(1) : T(0));
+        }
+#pragma unroll
+        for (int i = 0; i < TS; ++i) col[i] = i == lane ? T(0) : col[i];
+        const int dl = min(lane, kbj - 1);
+        rd = lane < kbj ? (double)Lr[dl * kbj + dl] : 0.0;
+    };
+    T ha[16], hb[16];
+    double xa = 0.0, xb = 0.0;
+    int i = nt - 1;
+    if (i > j) load(ha, xa, i);
+    // ha holds tile i; the last tile, j + 1, is applied after the diagonal copy is loaded
+#pragma unroll 1
+    while (i > j + 1) {
+        load(hb, xb, i - 1);
+        apply(ha, xa, i);
+        --i;
+        if (i == j + 1) break;
+        load(ha, xa, i - 1);
+        apply(hb, xb, i);
+        --i;
+    }
+    if (v == 0) load_diag();
+    if (i == j + 1) {
+        if (((nt - 1 - i) & 1) == 0) apply(ha, xa, i);
+        else apply(hb, xb, i);
+    }
+    if (!ok) s_ok = 0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) red[v][c][lane] = part[c];
+    __syncthreads();
+    if (!s_ok || v != 0) return;
+    // lane l: column l of the block = column l & 15 of wavefront l >> 4, rows in order
+    const double* rr = red[lane >> 4][lane & 15];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int r = 0; r < TS; r += 4) {
+        s0 += rr[r];
+        s1 += rr[r + 1];
+        s2 += rr[r + 2];
+        s3 += rr[r + 3];
+    }
+    const double s = (s0 + s1) + (s2 + s3);
+    const double val = lane < kbj ? rhs - s : 0.0;
+    const double sol = bsub_regs<T>(val, col, rd);
+    if (lane < kbj) wide_store(xs + (int64_t)j * TS + lane, sol);
+    wide_publish(flags + tk.px + nb + j, ep);
+}
+
 // a new solve: the epoch advances and the queue counters restart
 __global__ void k_epoch(uint32_t* epoch, uint32_t* cnt, int ncnt) {
     if (threadIdx.x == 0) atomicAdd(epoch, 1u);
@@ -645,6 +896,29 @@ int lkd_trsv_level(int prec, int dir, int nw, int g, int32_t ntask, const lkd_ta
     return prec == LKD_F32
                ? level_solve_cfg<float>(nw, g, dir, ntask, tasks, panel, diag, sn_rows, x, below, pbuf, flags, epoch, err, xq, grid, stream)
                : level_solve_cfg<double>(nw, g, dir, ntask, tasks, panel, diag, sn_rows, x, below, pbuf, flags, epoch, err, xq, grid, stream);
+}
+
+int lkd_trsv_wide(int prec, int dir, int32_t ntask, const lkd_task* tasks, const int32_t* order,
+                  uint32_t* cnt, const void* panel, const void* diag, const int32_t* sn_rows,
+                  double* x, uint32_t* flags, const uint32_t* epoch, int32_t* err, void* stream) {
+    if (ntask <= 0) return 0;
+    if ((dir != 0 && dir != 1) || !tasks || !order || !cnt || !flags || !epoch) return (int)hipErrorInvalidValue;
+    if (prec == LKD_F32) {
+        if (dir == 0)
+            LK_LAUNCH(k_fwd_wide<float>, ntask, 256, 0, stream, tasks, order, ntask, cnt, (const float*)panel,
+                      (const float*)diag, sn_rows, x, flags, epoch, err);
+        else
+            LK_LAUNCH(k_bwd_wide<float>, ntask, 256, 0, stream, tasks, order, ntask, cnt, (const float*)panel,
+                      (const float*)diag, sn_rows, x, flags, epoch, err);
+    } else {
+        if (dir == 0)
+            LK_LAUNCH(k_fwd_wide<double>, ntask, 256, 0, stream, tasks, order, ntask, cnt, (const double*)panel,
+                      (const double*)diag, sn_rows, x, flags, epoch, err);
+        else
+            LK_LAUNCH(k_bwd_wide<double>, ntask, 256, 0, stream, tasks, order, ntask, cnt, (const double*)panel,
+                      (const double*)diag, sn_rows, x, flags, epoch, err);
+    }
+    return lk_last_error();
 }
 
 int lkd_trsv_epoch(uint32_t* epoch, uint32_t* cnt, int32_t ncnt, void* stream) {

@@ -47,6 +47,8 @@ constexpr int SOLVE_WIDE_TILES = 8;   // level solves: 8 wavefronts per supernod
 constexpr int SOLVE_MAX_G = 4;        // forward level solves: workgroups per wide supernode
 constexpr int SOLVE_BWD_G = 2;        // backward level solves: workgroups per wide supernode
 constexpr int FWD_ROWS = 256;
+constexpr int WIDE_MIN_BLOCKS = 16;   // level solves: levels of at most WIDE_MAX_SN supernodes, one
+constexpr int WIDE_MAX_SN = 8;        // of at least WIDE_MIN_BLOCKS blocks, take the wide form
 
 // Order a launch's tasks for dispatch.  Workgroup b runs task (b % 8) * per + b / 8 with
 // per = ceil(n / 8): segment x of `per` consecutive tasks runs on XCD x, in list order.  Inside
@@ -467,6 +469,12 @@ struct GPUKKTCholeskySolver::Impl {
 
     void build_factor_plan() {
         const Index ns = S.nsuper;
+        // the UPDATE scatter places a row inside a target's columns at row - first without a
+        // search: every supernode's row list must start with its own columns, in order
+        for (Index s = 0; s < ns; ++s)
+            for (Index k = 0; k < S.sn_first[s + 1] - S.sn_first[s]; ++k)
+                if (S.sn_rows[S.sn_rowptr[s] + k] != S.sn_first[s] + k)
+                    throw std::logic_error("KKT plan: a supernode's rows do not start with its columns");
         diag_off_start.assign(static_cast<size_t>(ns) + 1, 0);
         for (Index s = 0; s < ns; ++s) {
             const Index nc = S.sn_first[s + 1] - S.sn_first[s];
@@ -961,6 +969,16 @@ struct GPUKKTCholeskySolver::Impl {
                     bad("solve diagonal copies");
                 break;
             }
+            case 22:
+            case 23: {  // wide level solves: a tile (forward) / block column (backward) of a supernode
+                const int nb = (nc + TILE - 1) / TILE;
+                const int nt = nb + static_cast<int>((ld - nc + TILE - 1) / TILE);
+                if (t.r0 < 0 || t.r0 >= (kind == 22 ? nt : nb)) bad("wide solve tile");
+                if (t.px < 0 || t.px + 2 * int64_t(nb) > solve_flags) bad("wide solve flags");
+                if (!diag_ok(t.aux, 0) || t.aux + int64_t(nb - 1) * TILE * TILE + int64_t(nc - (nb - 1) * TILE) * (nc - (nb - 1) * TILE) > diag_elems)
+                    bad("solve diagonal copies");
+                break;
+            }
             default: bad("kind");
         }
     }
@@ -980,15 +998,58 @@ struct GPUKKTCholeskySolver::Impl {
         int cus = 0;
         if (lkd_device_cus(&cus) != 0 || cus < 1) cus = 0;   // no device: one workgroup each
         int64_t nbelow = 0, npbuf = 0, nflags = 0;
+        std::vector<char> wide;
         for (const auto& lv : levels) {
             int nw = 4, maxnt = 1;
+            Index maxnc = 0;
             for (const Index s : lv) {
                 const Index nc = S.sn_first[s + 1] - S.sn_first[s];
                 const int64_t ld = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
                 const int nt = static_cast<int>((nc + TILE - 1) / TILE + (ld - nc + TILE - 1) / TILE);
                 maxnt = std::max(maxnt, nt);
+                maxnc = std::max(maxnc, nc);
                 if (nt > SOLVE_WIDE_TILES) nw = 8;
             }
+            if (static_cast<int>(lv.size()) <= WIDE_MAX_SN && maxnc >= WIDE_MIN_BLOCKS * TILE) {
+                // wide form: a task per row tile (forward, by tile) / block column (backward,
+                // by blocks from the last), in ticket order
+                std::vector<lkd_task> lt, lb;
+                std::vector<lkd_task> heads;
+                for (const Index s : lv) {
+                    lkd_task t{};
+                    t.off = S.sn_panel[s];
+                    t.ld = static_cast<int32_t>(S.sn_rowptr[s + 1] - S.sn_rowptr[s]);
+                    t.ncol = S.sn_first[s + 1] - S.sn_first[s];
+                    t.sn = s;
+                    t.first = S.sn_first[s];
+                    t.aux = diag_off(s, 0);
+                    t.aux2 = S.sn_rowptr[s];
+                    t.nsplit = 0;
+                    t.c0 = 1;
+                    t.px = nflags;
+                    nflags += 2 * ((t.ncol + TILE - 1) / TILE);
+                    heads.push_back(t);
+                }
+                for (int k = 0; k < maxnt; ++k)
+                    for (const auto& h : heads) {
+                        const int nb = (h.ncol + TILE - 1) / TILE;
+                        const int nt = nb + static_cast<int>((h.ld - h.ncol + TILE - 1) / TILE);
+                        lkd_task u = h;
+                        if (k < nt) {
+                            u.r0 = k;
+                            lt.push_back(u);
+                        }
+                        if (k < nb) {
+                            u.r0 = nb - 1 - k;
+                            lb.push_back(u);
+                        }
+                    }
+                per_level.push_back(std::move(lt));
+                per_level_bwd.push_back(std::move(lb));
+                wide.push_back(1);
+                continue;
+            }
+            wide.push_back(0);
             // forward: SOLVE_MAX_G workgroups per wide supernode when the level's workgroups all
             // fit on the compute units (one each); kernels exist for G = 1 and 4
             int G = 1;
@@ -1031,11 +1092,18 @@ struct GPUKKTCholeskySolver::Impl {
             per_level_bwd.push_back(std::move(lb));
         }
         auto emit = [&](int kind, const std::vector<lkd_task>& lt) {
+            if (lt.empty()) return;
             slaunch.push_back({kind, static_cast<int64_t>(tasks.size()), static_cast<int32_t>(lt.size()),
                                lt.empty() ? 4 : lt[0].nsplit, lt.empty() ? 1 : lt[0].c0});
             const int G = slaunch.back().g;
             squeue.resize(tasks.size(), -1);
-            if (G > 1) {   // XCD queues: supernodes dealt in turn, each one's G tasks together
+            if (kind >= 22) {   // wide: one ticket counter (queue 0), tasks in list order
+                Launch& L = slaunch.back();
+                L.q = n_sq++;
+                for (int x = 0; x <= n_xcd; ++x) sqoff.push_back(x == 0 ? 0 : static_cast<int32_t>(lt.size()));
+                for (size_t i = 0; i < lt.size(); ++i) squeue.push_back(static_cast<int32_t>(i));
+                L.grid = static_cast<int32_t>(lt.size());
+            } else if (G > 1) {   // XCD queues: supernodes dealt in turn, each one's G tasks together
                 Launch& L = slaunch.back();
                 L.q = n_sq++;
                 const int nsn = static_cast<int>(lt.size()) / G;
@@ -1053,13 +1121,31 @@ struct GPUKKTCholeskySolver::Impl {
             tasks.insert(tasks.end(), lt.begin(), lt.end());
             squeue.resize(tasks.size(), -1);
         };
-        for (const auto& lt : per_level) emit(20, lt);                                       // forward
-        for (auto it = per_level_bwd.rbegin(); it != per_level_bwd.rend(); ++it) emit(21, *it);   // backward
+        for (size_t l = 0; l < per_level.size(); ++l) emit(wide[l] ? 22 : 20, per_level[l]);   // forward
+        for (size_t l = per_level_bwd.size(); l-- > 0;) emit(wide[l] ? 23 : 21, per_level_bwd[l]);   // backward
         solve_pbuf = npbuf;
         solve_flags = nflags;
         solve_below = nbelow;
         for (const auto& L : slaunch) {
             for (int64_t q = L.begin; q < L.begin + L.count; ++q) validate(tasks[q], L.kind);
+            if (L.kind >= 22) {   // wide: every task once, each task's producers earlier
+                std::map<std::pair<int32_t, int32_t>, int32_t> pos;
+                for (int32_t i = 0; i < L.count; ++i) {
+                    const lkd_task& t = tasks[L.begin + i];
+                    if (squeue[static_cast<size_t>(L.begin + i)] != i || !pos.emplace(std::make_pair(t.sn, t.r0), i).second)
+                        throw std::logic_error("KKT plan: invalid wide solve queue");
+                }
+                for (int32_t i = 0; i < L.count; ++i) {
+                    const lkd_task& t = tasks[L.begin + i];
+                    const int nb = (t.ncol + TILE - 1) / TILE;
+                    const int lo = L.kind == 22 ? 0 : t.r0 + 1, hi = L.kind == 22 ? std::min(t.r0, nb) : nb;
+                    for (int j = lo; j < hi; ++j) {
+                        auto it = pos.find(std::make_pair(t.sn, static_cast<int32_t>(j)));
+                        if (it == pos.end() || it->second >= i) throw std::logic_error("KKT plan: wide solve order");
+                    }
+                }
+                continue;
+            }
             if (L.g == 1) continue;
             // queues: every task once; a supernode's tasks consecutive, in one queue, g in order
             const int32_t* off = sqoff.data() + static_cast<size_t>(L.q) * (n_xcd + 1);
@@ -1394,6 +1480,12 @@ struct GPUKKTCholeskySolver::Impl {
         uint32_t* epoch = d_flags + solve_flags;
         check(lkd_trsv_epoch(epoch, epoch + 1, n_sq * n_xcd, stream), "solve epoch");
         for (const auto& L : slaunch) {
+            if (L.kind >= 22) {
+                check(lkd_trsv_wide(prec, L.kind == 22 ? 0 : 1, L.count, d_stasks + L.begin, d_squeue + L.begin,
+                                    epoch + 1 + static_cast<int64_t>(L.q) * n_xcd, d_panel, d_diag, d_sn_rows, x,
+                                    d_flags, epoch, d_serr, stream), "solve kernel");
+                continue;
+            }
             const int dir = L.kind == 20 ? 0 : 1;
             const bool q = L.g > 1;
             check(lkd_trsv_level(prec, dir, L.nw, L.g, L.count, d_stasks + L.begin, d_panel, d_diag, d_sn_rows,

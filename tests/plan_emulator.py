@@ -201,25 +201,42 @@ class Emulator:
         return np.tril(Lr, -1) + np.diag(1.0 / np.diag(Lr))
 
     def solve(self, x):
-        """In-place L L^T solve of the permuted vector x with the level schedule (kinds 20 / 21):
+        """In-place L L^T solve of the permuted vector x with the level schedule (kinds 20 / 21:
         G workgroups per supernode of a level (task g = r0 of c0), all with the same wavefront
-        count; per supernode the blocks are processed in order, rows below go to the ancestors'
-        entries (forward) or are read from them (backward).  Checks the plan's workgroup sets
+        count; kinds 22 / 23, the wide form: one task per row tile / block column); per
+        supernode the blocks are processed in order, rows below go to the ancestors' entries
+        (forward) or are read from them (backward).  Checks the plan's workgroup sets
         and its scratch, exchange and flag ranges (disjoint per supernode) on the way."""
         p = self.p
         T = p["stasks"]
         seen_flags = []
         for kind, b, c in self.sl:
-            assert kind in (20, 21)
+            assert kind in (20, 21, 22, 23)
             tl = T[b:b + c]
-            assert len({int(u["nsplit"]) for u in tl}) == 1 and int(tl[0]["nsplit"]) in (4, 8)
-            for t in tl:                            # G workgroups per supernode: g = 0 .. G-1 once each
-                G = int(t["c0"])
-                assert 1 <= G and 0 <= int(t["r0"]) < G
-                gs = sorted(int(u["r0"]) for u in tl if u["sn"] == t["sn"])
-                assert gs == list(range(G))
-            heads = [t for t in tl if int(t["r0"]) == 0]
-            for t in heads:
+            if kind >= 22:                          # wide: one task per tile / block column
+                heads, seen = [], set()
+                for t in tl:
+                    nbk = (int(t["ncol"]) + TS - 1) // TS
+                    ntk = nbk + (int(t["ld"]) - int(t["ncol"]) + TS - 1) // TS
+                    key = (int(t["sn"]), int(t["r0"]))
+                    assert key not in seen and 0 <= key[1] < (ntk if kind == 22 else nbk)
+                    seen.add(key)
+                    if int(t["sn"]) not in {int(h["sn"]) for h in heads}:
+                        heads.append(t)
+                for t in heads:
+                    nbk = (int(t["ncol"]) + TS - 1) // TS
+                    ntk = nbk + (int(t["ld"]) - int(t["ncol"]) + TS - 1) // TS
+                    assert {k for s_, k in seen if s_ == int(t["sn"])} == set(range(ntk if kind == 22 else nbk))
+                    seen_flags.append((int(t["px"]), int(t["px"]) + 2 * nbk))
+            else:
+                assert len({int(u["nsplit"]) for u in tl}) == 1 and int(tl[0]["nsplit"]) in (4, 8)
+                for t in tl:                        # G workgroups per supernode: g = 0 .. G-1 once each
+                    G = int(t["c0"])
+                    assert 1 <= G and 0 <= int(t["r0"]) < G
+                    gs = sorted(int(u["r0"]) for u in tl if u["sn"] == t["sn"])
+                    assert gs == list(range(G))
+                heads = [t for t in tl if int(t["r0"]) == 0]
+            for t in (heads if kind < 22 else []):
                 nbk = (int(t["ncol"]) + TS - 1) // TS
                 G, nw = int(t["c0"]), int(t["nsplit"])
                 for u in heads:
@@ -242,7 +259,7 @@ class Emulator:
                     kb = min(TS, nc - j * TS)
                     Ld[j * TS:j * TS + kb, j * TS:j * TS + kb] = self._diag_L(int(t["aux"]) + j * TS * TS, kb)
                     Ld[j * TS + kb:, j * TS:j * TS + kb] = Lfull[j * TS + kb:nc, j * TS:j * TS + kb]
-                if kind == 20:
+                if kind in (20, 22):
                     z = np.linalg.solve(Ld, x[first:first + nc])
                     x[first:first + nc] = z
                     if ld > nc:

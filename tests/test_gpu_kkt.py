@@ -130,8 +130,9 @@ def test_direct_solve_multi_supernode(gpu, oracle, idx, mode):
 
 @pytest.mark.parametrize("mode", [1, 2])
 def test_direct_solve_wide_supernodes(gpu, mode):
-    """Wide supernodes (about 26 row tiles each): the forward level solve spreads every
-    supernode over four workgroups of one XCD.  Residual of the direct solve with sparse
+    """Four wide supernodes (about 26 row tiles each) and the linking rows below them: the level
+    solves take the wide form (a workgroup per row tile / block column, ancestor rows added
+    with atomics forward and read backward).  Residual of the direct solve with sparse
     products, M0 x = A (theta o (A^T x))."""
     import scipy.sparse as sp
     from lakshya import gen
@@ -148,6 +149,31 @@ def test_direct_solve_wide_supernodes(gpu, mode):
         x, info = kk.solve(rhs, tol=1e-300, max_refine=0)
         rel = np.abs(A @ (theta * (A.T @ x)) - rhs).max() / np.abs(rhs).max()
         assert rel <= (1e-7 if mode == 1 else 1e-4), rel
+    kk.close(); arena.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_direct_solve_dense_separator(gpu, oracle, mode):
+    """One dense supernode of 1,200 columns (19 blocks, no rows below): the level solves take
+    the wide form, a workgroup per row tile (forward) or block column (backward) on every XCD,
+    solved blocks handed over through device-coherent flags.  Residual against the dense M0,
+    repeated solves (new epochs)."""
+    from lakshya import gen
+    lp = gen.dense_small(1200, 2400, 0.02, seed=8)
+    rp, ci, va = lp.csr()
+    m, n = lp.m, lp.n
+    plan = gpu.plan_export(rp, ci, va, n)
+    assert {22, 23} <= set(plan["slaunch"].reshape(-1, 3)[:, 0].tolist())
+    theta = np.exp(np.random.default_rng(4).uniform(-2, 2, n))
+    arena = gpu.Arena(512 << 20)
+    kk = gpu.KKT(arena, rp, ci, va, n, reference_symbolic=False)
+    assert kk.factorize(theta, mode=mode)
+    M0 = oracle.assemble_dense(rp, ci, va, theta, np.arange(m, dtype=np.int32))
+    for seed in range(3):
+        rhs = np.random.default_rng(10 + seed).standard_normal(m)
+        x, info = kk.solve(rhs, tol=1e-300, max_refine=0)
+        rel = np.abs(M0 @ x - rhs).max() / np.abs(rhs).max()
+        assert rel <= (1e-7 if mode == 1 else 1e-4), (seed, rel)
     kk.close(); arena.close()
 
 

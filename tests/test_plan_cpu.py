@@ -19,12 +19,15 @@ def _cases():
         "dense": lambda: gen.dense_small(420, 900, 0.02, seed=5),
         "block_angular_wide": lambda: gen.block_angular(3, 150, 400, 6, 8, 0.2, seed=3),
         "dense_wide": lambda: gen.dense_small(700, 1400, 0.02, seed=6),
+        "dense_separator": lambda: gen.dense_small(1100, 2200, 0.02, seed=8),
+        "block_angular_2x1100": lambda: gen.block_angular(2, 1100, 2600, 6, 8, 0.05, seed=4),
     }
 
 
 @pytest.mark.parametrize("name,steps", [(c, 1) for c in ("random_sparse", "block_angular", "transport",
                                                          "dense", "block_angular_wide", "dense_wide")] +
-                         [("dense_wide", 0), ("dense_wide", 2), ("block_angular_wide", 2)])
+                         [("dense_wide", 0), ("dense_wide", 2), ("block_angular_wide", 2),
+                          ("dense_separator", 1), ("block_angular_2x1100", 1)])
 def test_schedule_reproduces_dense_cholesky(native, name, steps):
     lp = _cases()[name]()
     rp, ci, va = lp.csr()
@@ -201,3 +204,26 @@ def test_steps_launch_only_where_the_xcds_share_the_work(native):
     assert ncol.max() >= 0.9 * lp.m                      # one dense supernode
     kinds = [k for g in _level_kinds(plan) for k, _ in g]
     assert 7 not in kinds and kinds.count(5) >= ncol.max() // 64 - 1
+
+
+@pytest.mark.parametrize("name", ["dense_separator", "block_angular_2x1100"])
+def test_wide_level_solves(native, name):
+    """Levels of at most 8 supernodes, one of at least 16 blocks, take the wide solve form
+    (kinds 22 / 23): a task per row tile / block column in an order where every task's
+    producers come first (the plan checks it), flags of 2 nb per supernode."""
+    lp = _cases()[name]()
+    rp, ci, va = lp.csr()
+    plan = native.plan_export(rp, ci, va, lp.n, P=native.order_amd(rp, ci, lp.n))
+    sl = plan["slaunch"].reshape(-1, 3)
+    kinds = [int(k) for k in sl[:, 0]]
+    assert 22 in kinds and 23 in kinds
+    T = plan["stasks"]
+    for kind, b, c in sl:
+        if kind < 22:
+            continue
+        tl = T[b:b + c]
+        pos = {(int(t["sn"]), int(t["r0"])): i for i, t in enumerate(tl)}
+        for i, t in enumerate(tl):
+            nb = (int(t["ncol"]) + 63) // 64
+            deps = range(min(int(t["r0"]), nb)) if kind == 22 else range(int(t["r0"]) + 1, nb)
+            assert all(pos[(int(t["sn"]), j)] < i for j in deps)
