@@ -156,11 +156,15 @@ int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int6
  * Left-looking per 64-column block column, per etree level:
  *   DIAG    factor block column 0's diagonal block (ws = staging slot, aux = diag copy)
  *   PART    split K of a STEP launch with few tiles: one K range [kbeg, kend) of a tile's
- *           GEMM, written to work (LKD_PART elements per product, the tile's thread layout)
+ *           GEMM, written to work + px (LKD_PART elements per product, the tile's thread
+ *           layout); in the same launch, tasks with nsplit = 2 factor the step's diagonal
+ *           block k0 (as DIAG) for the one-launch-per-step form
+ *   PSUM    (after PART) the np partial products of a tile summed in order into the first,
+ *           task kbeg = slice of 512 elements (LKD_PART / 512 tasks per tile)
  *   STEP k  128-row tiles below the diagonal: GEMM update (own range kbeg..kend plus np
- *           partial products) + solve against the staged factor; the tile holding the next
- *           diagonal block (nsplit = 1, c0 rows) also factors it (staging slot aux2, diag
- *           copy aux)
+ *           partial products) + solve against the staged factor; in a STEPS launch the tile
+ *           holding the next diagonal block (nsplit = 1, c0 rows) also factors it (staging
+ *           slot aux2, diag copy aux)
  *   STEPS   (in place of the STEP / PART launches) every STEP tile of the level in one
  *           launch: a tile runs once the tiles it depends on are done (see lkd_chol_step_dag)
  *   UPDATE  ancestors (split K over kbeg..kend)
@@ -175,8 +179,9 @@ int lkd_assemble_pairs(int prec, int64_t nseg, const int64_t* segptr, const int6
 #define LKD_PART (128 * 64)
 int lkd_chol_diag(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* diag,
                   void* piv, void* stage, int32_t* err, void* stream);
-int lkd_chol_part(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* work,
-                  int32_t* err, void* stream);
+int lkd_chol_part(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* diag,
+                  void* piv, void* stage, void* work, int32_t* err, void* stream);
+int lkd_chol_psum(int prec, int32_t ntask, const lkd_task* tasks, void* work, void* stream);
 int lkd_chol_step(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
                   void* piv, void* stage, const void* work, int32_t* err, void* stream);
 /* STEPS: the STEP tiles of one level (np = 0).  deps: 4 task indices per task (-1 = none):

@@ -78,21 +78,14 @@ __device__ __forceinline__ void publish_diag(const T* Ds, const T* Lv, int kb, T
     write_diag_copy<T>(Ds, kb, dcopy);
 }
 
-// DIAG: factor the first diagonal block (columns k0 .. k0+kb, nothing to subtract yet) of each
-// task's supernode; staging slot at stage + ws, row-major copy at diag + aux.
+// Factor the diagonal block of columns k0 .. k0+kb of the task's supernode as it stands in the
+// panel (block 0: nothing to subtract; a later block: every earlier block column's update was
+// applied by the STEP tiles); staging slot at stage + ws, row-major copy at diag + aux.
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag(const lkd_task* __restrict__ tasks, int ntask,
-                                              const T* __restrict__ L, T* __restrict__ diag,
-                                              T* __restrict__ piv, T* __restrict__ stage,
-                                              int32_t* err) {
-    const int ti = task_index();
-    if (ti >= ntask) return;
-    __shared__ int s_err;
-    if (wg_error(err, &s_err)) return;
-    const lkd_task tk = tasks[ti];
-    __shared__ T Ds[TS * DP];
-    __shared__ T Lv[4 * 256];
-    __shared__ int s_ok;
+__device__ __forceinline__ void diag_task(const lkd_task& tk, const T* __restrict__ L,
+                                          T* __restrict__ diag, T* __restrict__ piv,
+                                          T* __restrict__ stage, int32_t* err, T* Ds, T* Lv,
+                                          int* s_ok) {
     const T* P = L + tk.off;
     const int64_t ld = tk.ld;
     const int k0 = tk.k0, kb = tk.kb;
@@ -103,11 +96,27 @@ __global__ __launch_bounds__(256) void k_diag(const lkd_task* __restrict__ tasks
         Ds[c * DP + r] = v;
     }
     __syncthreads();
-    if (!potrf64<T>(Ds, Lv, &s_ok)) {
+    if (!potrf64<T>(Ds, Lv, s_ok)) {
         if (threadIdx.x == 0) atomicOr(err, 1);
         return;
     }
     publish_diag<T>(Ds, Lv, kb, stage + tk.ws, piv + tk.first + k0, diag + tk.aux);
+}
+
+// DIAG: the first diagonal block of each task's supernode (diag_task).
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag(const lkd_task* __restrict__ tasks, int ntask,
+                                              const T* __restrict__ L, T* __restrict__ diag,
+                                              T* __restrict__ piv, T* __restrict__ stage,
+                                              int32_t* err) {
+    const int ti = task_index();
+    if (ti >= ntask) return;
+    __shared__ int s_err;
+    if (wg_error(err, &s_err)) return;
+    __shared__ T Ds[TS * DP];
+    __shared__ T Lv[4 * 256];
+    __shared__ int s_ok;
+    diag_task<T>(tasks[ti], L, diag, piv, stage, err, Ds, Lv, &s_ok);
 }
 
 // STEP k (one task = tile R = rows r0 .. r0+c0-1, all below the diagonal block of block column k):
@@ -140,16 +149,45 @@ __device__ __forceinline__ void part_tile(const lkd_task& tk, const T* __restric
             for (int e = 0; e < 4; ++e) out[((p * 2 + b) * 4 + e) * 256] = acc[p][b][e];
 }
 
+// PART launch of the one-launch-per-step form: PART tiles, and (nsplit = 2, listed first) the
+// factorization of the step's diagonal block, which runs beside the partial products (the
+// STEP launch after this one is the first to read it).
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_part(const lkd_task* __restrict__ tasks, int ntask, const T* __restrict__ L,
+            T* __restrict__ diag, T* __restrict__ piv, T* __restrict__ stage,
             T* __restrict__ work, int32_t* err) {
     const int ti = task_index();
     if (ti >= ntask) return;
     __shared__ int s_err;
     if (wg_error(err, &s_err)) return;
     __shared__ GemmLds<T> sm;
-    part_tile<T>(tasks[ti], L, work, sm);
+    __shared__ int s_ok;
+    const lkd_task tk = tasks[ti];
+    if (tk.nsplit == 2) {
+        T* Ds = reinterpret_cast<T*>(&sm);
+        diag_task<T>(tk, L, diag, piv, stage, err, Ds, Ds + TS * DP, &s_ok);
+        return;
+    }
+    part_tile<T>(tk, L, work, sm);
+}
+
+// PSUM: the np partial products of a STEP tile (work + px, LKD_PART elements each) summed in
+// order into the first one; task kbeg = slice of 512 elements (two per thread), so a tile with
+// many K ranges is summed by LKD_PART / 512 workgroups, each with many loads in flight.
+template <typename T>
+__global__ __launch_bounds__(256) void k_psum(const lkd_task* __restrict__ tasks, int ntask,
+                                              T* __restrict__ work) {
+    const int ti = task_index();
+    if (ti >= ntask) return;
+    const lkd_task tk = tasks[ti];
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    T2* p = reinterpret_cast<T2*>(work + tk.px + (int64_t)tk.kbeg * 512) + threadIdx.x;
+    constexpr int64_t stride = LKD_PART / 2;
+    T2 acc = p[0];
+#pragma unroll 16
+    for (int k = 1; k < tk.np; ++k) acc += p[(int64_t)k * stride];
+    p[0] = acc;
 }
 
 // Wait (one thread) until flag == ep; false on an error flag or after kSpinLimit polls (then
@@ -552,7 +590,8 @@ int launch_chol(int kind, int32_t ntask, const lkd_task* tasks, const CholArgs& 
         case 1: LK_LAUNCH(k_diag<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (T*)a.stage, a.err); break;
         case 2: LK_LAUNCH(k_update<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, *a.plan, a.err); break;
         case 3: LK_LAUNCH(k_diagwb<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (const T*)a.diag, (const T*)a.piv); break;
-        case 6: LK_LAUNCH(k_part<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.work, a.err); break;
+        case 8: LK_LAUNCH(k_psum<T>, grid8(ntask), 256, 0, stream, tasks, ntask, (T*)a.work); break;
+        case 6: LK_LAUNCH(k_part<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (T*)a.stage, (T*)a.work, a.err); break;
         default: LK_LAUNCH(k_step<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (T*)a.stage, (const T*)a.work, a.err); break;
     }
     return lk_last_error();
@@ -638,10 +677,13 @@ int lkd_chol_diag(int prec, int32_t ntask, const lkd_task* tasks, const void* pa
                   void* piv, void* stage, int32_t* err, void* stream) {
     return dispatch(prec, 1, ntask, tasks, {const_cast<void*>(panel), diag, piv, stage, nullptr, err, nullptr}, stream);
 }
-int lkd_chol_part(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* work,
-                  int32_t* err, void* stream) {
+int lkd_chol_part(int prec, int32_t ntask, const lkd_task* tasks, const void* panel, void* diag,
+                  void* piv, void* stage, void* work, int32_t* err, void* stream) {
     return dispatch(prec, 6, ntask, tasks,
-                    {const_cast<void*>(panel), nullptr, nullptr, nullptr, nullptr, err, work}, stream);
+                    {const_cast<void*>(panel), diag, piv, stage, nullptr, err, work}, stream);
+}
+int lkd_chol_psum(int prec, int32_t ntask, const lkd_task* tasks, void* work, void* stream) {
+    return dispatch(prec, 8, ntask, tasks, {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, work}, stream);
 }
 int lkd_chol_step(int prec, int32_t ntask, const lkd_task* tasks, void* panel, void* diag,
                   void* piv, void* stage, const void* work, int32_t* err, void* stream) {

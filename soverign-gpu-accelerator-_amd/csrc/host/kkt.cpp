@@ -40,6 +40,7 @@ constexpr int GEMM_TARGET_WG = 256;   // UPDATE K splitting aims at this many wo
 constexpr int KSPLIT_MIN = 128;     // minimum K columns per split part
 constexpr int STEP_TARGET_WG = 512; // a STEP launch with fewer tiles splits K (PART launch) ...
 constexpr int STEP_KMIN = 64;       // ... into parts of at least this many columns
+constexpr int PSUM_MIN = 3;         // tiles with at least this many parts get them summed by PSUM
 constexpr int STEPS_RECENT = 192;   // STEPS: K columns a chain tile computes itself
 constexpr int STEPS_PART_LAG = 1;   // STEPS: steps between a PART's inputs and its queue place
 constexpr int STEPS_SPLIT_CLS = 2;  // STEPS: tiles of class <= this (0 next-diagonal, 1 below it, 2 others) split
@@ -567,21 +568,31 @@ struct GPUKKTCholeskySolver::Impl {
             // STEP k: 128-row tiles below the diagonal block of block column k
             for (int k = 0; k < maxblk; ++k) {
                 std::vector<lkd_task> tl;
+                std::vector<lkd_task> dg;   // one launch per step: block k's factorization
                 for (size_t si = 0; si < lv.size(); ++si) {
                     lkd_task t = base(lv[si]);
                     if (k * TILE >= t.ncol) continue;
                     t.k0 = k * TILE;
                     t.kb = std::min(TILE, t.ncol - t.k0);
                     t.ws = slot(si, k);
+                    if (!dag && k > 0) {   // factored in the PART launch of step k (beside its
+                        lkd_task d = t;    // partial products), not at the end of STEP k - 1
+                        d.r0 = t.k0;
+                        d.aux = diag_off(lv[si], k);
+                        d.nsplit = 2;
+                        dg.push_back(d);
+                    }
                     const int k1 = t.k0 + t.kb;
                     int r0 = k1;
                     if (k1 < t.ncol) {   // the next diagonal block: its own tile, first in the list
                         lkd_task u = t;
                         u.r0 = k1;
                         u.c0 = std::min(TILE, t.ncol - k1);
-                        u.nsplit = 1;
-                        u.aux = diag_off(lv[si], k + 1);
-                        u.aux2 = slot(si, k + 1);
+                        if (dag) {       // STEPS: the tile factors the block once its rows are done
+                            u.nsplit = 1;
+                            u.aux = diag_off(lv[si], k + 1);
+                            u.aux2 = slot(si, k + 1);
+                        }
                         tl.push_back(u);
                         r0 += u.c0;
                     }
@@ -592,7 +603,7 @@ struct GPUKKTCholeskySolver::Impl {
                         tl.push_back(u);
                     }
                 }
-                if (tl.empty()) continue;
+                if (tl.empty() && dg.empty()) continue;   // (the last block: only its factorization)
                 if (dag) {   // whole K; in the queue, the step's next-diagonal tiles first, then
                              // the tiles right below them (the next one waits for those), then
                              // the others
@@ -652,11 +663,30 @@ struct GPUKKTCholeskySolver::Impl {
                     scrit.push_back(u.nsplit ? 1 : 0);
                 }
                 work_elems = std::max(work_elems, px);
-                if (!parts.empty()) {
+                if (!parts.empty() || !dg.empty()) {
                     order_for_dispatch(parts, pcrit);
+                    parts.insert(parts.begin(), dg.begin(), dg.end());   // first workgroups
                     b0 = static_cast<int64_t>(tasks.size());
                     tasks.insert(tasks.end(), parts.begin(), parts.end());
                     emit(6, b0);
+                }
+                // tiles with many parts: PSUM adds them (slices on many workgroups), the STEP
+                // then reads the one sum
+                std::vector<lkd_task> sums;
+                for (auto& u : tl) {
+                    if (u.np < PSUM_MIN) continue;
+                    for (int sl = 0; sl < LKD_PART / 512; ++sl) {
+                        lkd_task q = u;
+                        q.kbeg = sl;
+                        q.kend = 0;
+                        sums.push_back(q);
+                    }
+                    u.np = 1;
+                }
+                if (!sums.empty()) {
+                    b0 = static_cast<int64_t>(tasks.size());
+                    tasks.insert(tasks.end(), sums.begin(), sums.end());
+                    emit(8, b0);
                 }
                 order_for_dispatch(tl, scrit);
                 b0 = static_cast<int64_t>(tasks.size());
@@ -917,7 +947,7 @@ struct GPUKKTCholeskySolver::Impl {
         const int64_t ld = S.sn_rowptr[s + 1] - S.sn_rowptr[s];
         const Index nc = S.sn_first[s + 1] - S.sn_first[s];
         if (t.ld != ld || t.ncol != nc || t.off != S.sn_panel[s] || t.first != S.sn_first[s] ||
-            (kind != 5 && kind != 6 && t.aux2 != S.sn_rowptr[s]))   // STEP: aux2 = next staging slot
+            (kind != 5 && kind != 6 && kind != 8 && t.aux2 != S.sn_rowptr[s]))   // STEP: aux2 = next staging slot
             bad("geometry");
         if (S.sn_panel[s] + ld * nc > S.sn_panel[S.nsuper]) bad("panel extent");
         if (t.first + nc > m) bad("columns");
@@ -931,7 +961,18 @@ struct GPUKKTCholeskySolver::Impl {
             case 3:   // DIAGWB: a diagonal block and its copy
                 if (t.kb < 1 || t.r0 != t.k0 || !diag_ok(t.aux, t.kb)) bad("diagwb");
                 break;
-            case 6:   // PART: one K range of a STEP tile's GEMM into the work buffer
+            case 8:   // PSUM: a slice of a STEP tile's partial products
+                if (t.np < PSUM_MIN || t.px < 0 || t.px + int64_t(t.np) * LKD_PART > work_elems ||
+                    t.kbeg < 0 || t.kbeg >= LKD_PART / 512)
+                    bad("psum");
+                break;
+            case 6:   // PART launch: the step's diagonal block (nsplit = 2), like DIAG
+                if (t.nsplit == 2) {
+                    if (t.k0 % TILE || t.kb < 1 || t.r0 != t.k0 || !diag_ok(t.aux, t.kb) || !slot_ok(t.ws))
+                        bad("part-launch diag");
+                    break;
+                }
+                // PART: one K range of a STEP tile's GEMM into the work buffer
                 if (t.kb < 1 || t.r0 < t.k0 + t.kb || t.c0 < 1 || t.c0 > GEMM_MT || t.r0 + t.c0 > ld) bad("part rows");
                 if (t.kbeg < 0 || t.kbeg >= t.kend || t.kend > t.k0) bad("part K");
                 if (t.px < 0 || t.px + LKD_PART > work_elems) bad("part work");
@@ -1319,7 +1360,7 @@ struct GPUKKTCholeskySolver::Impl {
 
     // 0 STEP, 1 DIAG + DIAGWB, 2 UPDATE, 5 PART (3 assembly and 4 totals are not launch kinds)
     static int slot_of(const Launch& L) {
-        return L.kind == 5 || L.kind == 7 ? 0 : L.kind == 6 ? 5 : L.kind == 2 ? 2 : 1;
+        return L.kind == 5 || L.kind == 7 ? 0 : L.kind == 6 || L.kind == 8 ? 5 : L.kind == 2 ? 2 : 1;
     }
 
     // Live launch timing: HIP events on the solver's stream at every change of launch kind in
@@ -1383,7 +1424,8 @@ struct GPUKKTCholeskySolver::Impl {
             case 3:   // only the reference-layout export reads the panel's diagonal blocks (the
                       // solves use the row-major copies)
                 return d_csr ? lkd_chol_diagwb(prec, L.count, t, d_panel, d_diag, d_piv, stream) : 0;
-            case 6: return lkd_chol_part(prec, L.count, t, d_panel, d_work, d_err, stream);
+            case 6: return lkd_chol_part(prec, L.count, t, d_panel, d_diag, d_piv, d_stage, d_work, d_err, stream);
+            case 8: return lkd_chol_psum(prec, L.count, t, d_work, stream);
             case 7: return lkd_chol_step_dag(prec, L.count, t, d_fdeps + 4 * L.begin, d_fsdep + L.begin,
                                              d_fqueue + L.begin, d_fqoff + static_cast<int64_t>(L.q) * (n_xcd + 1),
                                              n_xcd, L.grid, d_dag_flags + L.begin, d_dag_flags2 + L.begin,
@@ -1449,7 +1491,8 @@ struct GPUKKTCholeskySolver::Impl {
     static double task_flops(const lkd_task& t, int kind) {
         const double kb = t.kb;
         switch (kind) {
-            case 6: return 2.0 * t.c0 * kb * (t.kend - t.kbeg);
+            case 6: return t.nsplit == 2 ? kb * kb * kb / 3.0 : 2.0 * t.c0 * kb * (t.kend - t.kbeg);
+            case 8: return 0.0;
             case 7:
                 if (t.part) return 2.0 * t.c0 * kb * (t.kend - t.kbeg);
                 [[fallthrough]];

@@ -81,7 +81,9 @@ class Emulator:
         return out
 
     def factor(self, rng=None):
-        """Left-looking schedule: DIAG (kind 1, block column 0), PART (6), STEP (5), STEPS (7:
+        """Left-looking schedule: DIAG (kind 1, block column 0), PART (6; with nsplit = 2 the
+        step's diagonal block, as DIAG), PSUM (8: a tile's partial products summed in order
+        into the first), STEP (5), STEPS (7:
         the STEP tiles of a level, run in list order or, with rng, in a random order that
         respects their dependencies), UPDATE (2), DIAGWB (3).
         Diagonal factors live in staging slots between launches; the panel's diagonal blocks
@@ -91,8 +93,9 @@ class Emulator:
         MT = 128
         self.stage = {}
         self.work = {}
+        self.psum_slices = {}
         for kind, b, c in self.fl:
-            assert kind in (1, 2, 3, 5, 6, 7)
+            assert kind in (1, 2, 3, 5, 6, 7, 8)
             order = self.dag_order(b, c, rng) if kind == 7 else range(b, b + c)
             steps = kind == 7
             for q in order:
@@ -103,12 +106,26 @@ class Emulator:
                     return False
                 k0, kb, r0, c0, ld, nc = (int(t[x]) for x in ("k0", "kb", "r0", "c0", "ld", "ncol"))
                 first = int(t["first"])
-                if kind == 1:
+                if kind == 1 or (kind == 6 and int(t["nsplit"]) == 2):
+                    assert kind == 6 or k0 == 0
                     D = self.blk(t, k0, k0 + kb, k0, k0 + kb).copy()
                     L = self._factor_block(np.tril(D) + np.tril(D, -1).T)
                     if L is None:
                         return False
                     self._publish(L, int(t["ws"]), first + k0, int(t["aux"]))
+                elif kind == 8:                   # PSUM: slice kbeg of the tile's np products
+                    px, npt, sl = int(t["px"]), int(t["np"]), int(t["kbeg"])
+                    assert 0 <= sl < 16 and npt >= 2
+                    self.psum_slices.setdefault(px, set()).add(sl)
+                    if len(self.psum_slices[px]) == 16:   # all slices: the products become one
+                        got = [self.work.pop(px + s_ * 128 * 64) for s_ in range(npt)]
+                        got.sort()
+                        assert all(got[i][1] == got[i + 1][0] for i in range(npt - 1))
+                        tot = got[0][2]
+                        for g_ in got[1:]:
+                            tot = tot + g_[2]
+                        self.work[px] = (got[0][0], got[-1][1], tot)
+                        del self.psum_slices[px]
                 elif kind == 6:                   # PART: one K range of a STEP tile's GEMM
                     ra, kbeg, kend = int(t["c0"]), int(t["kbeg"]), int(t["kend"])
                     assert 0 <= kbeg < kend <= k0
@@ -172,6 +189,7 @@ class Emulator:
                                 return False
                             self.panel[p["sn_panel"][s] + (gj - f) * (rp1 - rp0) + pos] -= U[i, j]
         assert not self.work, "partial products never added"
+        assert not self.psum_slices, "partial sums with missing slices"
         return True
 
     def _publish(self, L, slot, col, dcopy):

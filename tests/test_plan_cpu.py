@@ -37,7 +37,7 @@ def test_schedule_reproduces_dense_cholesky(native, name, steps):
     P = native.order_amd(rp, ci, n)
     plan = native.plan_export(rp, ci, va, n, P=P, step_dag=steps)
     kinds = set(plan["flaunch"].reshape(-1, 3)[:, 0].tolist())
-    assert kinds <= {1, 2, 3, 5, 6, 7}    # STEPS launches, or PART / STEP launches per step
+    assert kinds <= {1, 2, 3, 5, 6, 7, 8}  # STEPS launches, or PART / PSUM / STEP launches per step
     em = Emulator(plan)
     D = em.assemble(theta, equil=True, delta=1e-9)
     assert em.factor(), f"emulated factorization failed (err {em.err})"
