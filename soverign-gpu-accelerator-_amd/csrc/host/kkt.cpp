@@ -1039,7 +1039,7 @@ struct GPUKKTCholeskySolver::Impl {
         int cus = 0;
         if (lkd_device_cus(&cus) != 0 || cus < 1) cus = 0;   // no device: one workgroup each
         int64_t nbelow = 0, npbuf = 0, nflags = 0;
-        std::vector<char> wide;
+        std::vector<char> wide_f, wide_b;
         for (const auto& lv : levels) {
             int nw = 4, maxnt = 1;
             Index maxnc = 0;
@@ -1051,10 +1051,14 @@ struct GPUKKTCholeskySolver::Impl {
                 maxnc = std::max(maxnc, nc);
                 if (nt > SOLVE_WIDE_TILES) nw = 8;
             }
-            if (static_cast<int>(lv.size()) <= WIDE_MAX_SN && maxnc >= WIDE_MIN_BLOCKS * TILE) {
-                // wide form: a task per row tile (forward, by tile) / block column (backward,
-                // by blocks from the last), in ticket order
-                std::vector<lkd_task> lt, lb;
+            // wide form: a task per row tile (forward, by tile) / block column (backward, by
+            // blocks from the last), in ticket order.  Backward for every level with a supernode
+            // of WIDE_MIN_BLOCKS blocks; forward only where the level has at most WIDE_MAX_SN
+            // supernodes (with more, the grouped forward below is faster)
+            const bool big = maxnc >= WIDE_MIN_BLOCKS * TILE;
+            const bool wf = big && static_cast<int>(lv.size()) <= WIDE_MAX_SN, wb = big;
+            std::vector<lkd_task> wlt, wlb;
+            if (wf || wb) {
                 std::vector<lkd_task> heads;
                 for (const Index s : lv) {
                     lkd_task t{};
@@ -1076,21 +1080,23 @@ struct GPUKKTCholeskySolver::Impl {
                         const int nb = (h.ncol + TILE - 1) / TILE;
                         const int nt = nb + static_cast<int>((h.ld - h.ncol + TILE - 1) / TILE);
                         lkd_task u = h;
-                        if (k < nt) {
+                        if (wf && k < nt) {
                             u.r0 = k;
-                            lt.push_back(u);
+                            wlt.push_back(u);
                         }
-                        if (k < nb) {
+                        if (wb && k < nb) {
                             u.r0 = nb - 1 - k;
-                            lb.push_back(u);
+                            wlb.push_back(u);
                         }
                     }
-                per_level.push_back(std::move(lt));
-                per_level_bwd.push_back(std::move(lb));
-                wide.push_back(1);
+            }
+            wide_f.push_back(wf);
+            wide_b.push_back(wb);
+            if (wf && wb) {
+                per_level.push_back(std::move(wlt));
+                per_level_bwd.push_back(std::move(wlb));
                 continue;
             }
-            wide.push_back(0);
             // forward: SOLVE_MAX_G workgroups per wide supernode when the level's workgroups all
             // fit on the compute units (one each); kernels exist for G = 1 and 4
             int G = 1;
@@ -1129,8 +1135,8 @@ struct GPUKKTCholeskySolver::Impl {
                     lb.push_back(u);
                 }
             }
-            per_level.push_back(std::move(lt));
-            per_level_bwd.push_back(std::move(lb));
+            per_level.push_back(wf ? std::move(wlt) : std::move(lt));
+            per_level_bwd.push_back(wb ? std::move(wlb) : std::move(lb));
         }
         auto emit = [&](int kind, const std::vector<lkd_task>& lt) {
             if (lt.empty()) return;
@@ -1162,8 +1168,8 @@ struct GPUKKTCholeskySolver::Impl {
             tasks.insert(tasks.end(), lt.begin(), lt.end());
             squeue.resize(tasks.size(), -1);
         };
-        for (size_t l = 0; l < per_level.size(); ++l) emit(wide[l] ? 22 : 20, per_level[l]);   // forward
-        for (size_t l = per_level_bwd.size(); l-- > 0;) emit(wide[l] ? 23 : 21, per_level_bwd[l]);   // backward
+        for (size_t l = 0; l < per_level.size(); ++l) emit(wide_f[l] ? 22 : 20, per_level[l]);   // forward
+        for (size_t l = per_level_bwd.size(); l-- > 0;) emit(wide_b[l] ? 23 : 21, per_level_bwd[l]);   // backward
         solve_pbuf = npbuf;
         solve_flags = nflags;
         solve_below = nbelow;

@@ -129,15 +129,19 @@ def test_direct_solve_multi_supernode(gpu, oracle, idx, mode):
 
 
 @pytest.mark.parametrize("mode", [1, 2])
-def test_direct_solve_wide_supernodes(gpu, mode):
-    """Four wide supernodes (about 26 row tiles each) and the linking rows below them: the level
-    solves take the wide form (a workgroup per row tile / block column, ancestor rows added
-    with atomics forward and read backward).  Residual of the direct solve with sparse
-    products, M0 x = A (theta o (A^T x))."""
+@pytest.mark.parametrize("blocks,mb,nb", [(4, 1500, 4000), (12, 1100, 2600)])
+def test_direct_solve_wide_supernodes(gpu, mode, blocks, mb, nb):
+    """Wide supernodes (17 to 26 row tiles each) and the linking rows below them.  Four of
+    them: both level solves take the wide form (a workgroup per row tile / block column,
+    ancestor rows added with atomics forward and read backward); twelve: the forward is the
+    grouped level solve (four workgroups per supernode), the backward the wide form.  Residual
+    of the direct solve with sparse products, M0 x = A (theta o (A^T x))."""
     import scipy.sparse as sp
     from lakshya import gen
-    lp = gen.block_angular(4, 1500, 4000, 8, 50, 0.05, seed=0)
+    lp = gen.block_angular(blocks, mb, nb, 8, 50, 0.05, seed=0)
     rp, ci, va = lp.csr()
+    kinds = set(gpu.plan_export(rp, ci, va, lp.n)["slaunch"].reshape(-1, 3)[:, 0].tolist())
+    assert 23 in kinds and (22 in kinds) == (blocks <= 8)
     m, n = lp.m, lp.n
     A = sp.csr_matrix((va, ci, rp), shape=(m, n))
     theta = np.exp(np.random.default_rng(3).uniform(-2, 2, n))
