@@ -14,7 +14,6 @@ namespace {
 
 constexpr int TS = 64;
 constexpr int DP = 65;
-
 // Wave-uniform broadcast of lane `src`'s value (src is a compile-time constant at every use):
 // two readlanes into scalar registers, no LDS round trip.
 __device__ __forceinline__ double bcast_lane(double v, int src) {
@@ -25,8 +24,9 @@ __device__ __forceinline__ double bcast_lane(double v, int src) {
 
 // Stage the row-major copy Lr (kb x kb, reciprocal diagonal) of a diagonal block into Ds (64
 // rows of pitch 65 doubles; zero past kb) in four batches of 16 rows: coalesced loads, issued
-// ahead of the solve that reads them.
-template <typename T>
+// ahead of the solve that reads them.  Scaled: every entry below the diagonal is then divided
+// by the diagonal of its column (lane = column), Ds[r][c] = L[r][c] / L[c][c] (fsub_scaled).
+template <typename T, bool Scaled = false>
 __device__ __forceinline__ void stage_diag(const T* __restrict__ Lr, int kb, double* Ds) {
     const int lane = threadIdx.x & 63;
     if (kb == TS) {                               // full block: unconditional loads
@@ -43,6 +43,13 @@ __device__ __forceinline__ void stage_diag(const T* __restrict__ Lr, int kb, dou
         for (int r = 0; r < TS; ++r)
             Ds[r * 65 + lane] = (double)Lr[min(r, kb - 1) * kb + lc] * ((r < kb && lane < kb) ? 1.0 : 0.0);
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (!Scaled) return;
+    const double rc = Ds[lane * 65 + lane];
+#pragma unroll 8
+    for (int r = 0; r < TS; ++r) Ds[r * 65 + lane] *= r > lane ? rc : 1.0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -66,6 +73,23 @@ __device__ __forceinline__ double fsub_staged(double b, const double* Ds, int kb
     for (int j = 0; j < TS; ++j) rv[j] = lane == j ? 0.0 : rv[j];
 #pragma unroll
     for (int j = 0; j < TS; ++j) b = fma(-rv[j], bcast_lane(b * rd, j), b);
+    return b * rd;
+}
+
+// fsub_staged on a block staged with Scaled: the entries are M[i][j] = L[i][j] / L[j][j], so
+// step j is b -= M[i][j] b_j (lane j keeps b_j = L_jj z_j) and the multiply leaves the chain:
+// two readlanes and one fma per step.
+__device__ __forceinline__ double fsub_scaled(double b, const double* Ds, int kb) {
+    const int lane = threadIdx.x & 63;
+    const double* row = Ds + lane * 65;
+    double rv[TS];
+#pragma unroll
+    for (int j = 0; j < TS; ++j) rv[j] = row[j];
+    const double rd = lane < kb ? row[lane] : 0.0;
+#pragma unroll
+    for (int j = 0; j < TS; ++j) rv[j] = lane == j ? 0.0 : rv[j];
+#pragma unroll
+    for (int j = 0; j < TS; ++j) b = fma(-rv[j], bcast_lane(b, j), b);
     return b * rd;
 }
 
@@ -704,32 +728,58 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const lkd_task* __restrict__ t
     double* xs = x + tk.first;
     const int kbt = t < nb ? min(TS, nc - t * TS) : 0;
     if (threadIdx.x == 0) s_ok = 1;
-    if (t < nb && v == 0) stage_diag<T>(diag + tk.aux + (int64_t)t * TS * TS, kbt, Ds);
+    if (t < nb && v == 0) stage_diag<T, true>(diag + tk.aux + (int64_t)t * TS * TS, kbt, Ds);
     // the tile's right-hand side is final before the launch: loaded ahead of the chain
     const double b = lane < kbt ? xs[(int64_t)t * TS + lane] : 0.0;
     double acc = 0.0;
     bool ok = true;
+    // Blocks are published in order (block j + 1 is solved after block j arrived), so a flag
+    // seen set covers every block before it: blocks 0 .. known are published.  One flag wait
+    // reads the next 63 flags too, and z comes four blocks per load: lane l of wavefront v holds
+    // z of block zb + l / 16, column 16 v + l % 16 (blocks zb .. zmax).
+    int known = -1, zb = 0, zmax = -1;
+    double zg = 0.0;
     auto apply = [&](const T (&h)[16], int j) {
-        if (!ok || !wide_wait(fl + j, ep, err)) {
-            ok = false;
-            return;
+        if (!ok) return;
+        if (j > zmax) {
+            if (j > known) {
+                if (!wide_wait(fl + j, ep, err)) {
+                    ok = false;
+                    return;
+                }
+                const int jj = j + 1 + lane;
+                const bool set = jj < nj && __hip_atomic_load(fl + min(jj, nj - 1), __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT) == ep;
+                const uint64_t m = __ballot(set);
+                known = j + (~m == 0 ? 64 : __builtin_ctzll(~m));
+            }
+            zb = j;
+            zmax = min(known, j + 3);
+            const int jb = min(j + (lane >> 4), zmax);
+            const int kb = min(TS, nc - jb * TS);
+            const int cc = 16 * v + (lane & 15);
+            const double zr = wide_load(xs + (int64_t)jb * TS + min(cc, kb - 1));
+            zg = cc < kb ? zr : 0.0;
         }
-        const int kb = min(TS, nc - j * TS);
-        const int cc = 16 * v + (lane & 15);
-        const double zr = wide_load(xs + (int64_t)j * TS + min(cc, kb - 1));
-        const double z = cc < kb ? zr : 0.0;
+        const int base = 16 * (j - zb);
 #pragma unroll
-        for (int c = 0; c < 16; ++c) acc = fma((double)h[c], bcast_lane(z, c), acc);
+        for (int c = 0; c < 16; ++c) acc = fma((double)h[c], bcast_lane(zg, base + c), acc);
     };
-    T h0[16], h1[16];
-    if (nj > 0) wide_cols<T>(h0, P, ld, 0, min(TS, nc), row, rok);
+    // panel data two blocks ahead of the block applied
+    T h0[16], h1[16], h2[16];
+    auto ld_blk = [&](T (&h)[16], int j) { wide_cols<T>(h, P, ld, j, min(TS, nc - j * TS), row, rok); };
+    if (nj > 0) ld_blk(h0, 0);
+    if (nj > 1) ld_blk(h1, 1);
 #pragma unroll 1
-    for (int j = 0; j < nj; j += 2) {
-        if (j + 1 < nj) wide_cols<T>(h1, P, ld, j + 1, min(TS, nc - (j + 1) * TS), row, rok);
+    for (int j = 0; j < nj; j += 3) {
+        if (j + 2 < nj) ld_blk(h2, j + 2);
         apply(h0, j);
         if (j + 1 >= nj) break;
-        if (j + 2 < nj) wide_cols<T>(h0, P, ld, j + 2, min(TS, nc - (j + 2) * TS), row, rok);
+        if (j + 3 < nj) ld_blk(h0, j + 3);
         apply(h1, j + 1);
+        if (j + 2 >= nj) break;
+        if (j + 4 < nj) ld_blk(h1, j + 4);
+        apply(h2, j + 2);
     }
     if (!ok) s_ok = 0;
     red[v][lane] = acc;
@@ -737,7 +787,7 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const lkd_task* __restrict__ t
     if (!s_ok || v != 0) return;
     const double s = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
     if (t < nb) {
-        const double z = fsub_staged(b - s, Ds, kbt);
+        const double z = fsub_scaled(b - s, Ds, kbt);
         if (lane < kbt) wide_store(xs + (int64_t)t * TS + lane, z);
         wide_publish(flags + tk.px + t, ep);
     } else if (rok) {
