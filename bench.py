@@ -74,6 +74,9 @@ def build_workload(args):
     elif args.workload == "random_sparse":
         lp = gen.random_sparse(2000, 5000, 6, seed=args.seed)
         name = f"rs_2000_5000_6_s{args.seed}"
+    elif args.workload == "parametric":     # the base LP of the parametric family, 20,000 x 50,000
+        lp = gen.parametric_base(seed=args.seed)
+        name = f"rs_20000_50000_8_s{args.seed}"
     else:
         lp = gen.transport(seed=args.seed)
         name = f"tr_700x700_side20000_s{args.seed}"
@@ -361,7 +364,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="block_angular",
-                    choices=["block_angular", "random_sparse", "transport"])
+                    choices=["block_angular", "random_sparse", "transport", "parametric"])
     ap.add_argument("--blocks", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
