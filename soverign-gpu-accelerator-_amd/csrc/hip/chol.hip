@@ -135,11 +135,11 @@ __global__ __launch_bounds__(256) void k_diag(const lkd_task* __restrict__ tasks
 // launch that follows reads them back in the same layout (coalesced) and adds them in order.
 template <typename T>
 __device__ __forceinline__ void part_tile(const lkd_task& tk, const T* __restrict__ L,
-                                          T* __restrict__ work, GemmLds<T>& sm) {
+                                          T* __restrict__ work, StepLds<T>& sm) {
     const T* P = L + tk.off;
     Acc<T> acc[4][2];
     zero_acc<T, 4, 2>(acc);
-    gemm_tn<T>(P + tk.r0, tk.ld, tk.c0, P + tk.k0, tk.ld, tk.kb, tk.kbeg, tk.kend, sm, acc);
+    gemm_tn_step<T>(P + tk.r0, tk.ld, tk.c0, P + tk.k0, tk.ld, tk.kb, tk.kbeg, tk.kend, sm, acc);
     T* out = work + tk.px + threadIdx.x;
 #pragma unroll
     for (int p = 0; p < 4; ++p)
@@ -149,27 +149,73 @@ __device__ __forceinline__ void part_tile(const lkd_task& tk, const T* __restric
             for (int e = 0; e < 4; ++e) out[((p * 2 + b) * 4 + e) * 256] = acc[p][b][e];
 }
 
-// PART launch of the one-launch-per-step form: PART tiles, and (nsplit = 2, listed first) the
-// factorization of the step's diagonal block, which runs beside the partial products (the
-// STEP launch after this one is the first to read it).
+// PART launch of the one-launch-per-step form: PART tiles (part = 2: two block columns, see
+// part_tile2), and (nsplit = 2, listed first) the factorization of the step's diagonal block,
+// which runs beside the partial products (the STEP launch after this one is the first to read
+// it).
+// PART of two consecutive block columns (task part = 2: the one-launch-per-step form pairs an
+// even step k with step k + 1 on the same 128-row tile): L(R, kbeg:kend) [L(k, kbeg:kend);
+// L(k + 1, kbeg:kend)]^T, block column k's half to work + px and block column k + 1's to
+// work + ws, each in part_tile's layout: one 128 x 128 product, the A tile read once (FP64;
+// k_part runs FP32 paired tasks as two PART tiles).
+template <typename T>
+__device__ __forceinline__ void part_tile2(const lkd_task& tk, const T* __restrict__ L,
+                                           T* __restrict__ work, StepLds<T>& sm) {
+    const int rb = min(2 * TS, tk.ncol - tk.k0);
+    if constexpr (sizeof(T) == 8) {
+        const T* P = L + tk.off;
+        T* out1 = work + tk.px + threadIdx.x;
+        T* out2 = work + tk.ws + threadIdx.x;
+        Acc<T> acc[8][2];
+        zero_acc<T, 8, 2>(acc);
+        gemm_tn_dma2(P + tk.r0, tk.ld, tk.c0, P + tk.k0, tk.ld, rb, tk.kbeg, tk.kend,
+                     reinterpret_cast<GemmLdsDma2&>(sm), acc);
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    out1[((p * 2 + b) * 4 + e) * 256] = acc[p][b][e];
+                    out2[((p * 2 + b) * 4 + e) * 256] = acc[p + 4][b][e];
+                }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_part(const lkd_task* __restrict__ tasks, int ntask, const T* __restrict__ L,
             T* __restrict__ diag, T* __restrict__ piv, T* __restrict__ stage,
-            T* __restrict__ work, int32_t* err) {
-    const int ti = task_index();
-    if (ti >= ntask) return;
+            T* __restrict__ work, int32_t* err, int halves) {
+    // FP32 launches run every task as two halves (workgroup 2 i + h): a paired task's block
+    // columns one per workgroup, the second half of any other task has nothing to do
+    const int vi = task_index();
+    if (vi >= ntask * halves) return;
+    const int ti = halves == 2 ? vi >> 1 : vi, h = halves == 2 ? (vi & 1) : 0;
     __shared__ int s_err;
     if (wg_error(err, &s_err)) return;
-    __shared__ GemmLds<T> sm;
+    __shared__ StepLds<T> sm;
     __shared__ int s_ok;
     const lkd_task tk = tasks[ti];
     if (tk.nsplit == 2) {
+        if (h) return;
         T* Ds = reinterpret_cast<T*>(&sm);
         diag_task<T>(tk, L, diag, piv, stage, err, Ds, Ds + TS * DP, &s_ok);
         return;
     }
-    part_tile<T>(tk, L, work, sm);
+    if constexpr (sizeof(T) == 8) {
+        if (tk.part == 2) part_tile2<T>(tk, L, work, sm);
+        else part_tile<T>(tk, L, work, sm);
+    } else {
+        if (tk.part != 2 && h) return;
+        lkd_task u = tk;
+        if (tk.part == 2) {
+            u.k0 = tk.k0 + TS * h;
+            u.kb = min(TS, tk.ncol - u.k0);
+            u.px = h ? tk.ws : tk.px;
+        }
+        part_tile<T>(u, L, work, sm);
+    }
 }
 
 // PSUM: the np partial products of a STEP tile (work + px, LKD_PART elements each) summed in
@@ -222,7 +268,7 @@ template <typename T>
 __device__ __forceinline__ void step_tile(const lkd_task& tk, T* __restrict__ L,
                                           T* __restrict__ diag, T* __restrict__ piv,
                                           T* __restrict__ stage, const T* __restrict__ work,
-                                          int32_t* err, GemmLds<T>& sm, int* s_ok,
+                                          int32_t* err, StepLds<T>& sm, int* s_ok,
                                           const DagSync* ds = nullptr) {
     using R = Mfma<T>;
     T* P = L + tk.off;
@@ -273,7 +319,7 @@ __device__ __forceinline__ void step_tile(const lkd_task& tk, T* __restrict__ L,
 #pragma unroll
                 for (int e = 0; e < 4; ++e) acc[p][b][e] += in[((p * 2 + b) * 4 + e) * 256];
     }
-    gemm_tn<T>(P + tk.r0, ld, ra, P + k0, ld, kb, tk.kbeg, tk.kend, sm, acc);
+    gemm_tn_step<T>(P + tk.r0, ld, ra, P + k0, ld, kb, tk.kbeg, tk.kend, sm, acc);
     Acc<T> y[2][4];
 #pragma unroll
     for (int b = 0; b < 2; ++b)
@@ -411,7 +457,7 @@ void k_step(const lkd_task* __restrict__ tasks, int ntask, T* __restrict__ L, T*
     if (ti >= ntask) return;
     __shared__ int s_err;
     if (wg_error(err, &s_err)) return;
-    __shared__ GemmLds<T> sm;
+    __shared__ StepLds<T> sm;
     __shared__ int s_ok;
     step_tile<T>(tasks[ti], L, diag, piv, stage, work, err, sm, &s_ok);
 }
@@ -436,7 +482,7 @@ void k_step_dag(const lkd_task* __restrict__ tasks, const int32_t* __restrict__ 
                 uint32_t* __restrict__ flags2, const uint32_t* __restrict__ ep_ctl,
                 uint32_t* __restrict__ cnt, T* __restrict__ L, T* __restrict__ diag,
                 T* __restrict__ piv, T* __restrict__ stage, T* __restrict__ work, int32_t* err) {
-    __shared__ GemmLds<T> sm;
+    __shared__ StepLds<T> sm;
     __shared__ int s_ok, s_task;
     const uint32_t ep = ep_ctl[0];
     if (threadIdx.x == 0) {
@@ -591,7 +637,12 @@ int launch_chol(int kind, int32_t ntask, const lkd_task* tasks, const CholArgs& 
         case 2: LK_LAUNCH(k_update<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, *a.plan, a.err); break;
         case 3: LK_LAUNCH(k_diagwb<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (const T*)a.diag, (const T*)a.piv); break;
         case 8: LK_LAUNCH(k_psum<T>, grid8(ntask), 256, 0, stream, tasks, ntask, (T*)a.work); break;
-        case 6: LK_LAUNCH(k_part<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (T*)a.stage, (T*)a.work, a.err); break;
+        case 6: {
+            const int halves = sizeof(T) == 8 ? 1 : 2;
+            LK_LAUNCH(k_part<T>, grid8(ntask * halves), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv,
+                      (T*)a.stage, (T*)a.work, a.err, halves);
+            break;
+        }
         default: LK_LAUNCH(k_step<T>, grid8(ntask), 256, 0, stream, tasks, ntask, L, (T*)a.diag, (T*)a.piv, (T*)a.stage, (const T*)a.work, a.err); break;
     }
     return lk_last_error();

@@ -166,6 +166,166 @@ __device__ __forceinline__ void gemm_tn(const T* __restrict__ pa, int64_t lda, i
     });
 }
 
+// FP64 TN product with direct-to-LDS staging, for the supernodal STEP and PART tiles: each
+// wavefront moves its share of every K chunk with global_load_lds_dwordx4 (16 bytes per lane
+// straight into LDS, no staging registers, no LDS store instructions), three LDS stages deep,
+// so chunks c + 1 and c + 2 are in flight while chunk c is computed.  A chunk is 16 columns of
+// A (128 rows each: one instruction per column, lane l moves rows 2l and 2l + 1) and of B (64
+// rows: one instruction per two columns).  Each wavefront waits for its own loads of chunk c
+// (vmcnt: the six loads of chunk c + 1 may still run), then a barrier makes chunk c complete
+// for everyone and chunk c - 1's stage free for chunk c + 2.  The loads are issued in inline
+// assembly (m0 = the LDS address, set in the same statement): the instruction has no output
+// register, and the waits above are the only ones its LDS writes need; m0 is saved and restored
+// around it.
+// Rows: a lane whose pair starts at or past ra (rb) loads rows 0, 1 instead (product rows the
+// callers discard, as in gemm_loop); a pair starting at ra - 1 also reads row ra, which lies in
+// the panel: every column of A and B here is left of the block column k0 <= ncol - 1, so the
+// element after its last row is row 0 of the next column.  K ranges must be whole chunks
+// (multiples of 16; the supernodal plans' are).  Results are bit for bit those of gemm_tn.
+struct GemmLdsDma {
+    double a[3][16 * PA];
+    double b[3][16 * NC];
+};
+
+// m0 is saved and restored around the load, so the statement leaves it as it found it.
+__device__ __forceinline__ void dma16(const double* g, double* l) {
+    typedef __attribute__((address_space(3))) void lds_void;
+    const uint32_t lds = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(lds_void*)l);
+    uint32_t saved;
+    asm volatile("s_mov_b32 %0, m0\n\ts_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off\n\t"
+                 "s_mov_b32 m0, %0"
+                 : "=&s"(saved) : "v"(g), "s"(lds) : "memory");
+}
+
+__device__ __forceinline__ void gemm_tn_dma(const double* __restrict__ pa, int64_t lda, int ra,
+                                            const double* __restrict__ pb, int64_t ldb, int rb,
+                                            int kbeg, int kend, GemmLdsDma& sm, Acc<double> (&acc)[4][2]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const bool v0 = 32 * wu < ra, v1 = 32 * wu + 16 < ra;
+    const int ar = 2 * lane < ra ? 2 * lane : 0;
+    const int hb = lane & 31, bkk = lane >> 5;
+    const int br = 2 * hb < rb ? 2 * hb : 0;
+    const double* qa = pa + ar + (int64_t)(4 * w) * lda;
+    const double* qb = pb + br + (int64_t)(4 * w + bkk) * ldb;
+    auto issue = [&](int st, int kc) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dma16(qa + (int64_t)(kc + i) * lda, sm.a[st] + (4 * w + i) * PA);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) dma16(qb + (int64_t)(kc + 2 * i) * ldb, sm.b[st] + (4 * w + 2 * i) * NC);
+    };
+    const int nch = (kend - kbeg) / 16;
+    if (nch <= 0) return;
+    issue(0, kbeg);
+    if (nch > 1) issue(1, kbeg + 16);
+    int st = 0;
+#pragma unroll 1
+    for (int c = 0; c < nch; ++c) {
+        if (c + 1 < nch) asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
+        else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        asm volatile("s_barrier" ::: "memory");
+        if (c + 2 < nch) issue(st == 0 ? 2 : st - 1, kbeg + (c + 2) * 16);
+        const double* As = sm.a[st];
+        const double* Bs = sm.b[st];
+        if (v0) {
+#pragma unroll
+            for (int k = 0; k < 16; k += 4) {
+                const double a0 = As[(k + lk) * PA + 32 * w + li];
+                double b[4];
+#pragma unroll
+                for (int p = 0; p < 4; ++p) b[p] = Bs[(k + lk) * NC + 16 * p + li];
+                if (v1) {
+                    const double a1 = As[(k + lk) * PA + 32 * w + 16 + li];
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) {
+                        acc[p][0] = Mfma<double>::mma(b[p], a0, acc[p][0]);
+                        acc[p][1] = Mfma<double>::mma(b[p], a1, acc[p][1]);
+                    }
+                } else {
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) acc[p][0] = Mfma<double>::mma(b[p], a0, acc[p][0]);
+                }
+            }
+        }
+        st = st == 2 ? 0 : st + 1;
+    }
+    __syncthreads();   // every wavefront done with the stages: the caller may reuse the LDS
+}
+
+// gemm_tn_dma for two block columns at once (128 B rows: a 128 x 128 product, accumulators
+// acc[p][b] for B columns 16 p .., p < 8): the A tile is read once for both.  Two LDS stages of
+// 16 columns of A and B (pitch PA both; one instruction per column and wavefront share); chunk
+// c + 1 loads while chunk c is computed.  Same row and K conventions as gemm_tn_dma, except
+// that every wavefront computes all its rows (skipping the 16-row blocks past ra, a rare partial
+// tile here, makes the compiler keep two MFMA sequences and spill).
+struct GemmLdsDma2 {
+    double a[2][16 * PA];
+    double b[2][16 * PA];
+};
+
+__device__ __forceinline__ void gemm_tn_dma2(const double* __restrict__ pa, int64_t lda, int ra,
+                                             const double* __restrict__ pb, int64_t ldb, int rb,
+                                             int kbeg, int kend, GemmLdsDma2& sm, Acc<double> (&acc)[8][2]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+    const int ar = 2 * lane < ra ? 2 * lane : 0;
+    const int br = 2 * lane < rb ? 2 * lane : 0;
+    const double* qa = pa + ar + (int64_t)(4 * w) * lda;
+    const double* qb = pb + br + (int64_t)(4 * w) * ldb;
+    auto issue = [&](int st, int kc) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dma16(qa + (int64_t)(kc + i) * lda, sm.a[st] + (4 * w + i) * PA);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dma16(qb + (int64_t)(kc + i) * ldb, sm.b[st] + (4 * w + i) * PA);
+    };
+    const int nch = (kend - kbeg) / 16;
+    if (nch <= 0) return;
+    issue(0, kbeg);
+#pragma unroll 1
+    for (int c = 0; c < nch; ++c) {
+        const int st = c & 1;
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        asm volatile("s_barrier" ::: "memory");
+        if (c + 1 < nch) issue(st ^ 1, kbeg + (c + 1) * 16);
+        const double* As = sm.a[st];
+        const double* Bs = sm.b[st];
+#pragma unroll
+        for (int k = 0; k < 16; k += 4) {
+            const double a0 = As[(k + lk) * PA + 32 * w + li];
+            const double a1 = As[(k + lk) * PA + 32 * w + 16 + li];
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+                const double bp = Bs[(k + lk) * PA + 16 * p + li];
+                acc[p][0] = Mfma<double>::mma(bp, a0, acc[p][0]);
+                acc[p][1] = Mfma<double>::mma(bp, a1, acc[p][1]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// LDS of the supernodal STEP / PART kernels and their TN product: direct-to-LDS staging in
+// FP64 (whole 16-column chunks), gemm_tn otherwise.
+template <typename T> struct StepLdsT { using type = GemmLds<T>; };
+template <> struct StepLdsT<double> { using type = GemmLdsDma; };
+template <typename T> using StepLds = typename StepLdsT<T>::type;
+static_assert(sizeof(GemmLdsDma) >= sizeof(GemmLds<double>), "fallback layout");
+static_assert(sizeof(GemmLdsDma) >= sizeof(GemmLdsDma2), "two-column product layout");
+
+template <typename T>
+__device__ __forceinline__ void gemm_tn_step(const T* __restrict__ pa, int64_t lda, int ra,
+                                             const T* __restrict__ pb, int64_t ldb, int rb, int kbeg,
+                                             int kend, StepLds<T>& sm, Acc<T> (&acc)[4][2]) {
+    if constexpr (sizeof(T) == 8) {
+        if ((kend - kbeg) % 16 == 0) {
+            gemm_tn_dma(pa, lda, ra, pb, ldb, rb, kbeg, kend, sm, acc);
+            return;
+        }
+        gemm_tn<T>(pa, lda, ra, pb, ldb, rb, kbeg, kend, reinterpret_cast<GemmLds<T>&>(sm), acc);
+    } else {
+        gemm_tn<T>(pa, lda, ra, pb, ldb, rb, kbeg, kend, sm, acc);
+    }
+}
+
 // f(row, col, value) for every NT accumulator element of this thread (tile coordinates).
 template <typename T, typename F>
 __device__ __forceinline__ void for_acc(Acc<T> (&acc)[2][4], F&& f) {

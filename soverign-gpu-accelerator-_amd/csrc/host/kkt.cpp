@@ -566,6 +566,8 @@ struct GPUKKTCholeskySolver::Impl {
             }
             emit(1, b0);
             // STEP k: 128-row tiles below the diagonal block of block column k
+            std::map<std::pair<Index, int32_t>, std::pair<int64_t, int>> carried;   // (sn, r0) -> summed products
+            int64_t carried_end = 0;
             for (int k = 0; k < maxblk; ++k) {
                 std::vector<lkd_task> tl;
                 std::vector<lkd_task> dg;   // one launch per step: block k's factorization
@@ -592,6 +594,10 @@ struct GPUKKTCholeskySolver::Impl {
                             u.nsplit = 1;
                             u.aux = diag_off(lv[si], k + 1);
                             u.aux2 = slot(si, k + 1);
+                        } else if (k1 % GEMM_MT == 0) {
+                            // one launch per step: tiles on 128-row boundaries, the same for
+                            // consecutive steps below their diagonal blocks (paired PART tasks)
+                            u.c0 = static_cast<int>(std::min<int64_t>(GEMM_MT, t.ld - k1));
                         }
                         tl.push_back(u);
                         r0 += u.c0;
@@ -637,9 +643,55 @@ struct GPUKKTCholeskySolver::Impl {
                     S = static_cast<int>(std::min<int64_t>(STEP_TARGET_WG / nt, kmax / STEP_KMIN));
                 std::vector<lkd_task> parts;
                 std::vector<char> pcrit, scrit;
-                int64_t px = 0;
+                int64_t px = carried_end;   // slots below carried_end hold products of step k
+                // Tiles of step k - 1's pairs: their products over [0, k0 - 64) are summed in the
+                // work buffer; they compute block column k - 1 (the last 64 columns) themselves.
+                // An even step k pairs its tiles below block k + 1 with step k + 1 (same rows):
+                // PART tasks of two block columns read each A tile once (part = 2).
+                std::vector<lkd_task> next_sums;
+                std::map<std::pair<Index, int32_t>, std::pair<int64_t, int>> pairs;
+                const bool pair_step = k % 2 == 0 && S > 1;
                 for (auto& u : tl) {
+                    const auto c = carried.find({u.sn, u.r0});
+                    if (c != carried.end()) {
+                        u.px = c->second.first;
+                        u.np = c->second.second;
+                        u.kbeg = u.k0 - TILE;
+                        u.kend = u.k0;
+                        scrit.push_back(0);
+                        continue;
+                    }
                     const int Su = std::min(S, std::max(1, u.k0 / STEP_KMIN));
+                    const bool paired = pair_step && Su > 1 && u.k0 + TILE < u.ncol && u.r0 >= u.k0 + 2 * TILE;
+                    if (paired) {
+                        const int64_t p1 = px, p2 = px + static_cast<int64_t>(Su) * LKD_PART;
+                        u.kbeg = u.kend = 0;
+                        u.np = Su;
+                        u.px = p1;
+                        for (int j = 0; j < Su; ++j) {
+                            lkd_task q = u;
+                            q.kbeg = kpart(u.k0, Su, j);
+                            q.kend = j + 1 == Su ? u.k0 : kpart(u.k0, Su, j + 1);
+                            q.np = 0;
+                            q.part = 2;
+                            q.px = p1 + static_cast<int64_t>(j) * LKD_PART;
+                            q.ws = p2 + static_cast<int64_t>(j) * LKD_PART;
+                            parts.push_back(q);
+                            pcrit.push_back(0);
+                        }
+                        pairs[{u.sn, u.r0}] = {p2, Su};
+                        if (Su >= PSUM_MIN)
+                            for (int sl = 0; sl < LKD_PART / 512; ++sl) {
+                                lkd_task q = u;
+                                q.px = p2;
+                                q.kbeg = sl;
+                                q.kend = 0;
+                                next_sums.push_back(q);
+                            }
+                        px = p2 + static_cast<int64_t>(Su) * LKD_PART;
+                        scrit.push_back(0);
+                        continue;
+                    }
                     if (Su > 1) {
                         u.kbeg = u.kend = 0;
                         u.np = Su;
@@ -663,6 +715,12 @@ struct GPUKKTCholeskySolver::Impl {
                     scrit.push_back(u.nsplit ? 1 : 0);
                 }
                 work_elems = std::max(work_elems, px);
+                carried.clear();
+                carried_end = 0;
+                for (const auto& e : pairs) {
+                    carried[e.first] = {e.second.first, e.second.second >= PSUM_MIN ? 1 : e.second.second};
+                    carried_end = px;   // step k + 1 keeps these slots; its own parts go above
+                }
                 if (!parts.empty() || !dg.empty()) {
                     order_for_dispatch(parts, pcrit);
                     parts.insert(parts.begin(), dg.begin(), dg.end());   // first workgroups
@@ -672,9 +730,9 @@ struct GPUKKTCholeskySolver::Impl {
                 }
                 // tiles with many parts: PSUM adds them (slices on many workgroups), the STEP
                 // then reads the one sum
-                std::vector<lkd_task> sums;
+                std::vector<lkd_task> sums = next_sums;
                 for (auto& u : tl) {
-                    if (u.np < PSUM_MIN) continue;
+                    if (u.np < PSUM_MIN || u.kend > 0) continue;   // (carried tiles: summed before)
                     for (int sl = 0; sl < LKD_PART / 512; ++sl) {
                         lkd_task q = u;
                         q.kbeg = sl;
@@ -972,7 +1030,11 @@ struct GPUKKTCholeskySolver::Impl {
                         bad("part-launch diag");
                     break;
                 }
-                // PART: one K range of a STEP tile's GEMM into the work buffer
+                // PART: one K range of a STEP tile's GEMM into the work buffer; part = 2: of two
+                // block columns, the second one's product at ws
+                if (t.part == 2 && (t.k0 + t.kb >= nc || t.r0 < t.k0 + 2 * TILE || t.ws < 0 ||
+                                    t.ws + LKD_PART > work_elems || (t.ws < t.px + LKD_PART && t.px < t.ws + LKD_PART)))
+                    bad("paired part");
                 if (t.kb < 1 || t.r0 < t.k0 + t.kb || t.c0 < 1 || t.c0 > GEMM_MT || t.r0 + t.c0 > ld) bad("part rows");
                 if (t.kbeg < 0 || t.kbeg >= t.kend || t.kend > t.k0) bad("part K");
                 if (t.px < 0 || t.px + LKD_PART > work_elems) bad("part work");
@@ -1497,7 +1559,9 @@ struct GPUKKTCholeskySolver::Impl {
     static double task_flops(const lkd_task& t, int kind) {
         const double kb = t.kb;
         switch (kind) {
-            case 6: return t.nsplit == 2 ? kb * kb * kb / 3.0 : 2.0 * t.c0 * kb * (t.kend - t.kbeg);
+            case 6:
+                if (t.nsplit == 2) return kb * kb * kb / 3.0;
+                return 2.0 * t.c0 * (t.part == 2 ? std::min(2 * TILE, t.ncol - t.k0) : kb) * (t.kend - t.kbeg);
             case 8: return 0.0;
             case 7:
                 if (t.part) return 2.0 * t.c0 * kb * (t.kend - t.kbeg);

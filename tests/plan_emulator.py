@@ -131,8 +131,12 @@ class Emulator:
                     assert 0 <= kbeg < kend <= k0
                     px = int(t["px"])
                     assert px not in self.work
-                    self.work[px] = (kbeg, kend, self.blk(t, r0, r0 + ra, kbeg, kend)
-                                     @ self.blk(t, k0, k0 + kb, kbeg, kend).T)
+                    Ar = self.blk(t, r0, r0 + ra, kbeg, kend)
+                    self.work[px] = (kbeg, kend, Ar @ self.blk(t, k0, k0 + kb, kbeg, kend).T)
+                    if int(t["part"]) == 2:       # and of the next block column, into ws
+                        ws, kb2 = int(t["ws"]), min(64, nc - k0 - 64)
+                        assert ws not in self.work and kb2 > 0 and r0 >= k0 + 128
+                        self.work[ws] = (kbeg, kend, Ar @ self.blk(t, k0 + 64, k0 + 64 + kb2, kbeg, kend).T)
                 elif kind == 5:
                     ra = int(t["c0"])
                     X = self.blk(t, r0, r0 + ra, k0, k0 + kb)
