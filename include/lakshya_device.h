@@ -513,6 +513,12 @@ int lkd_probe_mfma(int prec, const void* a, const void* b, void* out, void* stre
  * from the NT GEMM core; out[8192, 16384) = the same from the transposed (TN) core */
 int lkd_probe_tile_gemm(int prec, const void* a, int64_t lda, int ra, const void* b, int64_t ldb,
                         int rb, int K, void* out, void* stream);
+/* FP64 direct-to-LDS TN products (the STEP / PART staging) against the register-staged TN core:
+ * out (4 x 128 x 64 doubles, column-major): gemm_tn, the direct-to-LDS 128 x 64 product, and the
+ * two 64-column halves of the 128 x 128 product (B rows 0..63, 64..127).  K a multiple of 16,
+ * ra <= 128, rb <= 128, lda > ra, ldb > rb. */
+int lkd_probe_tile_gemm_dma(const double* a, int64_t lda, int ra, const double* b, int64_t ldb, int rb,
+                            int K, double* out, void* stream);
 /* in-place Cholesky of one 64x64 column-major SPD block (lower part returned, upper zeroed);
  * *ok = 0 when a pivot is not positive */
 int lkd_probe_potrf(int prec, void* a, int32_t* ok, void* stream);

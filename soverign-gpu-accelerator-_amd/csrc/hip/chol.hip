@@ -682,6 +682,55 @@ __global__ __launch_bounds__(256) void k_probe_gemm(const T* pa, int64_t lda, in
             }
 }
 
+// The FP64 direct-to-LDS products against the register-staged TN core: out[0 ..) gemm_tn,
+// out[8192 ..) gemm_tn_dma, out[16384 ..) and out[24576 ..) the two 64-column halves of
+// gemm_tn_dma2 (B rows 0 .. 63 and 64 .. 127); each 128 x 64 column-major, zero past ra / the
+// half's B rows.
+__global__ __launch_bounds__(256) void k_probe_dma(const double* pa, int64_t lda, int ra, const double* pb,
+                                                   int64_t ldb, int rb, int K, double* out) {
+    __shared__ GemmLdsDma sm;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    auto put = [&](double* o, Acc<double> (&tr)[4][2], int rows_b) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int c = 16 * p + Mfma<double>::row(lane, e), r = 32 * w + 16 * b + (lane & 15);
+                    o[c * MT + r] = r < ra && c < rows_b ? tr[p][b][e] : 0.0;
+                }
+    };
+    {
+        Acc<double> tr[4][2];
+        zero_acc<double, 4, 2>(tr);
+        gemm_tn<double>(pa, lda, ra, pb, ldb, min(rb, NC), 0, K, reinterpret_cast<GemmLds<double>&>(sm), tr);
+        put(out, tr, min(rb, NC));
+        __syncthreads();
+    }
+    {
+        Acc<double> tr[4][2];
+        zero_acc<double, 4, 2>(tr);
+        gemm_tn_dma(pa, lda, ra, pb, ldb, min(rb, NC), 0, K, sm, tr);
+        put(out + MT * NC, tr, min(rb, NC));
+    }
+    {
+        Acc<double> tr2[8][2];
+        zero_acc<double, 8, 2>(tr2);
+        gemm_tn_dma2(pa, lda, ra, pb, ldb, rb, 0, K, reinterpret_cast<GemmLdsDma2&>(sm), tr2);
+        Acc<double> h0[4][2], h1[4][2];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                h0[p][b] = tr2[p][b];
+                h1[p][b] = tr2[p + 4][b];
+            }
+        put(out + 2 * MT * NC, h0, min(rb, NC));
+        put(out + 3 * MT * NC, h1, max(rb - NC, 0));
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_probe_potrf(T* a, int32_t* ok) {
     __shared__ T Ds[TS * DP];
@@ -714,6 +763,16 @@ int lkd_probe_tile_gemm(int prec, const void* a, int64_t lda, int ra, const void
         LK_LAUNCH(k_probe_gemm<float>, 1, 256, 0, stream, (const float*)a, lda, ra, (const float*)b, ldb, rb, K, (float*)out);
     else
         LK_LAUNCH(k_probe_gemm<double>, 1, 256, 0, stream, (const double*)a, lda, ra, (const double*)b, ldb, rb, K, (double*)out);
+    return lk_last_error();
+}
+
+// FP64 direct-to-LDS TN products against the register-staged one (k_probe_dma); K a multiple
+// of 16, ra <= 128, rb <= 128, lda > ra and ldb > rb (a row pair may read one row past ra / rb)
+int lkd_probe_tile_gemm_dma(const double* a, int64_t lda, int ra, const double* b, int64_t ldb, int rb,
+                            int K, double* out, void* stream) {
+    if (ra < 1 || ra > MT || rb < 1 || rb > 2 * NC || K < 0 || K % 16 || lda <= ra || ldb <= rb)
+        return (int)hipErrorInvalidValue;
+    LK_LAUNCH(k_probe_dma, 1, 256, 0, stream, a, lda, ra, b, ldb, rb, K, out);
     return lk_last_error();
 }
 

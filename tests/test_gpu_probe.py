@@ -1,5 +1,6 @@
 """Lane layouts of the MFMA builtins, the 128x64 tile GEMM (NT and transposed TN accumulator
-layouts) and the 64x64 diagonal factorization used by the Cholesky kernels."""
+layouts), its FP64 direct-to-LDS forms (128x64 and 128x128) and the 64x64 diagonal
+factorization used by the Cholesky kernels."""
 import ctypes as C
 
 import numpy as np
@@ -54,6 +55,28 @@ def test_tile_gemm(gpu, prec, dt, K, ra, rb):
     tol = 1e-12 if dt == np.float64 else 1e-4
     assert np.abs(got - exp).max() <= tol * max(1.0, np.abs(exp).max())
     assert np.abs(got_tn - exp).max() <= tol * max(1.0, np.abs(exp).max())
+
+
+@pytest.mark.parametrize("K,ra,rb", [(64, 128, 128), (48, 77, 99), (256, 1, 65), (16, 128, 64), (160, 63, 127)])
+def test_tile_gemm_direct_to_lds(gpu, K, ra, rb):
+    """The FP64 STEP / PART products staged with direct-to-LDS loads (one 128 x 64 and one
+    128 x 128 product) equal the register-staged TN core bit for bit, and the product."""
+    L = gpu.lib()
+    L.lkd_probe_tile_gemm_dma.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_int64, C.c_int,
+                                          C.c_int, C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(K + ra)
+    lda, ldb = 130, 136
+    A = rng.standard_normal((K, lda))
+    B = rng.standard_normal((K, ldb))
+    da, db = _dev(gpu, A), _dev(gpu, B)
+    out = gpu.DeviceBuffer(4 * 128 * 64 * 8, np.float64)
+    assert L.lkd_probe_tile_gemm_dma(da.addr, lda, ra, db.addr, ldb, rb, K, out.addr, None) == 0
+    reg, dma, h0, h1 = (m.T for m in out.download().reshape(4, 64, 128))
+    np.testing.assert_array_equal(dma, reg)
+    np.testing.assert_array_equal(h0, reg)
+    exp = A[:, :ra].T @ B[:, :rb]
+    got = np.concatenate([h0, h1], axis=1)[:ra, :rb]
+    assert np.abs(got - exp).max() <= 1e-12 * max(1.0, np.abs(exp).max())
 
 
 @pytest.mark.parametrize("prec,dt,kb", [(0, np.float64, 64), (0, np.float64, 17), (1, np.float32, 64)])
