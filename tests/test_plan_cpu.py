@@ -27,7 +27,8 @@ def _cases():
 @pytest.mark.parametrize("name,steps", [(c, 1) for c in ("random_sparse", "block_angular", "transport",
                                                          "dense", "block_angular_wide", "dense_wide")] +
                          [("dense_wide", 0), ("dense_wide", 2), ("block_angular_wide", 2),
-                          ("dense_separator", 1), ("block_angular_2x1100", 1)])
+                          ("dense_separator", 1), ("block_angular_2x1100", 1),
+                          ("dense_separator", 2), ("block_angular_2x1100", 2)])
 def test_schedule_reproduces_dense_cholesky(native, name, steps):
     lp = _cases()[name]()
     rp, ci, va = lp.csr()
@@ -117,7 +118,8 @@ def _tile_phases(t):
     return [P, S, B]
 
 
-@pytest.mark.parametrize("name", ["random_sparse", "dense", "block_angular_wide", "dense_wide"])
+@pytest.mark.parametrize("name", ["random_sparse", "dense", "block_angular_wide", "dense_wide",
+                                  "block_angular_2x1100"])
 def test_steps_dependencies_order_every_conflict(native, name):
     """Inside a STEPS launch, two phases of tiles that touch the same data with at least one
     write must be ordered: by program order inside a tile, or through the hand-offs (a
